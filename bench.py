@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Benchmark: IMRPhenomD 3-detector network log-likelihood evaluations per second on one MI355X.
+
+Workload (the batched parallel-tempering ensemble, 16225 bins 20-2048 Hz at df = 1/8 Hz, Hanford /
+Livingston / Virgo, 16 temperatures x 32 chains = 512 instances per step): one step evaluates the
+network log-likelihood of 512 parameter vectors scattered around the injection.  All parameter batches
+and the event (data, PSD, weights) are resident in HBM before the timed region; every step is one
+enqueue of the batched engine followed by a stream synchronisation (a sampler reads its logL every
+step), so per-step latencies give p50/p99.
+
+`value` is the mixed-precision batched engine (FP64 phase, FP32 amplitude/projection/products with
+Kahan partials, FP64 reductions).  Reported beside it: the FP64 batched engine, the plain FP64
+one-instance-per-launch path, max |delta logL| mixed vs FP64 over every timed evaluation, the same
+for IMRPhenomPv2 on the same grid (1024 chains), kernel-level roofline from HIP events, and the CPU
+oracle (the reference algorithm restated in C, one instance per call, OpenMP over instances) timed on
+a bounded sample on the host cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "IMRPhenomD 3-det logL evals/s, 1 GPU, mixed vs FP64 at |ΔlogL|≤1e-3; p99 ms"
+PEAK_VALU_TFLOPS = 78.6   # MI355X vector FMA peak without packed FP32 (FP64 and non-packed FP32 issue alike)
+PEAK_HBM_GBS = 8000.0
+# FLOP-equivalents per bin (frozen table): inspiral, intermediate, merger-ringdown, per detector
+FE_REGION = (121.0, 86.0, 124.0)
+FE_DET = 46.0
+FE_PV2_BIN = 280.0
+FE_PV2_DET = 54.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def fe_per_launch(counts, ndet, pv2=False):
+    nz = counts[:, 0] + counts[:, 1] + counts[:, 2]
+    fe = counts[:, 0] * FE_REGION[0] + counts[:, 1] * FE_REGION[1] + counts[:, 2] * FE_REGION[2]
+    fe = fe + ndet * nz * FE_DET
+    if pv2:
+        fe = fe + nz * FE_PV2_BIN + ndet * nz * FE_PV2_DET
+    return float(fe.sum()), float(nz.sum())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--bpl", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("this engine measures one GPU: every GPU box here has one GPU; multi-GPU sharding is a design "
+              "note only (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import gw_analysis_tools_amd as G
+    from gw_analysis_tools_amd import workloads as W
+
+    G.check(G.lib().gwh_init(0))
+    if args.bpl:
+        G.set_bins_per_lane(args.bpl)
+    lib = G.lib()
+    cfg = W.config1()
+    f, psd, dets = cfg["f"], cfg["psd"], cfg["dets"]
+    inj = W.injection_params(1)[0]
+    pinj = G.make_params(**{k: (list(inj[k]) if k in ("spin1", "spin2") else inj[k].item())
+                            for k in inj.dtype.names})
+    resp = G.create_coherent_GW_detection(f, pinj, dets)
+    g = W.rng(1, 0)
+    data = [r + W.noise(f, psd, g) for r in resp]
+    ev = G.Event(f, data, [psd] * 3, dets)
+    n = len(cfg["temps"]) * cfg["n_chains"]
+    K, Wm = args.steps, args.warmup
+    total = K + Wm
+    P = np.concatenate([W.perturbed_params(inj, n, step=s) for s in range(total)])
+    psz = P.dtype.itemsize
+    d_params = G.DeviceArray.from_host(P)
+    d_mixed = G.DeviceArray(total * n * 8)
+    d_fp64 = G.DeviceArray(total * n * 8)
+
+    def enqueue(s, prec, out, model=G.IMRPHENOMD, dev_params=d_params, nn=n):
+        G.check(lib.gwh_loglike_batch_device(None, ev.id, model, prec, dev_params.ptr + s * nn * psz, nn,
+                                             out.ptr + s * nn * 8, None))
+
+    for s in range(Wm):
+        enqueue(s, G.MIXED, d_mixed)
+        G.synchronize()
+    # ---- timed region: K steps of the mixed engine ----
+    G.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for s in range(Wm, total):
+        ts = time.perf_counter()
+        enqueue(s, G.MIXED, d_mixed)
+        G.synchronize()
+        lat.append(time.perf_counter() - ts)
+    G.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = K * n / wall
+    lat_ms = np.array(lat) * 1e3
+    log("mixed: %.3f ms/step  %.3e evals/s  p50 %.3f ms  p99 %.3f ms" % (wall / K * 1e3, value,
+                                                                         np.percentile(lat_ms, 50),
+                                                                         np.percentile(lat_ms, 99)))
+    # ---- FP64 batched on the same batches (untimed for value) ----
+    tf0 = time.perf_counter()
+    for s in range(Wm, total):
+        enqueue(s, G.FP64, d_fp64)
+        G.synchronize()
+    tf1 = time.perf_counter()
+    fp64_rate = K * n / (tf1 - tf0)
+    mx = d_mixed.to_host(np.float64, total * n)[Wm * n:]
+    fx = d_fp64.to_host(np.float64, total * n)[Wm * n:]
+    dmax = float(np.max(np.abs(mx - fx)))
+    log("fp64 batched: %.3e evals/s ; max |dlogL| mixed vs fp64 over %d evals: %.3e" % (fp64_rate, len(mx), dmax))
+    # ---- plain FP64 one-instance-per-launch ----
+    reps = min(n, 256)
+    ts0 = time.perf_counter()
+    for i in range(reps):
+        G.check(lib.gwh_loglike_batch_device(None, ev.id, G.IMRPHENOMD, G.FP64, d_params.ptr + i * psz, 1,
+                                             d_fp64.ptr + i * 8, None))
+    G.synchronize()
+    single_rate = reps / (time.perf_counter() - ts0)
+    log("fp64 one-instance-per-launch: %.3e evals/s" % single_rate)
+    # ---- kernel-level timing of the dominant kernel (HIP events on the engine stream) ----
+    ms = np.zeros(1)
+    kms = np.zeros(1)
+    dp = ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    kp = kms.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    G.check(lib.gwh_time_loglike_batch_device(ev.id, G.IMRPHENOMD, G.MIXED, d_params.ptr, n, d_mixed.ptr, 50, dp, kp))
+    kernel_ms = float(kms[0])
+    launch_ms = float(ms[0])
+    counts = G.work_counts(ev, P[:n], G.IMRPHENOMD)
+    fe, nz = fe_per_launch(counts, len(dets))
+    achieved = fe / (kernel_ms * 1e-3) / 1e12
+    bytes_alg = nz * (8 + 24 * len(dets))
+    log("kernel %.4f ms per launch (setup+kernel+finalize %.4f ms); %.3e fe/launch -> %.2f TFLOP/s-eq; "
+        "algorithmic bytes %.3e -> %.1f GB/s" % (kernel_ms, launch_ms, fe, achieved, bytes_alg,
+                                                 bytes_alg / (kernel_ms * 1e-3) / 1e9))
+    G.check(lib.gwh_time_loglike_batch_device(ev.id, G.IMRPHENOMD, G.FP64, d_params.ptr, n, d_fp64.ptr, 20, dp, kp))
+    kernel_ms_fp64 = float(kms[0])
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "loglike_mixed_c1_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    # ---- IMRPhenomPv2 on the same grid: 1024 chains, mixed batched vs plain FP64 single launches ----
+    pv2 = {}
+    try:
+        n3 = 1024
+        P3 = W.precessing_params(n3)
+        d3 = G.DeviceArray.from_host(P3)
+        o3 = G.DeviceArray(n3 * 8)
+        o3b = G.DeviceArray(n3 * 8)
+        for _ in range(3):
+            enqueue(0, G.MIXED, o3, G.IMRPHENOMPV2, d3, n3)
+            G.synchronize()
+        reps3 = 20
+        a = time.perf_counter()
+        for _ in range(reps3):
+            enqueue(0, G.MIXED, o3, G.IMRPHENOMPV2, d3, n3)
+            G.synchronize()
+        pv2_mixed = reps3 * n3 / (time.perf_counter() - a)
+        enqueue(0, G.FP64, o3b, G.IMRPHENOMPV2, d3, n3)
+        G.synchronize()
+        pd = float(np.max(np.abs(o3.to_host(np.float64, n3) - o3b.to_host(np.float64, n3))))
+        a = time.perf_counter()
+        for _ in range(reps3):
+            enqueue(0, G.FP64, o3b, G.IMRPHENOMPV2, d3, n3)
+            G.synchronize()
+        pv2_fp64 = reps3 * n3 / (time.perf_counter() - a)
+        r1 = 256
+        a = time.perf_counter()
+        for i in range(r1):
+            G.check(lib.gwh_loglike_batch_device(None, ev.id, G.IMRPHENOMPV2, G.FP64, d3.ptr + i * psz, 1,
+                                                 o3b.ptr + i * 8, None))
+        G.synchronize()
+        pv2_single = r1 / (time.perf_counter() - a)
+        pv2 = dict(mixed_batched_evals_per_s=pv2_mixed, fp64_batched_evals_per_s=pv2_fp64,
+                   fp64_one_instance_per_launch_evals_per_s=pv2_single,
+                   speedup_mixed_batched_vs_fp64_single=pv2_mixed / pv2_single, max_abs_dlogL_mixed_vs_fp64=pd)
+        log("Pv2: mixed %.3e, fp64 batched %.3e, fp64 single %.3e evals/s, speedup %.1fx, max|dlogL| %.2e"
+            % (pv2_mixed, pv2_fp64, pv2_single, pv2_mixed / pv2_single, pd))
+    except Exception as e:  # reported, not fatal for the headline
+        log("Pv2 leg failed:", e)
+        pv2 = {"error": str(e)}
+
+    # ---- CPU baseline: the oracle (reference algorithm, C, one instance per call) ----
+    cpu = None
+    if not args.no_cpu:
+        try:
+            import oracle as O
+            cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+            plist = [O.make_params(**{k: (list(P[i][k]) if k in ("spin1", "spin2") else P[i][k].item())
+                                      for k in P.dtype.names}) for i in range(n)]
+            a = time.perf_counter()
+            O.network_loglike_batch(f, data, [psd] * 3, dets, plist[:8], nthreads=1)
+            per_eval_1t = (time.perf_counter() - a) / 8
+            m = int(max(cores, min(n * 4, args.cpu_seconds * cores / per_eval_1t)))
+            sample = [plist[i % n] for i in range(m)]
+            a = time.perf_counter()
+            out, used = O.network_loglike_batch(f, data, [psd] * 3, dets, sample, nthreads=cores)
+            el = time.perf_counter() - a
+            cpu = {"value": m / el, "unit": "logL evals/s", "cores": int(used), "kind": "port",
+                   "sample": "%d evals of the same workload (first %d instances of the first batch, cycled), "
+                             "oracle/gwo network_loglike with OpenMP over instances; single-thread %.2f ms/eval"
+                             % (m, min(m, n), per_eval_1t * 1e3),
+                   "seconds": el}
+            log("cpu baseline: %.1f evals/s on %d threads (%.1f s)" % (m / el, used, el))
+        except Exception as e:
+            log("cpu baseline failed:", e)
+            cpu = {"error": str(e)}
+
+    rec = {
+        "metric": METRIC, "value": value, "unit": "logL evals/s", "n_gpus": 1, "steps": K, "warmup": Wm,
+        "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 phase + f32 amplitude/products (mixed)", "data": "synthetic",
+        "config": {"workload": "PT ensemble step: IMRPhenomD, H1/L1/V1, 16225 bins 20-2048 Hz df=1/8 Hz, "
+                               "16 temperatures x 32 chains = 512 instances per step",
+                   "instances_per_step": n, "bins": len(f), "detectors": len(dets)},
+        "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+        "fp64_batched_evals_per_s": fp64_rate, "fp64_one_instance_per_launch_evals_per_s": single_rate,
+        "speedup_mixed_vs_fp64_one_instance_per_launch": value / single_rate,
+        "max_abs_dlogL_mixed_vs_fp64": dmax,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_VALU_TFLOPS, "traffic": traffic,
+                     "kernel": "gwh_loglike_kernel<IMRPhenomD, mixed>", "kernel_ms": kernel_ms,
+                     "fe_per_launch": fe, "launch_ms_setup_kernel_finalize": launch_ms,
+                     "hbm_algorithmic_GBs": bytes_alg / (kernel_ms * 1e-3) / 1e9, "hbm_peak_GBs": PEAK_HBM_GBS,
+                     "kernel_ms_fp64": kernel_ms_fp64},
+        "cpu_baseline": cpu,
+        "imrphenompv2_c3_likelihood": pv2,
+    }
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,146 @@
+"""MI355X engine for the gw_analysis_tools hot path.
+
+The compute lives in libgwat_hip.so (HIP for gfx950) behind the C ABI of
+include/gwat_hip.h; this package only binds it with ctypes.  There is no CPU
+fallback: importing works anywhere, but every compute call raises if the
+shared object or a HIP device is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgwat_hip.so")
+
+GWH_OK = 1
+IMRPHENOMD = 0
+IMRPHENOMPV2 = 1
+FP64 = 0
+MIXED = 1
+SIMPSONS = 0
+GAUSSLEG = 1
+DETECTORS = {"Hanford": 0, "Livingston": 1, "Virgo": 2, "Kagra": 3, "Indigo": 4, "CE": 5, "ET1": 6, "ET2": 7,
+             "ET3": 8}
+BASIS_PHENOMD_SKYAVG = 0
+BASIS_PHENOMD_FULL = 1
+BASIS_PHENOMD_MCMC = 2
+BASIS_PV2_FULL = 3
+BASIS_PV2_MCMC = 4
+
+# Every entry point of include/gwat_hip.h (checked by the CPU test suite).
+EXPORTS = [
+    "gwh_init", "gwh_last_error", "gwh_synchronize", "gwh_version",
+    "gwh_fourier_waveform", "gwh_fourier_detector_response", "gwh_log_likelihood",
+    "gwh_log_likelihood_internal", "gwh_mcmc_likelihood_extrinsic", "gwh_fisher_numerical",
+    "gwh_event_create", "gwh_event_destroy", "gwh_event_bytes", "gwh_loglike_batch",
+    "gwh_loglike_batch_device", "gwh_set_bins_per_lane", "gwh_fisher_batch",
+    "gwh_time_loglike_batch_device", "gwh_device_alloc", "gwh_device_free", "gwh_memcpy_h2d",
+    "gwh_memcpy_d2h", "gwh_work_counts", "gwh_create_coherent_GW_detection",
+]
+
+
+class GWHError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    """gwh_params: flat POD view of gen_params_base<double> (include/gwat/util.h:121-389)."""
+    _fields_ = [("mass1", ctypes.c_double), ("mass2", ctypes.c_double),
+                ("spin1", ctypes.c_double * 3), ("spin2", ctypes.c_double * 3),
+                ("DL", ctypes.c_double), ("tc", ctypes.c_double), ("phiRef", ctypes.c_double),
+                ("f_ref", ctypes.c_double), ("incl", ctypes.c_double), ("RA", ctypes.c_double),
+                ("DEC", ctypes.c_double), ("psi", ctypes.c_double), ("gmst", ctypes.c_double),
+                ("chip", ctypes.c_double), ("phip", ctypes.c_double),
+                ("shift_time", ctypes.c_int32), ("shift_phase", ctypes.c_int32),
+                ("sky_average", ctypes.c_int32), ("dep_postmerger", ctypes.c_int32)]
+
+
+PARAMS_DTYPE = np.dtype([("mass1", "f8"), ("mass2", "f8"), ("spin1", "f8", 3), ("spin2", "f8", 3), ("DL", "f8"),
+                         ("tc", "f8"), ("phiRef", "f8"), ("f_ref", "f8"), ("incl", "f8"), ("RA", "f8"),
+                         ("DEC", "f8"), ("psi", "f8"), ("gmst", "f8"), ("chip", "f8"), ("phip", "f8"),
+                         ("shift_time", "i4"), ("shift_phase", "i4"), ("sky_average", "i4"),
+                         ("dep_postmerger", "i4")])
+assert PARAMS_DTYPE.itemsize == ctypes.sizeof(Params) == 168
+
+
+def make_params(**kw):
+    p = Params()
+    p.chip = -1.0
+    p.phip = -1.0
+    p.shift_time = 1
+    p.shift_phase = 1
+    for k, v in kw.items():
+        if k in ("spin1", "spin2"):
+            getattr(p, k)[:] = list(v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def params_array(n):
+    a = np.zeros(n, dtype=PARAMS_DTYPE)
+    a["chip"] = -1.0
+    a["phip"] = -1.0
+    a["shift_time"] = 1
+    a["shift_phase"] = 1
+    return a
+
+
+_lib = None
+
+
+def lib():
+    """Load libgwat_hip.so; raises when it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GWHError("libgwat_hip.so is not built (python -m gw_analysis_tools_amd.build); "
+                       "the engine has no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    D, I, V = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    pp = ctypes.POINTER(Params)
+    dpp = ctypes.POINTER(dp)
+    sig = {
+        "gwh_init": ([I], I), "gwh_last_error": ([], ctypes.c_char_p), "gwh_synchronize": ([], I),
+        "gwh_version": ([], ctypes.c_char_p),
+        "gwh_fourier_waveform": ([dp, I, dp, dp, I, pp], I),
+        "gwh_fourier_detector_response": ([dp, I, dp, I, I, pp], I),
+        "gwh_log_likelihood": ([dp, dp, dp, ctypes.c_size_t, pp, I, I], D),
+        "gwh_log_likelihood_internal": ([dp, dp, dp, dp, dp, I, I, I], D),
+        "gwh_mcmc_likelihood_extrinsic": ([I, pp, I, ip, dpp, dpp, dpp, dpp, I, I, ip, I, D], D),
+        "gwh_fisher_numerical": ([dp, I, I, I, I, dp, I, pp, I, dp, D], I),
+        "gwh_event_create": ([dp, I, I, ip, dp, dp, ip], I),
+        "gwh_event_destroy": ([I], I), "gwh_event_bytes": ([I], ctypes.c_size_t),
+        "gwh_loglike_batch": ([ip, I, I, I, V, I, dp], I),
+        "gwh_loglike_batch_device": ([V, I, I, I, V, I, V, V], I),
+        "gwh_set_bins_per_lane": ([I], I),
+        "gwh_fisher_batch": ([I, I, I, I, V, I, I, D, dp, dp], I),
+        "gwh_time_loglike_batch_device": ([I, I, I, V, I, V, I, dp, dp], I),
+        "gwh_device_alloc": ([ctypes.c_size_t], V), "gwh_device_free": ([V], I),
+        "gwh_memcpy_h2d": ([V, V, ctypes.c_size_t], I), "gwh_memcpy_d2h": ([V, V, ctypes.c_size_t], I),
+        "gwh_work_counts": ([I, I, V, I, ip], I),
+        "gwh_create_coherent_GW_detection": ([ip, I, dp, I, I, pp, dp], I),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def last_error():
+    return lib().gwh_last_error().decode(errors="replace")
+
+
+def check(rc):
+    if rc != GWH_OK:
+        raise GWHError("gwat_hip error %d: %s" % (rc, last_error()))
+    return rc
+
+
+from .engine import *  # noqa: E402,F401,F403

@@ -1,0 +1,105 @@
+/*
+ * gwh_engine.h -- internal runtime of the engine (not part of the C ABI).
+ *
+ * One process-wide Engine per device holds the read-only model tables, the
+ * resident events (frequency grid, Simpson/Gauss-Legendre weights folded with
+ * 1/S, weighted data, FP32 copies) and grow-only scratch buffers, so the hot
+ * path never calls hipMalloc once sizes have been seen.
+ */
+#ifndef GWH_ENGINE_H
+#define GWH_ENGINE_H
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gwh_model.h"
+
+#define GWH_TPB 256          /* threads per block of the per-bin kernels (4 wave64) */
+#define GWH_MAX_EVENTS 4096
+#define GWH_PARTIAL_STRIDE (3 * GWH_MAXDET)
+
+/* Device-side view of one resident event. */
+struct EventDesc {
+    const double *f;        /* N */
+    const double *wS;       /* D x N : integration weight * 4 / S  (FP64) */
+    const double2 *dwS;     /* D x N : data * wS */
+    const float *wS32;      /* D x N : wS * sw[d] */
+    const float2 *dwS32;    /* D x N : dwS * sd[d] */
+    const double *chunk_fmin; /* ceil(N / GWH_TPB): smallest frequency of every 256-bin chunk */
+    double sw[GWH_MAXDET], sd[GWH_MAXDET];
+    int N, D, nchunk, sorted; /* sorted: frequencies ascending */
+    int det[GWH_MAXDET];
+};
+
+struct Event {
+    bool live = false;
+    int N = 0, D = 0;
+    int det[GWH_MAXDET] = {0, 0, 0, 0};
+    void *block = nullptr; /* one allocation holding every array of the event */
+    size_t bytes = 0;
+    EventDesc desc;
+    uint64_t key = 0;      /* content key for the single-call cache (0 = explicit event) */
+    double df_like = 0;    /* f[N/2] - f[N/2-1]: the likelihood's Simpson step (mcmc_gw.cpp:811) */
+    double df_fisher = 0;  /* f[1] - f[0]: the Fisher's Simpson step (fisher.cpp:2746) */
+    uint64_t last_use = 0;
+};
+
+/* grow-only device buffer */
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);
+    template <class T> T *as() { return (T *)p; }
+};
+
+struct Engine {
+    std::recursive_mutex mu;
+    bool ready = false;
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    double *d_fits = nullptr;
+    DetGeom *d_geo = nullptr;
+    double *d_qnm = nullptr;
+    QnmTable qnm;
+    EventDesc *d_events = nullptr;
+    std::vector<Event> events;
+    uint64_t use_clock = 0;
+    int bpl = 0; /* 0 = automatic */
+    /* scratch */
+    DevBuf params, coef, partial, active, logl, inst_event, f, data, psd, wts, out1, out2, misc;
+    DevBuf fisher_deriv, fisher_gram, fisher_out;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+Engine &gwh_engine();
+int gwh_set_error(int code, const std::string &msg);
+int gwh_ensure_init();
+
+/* Create an event from host arrays. method: GWH_SIMPSONS / GWH_GAUSSLEG (weights used), log10F as in
+ * Log_Likelihood_internal.  df_mode 0: the likelihood's f[N/2]-f[N/2-1]; 1: the Fisher's f[1]-f[0]. */
+int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const double *data, const double *psd,
+                        int method, const double *weights, int log10F, int df_mode, uint64_t key, int *event);
+int gwh_choose_bpl(int N);
+
+/* the batched likelihood on device buffers (engine lock held by caller) */
+int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
+                       double *d_logl, hipStream_t stream, bool timed, double *kernel_ms);
+
+/* per-instance setup on device buffers */
+int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_params *d_params, int n, InstCoef *d_coef,
+                     hipStream_t stream);
+
+uint64_t gwh_hash_bytes(const void *p, size_t n, uint64_t seed);
+
+#define GWH_HIP(call)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return gwh_set_error(GWH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+#endif

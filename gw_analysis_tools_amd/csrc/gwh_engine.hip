@@ -1,0 +1,1382 @@
+/*
+ * gwh_engine.hip -- runtime, resident events and the batched network
+ * log-likelihood of IMRPhenomD / IMRPhenomPv2 on MI355X (gfx950).
+ *
+ * Hot path (one MCMC step):
+ *   1. gwh_setup_kernel      one lane per instance: FP64 InstCoef record
+ *   2. loglike kernel        one block per (bin-tile, instance); tiles above
+ *                            0.2/M of an ascending grid exit at once; one lane
+ *                            per bin, the instance record is uniform (scalar
+ *                            loads); per
+ *                            detector accumulate |h|^2 w/S and h* d w/S; wave
+ *                            shuffle + LDS tree per block -> per-item partials
+ *   3. gwh_finalize_kernel   one lane per instance sums its items in a fixed
+ *                            order (deterministic, batch-composition free)
+ *
+ * Reference: src/mcmc_gw.cpp:2374-2489 (MCMC_likelihood_extrinsic),
+ * :801-868 (Log_Likelihood_internal), src/waveform_util.cpp:152-184
+ * (create_coherent_GW_detection_reuse_WF), src/IMRPhenomD.cpp:403-504.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gwh_engine.h"
+#include "gwh_tables.h"
+
+/* ------------------------------------------------------------------------- */
+/* runtime                                                                    */
+/* ------------------------------------------------------------------------- */
+
+static thread_local std::string g_err;
+
+int gwh_set_error(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+Engine &gwh_engine()
+{
+    static Engine *e = new Engine(); /* lives until process exit */
+    return *e;
+}
+
+int DevBuf::ensure(size_t bytes)
+{
+    if (bytes <= cap) return GWH_OK;
+    if (p) {
+        hipError_t e = hipFree(p);
+        (void)e;
+        p = nullptr;
+        cap = 0;
+    }
+    size_t nb = bytes < 4096 ? 4096 : bytes + bytes / 4;
+    if (hipMalloc(&p, nb) != hipSuccess) {
+        p = nullptr;
+        return gwh_set_error(GWH_ERR_NOMEM, "hipMalloc of " + std::to_string(nb) + " bytes failed");
+    }
+    cap = nb;
+    return GWH_OK;
+}
+
+uint64_t gwh_hash_bytes(const void *p, size_t n, uint64_t seed)
+{
+    const unsigned char *b = (const unsigned char *)p;
+    uint64_t h = seed ^ 0x9e3779b97f4a7c15ULL;
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        memcpy(&w, b + i, 8);
+        h ^= w;
+        h *= 0xff51afd7ed558ccdULL;
+        h ^= h >> 29;
+    }
+    for (; i < n; i++) {
+        h ^= b[i];
+        h *= 0x100000001b3ULL;
+    }
+    h ^= h >> 31;
+    return h;
+}
+
+/* natural cubic spline second derivatives / 2 (the gsl_interp_cspline system), host side */
+static void spline_c(const double *x, const double *y, int n, double *c)
+{
+    std::vector<double> dg(n), rh(n), of(n);
+    c[0] = c[n - 1] = 0;
+    int m = n - 2;
+    for (int i = 0; i < m; i++) {
+        double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
+        dg[i] = 2.0 * (h0 + h1);
+        of[i] = h1;
+        rh[i] = 3.0 * ((y[i + 2] - y[i + 1]) / h1 - (y[i + 1] - y[i]) / h0);
+    }
+    for (int i = 1; i < m; i++) {
+        double w = of[i - 1] / dg[i - 1];
+        dg[i] -= w * of[i - 1];
+        rh[i] -= w * rh[i - 1];
+    }
+    c[m] = rh[m - 1] / dg[m - 1];
+    for (int i = m - 2; i >= 0; i--) c[i + 1] = (rh[i] - of[i] * c[i + 2]) / dg[i];
+}
+
+static const double k_geometric_factor[GWH_NDET_TABLE] = {1., 1., 1., 1., 1., 1., 0.86602540378, 0.86602540378,
+                                                          0.86602540378};
+
+int gwh_ensure_init()
+{
+    Engine &E = gwh_engine();
+    if (E.ready) return GWH_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return gwh_set_error(GWH_ERR_HIP, "no HIP device visible: the engine has no CPU fallback");
+    if (E.device >= ndev) E.device = 0;
+    GWH_HIP(hipSetDevice(E.device));
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, E.device) == hipSuccess && cu > 0)
+        E.num_cu = cu;
+    GWH_HIP(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+    GWH_HIP(hipEventCreate(&E.ev0));
+    GWH_HIP(hipEventCreate(&E.ev1));
+    GWH_HIP(hipMalloc(&E.d_fits, sizeof(gwh_phenomd_fit)));
+    GWH_HIP(hipMemcpy(E.d_fits, gwh_phenomd_fit, sizeof(gwh_phenomd_fit), hipMemcpyHostToDevice));
+    DetGeom geo[GWH_NDET_TABLE];
+    for (int d = 0; d < GWH_NDET_TABLE; d++) {
+        for (int i = 0; i < 9; i++) geo[d].tensor[i] = gwh_det_tensor[d][i];
+        for (int i = 0; i < 3; i++) geo[d].loc[i] = gwh_det_location[d][i];
+        geo[d].gfac = k_geometric_factor[d];
+    }
+    GWH_HIP(hipMalloc(&E.d_geo, sizeof(geo)));
+    GWH_HIP(hipMemcpy(E.d_geo, geo, sizeof(geo), hipMemcpyHostToDevice));
+    const int n = GWH_QNM_N;
+    std::vector<double> host(5 * n);
+    memcpy(&host[0], gwh_qnm_spin, n * sizeof(double));
+    memcpy(&host[n], gwh_qnm_fring, n * sizeof(double));
+    memcpy(&host[2 * n], gwh_qnm_fdamp, n * sizeof(double));
+    spline_c(gwh_qnm_spin, gwh_qnm_fring, n, &host[3 * n]);
+    spline_c(gwh_qnm_spin, gwh_qnm_fdamp, n, &host[4 * n]);
+    GWH_HIP(hipMalloc(&E.d_qnm, host.size() * sizeof(double)));
+    GWH_HIP(hipMemcpy(E.d_qnm, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+    E.qnm.spin = E.d_qnm;
+    E.qnm.fring = E.d_qnm + n;
+    E.qnm.fdamp = E.d_qnm + 2 * n;
+    E.qnm.c_ring = E.d_qnm + 3 * n;
+    E.qnm.c_damp = E.d_qnm + 4 * n;
+    E.qnm.n = n;
+    GWH_HIP(hipMalloc(&E.d_events, sizeof(EventDesc) * GWH_MAX_EVENTS));
+    GWH_HIP(hipMemset(E.d_events, 0, sizeof(EventDesc) * GWH_MAX_EVENTS));
+    E.events.assign(GWH_MAX_EVENTS, Event());
+    E.ready = true;
+    return GWH_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* event preparation                                                          */
+/* ------------------------------------------------------------------------- */
+
+/* w_k of simpsons_sum (util.h:841-856): 1,4,2,4,...,(2|4),1 by index parity for any n */
+__device__ __forceinline__ double simpson_w(int k, int n)
+{
+    if (k == 0 || k == n - 1) return (n == 1) ? 2.0 : 1.0;
+    return (k & 1) ? 4.0 : 2.0;
+}
+
+__global__ void gwh_event_prep_kernel(const double *__restrict__ f, const double *__restrict__ psd,
+                                      const double2 *__restrict__ data, const double *__restrict__ wts, int N, int D,
+                                      int method, int log10F, double dfac, double *__restrict__ wS,
+                                      double2 *__restrict__ dwS)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int d = blockIdx.y;
+    if (k >= N || d >= D) return;
+    double w;
+    if (method == GWH_GAUSSLEG) {
+        w = 4.0 * wts[k];
+        if (log10F) w *= f[k] * 2.302585092994045684017991454684364208;
+    } else {
+        w = 4.0 * dfac * simpson_w(k, N);
+    }
+    double s = w / psd[(size_t)d * N + k];
+    wS[(size_t)d * N + k] = s;
+    double2 x = data[(size_t)d * N + k];
+    dwS[(size_t)d * N + k] = make_double2(x.x * s, x.y * s);
+}
+
+__global__ void gwh_event_scale_kernel(const double *__restrict__ wS, const double2 *__restrict__ dwS, int N,
+                                       const double *__restrict__ sw, const double *__restrict__ sd,
+                                       float *__restrict__ wS32, float2 *__restrict__ dwS32)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int d = blockIdx.y;
+    if (k >= N) return;
+    size_t i = (size_t)d * N + k;
+    wS32[i] = (float)(wS[i] * sw[d]);
+    double2 x = dwS[i];
+    dwS32[i] = make_float2((float)(x.x * sd[d]), (float)(x.y * sd[d]));
+}
+
+__global__ void gwh_chunk_fmin_kernel(const double *__restrict__ f, int N, double *__restrict__ out)
+{
+    __shared__ double s[GWH_TPB];
+    int k = blockIdx.x * GWH_TPB + threadIdx.x;
+    s[threadIdx.x] = (k < N) ? f[k] : __longlong_as_double(0x7ff0000000000000LL);
+    __syncthreads();
+    for (int w = GWH_TPB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const double *data, const double *psd,
+                        int method, const double *weights, int log10F, int df_mode, uint64_t key, int *event)
+{
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    Engine &E = gwh_engine();
+    if (!f || !data || !psd || !dets || !event) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    if (N < 2) return gwh_set_error(GWH_ERR_INVALID, "need at least 2 frequency bins");
+    if (ndet < 1 || ndet > GWH_MAXDET) return gwh_set_error(GWH_ERR_INVALID, "1 to 4 detectors per event");
+    for (int d = 0; d < ndet; d++)
+        if (dets[d] < 0 || dets[d] >= GWH_NDET_TABLE)
+            return gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported detector id " + std::to_string(dets[d]));
+    if (method == GWH_GAUSSLEG && !weights) return gwh_set_error(GWH_ERR_INVALID, "GAUSSLEG needs weights");
+    int slot = -1;
+    /* the last slot is reserved for the single-call waveform/response descriptor */
+    for (int i = 0; i < GWH_MAX_EVENTS - 1; i++)
+        if (!E.events[i].live) {
+            slot = i;
+            break;
+        }
+    if (slot < 0) return gwh_set_error(GWH_ERR_NOMEM, "event table full");
+    int nchunk = (N + GWH_TPB - 1) / GWH_TPB;
+    size_t Nd = (size_t)N * ndet;
+    size_t off_f = 0, off_wS = align256(off_f + N * 8), off_dwS = align256(off_wS + Nd * 8),
+           off_w32 = align256(off_dwS + Nd * 16), off_dw32 = align256(off_w32 + Nd * 4),
+           off_cm = align256(off_dw32 + Nd * 8), total = align256(off_cm + nchunk * 8);
+    char *blk = nullptr;
+    if (hipMalloc((void **)&blk, total) != hipSuccess) return gwh_set_error(GWH_ERR_NOMEM, "event allocation failed");
+    /* staging: psd, data, weights */
+    int rc = E.psd.ensure(Nd * 8);
+    if (rc == GWH_OK) rc = E.data.ensure(Nd * 16);
+    if (rc == GWH_OK && weights) rc = E.wts.ensure((size_t)N * 8);
+    if (rc == GWH_OK) rc = E.misc.ensure(2 * GWH_MAXDET * 8);
+    if (rc != GWH_OK) {
+        (void)hipFree(blk);
+        return rc;
+    }
+    GWH_HIP(hipMemcpyAsync(blk + off_f, f, N * 8, hipMemcpyHostToDevice, E.stream));
+    GWH_HIP(hipMemcpyAsync(E.psd.p, psd, Nd * 8, hipMemcpyHostToDevice, E.stream));
+    GWH_HIP(hipMemcpyAsync(E.data.p, data, Nd * 16, hipMemcpyHostToDevice, E.stream));
+    if (weights) GWH_HIP(hipMemcpyAsync(E.wts.p, weights, (size_t)N * 8, hipMemcpyHostToDevice, E.stream));
+    double dfac = (df_mode == 0) ? (f[N / 2] - f[N / 2 - 1]) / 3.0 : (f[1] - f[0]) / 3.0;
+    dim3 grid((N + GWH_TPB - 1) / GWH_TPB, ndet);
+    hipLaunchKernelGGL(gwh_event_prep_kernel, grid, dim3(GWH_TPB), 0, E.stream, (const double *)(blk + off_f),
+                       E.psd.as<const double>(), E.data.as<const double2>(), E.wts.as<const double>(), N, ndet, method,
+                       log10F, dfac, (double *)(blk + off_wS), (double2 *)(blk + off_dwS));
+    GWH_HIP(hipGetLastError());
+    /* FP32 scales: largest weight and largest |weighted data| map to 1 */
+    std::vector<double> hwS(Nd), hdw(2 * Nd);
+    GWH_HIP(hipMemcpyAsync(hwS.data(), blk + off_wS, Nd * 8, hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipMemcpyAsync(hdw.data(), blk + off_dwS, Nd * 16, hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    double sc[2 * GWH_MAXDET];
+    for (int d = 0; d < GWH_MAXDET; d++) sc[d] = sc[GWH_MAXDET + d] = 1.0;
+    for (int d = 0; d < ndet; d++) {
+        double mw = 0, md = 0;
+        for (int k = 0; k < N; k++) {
+            double a = std::fabs(hwS[(size_t)d * N + k]);
+            if (std::isfinite(a) && a > mw) mw = a;
+            double b = std::hypot(hdw[2 * ((size_t)d * N + k)], hdw[2 * ((size_t)d * N + k) + 1]);
+            if (std::isfinite(b) && b > md) md = b;
+        }
+        sc[d] = (mw > 0) ? 1.0 / mw : 1.0;
+        sc[GWH_MAXDET + d] = (md > 0) ? 1.0 / md : 1.0;
+    }
+    GWH_HIP(hipMemcpyAsync(E.misc.p, sc, sizeof(sc), hipMemcpyHostToDevice, E.stream));
+    hipLaunchKernelGGL(gwh_event_scale_kernel, grid, dim3(GWH_TPB), 0, E.stream, (const double *)(blk + off_wS),
+                       (const double2 *)(blk + off_dwS), N, E.misc.as<const double>(),
+                       E.misc.as<const double>() + GWH_MAXDET, (float *)(blk + off_w32), (float2 *)(blk + off_dw32));
+    GWH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gwh_chunk_fmin_kernel, dim3(nchunk), dim3(GWH_TPB), 0, E.stream,
+                       (const double *)(blk + off_f), N, (double *)(blk + off_cm));
+    GWH_HIP(hipGetLastError());
+    Event &ev = E.events[slot];
+    ev.live = true;
+    ev.N = N;
+    ev.D = ndet;
+    ev.block = blk;
+    ev.bytes = total;
+    ev.key = key;
+    ev.last_use = ++E.use_clock;
+    /* the stored weights carry the step of df_mode (df_like below); the Fisher rescales to f[1]-f[0] */
+    double df_l = f[N / 2] - f[N / 2 - 1], df_f = f[1] - f[0];
+    ev.df_like = (df_mode == 0) ? df_l : df_f;
+    ev.df_fisher = df_f;
+    EventDesc &ds = ev.desc;
+    memset(&ds, 0, sizeof(ds));
+    ds.f = (const double *)(blk + off_f);
+    ds.wS = (const double *)(blk + off_wS);
+    ds.dwS = (const double2 *)(blk + off_dwS);
+    ds.wS32 = (const float *)(blk + off_w32);
+    ds.dwS32 = (const float2 *)(blk + off_dw32);
+    ds.chunk_fmin = (const double *)(blk + off_cm);
+    ds.N = N;
+    ds.D = ndet;
+    ds.nchunk = nchunk;
+    ds.sorted = 1;
+    for (int k = 1; k < N; k++)
+        if (!(f[k] >= f[k - 1])) {
+            ds.sorted = 0;
+            break;
+        }
+    for (int d = 0; d < GWH_MAXDET; d++) {
+        ds.det[d] = (d < ndet) ? dets[d] : dets[0];
+        ev.det[d] = ds.det[d];
+        ds.sw[d] = sc[d];
+        ds.sd[d] = sc[GWH_MAXDET + d];
+    }
+    GWH_HIP(hipMemcpyAsync(E.d_events + slot, &ds, sizeof(ds), hipMemcpyHostToDevice, E.stream));
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    *event = slot;
+    return GWH_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* setup                                                                      */
+/* ------------------------------------------------------------------------- */
+
+__global__ void gwh_setup_kernel(const gwh_params *__restrict__ params, const int *__restrict__ inst_event,
+                                 int event0, int n, int model, const double *__restrict__ fits, QnmTable qnm,
+                                 const DetGeom *__restrict__ geo, const EventDesc *__restrict__ events,
+                                 InstCoef *__restrict__ coef, int tile, int *__restrict__ active)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ev = inst_event ? inst_event[i] : event0;
+    const EventDesc &E = events[ev];
+    gwh_params p = params[i];
+    InstCoef C;
+    gwh_setup_instance(p, model, fits, qnm, geo, E.det, E.D, C);
+    coef[i] = C;
+    if (active) {
+        /* tiles to visit: all of them, or on an ascending grid those starting at or below 0.2/M */
+        int nt = (E.N + tile - 1) / tile;
+        int cpt = tile / GWH_TPB; /* 256-bin chunks per tile */
+        if (E.sorted) {
+            int lo = 0, hi = nt; /* first tile whose smallest frequency exceeds the cutoff */
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (E.chunk_fmin[mid * cpt] > C.fcut)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            nt = (C.fcut == C.fcut) ? lo : nt; /* NaN cutoff: visit everything, NaN propagates */
+        }
+        active[i] = nt;
+    }
+}
+
+int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_params *d_params, int n, InstCoef *d_coef,
+                     hipStream_t stream)
+{
+    Engine &E = gwh_engine();
+    hipLaunchKernelGGL(gwh_setup_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_params, d_events, events0, n,
+                       model, (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo,
+                       (const EventDesc *)E.d_events, d_coef, GWH_TPB, (int *)nullptr);
+    GWH_HIP(hipGetLastError());
+    return GWH_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* reductions                                                                 */
+/* ------------------------------------------------------------------------- */
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+/* Block-reduce NV values per thread (fixed order) and write them to out[0..NV) */
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double *__restrict__ out, double (*lds)[NV])
+{
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; j++) acc[j] = wave_sum(acc[j]);
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) lds[wv][j] = acc[j];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < NV) {
+        int j = threadIdx.x;
+        double s = lds[0][j];
+        for (int w = 1; w < GWH_TPB / 64; w++) s += lds[w][j];
+        out[j] = s;
+    }
+    __syncthreads();
+}
+
+struct KahanF {
+    float s, c;
+    __device__ __forceinline__ void zero()
+    {
+        s = 0.f;
+        c = 0.f;
+    }
+    __device__ __forceinline__ void add(float x)
+    {
+        float y = x - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+
+/* sin and cos of an FP64 phase: FP64 reduction to a fraction of a turn in [-1/2, 1/2], then the FP32
+ * sin(pi x), cos(pi x) of x = 2 * fraction (exact scaling; a short polynomial, no large-argument path) */
+__device__ __forceinline__ void sincos_turns(double phase, float &s, float &c)
+{
+    double r = phase * 0.15915494309189533577;
+    r = r - rint(r);
+    sincospif((float)(2.0 * r), &s, &c);
+}
+
+/* e^x for x <= 0 of moderate size: FP64 split x log2(e) = n + t, FP32 2^t, exact scaling by 2^n */
+__device__ __forceinline__ double exp_mixed(double x)
+{
+    double y = x * 1.4426950408889634074;
+    double n = rint(y);
+    float t = (float)(y - n);
+    return ldexp((double)__builtin_amdgcn_exp2f(t), (int)n);
+}
+
+/* ------------------------------------------------------------------------- */
+/* per-bin mixed-precision evaluators                                         */
+/* ------------------------------------------------------------------------- */
+
+/* x^{-1/3} in FP64 from an FP32 seed (v_log_f32 / v_exp_f32) and two Newton steps (|rel err| ~ 1e-16) */
+__device__ __forceinline__ double rcbrt_mixed(double x)
+{
+    float xf = (float)x;
+    double r = (double)__builtin_amdgcn_exp2f(-0.333333333f * __builtin_amdgcn_logf(xf));
+    r = r + r * (1.0 - x * r * r * r) * 0.33333333333333333;
+    r = r + r * (1.0 - x * r * r * r) * 0.33333333333333333;
+    return r;
+}
+
+/* z^{-1/2} in FP64 from an FP32 v_rsq_f32 seed and two Newton steps */
+__device__ __forceinline__ double rsqrt_mixed(double z)
+{
+    double y = (double)__builtin_amdgcn_rsqf((float)z);
+    y = y * (1.5 - 0.5 * z * y * y);
+    y = y * (1.5 - 0.5 * z * y * y);
+    return y;
+}
+
+/* IMRPhenomD in mixed precision.  Phase: FP64 arithmetic with FP64 log (inspiral, intermediate) and an
+ * FP32 arctangent for the small merger-ringdown atan term.  Amplitude: FP64 polynomials, the powers of Mf
+ * from Newton-refined FP32 seeds and an FP32 exponential (merger-ringdown), rounded once to FP32. */
+__device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, float &a, double &phi, double &u_out,
+                                                  double &lx_out)
+{
+    double Mf = C.M * f;
+    double r = rcbrt_mixed(Mf);          /* (Mf)^{-1/3} */
+    double u = Mf * r * r;               /* (Mf)^{1/3} */
+    double r2 = r * r;
+    double inv76 = r2 * r2 * rsqrt_mixed(r); /* (Mf)^{-4/3} (Mf)^{1/6} = (Mf)^{-7/6} */
+    double ar;
+    if (f < C.f1) {
+        double pn = C.ca[0] + u * u * (C.ca[2] + u * (C.ca[3] + u * (C.ca[4] + u * (C.ca[5] + u * C.ca[6]))));
+        double u3 = u * u * u;
+        ar = pn + u3 * u3 * u * (C.rho[0] + u * (C.rho[1] + u * C.rho[2]));
+    } else if (f > C.f3) {
+        double dl = f - C.fRD;
+        ar = C.mr_k * exp_mixed(-C.mr_g2 * dl) / (dl * dl + C.mr_w2);
+    } else {
+        ar = C.delta[0] + Mf * (C.delta[1] + Mf * (C.delta[2] + Mf * (C.delta[3] + Mf * C.delta[4])));
+    }
+    a = (float)(ar * inv76);
+    double psi;
+    double lx = 0;
+    if (f < C.f1p) {
+        double v = u * 1.4645918875615231;
+        lx = log(GWH_PI * Mf);
+        double c5 = C.c8 + lx * C.c9;
+        double c6 = C.c10 - C.c11 * lx;
+        double v2 = v * v;
+        double poly = C.cp[0] + v2 * (C.cp[2] + v * (C.cp[3] + v * (C.cp[4] + v * (c5 + v * (c6 + v * C.cp[7])))));
+        double iv = r * 0.68278406325529568; /* 1/v = pi^{-1/3} (Mf)^{-1/3} */
+        double iv2 = iv * iv;
+        double Mf43 = Mf * u;
+        psi = -GWH_PI / 4. + C.K * poly * (iv2 * iv2 * iv) +
+              (C.sig[0] * Mf + C.sig[1] * Mf43 + C.sig[2] * Mf43 * u + C.sig[3] * Mf * Mf);
+    } else if (f > C.f2p) {
+        double s4 = rsqrt_mixed(rsqrt_mixed(Mf)); /* (Mf)^{1/4} */
+        psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] * (r2 * r) + C.alpha[3] * Mf * rsqrt_mixed(Mf) * s4 +
+              C.alpha[4] * (double)atanf((float)((f - C.alpha[5]) / C.fdamp));
+    } else {
+        double r3 = r2 * r;
+        psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * log(Mf) - C.beta[3] * (r3 * r3 * r3);
+    }
+    phi = psi - (C.tc_total * (f - C.f_ref) + C.phic);
+    u_out = u;
+    lx_out = lx;
+}
+
+/* ------------------------------------------------------------------------- */
+/* the batched likelihood kernels                                             */
+/* ------------------------------------------------------------------------- */
+
+struct WorkMap {
+    const int *inst_event; /* NULL: every instance uses event0 */
+    int event0, n, tiles0; /* tiles0: tile stride of the partial buffer */
+};
+
+template <int MODEL, int PREC, int BPL>
+__global__ __launch_bounds__(GWH_TPB) void gwh_loglike_kernel(WorkMap W, const EventDesc *__restrict__ events,
+                                                              const InstCoef *__restrict__ coefs,
+                                                              double *__restrict__ partial, const int *__restrict__ active)
+{
+    __shared__ double lds[GWH_TPB / 64][GWH_PARTIAL_STRIDE];
+    {
+        const int inst = blockIdx.y;
+        const int tile = blockIdx.x;
+        if (tile >= active[inst]) return; /* above 0.2/M on an ascending grid: h = 0, nothing to add */
+        const int w = inst * W.tiles0 + tile;
+        const int ev = W.inst_event ? W.inst_event[inst] : W.event0;
+        const EventDesc &E = events[ev];
+        const InstCoef &C = coefs[inst];
+        const int N = E.N, D = E.D;
+        double acc[GWH_PARTIAL_STRIDE];
+#pragma unroll
+        for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) acc[j] = 0;
+        /* skip tiles entirely above the 0.2/M cutoff (h = 0 there, IMRPhenomD.cpp:475) */
+        double fmin_tile = E.chunk_fmin[tile * BPL];
+#pragma unroll
+        for (int j = 1; j < BPL; j++)
+            if (tile * BPL + j < E.nchunk) fmin_tile = fmin(fmin_tile, E.chunk_fmin[tile * BPL + j]);
+        if (!(fmin_tile > C.fcut)) {
+            if (PREC == GWH_FP64) {
+#pragma unroll
+                for (int j = 0; j < BPL; j++) {
+                    int k = tile * GWH_TPB * BPL + j * GWH_TPB + threadIdx.x;
+                    if (k >= N) break;
+                    double f = E.f[k];
+                    if (f > C.fcut) continue;
+                    if (MODEL == GWH_IMRPHENOMD) {
+                        double A, phi, u, lx;
+                        gwh_phenomd_fp64(C, f, A, phi, u, lx);
+#pragma unroll
+                        for (int d = 0; d < GWH_MAXDET; d++) {
+                            if (d >= D) break;
+                            double s, c;
+                            sincos(phi + C.det[d][3] * f, &s, &c);
+                            double ws = E.wS[(size_t)d * N + k];
+                            double2 dw = E.dwS[(size_t)d * N + k];
+                            acc[3 * d] += A * A * ws;
+                            acc[3 * d + 1] += A * (c * dw.x - s * dw.y);
+                            acc[3 * d + 2] += A * (c * dw.y + s * dw.x);
+                        }
+                    } else {
+                        double A, Phi, hpr, hpi, hcr, hci;
+                        gwh_pv2_fp64(C, f, A, Phi, hpr, hpi, hcr, hci);
+#pragma unroll
+                        for (int d = 0; d < GWH_MAXDET; d++) {
+                            if (d >= D) break;
+                            double Gr = C.det[d][0] * hpr + C.det[d][1] * hcr;
+                            double Gi = C.det[d][0] * hpi + C.det[d][1] * hci;
+                            double s, c;
+                            sincos(Phi + C.det[d][3] * f, &s, &c);
+                            double ws = E.wS[(size_t)d * N + k];
+                            double2 dw = E.dwS[(size_t)d * N + k];
+                            acc[3 * d] += A * A * (Gr * Gr + Gi * Gi) * ws;
+                            acc[3 * d + 1] += A * (dw.x * (Gr * c + Gi * s) - dw.y * (Gr * s - Gi * c));
+                        }
+                    }
+                }
+            } else {
+                KahanF k3[GWH_PARTIAL_STRIDE];
+#pragma unroll
+                for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) k3[j].zero();
+#pragma unroll
+                for (int j = 0; j < BPL; j++) {
+                    int k = tile * GWH_TPB * BPL + j * GWH_TPB + threadIdx.x;
+                    if (k >= N) break;
+                    double f = E.f[k];
+                    if (f > C.fcut) continue;
+                    if (MODEL == GWH_IMRPHENOMD) {
+                        float a;
+                        double phi, u, lx;
+                        gwh_phenomd_mixed(C, f, a, phi, u, lx);
+#pragma unroll
+                        for (int d = 0; d < GWH_MAXDET; d++) {
+                            if (d >= D) break;
+                            float s, c;
+                            sincos_turns(phi + C.det[d][3] * f, s, c);
+                            float ws = E.wS32[(size_t)d * N + k];
+                            float2 dw = E.dwS32[(size_t)d * N + k];
+                            k3[3 * d].add(a * a * ws);
+                            k3[3 * d + 1].add(a * (c * dw.x - s * dw.y));
+                            k3[3 * d + 2].add(a * (c * dw.y + s * dw.x));
+                        }
+                    } else {
+                        float a;
+                        double phi, u, lx;
+                        gwh_phenomd_mixed(C, f, a, phi, u, lx);
+                        /* precession in FP64 for the angles, FP32 for the Wigner factors */
+                        double v = u * 1.4645918875615231;
+                        double logw = (f < C.f1p) ? lx : log(GWH_PI * C.M * f);
+                        double x = v * v;
+                        double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) / v;
+                        double sv = C.SP / (L + C.SL);
+                        double cbd = rsqrt_mixed(1.0 + sv * sv);
+                        double cbhd = sqrt(0.5 * (1.0 + cbd)), sbhd = sqrt(0.5 * (1.0 - cbd));
+                        double c2d = cbhd * cbhd, s2d = sbhd * sbhd;
+                        double d2[5] = {s2d * s2d, 2. * cbhd * s2d * sbhd, GWH_SQRT_6 * s2d * c2d, 2. * c2d * cbhd * sbhd,
+                                        c2d * c2d};
+                        double alpha = gwh_euler_eval(C.a_c, v, logw) + C.alpha_add;
+                        double eps = gwh_euler_eval(C.e_c, v, logw) + C.eps_add;
+                        float saf, caf;
+                        sincos_turns(alpha, saf, caf);
+                        double sa = saf, ca = caf;
+                        double er[5], ei[5];
+                        er[2] = 1.;
+                        ei[2] = 0.;
+                        er[3] = ca;
+                        ei[3] = sa;
+                        er[1] = ca;
+                        ei[1] = -sa;
+                        er[4] = ca * ca - sa * sa;
+                        ei[4] = 2. * ca * sa;
+                        er[0] = er[4];
+                        ei[0] = -ei[4];
+                        double hpr = 0, hpi = 0, hcr = 0, hci = 0;
+#pragma unroll
+                        for (int m = -2; m <= 2; m++) {
+                            double dm2 = (m == -2) ? d2[4] : (m == -1) ? -d2[3] : (m == 0) ? d2[2] : (m == 1) ? -d2[1] : d2[0];
+                            double y = C.ylm[m + 2];
+                            double t2r = er[-m + 2] * dm2 * y, t2i = ei[-m + 2] * dm2 * y;
+                            double tmr = er[m + 2] * d2[m + 2] * y, tmi = ei[m + 2] * d2[m + 2] * y;
+                            hpr += t2r + tmr;
+                            hpi += t2i + tmi;
+                            hcr -= t2i - tmi;
+                            hci += t2r - tmr;
+                        }
+                        double Phi = phi + 2. * eps + C.t_corr_2pi * f;
+                        double ad = a;
+#pragma unroll
+                        for (int d = 0; d < GWH_MAXDET; d++) {
+                            if (d >= D) break;
+                            double P = C.det[d][0], Q = C.det[d][1];
+                            double Gr = P * hpr + Q * hcr, Gi = P * hpi + Q * hci;
+                            float sf, cf;
+                            sincos_turns(Phi + C.det[d][3] * f, sf, cf);
+                            double s = sf, c = cf;
+                            double ws = E.wS32[(size_t)d * N + k];
+                            float2 dwf = E.dwS32[(size_t)d * N + k];
+                            double dwx = dwf.x, dwy = dwf.y;
+                            k3[3 * d].add((float)(ad * ad * (Gr * Gr + Gi * Gi) * ws));
+                            k3[3 * d + 1].add((float)(ad * (dwx * (Gr * c + Gi * s) - dwy * (Gr * s - Gi * c))));
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) acc[j] = k3[j].value();
+            }
+        }
+        block_reduce_store<GWH_PARTIAL_STRIDE>(acc, partial + (size_t)w * GWH_PARTIAL_STRIDE, lds);
+    }
+}
+
+__global__ void gwh_finalize_kernel(WorkMap W, const EventDesc *__restrict__ events, const InstCoef *__restrict__ coefs,
+                                    const double *__restrict__ partial, const int *__restrict__ active, int model,
+                                    int prec, double *__restrict__ logl)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W.n) return;
+    int ev = W.inst_event ? W.inst_event[i] : W.event0;
+    const EventDesc &E = events[ev];
+    const InstCoef &C = coefs[i];
+    int t0 = i * W.tiles0, nt = active[i];
+    double s[GWH_PARTIAL_STRIDE];
+    for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) s[j] = 0;
+    for (int t = 0; t < nt; t++)
+        for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) s[j] += partial[(size_t)(t0 + t) * GWH_PARTIAL_STRIDE + j];
+    double ll = 0;
+    for (int d = 0; d < E.D; d++) {
+        double hh, dh;
+        if (model == GWH_IMRPHENOMD) {
+            double ha = s[3 * d], zr = s[3 * d + 1], zi = s[3 * d + 2];
+            if (prec == GWH_MIXED) {
+                ha *= C.amp_pref * C.amp_pref / E.sw[d];
+                zr *= C.amp_pref / E.sd[d];
+                zi *= C.amp_pref / E.sd[d];
+            }
+            hh = C.det[d][2] * ha;
+            dh = C.det[d][0] * zr + C.det[d][1] * zi;
+        } else {
+            hh = s[3 * d];
+            dh = s[3 * d + 1];
+            if (prec == GWH_MIXED) {
+                hh *= C.amp_pref * C.amp_pref / E.sw[d];
+                dh *= C.amp_pref / E.sd[d];
+            }
+        }
+        ll += -0.5 * (hh - 2 * dh);
+    }
+    logl[i] = ll;
+}
+
+int gwh_choose_bpl(int N)
+{
+    Engine &E = gwh_engine();
+    if (E.bpl > 0) return E.bpl;
+    if (N <= 8192) return 1;
+    if (N <= 65536) return 2;
+    return 4;
+}
+
+template <int MODEL, int PREC>
+static void launch_bpl(int bpl, dim3 grid, hipStream_t st, const WorkMap &W, const EventDesc *ev, const InstCoef *c,
+                       double *part, const int *act)
+{
+    switch (bpl) {
+    case 1: hipLaunchKernelGGL((gwh_loglike_kernel<MODEL, PREC, 1>), grid, dim3(GWH_TPB), 0, st, W, ev, c, part, act); break;
+    case 2: hipLaunchKernelGGL((gwh_loglike_kernel<MODEL, PREC, 2>), grid, dim3(GWH_TPB), 0, st, W, ev, c, part, act); break;
+    case 4: hipLaunchKernelGGL((gwh_loglike_kernel<MODEL, PREC, 4>), grid, dim3(GWH_TPB), 0, st, W, ev, c, part, act); break;
+    default: hipLaunchKernelGGL((gwh_loglike_kernel<MODEL, PREC, 8>), grid, dim3(GWH_TPB), 0, st, W, ev, c, part, act); break;
+    }
+}
+
+int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
+                       double *d_logl, hipStream_t stream, bool timed, double *kernel_ms)
+{
+    Engine &E = gwh_engine();
+    if (n <= 0) return GWH_OK;
+    if (model != GWH_IMRPHENOMD && model != GWH_IMRPHENOMPV2)
+        return gwh_set_error(GWH_ERR_UNSUPPORTED, "model must be GWH_IMRPHENOMD or GWH_IMRPHENOMPV2");
+    if (precision != GWH_FP64 && precision != GWH_MIXED) return gwh_set_error(GWH_ERR_INVALID, "bad precision");
+    if (events0 < 0 || events0 >= GWH_MAX_EVENTS || (!d_events && !E.events[events0].live))
+        return gwh_set_error(GWH_ERR_INVALID, "unknown event " + std::to_string(events0));
+    if (n > 65535) return gwh_set_error(GWH_ERR_INVALID, "at most 65535 instances per launch");
+    /* bins per lane from the largest event involved: the tiling of an instance depends only on the
+     * registered events, never on the batch composition (bitwise-reproducible results) */
+    int maxN = 0;
+    if (d_events) {
+        for (auto &ev : E.events)
+            if (ev.live && ev.N > maxN) maxN = ev.N;
+    } else {
+        maxN = E.events[events0].N;
+    }
+    int bpl = gwh_choose_bpl(maxN);
+    int tile = GWH_TPB * bpl;
+    int maxtiles = (maxN + tile - 1) / tile;
+    int rc;
+    if ((rc = E.coef.ensure((size_t)n * sizeof(InstCoef))) != GWH_OK) return rc;
+    if ((rc = E.partial.ensure((size_t)n * maxtiles * GWH_PARTIAL_STRIDE * sizeof(double))) != GWH_OK) return rc;
+    if ((rc = E.active.ensure((size_t)n * sizeof(int))) != GWH_OK) return rc;
+    WorkMap W;
+    W.inst_event = d_events;
+    W.event0 = events0;
+    W.n = n;
+    W.tiles0 = maxtiles;
+    hipLaunchKernelGGL(gwh_setup_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_params, d_events, events0, n, model,
+                       (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
+                       E.coef.as<InstCoef>(), tile, E.active.as<int>());
+    GWH_HIP(hipGetLastError());
+    if (timed) GWH_HIP(hipEventRecord(E.ev0, stream));
+    const EventDesc *evp = (const EventDesc *)E.d_events;
+    const InstCoef *cp = E.coef.as<const InstCoef>();
+    double *pp = E.partial.as<double>();
+    const int *ap = E.active.as<const int>();
+    dim3 grid(maxtiles, n);
+    if (model == GWH_IMRPHENOMD) {
+        if (precision == GWH_FP64)
+            launch_bpl<GWH_IMRPHENOMD, GWH_FP64>(bpl, grid, stream, W, evp, cp, pp, ap);
+        else
+            launch_bpl<GWH_IMRPHENOMD, GWH_MIXED>(bpl, grid, stream, W, evp, cp, pp, ap);
+    } else {
+        if (precision == GWH_FP64)
+            launch_bpl<GWH_IMRPHENOMPV2, GWH_FP64>(bpl, grid, stream, W, evp, cp, pp, ap);
+        else
+            launch_bpl<GWH_IMRPHENOMPV2, GWH_MIXED>(bpl, grid, stream, W, evp, cp, pp, ap);
+    }
+    GWH_HIP(hipGetLastError());
+    if (timed) GWH_HIP(hipEventRecord(E.ev1, stream));
+    hipLaunchKernelGGL(gwh_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, W, evp, cp,
+                       (const double *)pp, ap, model, precision, d_logl);
+    GWH_HIP(hipGetLastError());
+    if (timed && kernel_ms) {
+        GWH_HIP(hipEventSynchronize(E.ev1));
+        float ms = 0;
+        GWH_HIP(hipEventElapsedTime(&ms, E.ev0, E.ev1));
+        *kernel_ms = ms;
+    }
+    return GWH_OK;
+}
+
+/* bins per IMRPhenomD phase region of every instance (for the algorithmic-work figure of benchmarks):
+ * counts[4 i + 0..3] = inspiral (f < f1_phase), intermediate, merger-ringdown (f2_phase < f <= 0.2/M), zero */
+__global__ void gwh_region_count_kernel(const InstCoef *__restrict__ coefs, const EventDesc *__restrict__ events,
+                                        int event0, int *__restrict__ counts)
+{
+    __shared__ int s[4][GWH_TPB];
+    const InstCoef &C = coefs[blockIdx.x];
+    const EventDesc &E = events[event0];
+    int c[4] = {0, 0, 0, 0};
+    for (int k = threadIdx.x; k < E.N; k += GWH_TPB) {
+        double f = E.f[k];
+        int r = (f > C.fcut) ? 3 : (f < C.f1p) ? 0 : (f > C.f2p) ? 2 : 1;
+        c[r]++;
+    }
+    for (int j = 0; j < 4; j++) s[j][threadIdx.x] = c[j];
+    __syncthreads();
+    for (int w = GWH_TPB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int j = 0; j < 4; j++) s[j][threadIdx.x] += s[j][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) counts[4 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+/* ------------------------------------------------------------------------- */
+/* waveform / response output kernels (single-call drop-in path)              */
+/* ------------------------------------------------------------------------- */
+
+/* mode 0: hplus/hcross; mode 1: detector response (detector 0 of the instance record) */
+template <int MODEL>
+__global__ void gwh_waveform_kernel(const double *__restrict__ f, int N, const InstCoef *__restrict__ coef, int mode,
+                                    double2 *__restrict__ out1, double2 *__restrict__ out2, int det = 0)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const InstCoef &C = coef[0];
+    double fk = f[k];
+    double2 hp = make_double2(0, 0), hc = make_double2(0, 0);
+    if (fk <= C.fcut) {
+        if (MODEL == GWH_IMRPHENOMD) {
+            double A, phi, u, lx;
+            gwh_phenomd_fp64(C, fk, A, phi, u, lx);
+            double s, c;
+            sincos(phi, &s, &c);
+            double hr = A * c, hi = -A * s; /* h = A e^{-i phi} */
+            hp = make_double2(hr * C.wf_a, hi * C.wf_a);
+            hc = make_double2(hi * C.wf_b, -hr * C.wf_b); /* -i h cos(iota) */
+        } else {
+            double A, Phi, hpr, hpi, hcr, hci;
+            gwh_pv2_fp64(C, fk, A, Phi, hpr, hpi, hcr, hci);
+            double s, c;
+            sincos(Phi, &s, &c);
+            /* (A hpf) e^{-i Phi} */
+            double pr = A * (hpr * c + hpi * s), pi = A * (hpi * c - hpr * s);
+            double xr = A * (hcr * c + hci * s), xi = A * (hci * c - hcr * s);
+            double c2z = C.wf_a, s2z = C.wf_b;
+            hp = make_double2(c2z * pr + s2z * xr, c2z * pi + s2z * xi);
+            hc = make_double2(c2z * xr - s2z * pr, c2z * xi - s2z * pi);
+        }
+    }
+    if (mode == 0) {
+        out1[k] = hp;
+        out2[k] = hc;
+    } else {
+        /* F+ h+ + Fx hx; the record holds F+, Fx through the projection constants */
+        if (MODEL == GWH_IMRPHENOMD) {
+            /* response = h K with K = F+ (1+c^2)/2 - i Fx c : rebuild from h+ (= h k+) */
+            double hr = (C.wf_a != 0) ? hp.x / C.wf_a : 0, hi = (C.wf_a != 0) ? hp.y / C.wf_a : 0;
+            double Kr = C.det[det][0], Ki = C.det[det][1];
+            hp = make_double2(hr * Kr - hi * Ki, hr * Ki + hi * Kr);
+        } else {
+            /* P h+0 + Q hx0 on the unrotated polarisations equals F+ h+ + Fx hx on the rotated ones */
+            double c2z = C.wf_a, s2z = C.wf_b;
+            double p0r = c2z * hp.x - s2z * hc.x, p0i = c2z * hp.y - s2z * hc.y;
+            double x0r = s2z * hp.x + c2z * hc.x, x0i = s2z * hp.y + c2z * hc.y;
+            hp = make_double2(C.det[det][0] * p0r + C.det[det][1] * x0r, C.det[det][0] * p0i + C.det[det][1] * x0i);
+        }
+        if (mode == 2) {
+            /* time-of-flight phase e^{-2 pi i f dt} relative to the first detector (waveform_util.cpp:170-179) */
+            double s, c;
+            sincos(-C.det[det][3] * fk, &s, &c);
+            hp = make_double2(hp.x * c - hp.y * s, hp.x * s + hp.y * c);
+        }
+        out1[k] = hp;
+    }
+}
+
+/* plain weighted inner products of a given response (Log_Likelihood_internal with the response passed in) */
+__global__ void gwh_inner_kernel(const double2 *__restrict__ dwS, const double2 *__restrict__ resp,
+                                 const double *__restrict__ wS, int N, double *__restrict__ partial)
+{
+    __shared__ double lds[GWH_TPB / 64][2];
+    double acc[2] = {0, 0};
+    for (int k = blockIdx.x * GWH_TPB + threadIdx.x; k < N; k += gridDim.x * GWH_TPB) {
+        double2 r = resp[k], d = dwS[k];
+        acc[0] += (r.x * r.x + r.y * r.y) * wS[k];
+        acc[1] += d.x * r.x + d.y * r.y;
+    }
+    block_reduce_store<2>(acc, partial + 2 * blockIdx.x, lds);
+}
+
+/* ------------------------------------------------------------------------- */
+/* C ABI                                                                      */
+/* ------------------------------------------------------------------------- */
+
+extern "C" {
+
+const char *gwh_last_error(void) { return g_err.c_str(); }
+
+const char *gwh_version(void) { return "gwat-hip 0.1 (gfx950)"; }
+
+int gwh_init(int device)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    if (!E.ready) E.device = device;
+    return gwh_ensure_init();
+}
+
+int gwh_synchronize(void)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
+}
+
+int gwh_set_bins_per_lane(int bpl)
+{
+    if (!(bpl == 0 || bpl == 1 || bpl == 2 || bpl == 4 || bpl == 8))
+        return gwh_set_error(GWH_ERR_INVALID, "bins per lane must be 0, 1, 2, 4 or 8");
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    E.bpl = bpl;
+    return GWH_OK;
+}
+
+void *gwh_device_alloc(size_t bytes)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    if (gwh_ensure_init() != GWH_OK) return nullptr;
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        gwh_set_error(GWH_ERR_NOMEM, "hipMalloc failed");
+        return nullptr;
+    }
+    return p;
+}
+
+int gwh_device_free(void *ptr)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    GWH_HIP(hipFree(ptr));
+    return GWH_OK;
+}
+
+int gwh_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    GWH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, E.stream));
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
+}
+
+int gwh_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    GWH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
+}
+
+int gwh_event_create(const double *frequencies, int length, int ndet, const int *detectors, const double *data,
+                     const double *psd, int *event)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    return gwh_event_create_ex(frequencies, length, ndet, detectors, data, psd, GWH_SIMPSONS, nullptr, 0, 0, 0, event);
+}
+
+int gwh_event_destroy(int event)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    if (!E.ready || event < 0 || event >= GWH_MAX_EVENTS || !E.events[event].live)
+        return gwh_set_error(GWH_ERR_INVALID, "unknown event");
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    GWH_HIP(hipFree(E.events[event].block));
+    E.events[event] = Event();
+    return GWH_OK;
+}
+
+size_t gwh_event_bytes(int event)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    if (!E.ready || event < 0 || event >= GWH_MAX_EVENTS || !E.events[event].live) return 0;
+    return E.events[event].bytes;
+}
+
+int gwh_loglike_batch_device(const int *d_events, int events0, int model, int precision, const void *d_params, int n,
+                             double *d_logl, void *stream)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+    return gwh_loglike_launch(d_events, events0, model, precision, (const gwh_params *)d_params, n, d_logl, s, false,
+                              nullptr);
+}
+
+int gwh_loglike_batch(const int *events, int events0, int model, int precision, const gwh_params *params, int n,
+                      double *logl)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    if (n <= 0) return GWH_OK;
+    if (!params || !logl) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    int rc;
+    if ((rc = E.params.ensure((size_t)n * sizeof(gwh_params))) != GWH_OK) return rc;
+    if ((rc = E.logl.ensure((size_t)n * sizeof(double))) != GWH_OK) return rc;
+    GWH_HIP(hipMemcpyAsync(E.params.p, params, (size_t)n * sizeof(gwh_params), hipMemcpyHostToDevice, E.stream));
+    const int *dev = nullptr;
+    if (events) {
+        for (int i = 0; i < n; i++)
+            if (events[i] < 0 || events[i] >= GWH_MAX_EVENTS || !E.events[events[i]].live)
+                return gwh_set_error(GWH_ERR_INVALID, "unknown event in batch");
+        if ((rc = E.inst_event.ensure((size_t)n * sizeof(int))) != GWH_OK) return rc;
+        GWH_HIP(hipMemcpyAsync(E.inst_event.p, events, (size_t)n * sizeof(int), hipMemcpyHostToDevice, E.stream));
+        dev = E.inst_event.as<const int>();
+        events0 = events[0];
+    }
+    rc = gwh_loglike_launch(dev, events0, model, precision, E.params.as<const gwh_params>(), n, E.logl.as<double>(),
+                            E.stream, false, nullptr);
+    if (rc != GWH_OK) return rc;
+    GWH_HIP(hipMemcpyAsync(logl, E.logl.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
+}
+
+int gwh_time_loglike_batch_device(int events0, int model, int precision, const void *d_params, int n, double *d_logl,
+                                  int reps, double *ms_per_launch, double *kernel_ms)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    if (reps < 1) reps = 1;
+    hipEvent_t a, b;
+    GWH_HIP(hipEventCreate(&a));
+    GWH_HIP(hipEventCreate(&b));
+    double ksum = 0;
+    GWH_HIP(hipEventRecord(a, E.stream));
+    for (int r = 0; r < reps; r++) {
+        double km = 0;
+        int rc = gwh_loglike_launch(nullptr, events0, model, precision, (const gwh_params *)d_params, n, d_logl,
+                                    E.stream, true, &km);
+        if (rc != GWH_OK) return rc;
+        ksum += km;
+    }
+    GWH_HIP(hipEventRecord(b, E.stream));
+    GWH_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    GWH_HIP(hipEventElapsedTime(&ms, a, b));
+    GWH_HIP(hipEventDestroy(a));
+    GWH_HIP(hipEventDestroy(b));
+    if (ms_per_launch) *ms_per_launch = ms / reps;
+    if (kernel_ms) *kernel_ms = ksum / reps;
+    return GWH_OK;
+}
+
+/* ---- single-call drop-in entry points ---- */
+
+static int upload_single(const double *f, int n, const gwh_params *p, int model, int det, const int *dets = nullptr,
+                         int ndet = 1)
+{
+    Engine &E = gwh_engine();
+    int rc;
+    if ((rc = E.f.ensure((size_t)n * 8)) != GWH_OK) return rc;
+    if ((rc = E.params.ensure(sizeof(gwh_params))) != GWH_OK) return rc;
+    if ((rc = E.coef.ensure(sizeof(InstCoef))) != GWH_OK) return rc;
+    if ((rc = E.out1.ensure((size_t)n * 16)) != GWH_OK) return rc;
+    if ((rc = E.out2.ensure((size_t)n * 16)) != GWH_OK) return rc;
+    if ((rc = E.misc.ensure(256)) != GWH_OK) return rc;
+    GWH_HIP(hipMemcpyAsync(E.f.p, f, (size_t)n * 8, hipMemcpyHostToDevice, E.stream));
+    GWH_HIP(hipMemcpyAsync(E.params.p, p, sizeof(gwh_params), hipMemcpyHostToDevice, E.stream));
+    /* a one-detector descriptor for the setup (only the detector ids are read) */
+    EventDesc ds;
+    memset(&ds, 0, sizeof(ds));
+    ds.D = ndet;
+    ds.N = n;
+    for (int d = 0; d < GWH_MAXDET; d++) ds.det[d] = (dets && d < ndet) ? dets[d] : det;
+    GWH_HIP(hipMemcpyAsync(E.d_events + (GWH_MAX_EVENTS - 1), &ds, sizeof(ds), hipMemcpyHostToDevice, E.stream));
+    hipLaunchKernelGGL(gwh_setup_kernel, dim3(1), dim3(64), 0, E.stream, E.params.as<const gwh_params>(),
+                       (const int *)nullptr, GWH_MAX_EVENTS - 1, 1, model, (const double *)E.d_fits, E.qnm,
+                       (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events, E.coef.as<InstCoef>(), GWH_TPB,
+                       (int *)nullptr);
+    GWH_HIP(hipGetLastError());
+    return GWH_OK;
+}
+
+static int check_common(const double *f, int n, const gwh_params *p, int model)
+{
+    if (!f || !p) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    if (n < 1) return gwh_set_error(GWH_ERR_INVALID, "length must be positive");
+    if (model != GWH_IMRPHENOMD && model != GWH_IMRPHENOMPV2)
+        return gwh_set_error(GWH_ERR_UNSUPPORTED, "only IMRPhenomD and IMRPhenomPv2 are provided");
+    return GWH_OK;
+}
+
+int gwh_fourier_waveform(const double *frequencies, int length, double *hplus, double *hcross, int model,
+                         const gwh_params *params)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int rc = gwh_ensure_init();
+    if (rc != GWH_OK) return rc;
+    if ((rc = check_common(frequencies, length, params, model)) != GWH_OK) return rc;
+    if (!hplus || !hcross) return gwh_set_error(GWH_ERR_INVALID, "null output");
+    if ((rc = upload_single(frequencies, length, params, model, 0)) != GWH_OK) return rc;
+    dim3 g((length + 255) / 256);
+    if (model == GWH_IMRPHENOMD)
+        hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMD>, g, dim3(256), 0, E.stream, E.f.as<const double>(),
+                           length, E.coef.as<const InstCoef>(), 0, E.out1.as<double2>(), E.out2.as<double2>());
+    else
+        hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMPV2>, g, dim3(256), 0, E.stream, E.f.as<const double>(),
+                           length, E.coef.as<const InstCoef>(), 0, E.out1.as<double2>(), E.out2.as<double2>());
+    GWH_HIP(hipGetLastError());
+    GWH_HIP(hipMemcpyAsync(hplus, E.out1.p, (size_t)length * 16, hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipMemcpyAsync(hcross, E.out2.p, (size_t)length * 16, hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
+}
+
+int gwh_fourier_detector_response(const double *frequencies, int length, double *response, int detector, int model,
+                                  const gwh_params *params)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int rc = gwh_ensure_init();
+    if (rc != GWH_OK) return rc;
+    if ((rc = check_common(frequencies, length, params, model)) != GWH_OK) return rc;
+    if (!response) return gwh_set_error(GWH_ERR_INVALID, "null output");
+    if (detector < 0 || detector >= GWH_NDET_TABLE)
+        return gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported detector (the reference exits here)");
+    if ((rc = upload_single(frequencies, length, params, model, detector)) != GWH_OK) return rc;
+    dim3 g((length + 255) / 256);
+    if (model == GWH_IMRPHENOMD)
+        hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMD>, g, dim3(256), 0, E.stream, E.f.as<const double>(),
+                           length, E.coef.as<const InstCoef>(), 1, E.out1.as<double2>(), E.out2.as<double2>());
+    else
+        hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMPV2>, g, dim3(256), 0, E.stream, E.f.as<const double>(),
+                           length, E.coef.as<const InstCoef>(), 1, E.out1.as<double2>(), E.out2.as<double2>());
+    GWH_HIP(hipGetLastError());
+    GWH_HIP(hipMemcpyAsync(response, E.out1.p, (size_t)length * 16, hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
+}
+
+/* find or create a cached event for host arrays used by the single-call likelihoods */
+static int cached_event(const double *const *f, int n, int ndet, const int *dets, const double *const *data,
+                        const double *const *psd, int method, const double *weights, int log10F, int *event)
+{
+    Engine &E = gwh_engine();
+    uint64_t key = 0x12345;
+    key = gwh_hash_bytes(f[0], (size_t)n * 8, key);
+    for (int d = 0; d < ndet; d++) {
+        key = gwh_hash_bytes(data[d], (size_t)n * 16, key);
+        key = gwh_hash_bytes(psd[d], (size_t)n * 8, key);
+        key = gwh_hash_bytes(&dets[d], sizeof(int), key);
+    }
+    key = gwh_hash_bytes(&method, sizeof(int), key);
+    key = gwh_hash_bytes(&log10F, sizeof(int), key);
+    if (weights) key = gwh_hash_bytes(weights, (size_t)n * 8, key);
+    if (key == 0) key = 1;
+    int live_cached = 0, oldest = -1;
+    for (int i = 0; i < GWH_MAX_EVENTS - 1; i++) {
+        Event &ev = E.events[i];
+        if (!ev.live || ev.key == 0) continue;
+        if (ev.key == key && ev.N == n && ev.D == ndet) {
+            ev.last_use = ++E.use_clock;
+            *event = i;
+            return GWH_OK;
+        }
+        live_cached++;
+        if (oldest < 0 || ev.last_use < E.events[oldest].last_use) oldest = i;
+    }
+    if (live_cached >= 64 && oldest >= 0) { /* LRU of implicit events */
+        GWH_HIP(hipStreamSynchronize(E.stream));
+        GWH_HIP(hipFree(E.events[oldest].block));
+        E.events[oldest] = Event();
+    }
+    std::vector<double> dat((size_t)ndet * n * 2), ps((size_t)ndet * n);
+    for (int d = 0; d < ndet; d++) {
+        memcpy(&dat[(size_t)d * n * 2], data[d], (size_t)n * 16);
+        memcpy(&ps[(size_t)d * n], psd[d], (size_t)n * 8);
+    }
+    return gwh_event_create_ex(f[0], n, ndet, dets, dat.data(), ps.data(), method, weights, log10F, 0, key, event);
+}
+
+double gwh_log_likelihood(const double *data, const double *psd, const double *frequencies, size_t length,
+                          const gwh_params *params, int detector, int model)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    const double nan = std::nan("");
+    if (gwh_ensure_init() != GWH_OK) return nan;
+    int n = (int)length;
+    if (check_common(frequencies, n, params, model) != GWH_OK || !data || !psd) return nan;
+    if (detector < 0 || detector >= GWH_NDET_TABLE) {
+        gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported detector");
+        return nan;
+    }
+    int ev;
+    const double *fp[1] = {frequencies}, *dp[1] = {data}, *pp[1] = {psd};
+    if (cached_event(fp, n, 1, &detector, dp, pp, GWH_SIMPSONS, nullptr, 0, &ev) != GWH_OK) return nan;
+    double out = nan;
+    if (gwh_loglike_batch(nullptr, ev, model, GWH_FP64, params, 1, &out) != GWH_OK) return nan;
+    return out;
+}
+
+double gwh_mcmc_likelihood_extrinsic(int save_waveform, const gwh_params *params, int model, const int *data_length,
+                                     const double *const *frequencies, const double *const *data,
+                                     const double *const *psd, const double *const *weights, int integration_method,
+                                     int log10F, const int *detectors, int num_detectors, double T_eff)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    const double nan = std::nan("");
+    if (gwh_ensure_init() != GWH_OK) return nan;
+    if (!params || !data_length || !frequencies || !data || !psd || !detectors || num_detectors < 1) {
+        gwh_set_error(GWH_ERR_INVALID, "null argument");
+        return nan;
+    }
+    if (!save_waveform) {
+        /* waveform_util.cpp:144-147: independent waveforms per detector are not supported */
+        gwh_set_error(GWH_ERR_UNSUPPORTED, "Independent WFs for each detector currently not supported");
+        return nan;
+    }
+    int n = data_length[0];
+    for (int d = 1; d < num_detectors; d++) {
+        if (data_length[d] != n || memcmp(frequencies[d], frequencies[0], (size_t)n * 8) != 0) {
+            gwh_set_error(GWH_ERR_UNSUPPORTED, "all detectors must share one frequency grid");
+            return nan;
+        }
+    }
+    if (num_detectors > GWH_MAXDET) {
+        gwh_set_error(GWH_ERR_UNSUPPORTED, "at most 4 detectors");
+        return nan;
+    }
+    int ev;
+    if (cached_event(frequencies, n, num_detectors, detectors, data, psd, integration_method,
+                     (integration_method == GWH_GAUSSLEG && weights) ? weights[0] : nullptr, log10F, &ev) != GWH_OK)
+        return nan;
+    gwh_params p = *params;
+    p.tc = T_eff - params->tc; /* mcmc_gw.cpp:2466-2468 */
+    double out = nan;
+    if (gwh_loglike_batch(nullptr, ev, model, GWH_FP64, &p, 1, &out) != GWH_OK) return nan;
+    return out;
+}
+
+double gwh_log_likelihood_internal(const double *data, const double *psd, const double *frequencies,
+                                   const double *weights, const double *response, int length, int log10F,
+                                   int integration_method)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    const double nan = std::nan("");
+    if (gwh_ensure_init() != GWH_OK) return nan;
+    if (!data || !psd || !frequencies || !response || length < 2) {
+        gwh_set_error(GWH_ERR_INVALID, "null argument or length < 2");
+        return nan;
+    }
+    int det = 0, ev;
+    const double *fp[1] = {frequencies}, *dp[1] = {data}, *pp[1] = {psd};
+    if (cached_event(fp, length, 1, &det, dp, pp, integration_method,
+                     integration_method == GWH_GAUSSLEG ? weights : nullptr, log10F, &ev) != GWH_OK)
+        return nan;
+    const EventDesc &ds = E.events[ev].desc;
+    if (E.out1.ensure((size_t)length * 16) != GWH_OK) return nan;
+    int nb = (length + GWH_TPB - 1) / GWH_TPB;
+    if (nb > 1024) nb = 1024;
+    if (E.partial.ensure((size_t)nb * 2 * 8) != GWH_OK) return nan;
+    if (hipMemcpyAsync(E.out1.p, response, (size_t)length * 16, hipMemcpyHostToDevice, E.stream) != hipSuccess) {
+        gwh_set_error(GWH_ERR_HIP, "copy failed");
+        return nan;
+    }
+    hipLaunchKernelGGL(gwh_inner_kernel, dim3(nb), dim3(GWH_TPB), 0, E.stream, ds.dwS, E.out1.as<const double2>(),
+                       ds.wS, length, E.partial.as<double>());
+    std::vector<double> part(2 * nb);
+    if (hipMemcpyAsync(part.data(), E.partial.p, part.size() * 8, hipMemcpyDeviceToHost, E.stream) != hipSuccess ||
+        hipStreamSynchronize(E.stream) != hipSuccess) {
+        gwh_set_error(GWH_ERR_HIP, "kernel failed");
+        return nan;
+    }
+    double hh = 0, dh = 0;
+    for (int b = 0; b < nb; b++) {
+        hh += part[2 * b];
+        dh += part[2 * b + 1];
+    }
+    return -0.5 * (hh - 2 * dh);
+}
+
+int gwh_work_counts(int event, int model, const gwh_params *params, int n, int *counts)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int rc = gwh_ensure_init();
+    if (rc != GWH_OK) return rc;
+    if (n <= 0 || !params || !counts) return gwh_set_error(GWH_ERR_INVALID, "bad arguments");
+    if (event < 0 || event >= GWH_MAX_EVENTS || !E.events[event].live) return gwh_set_error(GWH_ERR_INVALID, "unknown event");
+    if ((rc = E.params.ensure((size_t)n * sizeof(gwh_params))) != GWH_OK) return rc;
+    if ((rc = E.coef.ensure((size_t)n * sizeof(InstCoef))) != GWH_OK) return rc;
+    if ((rc = E.misc.ensure((size_t)n * 4 * sizeof(int))) != GWH_OK) return rc;
+    GWH_HIP(hipMemcpyAsync(E.params.p, params, (size_t)n * sizeof(gwh_params), hipMemcpyHostToDevice, E.stream));
+    if ((rc = gwh_setup_launch(nullptr, event, model, E.params.as<const gwh_params>(), n, E.coef.as<InstCoef>(),
+                               E.stream)) != GWH_OK)
+        return rc;
+    hipLaunchKernelGGL(gwh_region_count_kernel, dim3(n), dim3(GWH_TPB), 0, E.stream, E.coef.as<const InstCoef>(),
+                       (const EventDesc *)E.d_events, event, E.misc.as<int>());
+    GWH_HIP(hipGetLastError());
+    GWH_HIP(hipMemcpyAsync(counts, E.misc.p, (size_t)n * 4 * sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
+}
+
+int gwh_create_coherent_GW_detection(const int *detectors, int num_detectors, const double *frequencies, int length,
+                                     int model, const gwh_params *params, double *responses)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int rc = gwh_ensure_init();
+    if (rc != GWH_OK) return rc;
+    if ((rc = check_common(frequencies, length, params, model)) != GWH_OK) return rc;
+    if (!detectors || !responses || num_detectors < 1 || num_detectors > GWH_MAXDET)
+        return gwh_set_error(GWH_ERR_INVALID, "1 to 4 detectors and an output buffer are required");
+    for (int d = 0; d < num_detectors; d++)
+        if (detectors[d] < 0 || detectors[d] >= GWH_NDET_TABLE)
+            return gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported detector");
+    if ((rc = upload_single(frequencies, length, params, model, detectors[0], detectors, num_detectors)) != GWH_OK)
+        return rc;
+    dim3 g((length + 255) / 256);
+    for (int d = 0; d < num_detectors; d++) {
+        if (model == GWH_IMRPHENOMD)
+            hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMD>, g, dim3(256), 0, E.stream, E.f.as<const double>(),
+                               length, E.coef.as<const InstCoef>(), 2, E.out1.as<double2>(), E.out2.as<double2>(), d);
+        else
+            hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMPV2>, g, dim3(256), 0, E.stream,
+                               E.f.as<const double>(), length, E.coef.as<const InstCoef>(), 2, E.out1.as<double2>(),
+                               E.out2.as<double2>(), d);
+        GWH_HIP(hipGetLastError());
+        GWH_HIP(hipMemcpyAsync(responses + (size_t)2 * length * d, E.out1.p, (size_t)length * 16,
+                               hipMemcpyDeviceToHost, E.stream));
+    }
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
+}
+
+} /* extern "C" */

@@ -1,0 +1,649 @@
+/*
+ * gwh_fisher.hip -- numerical Fisher matrices on MI355X.
+ *
+ *   host      unpack the point into the basis of the generation method, build the central-difference
+ *             stencil (the perturbed parameter sets, fisher.cpp:148-557), run the FP64 setup for all of
+ *             them in one launch
+ *   stencil   one lane per bin and point: every perturbed waveform/response evaluated in FP64, the FP64
+ *             difference quotients D_i(f) scaled by sqrt(w/S) written as rows of 16 (parameters padded
+ *             to 16) -- the operand layout of the matrix-core Gram
+ *   gram      F = sum_rows a a^T with v_mfma_f64_16x16x4 (FP64 path) or v_mfma_f32_16x16x4 (mixed path),
+ *             64 rows of 4 per instruction per wave, FP64 sums across waves and row chunks
+ *   invert    FP64 LU inverse per point (util.cpp:828-855) -> marginal sigmas
+ *
+ * Reference: src/fisher.cpp:81-145 (fisher_numerical), :148-557 (calculate_derivatives),
+ * :1754-2572 (basis maps), :2704-2781 (calculate_fisher_elements).
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gwh_engine.h"
+#include "gwh_tables.h"
+
+typedef double gwh_d4 __attribute__((ext_vector_type(4)));
+typedef float gwh_f4 __attribute__((ext_vector_type(4)));
+
+#define GWH_FROWS 16      /* parameters padded to the MFMA tile */
+#define GWH_MAXSTEN 4     /* order 4 uses 4 evaluations per parameter */
+#define GWH_GRAM_ROWS 4096 /* operand rows per Gram block */
+
+/* ------------------------------------------------------------------------- */
+/* host: parameter bases (fisher.cpp:1841-2511)                               */
+/* ------------------------------------------------------------------------- */
+
+static int basis_dim(int basis)
+{
+    switch (basis) {
+    case GWH_BASIS_PHENOMD_SKYAVG: return 7;
+    case GWH_BASIS_PHENOMD_FULL: return 11;
+    case GWH_BASIS_PHENOMD_MCMC: return 11;
+    case GWH_BASIS_PV2_FULL: return 13;
+    case GWH_BASIS_PV2_MCMC: return 15;
+    }
+    return -1;
+}
+
+static const double kMsolSec = GWH_MSOL_SEC;
+static double mpc_sec() { return GWH_MPC_SI / GWH_C_SI; }
+static double chirpmass(double m1, double m2) { return pow(m1 * m2, 3. / 5) / pow(m1 + m2, 1. / 5); }
+static double sym_eta(double m1, double m2) { return (m1 * m2) / pow(m1 + m2, 2); }
+static double mass1_of(double mc, double eta)
+{
+    double e = pow(eta, 3. / 5);
+    return 1. / 2 * (mc / e + sqrt(1. - 4 * eta) * mc / e);
+}
+static double mass2_of(double mc, double eta)
+{
+    double e = pow(eta, 3. / 5);
+    return 1. / 2 * (mc / e - sqrt(1. - 4 * eta) * mc / e);
+}
+static double amp_k(int sky) { return sky ? sqrt(M_PI / 30) : sqrt(M_PI * 40. / 192.); }
+
+static void cart_to_sph(const double *c, double *s)
+{
+    s[0] = sqrt(c[0] * c[0] + c[2] * c[2] + c[1] * c[1]);
+    s[1] = acos(c[2] / s[0]);
+    s[2] = atan2(c[1], c[0]);
+    if (s[2] < 0) s[2] += 2 * M_PI;
+}
+static void sph_to_cart(const double *s, double *c)
+{
+    c[0] = s[0] * sin(s[1]) * cos(s[2]);
+    c[1] = s[0] * sin(s[1]) * sin(s[2]);
+    c[2] = s[0] * cos(s[1]);
+}
+
+static void unpack(const gwh_params &p, int basis, double *v, int *lf)
+{
+    int dim = basis_dim(basis);
+    for (int i = 0; i < dim; i++) lf[i] = 0;
+    double mc = chirpmass(p.mass1, p.mass2), eta = sym_eta(p.mass1, p.mass2);
+    if (basis == GWH_BASIS_PHENOMD_SKYAVG) {
+        lf[0] = lf[3] = lf[4] = 1;
+        v[3] = mc;
+        v[0] = amp_k(p.sky_average) * (mc * kMsolSec) * (mc * kMsolSec) / (p.DL * mpc_sec()) *
+               pow(M_PI * mc * kMsolSec, -7. / 6);
+        v[1] = p.phiRef;
+        v[2] = p.tc;
+        v[4] = eta;
+        v[5] = (p.spin1[2] + p.spin2[2]) / 2.;
+        v[6] = (p.spin1[2] - p.spin2[2]) / 2.;
+        return;
+    }
+    bool mcmc = (basis == GWH_BASIS_PHENOMD_MCMC || basis == GWH_BASIS_PV2_MCMC);
+    v[0] = p.RA;
+    v[1] = mcmc ? sin(p.DEC) : p.DEC;
+    v[2] = p.psi;
+    v[3] = mcmc ? cos(p.incl) : p.incl;
+    v[4] = p.phiRef;
+    v[5] = p.tc;
+    if (mcmc) {
+        v[6] = log(p.DL);
+        v[7] = log(mc);
+    } else {
+        lf[6] = lf[7] = 1;
+        v[6] = p.DL;
+        v[7] = mc;
+    }
+    v[8] = eta;
+    if (basis == GWH_BASIS_PV2_MCMC) {
+        double s1[3], s2[3];
+        cart_to_sph(p.spin1, s1);
+        cart_to_sph(p.spin2, s2);
+        v[9] = s1[0];
+        v[10] = s2[0];
+        v[11] = cos(s1[1]);
+        v[12] = cos(s2[1]);
+        v[13] = s1[2];
+        v[14] = s2[2];
+    } else {
+        v[9] = p.spin1[2];
+        v[10] = p.spin2[2];
+        if (basis == GWH_BASIS_PV2_FULL) {
+            v[11] = p.chip;
+            v[12] = p.phip;
+        }
+    }
+}
+
+/* a: a gen_params carrying only the non-parameter options (fisher.cpp:2513-2572) */
+static void repack(const double *v, int basis, gwh_params &a)
+{
+    if (basis == GWH_BASIS_PHENOMD_SKYAVG) {
+        a.mass1 = mass1_of(v[3], v[4]);
+        a.mass2 = mass2_of(v[3], v[4]);
+        double mcs = v[3] * kMsolSec;
+        a.DL = amp_k(a.sky_average) * mcs * mcs / v[0] * pow(M_PI * mcs, -7. / 6) / mpc_sec();
+        a.tc = v[2];
+        a.phiRef = v[1];
+        a.spin1[2] = v[5] + v[6];
+        a.spin2[2] = v[5] - v[6];
+        return;
+    }
+    bool mcmc = (basis == GWH_BASIS_PHENOMD_MCMC || basis == GWH_BASIS_PV2_MCMC);
+    a.mass1 = mass1_of(mcmc ? exp(v[7]) : v[7], v[8]);
+    a.mass2 = mass2_of(mcmc ? exp(v[7]) : v[7], v[8]);
+    a.DL = mcmc ? exp(v[6]) : v[6];
+    a.RA = v[0];
+    a.DEC = mcmc ? asin(v[1]) : v[1];
+    a.psi = v[2];
+    a.incl = mcmc ? acos(v[3]) : v[3];
+    a.phiRef = v[4];
+    a.tc = v[5];
+    if (basis == GWH_BASIS_PV2_MCMC) {
+        /* bump cos(theta) back into [-1, 1] (fisher.cpp:2188-2210) */
+        double t1 = v[11] > 1 ? 0 : (v[11] < -1 ? M_PI : acos(v[11]));
+        double t2 = v[12] > 1 ? 0 : (v[12] < -1 ? M_PI : acos(v[12]));
+        double s1[3] = {v[9], t1, v[13]}, s2[3] = {v[10], t2, v[14]};
+        sph_to_cart(s1, a.spin1);
+        sph_to_cart(s2, a.spin2);
+    } else {
+        a.spin1[2] = v[9];
+        a.spin2[2] = v[10];
+        if (basis == GWH_BASIS_PV2_FULL) {
+            a.chip = v[11];
+            a.phip = v[12];
+        }
+    }
+}
+
+static gwh_params non_parameter_options(const gwh_params &in)
+{
+    gwh_params w;
+    memset(&w, 0, sizeof(w));
+    w.chip = -1;
+    w.phip = -1;
+    w.sky_average = in.sky_average;
+    w.f_ref = in.f_ref;
+    w.gmst = in.gmst;
+    w.shift_time = 0; /* fisher.cpp:2527 */
+    w.shift_phase = in.shift_phase;
+    w.dep_postmerger = in.dep_postmerger;
+    return w;
+}
+
+static double host_dtoa(double ra, double dec, double gmst, int d1, int d2)
+{
+    const double *l1 = gwh_det_location[d1], *l2 = gwh_det_location[d2];
+    double ha = gmst - ra;
+    double e0 = cos(dec) * cos(ha), e1 = cos(dec) * -sin(ha), e2 = sin(dec);
+    return ((l1[0] - l2[0]) * e0 + (l1[1] - l2[1]) * e1 + (l1[2] - l2[2]) * e2) / GWH_C_SI;
+}
+
+/* Stencil of one point: for every parameter i, S evaluations with weights c[i][s] (the difference
+ * quotient including 1/eps and the log factor theta_i); instance parameter sets are appended to
+ * `inst`, `no_tof` marks the order-4 +-2 eps evaluations that shift tc instead of using the time-of-flight
+ * phase (fisher.cpp:433-453). */
+struct PointStencil {
+    int S;
+    double c[GWH_FROWS][GWH_MAXSTEN];
+};
+
+static int build_stencil(const gwh_params &p, int basis, int order, double eps, int det, int ref_det,
+                         std::vector<gwh_params> &inst, std::vector<unsigned char> &no_tof, PointStencil &ps)
+{
+    int dim = basis_dim(basis);
+    double v[GWH_FROWS];
+    int lf[GWH_FROWS];
+    unpack(p, basis, v, lf);
+    gwh_params base = non_parameter_options(p);
+    ps.S = (order >= 4) ? 4 : 2;
+    for (int i = 0; i < GWH_FROWS; i++)
+        for (int s = 0; s < GWH_MAXSTEN; s++) ps.c[i][s] = 0;
+    for (int i = 0; i < dim; i++) {
+        bool edge = (i == 8 && v[i] > .25 - eps); /* eta at the equal-mass boundary: one-sided */
+        double fac = lf[i] ? v[i] : 1.0;
+        double off[4];
+        int S = ps.S;
+        if (S == 2) {
+            off[0] = edge ? 0 : eps;
+            off[1] = -eps;
+            double den = edge ? eps : 2 * eps;
+            ps.c[i][0] = fac / den;
+            ps.c[i][1] = -fac / den;
+        } else {
+            off[0] = edge ? 0 : 2 * eps;
+            off[1] = eps;
+            off[2] = -eps;
+            off[3] = -2 * eps;
+            double den = edge ? 6. * eps : 12. * eps;
+            ps.c[i][0] = -fac / den;
+            ps.c[i][1] = 8. * fac / den;
+            ps.c[i][2] = -8. * fac / den;
+            ps.c[i][3] = fac / den;
+        }
+        for (int s = 0; s < S; s++) {
+            double w[GWH_FROWS];
+            for (int j = 0; j < dim; j++) w[j] = v[j];
+            w[i] = v[i] + off[s];
+            gwh_params q = base;
+            repack(w, basis, q);
+            bool two = (S == 4) && (s == 0 || s == 3);
+            if (two && det != ref_det) q.tc -= host_dtoa(q.RA, q.DEC, q.gmst, ref_det, det);
+            inst.push_back(q);
+            no_tof.push_back(two ? 1 : 0);
+        }
+    }
+    return dim;
+}
+
+/* ------------------------------------------------------------------------- */
+/* device: stencil                                                            */
+/* ------------------------------------------------------------------------- */
+
+struct StencilArgs {
+    const InstCoef *coef;        /* points x (dim*S [+1 central]) records */
+    const unsigned char *no_tof;
+    const double *cw;            /* points x 16 x 4 stencil weights */
+    int dim, S, per_point;       /* records per point */
+    int d_first, d_count;        /* detectors of the event accumulated */
+    int skyavg;                  /* sky-averaged IMRPhenomD: amplitude / phase derivatives */
+    double sa_sign;              /* +1 (order 2) or -1 (order 4): i d(phase) A sign of fisher.cpp:299-315 */
+};
+
+template <int MODEL, typename T>
+__global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs a, const EventDesc *__restrict__ events,
+                                                                     int event0, T *__restrict__ rows)
+{
+    const int pt = blockIdx.y;
+    const int k = blockIdx.x * GWH_TPB + threadIdx.x;
+    const EventDesc &E = events[event0];
+    const int N = E.N;
+    if (k >= N) return;
+    const double f = E.f[k];
+    const InstCoef *C = a.coef + (size_t)pt * a.per_point;
+    const double *cw = a.cw + (size_t)pt * GWH_FROWS * GWH_MAXSTEN;
+    const unsigned char *nt = a.no_tof + (size_t)pt * a.per_point;
+    const size_t nrows = (size_t)a.d_count * N * 2;
+    T *base = rows + (size_t)pt * nrows * GWH_FROWS;
+    if (a.skyavg) {
+        /* d h = dA + i sign d(phase) A_central (fisher.cpp:292-316); no detector projection */
+        const InstCoef &Cc = C[a.dim * a.S];
+        double Ac = 0, u, lx, ph;
+        if (f <= Cc.fcut) gwh_phenomd_fp64(Cc, f, Ac, ph, u, lx);
+        double sw = sqrt(E.wS[(size_t)a.d_first * N + k]);
+        T *r0 = base + (size_t)(2 * k) * GWH_FROWS;
+        for (int i = 0; i < GWH_FROWS; i++) {
+            double dA = 0, dP = 0;
+            if (i < a.dim) {
+                for (int s = 0; s < a.S; s++) {
+                    const InstCoef &Ci = C[i * a.S + s];
+                    double A = 0, phi;
+                    gwh_phenomd_fp64(Ci, f, A, phi, u, lx);
+                    if (!(f <= Ci.fcut)) A = 0;
+                    dA += cw[i * GWH_MAXSTEN + s] * A;
+                    dP += cw[i * GWH_MAXSTEN + s] * (-phi); /* construct_phase returns -(Psi - tc (f-fref) - phic) */
+                }
+            }
+            r0[i] = (T)(dA * sw);
+            r0[GWH_FROWS + i] = (T)(a.sa_sign * dP * Ac * sw);
+        }
+        return;
+    }
+    double sw[GWH_MAXDET];
+    for (int dd = 0; dd < GWH_MAXDET; dd++) sw[dd] = (dd < a.d_count) ? sqrt(E.wS[(size_t)(a.d_first + dd) * N + k]) : 0;
+    for (int i = 0; i < GWH_FROWS; i++) {
+        double dr[GWH_MAXDET] = {0, 0, 0, 0}, di[GWH_MAXDET] = {0, 0, 0, 0};
+        if (i < a.dim) {
+            for (int s = 0; s < a.S; s++) {
+                const InstCoef &Ci = C[i * a.S + s];
+                if (!(f <= Ci.fcut)) continue; /* h = 0 above 0.2/M of this evaluation */
+                const double c = cw[i * GWH_MAXSTEN + s];
+                const bool tof = !nt[i * a.S + s];
+                if (MODEL == GWH_IMRPHENOMD) {
+                    double A, phi, u, lx;
+                    gwh_phenomd_fp64(Ci, f, A, phi, u, lx);
+                    for (int dd = 0; dd < GWH_MAXDET; dd++) {
+                        if (dd >= a.d_count) break;
+                        int d = a.d_first + dd;
+                        double sn, cs;
+                        sincos(phi + (tof ? Ci.det[d][3] : 0.0) * f, &sn, &cs);
+                        double er = A * cs, ei = -A * sn; /* A K e^{-i(phi + 2 pi dt f)} */
+                        dr[dd] += c * (er * Ci.det[d][0] - ei * Ci.det[d][1]);
+                        di[dd] += c * (er * Ci.det[d][1] + ei * Ci.det[d][0]);
+                    }
+                } else {
+                    double A, Phi, hpr, hpi, hcr, hci;
+                    gwh_pv2_fp64(Ci, f, A, Phi, hpr, hpi, hcr, hci);
+                    for (int dd = 0; dd < GWH_MAXDET; dd++) {
+                        if (dd >= a.d_count) break;
+                        int d = a.d_first + dd;
+                        double Gr = Ci.det[d][0] * hpr + Ci.det[d][1] * hcr;
+                        double Gi = Ci.det[d][0] * hpi + Ci.det[d][1] * hci;
+                        double sn, cs;
+                        sincos(Phi + (tof ? Ci.det[d][3] : 0.0) * f, &sn, &cs);
+                        dr[dd] += c * A * (Gr * cs + Gi * sn);
+                        di[dd] += c * A * (Gi * cs - Gr * sn);
+                    }
+                }
+            }
+        }
+        for (int dd = 0; dd < GWH_MAXDET; dd++) {
+            if (dd >= a.d_count) break;
+            T *r0 = base + ((size_t)dd * N + k) * 2 * GWH_FROWS;
+            r0[i] = (T)(dr[dd] * sw[dd]);
+            r0[GWH_FROWS + i] = (T)(di[dd] * sw[dd]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* device: Gram on the matrix cores                                           */
+/* ------------------------------------------------------------------------- */
+
+/* Each block: GWH_GRAM_ROWS operand rows of one point, 4 waves; each wave accumulates a 16x16 tile of
+ * a a^T over its rows, 4 rows per MFMA (lane l supplies a[row0 + l/16][l%16] as both A and B operand,
+ * coalesced 512 B per wave per instruction).  Output: per block 256 FP64 partial sums. */
+template <typename T>
+__global__ __launch_bounds__(GWH_TPB) void gwh_gram_kernel(const T *__restrict__ rows, size_t nrows,
+                                                           double *__restrict__ partial, int nblk)
+{
+    __shared__ double lds[GWH_TPB / 64][256];
+    const int pt = blockIdx.y, blk = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const T *a = rows + (size_t)pt * nrows * GWH_FROWS;
+    size_t r0 = (size_t)blk * GWH_GRAM_ROWS + (size_t)wv * (GWH_GRAM_ROWS / 4);
+    size_t r1 = r0 + GWH_GRAM_ROWS / 4;
+    if (r1 > nrows) r1 = nrows;
+    double out[4];
+    if (sizeof(T) == 8) {
+        gwh_d4 acc = {0, 0, 0, 0};
+        for (size_t r = r0; r < r1; r += 4) {
+            size_t rr = r + (lane >> 4);
+            double x = (rr < r1) ? (double)a[rr * GWH_FROWS + (lane & 15)] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, acc, 0, 0, 0);
+        }
+        for (int j = 0; j < 4; j++) out[j] = acc[j];
+    } else {
+        gwh_f4 acc = {0, 0, 0, 0};
+        for (size_t r = r0; r < r1; r += 4) {
+            size_t rr = r + (lane >> 4);
+            float x = (rr < r1) ? (float)a[rr * GWH_FROWS + (lane & 15)] : 0.0f;
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x, x, acc, 0, 0, 0);
+        }
+        for (int j = 0; j < 4; j++) out[j] = acc[j];
+    }
+    /* lane l, register j holds element (4 (l/16) + j, l % 16) of the 16x16 result */
+    for (int j = 0; j < 4; j++) lds[wv][(4 * (lane >> 4) + j) * 16 + (lane & 15)] = out[j];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 256; e += GWH_TPB) {
+        double s = lds[0][e];
+        for (int w = 1; w < GWH_TPB / 64; w++) s += lds[w][e];
+        partial[((size_t)pt * nblk + blk) * 256 + e] = s;
+    }
+}
+
+/* fixed-order FP64 sum of the Gram partials; symmetric output (the upper and lower triangles are
+ * averaged so the result is exactly symmetric, as calculate_fisher_elements assigns out[k][j] = out[j][k]) */
+__global__ void gwh_gram_reduce_kernel(const double *__restrict__ partial, int nblk, int dim, double scale,
+                                       double *__restrict__ out)
+{
+    const int pt = blockIdx.x;
+    for (int e = threadIdx.x; e < dim * dim; e += blockDim.x) {
+        int i = e / dim, j = e % dim;
+        double s1 = 0, s2 = 0;
+        for (int b = 0; b < nblk; b++) {
+            s1 += partial[((size_t)pt * nblk + b) * 256 + i * 16 + j];
+            s2 += partial[((size_t)pt * nblk + b) * 256 + j * 16 + i];
+        }
+        out[(size_t)pt * dim * dim + e] = 0.5 * (s1 + s2) * scale;
+    }
+}
+
+/* FP64 LU inverse with partial pivoting per point -> marginal sigmas sqrt(Sigma_ii) */
+__global__ void gwh_fisher_invert_kernel(const double *__restrict__ F, int n, int dim, double *__restrict__ sigma)
+{
+    int pt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pt >= n) return;
+    double a[GWH_FROWS][GWH_FROWS], x[GWH_FROWS];
+    int perm[GWH_FROWS];
+    const double *M = F + (size_t)pt * dim * dim;
+    for (int i = 0; i < dim; i++) {
+        perm[i] = i;
+        for (int j = 0; j < dim; j++) a[i][j] = M[i * dim + j];
+    }
+    bool sing = false;
+    for (int k = 0; k < dim; k++) {
+        int piv = k;
+        double mx = fabs(a[k][k]);
+        for (int i = k + 1; i < dim; i++)
+            if (fabs(a[i][k]) > mx) {
+                mx = fabs(a[i][k]);
+                piv = i;
+            }
+        if (mx == 0) {
+            sing = true;
+            break;
+        }
+        if (piv != k) {
+            for (int j = 0; j < dim; j++) {
+                double t = a[k][j];
+                a[k][j] = a[piv][j];
+                a[piv][j] = t;
+            }
+            int t = perm[k];
+            perm[k] = perm[piv];
+            perm[piv] = t;
+        }
+        for (int i = k + 1; i < dim; i++) {
+            a[i][k] /= a[k][k];
+            for (int j = k + 1; j < dim; j++) a[i][j] -= a[i][k] * a[k][j];
+        }
+    }
+    for (int c = 0; c < dim; c++) {
+        if (sing) {
+            sigma[(size_t)pt * dim + c] = __longlong_as_double(0x7ff8000000000000LL);
+            continue;
+        }
+        for (int i = 0; i < dim; i++) {
+            double s = (perm[i] == c) ? 1.0 : 0.0;
+            for (int j = 0; j < i; j++) s -= a[i][j] * x[j];
+            x[i] = s;
+        }
+        for (int i = dim - 1; i >= 0; i--) {
+            double s = x[i];
+            for (int j = i + 1; j < dim; j++) s -= a[i][j] * x[j];
+            x[i] = s / a[i][i];
+        }
+        sigma[(size_t)pt * dim + c] = sqrt(x[c]);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* host driver                                                                */
+/* ------------------------------------------------------------------------- */
+
+/* Fisher matrices of `n` points on a resident event.  d_first/d_count select the detectors whose
+ * responses enter (network sum = all); ref_det is the arrival-time reference. */
+static int fisher_run(int event, int model, int basis, int precision, const gwh_params *params, int n, int order,
+                      double eps, int d_first, int d_count, int det_for_tof, int ref_det, double scale, double *out,
+                      double *sigma)
+{
+    Engine &E = gwh_engine();
+    const Event &ev = E.events[event];
+    int dim = basis_dim(basis);
+    bool sky = (basis == GWH_BASIS_PHENOMD_SKYAVG);
+    if (sky && model != GWH_IMRPHENOMD)
+        return gwh_set_error(GWH_ERR_UNSUPPORTED, "Sky averaged IMRPhenomP is not supported for regular fishers.");
+    int S = (order >= 4) ? 4 : 2;
+    int per_point = dim * S + (sky ? 1 : 0);
+    const int N = ev.N;
+    const size_t nrows = (size_t)(sky ? 1 : d_count) * N * 2;
+    const size_t row_bytes = GWH_FROWS * ((precision == GWH_FP64) ? 8 : 4);
+    /* points per pass: bounded operand buffer (~1 GiB) */
+    size_t per_pt_bytes = nrows * row_bytes;
+    int chunk = (int)std::max<size_t>(1, ((size_t)1 << 30) / per_pt_bytes);
+    if (chunk > n) chunk = n;
+    if (chunk > 65535) chunk = 65535;
+    int nblk = (int)((nrows + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS);
+    int rc;
+    if ((rc = E.fisher_deriv.ensure((size_t)chunk * per_pt_bytes)) != GWH_OK) return rc;
+    if ((rc = E.fisher_gram.ensure((size_t)chunk * nblk * 256 * 8)) != GWH_OK) return rc;
+    if ((rc = E.fisher_out.ensure((size_t)n * dim * dim * 8)) != GWH_OK) return rc;
+    for (int p0 = 0; p0 < n; p0 += chunk) {
+        int np = std::min(chunk, n - p0);
+        std::vector<gwh_params> inst;
+        std::vector<unsigned char> no_tof;
+        std::vector<double> cw((size_t)np * GWH_FROWS * GWH_MAXSTEN, 0.0);
+        inst.reserve((size_t)np * per_point);
+        for (int q = 0; q < np; q++) {
+            PointStencil ps;
+            build_stencil(params[p0 + q], basis, order, eps, det_for_tof, ref_det, inst, no_tof, ps);
+            if (sky) {
+                inst.push_back(params[p0 + q]); /* central amplitude: the caller's parameters */
+                no_tof.push_back(0);
+            }
+            for (int i = 0; i < GWH_FROWS; i++)
+                for (int s = 0; s < GWH_MAXSTEN; s++) cw[((size_t)q * GWH_FROWS + i) * GWH_MAXSTEN + s] = ps.c[i][s];
+        }
+        int ni = (int)inst.size();
+        if ((rc = E.params.ensure((size_t)ni * sizeof(gwh_params))) != GWH_OK) return rc;
+        if ((rc = E.coef.ensure((size_t)ni * sizeof(InstCoef))) != GWH_OK) return rc;
+        if ((rc = E.misc.ensure(cw.size() * 8 + no_tof.size() + 256)) != GWH_OK) return rc;
+        GWH_HIP(hipMemcpyAsync(E.params.p, inst.data(), (size_t)ni * sizeof(gwh_params), hipMemcpyHostToDevice,
+                               E.stream));
+        char *misc = (char *)E.misc.p;
+        GWH_HIP(hipMemcpyAsync(misc, cw.data(), cw.size() * 8, hipMemcpyHostToDevice, E.stream));
+        GWH_HIP(hipMemcpyAsync(misc + cw.size() * 8, no_tof.data(), no_tof.size(), hipMemcpyHostToDevice, E.stream));
+        if ((rc = gwh_setup_launch(nullptr, event, model, E.params.as<const gwh_params>(), ni, E.coef.as<InstCoef>(),
+                                   E.stream)) != GWH_OK)
+            return rc;
+        StencilArgs a;
+        a.coef = E.coef.as<const InstCoef>();
+        a.cw = (const double *)misc;
+        a.no_tof = (const unsigned char *)(misc + cw.size() * 8);
+        a.dim = dim;
+        a.S = S;
+        a.per_point = per_point;
+        a.d_first = d_first;
+        a.d_count = sky ? 1 : d_count;
+        a.skyavg = sky;
+        a.sa_sign = (S == 4) ? -1.0 : 1.0;
+        dim3 g((N + GWH_TPB - 1) / GWH_TPB, np);
+        const EventDesc *evp = (const EventDesc *)E.d_events;
+        if (precision == GWH_FP64) {
+            if (model == GWH_IMRPHENOMD)
+                hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, double>), g, dim3(GWH_TPB), 0, E.stream,
+                                   a, evp, event, E.fisher_deriv.as<double>());
+            else
+                hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMPV2, double>), g, dim3(GWH_TPB), 0,
+                                   E.stream, a, evp, event, E.fisher_deriv.as<double>());
+            GWH_HIP(hipGetLastError());
+            hipLaunchKernelGGL(gwh_gram_kernel<double>, dim3(nblk, np), dim3(GWH_TPB), 0, E.stream,
+                               E.fisher_deriv.as<const double>(), nrows, E.fisher_gram.as<double>(), nblk);
+        } else {
+            if (model == GWH_IMRPHENOMD)
+                hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, float>), g, dim3(GWH_TPB), 0, E.stream,
+                                   a, evp, event, E.fisher_deriv.as<float>());
+            else
+                hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMPV2, float>), g, dim3(GWH_TPB), 0,
+                                   E.stream, a, evp, event, E.fisher_deriv.as<float>());
+            GWH_HIP(hipGetLastError());
+            hipLaunchKernelGGL(gwh_gram_kernel<float>, dim3(nblk, np), dim3(GWH_TPB), 0, E.stream,
+                               E.fisher_deriv.as<const float>(), nrows, E.fisher_gram.as<double>(), nblk);
+        }
+        GWH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(gwh_gram_reduce_kernel, dim3(np), dim3(256), 0, E.stream, E.fisher_gram.as<const double>(),
+                           nblk, dim, scale, E.fisher_out.as<double>() + (size_t)p0 * dim * dim);
+        GWH_HIP(hipGetLastError());
+    }
+    if (sigma) {
+        if ((rc = E.out2.ensure((size_t)n * dim * 8)) != GWH_OK) return rc;
+        hipLaunchKernelGGL(gwh_fisher_invert_kernel, dim3((n + 63) / 64), dim3(64), 0, E.stream,
+                           E.fisher_out.as<const double>(), n, dim, E.out2.as<double>());
+        GWH_HIP(hipGetLastError());
+        GWH_HIP(hipMemcpyAsync(sigma, E.out2.p, (size_t)n * dim * 8, hipMemcpyDeviceToHost, E.stream));
+    }
+    GWH_HIP(hipMemcpyAsync(out, E.fisher_out.p, (size_t)n * dim * dim * 8, hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
+}
+
+extern "C" {
+
+int gwh_fisher_numerical(const double *frequencies, int length, int basis, int detector, int reference_detector,
+                         double *out, int dim, const gwh_params *params, int order, const double *psd, double epsilon)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int rc = gwh_ensure_init();
+    if (rc != GWH_OK) return rc;
+    if (!frequencies || !out || !params || !psd || length < 2)
+        return gwh_set_error(GWH_ERR_INVALID, "null argument or length < 2");
+    if (basis_dim(basis) < 0 || basis_dim(basis) != dim)
+        return gwh_set_error(GWH_ERR_INVALID, "dimension does not match the parameter basis");
+    if (order != 2 && order != 4) return gwh_set_error(GWH_ERR_INVALID, "order must be 2 or 4");
+    if (detector < 0 || detector >= GWH_NDET_TABLE || reference_detector < 0 || reference_detector >= GWH_NDET_TABLE)
+        return gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported detector");
+    int model = (basis == GWH_BASIS_PV2_FULL || basis == GWH_BASIS_PV2_MCMC) ? GWH_IMRPHENOMPV2 : GWH_IMRPHENOMD;
+    if (basis == GWH_BASIS_PHENOMD_SKYAVG && !params->sky_average)
+        return gwh_set_error(GWH_ERR_INVALID, "the sky-averaged basis needs params->sky_average");
+    if (basis != GWH_BASIS_PHENOMD_SKYAVG && params->sky_average)
+        return gwh_set_error(GWH_ERR_UNSUPPORTED, "sky_average selects the sky-averaged IMRPhenomD basis only");
+    /* an event holding the PSD twice (reference detector, detector): the time-of-flight phase of the
+     * second one is relative to the first; Fisher weights use df = f[1]-f[0] (fisher.cpp:2746) */
+    int dets[2] = {reference_detector, detector};
+    int nd = (detector != reference_detector) ? 2 : 1;
+    if (nd == 1) dets[0] = detector;
+    std::vector<double> zeros((size_t)2 * length * nd, 0.0), ps((size_t)length * nd);
+    for (int d = 0; d < nd; d++) memcpy(&ps[(size_t)d * length], psd, (size_t)length * 8);
+    int ev;
+    if ((rc = gwh_event_create_ex(frequencies, length, nd, dets, zeros.data(), ps.data(), GWH_SIMPSONS, nullptr, 0, 1,
+                                  0, &ev)) != GWH_OK)
+        return rc;
+    rc = fisher_run(ev, model, basis, GWH_FP64, params, 1, order, epsilon, nd - 1, 1, detector, reference_detector,
+                    1.0, out, nullptr);
+    (void)hipStreamSynchronize(E.stream);
+    (void)hipFree(E.events[ev].block);
+    E.events[ev] = Event();
+    return rc;
+}
+
+int gwh_fisher_batch(int event, int model, int basis, int precision, const gwh_params *params, int n, int dim,
+                     double epsilon, double *out, double *sigma)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int rc = gwh_ensure_init();
+    if (rc != GWH_OK) return rc;
+    if (n <= 0) return GWH_OK;
+    if (!params || !out) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    if (event < 0 || event >= GWH_MAX_EVENTS || !E.events[event].live)
+        return gwh_set_error(GWH_ERR_INVALID, "unknown event");
+    if (basis_dim(basis) < 0 || basis_dim(basis) != dim)
+        return gwh_set_error(GWH_ERR_INVALID, "dimension does not match the parameter basis");
+    if (precision != GWH_FP64 && precision != GWH_MIXED) return gwh_set_error(GWH_ERR_INVALID, "bad precision");
+    if ((basis == GWH_BASIS_PV2_FULL || basis == GWH_BASIS_PV2_MCMC) != (model == GWH_IMRPHENOMPV2))
+        return gwh_set_error(GWH_ERR_INVALID, "basis does not belong to the model");
+    const Event &ev = E.events[event];
+    /* the event's weights use the likelihood's df = f[N/2]-f[N/2-1]; the Fisher uses f[1]-f[0] */
+    double scale = ev.df_fisher / ev.df_like;
+    return fisher_run(event, model, basis, precision, params, n, 2, epsilon, 0, ev.D, ev.det[0], ev.det[0], scale, out,
+                      sigma);
+}
+
+} /* extern "C" */

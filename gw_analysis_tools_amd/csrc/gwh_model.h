@@ -1,0 +1,806 @@
+/*
+ * gwh_model.h -- per-instance setup and per-bin evaluation of IMRPhenomD and
+ * IMRPhenomPv2 on the GPU (gfx950).
+ *
+ * The setup (one lane per instance, FP64) folds everything that does not
+ * depend on frequency into one InstCoef record: the 19 calibration fits, the
+ * ringdown/damping frequencies, the C1 connection coefficients of amplitude
+ * and phase, the phase/time alignment, the precession constants, and for each
+ * detector the complex projection constant and the time-of-flight phase slope.
+ * The per-bin kernels then only evaluate the closed forms of one region.
+ *
+ * Reference: src/IMRPhenomD.cpp, src/IMRPhenomP.cpp, src/util.cpp,
+ * src/waveform_generator.cpp, src/detector_util.cpp, src/waveform_util.cpp.
+ */
+#ifndef GWH_MODEL_H
+#define GWH_MODEL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gwat_hip.h"
+
+#define GWH_MAXDET 4
+#define GWH_PI 3.141592653589793238462643383279502884
+#define GWH_2PI 6.283185307179586476925286766559005768
+#define GWH_MSOL_SEC 4.925491025543575903411922162094833998e-6
+#define GWH_C_SI 299792458.
+#define GWH_MPC_SI 3.085677581491367278913937957796471611e22
+#define GWH_GAMMA_E 0.5772156649015328606065120900824024310421
+#define GWH_LOG_64 4.15888308336 /* the reference's truncated ln 64 (IMRPhenomD.cpp:25) */
+#define GWH_SQRT_6 2.44948974278317788
+
+/* Frequency-independent constants of one instance (one parameter vector on
+ * one event).  All dimensionful quantities in seconds / Hz. */
+struct InstCoef {
+    double M, eta, fcut;
+    double f1, f3, f1p, f2p; /* amplitude and phase transition frequencies */
+    double fRD, fdamp;
+    double amp_pref;         /* A0 M^{7/6} (Pv2: / (2 sqrt(5/64 pi)) / 2) */
+    double ca[7];            /* inspiral amplitude PN coefficients times pi^{i/3}: polynomial in u = (Mf)^{1/3} */
+    double rho[3];           /* rho_1..3 (powers u^7, u^8, u^9) */
+    double delta[5];         /* intermediate amplitude polynomial in Mf */
+    double mr_k;             /* gamma1 gamma3 fdamp / M */
+    double mr_g2;            /* gamma2 / (gamma3 fdamp) */
+    double mr_w2;            /* (gamma3 fdamp)^2 */
+    double K;                /* 3 / (128 eta) */
+    double cp[8];            /* TF2 phase coefficients in v = (pi M f)^{1/3}; cp[5], cp[6] are log dependent */
+    double c8, c9, c10, c11; /* cp5 = c8 + c9 ln v^3, cp6 = c10 - c11 ln v^3 */
+    double sig[4];           /* sigma_1/eta, (3/4) sigma_2/eta, (3/5) sigma_3/eta, (1/2) sigma_4/eta */
+    double beta[4];          /* beta_k / eta (beta_3 / (3 eta)) */
+    double alpha[6];         /* alpha_k / eta ((4/3) alpha_3 / eta) ; alpha[5] = alpha_5 fRD */
+    double tc_total, phic, f_ref; /* phi = Psi - tc_total (f - f_ref) - phic (Pv2: + t_corr term) */
+    double t_corr_2pi;       /* Pv2: 2 pi t_corr */
+    /* Pv2 precession */
+    double SP, SL, L_c1, L_c2; /* L2PN: eta (1 + L_c1 x + L_c2 x^2) / sqrt(x) */
+    double a_c[5], e_c[5];   /* Euler alpha / epsilon coefficients */
+    double alpha_add;        /* alpha0 - alpha(f_ref) */
+    double eps_add;          /* -epsilon(f_ref) */
+    double ylm[5];           /* -2Y_2m(thetaJN, 0), m = -2..2 */
+    /* per detector: PhenomD: (Re K, Im K, |K|^2, 2 pi dt) ; Pv2: (P, Q, 0, 2 pi dt) */
+    double det[GWH_MAXDET][4];
+    double wf_a, wf_b; /* polarisation constants: PhenomD ((1+cos^2 i)/2, cos i); Pv2 (cos 2 zeta, sin 2 zeta) */
+    double pad;
+};
+
+/* QNM spline table in device memory: knots, values and S''/2 of the natural spline */
+struct QnmTable {
+    const double *spin, *fring, *fdamp, *c_ring, *c_damp;
+    int n;
+};
+
+/* detector geometry in device memory */
+struct DetGeom {
+    double tensor[9];
+    double loc[3];
+    double gfac;
+};
+
+/* ------------------------------------------------------------------------- */
+/* FP64 setup helpers (device, one lane per instance)                          */
+/* ------------------------------------------------------------------------- */
+
+__device__ __forceinline__ double gwh_spline_eval(const double *x, const double *y, const double *c, int n,
+                                                  double xq)
+{
+    if (!(xq >= x[0] && xq <= x[n - 1])) return __longlong_as_double(0x7ff8000000000000LL);
+    int lo = 0, hi = n - 1;
+    while (hi > lo + 1) {
+        int mid = (hi + lo) >> 1;
+        if (x[mid] > xq)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    double h = x[lo + 1] - x[lo];
+    double b = (y[lo + 1] - y[lo]) / h - h * (c[lo + 1] + 2.0 * c[lo]) / 3.0;
+    double d = (c[lo + 1] - c[lo]) / (3.0 * h);
+    double dx = xq - x[lo];
+    return y[lo] + dx * (b + dx * (c[lo] + dx * d));
+}
+
+/* 19 calibration fits (IMRPhenomD.cpp:852-870) */
+__device__ __forceinline__ double gwh_fit(const double *tab, int row, double eta, double x)
+{
+    const double *p = tab + 11 * row;
+    return p[0] + p[1] * eta + x * (p[2] + p[3] * eta + p[4] * eta * eta) +
+           x * x * (p[5] + p[6] * eta + p[7] * eta * eta) + x * x * x * (p[8] + p[9] * eta + p[10] * eta * eta);
+}
+
+__device__ __forceinline__ double gwh_final_spin0815(double eta, double chi1, double chi2)
+{
+    double Seta = sqrt(1.0 - 4.0 * eta);
+    double m1 = 0.5 * (1.0 + Seta), m2 = 0.5 * (1.0 - Seta);
+    double s = m1 * m1 * chi1 + m2 * m2 * chi2;
+    double eta2 = eta * eta, eta3 = eta2 * eta, s2 = s * s, s3 = s2 * s;
+    return eta * (3.4641016151377544 - 4.399247300629289 * eta + 9.397292189321194 * eta2 -
+                  13.180949901606242 * eta3 +
+                  s * ((1.0 / eta - 0.0850917821418767 - 5.837029316602263 * eta) +
+                       (0.1014665242971878 - 2.0967746996832157 * eta) * s +
+                       (-1.3546806617824356 + 4.108962025369336 * eta) * s2 +
+                       (-0.8676969352555539 + 2.064046835273906 * eta) * s3));
+}
+
+__device__ __forceinline__ double gwh_erad0815(double eta, double chi1, double chi2)
+{
+    double Seta = sqrt(1.0 - 4.0 * eta);
+    double m1 = 0.5 * (1.0 + Seta), m2 = 0.5 * (1.0 - Seta);
+    double m1s = m1 * m1, m2s = m2 * m2;
+    double s = (m1s * chi1 + m2s * chi2) / (m1s + m2s);
+    double eta2 = eta * eta, eta3 = eta2 * eta;
+    return (eta * (0.055974469826360077 + 0.5809510763115132 * eta - 0.9606726679372312 * eta2 +
+                   3.352411249771192 * eta3) *
+            (1. + (-0.0030302335878845507 - 2.0066110851351073 * eta + 7.7050567802399215 * eta2) * s)) /
+           (1. + (-0.6714403054720589 - 1.4756929437702908 * eta + 7.304676214885011 * eta2) * s);
+}
+
+/* Everything the setup needs while building the record */
+struct SetupState {
+    double m1, m2, M, eta, chi1, chi2, chi_s, chi_a, delta, chi_pn, Mc, A0;
+    double fRD, fdamp, f1, f3, f1p, f2p;
+    double rho[3], v2, g[3], sigma[5], beta[4], alpha[6];
+    double pa[7];  /* amplitude PN coefficients (reference numbering) */
+    double pp[12]; /* phase PN coefficients (reference numbering), [5],[6] dynamic */
+    double delta_c[5];
+};
+
+/* TF2 + sigma inspiral phase at f (IMRPhenomD.cpp:1364-1399) */
+__device__ __forceinline__ double gwh_setup_phase_ins(const SetupState &S, double f)
+{
+    double x = GWH_PI * S.M * f;
+    double v = cbrt(x);
+    double lx = log(x);
+    double c5 = S.pp[8] + lx * S.pp[9];
+    double c6 = S.pp[10] - S.pp[11] * lx;
+    double v2 = v * v;
+    double poly = S.pp[0] + v2 * (S.pp[2] + v * (S.pp[3] + v * (S.pp[4] + v * (c5 + v * (c6 + v * S.pp[7])))));
+    double Mf = S.M * f;
+    double u = cbrt(Mf);
+    double Mf43 = Mf * u, Mf53 = Mf43 * u;
+    return -GWH_PI / 4. + 3. / (128. * S.eta) * poly / (v2 * v2 * v) +
+           (1. / S.eta) * (S.sigma[1] * Mf + 0.75 * S.sigma[2] * Mf43 + 0.6 * S.sigma[3] * Mf53 +
+                           0.5 * S.sigma[4] * Mf * Mf);
+}
+
+/* d/df of the inspiral phase (IMRPhenomD.cpp:1426-1460) */
+__device__ __forceinline__ double gwh_setup_dphase_ins(const SetupState &S, double f)
+{
+    double M = S.M;
+    double x = GWH_PI * M * f;
+    double lx = log(x);
+    double c[8];
+    for (int i = 0; i < 8; i++) c[i] = S.pp[i];
+    c[5] = S.pp[8] + lx * S.pp[9];
+    c[6] = S.pp[10] - S.pp[11] * lx;
+    double K = 3. / (128. * S.eta);
+    double sum = 0;
+    for (int i = 0; i < 8; i++) sum += c[i] * ((i - 5.) / 3.) * pow(x, (i - 8.) / 3.);
+    double d = K * (GWH_PI * M) * sum + K * (S.pp[9] - S.pp[11] * cbrt(x)) / f;
+    double sg = S.sigma[1] * M + S.sigma[2] * pow(M, 4. / 3.) * cbrt(f) + S.sigma[3] * pow(M, 5. / 3.) * pow(f, 2. / 3.) +
+                S.sigma[4] * M * M * f;
+    return d + sg / S.eta;
+}
+
+__device__ __forceinline__ double gwh_setup_phase_int(const SetupState &S, double f)
+{
+    double Mf = S.M * f;
+    return (1. / S.eta) * (S.beta[0] + S.beta[1] * Mf + S.beta[2] * log(Mf) - S.beta[3] / 3. / (Mf * Mf * Mf));
+}
+
+__device__ __forceinline__ double gwh_setup_dphase_int(const SetupState &S, double f)
+{
+    double M = S.M;
+    return (1. / S.eta) * (S.beta[1] * M + S.beta[2] / f + S.beta[3] / (M * M * M * f * f * f * f));
+}
+
+__device__ __forceinline__ double gwh_setup_phase_mr(const SetupState &S, double f)
+{
+    double Mf = S.M * f;
+    return (1. / S.eta) * (S.alpha[0] + S.alpha[1] * Mf - S.alpha[2] / Mf + (4. / 3.) * S.alpha[3] * sqrt(sqrt(Mf * Mf * Mf)) +
+                           S.alpha[4] * atan((f - S.alpha[5] * S.fRD) / S.fdamp));
+}
+
+__device__ __forceinline__ double gwh_setup_dphase_mr(const SetupState &S, double f)
+{
+    double M = S.M;
+    double t = (f - S.alpha[5] * S.fRD) / S.fdamp;
+    return (S.alpha[4] / (S.fdamp * (1. + t * t)) + S.alpha[2] / (f * f * M) + M * (S.alpha[1] + S.alpha[3] / sqrt(sqrt(f * M)))) /
+           S.eta;
+}
+
+__device__ __forceinline__ double gwh_setup_build_phase(const SetupState &S, double f)
+{
+    if (f < S.f1p) return gwh_setup_phase_ins(S, f);
+    if (f > S.f2p) return gwh_setup_phase_mr(S, f);
+    return gwh_setup_phase_int(S, f);
+}
+
+/* reduced inspiral amplitude and its frequency derivative (IMRPhenomD.cpp:1335-1420) */
+__device__ __forceinline__ void gwh_setup_amp_ins(const SetupState &S, double f, double *val, double *der)
+{
+    double M = S.M;
+    double x = GWH_PI * M * f, y = M * f;
+    double vx = cbrt(x), vy = cbrt(y);
+    double v = 0, d = 0, p = 1.0;
+    for (int i = 0; i < 7; i++) {
+        v += S.pa[i] * p;
+        if (i > 0) d += S.pa[i] * (i / 3.) * p / x * (GWH_PI * M);
+        p *= vx;
+    }
+    double y73 = y * y * vy, y83 = y73 * vy, y3 = y * y * y;
+    v += S.rho[0] * y73 + S.rho[1] * y83 + S.rho[2] * y3;
+    d += (S.rho[0] * (7. / 3.) * y73 + S.rho[1] * (8. / 3.) * y83 + S.rho[2] * 3. * y3) / y * M;
+    *val = v;
+    *der = d;
+}
+
+/* reduced merger-ringdown amplitude and its derivative (IMRPhenomD.cpp:1467-1532) */
+__device__ __forceinline__ void gwh_setup_amp_mr(const SetupState &S, double f, double *val, double *der)
+{
+    double dl = f - S.fRD, w = S.g[2] * S.fdamp;
+    double L = dl * dl + w * w;
+    double a = S.g[0] * S.g[2] * S.fdamp * S.M * exp(-S.g[1] * dl / w) / (S.M * S.M * L);
+    *val = a;
+    *der = a * (-S.g[1] / w - 2.0 * dl / L);
+}
+
+/* ---- Pv2 precession helpers (IMRPhenomP.cpp) ---- */
+__device__ __forceinline__ void gwh_rotz(double a, double &x, double &y)
+{
+    double t1 = x * cos(a) - y * sin(a), t2 = x * sin(a) + y * cos(a);
+    x = t1;
+    y = t2;
+}
+__device__ __forceinline__ void gwh_roty(double a, double &x, double &z)
+{
+    double t1 = x * cos(a) + z * sin(a), t2 = -x * sin(a) + z * cos(a);
+    x = t1;
+    z = t2;
+}
+
+__device__ __forceinline__ double gwh_l2pn(double eta, double x)
+{
+    double eta2 = eta * eta;
+    return (eta * (1.0 + (1.5 + eta / 6.0) * x + (3.375 - (19.0 * eta) / 8. - eta2 / 24.0) * x * x)) / sqrt(x);
+}
+
+/* NNLO Euler-angle coefficients (IMRPhenomP.cpp:64-144) */
+__device__ inline void gwh_euler_coeffs(double q, double chil, double chip, double *a, double *e)
+{
+    double m2 = q / (1. + q), m1 = 1. / (1. + q);
+    double dm = m1 - m2;
+    double eta = m1 * m2, eta2 = eta * eta, eta3 = eta2 * eta, eta4 = eta3 * eta;
+    double chil2 = chil * chil, chip2 = chip * chip, chip4 = chip2 * chip2;
+    double dm2 = dm * dm, dm3 = dm2 * dm;
+    double m2_2 = m2 * m2, m2_3 = m2_2 * m2, m2_4 = m2_3 * m2, m2_5 = m2_4 * m2, m2_6 = m2_5 * m2, m2_7 = m2_6 * m2,
+           m2_8 = m2_7 * m2;
+    a[0] = (-0.18229166666666666 - (5 * dm) / (64. * m2));
+    a[1] = ((-15 * dm * m2 * chil) / (128. * eta) - (35 * m2_2 * chil) / (128. * eta));
+    a[2] = (-1.7952473958333333 - (4555 * dm) / (7168. * m2) - (15 * chip2 * dm * m2_3) / (128. * eta2) -
+            (35 * chip2 * m2_4) / (128. * eta2) - (515 * eta) / 384. - (15 * dm2 * eta) / (256. * m2_2) -
+            (175 * dm * eta) / (256. * m2));
+    a[3] = -(35 * GWH_PI) / 48. - (5 * dm * GWH_PI) / (16. * m2) + (5 * dm2 * chil) / 16. + (5 * dm * m2 * chil) / 3. +
+           (2545 * m2_2 * chil) / 1152. - (5 * chip2 * dm * m2_5 * chil) / (128. * eta3) -
+           (35 * chip2 * m2_6 * chil) / (384. * eta3) + (2035 * dm * m2 * chil) / (21504. * eta) +
+           (2995 * m2_2 * chil) / (9216. * eta);
+    a[4] = (4.318908476114694 + (27895885 * dm) / (2.1676032e7 * m2) - (15 * chip4 * dm * m2_7) / (512. * eta4) -
+            (35 * chip4 * m2_8) / (512. * eta4) - (485 * chip2 * dm * m2_3) / (14336. * eta2) +
+            (475 * chip2 * m2_4) / (6144. * eta2) + (15 * chip2 * dm2 * m2_2) / (256. * eta) +
+            (145 * chip2 * dm * m2_3) / (512. * eta) + (575 * chip2 * m2_4) / (1536. * eta) + (39695 * eta) / 86016. +
+            (1615 * dm2 * eta) / (28672. * m2_2) - (265 * dm * eta) / (14336. * m2) + (955 * eta2) / 576. +
+            (15 * dm3 * eta2) / (1024. * m2_3) + (35 * dm2 * eta2) / (256. * m2_2) + (2725 * dm * eta2) / (3072. * m2) -
+            (15 * dm * m2 * GWH_PI * chil) / (16. * eta) - (35 * m2_2 * GWH_PI * chil) / (16. * eta) +
+            (15 * chip2 * dm * m2_7 * chil2) / (128. * eta4) + (35 * chip2 * m2_8 * chil2) / (128. * eta4) +
+            (375 * dm2 * m2_2 * chil2) / (256. * eta) + (1815 * dm * m2_3 * chil2) / (256. * eta) +
+            (1645 * m2_4 * chil2) / (192. * eta));
+    e[0] = a[0];
+    e[1] = a[1];
+    e[2] = (-1.7952473958333333 - (4555 * dm) / (7168. * m2) - (515 * eta) / 384. - (15 * dm2 * eta) / (256. * m2_2) -
+            (175 * dm * eta) / (256. * m2));
+    e[3] = -(35 * GWH_PI) / 48. - (5 * dm * GWH_PI) / (16. * m2) + (5 * dm2 * chil) / 16. + (5 * dm * m2 * chil) / 3. +
+           (2545 * m2_2 * chil) / 1152. + (2035 * dm * m2 * chil) / (21504. * eta) + (2995 * m2_2 * chil) / (9216. * eta);
+    e[4] = (4.318908476114694 + (27895885 * dm) / (2.1676032e7 * m2) + (39695 * eta) / 86016. +
+            (1615 * dm2 * eta) / (28672. * m2_2) - (265 * dm * eta) / (14336. * m2) + (955 * eta2) / 576. +
+            (15 * dm3 * eta2) / (1024. * m2_3) + (35 * dm2 * eta2) / (256. * m2_2) + (2725 * dm * eta2) / (3072. * m2) -
+            (15 * dm * m2 * GWH_PI * chil) / (16. * eta) - (35 * m2_2 * GWH_PI * chil) / (16. * eta) +
+            (375 * dm2 * m2_2 * chil2) / (256. * eta) + (1815 * dm * m2_3 * chil2) / (256. * eta) +
+            (1645 * m2_4 * chil2) / (192. * eta));
+}
+
+__device__ __forceinline__ double gwh_euler_eval(const double *c, double v, double logw)
+{
+    double v2 = v * v;
+    return c[0] / (v2 * v) + c[1] / v2 + c[2] / v + c[3] * logw + c[4] * v;
+}
+
+/* ------------------------------------------------------------------------- */
+/* the setup proper                                                          */
+/* ------------------------------------------------------------------------- */
+
+__device__ inline int gwh_setup_instance(const gwh_params &p, int model, const double *fits, const QnmTable &qnm,
+                                         const DetGeom *geo, const int *dets, int ndet, InstCoef &C)
+{
+    SetupState S;
+    /* util.cpp:996-1028 */
+    S.m1 = p.mass1 * GWH_MSOL_SEC;
+    S.m2 = p.mass2 * GWH_MSOL_SEC;
+    double s1x = p.spin1[0], s1y = p.spin1[1], s2x = p.spin2[0], s2y = p.spin2[1];
+    S.chi1 = p.spin1[2];
+    S.chi2 = p.spin2[2];
+    S.M = S.m1 + S.m2;
+    S.eta = (S.m1 * S.m2) / (S.M * S.M);
+    S.Mc = pow(S.m1 * S.m2, 0.6) / pow(S.M, 0.2);
+    S.chi_s = 0.5 * (S.chi1 + S.chi2);
+    S.chi_a = 0.5 * (S.chi1 - S.chi2);
+    S.delta = sqrt(1. - 4. * S.eta);
+    double chi_eff = (S.m1 * S.chi1 + S.m2 * S.chi2) / S.M;
+    S.chi_pn = chi_eff - (38. * S.eta / 113.) * (2. * S.chi_s);
+    double DL = p.DL * (GWH_MPC_SI / GWH_C_SI);
+    double kA = p.sky_average ? sqrt(GWH_PI / 30.) : sqrt(GWH_PI * 40. / 192.);
+    S.A0 = kA * S.Mc * S.Mc / DL * pow(GWH_PI * S.Mc, -7. / 6.);
+
+    /* Pv2 frame (IMRPhenomP.cpp:821-938, :1081-1211) */
+    double chip = -1, chil = 0, SP = 0, SL = 0, thetaJN = 0, alpha0 = 0, zeta = 0, phi_aligned = 0;
+    bool pv2 = (model == GWH_IMRPHENOMPV2);
+    double q = S.m1 / S.m2;
+    if (pv2) {
+        bool reduced = (p.chip + 1) > 1e-10;
+        chil = (1.0 + q) / q * chi_eff;
+        double m1_2 = S.m1 * S.m1, m2_2 = S.m2 * S.m2;
+        if (reduced) {
+            chip = p.chip;
+        } else {
+            double S1p = m1_2 * sqrt(s1y * s1y + s1x * s1x), S2p = m2_2 * sqrt(s2y * s2y + s2x * s2x);
+            double A1 = 2 + (3 * S.m2) / (2 * S.m1), A2 = 2 + (3 * S.m1) / (2 * S.m2);
+            double num = (A2 * S2p > A1 * S1p) ? A2 * S2p : A1 * S1p;
+            double den = (S.m2 > S.m1) ? A2 * m2_2 : A1 * m1_2;
+            chip = num / den;
+        }
+        double mm1 = q / (1 + q), mm2 = 1. / (1 + q);
+        SP = chip * mm1 * mm1;
+        SL = S.chi1 * mm1 * mm1 + S.chi2 * mm2 * mm2;
+        double xr = cbrt(GWH_PI * S.M * p.f_ref);
+        double L0 = S.M * S.M * gwh_l2pn(S.eta, xr * xr);
+        double J0x, J0y;
+        if (reduced) {
+            J0x = m1_2 * chip * cos(p.phip);
+            J0y = m1_2 * chip * sin(p.phip);
+        } else {
+            J0x = m1_2 * s1x + m2_2 * s2x;
+            J0y = m1_2 * s1y + m2_2 * s2y;
+        }
+        double J0z = L0 + m1_2 * S.chi1 + m2_2 * S.chi2;
+        double J0 = sqrt(J0x * J0x + J0y * J0y + J0z * J0z);
+        double thetaJ = acos(J0z / J0), phiJ = atan2(J0y, J0x);
+        phi_aligned = -phiJ;
+        double incl = p.incl, phiRef = p.phiRef;
+        double Nx = sin(incl) * cos(GWH_PI / 2. - phiRef), Ny = sin(incl) * sin(GWH_PI / 2. - phiRef), Nz = cos(incl);
+        double tx = Nx, ty = Ny, tz = Nz;
+        gwh_rotz(-phiJ, tx, ty);
+        gwh_roty(-thetaJ, tx, tz);
+        double kappa = -atan2(ty, tx);
+        tx = 0.;
+        ty = 0.;
+        tz = 1.;
+        gwh_rotz(-phiJ, tx, ty);
+        gwh_roty(-thetaJ, tx, tz);
+        gwh_rotz(kappa, tx, ty);
+        alpha0 = atan2(ty, tx);
+        tx = Nx;
+        ty = Ny;
+        tz = Nz;
+        gwh_rotz(-phiJ, tx, ty);
+        gwh_roty(-thetaJ, tx, tz);
+        gwh_rotz(kappa, tx, ty);
+        double NxJ = tx, NzJ = tz;
+        thetaJN = acos(NzJ);
+        tx = -cos(incl) * sin(phiRef);
+        ty = -cos(incl) * cos(phiRef);
+        tz = sin(incl);
+        gwh_rotz(-phiJ, tx, ty);
+        gwh_roty(-thetaJ, tx, tz);
+        gwh_rotz(kappa, tx, ty);
+        zeta = atan2(tx * NzJ - tz * NxJ, -ty);
+    }
+
+    /* calibration fits */
+    double xfit = S.chi_pn - 1;
+    for (int i = 0; i < 3; i++) S.rho[i] = gwh_fit(fits, i, S.eta, xfit);
+    S.v2 = gwh_fit(fits, 3, S.eta, xfit);
+    for (int i = 0; i < 3; i++) S.g[i] = gwh_fit(fits, 4 + i, S.eta, xfit);
+    S.sigma[0] = 0;
+    for (int i = 0; i < 4; i++) S.sigma[i + 1] = gwh_fit(fits, 7 + i, S.eta, xfit);
+    S.beta[0] = 0;
+    for (int i = 0; i < 3; i++) S.beta[i + 1] = gwh_fit(fits, 11 + i, S.eta, xfit);
+    S.alpha[0] = 0;
+    for (int i = 0; i < 5; i++) S.alpha[i + 1] = gwh_fit(fits, 14 + i, S.eta, xfit);
+    double alpha1_fit = S.alpha[1];
+
+    /* remnant spin -> ringdown and damping frequencies */
+    double af;
+    if (pv2) {
+        double Mt = S.m1 + S.m2;
+        double et = S.m1 * S.m2 / (Mt * Mt);
+        double qf, afp;
+        if (S.m1 >= S.m2) {
+            qf = S.m1 / Mt;
+            afp = gwh_final_spin0815(et, S.chi1, S.chi2);
+        } else {
+            qf = S.m2 / Mt;
+            afp = gwh_final_spin0815(et, S.chi2, S.chi1);
+        }
+        double Sperp = chip * qf * qf;
+        af = sqrt(1.0 / (afp * afp)) * afp * sqrt(Sperp * Sperp + afp * afp);
+    } else {
+        af = gwh_final_spin0815(S.eta, S.chi1, S.chi2);
+    }
+    if (p.dep_postmerger) {
+        /* IMRPhenomD.cpp:1112-1179 */
+        double eta = S.eta, eta2 = eta * eta, eta3 = eta2 * eta, eta4 = eta3 * eta;
+        double Ss = (S.chi1 * S.m1 * S.m1 + S.chi2 * S.m2 * S.m2) / (S.M * S.M);
+        double Sred = Ss / (1. - 2. * eta);
+        double Erns = 0.0559745 * eta + 0.580951 * eta2 - 0.960673 * eta3 + 3.35241 * eta4;
+        double Er = Erns * (1. + Sred * (-0.00303023 - 2.00661 * eta + 7.70506 * eta2)) /
+                    (1 + Sred * (-0.67144 - 1.47569 * eta + 7.30468 * eta2));
+        double MWRD = 1.5251 - 1.1568 * pow(1 - af, 0.1292);
+        double MWd = MWRD / (2. * (0.700 + 1.4187 * pow(1. - af, -.4990)));
+        S.fRD = (1. / GWH_2PI) * MWRD / (S.M * (1. - Er));
+        S.fdamp = (1. / GWH_2PI) * MWd / (S.M * (1. - Er));
+    } else {
+        double Er = gwh_erad0815(S.eta, S.chi1, S.chi2);
+        S.fRD = gwh_spline_eval(qnm.spin, qnm.fring, qnm.c_ring, qnm.n, af) / (1.0 - Er) / S.M;
+        S.fdamp = gwh_spline_eval(qnm.spin, qnm.fdamp, qnm.c_damp, qnm.n, af) / (1.0 - Er) / S.M;
+    }
+    S.f1p = 0.018 / S.M;
+    S.f2p = S.fRD / 2.;
+    S.f1 = 0.014 / S.M;
+    {
+        double g2 = S.g[1], g3 = S.g[2];
+        double v = (g2 > 1) ? S.fRD + (S.fdamp * (-1.) * g3) / g2 : S.fRD + S.fdamp * g3 * (sqrt(1 - g2 * g2) - 1) / g2;
+        S.f3 = fabs(v);
+    }
+
+    /* PN coefficients (IMRPhenomD.cpp:951-1032) */
+    {
+        double dm = S.delta, eta = S.eta, ca = S.chi_a, cs = S.chi_s;
+        double eta2 = eta * eta, eta3 = eta2 * eta, cs2 = cs * cs, ca2 = ca * ca, pi2 = GWH_PI * GWH_PI;
+        S.pa[0] = 1.;
+        S.pa[1] = 0.;
+        S.pa[2] = (-323. / 224 + 451 * eta / 168);
+        S.pa[3] = (27. * dm * ca / 8 + (27. / 8 - 11. * eta / 6) * cs);
+        S.pa[4] = (-27312085. / 8128512 - 1975055 * eta / 338688 + 105271 * eta2 / 24192 + (-81. / 32 + 8 * eta) * ca2 -
+                   81 * dm * ca * cs / 16 + (-81. / 32 + 17 * eta / 8) * cs2);
+        S.pa[5] = (-85 * GWH_PI / 64 + 85 * GWH_PI * eta / 16 + dm * (285197. / 16128 - 1579 * eta / 4032) * ca +
+                   (285197. / 16128 - 15317 * eta / 672 - 2227 * eta2 / 1008) * cs);
+        S.pa[6] = (-177520268561. / 8583708672 + (545384828789. / 5007163392 - 205 * pi2 / 48) * eta -
+                   3248849057 * eta2 / 178827264 + 34473079 * eta3 / 6386688 +
+                   (1614569. / 64512 - 1873643. * eta / 16128 + 2167 * eta2 / 42) * ca2 +
+                   (31 * GWH_PI / 12 - 7 * GWH_PI * eta / 3) * cs +
+                   (1614569. / 64512 - 61391 * eta / 1344 + 57451 * eta2 / 4032) * cs2 +
+                   dm * ca * (31 * GWH_PI / 12 + (1614569. / 32256 - 165961 * eta / 2688) * cs));
+        for (int i = 0; i < 12; i++) S.pp[i] = 0;
+        S.pp[0] = 1.;
+        S.pp[2] = 3715. / 756 + 55. * eta / 9;
+        S.pp[3] = -16. * GWH_PI + 113. * dm * ca / 3 + (113. / 3 - 76. * eta / 3) * cs;
+        S.pp[4] = 15293365. / 508032 + 27145. * eta / 504 + 3085. * eta2 / 72 + (-405. / 8 + 200. * eta) * ca2 -
+                  (405. / 4) * dm * ca * cs + (-405. / 8 + 5. * eta / 2) * cs2;
+        S.pp[7] = 77096675. * GWH_PI / 254016 + 378515. * GWH_PI * eta / 1512 - 74045. * GWH_PI * eta2 / 756 +
+                  dm * (-25150083775. / 3048192 + 26804935. * eta / 6048 - 1985. * eta2 / 48) * ca +
+                  (-25150083775. / 3048192 + 10566655595. * eta / 762048 - 1042165. * eta2 / 3024 + 5345. * eta3 / 36) * cs;
+        S.pp[8] = (38645. * GWH_PI / 756 - 65. * GWH_PI * eta / 9 + dm * (-732985. / 2268 - 140. * eta / 9) * ca +
+                   (-732985. / 2268 + 24260. * eta / 81 + 340. * eta2 / 9) * cs);
+        S.pp[9] = S.pp[8];
+        S.pp[10] = 11583231236531. / 4694215680 - 6848. * GWH_GAMMA_E / 21 - 640. * pi2 / 3 +
+                   (-15737765635. / 3048192 + 2255. * pi2 / 12) * eta + 76055. * eta2 / 1728 - 127825. * eta3 / 1296 +
+                   2270. * dm * ca * GWH_PI / 3 + (2270. * GWH_PI / 3 - 520. * GWH_PI * eta) * cs -
+                   6848. * (GWH_LOG_64) / 63;
+        S.pp[11] = 6848. / 63.;
+    }
+
+    /* amplitude C1 connection: Hermite quartic through (f1, v1, v1'), (f2, v2), (f3, v3, v3') */
+    {
+        double f1 = S.f1, f3 = S.f3, f2 = (f1 + f3) / 2.;
+        double v1, d1, v3, d3;
+        gwh_setup_amp_ins(S, f1, &v1, &d1);
+        gwh_setup_amp_mr(S, f3, &v3, &d3);
+        double v2 = S.v2;
+        double q12 = (v2 - v1) / (f2 - f1), q23 = (v3 - v2) / (f3 - f2);
+        double q012 = (q12 - d1) / (f2 - f1), q123 = (q23 - q12) / (f3 - f1), q234 = (d3 - q23) / (f3 - f2);
+        double q0123 = (q123 - q012) / (f3 - f1), q1234 = (q234 - q123) / (f3 - f1);
+        double q01234 = (q1234 - q0123) / (f3 - f1);
+        double a[5] = {v1, d1, q012, q0123, q01234};
+        double z[4] = {f1, f1, f2, f3};
+        double poly[5] = {a[4], 0, 0, 0, 0};
+        for (int k = 3; k >= 0; k--) {
+            double nw[5] = {0, 0, 0, 0, 0};
+            for (int j = 0; j < 4 - k; j++) {
+                nw[j + 1] += poly[j];
+                nw[j] -= z[k] * poly[j];
+            }
+            nw[0] += a[k];
+            for (int j = 0; j < 5; j++) poly[j] = nw[j];
+        }
+        double Mk = 1.0;
+        for (int k = 0; k < 5; k++) {
+            S.delta_c[k] = poly[k] / Mk;
+            Mk *= S.M;
+        }
+    }
+
+    /* phase C1 connection (IMRPhenomD.cpp:1613-1674) */
+    S.beta[0] = 0;
+    S.beta[1] = 0;
+    S.alpha[0] = 0;
+    S.alpha[1] = 0;
+    S.beta[1] = (S.eta / S.M) * gwh_setup_dphase_ins(S, S.f1p) - (S.eta / S.M) * gwh_setup_dphase_int(S, S.f1p);
+    S.beta[0] = S.eta * gwh_setup_phase_ins(S, S.f1p) - S.eta * gwh_setup_phase_int(S, S.f1p);
+    S.alpha[1] = (S.eta / S.M) * gwh_setup_dphase_int(S, S.f2p) - (S.eta / S.M) * gwh_setup_dphase_mr(S, S.f2p);
+    S.alpha[0] = S.eta * gwh_setup_phase_int(S, S.f2p) - S.eta * gwh_setup_phase_mr(S, S.f2p);
+
+    /* ---- record ---- */
+    C.M = S.M;
+    C.eta = S.eta;
+    C.fcut = .2 / S.M;
+    C.f1 = S.f1;
+    C.f3 = S.f3;
+    C.f1p = S.f1p;
+    C.f2p = S.f2p;
+    C.fRD = S.fRD;
+    C.fdamp = S.fdamp;
+    {
+        double pi13 = cbrt(GWH_PI), pw = 1.0;
+        for (int i = 0; i < 7; i++) {
+            C.ca[i] = S.pa[i] * pw;
+            pw *= pi13;
+        }
+    }
+    for (int i = 0; i < 3; i++) C.rho[i] = S.rho[i];
+    for (int i = 0; i < 5; i++) C.delta[i] = S.delta_c[i];
+    C.mr_k = S.g[0] * S.g[2] * S.fdamp / S.M;
+    C.mr_g2 = S.g[1] / (S.g[2] * S.fdamp);
+    C.mr_w2 = (S.g[2] * S.fdamp) * (S.g[2] * S.fdamp);
+    C.K = 3. / (128. * S.eta);
+    for (int i = 0; i < 8; i++) C.cp[i] = S.pp[i];
+    C.c8 = S.pp[8];
+    C.c9 = S.pp[9];
+    C.c10 = S.pp[10];
+    C.c11 = S.pp[11];
+    C.sig[0] = S.sigma[1] / S.eta;
+    C.sig[1] = 0.75 * S.sigma[2] / S.eta;
+    C.sig[2] = 0.6 * S.sigma[3] / S.eta;
+    C.sig[3] = 0.5 * S.sigma[4] / S.eta;
+    C.beta[0] = S.beta[0] / S.eta;
+    C.beta[1] = S.beta[1] / S.eta;
+    C.beta[2] = S.beta[2] / S.eta;
+    C.beta[3] = S.beta[3] / (3. * S.eta);
+    C.alpha[0] = S.alpha[0] / S.eta;
+    C.alpha[1] = S.alpha[1] / S.eta;
+    C.alpha[2] = S.alpha[2] / S.eta;
+    C.alpha[3] = (4. / 3.) * S.alpha[3] / S.eta;
+    C.alpha[4] = S.alpha[4] / S.eta;
+    C.alpha[5] = S.alpha[5] * S.fRD;
+
+    if (!pv2) {
+        /* phase and time alignment (IMRPhenomD.cpp:441-466) */
+        double phic, fref;
+        if (p.shift_phase) {
+            fref = p.f_ref;
+            phic = 2 * p.phiRef + gwh_setup_build_phase(S, fref);
+        } else {
+            fref = 0;
+            phic = p.phiRef;
+        }
+        double tc_shift = 0;
+        if (p.shift_time) tc_shift = gwh_setup_dphase_mr(S, S.f3) + (-S.alpha[1] + alpha1_fit) * S.M / S.eta;
+        C.tc_total = GWH_2PI * p.tc + tc_shift;
+        C.phic = phic;
+        C.f_ref = fref;
+        C.t_corr_2pi = 0;
+        C.amp_pref = S.A0 * pow(S.M, 7. / 6.);
+    } else {
+        C.phic = 2 * phi_aligned;
+        C.tc_total = GWH_2PI * p.tc;
+        C.f_ref = p.f_ref;
+        C.amp_pref = S.A0 * pow(S.M, 7. / 6.) / (2. * sqrt(5. / (64. * GWH_PI))) / 2.;
+        C.SP = SP;
+        C.SL = SL;
+        C.L_c1 = 1.5 + S.eta / 6.0;
+        C.L_c2 = 3.375 - (19.0 * S.eta) / 8. - S.eta * S.eta / 24.0;
+        gwh_euler_coeffs(q, chil, chip, C.a_c, C.e_c);
+        double vr = cbrt(GWH_PI * S.M * p.f_ref);
+        double lw = log(vr * vr * vr);
+        C.alpha_add = alpha0 - gwh_euler_eval(C.a_c, vr, lw);
+        C.eps_add = -gwh_euler_eval(C.e_c, vr, lw);
+        double ct = cos(thetaJN), st = sin(thetaJN);
+        C.ylm[0] = sqrt(5.0 / (64.0 * GWH_PI)) * (1.0 - ct) * (1.0 - ct);
+        C.ylm[1] = sqrt(5.0 / (16.0 * GWH_PI)) * st * (1.0 - ct);
+        C.ylm[2] = sqrt(15.0 / (32.0 * GWH_PI)) * st * st;
+        C.ylm[3] = sqrt(5.0 / (16.0 * GWH_PI)) * st * (1.0 + ct);
+        C.ylm[4] = sqrt(5.0 / (64.0 * GWH_PI)) * (1.0 + ct) * (1.0 + ct);
+        /* t_c alignment: derivative at fRD of a 10-knot natural spline of -Psi_MR (IMRPhenomP.cpp:550-606) */
+        double t_corr = 0;
+        if (p.shift_time) {
+            const int n = 10;
+            double start = .8 * S.fRD, stop = 1.2 * S.fRD;
+            double df = (stop - start) / (n - 1);
+            if (df > 0) {
+                double fs[10], ph[10], c[10], dg[10], rh[10], of[10];
+                for (int j = 0; j < n; j++) {
+                    fs[j] = start + j * df;
+                    ph[j] = -gwh_setup_build_phase(S, fs[j]);
+                }
+                int m = n - 2;
+                for (int i = 0; i < m; i++) {
+                    double h0 = fs[i + 1] - fs[i], h1 = fs[i + 2] - fs[i + 1];
+                    dg[i] = 2.0 * (h0 + h1);
+                    of[i] = h1;
+                    rh[i] = 3.0 * ((ph[i + 2] - ph[i + 1]) / h1 - (ph[i + 1] - ph[i]) / h0);
+                }
+                for (int i = 1; i < m; i++) {
+                    double w = of[i - 1] / dg[i - 1];
+                    dg[i] -= w * of[i - 1];
+                    rh[i] -= w * rh[i - 1];
+                }
+                c[0] = 0;
+                c[n - 1] = 0;
+                c[m] = rh[m - 1] / dg[m - 1];
+                for (int i = m - 2; i >= 0; i--) c[i + 1] = (rh[i] - of[i] * c[i + 2]) / dg[i];
+                int lo = 0, hi = n - 1;
+                while (hi > lo + 1) {
+                    int mid = (hi + lo) >> 1;
+                    if (fs[mid] > S.fRD)
+                        hi = mid;
+                    else
+                        lo = mid;
+                }
+                double h = fs[lo + 1] - fs[lo];
+                double b = (ph[lo + 1] - ph[lo]) / h - h * (c[lo + 1] + 2.0 * c[lo]) / 3.0;
+                double dd = (c[lo + 1] - c[lo]) / (3.0 * h);
+                double dx = S.fRD - fs[lo];
+                t_corr = (b + dx * (2.0 * c[lo] + 3.0 * dd * dx)) / GWH_2PI;
+            }
+        }
+        C.t_corr_2pi = GWH_2PI * t_corr;
+    }
+
+    /* detector projection constants and time-of-flight phase (waveform_util.cpp:152-184) */
+    double ci = cos(p.incl);
+    double kplus = .5 * (1. + ci * ci);
+    double c2z = cos(2. * zeta), s2z = sin(2. * zeta);
+    const DetGeom &g0 = geo[dets[0]];
+    C.wf_a = pv2 ? c2z : kplus;
+    C.wf_b = pv2 ? s2z : ci;
+    C.pad = 0;
+    for (int d = 0; d < GWH_MAXDET; d++) {
+        C.det[d][0] = C.det[d][1] = C.det[d][2] = C.det[d][3] = 0;
+        if (d >= ndet) continue;
+        const DetGeom &g = geo[dets[d]];
+        /* detector_util.cpp:900-1016 */
+        double gha = p.gmst - p.RA;
+        double cg = cos(gha), sg = sin(gha), cd = cos(p.DEC), sd = sin(p.DEC), cp = cos(p.psi), sp = sin(p.psi);
+        double X[3] = {-cp * sg - sp * cg * sd, -cp * cg + sp * sg * sd, sp * cd};
+        double Y[3] = {sp * sg - cp * cg * sd, sp * cg + cp * sg * sd, cp * cd};
+        double fp = 0, fx = 0;
+        for (int i = 0; i < 3; i++) {
+            double DX = g.tensor[3 * i] * X[0] + g.tensor[3 * i + 1] * X[1] + g.tensor[3 * i + 2] * X[2];
+            double DY = g.tensor[3 * i] * Y[0] + g.tensor[3 * i + 1] * Y[1] + g.tensor[3 * i + 2] * Y[2];
+            fp += X[i] * DX - Y[i] * DY;
+            fx += X[i] * DY + Y[i] * DX;
+        }
+        fp *= g.gfac;
+        fx *= g.gfac;
+        /* detector_util.cpp:768-790: dt = (r0 - rd).n / c */
+        double ha = p.gmst - p.RA;
+        double e0 = cos(p.DEC) * cos(ha), e1 = cos(p.DEC) * -sin(ha), e2 = sin(p.DEC);
+        double dt = ((g0.loc[0] - g.loc[0]) * e0 + (g0.loc[1] - g.loc[1]) * e1 + (g0.loc[2] - g.loc[2]) * e2) / GWH_C_SI;
+        if (!pv2) {
+            C.det[d][0] = fp * kplus;
+            C.det[d][1] = -fx * ci;
+            C.det[d][2] = C.det[d][0] * C.det[d][0] + C.det[d][1] * C.det[d][1];
+        } else {
+            C.det[d][0] = fp * c2z - fx * s2z;
+            C.det[d][1] = fp * s2z + fx * c2z;
+        }
+        C.det[d][3] = GWH_2PI * dt;
+    }
+    return 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* per-bin FP64 evaluation                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* IMRPhenomD amplitude A(f) and phase phi(f) = Psi(f) - tc_total (f - f_ref) - phic
+ * (IMRPhenomD.cpp:471-501); caller guarantees 0 < f <= fcut. */
+__device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, double &A, double &phi, double &u_out,
+                                                 double &lx_out)
+{
+    double Mf = C.M * f;
+    double u = cbrt(Mf);          /* (Mf)^{1/3} */
+    double su = sqrt(u);          /* (Mf)^{1/6} */
+    double inv76 = 1.0 / (Mf * su); /* (Mf)^{-7/6} */
+    double ar;
+    if (f < C.f1) {
+        double u2 = u * u, u3 = u2 * u;
+        double pn = C.ca[0] + u2 * (C.ca[2] + u * (C.ca[3] + u * (C.ca[4] + u * (C.ca[5] + u * C.ca[6]))));
+        double nr = u3 * u3 * u * (C.rho[0] + u * (C.rho[1] + u * C.rho[2]));
+        ar = pn + nr;
+    } else if (f > C.f3) {
+        double dl = f - C.fRD;
+        ar = C.mr_k * exp(-C.mr_g2 * dl) / (dl * dl + C.mr_w2);
+    } else {
+        ar = C.delta[0] + Mf * (C.delta[1] + Mf * (C.delta[2] + Mf * (C.delta[3] + Mf * C.delta[4])));
+    }
+    A = C.amp_pref * ar * inv76;
+    double psi;
+    double lx = 0;
+    if (f < C.f1p) {
+        double v = u * 1.4645918875615231; /* pi^{1/3} */
+        lx = log(GWH_PI * Mf);
+        double c5 = C.c8 + lx * C.c9;
+        double c6 = C.c10 - C.c11 * lx;
+        double v2 = v * v;
+        double poly = C.cp[0] + v2 * (C.cp[2] + v * (C.cp[3] + v * (C.cp[4] + v * (c5 + v * (c6 + v * C.cp[7])))));
+        double Mf43 = Mf * u;
+        psi = -GWH_PI / 4. + C.K * poly / (v2 * v2 * v) +
+              (C.sig[0] * Mf + C.sig[1] * Mf43 + C.sig[2] * Mf43 * u + C.sig[3] * Mf * Mf);
+    } else if (f > C.f2p) {
+        psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] / Mf + C.alpha[3] * sqrt(sqrt(Mf * Mf * Mf)) +
+              C.alpha[4] * atan((f - C.alpha[5]) / C.fdamp);
+    } else {
+        psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * log(Mf) - C.beta[3] / (Mf * Mf * Mf);
+    }
+    phi = psi - (C.tc_total * (f - C.f_ref) + C.phic);
+    u_out = u;
+    lx_out = lx;
+}
+
+/* IMRPhenomPv2 twist-up at one bin (IMRPhenomP.cpp:290-361): returns the common amplitude
+ * (already including the 1/2), the total phase and the complex h+ / hx twist-up factors. */
+__device__ __forceinline__ void gwh_pv2_fp64(const InstCoef &C, double f, double &A, double &Phi, double &hpr,
+                                             double &hpi, double &hcr, double &hci)
+{
+    double u, lx;
+    gwh_phenomd_fp64(C, f, A, Phi, u, lx);
+    /* gwh_phenomd_fp64 already applied -tc (f - f_ref) - phic; Pv2 adds 2 eps and the t_c correction */
+    double v = u * 1.4645918875615231;
+    double logw = (f < C.f1p) ? lx : log(GWH_PI * C.M * f);
+    double x = v * v;
+    double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) / v;
+    double s = C.SP / (L + C.SL);
+    double cb = 1. / sqrt(1.0 + s * s);
+    double cbh = sqrt((1.0 + cb) / 2.0), sbh = sqrt((1.0 - cb) / 2.0);
+    double c2 = cbh * cbh, s2 = sbh * sbh, c3 = c2 * cbh, s3 = s2 * sbh;
+    double d2[5] = {s3 * sbh, 2 * cbh * s3, GWH_SQRT_6 * s2 * c2, 2 * c3 * sbh, c3 * cbh};
+    double alpha = gwh_euler_eval(C.a_c, v, logw) + C.alpha_add;
+    double eps = gwh_euler_eval(C.e_c, v, logw) + C.eps_add;
+    double sa, ca;
+    sincos(alpha, &sa, &ca);
+    /* e^{i m alpha}, m=-2..2 */
+    double er[5], ei[5];
+    er[2] = 1;
+    ei[2] = 0;
+    er[3] = ca;
+    ei[3] = sa;
+    er[1] = ca;
+    ei[1] = -sa;
+    er[4] = ca * ca - sa * sa;
+    ei[4] = 2 * ca * sa;
+    er[0] = er[4];
+    ei[0] = -ei[4];
+    hpr = hpi = hcr = hci = 0;
+#pragma unroll
+    for (int m = -2; m <= 2; m++) {
+        double dm2 = (m == -2) ? d2[4] : (m == -1) ? -d2[3] : (m == 0) ? d2[2] : (m == 1) ? -d2[1] : d2[0];
+        double y = C.ylm[m + 2];
+        double t2r = er[-m + 2] * dm2 * y, t2i = ei[-m + 2] * dm2 * y;
+        double tmr = er[m + 2] * d2[m + 2] * y, tmi = ei[m + 2] * d2[m + 2] * y;
+        hpr += t2r + tmr;
+        hpi += t2i + tmi;
+        hcr -= t2i - tmi;
+        hci += t2r - tmr;
+    }
+    Phi = Phi + 2. * eps + C.t_corr_2pi * f;
+}
+
+#endif

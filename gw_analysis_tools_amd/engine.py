@@ -1,0 +1,223 @@
+"""Thin Python view of the C ABI (include/gwat_hip.h), used by tests and bench.py.
+
+Names mirror the reference's entry points (src/waveform_generator.cpp
+fourier_waveform, src/waveform_util.cpp fourier_detector_response,
+src/mcmc_gw.cpp Log_Likelihood / MCMC_likelihood_extrinsic, src/fisher.cpp
+fisher_numerical); all compute happens in libgwat_hip.so on the GPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import GWHError, Params, check, lib, PARAMS_DTYPE, IMRPHENOMD, FP64
+
+__all__ = ["fourier_waveform", "fourier_detector_response", "create_coherent_GW_detection", "work_counts", "log_likelihood", "log_likelihood_internal",
+           "mcmc_likelihood_extrinsic", "fisher_numerical", "Event", "loglike_batch", "fisher_batch",
+           "DeviceArray", "set_bins_per_lane", "synchronize"]
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _interleave(z):
+    z = np.asarray(z, dtype=np.complex128)
+    return np.ascontiguousarray(z).view(np.float64)
+
+
+def _cplx(x):
+    return x.view(np.complex128)
+
+
+def synchronize():
+    check(lib().gwh_synchronize())
+
+
+def set_bins_per_lane(bpl):
+    check(lib().gwh_set_bins_per_lane(int(bpl)))
+
+
+def fourier_waveform(f, p, model=IMRPHENOMD):
+    f = _f64(f)
+    n = len(f)
+    hp = np.zeros(2 * n)
+    hc = np.zeros(2 * n)
+    check(lib().gwh_fourier_waveform(_dp(f), n, _dp(hp), _dp(hc), model, ctypes.byref(p)))
+    return _cplx(hp), _cplx(hc)
+
+
+def fourier_detector_response(f, p, det, model=IMRPHENOMD):
+    f = _f64(f)
+    r = np.zeros(2 * len(f))
+    check(lib().gwh_fourier_detector_response(_dp(f), len(f), _dp(r), det, model, ctypes.byref(p)))
+    return _cplx(r)
+
+
+def create_coherent_GW_detection(f, p, dets, model=IMRPHENOMD):
+    f = _f64(f)
+    D = len(dets)
+    out = np.zeros(2 * D * len(f))
+    di = (ctypes.c_int * D)(*dets)
+    check(lib().gwh_create_coherent_GW_detection(di, D, _dp(f), len(f), model, ctypes.byref(p), _dp(out)))
+    return [_cplx(out[2 * len(f) * d:2 * len(f) * (d + 1)]) for d in range(D)]
+
+
+def work_counts(event, params, model=IMRPHENOMD):
+    arr = _as_params_array(params)
+    c = np.zeros(4 * len(arr), dtype=np.int32)
+    ev = event.id if hasattr(event, "id") else int(event)
+    check(lib().gwh_work_counts(ev, model, arr.ctypes.data_as(ctypes.c_void_p), len(arr),
+                                c.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return c.reshape(-1, 4)
+
+
+def _nan_check(v):
+    if np.isnan(v):
+        msg = lib().gwh_last_error().decode(errors="replace")
+        if msg:
+            raise GWHError(msg)
+    return v
+
+
+def log_likelihood(data, psd, f, p, det, model=IMRPHENOMD):
+    d = _interleave(data)
+    s = _f64(psd)
+    f = _f64(f)
+    return lib().gwh_log_likelihood(_dp(d), _dp(s), _dp(f), len(f), ctypes.byref(p), det, model)
+
+
+def log_likelihood_internal(data, psd, f, response, weights=None, log10F=False, method=0):
+    d = _interleave(data)
+    r = _interleave(response)
+    s = _f64(psd)
+    f = _f64(f)
+    w = _f64(weights) if weights is not None else None
+    return lib().gwh_log_likelihood_internal(_dp(d), _dp(s), _dp(f), _dp(w) if w is not None else None, _dp(r),
+                                             len(f), int(log10F), method)
+
+
+def mcmc_likelihood_extrinsic(f, data, psd, dets, p, model=IMRPHENOMD, T_eff=0.0, save_waveform=True):
+    f = _f64(f)
+    n = len(f)
+    D = len(dets)
+    dd = [_interleave(x) for x in data]
+    ss = [_f64(x) for x in psd]
+    DP = ctypes.POINTER(ctypes.c_double)
+    fl = (DP * D)(*[_dp(f)] * D)
+    dl = (DP * D)(*[_dp(x) for x in dd])
+    sl = (DP * D)(*[_dp(x) for x in ss])
+    lens = (ctypes.c_int * D)(*[n] * D)
+    di = (ctypes.c_int * D)(*dets)
+    return lib().gwh_mcmc_likelihood_extrinsic(int(save_waveform), ctypes.byref(p), model, lens, fl, dl, sl, None, 0,
+                                               0, di, D, T_eff)
+
+
+def fisher_numerical(f, psd, basis, det, ref_det, dim, p, order=2, epsilon=1e-8):
+    f = _f64(f)
+    s = _f64(psd)
+    out = np.zeros(dim * dim)
+    check(lib().gwh_fisher_numerical(_dp(f), len(f), basis, det, ref_det, _dp(out), dim, ctypes.byref(p), order,
+                                     _dp(s), epsilon))
+    return out.reshape(dim, dim)
+
+
+class Event:
+    """A resident event: shared grid, per-detector data and PSD kept in HBM."""
+
+    def __init__(self, f, data, psd, dets):
+        f = _f64(f)
+        D = len(dets)
+        dat = np.concatenate([_interleave(x) for x in data])
+        ps = np.concatenate([_f64(x) for x in psd])
+        ev = ctypes.c_int(-1)
+        di = (ctypes.c_int * D)(*dets)
+        check(lib().gwh_event_create(_dp(f), len(f), D, di, _dp(dat), _dp(ps), ctypes.byref(ev)))
+        self.id = ev.value
+        self.n = len(f)
+        self.ndet = D
+
+    @property
+    def nbytes(self):
+        return lib().gwh_event_bytes(self.id)
+
+    def close(self):
+        if self.id >= 0:
+            check(lib().gwh_event_destroy(self.id))
+            self.id = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _as_params_array(params):
+    if isinstance(params, np.ndarray) and params.dtype == PARAMS_DTYPE:
+        return np.ascontiguousarray(params)
+    arr = np.zeros(len(params), dtype=PARAMS_DTYPE)
+    for i, p in enumerate(params):
+        arr[i] = np.frombuffer(bytes(p), dtype=PARAMS_DTYPE)[0]
+    return arr
+
+
+def loglike_batch(event, params, model=IMRPHENOMD, precision=FP64, events=None):
+    arr = _as_params_array(params)
+    n = len(arr)
+    out = np.zeros(n)
+    ev0 = event.id if isinstance(event, Event) else int(event)
+    evs = None
+    if events is not None:
+        e = np.ascontiguousarray([x.id if isinstance(x, Event) else int(x) for x in events], dtype=np.int32)
+        evs = e.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    check(lib().gwh_loglike_batch(evs, ev0, model, precision, arr.ctypes.data_as(ctypes.c_void_p), n, _dp(out)))
+    return out
+
+
+def fisher_batch(event, params, model, basis, dim, precision=FP64, epsilon=1e-8, with_sigma=False):
+    arr = _as_params_array(params)
+    n = len(arr)
+    out = np.zeros(n * dim * dim)
+    sig = np.zeros(n * dim) if with_sigma else None
+    ev0 = event.id if isinstance(event, Event) else int(event)
+    check(lib().gwh_fisher_batch(ev0, model, basis, precision, arr.ctypes.data_as(ctypes.c_void_p), n, dim, epsilon,
+                                 _dp(out), _dp(sig) if sig is not None else None))
+    F = out.reshape(n, dim, dim)
+    return (F, sig.reshape(n, dim)) if with_sigma else F
+
+
+class DeviceArray:
+    """Raw device allocation owned by the engine's allocator (bench/test plumbing)."""
+
+    def __init__(self, nbytes):
+        self.ptr = lib().gwh_device_alloc(nbytes)
+        if not self.ptr:
+            raise GWHError("device allocation failed: " + lib().gwh_last_error().decode())
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_host(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.nbytes)
+        check(lib().gwh_memcpy_h2d(d.ptr, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes))
+        return d
+
+    def to_host(self, dtype, count):
+        out = np.zeros(count, dtype=dtype)
+        check(lib().gwh_memcpy_d2h(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(lib().gwh_device_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,194 @@
+/*
+ * gwat_hip.h -- C ABI of the MI355X engine for the gw_analysis_tools hot path:
+ * IMRPhenomD / IMRPhenomPv2 frequency-domain waveforms, projection onto
+ * terrestrial detectors, the Simpson-rule inner product behind the MCMC
+ * log-likelihood, and the central-difference numerical Fisher matrix.
+ *
+ * Plain C types only (pointers, sizes, POD structs); every entry point is
+ * thread-safe.  Status convention follows the reference: 1 (GWH_OK) means
+ * success; negative values are errors described by gwh_last_error().
+ * Functions returning a double return NaN on error.  NaN produced by the
+ * physics (eta > 1/4, f_ref = 0 with shift_phase, PSD = 0, ...) propagates
+ * into the result exactly as in the reference.
+ *
+ * Complex arrays are interleaved (re, im) doubles, matching
+ * std::complex<double> storage.  Units follow the reference: masses in solar
+ * masses (detector frame), distance in Mpc, times in seconds, angles in
+ * radians, frequencies in Hz.
+ */
+#ifndef GWAT_HIP_H
+#define GWAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GWH_OK 1
+#define GWH_ERR_INVALID (-1)
+#define GWH_ERR_HIP (-2)
+#define GWH_ERR_UNSUPPORTED (-3)
+#define GWH_ERR_NOMEM (-4)
+#define GWH_ERR_BUSY (-5)
+
+/* waveform models (the IMRPhenomD / IMRPhenomPv2 branches of
+ * fourier_waveform, src/waveform_generator.cpp:129-266) */
+#define GWH_IMRPHENOMD 0
+#define GWH_IMRPHENOMPV2 1
+
+/* arithmetic of the batched kernels */
+#define GWH_FP64 0  /* everything in FP64 */
+#define GWH_MIXED 1 /* FP64 phase and reductions, FP32 amplitude/projection/products */
+
+/* integration rules of Log_Likelihood_internal (mcmc_gw.cpp:801-868) */
+#define GWH_SIMPSONS 0
+#define GWH_GAUSSLEG 1
+
+/* detector ids (the terrestrial names accepted by detector_util.cpp:1083-1158) */
+#define GWH_HANFORD 0
+#define GWH_LIVINGSTON 1
+#define GWH_VIRGO 2
+#define GWH_KAGRA 3
+#define GWH_INDIGO 4
+#define GWH_CE 5
+#define GWH_ET1 6
+#define GWH_ET2 7
+#define GWH_ET3 8
+
+/* Fisher parameter bases (fisher.cpp:1841-2161 unpack_parameters) */
+#define GWH_BASIS_PHENOMD_SKYAVG 0   /* "IMRPhenomD", sky_average: (A0, phiRef, tc, Mc, eta, chi_s, chi_a), log on 0,3,4 */
+#define GWH_BASIS_PHENOMD_FULL 1     /* "IMRPhenomD": (RA, DEC, psi, incl, phiRef, tc, DL, Mc, eta, chi1, chi2), log on 6,7 */
+#define GWH_BASIS_PHENOMD_MCMC 2     /* "MCMC_IMRPhenomD": (RA, sinDEC, psi, cos incl, phiRef, tc, lnDL, lnMc, eta, chi1, chi2) */
+#define GWH_BASIS_PV2_FULL 3         /* "IMRPhenomPv2": (..., chi1z, chi2z, chip, phip), 13 parameters */
+#define GWH_BASIS_PV2_MCMC 4         /* "MCMC_IMRPhenomPv2": 15 parameters with spherical spins */
+
+/* Flat POD view of gen_params_base<double> (include/gwat/util.h:121-389):
+ * the fields the hot path reads. */
+typedef struct gwh_params {
+    double mass1, mass2;
+    double spin1[3], spin2[3];
+    double DL;
+    double tc;
+    double phiRef;
+    double f_ref;
+    double incl;
+    double RA, DEC, psi;
+    double gmst;
+    double chip, phip; /* chip <= -1 + 1e-10: precessing spins are taken from spin1/spin2 vectors */
+    int32_t shift_time, shift_phase, sky_average, dep_postmerger;
+} gwh_params;
+
+/* ---- runtime ---- */
+int gwh_init(int device);              /* idempotent; called implicitly by every entry point */
+const char *gwh_last_error(void);      /* message of this thread's last failure */
+int gwh_synchronize(void);
+const char *gwh_version(void);
+
+/* ---- single-call drop-in entry points (host buffers in and out) ---- */
+
+/* fourier_waveform(double*, int, waveform_polarizations<double>*, std::string, gen_params_base<double>*)
+ * -- include/gwat/waveform_generator.h:43-49 */
+int gwh_fourier_waveform(const double *frequencies, int length, double *hplus, double *hcross, int model,
+                         const gwh_params *params);
+
+/* fourier_detector_response(T*, int, std::complex<T>*, std::string det, std::string method,
+ * gen_params_base<T>*, T* times=NULL) -- include/gwat/waveform_util.h:472-479 (terrestrial,
+ * equatorial branch) */
+int gwh_fourier_detector_response(const double *frequencies, int length, double *response, int detector,
+                                  int model, const gwh_params *params);
+
+/* create_coherent_GW_detection(std::string* detectors, int, T** frequencies, int* lengths, bool reuse_WF,
+ * gen_params_base<T>*, std::string method, std::complex<T>** responses) -- include/gwat/waveform_util.h:297-315
+ * (reuse_WF path, waveform_util.cpp:152-184): one waveform, projected on every detector and shifted by
+ * the arrival-time difference to detectors[0].  responses is num_detectors*2*length (detector-major). */
+int gwh_create_coherent_GW_detection(const int *detectors, int num_detectors, const double *frequencies, int length,
+                                     int model, const gwh_params *params, double *responses);
+
+/* Log_Likelihood(std::complex<double>*, double* psd, double* freqs, size_t, gen_params_base<double>*,
+ * std::string det, std::string method) -- include/gwat/mcmc_gw.h:210-217 */
+double gwh_log_likelihood(const double *data, const double *psd, const double *frequencies, size_t length,
+                          const gwh_params *params, int detector, int model);
+
+/* Log_Likelihood_internal(std::complex<double>*, double*, double*, double* weights,
+ * std::complex<double>* response, int, bool log10F, std::string integration_method)
+ * -- include/gwat/mcmc_gw.h:276-284 */
+double gwh_log_likelihood_internal(const double *data, const double *psd, const double *frequencies,
+                                   const double *weights, const double *response, int length, int log10F,
+                                   int integration_method);
+
+/* MCMC_likelihood_extrinsic(bool, gen_params_base<double>*, std::string, int* data_length,
+ * double** frequencies, std::complex<double>** data, double** psd, double** weights,
+ * std::string integration_method, bool log10F, std::string* detectors, int num_detectors)
+ * -- include/gwat/mcmc_gw.h:473-485.  The reference derives T from a pointer difference
+ * (mcmc_gw.cpp:2466); here T_eff is explicit and the waveform uses tc_ref = T_eff - tc.
+ * All detectors must share one frequency grid (the reference's reuse_WF path). */
+double gwh_mcmc_likelihood_extrinsic(int save_waveform, const gwh_params *params, int model,
+                                     const int *data_length, const double *const *frequencies,
+                                     const double *const *data, const double *const *psd,
+                                     const double *const *weights, int integration_method, int log10F,
+                                     const int *detectors, int num_detectors, double T_eff);
+
+/* fisher_numerical(double*, int, string method, string det, string ref_det, double** out, int dim,
+ * gen_params_base<double>*, int order, int*, int*, double* noise) -- include/gwat/fisher.h:35-48.
+ * out is dim*dim row major; psd is the one-sided PSD on the same grid; epsilon is the
+ * absolute step of the unpacked basis (the reference uses 1e-8). */
+int gwh_fisher_numerical(const double *frequencies, int length, int basis, int detector, int reference_detector,
+                         double *out, int dim, const gwh_params *params, int order, const double *psd,
+                         double epsilon);
+
+/* ---- resident events and batches (the MCMC / serving path) ---- */
+
+/* Upload one event: shared frequency grid, per-detector complex data and PSD (host buffers,
+ * data is ndet*2*length, psd is ndet*length, detector-major).  Data stays resident on the GPU
+ * until gwh_event_destroy.  Returns an event id >= 0 in *event. */
+int gwh_event_create(const double *frequencies, int length, int ndet, const int *detectors, const double *data,
+                     const double *psd, int *event);
+int gwh_event_destroy(int event);
+size_t gwh_event_bytes(int event);
+
+/* Network log-likelihood (the MCMC_likelihood_extrinsic semantics, waveform tc used as given)
+ * for n instances.  events[i] selects the resident event of instance i (NULL: all use events0).
+ * Host buffers in and out. */
+int gwh_loglike_batch(const int *events, int events0, int model, int precision, const gwh_params *params, int n,
+                      double *logl);
+
+/* Same with device buffers already resident in HBM (d_params: n gwh_params, d_logl: n doubles),
+ * enqueued on the given HIP stream (NULL: the engine's stream) without a host synchronisation.
+ * d_events may be NULL (all instances use events0). */
+int gwh_loglike_batch_device(const int *d_events, int events0, int model, int precision, const void *d_params,
+                             int n, double *d_logl, void *stream);
+
+/* Tuning of the bins-per-lane of the likelihood kernels (1, 2, 4 or 8; 0 = automatic). */
+int gwh_set_bins_per_lane(int bpl);
+
+/* Batched network Fisher: n points, order 2 central differences in the given basis, per-detector
+ * projection summed over the event's detectors.  out: n*dim*dim (host).  precision GWH_FP64 uses an
+ * FP64 matrix-core Gram, GWH_MIXED an FP32 matrix-core Gram with FP64 accumulation across tiles;
+ * the stencil numerators are FP64 in both.  sigma (optional, n*dim): marginal standard deviations
+ * from the FP64 inverse of each Fisher matrix. */
+int gwh_fisher_batch(int event, int model, int basis, int precision, const gwh_params *params, int n, int dim,
+                     double epsilon, double *out, double *sigma);
+
+/* Device-resident timing helper for benchmarks: time `reps` launches of the batched likelihood on
+ * device buffers with HIP events on the engine stream; returns milliseconds per launch and the
+ * average duration of the dominant (per-bin) kernel in *kernel_ms. */
+int gwh_time_loglike_batch_device(int events0, int model, int precision, const void *d_params, int n,
+                                  double *d_logl, int reps, double *ms_per_launch, double *kernel_ms);
+
+/* Bins of every instance in each IMRPhenomD phase region on `event` (counts: n*4 ints: inspiral
+ * f < 0.018/M, intermediate, merger-ringdown up to 0.2/M, and zero bins above 0.2/M).  Used to
+ * price the algorithmic work of a batch. */
+int gwh_work_counts(int event, int model, const gwh_params *params, int n, int *counts);
+
+/* device memory helpers for callers without their own allocator */
+void *gwh_device_alloc(size_t bytes);
+int gwh_device_free(void *ptr);
+int gwh_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int gwh_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
