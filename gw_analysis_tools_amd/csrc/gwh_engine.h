@@ -31,6 +31,8 @@ struct EventDesc {
     const double *chunk_fmin; /* ceil(N / GWH_TPB): smallest frequency of every 256-bin chunk */
     double sw[GWH_MAXDET], sd[GWH_MAXDET];
     int N, D, nchunk, sorted; /* sorted: frequencies ascending */
+    int bpl;                  /* bins per lane of this event's tiles (fixed per event: results never depend
+                                 on which other events share a batch) */
     int det[GWH_MAXDET];
 };
 
@@ -73,6 +75,10 @@ struct Engine {
     DevBuf params, coef, partial, active, logl, inst_event, f, data, psd, wts, out1, out2, misc;
     DevBuf fisher_deriv, fisher_gram, fisher_out;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    /* live timing of the per-bin kernel inside a caller's timed region: event pairs around each launch */
+    std::vector<hipEvent_t> kt_ev;
+    int kt_used = 0;
+    bool kt_on = false;
 };
 
 Engine &gwh_engine();

@@ -311,6 +311,7 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     ds.N = N;
     ds.D = ndet;
     ds.nchunk = nchunk;
+    ds.bpl = gwh_choose_bpl(N);
     ds.sorted = 1;
     for (int k = 1; k < N; k++)
         if (!(f[k] >= f[k - 1])) {
@@ -336,7 +337,7 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
 __global__ void gwh_setup_kernel(const gwh_params *__restrict__ params, const int *__restrict__ inst_event,
                                  int event0, int n, int model, const double *__restrict__ fits, QnmTable qnm,
                                  const DetGeom *__restrict__ geo, const EventDesc *__restrict__ events,
-                                 InstCoef *__restrict__ coef, int tile, int *__restrict__ active)
+                                 InstCoef *__restrict__ coef, int *__restrict__ active)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -348,8 +349,9 @@ __global__ void gwh_setup_kernel(const gwh_params *__restrict__ params, const in
     coef[i] = C;
     if (active) {
         /* tiles to visit: all of them, or on an ascending grid those starting at or below 0.2/M */
+        const int tile = GWH_TPB * E.bpl;
         int nt = (E.N + tile - 1) / tile;
-        int cpt = tile / GWH_TPB; /* 256-bin chunks per tile */
+        int cpt = E.bpl; /* 256-bin chunks per tile */
         if (E.sorted) {
             int lo = 0, hi = nt; /* first tile whose smallest frequency exceeds the cutoff */
             while (lo < hi) {
@@ -371,7 +373,7 @@ int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_para
     Engine &E = gwh_engine();
     hipLaunchKernelGGL(gwh_setup_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_params, d_events, events0, n,
                        model, (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo,
-                       (const EventDesc *)E.d_events, d_coef, GWH_TPB, (int *)nullptr);
+                       (const EventDesc *)E.d_events, d_coef, (int *)nullptr);
     GWH_HIP(hipGetLastError());
     return GWH_OK;
 }
@@ -534,10 +536,11 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_loglike_kernel(WorkMap W, const E
     {
         const int inst = blockIdx.y;
         const int tile = blockIdx.x;
-        if (tile >= active[inst]) return; /* above 0.2/M on an ascending grid: h = 0, nothing to add */
-        const int w = inst * W.tiles0 + tile;
         const int ev = W.inst_event ? W.inst_event[inst] : W.event0;
         const EventDesc &E = events[ev];
+        if (E.bpl != BPL) return;           /* the instance's event is tiled by another launch */
+        if (tile >= active[inst]) return; /* above 0.2/M on an ascending grid: h = 0, nothing to add */
+        const int w = inst * W.tiles0 + tile;
         const InstCoef &C = coefs[inst];
         const int N = E.N, D = E.D;
         double acc[GWH_PARTIAL_STRIDE];
@@ -752,18 +755,23 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     if (events0 < 0 || events0 >= GWH_MAX_EVENTS || (!d_events && !E.events[events0].live))
         return gwh_set_error(GWH_ERR_INVALID, "unknown event " + std::to_string(events0));
     if (n > 65535) return gwh_set_error(GWH_ERR_INVALID, "at most 65535 instances per launch");
-    /* bins per lane from the largest event involved: the tiling of an instance depends only on the
-     * registered events, never on the batch composition (bitwise-reproducible results) */
-    int maxN = 0;
+    /* each event carries its own bins per lane; one launch per tiling present (instances of other
+     * tilings leave at once), so an instance's partial sums never depend on the batch composition */
+    int tiles_b[4] = {0, 0, 0, 0}; /* bpl 1, 2, 4, 8 */
+    auto slot_of = [](int b) { return b == 1 ? 0 : b == 2 ? 1 : b == 4 ? 2 : 3; };
+    auto note = [&](const Event &ev) {
+        int b = ev.desc.bpl, t = (ev.N + GWH_TPB * b - 1) / (GWH_TPB * b);
+        int sl = slot_of(b);
+        if (t > tiles_b[sl]) tiles_b[sl] = t;
+    };
     if (d_events) {
         for (auto &ev : E.events)
-            if (ev.live && ev.N > maxN) maxN = ev.N;
+            if (ev.live) note(ev);
     } else {
-        maxN = E.events[events0].N;
+        note(E.events[events0]);
     }
-    int bpl = gwh_choose_bpl(maxN);
-    int tile = GWH_TPB * bpl;
-    int maxtiles = (maxN + tile - 1) / tile;
+    int maxtiles = 0;
+    for (int b = 0; b < 4; b++) maxtiles = std::max(maxtiles, tiles_b[b]);
     int rc;
     if ((rc = E.coef.ensure((size_t)n * sizeof(InstCoef))) != GWH_OK) return rc;
     if ((rc = E.partial.ensure((size_t)n * maxtiles * GWH_PARTIAL_STRIDE * sizeof(double))) != GWH_OK) return rc;
@@ -775,27 +783,35 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     W.tiles0 = maxtiles;
     hipLaunchKernelGGL(gwh_setup_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_params, d_events, events0, n, model,
                        (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
-                       E.coef.as<InstCoef>(), tile, E.active.as<int>());
+                       E.coef.as<InstCoef>(), E.active.as<int>());
     GWH_HIP(hipGetLastError());
     if (timed) GWH_HIP(hipEventRecord(E.ev0, stream));
+    const bool live = E.kt_on && 2 * (size_t)E.kt_used + 1 < E.kt_ev.size();
+    if (live) GWH_HIP(hipEventRecord(E.kt_ev[2 * E.kt_used], stream));
     const EventDesc *evp = (const EventDesc *)E.d_events;
     const InstCoef *cp = E.coef.as<const InstCoef>();
     double *pp = E.partial.as<double>();
     const int *ap = E.active.as<const int>();
-    dim3 grid(maxtiles, n);
-    if (model == GWH_IMRPHENOMD) {
-        if (precision == GWH_FP64)
-            launch_bpl<GWH_IMRPHENOMD, GWH_FP64>(bpl, grid, stream, W, evp, cp, pp, ap);
-        else
-            launch_bpl<GWH_IMRPHENOMD, GWH_MIXED>(bpl, grid, stream, W, evp, cp, pp, ap);
-    } else {
-        if (precision == GWH_FP64)
-            launch_bpl<GWH_IMRPHENOMPV2, GWH_FP64>(bpl, grid, stream, W, evp, cp, pp, ap);
-        else
-            launch_bpl<GWH_IMRPHENOMPV2, GWH_MIXED>(bpl, grid, stream, W, evp, cp, pp, ap);
+    static const int kB[4] = {1, 2, 4, 8};
+    for (int sl = 0; sl < 4; sl++) {
+        if (!tiles_b[sl]) continue;
+        dim3 grid(tiles_b[sl], n);
+        int bpl = kB[sl];
+        if (model == GWH_IMRPHENOMD) {
+            if (precision == GWH_FP64)
+                launch_bpl<GWH_IMRPHENOMD, GWH_FP64>(bpl, grid, stream, W, evp, cp, pp, ap);
+            else
+                launch_bpl<GWH_IMRPHENOMD, GWH_MIXED>(bpl, grid, stream, W, evp, cp, pp, ap);
+        } else {
+            if (precision == GWH_FP64)
+                launch_bpl<GWH_IMRPHENOMPV2, GWH_FP64>(bpl, grid, stream, W, evp, cp, pp, ap);
+            else
+                launch_bpl<GWH_IMRPHENOMPV2, GWH_MIXED>(bpl, grid, stream, W, evp, cp, pp, ap);
+        }
     }
     GWH_HIP(hipGetLastError());
     if (timed) GWH_HIP(hipEventRecord(E.ev1, stream));
+    if (live) GWH_HIP(hipEventRecord(E.kt_ev[2 * E.kt_used++ + 1], stream));
     hipLaunchKernelGGL(gwh_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, W, evp, cp,
                        (const double *)pp, ap, model, precision, d_logl);
     GWH_HIP(hipGetLastError());
@@ -943,7 +959,17 @@ int gwh_set_bins_per_lane(int bpl)
         return gwh_set_error(GWH_ERR_INVALID, "bins per lane must be 0, 1, 2, 4 or 8");
     Engine &E = gwh_engine();
     std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
     E.bpl = bpl;
+    /* re-tile the resident events (later results may differ in the last bits from earlier ones) */
+    for (int i = 0; i < GWH_MAX_EVENTS; i++) {
+        Event &ev = E.events[i];
+        if (!ev.live) continue;
+        ev.desc.bpl = gwh_choose_bpl(ev.N);
+        GWH_HIP(hipMemcpyAsync(E.d_events + i, &ev.desc, sizeof(EventDesc), hipMemcpyHostToDevice, E.stream));
+    }
+    GWH_HIP(hipStreamSynchronize(E.stream));
     return GWH_OK;
 }
 
@@ -1092,6 +1118,63 @@ int gwh_time_loglike_batch_device(int events0, int model, int precision, const v
     return GWH_OK;
 }
 
+int gwh_kernel_timing_start(int max_launches)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    if (max_launches < 1 || max_launches > (1 << 20)) return gwh_set_error(GWH_ERR_INVALID, "bad max_launches");
+    while (E.kt_ev.size() < 2 * (size_t)max_launches) {
+        hipEvent_t e;
+        GWH_HIP(hipEventCreate(&e));
+        E.kt_ev.push_back(e);
+    }
+    E.kt_used = 0;
+    E.kt_on = true;
+    return GWH_OK;
+}
+
+int gwh_kernel_timing_stop(double *total_ms, int *launches)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    if (!E.kt_on) return gwh_set_error(GWH_ERR_INVALID, "kernel timing was not started");
+    E.kt_on = false;
+    double sum = 0;
+    for (int i = 0; i < E.kt_used; i++) {
+        GWH_HIP(hipEventSynchronize(E.kt_ev[2 * i + 1]));
+        float ms = 0;
+        GWH_HIP(hipEventElapsedTime(&ms, E.kt_ev[2 * i], E.kt_ev[2 * i + 1]));
+        sum += ms;
+    }
+    if (total_ms) *total_ms = sum;
+    if (launches) *launches = E.kt_used;
+    return GWH_OK;
+}
+
+__global__ void gwh_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int gwh_device_copy(void *dst, const void *src, size_t bytes)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    if (!dst || !src || (bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return gwh_set_error(GWH_ERR_INVALID, "copy needs 16-byte aligned pointers and a multiple of 16 bytes");
+    size_t n = bytes / 16;
+    if (n == 0) return GWH_OK;
+    hipLaunchKernelGGL(gwh_copy_kernel, dim3(4096), dim3(256), 0, E.stream, (const uint4 *)src, (uint4 *)dst, n);
+    GWH_HIP(hipGetLastError());
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
+}
+
 /* ---- single-call drop-in entry points ---- */
 
 static int upload_single(const double *f, int n, const gwh_params *p, int model, int det, const int *dets = nullptr,
@@ -1112,11 +1195,12 @@ static int upload_single(const double *f, int n, const gwh_params *p, int model,
     memset(&ds, 0, sizeof(ds));
     ds.D = ndet;
     ds.N = n;
+    ds.bpl = 1;
     for (int d = 0; d < GWH_MAXDET; d++) ds.det[d] = (dets && d < ndet) ? dets[d] : det;
     GWH_HIP(hipMemcpyAsync(E.d_events + (GWH_MAX_EVENTS - 1), &ds, sizeof(ds), hipMemcpyHostToDevice, E.stream));
     hipLaunchKernelGGL(gwh_setup_kernel, dim3(1), dim3(64), 0, E.stream, E.params.as<const gwh_params>(),
                        (const int *)nullptr, GWH_MAX_EVENTS - 1, 1, model, (const double *)E.d_fits, E.qnm,
-                       (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events, E.coef.as<InstCoef>(), GWH_TPB,
+                       (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events, E.coef.as<InstCoef>(),
                        (int *)nullptr);
     GWH_HIP(hipGetLastError());
     return GWH_OK;

@@ -263,6 +263,8 @@ struct StencilArgs {
     int d_first, d_count;        /* detectors of the event accumulated */
     int skyavg;                  /* sky-averaged IMRPhenomD: amplitude / phase derivatives */
     double sa_sign;              /* +1 (order 2) or -1 (order 4): i d(phase) A sign of fisher.cpp:299-315 */
+    double *raw;                 /* non-null: write the bare derivatives dim x N complex (one point, one
+                                    detector) instead of weighted Gram operand rows */
 };
 
 template <int MODEL, typename T>
@@ -298,6 +300,13 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
                     dA += cw[i * GWH_MAXSTEN + s] * A;
                     dP += cw[i * GWH_MAXSTEN + s] * (-phi); /* construct_phase returns -(Psi - tc (f-fref) - phic) */
                 }
+            }
+            if (a.raw) {
+                if (i < a.dim) {
+                    a.raw[((size_t)i * N + k) * 2] = dA;
+                    a.raw[((size_t)i * N + k) * 2 + 1] = a.sa_sign * dP * Ac;
+                }
+                continue;
             }
             r0[i] = (T)(dA * sw);
             r0[GWH_FROWS + i] = (T)(a.sa_sign * dP * Ac * sw);
@@ -341,6 +350,13 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
                     }
                 }
             }
+        }
+        if (a.raw) {
+            if (i < a.dim) {
+                a.raw[((size_t)i * N + k) * 2] = dr[0];
+                a.raw[((size_t)i * N + k) * 2 + 1] = di[0];
+            }
+            continue;
         }
         for (int dd = 0; dd < GWH_MAXDET; dd++) {
             if (dd >= a.d_count) break;
@@ -387,8 +403,12 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_gram_kernel(const T *__restrict__
         }
         for (int j = 0; j < 4; j++) out[j] = acc[j];
     }
-    /* lane l, register j holds element (4 (l/16) + j, l % 16) of the 16x16 result */
-    for (int j = 0; j < 4; j++) lds[wv][(4 * (lane >> 4) + j) * 16 + (lane & 15)] = out[j];
+    /* result layout on gfx950 (checked on the device): column l % 16; row l / 16 + 4 j for the FP64
+     * instruction, 4 (l / 16) + j for the FP32 one */
+    for (int j = 0; j < 4; j++) {
+        int row = (sizeof(T) == 8) ? ((lane >> 4) + 4 * j) : (4 * (lane >> 4) + j);
+        lds[wv][row * 16 + (lane & 15)] = out[j];
+    }
     __syncthreads();
     for (int e = threadIdx.x; e < 256; e += GWH_TPB) {
         double s = lds[0][e];
@@ -481,7 +501,7 @@ __global__ void gwh_fisher_invert_kernel(const double *__restrict__ F, int n, in
  * responses enter (network sum = all); ref_det is the arrival-time reference. */
 static int fisher_run(int event, int model, int basis, int precision, const gwh_params *params, int n, int order,
                       double eps, int d_first, int d_count, int det_for_tof, int ref_det, double scale, double *out,
-                      double *sigma)
+                      double *sigma, double *raw = nullptr)
 {
     Engine &E = gwh_engine();
     const Event &ev = E.events[event];
@@ -543,8 +563,26 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
         a.d_count = sky ? 1 : d_count;
         a.skyavg = sky;
         a.sa_sign = (S == 4) ? -1.0 : 1.0;
+        a.raw = nullptr;
         dim3 g((N + GWH_TPB - 1) / GWH_TPB, np);
         const EventDesc *evp = (const EventDesc *)E.d_events;
+        if (raw) {
+            /* bare derivatives of one point on one detector (calculate_derivatives) */
+            if ((rc = E.out1.ensure((size_t)dim * N * 16)) != GWH_OK) return rc;
+            a.raw = E.out1.as<double>();
+            a.d_count = 1;
+            dim3 g1((N + GWH_TPB - 1) / GWH_TPB, 1);
+            if (model == GWH_IMRPHENOMD)
+                hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, double>), g1, dim3(GWH_TPB), 0,
+                                   E.stream, a, evp, event, E.fisher_deriv.as<double>());
+            else
+                hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMPV2, double>), g1, dim3(GWH_TPB), 0,
+                                   E.stream, a, evp, event, E.fisher_deriv.as<double>());
+            GWH_HIP(hipGetLastError());
+            GWH_HIP(hipMemcpyAsync(raw, E.out1.p, (size_t)dim * N * 16, hipMemcpyDeviceToHost, E.stream));
+            GWH_HIP(hipStreamSynchronize(E.stream));
+            return GWH_OK;
+        }
         if (precision == GWH_FP64) {
             if (model == GWH_IMRPHENOMD)
                 hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, double>), g, dim3(GWH_TPB), 0, E.stream,
@@ -570,6 +608,8 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
         hipLaunchKernelGGL(gwh_gram_reduce_kernel, dim3(np), dim3(256), 0, E.stream, E.fisher_gram.as<const double>(),
                            nblk, dim, scale, E.fisher_out.as<double>() + (size_t)p0 * dim * dim);
         GWH_HIP(hipGetLastError());
+        /* the host staging vectors of this chunk die with the iteration */
+        GWH_HIP(hipStreamSynchronize(E.stream));
     }
     if (sigma) {
         if ((rc = E.out2.ensure((size_t)n * dim * 8)) != GWH_OK) return rc;
@@ -585,14 +625,94 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
 
 extern "C" {
 
-int gwh_fisher_numerical(const double *frequencies, int length, int basis, int detector, int reference_detector,
-                         double *out, int dim, const gwh_params *params, int order, const double *psd, double epsilon)
+/* detector_util.cpp:573-636 name matching (exact strings only; anything else is unsupported) */
+int gwh_detector_id(const char *name)
+{
+    if (!name) return -1;
+    static const struct {
+        const char *s;
+        int id;
+    } names[] = {{"Hanford", GWH_HANFORD},
+                 {"hanford", GWH_HANFORD},
+                 {"Livingston", GWH_LIVINGSTON},
+                 {"livingston", GWH_LIVINGSTON},
+                 {"Virgo", GWH_VIRGO},
+                 {"virgo", GWH_VIRGO},
+                 {"Kagra", GWH_KAGRA},
+                 {"kagra", GWH_KAGRA},
+                 {"Indigo", GWH_INDIGO},
+                 {"indigo", GWH_INDIGO},
+                 {"Cosmic Explorer", GWH_CE},
+                 {"cosmic explorer", GWH_CE},
+                 {"CE", GWH_CE},
+                 {"Einstein Telescope 1", GWH_ET1},
+                 {"einstein telescope 1", GWH_ET1},
+                 {"ET1", GWH_ET1},
+                 {"Einstein Telescope 2", GWH_ET2},
+                 {"einstein telescope 2", GWH_ET2},
+                 {"ET2", GWH_ET2},
+                 {"Einstein Telescope 3", GWH_ET3},
+                 {"einstein telescope 3", GWH_ET3},
+                 {"ET3", GWH_ET3}};
+    for (auto &n : names)
+        if (!strcmp(n.s, name)) return n.id;
+    return -1;
+}
+
+double gwh_dtoa(double RA, double DEC, double gmst, int detector1, int detector2)
+{
+    if (detector1 < 0 || detector1 >= GWH_NDET_TABLE || detector2 < 0 || detector2 >= GWH_NDET_TABLE) {
+        gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported detector");
+        return __builtin_nan("");
+    }
+    return host_dtoa(RA, DEC, gmst, detector1, detector2);
+}
+
+/* util.cpp:1797-1846: GPS seconds -> Greenwich mean sidereal time in radians (the reference's
+ * low-order series in Julian days from J2000, hours wrapped to [0, 24)) */
+double gwh_gps_to_gmst_radian(double gps)
+{
+    const double jd_j2000 = 2451545.0, gps_at_j2000 = 630763213.0;
+    double jd = jd_j2000 + (gps - gps_at_j2000) / 86400.0;
+    double jd0 = floor(jd) + 0.5;
+    double hours_ut = (jd - jd0) * 24.0;
+    double d0 = jd0 - jd_j2000, centuries = (jd - jd_j2000) / 36525.0;
+    double g = 6.697374558 + 0.06570982441908 * d0 + 1.00273790935 * hours_ut + 0.000026 * centuries * centuries;
+    double whole = (double)((long long)floor(g) % 24);
+    return (whole + (g - floor(g))) * M_PI / 12.0;
+}
+
+int gwh_unpack_parameters(const gwh_params *params, int basis, double *v, int *log_factors)
+{
+    if (!params || !v) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    int dim = basis_dim(basis);
+    if (dim < 0) return gwh_set_error(GWH_ERR_INVALID, "unknown basis");
+    int lf[GWH_FROWS];
+    unpack(*params, basis, v, lf);
+    if (log_factors)
+        for (int i = 0; i < dim; i++) log_factors[i] = lf[i];
+    return GWH_OK;
+}
+
+int gwh_repack_parameters(const double *v, int basis, gwh_params *params)
+{
+    if (!params || !v) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    if (basis_dim(basis) < 0) return gwh_set_error(GWH_ERR_INVALID, "unknown basis");
+    repack(v, basis, *params);
+    return GWH_OK;
+}
+
+/* one-point Fisher on a temporary event holding the PSD for (reference detector, detector): the
+ * time-of-flight phase of the second is relative to the first; the weights use df = f[1]-f[0]
+ * (fisher.cpp:2746).  deriv != null: return the bare derivatives instead (calculate_derivatives). */
+static int single_point(const double *frequencies, int length, int basis, int detector, int reference_detector,
+                        double *out, int dim, const gwh_params *params, int order, const double *psd, double epsilon,
+                        double *deriv)
 {
     Engine &E = gwh_engine();
-    std::lock_guard<std::recursive_mutex> lk(E.mu);
     int rc = gwh_ensure_init();
     if (rc != GWH_OK) return rc;
-    if (!frequencies || !out || !params || !psd || length < 2)
+    if (!frequencies || !(out || deriv) || !params || length < 2 || (!psd && !deriv))
         return gwh_set_error(GWH_ERR_INVALID, "null argument or length < 2");
     if (basis_dim(basis) < 0 || basis_dim(basis) != dim)
         return gwh_set_error(GWH_ERR_INVALID, "dimension does not match the parameter basis");
@@ -604,23 +724,42 @@ int gwh_fisher_numerical(const double *frequencies, int length, int basis, int d
         return gwh_set_error(GWH_ERR_INVALID, "the sky-averaged basis needs params->sky_average");
     if (basis != GWH_BASIS_PHENOMD_SKYAVG && params->sky_average)
         return gwh_set_error(GWH_ERR_UNSUPPORTED, "sky_average selects the sky-averaged IMRPhenomD basis only");
-    /* an event holding the PSD twice (reference detector, detector): the time-of-flight phase of the
-     * second one is relative to the first; Fisher weights use df = f[1]-f[0] (fisher.cpp:2746) */
     int dets[2] = {reference_detector, detector};
     int nd = (detector != reference_detector) ? 2 : 1;
     if (nd == 1) dets[0] = detector;
-    std::vector<double> zeros((size_t)2 * length * nd, 0.0), ps((size_t)length * nd);
-    for (int d = 0; d < nd; d++) memcpy(&ps[(size_t)d * length], psd, (size_t)length * 8);
+    std::vector<double> zeros((size_t)2 * length * nd, 0.0), ps((size_t)length * nd, 1.0);
+    if (psd)
+        for (int d = 0; d < nd; d++) memcpy(&ps[(size_t)d * length], psd, (size_t)length * 8);
     int ev;
     if ((rc = gwh_event_create_ex(frequencies, length, nd, dets, zeros.data(), ps.data(), GWH_SIMPSONS, nullptr, 0, 1,
                                   0, &ev)) != GWH_OK)
         return rc;
     rc = fisher_run(ev, model, basis, GWH_FP64, params, 1, order, epsilon, nd - 1, 1, detector, reference_detector,
-                    1.0, out, nullptr);
+                    1.0, out, nullptr, deriv);
     (void)hipStreamSynchronize(E.stream);
     (void)hipFree(E.events[ev].block);
     E.events[ev] = Event();
     return rc;
+}
+
+int gwh_fisher_derivatives(const double *frequencies, int length, int basis, int detector, int reference_detector,
+                           int dim, const gwh_params *params, int order, double epsilon, double *deriv)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    if (!deriv) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    return single_point(frequencies, length, basis, detector, reference_detector, nullptr, dim, params, order,
+                        nullptr, epsilon, deriv);
+}
+
+int gwh_fisher_numerical(const double *frequencies, int length, int basis, int detector, int reference_detector,
+                         double *out, int dim, const gwh_params *params, int order, const double *psd, double epsilon)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    if (!out || !psd) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    return single_point(frequencies, length, basis, detector, reference_detector, out, dim, params, order, psd,
+                        epsilon, nullptr);
 }
 
 int gwh_fisher_batch(int event, int model, int basis, int precision, const gwh_params *params, int n, int dim,

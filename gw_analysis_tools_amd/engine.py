@@ -11,9 +11,11 @@ import numpy as np
 
 from . import GWHError, Params, check, lib, PARAMS_DTYPE, IMRPHENOMD, FP64
 
-__all__ = ["fourier_waveform", "fourier_detector_response", "create_coherent_GW_detection", "work_counts", "log_likelihood", "log_likelihood_internal",
-           "mcmc_likelihood_extrinsic", "fisher_numerical", "Event", "loglike_batch", "fisher_batch",
-           "DeviceArray", "set_bins_per_lane", "synchronize"]
+__all__ = ["fourier_waveform", "fourier_detector_response", "create_coherent_GW_detection", "work_counts",
+           "log_likelihood", "log_likelihood_internal", "mcmc_likelihood_extrinsic", "fisher_numerical",
+           "fisher_derivatives", "unpack_parameters", "repack_parameters", "detector_id", "dtoa",
+           "gps_to_gmst_radian", "Event", "loglike_batch", "fisher_batch", "DeviceArray", "set_bins_per_lane",
+           "synchronize"]
 
 
 def _dp(a):
@@ -123,6 +125,42 @@ def fisher_numerical(f, psd, basis, det, ref_det, dim, p, order=2, epsilon=1e-8)
     check(lib().gwh_fisher_numerical(_dp(f), len(f), basis, det, ref_det, _dp(out), dim, ctypes.byref(p), order,
                                      _dp(s), epsilon))
     return out.reshape(dim, dim)
+
+
+def fisher_derivatives(f, basis, det, ref_det, dim, p, order=2, epsilon=1e-8):
+    """bare derivatives dh/dtheta_i (dim x len(f) complex), as calculate_derivatives returns them"""
+    f = _f64(f)
+    out = np.zeros(2 * dim * len(f))
+    check(lib().gwh_fisher_derivatives(_dp(f), len(f), basis, det, ref_det, dim, ctypes.byref(p), order, epsilon,
+                                       _dp(out)))
+    return _cplx(out).reshape(dim, len(f))
+
+
+def unpack_parameters(p, basis, dim):
+    v = np.zeros(dim)
+    lf = (ctypes.c_int * dim)()
+    check(lib().gwh_unpack_parameters(ctypes.byref(p), basis, _dp(v), lf))
+    return v, np.array(list(lf), dtype=bool)
+
+
+def repack_parameters(v, basis, template):
+    out = Params()
+    ctypes.pointer(out)[0] = template
+    v = _f64(v)
+    check(lib().gwh_repack_parameters(_dp(v), basis, ctypes.byref(out)))
+    return out
+
+
+def detector_id(name):
+    return lib().gwh_detector_id(name.encode())
+
+
+def dtoa(ra, dec, gmst, det1, det2):
+    return _nan_check(lib().gwh_dtoa(ra, dec, gmst, det1, det2))
+
+
+def gps_to_gmst_radian(gps):
+    return lib().gwh_gps_to_gmst_radian(gps)
 
 
 class Event:

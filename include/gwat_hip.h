@@ -138,6 +138,25 @@ int gwh_fisher_numerical(const double *frequencies, int length, int basis, int d
                          double *out, int dim, const gwh_params *params, int order, const double *psd,
                          double epsilon);
 
+/* calculate_derivatives(double **derivatives, double *frequencies, int length, int dimension,
+ * std::string detector, std::string reference_detector, std::string gen_method,
+ * gen_params_base<double> *parameters, int order) -- src/fisher.cpp:160-557.
+ * deriv: dim x length complex (interleaved re, im), parameter-major, with the log factors applied. */
+int gwh_fisher_derivatives(const double *frequencies, int length, int basis, int detector, int reference_detector,
+                           int dim, const gwh_params *params, int order, double epsilon, double *deriv);
+
+/* unpack_parameters / repack_parameters (src/fisher.cpp:1841-2161, :2162-2511): the basis vector of a
+ * parameter set and back.  log_factors (optional, dim ints) marks the parameters differentiated in log. */
+int gwh_unpack_parameters(const gwh_params *params, int basis, double *v, int *log_factors);
+int gwh_repack_parameters(const double *v, int basis, gwh_params *params);
+
+/* Detector helpers: the name lookup of src/detector_util.cpp:573-636 (-1 if unknown), DTOA_DETECTOR
+ * (src/detector_util.cpp:676-790; seconds, detector1 minus detector2) and gps_to_GMST_radian
+ * (src/util.cpp:1797-1846). */
+int gwh_detector_id(const char *name);
+double gwh_dtoa(double RA, double DEC, double gmst, int detector1, int detector2);
+double gwh_gps_to_gmst_radian(double gps);
+
 /* ---- resident events and batches (the MCMC / serving path) ---- */
 
 /* Upload one event: shared frequency grid, per-detector complex data and PSD (host buffers,
@@ -187,6 +206,15 @@ void *gwh_device_alloc(size_t bytes);
 int gwh_device_free(void *ptr);
 int gwh_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int gwh_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+/* Measurement helpers (no counterpart in the reference).  Between start and stop, every batched
+ * likelihood launch records a HIP event pair around its per-bin kernel on the launch stream (at most
+ * max_launches pairs); stop synchronises and returns the summed kernel time and the launch count. */
+int gwh_kernel_timing_start(int max_launches);
+int gwh_kernel_timing_stop(double *total_ms, int *launches);
+/* A plain 16-byte-vector copy kernel of `bytes` (device to device, synchronous), the reference point
+ * for scaling hardware traffic counters. */
+int gwh_device_copy(void *dst, const void *src, size_t bytes);
 
 #ifdef __cplusplus
 }
