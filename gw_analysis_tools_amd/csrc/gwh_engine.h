@@ -29,12 +29,25 @@ struct EventDesc {
     const float *wS32;      /* D x N : wS * sw[d] */
     const float2 *dwS32;    /* D x N : dwS * sd[d] */
     const double *chunk_fmin; /* ceil(N / GWH_TPB): smallest frequency of every 256-bin chunk */
+    /* N each: log f, f^(1/3), f^(-1/3), f^(-7/6), f^(3/4) of every bin (gwh_bin_powers) */
+    const double *lgf, *f13, *fm13, *fm76, *f34;
     double sw[GWH_MAXDET], sd[GWH_MAXDET];
     int N, D, nchunk, sorted; /* sorted: frequencies ascending */
     int bpl;                  /* bins per lane of this event's tiles (fixed per event: results never depend
                                  on which other events share a batch) */
     int det[GWH_MAXDET];
 };
+
+__device__ __forceinline__ BinPow gwh_load_bin(const EventDesc &E, int k)
+{
+    BinPow b;
+    b.lf = E.lgf[k];
+    b.f13 = E.f13[k];
+    b.fm13 = E.fm13[k];
+    b.fm76 = E.fm76[k];
+    b.f34 = E.f34[k];
+    return b;
+}
 
 struct Event {
     bool live = false;

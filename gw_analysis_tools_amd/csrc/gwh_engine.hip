@@ -166,6 +166,19 @@ __device__ __forceinline__ double simpson_w(int k, int n)
     return (k & 1) ? 4.0 : 2.0;
 }
 
+/* the tabulated powers of every bin (five arrays of `stride` doubles) */
+__global__ void gwh_bin_powers_kernel(const double *__restrict__ f, int N, double *__restrict__ out, size_t stride)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    BinPow b = gwh_bin_powers(f[k]);
+    out[k] = b.lf;
+    out[stride + k] = b.f13;
+    out[2 * stride + k] = b.fm13;
+    out[3 * stride + k] = b.fm76;
+    out[4 * stride + k] = b.f34;
+}
+
 __global__ void gwh_event_prep_kernel(const double *__restrict__ f, const double *__restrict__ psd,
                                       const double2 *__restrict__ data, const double *__restrict__ wts, int N, int D,
                                       int method, int log10F, double dfac, double *__restrict__ wS,
@@ -240,7 +253,9 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     size_t Nd = (size_t)N * ndet;
     size_t off_f = 0, off_wS = align256(off_f + N * 8), off_dwS = align256(off_wS + Nd * 8),
            off_w32 = align256(off_dwS + Nd * 16), off_dw32 = align256(off_w32 + Nd * 4),
-           off_cm = align256(off_dw32 + Nd * 8), total = align256(off_cm + nchunk * 8);
+           off_cm = align256(off_dw32 + Nd * 8), off_pw = align256(off_cm + nchunk * 8),
+           total = align256(off_pw + 5 * align256((size_t)N * 8));
+    const size_t pw_stride = align256((size_t)N * 8);
     char *blk = nullptr;
     if (hipMalloc((void **)&blk, total) != hipSuccess) return gwh_set_error(GWH_ERR_NOMEM, "event allocation failed");
     /* staging: psd, data, weights */
@@ -288,6 +303,9 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     hipLaunchKernelGGL(gwh_chunk_fmin_kernel, dim3(nchunk), dim3(GWH_TPB), 0, E.stream,
                        (const double *)(blk + off_f), N, (double *)(blk + off_cm));
     GWH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gwh_bin_powers_kernel, dim3(nchunk), dim3(GWH_TPB), 0, E.stream,
+                       (const double *)(blk + off_f), N, (double *)(blk + off_pw), pw_stride / 8);
+    GWH_HIP(hipGetLastError());
     Event &ev = E.events[slot];
     ev.live = true;
     ev.N = N;
@@ -308,6 +326,11 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     ds.wS32 = (const float *)(blk + off_w32);
     ds.dwS32 = (const float2 *)(blk + off_dw32);
     ds.chunk_fmin = (const double *)(blk + off_cm);
+    ds.lgf = (const double *)(blk + off_pw);
+    ds.f13 = (const double *)(blk + off_pw + pw_stride);
+    ds.fm13 = (const double *)(blk + off_pw + 2 * pw_stride);
+    ds.fm76 = (const double *)(blk + off_pw + 3 * pw_stride);
+    ds.f34 = (const double *)(blk + off_pw + 4 * pw_stride);
     ds.N = N;
     ds.D = ndet;
     ds.nchunk = nchunk;
@@ -334,11 +357,17 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
 /* setup                                                                      */
 /* ------------------------------------------------------------------------- */
 
-__global__ void gwh_setup_kernel(const gwh_params *__restrict__ params, const int *__restrict__ inst_event,
-                                 int event0, int n, int model, const double *__restrict__ fits, QnmTable qnm,
-                                 const DetGeom *__restrict__ geo, const EventDesc *__restrict__ events,
-                                 InstCoef *__restrict__ coef, int *__restrict__ active)
+/* One lane per instance: a long serial FP64 chain per lane, so the launch bound of 64 lets the
+ * compiler keep the whole working set in registers instead of spilling it to scratch memory. */
+template <int MODEL>
+__global__ __launch_bounds__(64) void gwh_setup_kernel(const gwh_params *__restrict__ params,
+                                                       const int *__restrict__ inst_event, int event0, int n,
+                                                       const double *__restrict__ fits, QnmTable qnm,
+                                                       const DetGeom *__restrict__ geo,
+                                                       const EventDesc *__restrict__ events,
+                                                       InstCoef *__restrict__ coef, int *__restrict__ active)
 {
+    const int model = MODEL;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int ev = inst_event ? inst_event[i] : event0;
@@ -367,13 +396,26 @@ __global__ void gwh_setup_kernel(const gwh_params *__restrict__ params, const in
     }
 }
 
+static void launch_setup(int model, int n, hipStream_t st, const gwh_params *params, const int *inst_event,
+                         int event0, InstCoef *coef, int *active)
+{
+    Engine &E = gwh_engine();
+    dim3 g((n + 63) / 64), b(64);
+    if (model == GWH_IMRPHENOMPV2)
+        hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMPV2>, g, b, 0, st, params, inst_event, event0, n,
+                           (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
+                           coef, active);
+    else
+        hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMD>, g, b, 0, st, params, inst_event, event0, n,
+                           (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
+                           coef, active);
+}
+
 int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_params *d_params, int n, InstCoef *d_coef,
                      hipStream_t stream)
 {
     Engine &E = gwh_engine();
-    hipLaunchKernelGGL(gwh_setup_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_params, d_events, events0, n,
-                       model, (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo,
-                       (const EventDesc *)E.d_events, d_coef, (int *)nullptr);
+    launch_setup(model, n, stream, d_params, d_events, events0, d_coef, nullptr);
     GWH_HIP(hipGetLastError());
     return GWH_OK;
 }
@@ -391,17 +433,19 @@ __device__ __forceinline__ double wave_sum(double v)
 
 /* Block-reduce NV values per thread (fixed order) and write them to out[0..NV) */
 template <int NV>
-__device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double *__restrict__ out, double (*lds)[NV])
+__device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double *__restrict__ out, double (*lds)[NV],
+                                                   int nv = NV)
 {
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < NV; j++) acc[j] = wave_sum(acc[j]);
+    for (int j = 0; j < NV; j++)
+        if (j < nv) acc[j] = wave_sum(acc[j]); /* nv is uniform: unused slots are skipped, not masked */
     if (lane == 0) {
 #pragma unroll
         for (int j = 0; j < NV; j++) lds[wv][j] = acc[j];
     }
     __syncthreads();
-    if ((int)threadIdx.x < NV) {
+    if ((int)threadIdx.x < nv) {
         int j = threadIdx.x;
         double s = lds[0][j];
         for (int w = 1; w < GWH_TPB / 64; w++) s += lds[w][j];
@@ -449,16 +493,6 @@ __device__ __forceinline__ double exp_mixed(double x)
 /* per-bin mixed-precision evaluators                                         */
 /* ------------------------------------------------------------------------- */
 
-/* x^{-1/3} in FP64 from an FP32 seed (v_log_f32 / v_exp_f32) and two Newton steps (|rel err| ~ 1e-16) */
-__device__ __forceinline__ double rcbrt_mixed(double x)
-{
-    float xf = (float)x;
-    double r = (double)__builtin_amdgcn_exp2f(-0.333333333f * __builtin_amdgcn_logf(xf));
-    r = r + r * (1.0 - x * r * r * r) * 0.33333333333333333;
-    r = r + r * (1.0 - x * r * r * r) * 0.33333333333333333;
-    return r;
-}
-
 /* z^{-1/2} in FP64 from an FP32 v_rsq_f32 seed and two Newton steps */
 __device__ __forceinline__ double rsqrt_mixed(double z)
 {
@@ -471,14 +505,14 @@ __device__ __forceinline__ double rsqrt_mixed(double z)
 /* IMRPhenomD in mixed precision.  Phase: FP64 arithmetic with FP64 log (inspiral, intermediate) and an
  * FP32 arctangent for the small merger-ringdown atan term.  Amplitude: FP64 polynomials, the powers of Mf
  * from Newton-refined FP32 seeds and an FP32 exponential (merger-ringdown), rounded once to FP32. */
-__device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, float &a, double &phi, double &u_out,
-                                                  double &lx_out)
+__device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, const BinPow &bp, float &a, double &phi,
+                                                  double &u_out, double &lx_out)
 {
     double Mf = C.M * f;
-    double r = rcbrt_mixed(Mf);          /* (Mf)^{-1/3} */
-    double u = Mf * r * r;               /* (Mf)^{1/3} */
+    double r = C.Mm13 * bp.fm13;     /* (Mf)^{-1/3} */
+    double u = C.M13 * bp.f13;       /* (Mf)^{1/3} */
     double r2 = r * r;
-    double inv76 = r2 * r2 * rsqrt_mixed(r); /* (Mf)^{-4/3} (Mf)^{1/6} = (Mf)^{-7/6} */
+    double inv76 = C.Mm76 * bp.fm76; /* (Mf)^{-7/6} */
     double ar;
     if (f < C.f1) {
         double pn = C.ca[0] + u * u * (C.ca[2] + u * (C.ca[3] + u * (C.ca[4] + u * (C.ca[5] + u * C.ca[6]))));
@@ -495,7 +529,7 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, f
     double lx = 0;
     if (f < C.f1p) {
         double v = u * 1.4645918875615231;
-        lx = log(GWH_PI * Mf);
+        lx = C.lgPiM + bp.lf; /* log(pi M f) */
         double c5 = C.c8 + lx * C.c9;
         double c6 = C.c10 - C.c11 * lx;
         double v2 = v * v;
@@ -506,12 +540,11 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, f
         psi = -GWH_PI / 4. + C.K * poly * (iv2 * iv2 * iv) +
               (C.sig[0] * Mf + C.sig[1] * Mf43 + C.sig[2] * Mf43 * u + C.sig[3] * Mf * Mf);
     } else if (f > C.f2p) {
-        double s4 = rsqrt_mixed(rsqrt_mixed(Mf)); /* (Mf)^{1/4} */
-        psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] * (r2 * r) + C.alpha[3] * Mf * rsqrt_mixed(Mf) * s4 +
+        psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] * (r2 * r) + C.alpha[3] * (C.M34 * bp.f34) +
               C.alpha[4] * (double)atanf((float)((f - C.alpha[5]) / C.fdamp));
     } else {
         double r3 = r2 * r;
-        psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * log(Mf) - C.beta[3] * (r3 * r3 * r3);
+        psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] * (r3 * r3 * r3);
     }
     phi = psi - (C.tc_total * (f - C.f_ref) + C.phic);
     u_out = u;
@@ -559,9 +592,10 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_loglike_kernel(WorkMap W, const E
                     if (k >= N) break;
                     double f = E.f[k];
                     if (f > C.fcut) continue;
+                    const BinPow bp = gwh_load_bin(E, k);
                     if (MODEL == GWH_IMRPHENOMD) {
                         double A, phi, u, lx;
-                        gwh_phenomd_fp64(C, f, A, phi, u, lx);
+                        gwh_phenomd_fp64(C, f, bp, A, phi, u, lx);
 #pragma unroll
                         for (int d = 0; d < GWH_MAXDET; d++) {
                             if (d >= D) break;
@@ -575,7 +609,7 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_loglike_kernel(WorkMap W, const E
                         }
                     } else {
                         double A, Phi, hpr, hpi, hcr, hci;
-                        gwh_pv2_fp64(C, f, A, Phi, hpr, hpi, hcr, hci);
+                        gwh_pv2_fp64(C, f, bp, A, Phi, hpr, hpi, hcr, hci);
 #pragma unroll
                         for (int d = 0; d < GWH_MAXDET; d++) {
                             if (d >= D) break;
@@ -600,10 +634,11 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_loglike_kernel(WorkMap W, const E
                     if (k >= N) break;
                     double f = E.f[k];
                     if (f > C.fcut) continue;
+                    const BinPow bp = gwh_load_bin(E, k);
                     if (MODEL == GWH_IMRPHENOMD) {
                         float a;
                         double phi, u, lx;
-                        gwh_phenomd_mixed(C, f, a, phi, u, lx);
+                        gwh_phenomd_mixed(C, f, bp, a, phi, u, lx);
 #pragma unroll
                         for (int d = 0; d < GWH_MAXDET; d++) {
                             if (d >= D) break;
@@ -618,10 +653,10 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_loglike_kernel(WorkMap W, const E
                     } else {
                         float a;
                         double phi, u, lx;
-                        gwh_phenomd_mixed(C, f, a, phi, u, lx);
+                        gwh_phenomd_mixed(C, f, bp, a, phi, u, lx);
                         /* precession in FP64 for the angles, FP32 for the Wigner factors */
                         double v = u * 1.4645918875615231;
-                        double logw = (f < C.f1p) ? lx : log(GWH_PI * C.M * f);
+                        double logw = C.lgPiM + bp.lf; /* log(pi M f) */
                         double x = v * v;
                         double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) / v;
                         double sv = C.SP / (L + C.SL);
@@ -680,24 +715,40 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_loglike_kernel(WorkMap W, const E
                 for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) acc[j] = k3[j].value();
             }
         }
-        block_reduce_store<GWH_PARTIAL_STRIDE>(acc, partial + (size_t)w * GWH_PARTIAL_STRIDE, lds);
+        block_reduce_store<GWH_PARTIAL_STRIDE>(acc, partial + (size_t)w * GWH_PARTIAL_STRIDE, lds, 3 * D);
     }
 }
 
-__global__ void gwh_finalize_kernel(WorkMap W, const EventDesc *__restrict__ events, const InstCoef *__restrict__ coefs,
-                                    const double *__restrict__ partial, const int *__restrict__ active, int model,
-                                    int prec, double *__restrict__ logl)
+/* One wave per instance: lane l sums tiles l, l + 64, ... in order, then a fixed xor-shuffle tree; the
+ * order depends only on the instance's own tile count, so results do not depend on the batch. */
+__global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const EventDesc *__restrict__ events,
+                                                               const InstCoef *__restrict__ coefs,
+                                                               const double *__restrict__ partial,
+                                                               const int *__restrict__ active, int model, int prec,
+                                                               double *__restrict__ logl)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= W.n) return;
-    int ev = W.inst_event ? W.inst_event[i] : W.event0;
+    const int i = blockIdx.x * (GWH_TPB / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= W.n) return; /* whole waves leave together */
+    const int ev = W.inst_event ? W.inst_event[i] : W.event0;
     const EventDesc &E = events[ev];
-    const InstCoef &C = coefs[i];
-    int t0 = i * W.tiles0, nt = active[i];
+    const int nv = 3 * E.D;
+    const size_t t0 = (size_t)i * W.tiles0;
+    const int nt = active[i];
     double s[GWH_PARTIAL_STRIDE];
+#pragma unroll
     for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) s[j] = 0;
-    for (int t = 0; t < nt; t++)
-        for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) s[j] += partial[(size_t)(t0 + t) * GWH_PARTIAL_STRIDE + j];
+    for (int t = lane; t < nt; t += 64) {
+        const double *p = partial + (t0 + t) * GWH_PARTIAL_STRIDE;
+#pragma unroll
+        for (int j = 0; j < GWH_PARTIAL_STRIDE; j++)
+            if (j < nv) s[j] += p[j];
+    }
+#pragma unroll
+    for (int j = 0; j < GWH_PARTIAL_STRIDE; j++)
+        if (j < nv) s[j] = wave_sum(s[j]);
+    if (lane != 0) return;
+    const InstCoef &C = coefs[i];
     double ll = 0;
     for (int d = 0; d < E.D; d++) {
         double hh, dh;
@@ -727,9 +778,12 @@ int gwh_choose_bpl(int N)
 {
     Engine &E = gwh_engine();
     if (E.bpl > 0) return E.bpl;
-    if (N <= 8192) return 1;
-    if (N <= 65536) return 2;
-    return 4;
+    /* 8 bins per lane amortise the per-tile reduction best (measured on the 16225-bin network grid:
+     * 1/2/4/8 -> 92/96/77/72 us per 512-instance launch); short grids keep at least two tiles */
+    if (N >= 2 * GWH_TPB * 8) return 8;
+    if (N >= 2 * GWH_TPB * 4) return 4;
+    if (N >= 2 * GWH_TPB * 2) return 2;
+    return 1;
 }
 
 template <int MODEL, int PREC>
@@ -781,9 +835,7 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     W.event0 = events0;
     W.n = n;
     W.tiles0 = maxtiles;
-    hipLaunchKernelGGL(gwh_setup_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_params, d_events, events0, n, model,
-                       (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
-                       E.coef.as<InstCoef>(), E.active.as<int>());
+    launch_setup(model, n, stream, d_params, d_events, events0, E.coef.as<InstCoef>(), E.active.as<int>());
     GWH_HIP(hipGetLastError());
     if (timed) GWH_HIP(hipEventRecord(E.ev0, stream));
     const bool live = E.kt_on && 2 * (size_t)E.kt_used + 1 < E.kt_ev.size();
@@ -812,7 +864,8 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     GWH_HIP(hipGetLastError());
     if (timed) GWH_HIP(hipEventRecord(E.ev1, stream));
     if (live) GWH_HIP(hipEventRecord(E.kt_ev[2 * E.kt_used++ + 1], stream));
-    hipLaunchKernelGGL(gwh_finalize_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, W, evp, cp,
+    hipLaunchKernelGGL(gwh_finalize_kernel, dim3((n + GWH_TPB / 64 - 1) / (GWH_TPB / 64)), dim3(GWH_TPB), 0, stream,
+                       W, evp, cp,
                        (const double *)pp, ap, model, precision, d_logl);
     GWH_HIP(hipGetLastError());
     if (timed && kernel_ms) {
@@ -863,9 +916,10 @@ __global__ void gwh_waveform_kernel(const double *__restrict__ f, int N, const I
     double fk = f[k];
     double2 hp = make_double2(0, 0), hc = make_double2(0, 0);
     if (fk <= C.fcut) {
+        const BinPow bp = gwh_bin_powers(fk);
         if (MODEL == GWH_IMRPHENOMD) {
             double A, phi, u, lx;
-            gwh_phenomd_fp64(C, fk, A, phi, u, lx);
+            gwh_phenomd_fp64(C, fk, bp, A, phi, u, lx);
             double s, c;
             sincos(phi, &s, &c);
             double hr = A * c, hi = -A * s; /* h = A e^{-i phi} */
@@ -873,7 +927,7 @@ __global__ void gwh_waveform_kernel(const double *__restrict__ f, int N, const I
             hc = make_double2(hi * C.wf_b, -hr * C.wf_b); /* -i h cos(iota) */
         } else {
             double A, Phi, hpr, hpi, hcr, hci;
-            gwh_pv2_fp64(C, fk, A, Phi, hpr, hpi, hcr, hci);
+            gwh_pv2_fp64(C, fk, bp, A, Phi, hpr, hpi, hcr, hci);
             double s, c;
             sincos(Phi, &s, &c);
             /* (A hpf) e^{-i Phi} */
@@ -1198,10 +1252,8 @@ static int upload_single(const double *f, int n, const gwh_params *p, int model,
     ds.bpl = 1;
     for (int d = 0; d < GWH_MAXDET; d++) ds.det[d] = (dets && d < ndet) ? dets[d] : det;
     GWH_HIP(hipMemcpyAsync(E.d_events + (GWH_MAX_EVENTS - 1), &ds, sizeof(ds), hipMemcpyHostToDevice, E.stream));
-    hipLaunchKernelGGL(gwh_setup_kernel, dim3(1), dim3(64), 0, E.stream, E.params.as<const gwh_params>(),
-                       (const int *)nullptr, GWH_MAX_EVENTS - 1, 1, model, (const double *)E.d_fits, E.qnm,
-                       (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events, E.coef.as<InstCoef>(),
-                       (int *)nullptr);
+    launch_setup(model, 1, E.stream, E.params.as<const gwh_params>(), nullptr, GWH_MAX_EVENTS - 1,
+                 E.coef.as<InstCoef>(), nullptr);
     GWH_HIP(hipGetLastError());
     return GWH_OK;
 }

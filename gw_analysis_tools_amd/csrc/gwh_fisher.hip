@@ -285,8 +285,9 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
     if (a.skyavg) {
         /* d h = dA + i sign d(phase) A_central (fisher.cpp:292-316); no detector projection */
         const InstCoef &Cc = C[a.dim * a.S];
+        const BinPow bp = gwh_load_bin(E, k);
         double Ac = 0, u, lx, ph;
-        if (f <= Cc.fcut) gwh_phenomd_fp64(Cc, f, Ac, ph, u, lx);
+        if (f <= Cc.fcut) gwh_phenomd_fp64(Cc, f, bp, Ac, ph, u, lx);
         double sw = sqrt(E.wS[(size_t)a.d_first * N + k]);
         T *r0 = base + (size_t)(2 * k) * GWH_FROWS;
         for (int i = 0; i < GWH_FROWS; i++) {
@@ -295,7 +296,7 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
                 for (int s = 0; s < a.S; s++) {
                     const InstCoef &Ci = C[i * a.S + s];
                     double A = 0, phi;
-                    gwh_phenomd_fp64(Ci, f, A, phi, u, lx);
+                    gwh_phenomd_fp64(Ci, f, bp, A, phi, u, lx);
                     if (!(f <= Ci.fcut)) A = 0;
                     dA += cw[i * GWH_MAXSTEN + s] * A;
                     dP += cw[i * GWH_MAXSTEN + s] * (-phi); /* construct_phase returns -(Psi - tc (f-fref) - phic) */
@@ -313,6 +314,7 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
         }
         return;
     }
+    const BinPow bp = gwh_load_bin(E, k);
     double sw[GWH_MAXDET];
     for (int dd = 0; dd < GWH_MAXDET; dd++) sw[dd] = (dd < a.d_count) ? sqrt(E.wS[(size_t)(a.d_first + dd) * N + k]) : 0;
     for (int i = 0; i < GWH_FROWS; i++) {
@@ -325,7 +327,7 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
                 const bool tof = !nt[i * a.S + s];
                 if (MODEL == GWH_IMRPHENOMD) {
                     double A, phi, u, lx;
-                    gwh_phenomd_fp64(Ci, f, A, phi, u, lx);
+                    gwh_phenomd_fp64(Ci, f, bp, A, phi, u, lx);
                     for (int dd = 0; dd < GWH_MAXDET; dd++) {
                         if (dd >= a.d_count) break;
                         int d = a.d_first + dd;
@@ -337,7 +339,7 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
                     }
                 } else {
                     double A, Phi, hpr, hpi, hcr, hci;
-                    gwh_pv2_fp64(Ci, f, A, Phi, hpr, hpi, hcr, hci);
+                    gwh_pv2_fp64(Ci, f, bp, A, Phi, hpr, hpi, hcr, hci);
                     for (int dd = 0; dd < GWH_MAXDET; dd++) {
                         if (dd >= a.d_count) break;
                         int d = a.d_first + dd;

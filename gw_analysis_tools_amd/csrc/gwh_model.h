@@ -60,8 +60,27 @@ struct InstCoef {
     /* per detector: PhenomD: (Re K, Im K, |K|^2, 2 pi dt) ; Pv2: (P, Q, 0, 2 pi dt) */
     double det[GWH_MAXDET][4];
     double wf_a, wf_b; /* polarisation constants: PhenomD ((1+cos^2 i)/2, cos i); Pv2 (cos 2 zeta, sin 2 zeta) */
+    /* powers of the total mass that pair with the tabulated powers of each bin: (Mf)^p = M^p f^p */
+    double M13, Mm13, Mm76, M34, lgM, lgPiM;
     double pad;
 };
+
+/* Powers of one frequency bin that every instance needs (log f, f^(1/3), f^(-1/3), f^(-7/6), f^(3/4)).
+ * Resident events tabulate them once; single calls evaluate them per bin with the same function. */
+struct BinPow {
+    double lf, f13, fm13, fm76, f34;
+};
+
+__device__ __forceinline__ BinPow gwh_bin_powers(double f)
+{
+    BinPow b;
+    b.lf = log(f);
+    b.f13 = cbrt(f);
+    b.fm13 = 1.0 / b.f13;
+    b.fm76 = 1.0 / (f * sqrt(b.f13));
+    b.f34 = sqrt(sqrt(f * f * f));
+    return b;
+}
 
 /* QNM spline table in device memory: knots, values and S''/2 of the natural spline */
 struct QnmTable {
@@ -84,14 +103,12 @@ __device__ __forceinline__ double gwh_spline_eval(const double *x, const double 
                                                   double xq)
 {
     if (!(xq >= x[0] && xq <= x[n - 1])) return __longlong_as_double(0x7ff8000000000000LL);
-    int lo = 0, hi = n - 1;
-    while (hi > lo + 1) {
-        int mid = (hi + lo) >> 1;
-        if (x[mid] > xq)
-            hi = mid;
-        else
-            lo = mid;
-    }
+    /* interval with x[lo] <= xq < x[lo + 1] (lo = n - 2 at the last knot), the one a bisection finds:
+     * start from the uniform-spacing guess and walk (a step or two on the near-uniform spin grid) */
+    int lo = (int)((xq - x[0]) / (x[n - 1] - x[0]) * (n - 1));
+    lo = lo < 0 ? 0 : (lo > n - 2 ? n - 2 : lo);
+    while (lo > 0 && x[lo] > xq) lo--;
+    while (lo < n - 2 && x[lo + 1] <= xq) lo++;
     double h = x[lo + 1] - x[lo];
     double b = (y[lo + 1] - y[lo]) / h - h * (c[lo + 1] + 2.0 * c[lo]) / 3.0;
     double d = (c[lo + 1] - c[lo]) / (3.0 * h);
@@ -173,11 +190,18 @@ __device__ __forceinline__ double gwh_setup_dphase_ins(const SetupState &S, doub
     c[5] = S.pp[8] + lx * S.pp[9];
     c[6] = S.pp[10] - S.pp[11] * lx;
     double K = 3. / (128. * S.eta);
-    double sum = 0;
-    for (int i = 0; i < 8; i++) sum += c[i] * ((i - 5.) / 3.) * pow(x, (i - 8.) / 3.);
-    double d = K * (GWH_PI * M) * sum + K * (S.pp[9] - S.pp[11] * cbrt(x)) / f;
-    double sg = S.sigma[1] * M + S.sigma[2] * pow(M, 4. / 3.) * cbrt(f) + S.sigma[3] * pow(M, 5. / 3.) * pow(f, 2. / 3.) +
-                S.sigma[4] * M * M * f;
+    /* x^((i - 8) / 3) as powers of x^(-1/3) */
+    double v = cbrt(x), iv = 1.0 / v;
+    double sum = 0, pw = iv;
+#pragma unroll
+    for (int i = 7; i >= 0; i--) {
+        sum += c[i] * ((i - 5.) / 3.) * pw;
+        pw *= iv;
+    }
+    double d = K * (GWH_PI * M) * sum + K * (S.pp[9] - S.pp[11] * v) / f;
+    /* M^(4/3) f^(1/3) = M (Mf)^(1/3), M^(5/3) f^(2/3) = M (Mf)^(2/3) */
+    double u = cbrt(M * f);
+    double sg = S.sigma[1] * M + S.sigma[2] * M * u + S.sigma[3] * M * u * u + S.sigma[4] * M * M * f;
     return d + sg / S.eta;
 }
 
@@ -547,6 +571,12 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
     C.f2p = S.f2p;
     C.fRD = S.fRD;
     C.fdamp = S.fdamp;
+    C.M13 = cbrt(S.M);
+    C.Mm13 = 1.0 / C.M13;
+    C.Mm76 = 1.0 / (S.M * sqrt(C.M13));
+    C.M34 = sqrt(sqrt(S.M * S.M * S.M));
+    C.lgM = log(S.M);
+    C.lgPiM = log(GWH_PI * S.M);
     {
         double pi13 = cbrt(GWH_PI), pw = 1.0;
         for (int i = 0; i < 7; i++) {
@@ -645,19 +675,24 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
                 c[n - 1] = 0;
                 c[m] = rh[m - 1] / dg[m - 1];
                 for (int i = m - 2; i >= 0; i--) c[i + 1] = (rh[i] - of[i] * c[i + 2]) / dg[i];
-                int lo = 0, hi = n - 1;
-                while (hi > lo + 1) {
-                    int mid = (hi + lo) >> 1;
-                    if (fs[mid] > S.fRD)
-                        hi = mid;
-                    else
-                        lo = mid;
-                }
-                double h = fs[lo + 1] - fs[lo];
-                double b = (ph[lo + 1] - ph[lo]) / h - h * (c[lo + 1] + 2.0 * c[lo]) / 3.0;
-                double dd = (c[lo + 1] - c[lo]) / (3.0 * h);
-                double dx = S.fRD - fs[lo];
-                t_corr = (b + dx * (2.0 * c[lo] + 3.0 * dd * dx)) / GWH_2PI;
+                /* interval holding fRD (the bisection's answer, found by an unrolled scan so that every
+                 * array index is a compile-time constant and the arrays stay in registers) */
+                double x0 = fs[0], x1 = fs[1], y0 = ph[0], y1 = ph[1], c0 = c[0], c1 = c[1];
+#pragma unroll
+                for (int j = 1; j < n - 1; j++)
+                    if (fs[j] <= S.fRD) {
+                        x0 = fs[j];
+                        x1 = fs[j + 1];
+                        y0 = ph[j];
+                        y1 = ph[j + 1];
+                        c0 = c[j];
+                        c1 = c[j + 1];
+                    }
+                double h = x1 - x0;
+                double b = (y1 - y0) / h - h * (c1 + 2.0 * c0) / 3.0;
+                double dd = (c1 - c0) / (3.0 * h);
+                double dx = S.fRD - x0;
+                t_corr = (b + dx * (2.0 * c0 + 3.0 * dd * dx)) / GWH_2PI;
             }
         }
         C.t_corr_2pi = GWH_2PI * t_corr;
@@ -671,15 +706,20 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
     C.wf_a = pv2 ? c2z : kplus;
     C.wf_b = pv2 ? s2z : ci;
     C.pad = 0;
+    /* detector_util.cpp:900-1016: polarisation basis (shared by every detector) */
+    const double gha = p.gmst - p.RA;
+    double cg, sg, cd, sd, cp, sp;
+    sincos(gha, &sg, &cg);
+    sincos(p.DEC, &sd, &cd);
+    sincos(p.psi, &sp, &cp);
+    const double X[3] = {-cp * sg - sp * cg * sd, -cp * cg + sp * sg * sd, sp * cd};
+    const double Y[3] = {sp * sg - cp * cg * sd, sp * cg + cp * sg * sd, cp * cd};
+    /* detector_util.cpp:768-790: propagation direction for dt = (r0 - rd).n / c (hour angle gmst - RA) */
+    const double e0 = cd * cg, e1 = cd * -sg, e2 = sd;
     for (int d = 0; d < GWH_MAXDET; d++) {
         C.det[d][0] = C.det[d][1] = C.det[d][2] = C.det[d][3] = 0;
         if (d >= ndet) continue;
         const DetGeom &g = geo[dets[d]];
-        /* detector_util.cpp:900-1016 */
-        double gha = p.gmst - p.RA;
-        double cg = cos(gha), sg = sin(gha), cd = cos(p.DEC), sd = sin(p.DEC), cp = cos(p.psi), sp = sin(p.psi);
-        double X[3] = {-cp * sg - sp * cg * sd, -cp * cg + sp * sg * sd, sp * cd};
-        double Y[3] = {sp * sg - cp * cg * sd, sp * cg + cp * sg * sd, cp * cd};
         double fp = 0, fx = 0;
         for (int i = 0; i < 3; i++) {
             double DX = g.tensor[3 * i] * X[0] + g.tensor[3 * i + 1] * X[1] + g.tensor[3 * i + 2] * X[2];
@@ -689,9 +729,6 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
         }
         fp *= g.gfac;
         fx *= g.gfac;
-        /* detector_util.cpp:768-790: dt = (r0 - rd).n / c */
-        double ha = p.gmst - p.RA;
-        double e0 = cos(p.DEC) * cos(ha), e1 = cos(p.DEC) * -sin(ha), e2 = sin(p.DEC);
         double dt = ((g0.loc[0] - g.loc[0]) * e0 + (g0.loc[1] - g.loc[1]) * e1 + (g0.loc[2] - g.loc[2]) * e2) / GWH_C_SI;
         if (!pv2) {
             C.det[d][0] = fp * kplus;
@@ -712,13 +749,12 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
 
 /* IMRPhenomD amplitude A(f) and phase phi(f) = Psi(f) - tc_total (f - f_ref) - phic
  * (IMRPhenomD.cpp:471-501); caller guarantees 0 < f <= fcut. */
-__device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, double &A, double &phi, double &u_out,
-                                                 double &lx_out)
+__device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, const BinPow &bp, double &A, double &phi,
+                                                 double &u_out, double &lx_out)
 {
     double Mf = C.M * f;
-    double u = cbrt(Mf);          /* (Mf)^{1/3} */
-    double su = sqrt(u);          /* (Mf)^{1/6} */
-    double inv76 = 1.0 / (Mf * su); /* (Mf)^{-7/6} */
+    double u = C.M13 * bp.f13;      /* (Mf)^{1/3} */
+    double inv76 = C.Mm76 * bp.fm76; /* (Mf)^{-7/6} */
     double ar;
     if (f < C.f1) {
         double u2 = u * u, u3 = u2 * u;
@@ -736,7 +772,7 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, do
     double lx = 0;
     if (f < C.f1p) {
         double v = u * 1.4645918875615231; /* pi^{1/3} */
-        lx = log(GWH_PI * Mf);
+        lx = C.lgPiM + bp.lf;             /* log(pi M f) */
         double c5 = C.c8 + lx * C.c9;
         double c6 = C.c10 - C.c11 * lx;
         double v2 = v * v;
@@ -745,10 +781,10 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, do
         psi = -GWH_PI / 4. + C.K * poly / (v2 * v2 * v) +
               (C.sig[0] * Mf + C.sig[1] * Mf43 + C.sig[2] * Mf43 * u + C.sig[3] * Mf * Mf);
     } else if (f > C.f2p) {
-        psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] / Mf + C.alpha[3] * sqrt(sqrt(Mf * Mf * Mf)) +
+        psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] / Mf + C.alpha[3] * (C.M34 * bp.f34) +
               C.alpha[4] * atan((f - C.alpha[5]) / C.fdamp);
     } else {
-        psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * log(Mf) - C.beta[3] / (Mf * Mf * Mf);
+        psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] / (Mf * Mf * Mf);
     }
     phi = psi - (C.tc_total * (f - C.f_ref) + C.phic);
     u_out = u;
@@ -757,14 +793,14 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, do
 
 /* IMRPhenomPv2 twist-up at one bin (IMRPhenomP.cpp:290-361): returns the common amplitude
  * (already including the 1/2), the total phase and the complex h+ / hx twist-up factors. */
-__device__ __forceinline__ void gwh_pv2_fp64(const InstCoef &C, double f, double &A, double &Phi, double &hpr,
-                                             double &hpi, double &hcr, double &hci)
+__device__ __forceinline__ void gwh_pv2_fp64(const InstCoef &C, double f, const BinPow &bp, double &A, double &Phi,
+                                             double &hpr, double &hpi, double &hcr, double &hci)
 {
     double u, lx;
-    gwh_phenomd_fp64(C, f, A, Phi, u, lx);
+    gwh_phenomd_fp64(C, f, bp, A, Phi, u, lx);
     /* gwh_phenomd_fp64 already applied -tc (f - f_ref) - phic; Pv2 adds 2 eps and the t_c correction */
     double v = u * 1.4645918875615231;
-    double logw = (f < C.f1p) ? lx : log(GWH_PI * C.M * f);
+    double logw = C.lgPiM + bp.lf; /* log(pi M f) */
     double x = v * v;
     double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) / v;
     double s = C.SP / (L + C.SL);
