@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgwat_hip.so")
 
 GWH_OK = 1
+GWH_ERR_INVALID, GWH_ERR_HIP, GWH_ERR_UNSUPPORTED, GWH_ERR_NOMEM, GWH_ERR_BUSY = -1, -2, -3, -4, -5
 IMRPHENOMD = 0
 IMRPHENOMPV2 = 1
 FP64 = 0

@@ -1,0 +1,282 @@
+/*
+ * gwh_gwatpy.cpp -- the gwatpy C ABI of the hot path (include/gwat/gwatpy_wrapping.h) over the engine.
+ *
+ * gwatpy binds libgwat with ctypes and treats gen_params_base<double>* as an opaque handle made by
+ * gen_params_base_py.  Exporting the same names with the same argument order lets those ctypes
+ * declarations (gwatpy/gwatpy/waveform_generator.py, mcmc_routines.py) load libgwat_hip.so instead,
+ * for the functions on the hot path: parameter handles, fourier_waveform_py,
+ * fourier_detector_response_py, MCMC_likelihood_extrinsic_py, repack_parameters_py and the two
+ * geometry helpers.  The handle here is the engine's own record; parameters the engine does not model
+ * (equatorial or horizon orientation, neutron-star flags, ppE / gIMR modifications) are recorded and
+ * reported as GWH_ERR_UNSUPPORTED when the handle is used, never silently dropped.
+ */
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "gwat_hip.h"
+#include "gwatpy_abi.h"
+
+/* the engine's error recorder (gwh_engine.hip) */
+int gwh_set_error(int code, const std::string &msg);
+
+namespace {
+
+const uint64_t kMagic = 0x6777617470793031ull; /* "gwatpy01" */
+
+struct PyParams {
+    uint64_t magic;
+    gwh_params p;
+    std::string unsupported; /* empty: every field is modelled */
+};
+
+/* live handles: a pointer is dereferenced only after it is found here */
+std::mutex g_handles_mu;
+std::unordered_set<void *> g_handles;
+
+PyParams *handle(void *h)
+{
+    std::lock_guard<std::mutex> lk(g_handles_mu);
+    if (!h || !g_handles.count(h)) return nullptr;
+    PyParams *q = static_cast<PyParams *>(h);
+    return q->magic == kMagic ? q : nullptr;
+}
+
+int model_of(const char *method)
+{
+    if (!method) return -1;
+    if (!strcmp(method, "IMRPhenomD")) return GWH_IMRPHENOMD;
+    if (!strcmp(method, "IMRPhenomPv2")) return GWH_IMRPHENOMPV2;
+    return -1;
+}
+
+/* Fisher / MCMC basis of a generation method string and dimension (fisher.cpp:1841-2161) */
+int basis_of(const char *method, int dim, int sky_average)
+{
+    if (!method) return -1;
+    std::string m(method);
+    if (m == "IMRPhenomD") return (sky_average && dim == 7) ? GWH_BASIS_PHENOMD_SKYAVG : (dim == 11 ? GWH_BASIS_PHENOMD_FULL : -1);
+    if (m == "MCMC_IMRPhenomD") return dim == 11 ? GWH_BASIS_PHENOMD_MCMC : -1;
+    if (m == "IMRPhenomPv2") return dim == 13 ? GWH_BASIS_PV2_FULL : -1;
+    if (m == "MCMC_IMRPhenomPv2") return dim == 15 ? GWH_BASIS_PV2_MCMC : -1;
+    return -1;
+}
+
+/* failures found here are recorded like the engine's own (gwh_last_error) */
+int gwh_report(int code, const char *msg) { return gwh_set_error(code, msg); }
+
+} // namespace
+
+extern "C" {
+
+void *gen_params_base_py(double mass1, double mass2, double *spin1, double *spin2, double Luminosity_Distance,
+                         double incl_angle, double RA, double DEC, double psi, double gmst, double tc, double phiRef,
+                         double f_ref, double theta_l, double phi_l, double theta, double phi, char *cosmology,
+                         bool equatorial_orientation, bool horizon_coord, bool NSflag1, bool NSflag2,
+                         bool dep_postmerger, bool shift_time, bool shift_phase, bool sky_average, double LISA_alpha0,
+                         double LISA_phi0, int Nmod_phi, int Nmod_sigma, int Nmod_beta, int Nmod_alpha, int *phii,
+                         int *sigmai, int *betai, int *alphai, double *delta_phi, double *delta_sigma,
+                         double *delta_beta, double *delta_alpha, int Nmod, double *bppe, double *betappe)
+{
+    (void)theta_l, (void)phi_l, (void)theta, (void)phi, (void)cosmology, (void)LISA_alpha0, (void)LISA_phi0;
+    (void)phii, (void)sigmai, (void)betai, (void)alphai, (void)delta_phi, (void)delta_sigma, (void)delta_beta;
+    (void)delta_alpha, (void)bppe, (void)betappe;
+    PyParams *q = new PyParams;
+    q->magic = kMagic;
+    gwh_params &p = q->p;
+    memset(&p, 0, sizeof(p));
+    p.mass1 = mass1;
+    p.mass2 = mass2;
+    for (int i = 0; i < 3; i++) {
+        p.spin1[i] = spin1 ? spin1[i] : 0.0;
+        p.spin2[i] = spin2 ? spin2[i] : 0.0;
+    }
+    p.DL = Luminosity_Distance;
+    p.incl = incl_angle;
+    p.RA = RA;
+    p.DEC = DEC;
+    p.psi = psi;
+    p.gmst = gmst;
+    p.tc = tc;
+    p.phiRef = phiRef;
+    p.f_ref = f_ref;
+    p.chip = -1; /* gen_params_base defaults (util.h:231-236): spins from the vectors */
+    p.phip = -1;
+    p.dep_postmerger = dep_postmerger;
+    p.shift_time = shift_time;
+    p.shift_phase = shift_phase;
+    p.sky_average = sky_average;
+    if (equatorial_orientation) q->unsupported = "equatorial_orientation";
+    else if (horizon_coord) q->unsupported = "horizon_coord";
+    else if (NSflag1 || NSflag2) q->unsupported = "NSflag1/NSflag2";
+    else if (Nmod || Nmod_phi || Nmod_sigma || Nmod_beta || Nmod_alpha) q->unsupported = "ppE/gIMR modifications";
+    std::lock_guard<std::mutex> lk(g_handles_mu);
+    g_handles.insert(q);
+    return q;
+}
+
+void gen_params_base_py_destructor(void *p)
+{
+    std::lock_guard<std::mutex> lk(g_handles_mu);
+    if (!p || !g_handles.erase(p)) return;
+    PyParams *q = static_cast<PyParams *>(p);
+    q->magic = 0;
+    delete q;
+}
+
+static int usable(void *parameters, PyParams **out)
+{
+    PyParams *q = handle(parameters);
+    if (!q) return gwh_report(GWH_ERR_INVALID, "not a parameter handle from gen_params_base_py");
+    if (!q->unsupported.empty())
+        return gwh_report(GWH_ERR_UNSUPPORTED, ("the engine does not model " + q->unsupported).c_str());
+    *out = q;
+    return GWH_OK;
+}
+
+int fourier_waveform_py(double *frequencies, int length, double *wf_plus_real, double *wf_plus_imaginary,
+                        double *wf_cross_real, double *wf_cross_imaginary, char *generation_method, void *parameters)
+{
+    PyParams *q;
+    int rc = usable(parameters, &q);
+    if (rc != GWH_OK) return rc;
+    int model = model_of(generation_method);
+    if (model < 0) return gwh_report(GWH_ERR_UNSUPPORTED, "generation method not on the engine's path");
+    if (length < 1 || !wf_plus_real || !wf_plus_imaginary || !wf_cross_real || !wf_cross_imaginary)
+        return gwh_report(GWH_ERR_INVALID, "null output or empty grid");
+    std::vector<double> hp(2 * (size_t)length), hc(2 * (size_t)length);
+    rc = gwh_fourier_waveform(frequencies, length, hp.data(), hc.data(), model, &q->p);
+    if (rc != GWH_OK) return rc;
+    for (int i = 0; i < length; i++) {
+        wf_plus_real[i] = hp[2 * i];
+        wf_plus_imaginary[i] = hp[2 * i + 1];
+        wf_cross_real[i] = hc[2 * i];
+        wf_cross_imaginary[i] = hc[2 * i + 1];
+    }
+    return rc;
+}
+
+int fourier_detector_response_py(double *frequencies, int length, double *response_real, double *response_imaginary,
+                                 char *detector, char *generation_method, void *parameters)
+{
+    PyParams *q;
+    int rc = usable(parameters, &q);
+    if (rc != GWH_OK) return rc;
+    int model = model_of(generation_method);
+    if (model < 0) return gwh_report(GWH_ERR_UNSUPPORTED, "generation method not on the engine's path");
+    int det = gwh_detector_id(detector);
+    if (det < 0) return gwh_report(GWH_ERR_UNSUPPORTED, "unsupported detector");
+    if (length < 1 || !response_real || !response_imaginary)
+        return gwh_report(GWH_ERR_INVALID, "null output or empty grid");
+    std::vector<double> r(2 * (size_t)length);
+    rc = gwh_fourier_detector_response(frequencies, length, r.data(), det, model, &q->p);
+    if (rc != GWH_OK) return rc;
+    for (int i = 0; i < length; i++) {
+        response_real[i] = r[2 * i];
+        response_imaginary[i] = r[2 * i + 1];
+    }
+    return rc;
+}
+
+/* gwatpy_wrapping.cpp:184-241.  Detector letters follow the wrapper's own matching: the defaults
+ * Hanford, Livingston, Virgo, replaced only where the suffix starting at letter i is exactly one of
+ * "H", "L", "V", "K", "C" (so in practice only the last letter is ever matched).  The wrapper's T is a
+ * pointer difference (mcmc_gw.cpp:2466); the engine uses T_eff = 0, i.e. t_c enters as -tc. */
+double MCMC_likelihood_extrinsic_py(bool save_waveform, void *parameters, char *generation_method, int *data_length,
+                                    double *frequencies, double *dataREAL, double *dataIMAG, double *psd,
+                                    double *weights, char *integration_method, bool log10F, char *detectors,
+                                    int num_detectors)
+{
+    const double nan = std::nan("");
+    PyParams *q;
+    if (usable(parameters, &q) != GWH_OK) return nan;
+    int model = model_of(generation_method);
+    if (model < 0) {
+        gwh_report(GWH_ERR_UNSUPPORTED, "generation method not on the engine's path");
+        return nan;
+    }
+    if (!data_length || !frequencies || !dataREAL || !dataIMAG || !psd || !detectors || num_detectors < 1 ||
+        num_detectors > 4) {
+        gwh_report(GWH_ERR_INVALID, "bad arguments");
+        return nan;
+    }
+    int method = (integration_method && !strcmp(integration_method, "GAUSSLEG")) ? GWH_GAUSSLEG : GWH_SIMPSONS;
+    if (method == GWH_GAUSSLEG && !weights) {
+        gwh_report(GWH_ERR_INVALID, "GAUSSLEG needs weights");
+        return nan;
+    }
+    static const char *defaults[3] = {"Hanford", "Livingston", "Virgo"};
+    int dets[4];
+    size_t nlet = strlen(detectors);
+    for (int i = 0; i < num_detectors; i++) {
+        const char *name = i < 3 ? defaults[i] : nullptr;
+        if ((size_t)i < nlet) {
+            const char *suffix = detectors + i;
+            if (!strcmp(suffix, "H")) name = "Hanford";
+            else if (!strcmp(suffix, "L")) name = "Livingston";
+            else if (!strcmp(suffix, "V")) name = "Virgo";
+            else if (!strcmp(suffix, "K")) name = "KAGRA"; /* not a detector_util name: rejected below */
+            else if (!strcmp(suffix, "C")) name = "CE";
+        }
+        dets[i] = name ? gwh_detector_id(name) : -1;
+        if (dets[i] < 0) {
+            gwh_report(GWH_ERR_UNSUPPORTED, "unsupported detector");
+            return nan;
+        }
+    }
+    /* flattened [detector][bin] arrays at i * data_length[i] + j (gwatpy_wrapping.cpp:211-216) */
+    std::vector<std::vector<double>> dat(num_detectors);
+    std::vector<const double *> fp(num_detectors), dp(num_detectors), pp(num_detectors), wp(num_detectors);
+    for (int i = 0; i < num_detectors; i++) {
+        int n = data_length[i];
+        if (n < 2) {
+            gwh_report(GWH_ERR_INVALID, "need at least 2 bins");
+            return nan;
+        }
+        size_t off = (size_t)i * n;
+        dat[i].resize(2 * (size_t)n);
+        for (int j = 0; j < n; j++) {
+            dat[i][2 * j] = dataREAL[off + j];
+            dat[i][2 * j + 1] = dataIMAG[off + j];
+        }
+        fp[i] = frequencies + off;
+        dp[i] = dat[i].data();
+        pp[i] = psd + off;
+        wp[i] = weights ? weights + off : nullptr;
+    }
+    return gwh_mcmc_likelihood_extrinsic(save_waveform, &q->p, model, data_length, fp.data(), dp.data(), pp.data(),
+                                         weights ? wp.data() : nullptr, method, log10F, dets, num_detectors, 0.0);
+}
+
+void repack_parameters_py(double *parameters, void *gen_param, char *generation_method, int dim)
+{
+    PyParams *q = handle(gen_param);
+    if (!q || !parameters) {
+        gwh_report(GWH_ERR_INVALID, "not a parameter handle from gen_params_base_py");
+        return;
+    }
+    int basis = basis_of(generation_method, dim, q->p.sky_average);
+    if (basis < 0) {
+        gwh_report(GWH_ERR_UNSUPPORTED, "generation method / dimension not on the engine's path");
+        return;
+    }
+    gwh_repack_parameters(parameters, basis, &q->p);
+}
+
+double gps_to_GMST_radian_py(double gps) { return gwh_gps_to_gmst_radian(gps); }
+
+double DTOA_DETECTOR_py(double RA, double DEC, double GMST_rad, char *det1, char *det2)
+{
+    int a = gwh_detector_id(det1), b = gwh_detector_id(det2);
+    if (a < 0 || b < 0) {
+        gwh_report(GWH_ERR_UNSUPPORTED, "unsupported detector");
+        return std::nan("");
+    }
+    return gwh_dtoa(RA, DEC, GMST_rad, a, b);
+}
+
+} /* extern "C" */

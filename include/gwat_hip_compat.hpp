@@ -1,0 +1,201 @@
+/*
+ * gwat_hip_compat.hpp -- the reference's C++ entry points of the hot path, header-only, over the C ABI
+ * of gwat_hip.h.
+ *
+ * The functions take the reference's own parameter object (any type with the field names of
+ * gen_params_base<double>, include/gwat/util.h:121-389) and the reference's string arguments, so a
+ * translation unit built inside the reference tree can define the reference symbols with one line each
+ * (see the interposition unit in the integration notes):
+ *
+ *     template<> int fourier_detector_response<double>(double *f, int n, std::complex<double> *r,
+ *         std::string det, std::string method, gen_params_base<double> *p, double *times)
+ *     { return gwh::fourier_detector_response(f, n, r, det, method, p); }
+ *
+ * Return values follow the reference: 1 on success, and NaN log-likelihoods on failure (the reason is
+ * in gwh_last_error()); nothing here exits the process.
+ */
+#ifndef GWAT_HIP_COMPAT_HPP
+#define GWAT_HIP_COMPAT_HPP
+
+#include <cmath>
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "gwat_hip.h"
+
+namespace gwh {
+
+/* "IMRPhenomD" / "IMRPhenomPv2" (waveform_generator.cpp:103-294 dispatch); -1 otherwise */
+inline int model_id(const std::string &method)
+{
+    if (method == "IMRPhenomD") return GWH_IMRPHENOMD;
+    if (method == "IMRPhenomPv2") return GWH_IMRPHENOMPV2;
+    return -1;
+}
+
+/* "SIMPSONS" / "GAUSSLEG" (mcmc_gw.cpp:811-856) */
+inline int integration_id(const std::string &method) { return method == "GAUSSLEG" ? GWH_GAUSSLEG : GWH_SIMPSONS; }
+
+/* Fisher / MCMC bases of fisher_numerical's method strings (fisher.cpp:1841-2161) */
+inline int basis_id(const std::string &method, bool sky_average)
+{
+    if (method == "IMRPhenomD") return sky_average ? GWH_BASIS_PHENOMD_SKYAVG : GWH_BASIS_PHENOMD_FULL;
+    if (method == "MCMC_IMRPhenomD") return GWH_BASIS_PHENOMD_MCMC;
+    if (method == "IMRPhenomPv2") return GWH_BASIS_PV2_FULL;
+    if (method == "MCMC_IMRPhenomPv2") return GWH_BASIS_PV2_MCMC;
+    return -1;
+}
+
+/* the hot-path fields of a gen_params_base<double>-shaped object; false if it asks for physics the
+ * engine does not model (equatorial / horizon orientation, neutron-star flags, ppE or gIMR terms) */
+template <class Params>
+inline bool to_params(const Params &g, gwh_params &p)
+{
+    p = gwh_params();
+    p.mass1 = g.mass1;
+    p.mass2 = g.mass2;
+    for (int i = 0; i < 3; i++) {
+        p.spin1[i] = g.spin1[i];
+        p.spin2[i] = g.spin2[i];
+    }
+    p.DL = g.Luminosity_Distance;
+    p.tc = g.tc;
+    p.phiRef = g.phiRef;
+    p.f_ref = g.f_ref;
+    p.incl = g.incl_angle;
+    p.RA = g.RA;
+    p.DEC = g.DEC;
+    p.psi = g.psi;
+    p.gmst = g.gmst;
+    p.chip = g.chip;
+    p.phip = g.phip;
+    p.shift_time = g.shift_time;
+    p.shift_phase = g.shift_phase;
+    p.sky_average = g.sky_average;
+    p.dep_postmerger = g.dep_postmerger;
+    return !(g.equatorial_orientation || g.horizon_coord || g.NSflag1 || g.NSflag2 || g.Nmod || g.Nmod_phi ||
+             g.Nmod_sigma || g.Nmod_beta || g.Nmod_alpha);
+}
+
+/* write the basis fields back into the caller's object (repack_parameters, fisher.cpp:2162-2511) */
+template <class Params>
+inline void from_params(const gwh_params &p, Params &g)
+{
+    g.mass1 = p.mass1;
+    g.mass2 = p.mass2;
+    for (int i = 0; i < 3; i++) {
+        g.spin1[i] = p.spin1[i];
+        g.spin2[i] = p.spin2[i];
+    }
+    g.Luminosity_Distance = p.DL;
+    g.tc = p.tc;
+    g.phiRef = p.phiRef;
+    g.incl_angle = p.incl;
+    g.RA = p.RA;
+    g.DEC = p.DEC;
+    g.psi = p.psi;
+    g.chip = p.chip;
+    g.phip = p.phip;
+}
+
+inline double nan_with(int code) { return code == GWH_OK ? 0.0 : std::nan(""); }
+
+/* fourier_waveform (waveform_generator.h:43-49): h+ and hx into caller arrays of n bins */
+template <class Params>
+inline int fourier_waveform(const double *f, int n, std::complex<double> *hplus, std::complex<double> *hcross,
+                            const std::string &method, const Params *g)
+{
+    gwh_params p;
+    int model = model_id(method);
+    if (model < 0 || !to_params(*g, p)) return GWH_ERR_UNSUPPORTED;
+    return gwh_fourier_waveform(f, n, reinterpret_cast<double *>(hplus), reinterpret_cast<double *>(hcross), model,
+                                &p);
+}
+
+/* fourier_detector_response (waveform_util.h:472-479), terrestrial detectors */
+template <class Params>
+inline int fourier_detector_response(const double *f, int n, std::complex<double> *response, const std::string &det,
+                                     const std::string &method, const Params *g)
+{
+    gwh_params p;
+    int model = model_id(method), d = gwh_detector_id(det.c_str());
+    if (model < 0 || d < 0 || !to_params(*g, p)) return GWH_ERR_UNSUPPORTED;
+    return gwh_fourier_detector_response(f, n, reinterpret_cast<double *>(response), d, model, &p);
+}
+
+/* Log_Likelihood (mcmc_gw.h:210-217) */
+template <class Params>
+inline double Log_Likelihood(const std::complex<double> *data, const double *psd, const double *f, size_t n,
+                             const Params *g, const std::string &det, const std::string &method)
+{
+    gwh_params p;
+    int model = model_id(method), d = gwh_detector_id(det.c_str());
+    if (model < 0 || d < 0 || !to_params(*g, p)) return nan_with(GWH_ERR_UNSUPPORTED);
+    return gwh_log_likelihood(reinterpret_cast<const double *>(data), psd, f, n, &p, d, model);
+}
+
+/* Log_Likelihood_internal (mcmc_gw.h:276-284) */
+inline double Log_Likelihood_internal(const std::complex<double> *data, const double *psd, const double *f,
+                                      const double *weights, const std::complex<double> *response, int n, bool log10F,
+                                      const std::string &integration_method)
+{
+    return gwh_log_likelihood_internal(reinterpret_cast<const double *>(data), psd, f, weights,
+                                       reinterpret_cast<const double *>(response), n, log10F,
+                                       integration_id(integration_method));
+}
+
+/* MCMC_likelihood_extrinsic (mcmc_gw.h:477-490).  T_eff replaces the reference's pointer-difference
+ * time offset (mcmc_gw.cpp:2466): the waveform uses t_c = T_eff - tc. */
+template <class Params>
+inline double MCMC_likelihood_extrinsic(bool save_waveform, const Params *g, const std::string &method,
+                                        const int *data_length, double **frequencies,
+                                        std::complex<double> **data, double **psd, double **weights,
+                                        const std::string &integration_method, bool log10F,
+                                        const std::string *detectors, int num_detectors, double T_eff = 0.0)
+{
+    gwh_params p;
+    int model = model_id(method);
+    if (model < 0 || !to_params(*g, p) || num_detectors < 1 || num_detectors > 4) return nan_with(GWH_ERR_UNSUPPORTED);
+    int dets[4];
+    std::vector<const double *> dp(num_detectors);
+    for (int i = 0; i < num_detectors; i++) {
+        dets[i] = gwh_detector_id(detectors[i].c_str());
+        if (dets[i] < 0) return nan_with(GWH_ERR_UNSUPPORTED);
+        dp[i] = reinterpret_cast<const double *>(data[i]);
+    }
+    return gwh_mcmc_likelihood_extrinsic(save_waveform, &p, model, data_length, frequencies, dp.data(), psd, weights,
+                                         integration_id(integration_method), log10F, dets, num_detectors, T_eff);
+}
+
+/* fisher_numerical (fisher.h:35-48): output is the caller's dim x dim double** */
+template <class Params>
+inline void fisher_numerical(const double *f, int n, const std::string &method, const std::string &det,
+                             const std::string &ref_det, double **output, int dim, const Params *g, int order,
+                             const double *psd, double epsilon = 1e-8)
+{
+    gwh_params p;
+    int basis = basis_id(method, g->sky_average), d = gwh_detector_id(det.c_str()),
+        r = gwh_detector_id(ref_det.c_str());
+    std::vector<double> out((size_t)dim * dim, std::nan(""));
+    if (basis >= 0 && d >= 0 && r >= 0 && to_params(*g, p))
+        gwh_fisher_numerical(f, n, basis, d, r, out.data(), dim, &p, order, psd, epsilon);
+    for (int i = 0; i < dim; i++)
+        for (int j = 0; j < dim; j++) output[i][j] = out[(size_t)i * dim + j];
+}
+
+/* repack_parameters (fisher.cpp:2162-2511): basis vector -> the caller's parameter object */
+template <class Params>
+inline void repack_parameters(const double *v, Params *g, const std::string &method, int dim)
+{
+    gwh_params p;
+    int basis = basis_id(method, g->sky_average);
+    to_params(*g, p);
+    if (basis < 0 || gwh_repack_parameters(v, basis, &p) != GWH_OK) return;
+    (void)dim;
+    from_params(p, *g);
+}
+
+} // namespace gwh
+
+#endif

@@ -85,3 +85,48 @@ def test_unpack_repack_match_oracle(basis, method, dim, sky):
         assert abs(getattr(a, name) - getattr(b, name)) <= 1e-13 * max(1.0, abs(getattr(b, name))), name
     for i in range(3):
         assert abs(a.spin1[i] - b.spin1[i]) <= 1e-13 and abs(a.spin2[i] - b.spin2[i]) <= 1e-13
+
+
+# ---- the gwatpy C entry points (include/gwatpy_abi.h) ----
+
+def _declared_plain(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*_py(?:_destructor)?)\s*\(", text)))
+
+
+def test_gwatpy_entry_points_exported():
+    L = ctypes.CDLL(G.LIB_PATH)
+    names = _declared_plain("gwatpy_abi.h")
+    assert "MCMC_likelihood_extrinsic_py" in names and "gen_params_base_py_destructor" in names
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_gwatpy_host_helpers_and_errors():
+    from gw_analysis_tools_amd import gwatpy_compat as P
+    L = P.lib()
+    assert L.gps_to_GMST_radian_py(1126259462.4) == O.gps_to_gmst_radian(1126259462.4)
+    assert L.DTOA_DETECTOR_py(1.2, -0.3, 0.7, b"Hanford", b"Virgo") == O.dtoa(1.2, -0.3, 0.7, 0, 2)
+    assert np.isnan(L.DTOA_DETECTOR_py(1.2, -0.3, 0.7, b"Hanford", b"H1"))
+    f = np.linspace(20, 30, 5)
+    # fields the engine does not model are refused at use, before any device work
+    for kw in (dict(equatorial_orientation=True), dict(horizon_coord=True), dict(NSflag1=True), dict(Nmod=1)):
+        p = P.gen_params(mass1=30, mass2=20, **kw)
+        st, _, _ = P.fourier_waveform(f, "IMRPhenomD", p)
+        assert st == G.GWH_ERR_UNSUPPORTED, kw
+        assert b"does not model" in G.lib().gwh_last_error()
+        p.close()
+    p = P.gen_params(mass1=30, mass2=20)
+    st, _, _ = P.fourier_waveform(f, "ppE_IMRPhenomD_Inspiral", p)
+    assert st == G.GWH_ERR_UNSUPPORTED
+    st, _ = P.fourier_detector_response(f, "LIGO", "IMRPhenomD", p)
+    assert st == G.GWH_ERR_UNSUPPORTED
+    # a foreign pointer is not a handle
+    bad = type("H", (), {"handle": ctypes.c_void_p(12345).value})()
+    assert L.fourier_waveform_py(f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 5, None, None, None, None,
+                                 b"IMRPhenomD", bad.handle) == G.GWH_ERR_INVALID
+    # the 'K' letter maps to a name detector_util does not know
+    ll = P.mcmc_likelihood_extrinsic(p, "IMRPhenomD", f, [np.zeros(5, complex)], [np.ones(5)], "K")
+    assert np.isnan(ll)
+    p.close()

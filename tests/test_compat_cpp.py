@@ -1,0 +1,72 @@
+"""The header-only C++ layer (include/gwat_hip_compat.hpp) compiles against a parameter object shaped like
+the reference's gen_params_base<double> and maps its fields and strings onto the C ABI (host-side
+checks only; no device work)."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+import gw_analysis_tools_amd as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+SRC = r'''
+#include <cstdio>
+#include <cmath>
+#include "gwat_hip_compat.hpp"
+
+/* a stand-in with the field names (not the layout) of gen_params_base<double> */
+struct params_like {
+    double mass1 = 36, mass2 = 29, Luminosity_Distance = 410, spin1[3] = {0.1, 0.2, 0.3},
+           spin2[3] = {-0.1, 0, -0.2}, tc = 0.01, phiRef = 0.4, f_ref = 20, incl_angle = 2.9, RA = 1.37,
+           DEC = -1.26, psi = 2.66, gmst = 2.45, chip = -1, phip = -1;
+    bool shift_time = true, shift_phase = true, sky_average = false, dep_postmerger = false,
+         equatorial_orientation = false, horizon_coord = false, NSflag1 = false, NSflag2 = false;
+    int Nmod = 0, Nmod_phi = 0, Nmod_sigma = 0, Nmod_beta = 0, Nmod_alpha = 0;
+};
+
+int main()
+{
+    params_like g;
+    gwh_params p;
+    int ok = gwh::to_params(g, p);
+    printf("ok %d %.17g %.17g %.17g %.17g %.17g %d\n", ok, p.mass1, p.DL, p.spin2[2], p.incl, p.gmst, p.shift_time);
+    g.NSflag2 = true;
+    printf("ns %d\n", (int)gwh::to_params(g, p));
+    g.NSflag2 = false;
+    printf("models %d %d %d\n", gwh::model_id("IMRPhenomD"), gwh::model_id("IMRPhenomPv2"), gwh::model_id("PhenomD"));
+    printf("bases %d %d %d %d %d\n", gwh::basis_id("IMRPhenomD", true), gwh::basis_id("IMRPhenomD", false),
+           gwh::basis_id("MCMC_IMRPhenomD", false), gwh::basis_id("IMRPhenomPv2", false),
+           gwh::basis_id("MCMC_IMRPhenomPv2", false));
+    /* repack through the C ABI: the MCMC basis vector of these parameters, bumped, written back */
+    double v[11];
+    gwh::to_params(g, p);
+    gwh_unpack_parameters(&p, GWH_BASIS_PHENOMD_MCMC, v, nullptr);
+    v[6] += 0.1; /* ln DL */
+    gwh::repack_parameters(v, &g, "MCMC_IMRPhenomD", 11);
+    printf("repack %.17g %.17g\n", g.Luminosity_Distance, g.mass1);
+    return 0;
+}
+'''
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="no host C++ compiler")
+def test_compat_header_with_reference_shaped_params(tmp_path):
+    src = tmp_path / "t.cpp"
+    src.write_text(SRC)
+    exe = tmp_path / "t"
+    libdir = os.path.dirname(G.LIB_PATH)
+    cmd = ["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+           "-L", libdir, "-lgwat_hip", "-Wl,-rpath," + libdir, "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+    subprocess.check_call(cmd)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    f = out[0].split()
+    assert f[:2] == ["ok", "1"] and float(f[2]) == 36 and float(f[3]) == 410 and float(f[4]) == -0.2
+    assert float(f[5]) == 2.9 and float(f[6]) == 2.45 and f[7] == "1"
+    assert out[1] == "ns 0"
+    assert out[2] == "models 0 1 -1"
+    assert out[3] == "bases %d %d %d %d %d" % (G.BASIS_PHENOMD_SKYAVG, G.BASIS_PHENOMD_FULL, G.BASIS_PHENOMD_MCMC,
+                                             G.BASIS_PV2_FULL, G.BASIS_PV2_MCMC)
+    r = out[4].split()
+    assert abs(float(r[1]) - 410 * 2.718281828459045 ** 0.1) < 1e-9 and abs(float(r[2]) - 36) < 1e-9
