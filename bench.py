@@ -50,6 +50,103 @@ def fe_per_launch(counts, ndet, pv2=False):
     return float(fe.sum()), float(nz.sum())
 
 
+def bench_uneven(G, W, steps):
+    """256 events (total mass 5-200 Msun, durations 4-128 s, 20 Hz to min(2048, 0.3/M) Hz at df = 1/T, H/L/V),
+    8 chains each = 2048 instances per launch; the autotuner picks bins per lane per size bucket first."""
+    lib = G.lib()
+    ip = ctypes.POINTER(ctypes.c_int)
+    specs = W.config2()
+    g = W.rng(2, 1)
+    events, P, ids, nbins = [], [], [], []
+    t0 = time.perf_counter()
+    for sp in specs:
+        f, psd = sp["f"], sp["psd"]
+        inj = sp["injection"]
+        pinj = G.make_params(**{k: (list(inj[k]) if k in ("spin1", "spin2") else inj[k].item())
+                                for k in inj.dtype.names})
+        resp = G.create_coherent_GW_detection(f, pinj, sp["dets"])
+        ev = G.Event(f, [r + W.noise(f, psd, g) for r in resp], [psd] * 3, sp["dets"])
+        events.append(ev)
+        P.append(sp["params"])
+        ids += [ev.id] * len(sp["params"])
+        nbins += [len(f)] * len(sp["params"])
+    t_setup = time.perf_counter() - t0
+    P = np.concatenate(P)
+    ids = np.array(ids, dtype=np.int32)
+    nbins = np.array(nbins)
+    n = len(P)
+    d_params, d_ids = G.DeviceArray.from_host(P), G.DeviceArray.from_host(ids)
+    d_mixed, d_fp64 = G.DeviceArray(n * 8), G.DeviceArray(n * 8)
+    resident = sum(ev.nbytes for ev in events)
+    table = np.zeros(32, dtype=np.int32)
+    t0 = time.perf_counter()
+    G.check(lib.gwh_autotune_bins_per_lane(ids.ctypes.data_as(ip), d_ids.ptr, G.IMRPHENOMD, G.MIXED, d_params.ptr, n,
+                                           d_mixed.ptr, 3, table.ctypes.data_as(ip)))
+    t_tune = time.perf_counter() - t0
+
+    def launch(prec, out, idx=None):
+        G.check(lib.gwh_loglike_multi_device(ids.ctypes.data_as(ip), d_ids.ptr, G.IMRPHENOMD, prec, d_params.ptr, n,
+                                              out.ptr, None))
+
+    for _ in range(3):
+        launch(G.MIXED, d_mixed)
+        G.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        launch(G.MIXED, d_mixed)
+        G.synchronize()
+        lat.append(time.perf_counter() - a)
+    wall = time.perf_counter() - t0
+    launch(G.FP64, d_fp64)
+    G.synchronize()
+    dmax = float(np.max(np.abs(d_mixed.to_host(np.float64, n) - d_fp64.to_host(np.float64, n))))
+    t0 = time.perf_counter()
+    for _ in range(max(1, steps // 4)):
+        launch(G.FP64, d_fp64)
+        G.synchronize()
+    fp64_rate = max(1, steps // 4) * n / (time.perf_counter() - t0)
+    # per size bucket: the same instances as a batch of their own
+    buckets = {}
+    lg = np.floor(np.log2(nbins)).astype(int)
+    for b in sorted(set(lg)):
+        sel = np.nonzero(lg == b)[0]
+        Pb, idb = P[sel], ids[sel]
+        dpb, dib, dob = G.DeviceArray.from_host(Pb), G.DeviceArray.from_host(idb), G.DeviceArray(len(sel) * 8)
+        run = lambda: G.check(lib.gwh_loglike_multi_device(idb.ctypes.data_as(ip), dib.ptr, G.IMRPHENOMD, G.MIXED,
+                                                           dpb.ptr, len(sel), dob.ptr, None))
+        run()
+        G.synchronize()
+        a = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            run()
+            G.synchronize()
+        el = (time.perf_counter() - a) / reps
+        buckets[int(b)] = {"instances": int(len(sel)), "bins_per_instance_mean": float(nbins[sel].mean()),
+                           "bins_per_lane": int(table[b]), "ms_per_launch": el * 1e3,
+                           "bin_evals_per_s": float(nbins[sel].sum() / el)}
+        for d in (dpb, dib, dob):
+            d.free()
+    rec = {"instances_per_launch": n, "events": len(events), "bins_min": int(nbins.min()),
+           "bins_max": int(nbins.max()), "bins_total_per_launch": int(nbins.sum()),
+           "resident_event_bytes": int(resident), "setup_s": t_setup, "autotune_s": t_tune,
+           "autotuned_bins_per_lane": {int(i): int(table[i]) for i in range(32) if table[i]},
+           "mixed_evals_per_s": steps * n / wall, "mixed_bin_evals_per_s": steps * float(nbins.sum()) / wall,
+           "ms_per_launch": wall / steps * 1e3, "p50_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
+           "p99_ms": float(np.percentile(np.array(lat) * 1e3, 99)), "fp64_evals_per_s": fp64_rate,
+           "max_abs_dlogL_mixed_vs_fp64": dmax, "per_bucket": buckets}
+    log("uneven batch: %d instances, %d..%d bins, %.3f ms/launch, %.3e evals/s, %.3e bin-evals/s, max|dlogL| %.2e"
+        % (n, nbins.min(), nbins.max(), wall / steps * 1e3, steps * n / wall, steps * nbins.sum() / wall, dmax))
+    for d in (d_params, d_ids, d_mixed, d_fp64):
+        d.free()
+    for ev in events:
+        ev.close()
+    G.set_bins_per_lane(0)
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +157,8 @@ def main():
     ap.add_argument("--bpl", type=int, default=0)
     ap.add_argument("--legs", choices=["all", "mixed"], default="all",
                     help="mixed: only the timed mixed-precision leg (for counter runs)")
+    ap.add_argument("--no-c2", action="store_true", help="skip the uneven multi-event batch leg")
+    ap.add_argument("--c2-steps", type=int, default=20)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (reference point for traffic counters)")
     args = ap.parse_args()
@@ -236,6 +335,15 @@ def main():
         log("Pv2 leg failed:", e)
         pv2 = {"error": str(e)}
 
+    # ---- uneven batch: 256 events of 1e3..2.6e5 bins, 8 chains each, one launch per step ----
+    c2 = None
+    if not args.no_c2:
+        try:
+            c2 = bench_uneven(G, W, args.c2_steps)
+        except Exception as e:  # reported, not fatal for the headline
+            log("uneven-batch leg failed:", e)
+            c2 = {"error": str(e)}
+
     # ---- CPU baseline: the oracle (reference algorithm, C, one instance per call) ----
     cpu = None
     if not args.no_cpu:
@@ -276,6 +384,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "imrphenompv2_c3_likelihood": pv2,
+        "uneven_batch_c2": c2,
     }
     print(json.dumps(rec))
 

@@ -40,7 +40,7 @@ EXPORTS = [
     "gwh_memcpy_d2h", "gwh_work_counts", "gwh_create_coherent_GW_detection",
     "gwh_kernel_timing_start", "gwh_kernel_timing_stop", "gwh_device_copy",
     "gwh_fisher_derivatives", "gwh_unpack_parameters", "gwh_repack_parameters", "gwh_detector_id", "gwh_dtoa",
-    "gwh_gps_to_gmst_radian",
+    "gwh_gps_to_gmst_radian", "gwh_loglike_multi_device", "gwh_autotune_bins_per_lane",
 ]
 
 
@@ -134,6 +134,8 @@ def lib():
         "gwh_unpack_parameters": ([pp, I, dp, ip], I), "gwh_repack_parameters": ([dp, I, pp], I),
         "gwh_detector_id": ([ctypes.c_char_p], I), "gwh_dtoa": ([D, D, D, I, I], D),
         "gwh_gps_to_gmst_radian": ([D], D),
+        "gwh_loglike_multi_device": ([ip, V, I, I, V, I, V, V], I),
+        "gwh_autotune_bins_per_lane": ([ip, V, I, I, V, I, V, I, ip], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

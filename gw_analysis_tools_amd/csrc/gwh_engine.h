@@ -84,8 +84,10 @@ struct Engine {
     std::vector<Event> events;
     uint64_t use_clock = 0;
     int bpl = 0; /* 0 = automatic */
+    int tune[32] = {0}; /* autotuned bins per lane by floor(log2 N) of the event grid (0 = default rule) */
     /* scratch */
     DevBuf params, coef, partial, active, logl, inst_event, f, data, psd, wts, out1, out2, misc;
+    DevBuf work, wcount; /* flattened (instance, tile) lists of multi-event batches */
     DevBuf fisher_deriv, fisher_gram, fisher_out;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     /* live timing of the per-bin kernel inside a caller's timed region: event pairs around each launch */
@@ -106,7 +108,8 @@ int gwh_choose_bpl(int N);
 
 /* the batched likelihood on device buffers (engine lock held by caller) */
 int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
-                       double *d_logl, hipStream_t stream, bool timed, double *kernel_ms);
+                       double *d_logl, hipStream_t stream, bool timed, double *kernel_ms,
+                       const int *h_events = nullptr);
 
 /* per-instance setup on device buffers */
 int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_params *d_params, int n, InstCoef *d_coef,

@@ -19,6 +19,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -560,15 +561,15 @@ struct WorkMap {
     int event0, n, tiles0; /* tiles0: tile stride of the partial buffer */
 };
 
+/* One (instance, tile) work item: BPL x 256 bins of one instance's network log-likelihood, reduced
+ * to the instance's partial sums for that tile.  Every early return is uniform over the block. */
 template <int MODEL, int PREC, int BPL>
-__global__ __launch_bounds__(GWH_TPB) void gwh_loglike_kernel(WorkMap W, const EventDesc *__restrict__ events,
-                                                              const InstCoef *__restrict__ coefs,
-                                                              double *__restrict__ partial, const int *__restrict__ active)
+__device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDesc *__restrict__ events,
+                                                 const InstCoef *__restrict__ coefs, double *__restrict__ partial,
+                                                 const int *__restrict__ active, const int inst, const int tile,
+                                                 double (*lds)[GWH_PARTIAL_STRIDE])
 {
-    __shared__ double lds[GWH_TPB / 64][GWH_PARTIAL_STRIDE];
     {
-        const int inst = blockIdx.y;
-        const int tile = blockIdx.x;
         const int ev = W.inst_event ? W.inst_event[inst] : W.event0;
         const EventDesc &E = events[ev];
         if (E.bpl != BPL) return;           /* the instance's event is tiled by another launch */
@@ -719,6 +720,65 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_loglike_kernel(WorkMap W, const E
     }
 }
 
+/* the dense grid: tiles x instances (one event, or events whose sizes are known only on the device) */
+template <int MODEL, int PREC, int BPL>
+__global__ __launch_bounds__(GWH_TPB) void gwh_loglike_kernel(WorkMap W, const EventDesc *__restrict__ events,
+                                                              const InstCoef *__restrict__ coefs,
+                                                              double *__restrict__ partial, const int *__restrict__ active)
+{
+    __shared__ double lds[GWH_TPB / 64][GWH_PARTIAL_STRIDE];
+    gwh_loglike_tile<MODEL, PREC, BPL>(W, events, coefs, partial, active, blockIdx.y, blockIdx.x, lds);
+}
+
+/* the flattened (instance, tile) work list of one bins-per-lane class: one block per live work item,
+ * so batches mixing 10^3- and 10^5-bin events launch no empty blocks */
+template <int MODEL, int PREC, int BPL>
+__global__ __launch_bounds__(GWH_TPB) void gwh_loglike_wl_kernel(WorkMap W, const EventDesc *__restrict__ events,
+                                                                 const InstCoef *__restrict__ coefs,
+                                                                 double *__restrict__ partial,
+                                                                 const int *__restrict__ active,
+                                                                 const int2 *__restrict__ work,
+                                                                 const int *__restrict__ count)
+{
+    __shared__ double lds[GWH_TPB / 64][GWH_PARTIAL_STRIDE];
+    if ((int)blockIdx.x >= *count) return;
+    const int2 wi = work[blockIdx.x];
+    gwh_loglike_tile<MODEL, PREC, BPL>(W, events, coefs, partial, active, wi.x, wi.y, lds);
+}
+
+/* Work list of one bins-per-lane class (single block): instance i contributes active[i] items if its
+ * event is tiled with `bpl`; items are ordered by instance then tile.  At most `cap` are written. */
+__global__ __launch_bounds__(1024) void gwh_worklist_kernel(WorkMap W, const EventDesc *__restrict__ events,
+                                                            const int *__restrict__ active, int bpl,
+                                                            int2 *__restrict__ work, int *__restrict__ count, int cap)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x, n = W.n;
+    const int chunk = (n + 1023) / 1024;
+    const int i0 = min(n, t * chunk), i1 = min(n, i0 + chunk);
+    int local = 0;
+    for (int i = i0; i < i1; i++) {
+        int ev = W.inst_event ? W.inst_event[i] : W.event0;
+        if (events[ev].bpl == bpl) local += active[i];
+    }
+    part[t] = local;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) { /* inclusive scan */
+        int v = (t >= o) ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int off = part[t] - local;
+    for (int i = i0; i < i1; i++) {
+        int ev = W.inst_event ? W.inst_event[i] : W.event0;
+        if (events[ev].bpl != bpl) continue;
+        for (int k = 0; k < active[i]; k++, off++)
+            if (off < cap) work[off] = make_int2(i, k);
+    }
+    if (t == 1023) *count = min(part[1023], cap);
+}
+
 /* One wave per instance: lane l sums tiles l, l + 64, ... in order, then a fixed xor-shuffle tree; the
  * order depends only on the instance's own tile count, so results do not depend on the batch. */
 __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const EventDesc *__restrict__ events,
@@ -778,6 +838,9 @@ int gwh_choose_bpl(int N)
 {
     Engine &E = gwh_engine();
     if (E.bpl > 0) return E.bpl;
+    int lg = 0;
+    while ((2 << lg) <= N && lg < 31) lg++;
+    if (E.tune[lg] > 0) return E.tune[lg];
     /* 8 bins per lane amortise the per-tile reduction best (measured on the 16225-bin network grid:
      * 1/2/4/8 -> 92/96/77/72 us per 512-instance launch); short grids keep at least two tiles */
     if (N >= 2 * GWH_TPB * 8) return 8;
@@ -798,8 +861,21 @@ static void launch_bpl(int bpl, dim3 grid, hipStream_t st, const WorkMap &W, con
     }
 }
 
+template <int MODEL, int PREC>
+static void launch_wl(int bpl, int blocks, hipStream_t st, const WorkMap &W, const EventDesc *ev, const InstCoef *c,
+                      double *part, const int *act, const int2 *work, const int *count)
+{
+    dim3 g(blocks), b(GWH_TPB);
+    switch (bpl) {
+    case 1: hipLaunchKernelGGL((gwh_loglike_wl_kernel<MODEL, PREC, 1>), g, b, 0, st, W, ev, c, part, act, work, count); break;
+    case 2: hipLaunchKernelGGL((gwh_loglike_wl_kernel<MODEL, PREC, 2>), g, b, 0, st, W, ev, c, part, act, work, count); break;
+    case 4: hipLaunchKernelGGL((gwh_loglike_wl_kernel<MODEL, PREC, 4>), g, b, 0, st, W, ev, c, part, act, work, count); break;
+    default: hipLaunchKernelGGL((gwh_loglike_wl_kernel<MODEL, PREC, 8>), g, b, 0, st, W, ev, c, part, act, work, count); break;
+    }
+}
+
 int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
-                       double *d_logl, hipStream_t stream, bool timed, double *kernel_ms)
+                       double *d_logl, hipStream_t stream, bool timed, double *kernel_ms, const int *h_events)
 {
     Engine &E = gwh_engine();
     if (n <= 0) return GWH_OK;
@@ -818,7 +894,20 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
         int sl = slot_of(b);
         if (t > tiles_b[sl]) tiles_b[sl] = t;
     };
-    if (d_events) {
+    /* multi-event batch whose instance -> event map is known on the host: flattened work lists with
+     * per-class upper bounds (sum over instances of the event's tile count) */
+    const bool wl = d_events && h_events;
+    long long ub[4] = {0, 0, 0, 0};
+    if (wl) {
+        for (int i = 0; i < n; i++) {
+            const Event &ev = E.events[h_events[i]];
+            note(ev);
+            int b = ev.desc.bpl;
+            ub[slot_of(b)] += (ev.N + GWH_TPB * b - 1) / (GWH_TPB * b);
+        }
+        for (int c = 0; c < 4; c++)
+            if (ub[c] > (1ll << 30)) return gwh_set_error(GWH_ERR_INVALID, "work list too large");
+    } else if (d_events) {
         for (auto &ev : E.events)
             if (ev.live) note(ev);
     } else {
@@ -845,7 +934,32 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     double *pp = E.partial.as<double>();
     const int *ap = E.active.as<const int>();
     static const int kB[4] = {1, 2, 4, 8};
-    for (int sl = 0; sl < 4; sl++) {
+    if (wl) {
+        long long tot = ub[0] + ub[1] + ub[2] + ub[3];
+        if ((rc = E.work.ensure((size_t)tot * sizeof(int2))) != GWH_OK) return rc;
+        if ((rc = E.wcount.ensure(4 * sizeof(int))) != GWH_OK) return rc;
+        long long off = 0;
+        for (int sl = 0; sl < 4; sl++) {
+            if (!ub[sl]) continue;
+            int2 *wp = E.work.as<int2>() + off;
+            int *cnt = E.wcount.as<int>() + sl;
+            hipLaunchKernelGGL(gwh_worklist_kernel, dim3(1), dim3(1024), 0, stream, W, evp, ap, kB[sl], wp, cnt,
+                               (int)ub[sl]);
+            if (model == GWH_IMRPHENOMD) {
+                if (precision == GWH_FP64)
+                    launch_wl<GWH_IMRPHENOMD, GWH_FP64>(kB[sl], (int)ub[sl], stream, W, evp, cp, pp, ap, wp, cnt);
+                else
+                    launch_wl<GWH_IMRPHENOMD, GWH_MIXED>(kB[sl], (int)ub[sl], stream, W, evp, cp, pp, ap, wp, cnt);
+            } else {
+                if (precision == GWH_FP64)
+                    launch_wl<GWH_IMRPHENOMPV2, GWH_FP64>(kB[sl], (int)ub[sl], stream, W, evp, cp, pp, ap, wp, cnt);
+                else
+                    launch_wl<GWH_IMRPHENOMPV2, GWH_MIXED>(kB[sl], (int)ub[sl], stream, W, evp, cp, pp, ap, wp, cnt);
+            }
+            off += ub[sl];
+        }
+    }
+    for (int sl = 0; sl < 4 && !wl; sl++) {
         if (!tiles_b[sl]) continue;
         dim3 grid(tiles_b[sl], n);
         int bpl = kB[sl];
@@ -1134,7 +1248,7 @@ int gwh_loglike_batch(const int *events, int events0, int model, int precision, 
         events0 = events[0];
     }
     rc = gwh_loglike_launch(dev, events0, model, precision, E.params.as<const gwh_params>(), n, E.logl.as<double>(),
-                            E.stream, false, nullptr);
+                            E.stream, false, nullptr, events);
     if (rc != GWH_OK) return rc;
     GWH_HIP(hipMemcpyAsync(logl, E.logl.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, E.stream));
     GWH_HIP(hipStreamSynchronize(E.stream));
@@ -1169,6 +1283,97 @@ int gwh_time_loglike_batch_device(int events0, int model, int precision, const v
     GWH_HIP(hipEventDestroy(b));
     if (ms_per_launch) *ms_per_launch = ms / reps;
     if (kernel_ms) *kernel_ms = ksum / reps;
+    return GWH_OK;
+}
+
+int gwh_loglike_multi_device(const int *events, const int *d_events, int model, int precision, const void *d_params,
+                             int n, double *d_logl, void *stream)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    if (n <= 0) return GWH_OK;
+    if (!events || !d_events || !d_params || !d_logl) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    for (int i = 0; i < n; i++)
+        if (events[i] < 0 || events[i] >= GWH_MAX_EVENTS || !E.events[events[i]].live)
+            return gwh_set_error(GWH_ERR_INVALID, "unknown event in batch");
+    hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+    return gwh_loglike_launch(d_events, events[0], model, precision, (const gwh_params *)d_params, n, d_logl, s, false,
+                              nullptr, events);
+}
+
+/* Coordinate search over the bins per lane of each event-size bucket (floor(log2 N)): for every bucket
+ * present, time the whole batch with that bucket's events tiled at 1, 2, 4 and 8 bins per lane (the
+ * others held), keep the fastest (median of `reps`), and record it in the engine's table for later
+ * events of the same size.  table (optional, 32 ints) receives the choice per bucket (0 = absent). */
+int gwh_autotune_bins_per_lane(const int *events, const int *d_events, int model, int precision, const void *d_params,
+                               int n, double *d_logl, int reps, int *table)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    if (n <= 0 || !events || !d_events || !d_params || !d_logl) return gwh_set_error(GWH_ERR_INVALID, "bad arguments");
+    if (reps < 1) reps = 1;
+    std::vector<int> bucket_of(GWH_MAX_EVENTS, -1);
+    bool present[32] = {false};
+    for (int i = 0; i < n; i++) {
+        int e = events[i];
+        if (e < 0 || e >= GWH_MAX_EVENTS || !E.events[e].live)
+            return gwh_set_error(GWH_ERR_INVALID, "unknown event in batch");
+        int N = E.events[e].N, lg = 0;
+        while ((2 << lg) <= N && lg < 31) lg++;
+        bucket_of[e] = lg;
+        present[lg] = true;
+    }
+    auto retile = [&](int lg, int b) -> int {
+        for (int e = 0; e < GWH_MAX_EVENTS; e++) {
+            if (bucket_of[e] != lg) continue;
+            E.events[e].desc.bpl = b;
+            GWH_HIP(hipMemcpyAsync(E.d_events + e, &E.events[e].desc, sizeof(EventDesc), hipMemcpyHostToDevice,
+                                   E.stream));
+        }
+        return GWH_OK;
+    };
+    auto time_batch = [&](double *ms) -> int {
+        std::vector<float> t(reps);
+        for (int r = 0; r < reps; r++) {
+            GWH_HIP(hipEventRecord(E.ev0, E.stream));
+            int rc = gwh_loglike_launch(d_events, events[0], model, precision, (const gwh_params *)d_params, n, d_logl,
+                                        E.stream, false, nullptr, events);
+            if (rc != GWH_OK) return rc;
+            GWH_HIP(hipEventRecord(E.ev1, E.stream));
+            GWH_HIP(hipEventSynchronize(E.ev1));
+            GWH_HIP(hipEventElapsedTime(&t[r], E.ev0, E.ev1));
+        }
+        std::sort(t.begin(), t.end());
+        *ms = t[reps / 2];
+        return GWH_OK;
+    };
+    static const int cand[4] = {1, 2, 4, 8};
+    int rc;
+    double warm;
+    if ((rc = time_batch(&warm)) != GWH_OK) return rc; /* first-touch and code-load warm-up */
+    for (int lg = 0; lg < 32; lg++) {
+        if (!present[lg]) continue;
+        double best = 1e300;
+        int bestb = 8;
+        for (int b : cand) {
+            if ((rc = retile(lg, b)) != GWH_OK) return rc;
+            double ms;
+            if ((rc = time_batch(&ms)) != GWH_OK) return rc;
+            if (ms < best) {
+                best = ms;
+                bestb = b;
+            }
+        }
+        E.tune[lg] = bestb;
+        if ((rc = retile(lg, bestb)) != GWH_OK) return rc;
+    }
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    if (table)
+        for (int lg = 0; lg < 32; lg++) table[lg] = present[lg] ? E.tune[lg] : 0;
     return GWH_OK;
 }
 

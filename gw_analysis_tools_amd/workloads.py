@@ -152,3 +152,27 @@ def precessing_params(n, config=3, seed_stream=0):
         P[s][:, 2] = mag * ct
     P["chip"] = -1.0
     return P
+
+
+def config2(n_events=256, chains=8):
+    """Uneven batch: n_events sources with total mass log-uniform 5-200 Msun, q = m2/m1 uniform 0.1-1,
+    aligned spins uniform +-0.95, durations log-uniform 4-128 s; each grid runs from 20 Hz to
+    min(2048, 0.3 / (M G/c^3)) Hz at df = 1/T, with the analytic PSD on H/L/V.  Extrinsic parameters,
+    distance and reference frequency are those of the fixed injection.  Returns the event specs and the
+    `chains` parameter vectors per event (scattered around each injection)."""
+    g = rng(2, 0, stream=1)
+    M = np.exp(g.uniform(np.log(5.), np.log(200.), n_events))
+    q = g.uniform(0.1, 1.0, n_events)
+    c1 = g.uniform(-0.95, 0.95, n_events)
+    c2 = g.uniform(-0.95, 0.95, n_events)
+    T = np.exp(g.uniform(np.log(4.), np.log(128.), n_events))
+    events = []
+    for e in range(n_events):
+        df = 1.0 / T[e]
+        fmax = min(2048., 0.3 / (M[e] * MSUN_SEC))
+        f = 20. + df * np.arange(int(np.floor((fmax - 20.) / df)) + 1)
+        inj = injection_params(1, mass1=M[e] / (1 + q[e]), mass2=M[e] * q[e] / (1 + q[e]),
+                               spin1=(0., 0., c1[e]), spin2=(0., 0., c2[e]))[0]
+        P = perturbed_params(inj, chains, step=e, config=2, scale=0.05)
+        events.append(dict(f=f, psd=analytic_psd(f), dets=[0, 1, 2], injection=inj, params=P, T=T[e], M=M[e]))
+    return events

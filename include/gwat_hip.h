@@ -179,6 +179,20 @@ int gwh_loglike_batch(const int *events, int events0, int model, int precision, 
 int gwh_loglike_batch_device(const int *d_events, int events0, int model, int precision, const void *d_params,
                              int n, double *d_logl, void *stream);
 
+/* Multi-event batch on device buffers with the instance -> event map also given on the host
+ * (events: n ids, host; d_events: the same ids in device memory).  The host copy sizes one flattened
+ * (instance, tile) work list per bins-per-lane class, so events of very different lengths share one
+ * launch without empty work items. */
+int gwh_loglike_multi_device(const int *events, const int *d_events, int model, int precision, const void *d_params,
+                             int n, double *d_logl, void *stream);
+
+/* Autotuner: for each event-size bucket (floor(log2 N)) in the batch, time the batch with that bucket's
+ * events at 1, 2, 4 and 8 bins per lane and keep the fastest (median of reps); the choice is stored for
+ * later events of that size and written to table[32] (0 = bucket absent).  Results computed after the
+ * call may differ from earlier ones in the last bits (a different tiling sums in a different order). */
+int gwh_autotune_bins_per_lane(const int *events, const int *d_events, int model, int precision, const void *d_params,
+                               int n, double *d_logl, int reps, int *table);
+
 /* Tuning of the bins-per-lane of the likelihood kernels (1, 2, 4 or 8; 0 = automatic). */
 int gwh_set_bins_per_lane(int bpl);
 
