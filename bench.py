@@ -8,8 +8,8 @@ and the event (data, PSD, weights) are resident in HBM before the timed region; 
 enqueue of the batched engine followed by a stream synchronisation (a sampler reads its logL every
 step), so per-step latencies give p50/p99.
 
-`value` is the mixed-precision batched engine (FP64 phase, FP32 amplitude/projection/products with
-Kahan partials, FP64 reductions).  Reported beside it: the FP64 batched engine, the plain FP64
+`value` is the mixed-precision batched engine (FP64 phase, amplitude, products and sums; FP32 sin/cos
+of the detector phases and FP32 merger-ringdown exponential).  Reported beside it: the FP64 batched engine, the plain FP64
 one-instance-per-launch path, max |delta logL| mixed vs FP64 over every timed evaluation, the same
 for IMRPhenomPv2 on the same grid (1024 chains), kernel-level roofline from HIP events, and the CPU
 oracle (the reference algorithm restated in C, one instance per call, OpenMP over instances) timed on
@@ -355,7 +355,7 @@ def main():
             a = time.perf_counter()
             O.network_loglike_batch(f, data, [psd] * 3, dets, plist[:8], nthreads=1)
             per_eval_1t = (time.perf_counter() - a) / 8
-            m = int(max(cores, min(n * 4, args.cpu_seconds * cores / per_eval_1t)))
+            m = int(max(cores, args.cpu_seconds * cores / per_eval_1t))
             sample = [plist[i % n] for i in range(m)]
             a = time.perf_counter()
             out, used = O.network_loglike_batch(f, data, [psd] * 3, dets, sample, nthreads=cores)
@@ -373,7 +373,7 @@ def main():
     rec = {
         "metric": METRIC, "value": value, "unit": "logL evals/s", "n_gpus": 1, "steps": K, "warmup": Wm,
         "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 phase + f32 amplitude/products (mixed)", "data": "synthetic",
+        "dtype": "f64 with f32 sin/cos and exp (mixed)", "data": "synthetic",
         "config": {"workload": "PT ensemble step: IMRPhenomD, H1/L1/V1, 16225 bins 20-2048 Hz df=1/8 Hz, "
                                "16 temperatures x 32 chains = 512 instances per step",
                    "instances_per_step": n, "bins": len(f), "detectors": len(dets)},

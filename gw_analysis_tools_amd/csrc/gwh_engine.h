@@ -3,7 +3,7 @@
  *
  * One process-wide Engine per device holds the read-only model tables, the
  * resident events (frequency grid, Simpson/Gauss-Legendre weights folded with
- * 1/S, weighted data, FP32 copies) and grow-only scratch buffers, so the hot
+ * 1/S, weighted data, tabulated powers of f) and grow-only scratch buffers, so the hot
  * path never calls hipMalloc once sizes have been seen.
  */
 #ifndef GWH_ENGINE_H
@@ -26,12 +26,9 @@ struct EventDesc {
     const double *f;        /* N */
     const double *wS;       /* D x N : integration weight * 4 / S  (FP64) */
     const double2 *dwS;     /* D x N : data * wS */
-    const float *wS32;      /* D x N : wS * sw[d] */
-    const float2 *dwS32;    /* D x N : dwS * sd[d] */
     const double *chunk_fmin; /* ceil(N / GWH_TPB): smallest frequency of every 256-bin chunk */
     /* N each: log f, f^(1/3), f^(-1/3), f^(-7/6), f^(3/4) of every bin (gwh_bin_powers) */
     const double *lgf, *f13, *fm13, *fm76, *f34;
-    double sw[GWH_MAXDET], sd[GWH_MAXDET];
     int N, D, nchunk, sorted; /* sorted: frequencies ascending */
     int bpl;                  /* bins per lane of this event's tiles (fixed per event: results never depend
                                  on which other events share a batch) */

@@ -4,14 +4,24 @@
  *
  * Hot path (one MCMC step):
  *   1. gwh_setup_kernel      one lane per instance: FP64 InstCoef record
- *   2. loglike kernel        one block per (bin-tile, instance); tiles above
- *                            0.2/M of an ascending grid exit at once; one lane
- *                            per bin, the instance record is uniform (scalar
- *                            loads); per
- *                            detector accumulate |h|^2 w/S and h* d w/S; wave
- *                            shuffle + LDS tree per block -> per-item partials
- *   3. gwh_finalize_kernel   one lane per instance sums its items in a fixed
+ *   2. loglike kernel        one block per (bin-tile, instance) -- a dense
+ *                            tiles x instances grid for one event, flattened
+ *                            (instance, tile) work lists for multi-event
+ *                            batches; tiles above 0.2/M of an ascending grid
+ *                            do nothing; BPL bins per lane, the instance record
+ *                            uniform (scalar loads); per detector accumulate
+ *                            |h|^2 w/S and h* d w/S in FP64; wave shuffle + LDS
+ *                            tree per block -> per-item partials
+ *   3. gwh_finalize_kernel   one wave per instance sums its items in a fixed
  *                            order (deterministic, batch-composition free)
+ *
+ * Precision modes: GWH_FP64 evaluates everything in FP64.  GWH_MIXED differs
+ * only in the transcendentals that dominate the per-bin cost: sin/cos of the
+ * per-detector phase (FP64 reduction to turns, FP32 sincospi, first-order
+ * correction of the rounded argument) and the merger-ringdown exponential
+ * (FP32 exp2 with an FP64 exponent split).  Phase, amplitude polynomials,
+ * products and sums stay FP64: with packed FP32 disabled, FP32 and FP64 FMA
+ * issue at the same rate on gfx950, so FP32 products would only add error.
  *
  * Reference: src/mcmc_gw.cpp:2374-2489 (MCMC_likelihood_extrinsic),
  * :801-868 (Log_Likelihood_internal), src/waveform_util.cpp:152-184
@@ -201,18 +211,6 @@ __global__ void gwh_event_prep_kernel(const double *__restrict__ f, const double
     dwS[(size_t)d * N + k] = make_double2(x.x * s, x.y * s);
 }
 
-__global__ void gwh_event_scale_kernel(const double *__restrict__ wS, const double2 *__restrict__ dwS, int N,
-                                       const double *__restrict__ sw, const double *__restrict__ sd,
-                                       float *__restrict__ wS32, float2 *__restrict__ dwS32)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int d = blockIdx.y;
-    if (k >= N) return;
-    size_t i = (size_t)d * N + k;
-    wS32[i] = (float)(wS[i] * sw[d]);
-    double2 x = dwS[i];
-    dwS32[i] = make_float2((float)(x.x * sd[d]), (float)(x.y * sd[d]));
-}
 
 __global__ void gwh_chunk_fmin_kernel(const double *__restrict__ f, int N, double *__restrict__ out)
 {
@@ -253,8 +251,7 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     int nchunk = (N + GWH_TPB - 1) / GWH_TPB;
     size_t Nd = (size_t)N * ndet;
     size_t off_f = 0, off_wS = align256(off_f + N * 8), off_dwS = align256(off_wS + Nd * 8),
-           off_w32 = align256(off_dwS + Nd * 16), off_dw32 = align256(off_w32 + Nd * 4),
-           off_cm = align256(off_dw32 + Nd * 8), off_pw = align256(off_cm + nchunk * 8),
+           off_cm = align256(off_dwS + Nd * 16), off_pw = align256(off_cm + nchunk * 8),
            total = align256(off_pw + 5 * align256((size_t)N * 8));
     const size_t pw_stride = align256((size_t)N * 8);
     char *blk = nullptr;
@@ -263,7 +260,6 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     int rc = E.psd.ensure(Nd * 8);
     if (rc == GWH_OK) rc = E.data.ensure(Nd * 16);
     if (rc == GWH_OK && weights) rc = E.wts.ensure((size_t)N * 8);
-    if (rc == GWH_OK) rc = E.misc.ensure(2 * GWH_MAXDET * 8);
     if (rc != GWH_OK) {
         (void)hipFree(blk);
         return rc;
@@ -277,29 +273,6 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     hipLaunchKernelGGL(gwh_event_prep_kernel, grid, dim3(GWH_TPB), 0, E.stream, (const double *)(blk + off_f),
                        E.psd.as<const double>(), E.data.as<const double2>(), E.wts.as<const double>(), N, ndet, method,
                        log10F, dfac, (double *)(blk + off_wS), (double2 *)(blk + off_dwS));
-    GWH_HIP(hipGetLastError());
-    /* FP32 scales: largest weight and largest |weighted data| map to 1 */
-    std::vector<double> hwS(Nd), hdw(2 * Nd);
-    GWH_HIP(hipMemcpyAsync(hwS.data(), blk + off_wS, Nd * 8, hipMemcpyDeviceToHost, E.stream));
-    GWH_HIP(hipMemcpyAsync(hdw.data(), blk + off_dwS, Nd * 16, hipMemcpyDeviceToHost, E.stream));
-    GWH_HIP(hipStreamSynchronize(E.stream));
-    double sc[2 * GWH_MAXDET];
-    for (int d = 0; d < GWH_MAXDET; d++) sc[d] = sc[GWH_MAXDET + d] = 1.0;
-    for (int d = 0; d < ndet; d++) {
-        double mw = 0, md = 0;
-        for (int k = 0; k < N; k++) {
-            double a = std::fabs(hwS[(size_t)d * N + k]);
-            if (std::isfinite(a) && a > mw) mw = a;
-            double b = std::hypot(hdw[2 * ((size_t)d * N + k)], hdw[2 * ((size_t)d * N + k) + 1]);
-            if (std::isfinite(b) && b > md) md = b;
-        }
-        sc[d] = (mw > 0) ? 1.0 / mw : 1.0;
-        sc[GWH_MAXDET + d] = (md > 0) ? 1.0 / md : 1.0;
-    }
-    GWH_HIP(hipMemcpyAsync(E.misc.p, sc, sizeof(sc), hipMemcpyHostToDevice, E.stream));
-    hipLaunchKernelGGL(gwh_event_scale_kernel, grid, dim3(GWH_TPB), 0, E.stream, (const double *)(blk + off_wS),
-                       (const double2 *)(blk + off_dwS), N, E.misc.as<const double>(),
-                       E.misc.as<const double>() + GWH_MAXDET, (float *)(blk + off_w32), (float2 *)(blk + off_dw32));
     GWH_HIP(hipGetLastError());
     hipLaunchKernelGGL(gwh_chunk_fmin_kernel, dim3(nchunk), dim3(GWH_TPB), 0, E.stream,
                        (const double *)(blk + off_f), N, (double *)(blk + off_cm));
@@ -324,8 +297,6 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     ds.f = (const double *)(blk + off_f);
     ds.wS = (const double *)(blk + off_wS);
     ds.dwS = (const double2 *)(blk + off_dwS);
-    ds.wS32 = (const float *)(blk + off_w32);
-    ds.dwS32 = (const float2 *)(blk + off_dw32);
     ds.chunk_fmin = (const double *)(blk + off_cm);
     ds.lgf = (const double *)(blk + off_pw);
     ds.f13 = (const double *)(blk + off_pw + pw_stride);
@@ -345,8 +316,6 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     for (int d = 0; d < GWH_MAXDET; d++) {
         ds.det[d] = (d < ndet) ? dets[d] : dets[0];
         ev.det[d] = ds.det[d];
-        ds.sw[d] = sc[d];
-        ds.sd[d] = sc[GWH_MAXDET + d];
     }
     GWH_HIP(hipMemcpyAsync(E.d_events + slot, &ds, sizeof(ds), hipMemcpyHostToDevice, E.stream));
     GWH_HIP(hipStreamSynchronize(E.stream));
@@ -455,30 +424,22 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double *__
     __syncthreads();
 }
 
-struct KahanF {
-    float s, c;
-    __device__ __forceinline__ void zero()
-    {
-        s = 0.f;
-        c = 0.f;
-    }
-    __device__ __forceinline__ void add(float x)
-    {
-        float y = x - c;
-        float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
-};
 
 /* sin and cos of an FP64 phase: FP64 reduction to a fraction of a turn in [-1/2, 1/2], then the FP32
- * sin(pi x), cos(pi x) of x = 2 * fraction (exact scaling; a short polynomial, no large-argument path) */
+ * sin(pi x), cos(pi x) of x = 2 * fraction (exact scaling; a short polynomial, no large-argument path).
+ * The part of x lost in rounding it to FP32 (up to 6e-8 half-turns, i.e. 1.9e-7 rad, the largest single
+ * error of the mixed path) is restored to first order: sin(a + d) = sin a + d cos a, cos(a + d) = cos a - d sin a. */
 __device__ __forceinline__ void sincos_turns(double phase, float &s, float &c)
 {
     double r = phase * 0.15915494309189533577;
     r = r - rint(r);
-    sincospif((float)(2.0 * r), &s, &c);
+    const double x = 2.0 * r;
+    const float xf = (float)x;
+    const float d = (float)(x - (double)xf) * 3.14159265358979f;
+    float s0, c0;
+    sincospif(xf, &s0, &c0);
+    s = fmaf(d, c0, s0);
+    c = fmaf(-d, s0, c0);
 }
 
 /* e^x for x <= 0 of moderate size: FP64 split x log2(e) = n + t, FP32 2^t, exact scaling by 2^n */
@@ -503,10 +464,10 @@ __device__ __forceinline__ double rsqrt_mixed(double z)
     return y;
 }
 
-/* IMRPhenomD in mixed precision.  Phase: FP64 arithmetic with FP64 log (inspiral, intermediate) and an
- * FP32 arctangent for the small merger-ringdown atan term.  Amplitude: FP64 polynomials, the powers of Mf
- * from Newton-refined FP32 seeds and an FP32 exponential (merger-ringdown), rounded once to FP32. */
-__device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, const BinPow &bp, float &a, double &phi,
+/* IMRPhenomD in mixed precision: the FP64 evaluator with an FP32 exponential in the merger-ringdown
+ * amplitude (the phase, including its arctangent, stays FP64: an FP32 atan there was the largest error
+ * of the path, ~7e-4 in logL for heavy binaries). */
+__device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, const BinPow &bp, double &a, double &phi,
                                                   double &u_out, double &lx_out)
 {
     double Mf = C.M * f;
@@ -525,7 +486,7 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, c
     } else {
         ar = C.delta[0] + Mf * (C.delta[1] + Mf * (C.delta[2] + Mf * (C.delta[3] + Mf * C.delta[4])));
     }
-    a = (float)(ar * inv76);
+    a = C.amp_pref * ar * inv76;
     double psi;
     double lx = 0;
     if (f < C.f1p) {
@@ -542,7 +503,7 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, c
               (C.sig[0] * Mf + C.sig[1] * Mf43 + C.sig[2] * Mf43 * u + C.sig[3] * Mf * Mf);
     } else if (f > C.f2p) {
         psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] * (r2 * r) + C.alpha[3] * (C.M34 * bp.f34) +
-              C.alpha[4] * (double)atanf((float)((f - C.alpha[5]) / C.fdamp));
+              C.alpha[4] * atan((f - C.alpha[5]) / C.fdamp);
     } else {
         double r3 = r2 * r;
         psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] * (r3 * r3 * r3);
@@ -626,9 +587,6 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                     }
                 }
             } else {
-                KahanF k3[GWH_PARTIAL_STRIDE];
-#pragma unroll
-                for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) k3[j].zero();
 #pragma unroll
                 for (int j = 0; j < BPL; j++) {
                     int k = tile * GWH_TPB * BPL + j * GWH_TPB + threadIdx.x;
@@ -637,25 +595,24 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                     if (f > C.fcut) continue;
                     const BinPow bp = gwh_load_bin(E, k);
                     if (MODEL == GWH_IMRPHENOMD) {
-                        float a;
-                        double phi, u, lx;
+                        double a, phi, u, lx;
                         gwh_phenomd_mixed(C, f, bp, a, phi, u, lx);
 #pragma unroll
                         for (int d = 0; d < GWH_MAXDET; d++) {
                             if (d >= D) break;
-                            float s, c;
-                            sincos_turns(phi + C.det[d][3] * f, s, c);
-                            float ws = E.wS32[(size_t)d * N + k];
-                            float2 dw = E.dwS32[(size_t)d * N + k];
-                            k3[3 * d].add(a * a * ws);
-                            k3[3 * d + 1].add(a * (c * dw.x - s * dw.y));
-                            k3[3 * d + 2].add(a * (c * dw.y + s * dw.x));
+                            float sf, cf;
+                            sincos_turns(phi + C.det[d][3] * f, sf, cf);
+                            const double s = sf, c = cf;
+                            const double ws = E.wS[(size_t)d * N + k];
+                            const double2 dw = E.dwS[(size_t)d * N + k];
+                            acc[3 * d] += a * a * ws;
+                            acc[3 * d + 1] += a * (c * dw.x - s * dw.y);
+                            acc[3 * d + 2] += a * (c * dw.y + s * dw.x);
                         }
                     } else {
-                        float a;
-                        double phi, u, lx;
+                        double a, phi, u, lx;
                         gwh_phenomd_mixed(C, f, bp, a, phi, u, lx);
-                        /* precession in FP64 for the angles, FP32 for the Wigner factors */
+                        /* precession angles in FP64, sin/cos of the Euler angle in FP32 */
                         double v = u * 1.4645918875615231;
                         double logw = C.lgPiM + bp.lf; /* log(pi M f) */
                         double x = v * v;
@@ -695,7 +652,6 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                             hci += t2r - tmr;
                         }
                         double Phi = phi + 2. * eps + C.t_corr_2pi * f;
-                        double ad = a;
 #pragma unroll
                         for (int d = 0; d < GWH_MAXDET; d++) {
                             if (d >= D) break;
@@ -703,17 +659,14 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                             double Gr = P * hpr + Q * hcr, Gi = P * hpi + Q * hci;
                             float sf, cf;
                             sincos_turns(Phi + C.det[d][3] * f, sf, cf);
-                            double s = sf, c = cf;
-                            double ws = E.wS32[(size_t)d * N + k];
-                            float2 dwf = E.dwS32[(size_t)d * N + k];
-                            double dwx = dwf.x, dwy = dwf.y;
-                            k3[3 * d].add((float)(ad * ad * (Gr * Gr + Gi * Gi) * ws));
-                            k3[3 * d + 1].add((float)(ad * (dwx * (Gr * c + Gi * s) - dwy * (Gr * s - Gi * c))));
+                            const double s = sf, c = cf;
+                            const double ws = E.wS[(size_t)d * N + k];
+                            const double2 dw = E.dwS[(size_t)d * N + k];
+                            acc[3 * d] += a * a * (Gr * Gr + Gi * Gi) * ws;
+                            acc[3 * d + 1] += a * (dw.x * (Gr * c + Gi * s) - dw.y * (Gr * s - Gi * c));
                         }
                     }
                 }
-#pragma unroll
-                for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) acc[j] = k3[j].value();
             }
         }
         block_reduce_store<GWH_PARTIAL_STRIDE>(acc, partial + (size_t)w * GWH_PARTIAL_STRIDE, lds, 3 * D);
@@ -814,20 +767,11 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const 
         double hh, dh;
         if (model == GWH_IMRPHENOMD) {
             double ha = s[3 * d], zr = s[3 * d + 1], zi = s[3 * d + 2];
-            if (prec == GWH_MIXED) {
-                ha *= C.amp_pref * C.amp_pref / E.sw[d];
-                zr *= C.amp_pref / E.sd[d];
-                zi *= C.amp_pref / E.sd[d];
-            }
             hh = C.det[d][2] * ha;
             dh = C.det[d][0] * zr + C.det[d][1] * zi;
         } else {
             hh = s[3 * d];
             dh = s[3 * d + 1];
-            if (prec == GWH_MIXED) {
-                hh *= C.amp_pref * C.amp_pref / E.sw[d];
-                dh *= C.amp_pref / E.sd[d];
-            }
         }
         ll += -0.5 * (hh - 2 * dh);
     }

@@ -40,7 +40,7 @@ extern "C" {
 
 /* arithmetic of the batched kernels */
 #define GWH_FP64 0  /* everything in FP64 */
-#define GWH_MIXED 1 /* FP64 phase and reductions, FP32 amplitude/projection/products */
+#define GWH_MIXED 1 /* FP64 phase, amplitude, products and sums; FP32 sin/cos and merger-ringdown exp */
 
 /* integration rules of Log_Likelihood_internal (mcmc_gw.cpp:801-868) */
 #define GWH_SIMPSONS 0
