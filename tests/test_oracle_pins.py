@@ -241,3 +241,46 @@ def test_pv2_precessing_is_finite_and_continuous():
     a = np.abs(hp[nz]) ** 2 + np.abs(hc[nz]) ** 2
     rel = np.abs(np.diff(a)) / a[:-1]
     assert np.max(rel) < 0.1
+
+
+def test_gmst_against_meeus_series():
+    """gps_to_GMST_radian (util.cpp:1797-1846) against the IAU 1982 sidereal-time series (Meeus,
+    Astronomical Algorithms, eq. 12.4) with UTC = GPS - leap seconds.  The reference treats GPS seconds
+    as UT (no leap seconds), so agreement is to ~1e-3 rad; a wrong epoch, rate or hour wrap fails."""
+    for gps, leap in [(1126259462.4, 17.), (1187008882.4, 18.), (630763213., 13.), (1300000000., 18.)]:
+        jd = 2444244.5 + (gps - leap) / 86400.  # GPS epoch 1980-01-06 00:00 UTC
+        T = (jd - 2451545.0) / 36525.
+        deg = 280.46061837 + 360.98564736629 * (jd - 2451545.0) + 0.000387933 * T * T - T ** 3 / 38710000.
+        ref = np.deg2rad(deg % 360.)
+        got = O.gps_to_gmst_radian(gps)
+        d = (got - ref + np.pi) % (2 * np.pi) - np.pi
+        assert abs(d) < 2e-3, (gps, got, ref)
+
+
+@pytest.mark.parametrize("method,dim,sky", [("IMRPhenomD", 7, 1), ("IMRPhenomD", 11, 0), ("MCMC_IMRPhenomD", 11, 0),
+                                            ("IMRPhenomPv2", 13, 0), ("MCMC_IMRPhenomPv2", 15, 0)])
+def test_unpack_repack_round_trip_and_closed_forms(method, dim, sky):
+    """repack(unpack(p)) == p, and the basis entries equal their closed forms (chirp mass, eta, logs)"""
+    kw = dict(mass1=36., mass2=29., spin1=[0.2, -0.1, 0.3], spin2=[0.05, 0.3, -0.1], DL=410., tc=0.01, phiRef=0.3,
+              incl=2.9, RA=1.37, DEC=-1.26, psi=2.66, f_ref=20., sky_average=sky)
+    if method == "IMRPhenomPv2":
+        kw.update(chip=0.4, phip=1.2)
+    p = O.make_params(**kw)
+    v, lf = O.unpack_parameters(p, method, dim)
+    mc = (36. * 29.) ** 0.6 / 65. ** 0.2
+    eta = 36. * 29. / 65. ** 2
+    if dim == 7:
+        assert abs(v[3] - mc) < 1e-12 and abs(v[4] - eta) < 1e-15 and list(np.nonzero(lf)[0]) == [0, 3, 4]
+    elif method.startswith("MCMC"):
+        assert abs(v[6] - np.log(410.)) < 1e-14 and abs(v[7] - np.log(mc)) < 1e-14 and abs(v[1] - np.sin(-1.26)) < 1e-15
+        assert not lf.any()
+    else:
+        assert abs(v[6] - 410.) < 1e-12 and abs(v[7] - mc) < 1e-12 and list(np.nonzero(lf)[0]) == [6, 7]
+    q = O.repack_parameters(v, method, dim, p)
+    for name in ("mass1", "mass2", "DL", "tc", "phiRef"):
+        assert abs(getattr(q, name) - getattr(p, name)) <= 1e-10 * abs(getattr(p, name)) + 1e-14, name
+    if dim > 7:
+        for name in ("RA", "DEC", "psi", "incl"):
+            assert abs(getattr(q, name) - getattr(p, name)) < 1e-12, name
+    if method == "MCMC_IMRPhenomPv2":
+        assert np.allclose(list(q.spin1), kw["spin1"], atol=1e-12) and np.allclose(list(q.spin2), kw["spin2"], atol=1e-12)
