@@ -147,6 +147,41 @@ def bench_uneven(G, W, steps):
     return rec
 
 
+def bench_fisher(G, W, ev, n_points=2048):
+    """IMRPhenomPv2 13-parameter Fisher matrices at n_points precessing parameter points on the resident
+    3-detector event: order-2 central differences with FP64 numerators, Gram on the matrix cores (FP64, or
+    FP32 operands with FP64 accumulation across tiles), FP64 LU inversion -> marginal sigmas."""
+    P = W.precessing_params(n_points, seed_stream=7)
+    g = W.rng(3, 0, stream=8)
+    P["chip"] = g.uniform(0.05, 0.9, n_points)
+    P["phip"] = g.uniform(0, 2 * np.pi, n_points)
+    G.fisher_batch(ev, P[:8], G.IMRPHENOMPV2, G.BASIS_PV2_FULL, 13, precision=G.MIXED)  # warm-up
+    out = {}
+    res = {}
+    for name, prec in (("mixed", G.MIXED), ("fp64", G.FP64)):
+        a = time.perf_counter()
+        F, sig = G.fisher_batch(ev, P, G.IMRPHENOMPV2, G.BASIS_PV2_FULL, 13, precision=prec, with_sigma=True)
+        el = time.perf_counter() - a
+        res[name] = (F, sig)
+        out[name + "_points_per_s"] = n_points / el
+        out[name + "_s"] = el
+    (Fm, sm), (Fd, sd) = res["mixed"], res["fp64"]
+    dg = np.sqrt(np.abs(np.einsum("nii->ni", Fd)))
+    norm = np.abs(Fm - Fd) / (dg[:, :, None] * dg[:, None, :])
+    rel = np.abs(Fm - Fd) / np.maximum(np.abs(Fd), 1e-300)
+    fin = np.isfinite(sm) & np.isfinite(sd)
+    srel = np.abs(sm[fin] / sd[fin] - 1)
+    out.update({"points": n_points, "dim": 13, "bins": int(ev.n), "detectors": int(ev.ndet),
+                "mixed_vs_fp64_max_normalised_element_error": float(norm.max()),
+                "mixed_vs_fp64_median_element_rel_error": float(np.median(rel)),
+                "mixed_vs_fp64_sigma_rel_error_median": float(np.median(srel)) if srel.size else None,
+                "mixed_vs_fp64_sigma_rel_error_max": float(srel.max()) if srel.size else None,
+                "finite_sigma_fraction_fp64": float(np.isfinite(sd).mean())})
+    log("Fisher (Pv2, 13 params, %d points): mixed %.1f points/s, fp64 %.1f points/s, max normalised element "
+        "error %.2e" % (n_points, out["mixed_points_per_s"], out["fp64_points_per_s"], norm.max()))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +193,7 @@ def main():
     ap.add_argument("--legs", choices=["all", "mixed"], default="all",
                     help="mixed: only the timed mixed-precision leg (for counter runs)")
     ap.add_argument("--no-c2", action="store_true", help="skip the uneven multi-event batch leg")
+    ap.add_argument("--no-fisher", action="store_true", help="skip the batched Fisher leg")
     ap.add_argument("--c2-steps", type=int, default=20)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (reference point for traffic counters)")
@@ -335,6 +371,15 @@ def main():
         log("Pv2 leg failed:", e)
         pv2 = {"error": str(e)}
 
+    # ---- batched Fisher matrices (IMRPhenomPv2, 13 parameters, 2048 points) on the same event ----
+    fisher = None
+    if not args.no_fisher:
+        try:
+            fisher = bench_fisher(G, W, ev)
+        except Exception as e:  # reported, not fatal for the headline
+            log("Fisher leg failed:", e)
+            fisher = {"error": str(e)}
+
     # ---- uneven batch: 256 events of 1e3..2.6e5 bins, 8 chains each, one launch per step ----
     c2 = None
     if not args.no_c2:
@@ -385,6 +430,7 @@ def main():
         "cpu_baseline": cpu,
         "imrphenompv2_c3_likelihood": pv2,
         "uneven_batch_c2": c2,
+        "fisher_c3": fisher,
     }
     print(json.dumps(rec))
 

@@ -41,6 +41,7 @@ EXPORTS = [
     "gwh_kernel_timing_start", "gwh_kernel_timing_stop", "gwh_device_copy",
     "gwh_fisher_derivatives", "gwh_unpack_parameters", "gwh_repack_parameters", "gwh_detector_id", "gwh_dtoa",
     "gwh_gps_to_gmst_radian", "gwh_loglike_multi_device", "gwh_autotune_bins_per_lane",
+    "gwh_calculate_snr_internal", "gwh_data_snr", "gwh_snr_batch",
 ]
 
 
@@ -136,6 +137,9 @@ def lib():
         "gwh_gps_to_gmst_radian": ([D], D),
         "gwh_loglike_multi_device": ([ip, V, I, I, V, I, V, V], I),
         "gwh_autotune_bins_per_lane": ([ip, V, I, I, V, I, V, I, ip], I),
+        "gwh_calculate_snr_internal": ([dp, dp, dp, I, I, dp, I], D),
+        "gwh_data_snr": ([dp, I, dp, dp, dp], D),
+        "gwh_snr_batch": ([ip, I, I, I, V, I, dp], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

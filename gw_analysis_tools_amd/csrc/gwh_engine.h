@@ -30,6 +30,7 @@ struct EventDesc {
     /* N each: log f, f^(1/3), f^(-1/3), f^(-7/6), f^(3/4) of every bin (gwh_bin_powers) */
     const double *lgf, *f13, *fm13, *fm76, *f34;
     int N, D, nchunk, sorted; /* sorted: frequencies ascending */
+    double snr_scale;         /* (f[1]-f[0]) / (weights' df): the SNR's Simpson step (waveform_util.cpp:497) */
     int bpl;                  /* bins per lane of this event's tiles (fixed per event: results never depend
                                  on which other events share a batch) */
     int det[GWH_MAXDET];
@@ -106,7 +107,7 @@ int gwh_choose_bpl(int N);
 /* the batched likelihood on device buffers (engine lock held by caller) */
 int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
                        double *d_logl, hipStream_t stream, bool timed, double *kernel_ms,
-                       const int *h_events = nullptr);
+                       const int *h_events = nullptr, int what = 0);
 
 /* per-instance setup on device buffers */
 int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_params *d_params, int n, InstCoef *d_coef,

@@ -291,7 +291,7 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     /* the stored weights carry the step of df_mode (df_like below); the Fisher rescales to f[1]-f[0] */
     double df_l = f[N / 2] - f[N / 2 - 1], df_f = f[1] - f[0];
     ev.df_like = (df_mode == 0) ? df_l : df_f;
-    ev.df_fisher = df_f;
+    ev.df_fisher = (method == GWH_GAUSSLEG) ? ev.df_like : df_f;
     EventDesc &ds = ev.desc;
     memset(&ds, 0, sizeof(ds));
     ds.f = (const double *)(blk + off_f);
@@ -307,6 +307,8 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     ds.D = ndet;
     ds.nchunk = nchunk;
     ds.bpl = gwh_choose_bpl(N);
+    /* Gauss-Legendre weights carry no step: nothing to rescale */
+    ds.snr_scale = (method == GWH_GAUSSLEG) ? 1.0 : df_f / ev.df_like;
     ds.sorted = 1;
     for (int k = 1; k < N; k++)
         if (!(f[k] >= f[k - 1])) {
@@ -737,7 +739,7 @@ __global__ __launch_bounds__(1024) void gwh_worklist_kernel(WorkMap W, const Eve
 __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const EventDesc *__restrict__ events,
                                                                const InstCoef *__restrict__ coefs,
                                                                const double *__restrict__ partial,
-                                                               const int *__restrict__ active, int model, int prec,
+                                                               const int *__restrict__ active, int model, int what,
                                                                double *__restrict__ logl)
 {
     const int i = blockIdx.x * (GWH_TPB / 64) + (threadIdx.x >> 6);
@@ -773,9 +775,10 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const 
             hh = s[3 * d];
             dh = s[3 * d + 1];
         }
-        ll += -0.5 * (hh - 2 * dh);
+        ll += (what == 0) ? -0.5 * (hh - 2 * dh) : hh;
     }
-    logl[i] = ll;
+    /* what 1: network optimal SNR sqrt(sum_d (h_d|h_d)) with the SNR's Simpson step */
+    logl[i] = (what == 0) ? ll : sqrt(ll * E.snr_scale);
 }
 
 int gwh_choose_bpl(int N)
@@ -819,7 +822,8 @@ static void launch_wl(int bpl, int blocks, hipStream_t st, const WorkMap &W, con
 }
 
 int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
-                       double *d_logl, hipStream_t stream, bool timed, double *kernel_ms, const int *h_events)
+                       double *d_logl, hipStream_t stream, bool timed, double *kernel_ms, const int *h_events,
+                       int what)
 {
     Engine &E = gwh_engine();
     if (n <= 0) return GWH_OK;
@@ -924,7 +928,7 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     if (live) GWH_HIP(hipEventRecord(E.kt_ev[2 * E.kt_used++ + 1], stream));
     hipLaunchKernelGGL(gwh_finalize_kernel, dim3((n + GWH_TPB / 64 - 1) / (GWH_TPB / 64)), dim3(GWH_TPB), 0, stream,
                        W, evp, cp,
-                       (const double *)pp, ap, model, precision, d_logl);
+                       (const double *)pp, ap, model, what, d_logl);
     GWH_HIP(hipGetLastError());
     if (timed && kernel_ms) {
         GWH_HIP(hipEventSynchronize(E.ev1));
@@ -1467,10 +1471,11 @@ int gwh_fourier_detector_response(const double *frequencies, int length, double 
 
 /* find or create a cached event for host arrays used by the single-call likelihoods */
 static int cached_event(const double *const *f, int n, int ndet, const int *dets, const double *const *data,
-                        const double *const *psd, int method, const double *weights, int log10F, int *event)
+                        const double *const *psd, int method, const double *weights, int log10F, int *event,
+                        int df_mode = 0)
 {
     Engine &E = gwh_engine();
-    uint64_t key = 0x12345;
+    uint64_t key = 0x12345 + (uint64_t)df_mode;
     key = gwh_hash_bytes(f[0], (size_t)n * 8, key);
     for (int d = 0; d < ndet; d++) {
         key = gwh_hash_bytes(data[d], (size_t)n * 16, key);
@@ -1503,7 +1508,8 @@ static int cached_event(const double *const *f, int n, int ndet, const int *dets
         memcpy(&dat[(size_t)d * n * 2], data[d], (size_t)n * 16);
         memcpy(&ps[(size_t)d * n], psd[d], (size_t)n * 8);
     }
-    return gwh_event_create_ex(f[0], n, ndet, dets, dat.data(), ps.data(), method, weights, log10F, 0, key, event);
+    return gwh_event_create_ex(f[0], n, ndet, dets, dat.data(), ps.data(), method, weights, log10F, df_mode, key,
+                               event);
 }
 
 double gwh_log_likelihood(const double *data, const double *psd, const double *frequencies, size_t length,
@@ -1607,6 +1613,106 @@ double gwh_log_likelihood_internal(const double *data, const double *psd, const 
         dh += part[2 * b + 1];
     }
     return -0.5 * (hh - 2 * dh);
+}
+
+/* (a|b) sums of the one-detector event `ev` against a response: hh = (r|r), dh = (d|r) */
+static int inner_products(int ev, const double *response, int length, double *hh, double *dh)
+{
+    Engine &E = gwh_engine();
+    const EventDesc &ds = E.events[ev].desc;
+    int rc;
+    if ((rc = E.out1.ensure((size_t)length * 16)) != GWH_OK) return rc;
+    int nb = (length + GWH_TPB - 1) / GWH_TPB;
+    if (nb > 1024) nb = 1024;
+    if ((rc = E.partial.ensure((size_t)nb * 2 * 8)) != GWH_OK) return rc;
+    GWH_HIP(hipMemcpyAsync(E.out1.p, response, (size_t)length * 16, hipMemcpyHostToDevice, E.stream));
+    hipLaunchKernelGGL(gwh_inner_kernel, dim3(nb), dim3(GWH_TPB), 0, E.stream, ds.dwS, E.out1.as<const double2>(),
+                       ds.wS, length, E.partial.as<double>());
+    GWH_HIP(hipGetLastError());
+    std::vector<double> part(2 * nb);
+    GWH_HIP(hipMemcpyAsync(part.data(), E.partial.p, part.size() * 8, hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    double a = 0, b = 0;
+    for (int k = 0; k < nb; k++) { /* fixed order over the block partials */
+        a += part[2 * k];
+        b += part[2 * k + 1];
+    }
+    *hh = a;
+    *dh = b;
+    return GWH_OK;
+}
+
+double gwh_calculate_snr_internal(const double *psd, const double *waveform, const double *frequencies, int length,
+                                  int integration_method, const double *weights, int log10F)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    const double nan = std::nan("");
+    if (gwh_ensure_init() != GWH_OK) return nan;
+    if (!psd || !waveform || !frequencies || length < 2 ||
+        (integration_method == GWH_GAUSSLEG && !weights)) {
+        gwh_set_error(GWH_ERR_INVALID, "null argument or length < 2");
+        return nan;
+    }
+    std::vector<double> zeros((size_t)2 * length, 0.0);
+    int det = 0, ev;
+    const double *fp[1] = {frequencies}, *dp[1] = {zeros.data()}, *pp[1] = {psd};
+    if (cached_event(fp, length, 1, &det, dp, pp, integration_method,
+                     integration_method == GWH_GAUSSLEG ? weights : nullptr, log10F, &ev, 1) != GWH_OK)
+        return nan;
+    double hh, dh;
+    if (inner_products(ev, waveform, length, &hh, &dh) != GWH_OK) return nan;
+    return sqrt(hh);
+}
+
+double gwh_data_snr(const double *frequencies, int length, const double *data, const double *response,
+                    const double *psd)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    const double nan = std::nan("");
+    if (gwh_ensure_init() != GWH_OK) return nan;
+    if (!frequencies || !data || !response || !psd || length < 2) {
+        gwh_set_error(GWH_ERR_INVALID, "null argument or length < 2");
+        return nan;
+    }
+    int det = 0, ev;
+    const double *fp[1] = {frequencies}, *dp[1] = {data}, *pp[1] = {psd};
+    if (cached_event(fp, length, 1, &det, dp, pp, GWH_SIMPSONS, nullptr, 0, &ev, 1) != GWH_OK) return nan;
+    double hh, dh;
+    if (inner_products(ev, response, length, &hh, &dh) != GWH_OK) return nan;
+    return sqrt(dh);
+}
+
+int gwh_snr_batch(const int *events, int events0, int model, int precision, const gwh_params *params, int n,
+                  double *snr)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    if (n <= 0) return GWH_OK;
+    if (!params || !snr) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    int rc;
+    if ((rc = E.params.ensure((size_t)n * sizeof(gwh_params))) != GWH_OK) return rc;
+    if ((rc = E.logl.ensure((size_t)n * sizeof(double))) != GWH_OK) return rc;
+    GWH_HIP(hipMemcpyAsync(E.params.p, params, (size_t)n * sizeof(gwh_params), hipMemcpyHostToDevice, E.stream));
+    const int *dev = nullptr;
+    if (events) {
+        for (int i = 0; i < n; i++)
+            if (events[i] < 0 || events[i] >= GWH_MAX_EVENTS || !E.events[events[i]].live)
+                return gwh_set_error(GWH_ERR_INVALID, "unknown event in batch");
+        if ((rc = E.inst_event.ensure((size_t)n * sizeof(int))) != GWH_OK) return rc;
+        GWH_HIP(hipMemcpyAsync(E.inst_event.p, events, (size_t)n * sizeof(int), hipMemcpyHostToDevice, E.stream));
+        dev = E.inst_event.as<const int>();
+        events0 = events[0];
+    }
+    rc = gwh_loglike_launch(dev, events0, model, precision, E.params.as<const gwh_params>(), n, E.logl.as<double>(),
+                            E.stream, false, nullptr, events, 1);
+    if (rc != GWH_OK) return rc;
+    GWH_HIP(hipMemcpyAsync(snr, E.logl.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
 }
 
 int gwh_work_counts(int event, int model, const gwh_params *params, int n, int *counts)

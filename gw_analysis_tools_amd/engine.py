@@ -14,7 +14,7 @@ from . import GWHError, Params, check, lib, PARAMS_DTYPE, IMRPHENOMD, FP64
 __all__ = ["fourier_waveform", "fourier_detector_response", "create_coherent_GW_detection", "work_counts",
            "log_likelihood", "log_likelihood_internal", "mcmc_likelihood_extrinsic", "fisher_numerical",
            "fisher_derivatives", "unpack_parameters", "repack_parameters", "detector_id", "dtoa",
-           "gps_to_gmst_radian", "Event", "loglike_batch", "fisher_batch", "DeviceArray", "set_bins_per_lane",
+           "gps_to_gmst_radian", "calculate_snr_internal", "data_snr", "snr_batch", "Event", "loglike_batch", "fisher_batch", "DeviceArray", "set_bins_per_lane",
            "synchronize"]
 
 
@@ -161,6 +161,31 @@ def dtoa(ra, dec, gmst, det1, det2):
 
 def gps_to_gmst_radian(gps):
     return lib().gwh_gps_to_gmst_radian(gps)
+
+
+def calculate_snr_internal(psd, waveform, f, weights=None, log10F=False, method=0):
+    w = _f64(weights) if weights is not None else None
+    return _nan_check(lib().gwh_calculate_snr_internal(_dp(_f64(psd)), _dp(_interleave(waveform)), _dp(_f64(f)),
+                                                       len(f), method, _dp(w) if w is not None else None,
+                                                       int(log10F)))
+
+
+def data_snr(f, data, response, psd):
+    return _nan_check(lib().gwh_data_snr(_dp(_f64(f)), len(f), _dp(_interleave(data)), _dp(_interleave(response)),
+                                         _dp(_f64(psd))))
+
+
+def snr_batch(event, params, model=IMRPHENOMD, precision=FP64, events=None):
+    arr = _as_params_array(params)
+    n = len(arr)
+    out = np.zeros(n)
+    ev0 = event.id if isinstance(event, Event) else int(event)
+    evs = None
+    if events is not None:
+        e = np.ascontiguousarray([x.id if isinstance(x, Event) else int(x) for x in events], dtype=np.int32)
+        evs = e.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    check(lib().gwh_snr_batch(evs, ev0, model, precision, arr.ctypes.data_as(ctypes.c_void_p), n, _dp(out)))
+    return out
 
 
 class Event:

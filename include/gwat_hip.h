@@ -138,6 +138,17 @@ int gwh_fisher_numerical(const double *frequencies, int length, int basis, int d
                          double *out, int dim, const gwh_params *params, int order, const double *psd,
                          double epsilon);
 
+/* calculate_snr_internal(double *psd, std::complex<double> *waveform, double *frequencies, int length,
+ * std::string integration_method, double *weights, bool log10_freq) -- src/waveform_util.cpp:479-510:
+ * sqrt(4 Simpson(|h|^2 / S)) with the step f[1]-f[0] (or the Gauss-Legendre weights). */
+double gwh_calculate_snr_internal(const double *psd, const double *waveform, const double *frequencies, int length,
+                                  int integration_method, const double *weights, int log10F);
+
+/* data_snr(double *frequencies, int length, std::complex<double> *data, std::complex<double> *response,
+ * double *psd) -- src/waveform_util.cpp:108-127: sqrt((d|r)). */
+double gwh_data_snr(const double *frequencies, int length, const double *data, const double *response,
+                    const double *psd);
+
 /* calculate_derivatives(double **derivatives, double *frequencies, int length, int dimension,
  * std::string detector, std::string reference_detector, std::string gen_method,
  * gen_params_base<double> *parameters, int order) -- src/fisher.cpp:160-557.
@@ -172,6 +183,11 @@ size_t gwh_event_bytes(int event);
  * Host buffers in and out. */
 int gwh_loglike_batch(const int *events, int events0, int model, int precision, const gwh_params *params, int n,
                       double *logl);
+
+/* Network optimal SNR sqrt(sum_d (h_d|h_d)) of n instances on resident events (the SNR convention of
+ * calculate_snr_internal, src/waveform_util.cpp:479-510, summed over the event's detectors). */
+int gwh_snr_batch(const int *events, int events0, int model, int precision, const gwh_params *params, int n,
+                  double *snr);
 
 /* Same with device buffers already resident in HBM (d_params: n gwh_params, d_logl: n doubles),
  * enqueued on the given HIP stream (NULL: the engine's stream) without a host synchronisation.

@@ -204,3 +204,28 @@ def test_errors_are_reported_not_fatal():
     bad.mass2 = -3
     hp, hc = G.fourier_waveform(f, bad)
     assert np.isnan(hp).any()
+
+
+def test_snr_entry_points():
+    """calculate_snr_internal, data_snr and the batched network SNR against the oracle"""
+    f = W.uniform_grid(20., 1024., 0.25)
+    psd = W.analytic_psd(f)
+    g, o = both(**dict(CASES[0], **EXTR))
+    r = O.detector_response(f, o, 0)
+    a = G.calculate_snr_internal(psd, r, f)
+    b = O.snr(psd, r, f)
+    assert abs(a - b) <= 1e-12 * b, (a, b)
+    # data_snr: sqrt((d|r)) with d = 2 r -> sqrt(2) SNR
+    assert abs(G.data_snr(f, 2 * r, r, psd) - np.sqrt(2) * b) <= 1e-12 * b
+    # batched network SNR = sqrt(sum over detectors of the single-detector SNR^2)
+    f2, psd2, dets, data = _net_setup()
+    ev = G.Event(f2, data, [psd2] * 3, dets)
+    P = W.perturbed_params(W.injection_params(1)[0], 16, step=4)
+    s64 = G.snr_batch(ev, P, G.IMRPHENOMD, G.FP64)
+    smx = G.snr_batch(ev, P, G.IMRPHENOMD, G.MIXED)
+    for i in range(0, 16, 5):
+        op = _oracle_params(P[i])
+        ref = np.sqrt(sum(O.snr(psd2, O.detector_response(f2, op, d), f2) ** 2 for d in dets))
+        assert abs(s64[i] - ref) <= 1e-10 * ref, (s64[i], ref)
+    assert np.max(np.abs(smx / s64 - 1)) <= 1e-8
+    ev.close()
