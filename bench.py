@@ -182,6 +182,107 @@ def bench_fisher(G, W, ev, n_points=2048):
     return out
 
 
+def bench_serving(G, W, rates, seconds, fisher_fraction=0.1, clients=32):
+    """Open-loop Poisson arrivals from `clients` threads at each aggregate rate: a request is a
+    log-likelihood batch of U{16..1024} instances (IMRPhenomD or IMRPhenomPv2, 50/50, mixed precision) on
+    one of 64 resident events with the uneven-batch grid lengths, or (fisher_fraction) a 1..64-point
+    Fisher batch; admission by a 200 GB HBM budget.  Latency = submission to completion per request."""
+    import threading
+    specs = W.config2(n_events=64, chains=4)
+    g = W.rng(4, 1)
+    events = []
+    for sp in specs:
+        f, psd = sp["f"], sp["psd"]
+        inj = sp["injection"]
+        p = G.make_params(**{k: (list(inj[k]) if k in ("spin1", "spin2") else inj[k].item()) for k in inj.dtype.names})
+        data = [r + W.noise(f, psd, g) for r in G.create_coherent_GW_detection(f, p, sp["dets"])]
+        events.append((G.Event(f, data, [psd] * 3, sp["dets"]), sp))
+    rng = np.random.default_rng(20261204)
+    pool = []  # pre-generated requests (parameter generation stays out of the arrival loop)
+    for k in range(400):
+        e, sp = events[rng.integers(len(events))]
+        pv2 = rng.random() < 0.5
+        if rng.random() < fisher_fraction:
+            npts = int(rng.integers(1, 65))
+            P = W.perturbed_params(sp["injection"], npts, step=k, config=4, scale=0.1)
+            pool.append(("fisher", e, P, G.IMRPHENOMD, None))
+        else:
+            n = int(rng.integers(16, 1025))
+            P = W.perturbed_params(sp["injection"], n, step=k, config=4, scale=0.1)
+            if pv2:
+                P["spin1"][:, 0] = 0.2
+                P["spin2"][:, 1] = -0.1
+            pool.append(("loglike", e, P, G.IMRPHENOMPV2 if pv2 else G.IMRPHENOMD, None))
+    out = {"clients": clients, "events": len(events), "fisher_fraction": fisher_fraction,
+           "hbm_budget_bytes": 200 << 30, "resident_event_bytes": int(sum(e.nbytes for e, _ in events)), "rates": []}
+    br = G.Broker(hbm_budget_bytes=200 << 30, max_batch_instances=8192)
+    # warm-up: every event and both models once
+    for e, sp in events:
+        for m in (G.IMRPHENOMD, G.IMRPHENOMPV2):
+            t, _ = br.submit_loglike(e, W.perturbed_params(sp["injection"], 16, step=1, config=4, scale=0.1), m, G.MIXED)
+            br.wait(t)
+    for rate in rates:
+        lock = threading.Lock()
+        lats, done_t, kinds = [], [], []
+        t_start = time.perf_counter() + 0.05
+
+        def client(cid):
+            r2 = np.random.default_rng(1000 * rate + cid)
+            t = t_start
+            mine = []
+            while True:
+                t += r2.exponential(clients / rate)
+                if t - t_start > seconds:
+                    break
+                kind, e, P, model, _ = pool[r2.integers(len(pool))]
+                dt = t - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                if kind == "fisher":
+                    tk, _, _ = br.submit_fisher(e, P, model, G.BASIS_PHENOMD_MCMC, 11, precision=G.FP64)
+                else:
+                    tk, _ = br.submit_loglike(e, P, model, G.MIXED)
+                mine.append((tk, kind))
+            for tk, kind in mine:
+                lat = br.wait(tk)
+                with lock:
+                    lats.append(lat)
+                    kinds.append(kind)
+                    done_t.append(time.perf_counter())
+
+        threads = [threading.Thread(target=client, args=(c,)) for c in range(clients)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        wall = max(done_t) - t_start if done_t else float("nan")
+        lat_ms = np.array(lats) / 1e3
+        ll = lat_ms[np.array(kinds) == "loglike"]
+        rec = {"rate_req_per_s": rate, "requests": len(lats), "throughput_req_per_s": len(lats) / wall,
+               "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+               "loglike_p50_ms": float(np.percentile(ll, 50)) if ll.size else None,
+               "loglike_p99_ms": float(np.percentile(ll, 99)) if ll.size else None}
+        out["rates"].append(rec)
+        log("serving %5d req/s offered: %7.1f req/s served, p50 %.2f ms, p99 %.2f ms (%d requests)"
+            % (rate, rec["throughput_req_per_s"], rec["p50_ms"], rec["p99_ms"], len(lats)))
+    out["broker"] = br.stats()
+    # mixed vs FP64 on a sample of the likelihood requests
+    worst = 0.0
+    for kind, e, P, model, _ in pool[:60]:
+        if kind != "loglike":
+            continue
+        t1, a = br.submit_loglike(e, P, model, G.MIXED)
+        t2, b = br.submit_loglike(e, P, model, G.FP64)
+        br.wait(t1)
+        br.wait(t2)
+        worst = max(worst, float(np.max(np.abs(a - b))))
+    out["max_abs_dlogL_mixed_vs_fp64"] = worst
+    br.close()
+    for e, _ in events:
+        e.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +295,9 @@ def main():
                     help="mixed: only the timed mixed-precision leg (for counter runs)")
     ap.add_argument("--no-c2", action="store_true", help="skip the uneven multi-event batch leg")
     ap.add_argument("--no-fisher", action="store_true", help="skip the batched Fisher leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving (broker) leg")
+    ap.add_argument("--serving-rates", default="50,100,200,500,1000,2000")
+    ap.add_argument("--serving-seconds", type=float, default=2.0)
     ap.add_argument("--c2-steps", type=int, default=20)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (reference point for traffic counters)")
@@ -389,6 +493,15 @@ def main():
             log("uneven-batch leg failed:", e)
             c2 = {"error": str(e)}
 
+    # ---- serving: Poisson request streams from 32 clients through the continuous-batching broker ----
+    serving = None
+    if not args.no_serving:
+        try:
+            serving = bench_serving(G, W, [int(x) for x in args.serving_rates.split(",")], args.serving_seconds)
+        except Exception as e:  # reported, not fatal for the headline
+            log("serving leg failed:", e)
+            serving = {"error": str(e)}
+
     # ---- CPU baseline: the oracle (reference algorithm, C, one instance per call) ----
     cpu = None
     if not args.no_cpu:
@@ -431,6 +544,7 @@ def main():
         "imrphenompv2_c3_likelihood": pv2,
         "uneven_batch_c2": c2,
         "fisher_c3": fisher,
+        "serving_c4": serving,
     }
     print(json.dumps(rec))
 

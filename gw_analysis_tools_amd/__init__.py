@@ -42,6 +42,8 @@ EXPORTS = [
     "gwh_fisher_derivatives", "gwh_unpack_parameters", "gwh_repack_parameters", "gwh_detector_id", "gwh_dtoa",
     "gwh_gps_to_gmst_radian", "gwh_loglike_multi_device", "gwh_autotune_bins_per_lane",
     "gwh_calculate_snr_internal", "gwh_data_snr", "gwh_snr_batch",
+    "gwh_broker_create", "gwh_broker_destroy", "gwh_broker_submit_loglike", "gwh_broker_submit_fisher",
+    "gwh_broker_wait", "gwh_broker_stats",
 ]
 
 
@@ -140,6 +142,11 @@ def lib():
         "gwh_calculate_snr_internal": ([dp, dp, dp, I, I, dp, I], D),
         "gwh_data_snr": ([dp, I, dp, dp, dp], D),
         "gwh_snr_batch": ([ip, I, I, I, V, I, dp], I),
+        "gwh_broker_create": ([ctypes.c_size_t, I, ip], I), "gwh_broker_destroy": ([I], I),
+        "gwh_broker_submit_loglike": ([I, ip, I, I, I, V, I, dp, ctypes.POINTER(ctypes.c_longlong)], I),
+        "gwh_broker_submit_fisher": ([I, I, I, I, I, V, I, I, D, dp, dp, ctypes.POINTER(ctypes.c_longlong)], I),
+        "gwh_broker_wait": ([I, ctypes.c_longlong, D, dp], I),
+        "gwh_broker_stats": ([I, dp], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -68,6 +68,12 @@ struct DevBuf {
     template <class T> T *as() { return (T *)p; }
 };
 
+/* Device scratch of one batched-likelihood launch sequence (setup, tiles, finalize).  The engine owns one;
+ * the serving broker owns one per stream so batches can be in flight together. */
+struct Scratch {
+    DevBuf coef, partial, active, work, wcount;
+};
+
 struct Engine {
     std::recursive_mutex mu;
     bool ready = false;
@@ -107,7 +113,7 @@ int gwh_choose_bpl(int N);
 /* the batched likelihood on device buffers (engine lock held by caller) */
 int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
                        double *d_logl, hipStream_t stream, bool timed, double *kernel_ms,
-                       const int *h_events = nullptr, int what = 0);
+                       const int *h_events = nullptr, int what = 0, Scratch *scratch = nullptr);
 
 /* per-instance setup on device buffers */
 int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_params *d_params, int n, InstCoef *d_coef,

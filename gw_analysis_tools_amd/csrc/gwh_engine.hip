@@ -823,9 +823,12 @@ static void launch_wl(int bpl, int blocks, hipStream_t st, const WorkMap &W, con
 
 int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
                        double *d_logl, hipStream_t stream, bool timed, double *kernel_ms, const int *h_events,
-                       int what)
+                       int what, Scratch *scratch)
 {
     Engine &E = gwh_engine();
+    DevBuf &s_coef = scratch ? scratch->coef : E.coef, &s_partial = scratch ? scratch->partial : E.partial,
+           &s_active = scratch ? scratch->active : E.active, &s_work = scratch ? scratch->work : E.work,
+           &s_wcount = scratch ? scratch->wcount : E.wcount;
     if (n <= 0) return GWH_OK;
     if (model != GWH_IMRPHENOMD && model != GWH_IMRPHENOMPV2)
         return gwh_set_error(GWH_ERR_UNSUPPORTED, "model must be GWH_IMRPHENOMD or GWH_IMRPHENOMPV2");
@@ -864,33 +867,33 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     int maxtiles = 0;
     for (int b = 0; b < 4; b++) maxtiles = std::max(maxtiles, tiles_b[b]);
     int rc;
-    if ((rc = E.coef.ensure((size_t)n * sizeof(InstCoef))) != GWH_OK) return rc;
-    if ((rc = E.partial.ensure((size_t)n * maxtiles * GWH_PARTIAL_STRIDE * sizeof(double))) != GWH_OK) return rc;
-    if ((rc = E.active.ensure((size_t)n * sizeof(int))) != GWH_OK) return rc;
+    if ((rc = s_coef.ensure((size_t)n * sizeof(InstCoef))) != GWH_OK) return rc;
+    if ((rc = s_partial.ensure((size_t)n * maxtiles * GWH_PARTIAL_STRIDE * sizeof(double))) != GWH_OK) return rc;
+    if ((rc = s_active.ensure((size_t)n * sizeof(int))) != GWH_OK) return rc;
     WorkMap W;
     W.inst_event = d_events;
     W.event0 = events0;
     W.n = n;
     W.tiles0 = maxtiles;
-    launch_setup(model, n, stream, d_params, d_events, events0, E.coef.as<InstCoef>(), E.active.as<int>());
+    launch_setup(model, n, stream, d_params, d_events, events0, s_coef.as<InstCoef>(), s_active.as<int>());
     GWH_HIP(hipGetLastError());
     if (timed) GWH_HIP(hipEventRecord(E.ev0, stream));
     const bool live = E.kt_on && 2 * (size_t)E.kt_used + 1 < E.kt_ev.size();
     if (live) GWH_HIP(hipEventRecord(E.kt_ev[2 * E.kt_used], stream));
     const EventDesc *evp = (const EventDesc *)E.d_events;
-    const InstCoef *cp = E.coef.as<const InstCoef>();
-    double *pp = E.partial.as<double>();
-    const int *ap = E.active.as<const int>();
+    const InstCoef *cp = s_coef.as<const InstCoef>();
+    double *pp = s_partial.as<double>();
+    const int *ap = s_active.as<const int>();
     static const int kB[4] = {1, 2, 4, 8};
     if (wl) {
         long long tot = ub[0] + ub[1] + ub[2] + ub[3];
-        if ((rc = E.work.ensure((size_t)tot * sizeof(int2))) != GWH_OK) return rc;
-        if ((rc = E.wcount.ensure(4 * sizeof(int))) != GWH_OK) return rc;
+        if ((rc = s_work.ensure((size_t)tot * sizeof(int2))) != GWH_OK) return rc;
+        if ((rc = s_wcount.ensure(4 * sizeof(int))) != GWH_OK) return rc;
         long long off = 0;
         for (int sl = 0; sl < 4; sl++) {
             if (!ub[sl]) continue;
-            int2 *wp = E.work.as<int2>() + off;
-            int *cnt = E.wcount.as<int>() + sl;
+            int2 *wp = s_work.as<int2>() + off;
+            int *cnt = s_wcount.as<int>() + sl;
             hipLaunchKernelGGL(gwh_worklist_kernel, dim3(1), dim3(1024), 0, stream, W, evp, ap, kB[sl], wp, cnt,
                                (int)ub[sl]);
             if (model == GWH_IMRPHENOMD) {

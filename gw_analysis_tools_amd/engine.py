@@ -14,7 +14,7 @@ from . import GWHError, Params, check, lib, PARAMS_DTYPE, IMRPHENOMD, FP64
 __all__ = ["fourier_waveform", "fourier_detector_response", "create_coherent_GW_detection", "work_counts",
            "log_likelihood", "log_likelihood_internal", "mcmc_likelihood_extrinsic", "fisher_numerical",
            "fisher_derivatives", "unpack_parameters", "repack_parameters", "detector_id", "dtoa",
-           "gps_to_gmst_radian", "calculate_snr_internal", "data_snr", "snr_batch", "Event", "loglike_batch", "fisher_batch", "DeviceArray", "set_bins_per_lane",
+           "gps_to_gmst_radian", "calculate_snr_internal", "data_snr", "snr_batch", "Broker", "Event", "loglike_batch", "fisher_batch", "DeviceArray", "set_bins_per_lane",
            "synchronize"]
 
 
@@ -282,5 +282,70 @@ class DeviceArray:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Broker:
+    """Continuous-batching request broker (gwh_broker_*): submit from any thread, wait on tickets."""
+
+    def __init__(self, hbm_budget_bytes=200 << 30, max_batch_instances=8192):
+        b = ctypes.c_int(-1)
+        check(lib().gwh_broker_create(int(hbm_budget_bytes), int(max_batch_instances), ctypes.byref(b)))
+        self.id = b.value
+        self._keep = {}
+
+    def submit_loglike(self, events, params, model=IMRPHENOMD, precision=FP64):
+        """events: one event (Event or id) or one per instance; returns (ticket, output array)"""
+        arr = _as_params_array(params)
+        n = len(arr)
+        out = np.zeros(n)
+        if isinstance(events, (list, tuple, np.ndarray)):
+            ev = np.ascontiguousarray([x.id if isinstance(x, Event) else int(x) for x in events], dtype=np.int32)
+            evp, ev0 = ev.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(ev[0])
+        else:
+            ev, evp, ev0 = None, None, (events.id if isinstance(events, Event) else int(events))
+        t = ctypes.c_longlong(0)
+        check(lib().gwh_broker_submit_loglike(self.id, evp, ev0, model, precision, arr.ctypes.data_as(ctypes.c_void_p),
+                                              n, _dp(out), ctypes.byref(t)))
+        self._keep[t.value] = (arr, ev, out)
+        return t.value, out
+
+    def submit_fisher(self, event, params, model, basis, dim, precision=FP64, epsilon=1e-8, with_sigma=False):
+        arr = _as_params_array(params)
+        n = len(arr)
+        out = np.zeros(n * dim * dim)
+        sig = np.zeros(n * dim) if with_sigma else None
+        ev0 = event.id if isinstance(event, Event) else int(event)
+        t = ctypes.c_longlong(0)
+        check(lib().gwh_broker_submit_fisher(self.id, ev0, model, basis, precision, arr.ctypes.data_as(ctypes.c_void_p),
+                                             n, dim, epsilon, _dp(out), _dp(sig) if sig is not None else None,
+                                             ctypes.byref(t)))
+        self._keep[t.value] = (arr, out, sig)
+        return t.value, out.reshape(n, dim, dim), (sig.reshape(n, dim) if sig is not None else None)
+
+    def wait(self, ticket, timeout_ms=-1.0):
+        """waits for the ticket; returns the latency in microseconds (submission to completion)"""
+        lat = np.zeros(1)
+        st = lib().gwh_broker_wait(self.id, ticket, timeout_ms, _dp(lat))
+        self._keep.pop(ticket, None)
+        check(st)
+        return float(lat[0])
+
+    def stats(self):
+        s = np.zeros(7)
+        check(lib().gwh_broker_stats(self.id, _dp(s)))
+        keys = ("batches", "instances", "fisher_requests", "max_requests_per_batch", "admission_waits",
+                "peak_device_bytes", "tickets_outstanding")
+        return {k: (int(v) if k != "peak_device_bytes" else int(v)) for k, v in zip(keys, s)}
+
+    def close(self):
+        if self.id >= 0:
+            check(lib().gwh_broker_destroy(self.id))
+            self.id = -1
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

@@ -231,6 +231,27 @@ int gwh_time_loglike_batch_device(int events0, int model, int precision, const v
  * price the algorithmic work of a batch. */
 int gwh_work_counts(int event, int model, const gwh_params *params, int n, int *counts);
 
+/* ---- serving broker (no counterpart in the reference: its samplers call the likelihood from many
+ * threads at once, mcmc_sampler.cpp:4585-4600; the broker coalesces such calls) ----
+ * Requests from any number of host threads are batched continuously onto the GPU: likelihood requests
+ * of the same model and precision are concatenated (each instance keeps its event) up to
+ * max_batch_instances, two streams keep one batch in flight while the next is formed, and a batch is
+ * admitted only if resident events + broker scratch + its own needs fit in hbm_budget_bytes.  Results
+ * are bit-identical to the same request run alone.  Parameters are copied at submission; outputs are
+ * written before the ticket completes; events must outlive their pending requests. */
+int gwh_broker_create(size_t hbm_budget_bytes, int max_batch_instances, int *broker);
+int gwh_broker_destroy(int broker); /* drains pending work first */
+int gwh_broker_submit_loglike(int broker, const int *events, int event0, int model, int precision,
+                              const gwh_params *params, int n, double *logl, long long *ticket);
+int gwh_broker_submit_fisher(int broker, int event, int model, int basis, int precision, const gwh_params *params,
+                             int n, int dim, double epsilon, double *out, double *sigma, long long *ticket);
+/* waits for a ticket (timeout_ms < 0: no limit); returns the request's status (GWH_ERR_BUSY on timeout);
+ * latency_us (optional): submission to completion */
+int gwh_broker_wait(int broker, long long ticket, double timeout_ms, double *latency_us);
+/* stats[7]: batches, instances, Fisher requests, most requests merged into one batch, admission waits,
+ * peak device bytes (resident events + broker scratch), tickets not yet waited for */
+int gwh_broker_stats(int broker, double *stats);
+
 /* device memory helpers for callers without their own allocator */
 void *gwh_device_alloc(size_t bytes);
 int gwh_device_free(void *ptr);
