@@ -14,7 +14,8 @@
  *  - admission is by an HBM budget: resident events + the broker's scratch + the new batch's needs must
  *    fit, otherwise the batch waits for in-flight work to drain (and a request that cannot fit even on
  *    an idle device fails with GWH_ERR_NOMEM);
- *  - Fisher requests run one at a time on the engine's stream, between likelihood batches.
+ *  - Fisher requests run one at a time on a worker thread with its own stream and scratch, concurrently
+ *    with the likelihood batches.
  *
  * An instance's result does not depend on what it is batched with (per-event tiling, fixed-order sums),
  * so a request's results are bit-identical alone and in the mix.  Events referenced by pending
@@ -79,6 +80,11 @@ struct Broker {
     bool stop = false;
     Lane lanes[2];
     std::thread th;
+    /* Fisher requests: their own queue, worker thread, stream and scratch */
+    std::deque<std::shared_ptr<Request>> fisher_queue;
+    std::condition_variable cv_fisher;
+    FisherScratch fisher;
+    std::thread fisher_th;
     /* statistics */
     long long batches = 0, instances = 0, fishers = 0, max_merged = 0, admission_waits = 0;
     size_t peak_bytes = 0;
@@ -168,10 +174,11 @@ int launch_batch(Broker &B, Lane &L)
     if (rc != GWH_OK) return rc;
     GWH_HIP(hipMemcpyAsync(L.h_logl, L.d_logl.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, L.stream));
     GWH_HIP(hipEventRecord(L.done, L.stream));
+    size_t used = resident_bytes() + lane_bytes(B.lanes[0]) + lane_bytes(B.lanes[1]);
+    std::lock_guard<std::mutex> lk(B.m);
     B.batches++;
     B.instances += n;
     if ((long long)L.batch.size() > B.max_merged) B.max_merged = (long long)L.batch.size();
-    size_t used = resident_bytes() + lane_bytes(B.lanes[0]) + lane_bytes(B.lanes[1]);
     if (used > B.peak_bytes) B.peak_bytes = used;
     return GWH_OK;
 }
@@ -188,15 +195,26 @@ void reap(Broker &B, Lane &L)
     L.busy = false;
 }
 
-void run_fisher(Broker &B, const std::shared_ptr<Request> &r)
+void fisher_worker(Broker *bp)
 {
-    int st = gwh_fisher_batch(r->events[0], r->model, r->basis, r->precision, r->params.data(), r->n, r->dim, r->eps,
-                              r->out, r->sigma);
-    {
-        std::lock_guard<std::mutex> lk(B.m);
-        B.fishers++;
+    Broker &B = *bp;
+    for (;;) {
+        std::shared_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(B.m);
+            B.cv_fisher.wait(lk, [&] { return B.stop || !B.fisher_queue.empty(); });
+            if (B.fisher_queue.empty()) return; /* stop requested and nothing left */
+            r = B.fisher_queue.front();
+            B.fisher_queue.pop_front();
+        }
+        int st = gwh_fisher_batch_with(r->events[0], r->model, r->basis, r->precision, r->params.data(), r->n, r->dim,
+                                       r->eps, r->out, r->sigma, B.fisher);
+        {
+            std::lock_guard<std::mutex> lk(B.m);
+            B.fishers++;
+        }
+        finish(B, r, st);
     }
-    finish(B, r, st);
 }
 
 void dispatcher(Broker *bp)
@@ -213,7 +231,6 @@ void dispatcher(Broker *bp)
                 free_lane = &L;
                 break;
             }
-        std::shared_ptr<Request> fisher;
         std::vector<std::shared_ptr<Request>> batch;
         {
             std::unique_lock<std::mutex> lk(B.m);
@@ -229,29 +246,19 @@ void dispatcher(Broker *bp)
                 continue;
             }
             auto head = B.queue.front();
-            if (head->kind == REQ_FISHER) {
-                fisher = head;
-                B.queue.pop_front();
-            } else {
-                /* continuous batching: compatible requests in arrival order up to max_n instances */
-                int n = 0;
-                for (auto it = B.queue.begin(); it != B.queue.end();) {
-                    auto &r = *it;
-                    if (r->kind == REQ_LOGLIKE && r->model == head->model && r->precision == head->precision &&
-                        (n == 0 || n + r->n <= B.max_n)) {
-                        n += r->n;
-                        batch.push_back(r);
-                        it = B.queue.erase(it);
-                        if (n >= B.max_n) break;
-                    } else {
-                        ++it;
-                    }
+            /* continuous batching: compatible requests in arrival order up to max_n instances */
+            int n = 0;
+            for (auto it = B.queue.begin(); it != B.queue.end();) {
+                auto &r = *it;
+                if (r->model == head->model && r->precision == head->precision && (n == 0 || n + r->n <= B.max_n)) {
+                    n += r->n;
+                    batch.push_back(r);
+                    it = B.queue.erase(it);
+                    if (n >= B.max_n) break;
+                } else {
+                    ++it;
                 }
             }
-        }
-        if (fisher) {
-            run_fisher(B, fisher);
-            continue;
         }
         /* admission by the HBM budget */
         std::vector<int> evs;
@@ -315,10 +322,15 @@ int submit(Broker *B, const std::shared_ptr<Request> &r, long long *ticket)
     if (B->stop) return gwh_set_error(GWH_ERR_INVALID, "broker is shutting down");
     r->ticket = B->next_ticket++;
     r->submitted = clk::now();
-    B->queue.push_back(r);
     B->all[r->ticket] = r;
     *ticket = r->ticket;
-    B->cv_work.notify_one();
+    if (r->kind == REQ_FISHER) {
+        B->fisher_queue.push_back(r);
+        B->cv_fisher.notify_one();
+    } else {
+        B->queue.push_back(r);
+        B->cv_work.notify_one();
+    }
     return GWH_OK;
 }
 
@@ -342,7 +354,12 @@ int gwh_broker_create(size_t hbm_budget_bytes, int max_batch_instances, int *bro
             return gwh_set_error(GWH_ERR_HIP, "stream creation failed");
         }
     }
+    if (hipStreamCreateWithFlags(&B->fisher.stream, hipStreamNonBlocking) != hipSuccess) {
+        delete B;
+        return gwh_set_error(GWH_ERR_HIP, "stream creation failed");
+    }
     B->th = std::thread(dispatcher, B);
+    B->fisher_th = std::thread(fisher_worker, B);
     std::lock_guard<std::mutex> lk(g_brokers_mu);
     *broker = g_next_broker++;
     g_brokers[*broker] = B;
@@ -363,8 +380,15 @@ int gwh_broker_destroy(int broker)
         std::lock_guard<std::mutex> lk(B->m);
         B->stop = true;
         B->cv_work.notify_all();
+        B->cv_fisher.notify_all();
     }
     B->th.join(); /* drains the queue and the in-flight batches first */
+    B->fisher_th.join();
+    (void)hipStreamSynchronize(B->fisher.stream);
+    for (DevBuf *d : {&B->fisher.params, &B->fisher.coef, &B->fisher.misc, &B->fisher.deriv, &B->fisher.gram,
+                      &B->fisher.out, &B->fisher.sig, &B->fisher.raw})
+        if (d->p) (void)hipFree(d->p);
+    (void)hipStreamDestroy(B->fisher.stream);
     for (Lane &L : B->lanes) {
         (void)hipStreamSynchronize(L.stream);
         (void)hipEventDestroy(L.done);

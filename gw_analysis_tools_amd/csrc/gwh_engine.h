@@ -74,6 +74,12 @@ struct Scratch {
     DevBuf coef, partial, active, work, wcount;
 };
 
+/* Device scratch and stream of one Fisher computation (the engine's own, or a broker worker's). */
+struct FisherScratch {
+    hipStream_t stream = nullptr;
+    DevBuf params, coef, misc, deriv, gram, out, sig, raw;
+};
+
 struct Engine {
     std::recursive_mutex mu;
     bool ready = false;
@@ -92,7 +98,7 @@ struct Engine {
     /* scratch */
     DevBuf params, coef, partial, active, logl, inst_event, f, data, psd, wts, out1, out2, misc;
     DevBuf work, wcount; /* flattened (instance, tile) lists of multi-event batches */
-    DevBuf fisher_deriv, fisher_gram, fisher_out;
+    FisherScratch fisher; /* stream = the engine stream */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     /* live timing of the per-bin kernel inside a caller's timed region: event pairs around each launch */
     std::vector<hipEvent_t> kt_ev;
@@ -114,6 +120,11 @@ int gwh_choose_bpl(int N);
 int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
                        double *d_logl, hipStream_t stream, bool timed, double *kernel_ms,
                        const int *h_events = nullptr, int what = 0, Scratch *scratch = nullptr);
+
+/* Fisher matrices of n points on a resident event with the given scratch (no engine lock needed while
+ * it runs; the event must stay alive).  The public entry points use the engine's scratch. */
+int gwh_fisher_batch_with(int event, int model, int basis, int precision, const gwh_params *params, int n, int dim,
+                          double epsilon, double *out, double *sigma, FisherScratch &F);
 
 /* per-instance setup on device buffers */
 int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_params *d_params, int n, InstCoef *d_coef,

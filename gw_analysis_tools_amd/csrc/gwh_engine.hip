@@ -132,6 +132,7 @@ int gwh_ensure_init()
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, E.device) == hipSuccess && cu > 0)
         E.num_cu = cu;
     GWH_HIP(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+    E.fisher.stream = E.stream;
     GWH_HIP(hipEventCreate(&E.ev0));
     GWH_HIP(hipEventCreate(&E.ev1));
     GWH_HIP(hipMalloc(&E.d_fits, sizeof(gwh_phenomd_fit)));

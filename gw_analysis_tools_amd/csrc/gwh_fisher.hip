@@ -503,10 +503,14 @@ __global__ void gwh_fisher_invert_kernel(const double *__restrict__ F, int n, in
  * responses enter (network sum = all); ref_det is the arrival-time reference. */
 static int fisher_run(int event, int model, int basis, int precision, const gwh_params *params, int n, int order,
                       double eps, int d_first, int d_count, int det_for_tof, int ref_det, double scale, double *out,
-                      double *sigma, double *raw = nullptr)
+                      double *sigma, FisherScratch &F, double *raw = nullptr)
 {
     Engine &E = gwh_engine();
-    const Event &ev = E.events[event];
+    Event ev;
+    {
+        std::lock_guard<std::recursive_mutex> lk(E.mu); /* snapshot: the event stays alive meanwhile */
+        ev = E.events[event];
+    }
     int dim = basis_dim(basis);
     bool sky = (basis == GWH_BASIS_PHENOMD_SKYAVG);
     if (sky && model != GWH_IMRPHENOMD)
@@ -523,9 +527,9 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
     if (chunk > 65535) chunk = 65535;
     int nblk = (int)((nrows + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS);
     int rc;
-    if ((rc = E.fisher_deriv.ensure((size_t)chunk * per_pt_bytes)) != GWH_OK) return rc;
-    if ((rc = E.fisher_gram.ensure((size_t)chunk * nblk * 256 * 8)) != GWH_OK) return rc;
-    if ((rc = E.fisher_out.ensure((size_t)n * dim * dim * 8)) != GWH_OK) return rc;
+    if ((rc = F.deriv.ensure((size_t)chunk * per_pt_bytes)) != GWH_OK) return rc;
+    if ((rc = F.gram.ensure((size_t)chunk * nblk * 256 * 8)) != GWH_OK) return rc;
+    if ((rc = F.out.ensure((size_t)n * dim * dim * 8)) != GWH_OK) return rc;
     for (int p0 = 0; p0 < n; p0 += chunk) {
         int np = std::min(chunk, n - p0);
         std::vector<gwh_params> inst;
@@ -543,19 +547,19 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
                 for (int s = 0; s < GWH_MAXSTEN; s++) cw[((size_t)q * GWH_FROWS + i) * GWH_MAXSTEN + s] = ps.c[i][s];
         }
         int ni = (int)inst.size();
-        if ((rc = E.params.ensure((size_t)ni * sizeof(gwh_params))) != GWH_OK) return rc;
-        if ((rc = E.coef.ensure((size_t)ni * sizeof(InstCoef))) != GWH_OK) return rc;
-        if ((rc = E.misc.ensure(cw.size() * 8 + no_tof.size() + 256)) != GWH_OK) return rc;
-        GWH_HIP(hipMemcpyAsync(E.params.p, inst.data(), (size_t)ni * sizeof(gwh_params), hipMemcpyHostToDevice,
-                               E.stream));
-        char *misc = (char *)E.misc.p;
-        GWH_HIP(hipMemcpyAsync(misc, cw.data(), cw.size() * 8, hipMemcpyHostToDevice, E.stream));
-        GWH_HIP(hipMemcpyAsync(misc + cw.size() * 8, no_tof.data(), no_tof.size(), hipMemcpyHostToDevice, E.stream));
-        if ((rc = gwh_setup_launch(nullptr, event, model, E.params.as<const gwh_params>(), ni, E.coef.as<InstCoef>(),
-                                   E.stream)) != GWH_OK)
+        if ((rc = F.params.ensure((size_t)ni * sizeof(gwh_params))) != GWH_OK) return rc;
+        if ((rc = F.coef.ensure((size_t)ni * sizeof(InstCoef))) != GWH_OK) return rc;
+        if ((rc = F.misc.ensure(cw.size() * 8 + no_tof.size() + 256)) != GWH_OK) return rc;
+        GWH_HIP(hipMemcpyAsync(F.params.p, inst.data(), (size_t)ni * sizeof(gwh_params), hipMemcpyHostToDevice,
+                               F.stream));
+        char *misc = (char *)F.misc.p;
+        GWH_HIP(hipMemcpyAsync(misc, cw.data(), cw.size() * 8, hipMemcpyHostToDevice, F.stream));
+        GWH_HIP(hipMemcpyAsync(misc + cw.size() * 8, no_tof.data(), no_tof.size(), hipMemcpyHostToDevice, F.stream));
+        if ((rc = gwh_setup_launch(nullptr, event, model, F.params.as<const gwh_params>(), ni, F.coef.as<InstCoef>(),
+                                   F.stream)) != GWH_OK)
             return rc;
         StencilArgs a;
-        a.coef = E.coef.as<const InstCoef>();
+        a.coef = F.coef.as<const InstCoef>();
         a.cw = (const double *)misc;
         a.no_tof = (const unsigned char *)(misc + cw.size() * 8);
         a.dim = dim;
@@ -570,59 +574,85 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
         const EventDesc *evp = (const EventDesc *)E.d_events;
         if (raw) {
             /* bare derivatives of one point on one detector (calculate_derivatives) */
-            if ((rc = E.out1.ensure((size_t)dim * N * 16)) != GWH_OK) return rc;
-            a.raw = E.out1.as<double>();
+            if ((rc = F.raw.ensure((size_t)dim * N * 16)) != GWH_OK) return rc;
+            a.raw = F.raw.as<double>();
             a.d_count = 1;
             dim3 g1((N + GWH_TPB - 1) / GWH_TPB, 1);
             if (model == GWH_IMRPHENOMD)
                 hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, double>), g1, dim3(GWH_TPB), 0,
-                                   E.stream, a, evp, event, E.fisher_deriv.as<double>());
+                                   F.stream, a, evp, event, F.deriv.as<double>());
             else
                 hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMPV2, double>), g1, dim3(GWH_TPB), 0,
-                                   E.stream, a, evp, event, E.fisher_deriv.as<double>());
+                                   F.stream, a, evp, event, F.deriv.as<double>());
             GWH_HIP(hipGetLastError());
-            GWH_HIP(hipMemcpyAsync(raw, E.out1.p, (size_t)dim * N * 16, hipMemcpyDeviceToHost, E.stream));
-            GWH_HIP(hipStreamSynchronize(E.stream));
+            GWH_HIP(hipMemcpyAsync(raw, F.raw.p, (size_t)dim * N * 16, hipMemcpyDeviceToHost, F.stream));
+            GWH_HIP(hipStreamSynchronize(F.stream));
             return GWH_OK;
         }
         if (precision == GWH_FP64) {
             if (model == GWH_IMRPHENOMD)
-                hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, double>), g, dim3(GWH_TPB), 0, E.stream,
-                                   a, evp, event, E.fisher_deriv.as<double>());
+                hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, double>), g, dim3(GWH_TPB), 0, F.stream,
+                                   a, evp, event, F.deriv.as<double>());
             else
                 hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMPV2, double>), g, dim3(GWH_TPB), 0,
-                                   E.stream, a, evp, event, E.fisher_deriv.as<double>());
+                                   F.stream, a, evp, event, F.deriv.as<double>());
             GWH_HIP(hipGetLastError());
-            hipLaunchKernelGGL(gwh_gram_kernel<double>, dim3(nblk, np), dim3(GWH_TPB), 0, E.stream,
-                               E.fisher_deriv.as<const double>(), nrows, E.fisher_gram.as<double>(), nblk);
+            hipLaunchKernelGGL(gwh_gram_kernel<double>, dim3(nblk, np), dim3(GWH_TPB), 0, F.stream,
+                               F.deriv.as<const double>(), nrows, F.gram.as<double>(), nblk);
         } else {
             if (model == GWH_IMRPHENOMD)
-                hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, float>), g, dim3(GWH_TPB), 0, E.stream,
-                                   a, evp, event, E.fisher_deriv.as<float>());
+                hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, float>), g, dim3(GWH_TPB), 0, F.stream,
+                                   a, evp, event, F.deriv.as<float>());
             else
                 hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMPV2, float>), g, dim3(GWH_TPB), 0,
-                                   E.stream, a, evp, event, E.fisher_deriv.as<float>());
+                                   F.stream, a, evp, event, F.deriv.as<float>());
             GWH_HIP(hipGetLastError());
-            hipLaunchKernelGGL(gwh_gram_kernel<float>, dim3(nblk, np), dim3(GWH_TPB), 0, E.stream,
-                               E.fisher_deriv.as<const float>(), nrows, E.fisher_gram.as<double>(), nblk);
+            hipLaunchKernelGGL(gwh_gram_kernel<float>, dim3(nblk, np), dim3(GWH_TPB), 0, F.stream,
+                               F.deriv.as<const float>(), nrows, F.gram.as<double>(), nblk);
         }
         GWH_HIP(hipGetLastError());
-        hipLaunchKernelGGL(gwh_gram_reduce_kernel, dim3(np), dim3(256), 0, E.stream, E.fisher_gram.as<const double>(),
-                           nblk, dim, scale, E.fisher_out.as<double>() + (size_t)p0 * dim * dim);
+        hipLaunchKernelGGL(gwh_gram_reduce_kernel, dim3(np), dim3(256), 0, F.stream, F.gram.as<const double>(),
+                           nblk, dim, scale, F.out.as<double>() + (size_t)p0 * dim * dim);
         GWH_HIP(hipGetLastError());
         /* the host staging vectors of this chunk die with the iteration */
-        GWH_HIP(hipStreamSynchronize(E.stream));
+        GWH_HIP(hipStreamSynchronize(F.stream));
     }
     if (sigma) {
-        if ((rc = E.out2.ensure((size_t)n * dim * 8)) != GWH_OK) return rc;
-        hipLaunchKernelGGL(gwh_fisher_invert_kernel, dim3((n + 63) / 64), dim3(64), 0, E.stream,
-                           E.fisher_out.as<const double>(), n, dim, E.out2.as<double>());
+        if ((rc = F.sig.ensure((size_t)n * dim * 8)) != GWH_OK) return rc;
+        hipLaunchKernelGGL(gwh_fisher_invert_kernel, dim3((n + 63) / 64), dim3(64), 0, F.stream,
+                           F.out.as<const double>(), n, dim, F.sig.as<double>());
         GWH_HIP(hipGetLastError());
-        GWH_HIP(hipMemcpyAsync(sigma, E.out2.p, (size_t)n * dim * 8, hipMemcpyDeviceToHost, E.stream));
+        GWH_HIP(hipMemcpyAsync(sigma, F.sig.p, (size_t)n * dim * 8, hipMemcpyDeviceToHost, F.stream));
     }
-    GWH_HIP(hipMemcpyAsync(out, E.fisher_out.p, (size_t)n * dim * dim * 8, hipMemcpyDeviceToHost, E.stream));
-    GWH_HIP(hipStreamSynchronize(E.stream));
+    GWH_HIP(hipMemcpyAsync(out, F.out.p, (size_t)n * dim * dim * 8, hipMemcpyDeviceToHost, F.stream));
+    GWH_HIP(hipStreamSynchronize(F.stream));
     return GWH_OK;
+}
+
+int gwh_fisher_batch_with(int event, int model, int basis, int precision, const gwh_params *params, int n, int dim,
+                          double epsilon, double *out, double *sigma, FisherScratch &F)
+{
+    Engine &E = gwh_engine();
+    int rc = gwh_ensure_init();
+    if (rc != GWH_OK) return rc;
+    if (n <= 0) return GWH_OK;
+    if (!params || !out) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    Event ev;
+    {
+        std::lock_guard<std::recursive_mutex> lk(E.mu);
+        if (event < 0 || event >= GWH_MAX_EVENTS || !E.events[event].live)
+            return gwh_set_error(GWH_ERR_INVALID, "unknown event");
+        ev = E.events[event];
+    }
+    if (basis_dim(basis) < 0 || basis_dim(basis) != dim)
+        return gwh_set_error(GWH_ERR_INVALID, "dimension does not match the parameter basis");
+    if (precision != GWH_FP64 && precision != GWH_MIXED) return gwh_set_error(GWH_ERR_INVALID, "bad precision");
+    if ((basis == GWH_BASIS_PV2_FULL || basis == GWH_BASIS_PV2_MCMC) != (model == GWH_IMRPHENOMPV2))
+        return gwh_set_error(GWH_ERR_INVALID, "basis does not belong to the model");
+    /* the event's weights use the likelihood's df = f[N/2]-f[N/2-1]; the Fisher uses f[1]-f[0] */
+    double scale = ev.df_fisher / ev.df_like;
+    return fisher_run(event, model, basis, precision, params, n, 2, epsilon, 0, ev.D, ev.det[0], ev.det[0], scale, out,
+                      sigma, F);
 }
 
 extern "C" {
@@ -737,7 +767,7 @@ static int single_point(const double *frequencies, int length, int basis, int de
                                   0, &ev)) != GWH_OK)
         return rc;
     rc = fisher_run(ev, model, basis, GWH_FP64, params, 1, order, epsilon, nd - 1, 1, detector, reference_detector,
-                    1.0, out, nullptr, deriv);
+                    1.0, out, nullptr, E.fisher, deriv);
     (void)hipStreamSynchronize(E.stream);
     (void)hipFree(E.events[ev].block);
     E.events[ev] = Event();
@@ -771,20 +801,7 @@ int gwh_fisher_batch(int event, int model, int basis, int precision, const gwh_p
     std::lock_guard<std::recursive_mutex> lk(E.mu);
     int rc = gwh_ensure_init();
     if (rc != GWH_OK) return rc;
-    if (n <= 0) return GWH_OK;
-    if (!params || !out) return gwh_set_error(GWH_ERR_INVALID, "null argument");
-    if (event < 0 || event >= GWH_MAX_EVENTS || !E.events[event].live)
-        return gwh_set_error(GWH_ERR_INVALID, "unknown event");
-    if (basis_dim(basis) < 0 || basis_dim(basis) != dim)
-        return gwh_set_error(GWH_ERR_INVALID, "dimension does not match the parameter basis");
-    if (precision != GWH_FP64 && precision != GWH_MIXED) return gwh_set_error(GWH_ERR_INVALID, "bad precision");
-    if ((basis == GWH_BASIS_PV2_FULL || basis == GWH_BASIS_PV2_MCMC) != (model == GWH_IMRPHENOMPV2))
-        return gwh_set_error(GWH_ERR_INVALID, "basis does not belong to the model");
-    const Event &ev = E.events[event];
-    /* the event's weights use the likelihood's df = f[N/2]-f[N/2-1]; the Fisher uses f[1]-f[0] */
-    double scale = ev.df_fisher / ev.df_like;
-    return fisher_run(event, model, basis, precision, params, n, 2, epsilon, 0, ev.D, ev.det[0], ev.det[0], scale, out,
-                      sigma);
+    return gwh_fisher_batch_with(event, model, basis, precision, params, n, dim, epsilon, out, sigma, E.fisher);
 }
 
 } /* extern "C" */
