@@ -44,6 +44,9 @@ EXPORTS = [
     "gwh_calculate_snr_internal", "gwh_data_snr", "gwh_snr_batch",
     "gwh_broker_create", "gwh_broker_destroy", "gwh_broker_submit_loglike", "gwh_broker_submit_fisher",
     "gwh_broker_wait", "gwh_broker_stats",
+    "gwh_loglike_graph_create", "gwh_graph_launch", "gwh_graph_synchronize", "gwh_graph_destroy",
+    "gwh_pt_create", "gwh_pt_run", "gwh_pt_synchronize", "gwh_pt_state", "gwh_pt_last_step", "gwh_pt_counters",
+    "gwh_pt_destroy",
 ]
 
 
@@ -61,6 +64,14 @@ class Params(ctypes.Structure):
                 ("chip", ctypes.c_double), ("phip", ctypes.c_double),
                 ("shift_time", ctypes.c_int32), ("shift_phase", ctypes.c_int32),
                 ("sky_average", ctypes.c_int32), ("dep_postmerger", ctypes.c_int32)]
+
+
+class PTConfig(ctypes.Structure):
+    """gwh_pt_config (include/gwat_hip.h)."""
+    _fields_ = [("n_temps", ctypes.c_int), ("n_chains", ctypes.c_int), ("model", ctypes.c_int),
+                ("precision", ctypes.c_int), ("basis", ctypes.c_int), ("swap_every", ctypes.c_int),
+                ("guard", ctypes.c_int), ("steps_per_graph", ctypes.c_int), ("tol", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
 
 
 PARAMS_DTYPE = np.dtype([("mass1", "f8"), ("mass2", "f8"), ("spin1", "f8", 3), ("spin2", "f8", 3), ("DL", "f8"),
@@ -147,6 +158,15 @@ def lib():
         "gwh_broker_submit_fisher": ([I, I, I, I, I, V, I, I, D, dp, dp, ctypes.POINTER(ctypes.c_longlong)], I),
         "gwh_broker_wait": ([I, ctypes.c_longlong, D, dp], I),
         "gwh_broker_stats": ([I, dp], I),
+        "gwh_loglike_graph_create": ([ip, V, I, I, I, V, I, V, ip], I),
+        "gwh_graph_launch": ([I, V], I), "gwh_graph_synchronize": ([I], I), "gwh_graph_destroy": ([I], I),
+        "gwh_pt_create": ([I, ctypes.POINTER(PTConfig), dp, dp, dp, dp, pp, dp, ip], I),
+        "gwh_pt_run": ([I, I], I), "gwh_pt_synchronize": ([I], I),
+        "gwh_pt_state": ([I, dp, dp, ctypes.POINTER(ctypes.c_longlong)], I),
+        "gwh_pt_last_step": ([I, dp, dp, ip, ip], I),
+        "gwh_pt_counters": ([I, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                             ctypes.POINTER(ctypes.c_longlong)], I),
+        "gwh_pt_destroy": ([I], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

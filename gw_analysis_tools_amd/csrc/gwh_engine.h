@@ -58,6 +58,7 @@ struct Event {
     double df_like = 0;    /* f[N/2] - f[N/2-1]: the likelihood's Simpson step (mcmc_gw.cpp:811) */
     double df_fisher = 0;  /* f[1] - f[0]: the Fisher's Simpson step (fisher.cpp:2746) */
     uint64_t last_use = 0;
+    int pins = 0;          /* step graphs holding this event's buffers: no destroy, eviction or re-tiling */
 };
 
 /* grow-only device buffer */
@@ -65,6 +66,7 @@ struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
     int ensure(size_t bytes);
+    void release();
     template <class T> T *as() { return (T *)p; }
 };
 
@@ -72,6 +74,20 @@ struct DevBuf {
  * the serving broker owns one per stream so batches can be in flight together. */
 struct Scratch {
     DevBuf coef, partial, active, work, wcount;
+    void release() { coef.release(); partial.release(); active.release(); work.release(); wcount.release(); }
+};
+
+/* Options of one batched-likelihood launch sequence beyond the common arguments.
+ *   d_count       device int: only instances i < *d_count are evaluated (read when the kernels run)
+ *   d_mask        device ints, n: only instances with d_mask[i] != 0 are evaluated
+ *   reserve_only  size the scratch buffers for this launch and return without launching
+ *   capturing     the stream is being captured into a graph: no timing events are recorded
+ * Instances that are not evaluated get a NaN result. */
+struct LaunchExtra {
+    const int *d_count = nullptr;
+    const int *d_mask = nullptr;
+    bool reserve_only = false;
+    bool capturing = false;
 };
 
 /* Device scratch and stream of one Fisher computation (the engine's own, or a broker worker's). */
@@ -119,7 +135,8 @@ int gwh_choose_bpl(int N);
 /* the batched likelihood on device buffers (engine lock held by caller) */
 int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
                        double *d_logl, hipStream_t stream, bool timed, double *kernel_ms,
-                       const int *h_events = nullptr, int what = 0, Scratch *scratch = nullptr);
+                       const int *h_events = nullptr, int what = 0, Scratch *scratch = nullptr,
+                       const LaunchExtra *extra = nullptr);
 
 /* Fisher matrices of n points on a resident event with the given scratch (no engine lock needed while
  * it runs; the event must stay alive).  The public entry points use the engine's scratch. */

@@ -75,6 +75,13 @@ int DevBuf::ensure(size_t bytes)
     return GWH_OK;
 }
 
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
 uint64_t gwh_hash_bytes(const void *p, size_t n, uint64_t seed)
 {
     const unsigned char *b = (const unsigned char *)p;
@@ -338,11 +345,16 @@ __global__ __launch_bounds__(64) void gwh_setup_kernel(const gwh_params *__restr
                                                        const double *__restrict__ fits, QnmTable qnm,
                                                        const DetGeom *__restrict__ geo,
                                                        const EventDesc *__restrict__ events,
-                                                       InstCoef *__restrict__ coef, int *__restrict__ active)
+                                                       InstCoef *__restrict__ coef, int *__restrict__ active,
+                                                       const int *__restrict__ d_count, const int *__restrict__ d_mask)
 {
     const int model = MODEL;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if ((d_count && i >= *d_count) || (d_mask && !d_mask[i])) {
+        if (active) active[i] = -1; /* not evaluated: no tiles, NaN result */
+        return;
+    }
     int ev = inst_event ? inst_event[i] : event0;
     const EventDesc &E = events[ev];
     gwh_params p = params[i];
@@ -370,18 +382,19 @@ __global__ __launch_bounds__(64) void gwh_setup_kernel(const gwh_params *__restr
 }
 
 static void launch_setup(int model, int n, hipStream_t st, const gwh_params *params, const int *inst_event,
-                         int event0, InstCoef *coef, int *active)
+                         int event0, InstCoef *coef, int *active, const int *d_count = nullptr,
+                         const int *d_mask = nullptr)
 {
     Engine &E = gwh_engine();
     dim3 g((n + 63) / 64), b(64);
     if (model == GWH_IMRPHENOMPV2)
         hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMPV2>, g, b, 0, st, params, inst_event, event0, n,
                            (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
-                           coef, active);
+                           coef, active, d_count, d_mask);
     else
         hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMD>, g, b, 0, st, params, inst_event, event0, n,
                            (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
-                           coef, active);
+                           coef, active, d_count, d_mask);
 }
 
 int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_params *d_params, int n, InstCoef *d_coef,
@@ -715,7 +728,7 @@ __global__ __launch_bounds__(1024) void gwh_worklist_kernel(WorkMap W, const Eve
     int local = 0;
     for (int i = i0; i < i1; i++) {
         int ev = W.inst_event ? W.inst_event[i] : W.event0;
-        if (events[ev].bpl == bpl) local += active[i];
+        if (events[ev].bpl == bpl) local += max(active[i], 0);
     }
     part[t] = local;
     __syncthreads();
@@ -751,6 +764,10 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const 
     const int nv = 3 * E.D;
     const size_t t0 = (size_t)i * W.tiles0;
     const int nt = active[i];
+    if (nt < 0) { /* instance not evaluated (count or mask of the launch) */
+        if (lane == 0) logl[i] = __longlong_as_double(0x7ff8000000000000ll);
+        return;
+    }
     double s[GWH_PARTIAL_STRIDE];
 #pragma unroll
     for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) s[j] = 0;
@@ -824,9 +841,11 @@ static void launch_wl(int bpl, int blocks, hipStream_t st, const WorkMap &W, con
 
 int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
                        double *d_logl, hipStream_t stream, bool timed, double *kernel_ms, const int *h_events,
-                       int what, Scratch *scratch)
+                       int what, Scratch *scratch, const LaunchExtra *extra)
 {
     Engine &E = gwh_engine();
+    static const LaunchExtra kNone;
+    const LaunchExtra &X = extra ? *extra : kNone;
     DevBuf &s_coef = scratch ? scratch->coef : E.coef, &s_partial = scratch ? scratch->partial : E.partial,
            &s_active = scratch ? scratch->active : E.active, &s_work = scratch ? scratch->work : E.work,
            &s_wcount = scratch ? scratch->wcount : E.wcount;
@@ -871,15 +890,22 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     if ((rc = s_coef.ensure((size_t)n * sizeof(InstCoef))) != GWH_OK) return rc;
     if ((rc = s_partial.ensure((size_t)n * maxtiles * GWH_PARTIAL_STRIDE * sizeof(double))) != GWH_OK) return rc;
     if ((rc = s_active.ensure((size_t)n * sizeof(int))) != GWH_OK) return rc;
+    if (wl) {
+        if ((rc = s_work.ensure((size_t)(ub[0] + ub[1] + ub[2] + ub[3]) * sizeof(int2))) != GWH_OK) return rc;
+        if ((rc = s_wcount.ensure(4 * sizeof(int))) != GWH_OK) return rc;
+    }
+    if (X.reserve_only) return GWH_OK;
     WorkMap W;
     W.inst_event = d_events;
     W.event0 = events0;
     W.n = n;
     W.tiles0 = maxtiles;
-    launch_setup(model, n, stream, d_params, d_events, events0, s_coef.as<InstCoef>(), s_active.as<int>());
+    launch_setup(model, n, stream, d_params, d_events, events0, s_coef.as<InstCoef>(), s_active.as<int>(), X.d_count,
+                 X.d_mask);
     GWH_HIP(hipGetLastError());
+    if (X.capturing) timed = false;
     if (timed) GWH_HIP(hipEventRecord(E.ev0, stream));
-    const bool live = E.kt_on && 2 * (size_t)E.kt_used + 1 < E.kt_ev.size();
+    const bool live = !X.capturing && E.kt_on && 2 * (size_t)E.kt_used + 1 < E.kt_ev.size();
     if (live) GWH_HIP(hipEventRecord(E.kt_ev[2 * E.kt_used], stream));
     const EventDesc *evp = (const EventDesc *)E.d_events;
     const InstCoef *cp = s_coef.as<const InstCoef>();
@@ -887,9 +913,6 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     const int *ap = s_active.as<const int>();
     static const int kB[4] = {1, 2, 4, 8};
     if (wl) {
-        long long tot = ub[0] + ub[1] + ub[2] + ub[3];
-        if ((rc = s_work.ensure((size_t)tot * sizeof(int2))) != GWH_OK) return rc;
-        if ((rc = s_wcount.ensure(4 * sizeof(int))) != GWH_OK) return rc;
         long long off = 0;
         for (int sl = 0; sl < 4; sl++) {
             if (!ub[sl]) continue;
@@ -1085,7 +1108,7 @@ int gwh_set_bins_per_lane(int bpl)
     /* re-tile the resident events (later results may differ in the last bits from earlier ones) */
     for (int i = 0; i < GWH_MAX_EVENTS; i++) {
         Event &ev = E.events[i];
-        if (!ev.live) continue;
+        if (!ev.live || ev.pins > 0) continue; /* events held by a step graph keep their tiling */
         ev.desc.bpl = gwh_choose_bpl(ev.N);
         GWH_HIP(hipMemcpyAsync(E.d_events + i, &ev.desc, sizeof(EventDesc), hipMemcpyHostToDevice, E.stream));
     }
@@ -1150,6 +1173,7 @@ int gwh_event_destroy(int event)
     std::lock_guard<std::recursive_mutex> lk(E.mu);
     if (!E.ready || event < 0 || event >= GWH_MAX_EVENTS || !E.events[event].live)
         return gwh_set_error(GWH_ERR_INVALID, "unknown event");
+    if (E.events[event].pins > 0) return gwh_set_error(GWH_ERR_BUSY, "event is held by a step graph");
     GWH_HIP(hipStreamSynchronize(E.stream));
     GWH_HIP(hipFree(E.events[event].block));
     E.events[event] = Event();
@@ -1274,6 +1298,7 @@ int gwh_autotune_bins_per_lane(const int *events, const int *d_events, int model
         int e = events[i];
         if (e < 0 || e >= GWH_MAX_EVENTS || !E.events[e].live)
             return gwh_set_error(GWH_ERR_INVALID, "unknown event in batch");
+        if (E.events[e].pins > 0) return gwh_set_error(GWH_ERR_BUSY, "event is held by a step graph");
         int N = E.events[e].N, lg = 0;
         while ((2 << lg) <= N && lg < 31) lg++;
         bucket_of[e] = lg;
@@ -1500,6 +1525,7 @@ static int cached_event(const double *const *f, int n, int ndet, const int *dets
             return GWH_OK;
         }
         live_cached++;
+        if (ev.pins > 0) continue; /* held by a step graph: never evicted */
         if (oldest < 0 || ev.last_use < E.events[oldest].last_use) oldest = i;
     }
     if (live_cached >= 64 && oldest >= 0) { /* LRU of implicit events */

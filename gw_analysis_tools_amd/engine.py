@@ -9,13 +9,13 @@ import ctypes
 
 import numpy as np
 
-from . import GWHError, Params, check, lib, PARAMS_DTYPE, IMRPHENOMD, FP64
+from . import GWHError, Params, PTConfig, check, lib, PARAMS_DTYPE, IMRPHENOMD, FP64, BASIS_PHENOMD_MCMC
 
 __all__ = ["fourier_waveform", "fourier_detector_response", "create_coherent_GW_detection", "work_counts",
            "log_likelihood", "log_likelihood_internal", "mcmc_likelihood_extrinsic", "fisher_numerical",
            "fisher_derivatives", "unpack_parameters", "repack_parameters", "detector_id", "dtoa",
-           "gps_to_gmst_radian", "calculate_snr_internal", "data_snr", "snr_batch", "Broker", "Event", "loglike_batch", "fisher_batch", "DeviceArray", "set_bins_per_lane",
-           "synchronize"]
+           "gps_to_gmst_radian", "calculate_snr_internal", "data_snr", "snr_batch", "Broker", "Event", "loglike_batch",
+           "fisher_batch", "DeviceArray", "set_bins_per_lane", "synchronize", "LoglikeGraph", "DevicePT"]
 
 
 def _dp(a):
@@ -342,6 +342,111 @@ class Broker:
     def close(self):
         if self.id >= 0:
             check(lib().gwh_broker_destroy(self.id))
+            self.id = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LoglikeGraph:
+    """gwh_loglike_graph_*: the batched likelihood on fixed device buffers, captured once and replayed."""
+
+    def __init__(self, event, d_params, n, d_logl, model=IMRPHENOMD, precision=FP64, events=None, d_events=None):
+        g = ctypes.c_int(-1)
+        if events is not None:
+            self._ev = np.ascontiguousarray([x.id if isinstance(x, Event) else int(x) for x in events], dtype=np.int32)
+            evp, ev0 = self._ev.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(self._ev[0])
+        else:
+            evp, ev0 = None, (event.id if isinstance(event, Event) else int(event))
+        check(lib().gwh_loglike_graph_create(evp, d_events, ev0, model, precision, d_params, n, d_logl,
+                                             ctypes.byref(g)))
+        self.id = g.value
+
+    def launch(self, stream=None):
+        check(lib().gwh_graph_launch(self.id, stream))
+
+    def synchronize(self):
+        check(lib().gwh_graph_synchronize(self.id))
+
+    def close(self):
+        if self.id >= 0:
+            check(lib().gwh_graph_destroy(self.id))
+            self.id = -1
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DevicePT:
+    """gwh_pt_*: parallel tempering with the whole step (proposal, prior, likelihoods, Metropolis-Hastings,
+    ladder swaps) on the device, one graph replay per step.  Instance i = temp * n_chains + chain."""
+
+    def __init__(self, event, template, theta0, widths, temps, n_chains, lower, upper, model=IMRPHENOMD,
+                 precision=FP64, basis=BASIS_PHENOMD_MCMC, seed=20261203, swap_every=10, guard=True, tol=1e-3,
+                 steps_per_graph=1):
+        temps = np.ascontiguousarray(temps, dtype=np.float64)
+        self.nt, self.nc = len(temps), int(n_chains)
+        self.n = self.nt * self.nc
+        self.dim = len(lower)
+        th = np.ascontiguousarray(np.broadcast_to(np.asarray(theta0, dtype=np.float64), (self.n, self.dim)))
+        w = np.ascontiguousarray(np.broadcast_to(np.asarray(widths, dtype=np.float64), (self.n, self.dim)))
+        lo = np.ascontiguousarray(lower, dtype=np.float64)
+        hi = np.ascontiguousarray(upper, dtype=np.float64)
+        if isinstance(template, np.void):
+            tmpl = np.array([template], dtype=PARAMS_DTYPE)
+        else:
+            tmpl = _as_params_array(template)[:1].copy()
+        cfg = PTConfig(n_temps=self.nt, n_chains=self.nc, model=model, precision=precision, basis=basis,
+                       swap_every=int(swap_every), guard=int(bool(guard)), steps_per_graph=int(steps_per_graph),
+                       tol=float(tol), seed=int(seed))
+        h = ctypes.c_int(-1)
+        check(lib().gwh_pt_create(event.id if isinstance(event, Event) else int(event), ctypes.byref(cfg),
+                                  _dp(temps), _dp(w), _dp(lo), _dp(hi),
+                                  tmpl.ctypes.data_as(ctypes.POINTER(Params)), _dp(th), ctypes.byref(h)))
+        self.id = h.value
+        self.steps_per_graph = int(steps_per_graph)
+
+    def run(self, steps, sync=True):
+        check(lib().gwh_pt_run(self.id, int(steps)))
+        if sync:
+            check(lib().gwh_pt_synchronize(self.id))
+
+    def synchronize(self):
+        check(lib().gwh_pt_synchronize(self.id))
+
+    def state(self):
+        th = np.zeros((self.n, self.dim))
+        ll = np.zeros(self.n)
+        st = ctypes.c_longlong(0)
+        check(lib().gwh_pt_state(self.id, _dp(th), _dp(ll), ctypes.byref(st)))
+        return th, ll, st.value
+
+    def last_step(self):
+        prop = np.zeros((self.n, self.dim))
+        llp = np.zeros(self.n)
+        acc = np.zeros(self.n, dtype=np.int32)
+        sw = np.zeros(max(self.nt - 1, 0) * self.nc, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        check(lib().gwh_pt_last_step(self.id, _dp(prop), _dp(llp), acc.ctypes.data_as(ip), sw.ctypes.data_as(ip)))
+        return prop, llp, acc.astype(bool), sw.reshape(max(self.nt - 1, 0), self.nc).astype(bool)
+
+    def counters(self):
+        acc = np.zeros(self.n, dtype=np.int64)
+        sw = np.zeros(max(self.nt - 1, 0) * self.nc, dtype=np.int64)
+        g = ctypes.c_longlong(0)
+        lp = ctypes.POINTER(ctypes.c_longlong)
+        check(lib().gwh_pt_counters(self.id, acc.ctypes.data_as(lp), sw.ctypes.data_as(lp), ctypes.byref(g)))
+        return acc, sw.reshape(max(self.nt - 1, 0), self.nc), g.value
+
+    def close(self):
+        if self.id >= 0:
+            check(lib().gwh_pt_destroy(self.id))
             self.id = -1
 
     def __del__(self):

@@ -252,6 +252,59 @@ int gwh_broker_wait(int broker, long long ticket, double timeout_ms, double *lat
  * peak device bytes (resident events + broker scratch), tickets not yet waited for */
 int gwh_broker_stats(int broker, double *stats);
 
+/* ---- sampler steps as replayable HIP graphs ---- */
+
+/* Likelihood step of a sampler that makes its own proposals: the launch sequence of
+ * gwh_loglike_batch_device (d_events, events NULL) or gwh_loglike_multi_device (both given) on fixed device
+ * buffers, captured once; every gwh_graph_launch reads d_params and writes d_logl as they are when it runs
+ * (the per-chain likelihood calls of the reference sampler, mcmc_sampler.cpp:4585-4600, in one replay).
+ * The events stay held (gwh_event_destroy returns GWH_ERR_BUSY) until gwh_graph_destroy.  Launches of one
+ * graph must be ordered (one stream): they share the graph's scratch. */
+int gwh_loglike_graph_create(const int *events, const int *d_events, int events0, int model, int precision,
+                             const void *d_params, int n, double *d_logl, int *graph);
+int gwh_graph_launch(int graph, void *stream); /* NULL: the graph's own stream */
+int gwh_graph_synchronize(int graph);
+int gwh_graph_destroy(int graph);
+
+/* Parallel tempering on the device: n_temps x n_chains chains (instance i = temp * n_chains + chain) on one
+ * resident event, one whole step per graph replay with no host round trip:
+ *   proposal   gaussian_step (src/mcmc_sampler_internals.cpp:364-409): one dimension chosen uniformly,
+ *              moved by widths[i * dim + d] * N(0, 1)
+ *   prior      a uniform box [lower, upper] in the sampling basis (outside: rejected, no likelihood)
+ *   MH         mcmc_step (src/mcmc_sampler_internals.cpp:35-145): reject iff (ll' - ll) / T < log U
+ *   swaps      after every swap_every-th step, chain_swap / single_chain_swap
+ *              (src/mcmc_sampler_internals.cpp:1086-1180) along every ladder, pairs (k, k+1) in increasing k:
+ *              reject iff exp((ll1 - ll2) / T2 - (ll1 - ll2) / T1) < U
+ * The sampling basis is "MCMC_IMRPhenomD" (GWH_BASIS_PHENOMD_MCMC, 11 parameters) or "MCMC_IMRPhenomPv2"
+ * (GWH_BASIS_PV2_MCMC, 15), mapped to parameter sets by repack_parameters (src/fisher.cpp:2162-2511) over
+ * `tmpl`, which supplies every field the basis does not hold.  Random numbers: Philox4x32-10 keyed by the
+ * seed, counter (instance, step, stream); a run is reproducible from (seed, step).  In GWH_MIXED precision
+ * with guard = 1, decisions within 2 tol of their threshold (in log-likelihood, scaled by the temperatures)
+ * are re-decided with FP64 likelihoods inside the same graph, so the run takes the FP64 run's decisions
+ * while the mixed error stays within tol.  steps_per_graph steps are captured in one graph. */
+typedef struct gwh_pt_config {
+    int n_temps, n_chains; /* 1..64, 1..1024 */
+    int model, precision, basis;
+    int swap_every;        /* 0: never */
+    int guard;
+    int steps_per_graph;   /* >= 1 */
+    double tol;            /* the mixed likelihood error bound of the guard band (1e-3) */
+    uint64_t seed;
+} gwh_pt_config;
+/* temps: n_temps; widths: n x dim; lower, upper: dim; theta0: n x dim.  Computes the initial likelihoods. */
+int gwh_pt_create(int event, const gwh_pt_config *cfg, const double *temps, const double *widths,
+                  const double *lower, const double *upper, const gwh_params *tmpl, const double *theta0, int *pt);
+int gwh_pt_run(int pt, int steps); /* enqueues `steps` steps (a multiple of steps_per_graph); returns at once */
+int gwh_pt_synchronize(int pt);
+/* after the enqueued steps: states (n x dim), their log-likelihoods, steps done (any pointer may be NULL) */
+int gwh_pt_state(int pt, double *theta, double *ll, long long *step);
+/* the last step: proposals (n x dim), their log-likelihoods (NaN outside the prior), accepted (0/1),
+ * swapped pairs ((n_temps - 1) x n_chains, pair k of chain c at k * n_chains + c) */
+int gwh_pt_last_step(int pt, double *prop, double *llp, int *accept, int *swapped);
+/* accepted proposals per chain (n), accepted swaps per pair ((n_temps - 1) x n_chains), FP64 re-evaluations */
+int gwh_pt_counters(int pt, long long *accepts, long long *swaps, long long *guard_evals);
+int gwh_pt_destroy(int pt);
+
 /* device memory helpers for callers without their own allocator */
 void *gwh_device_alloc(size_t bytes);
 int gwh_device_free(void *ptr);
