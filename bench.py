@@ -182,6 +182,62 @@ def bench_fisher(G, W, ev, n_points=2048):
     return out
 
 
+def bench_pt_device(G, W, ev, inj, temps, n_chains, d_params, n, steps=200):
+    """The whole parallel-tempering step on the device (proposal, prior, likelihood of every chain,
+    Metropolis-Hastings, ladder swaps every 10th step) replayed from one HIP graph per step, against the
+    host-driven step (numpy proposals and decisions, one engine launch per step) and the captured
+    likelihood step alone (graph replay vs eager enqueue, each followed by a synchronisation)."""
+    from gw_analysis_tools_amd import pt
+    th0 = pt.theta_from_params(inj)
+    widths = np.array([0.02, 0.02, 0.05, 0.03, 0.05, 2e-4, 0.03, 2e-4, 2e-3, 0.05, 0.05])
+    out = {"instances": len(temps) * n_chains, "steps": steps, "swap_every": 10}
+    for name, prec in (("fp64", G.FP64), ("mixed_guard", G.MIXED)):
+        d = G.DevicePT(ev, inj, th0, widths, temps, n_chains, pt.LOWER, pt.UPPER, precision=prec, seed=1,
+                       swap_every=10, steps_per_graph=10)
+        d.run(20)
+        a = time.perf_counter()
+        d.run(steps)
+        el = time.perf_counter() - a
+        acc, sw, guard = d.counters()
+        out["device_step_us_" + name] = el / steps * 1e6
+        out["device_evals_per_s_" + name] = steps * d.n / el
+        out["guard_fp64_evals_" + name] = int(guard)
+        out["acceptance_" + name] = float(acc.sum()) / ((steps + 20) * d.n)
+        d.close()
+    for name, prec in (("fp64", G.FP64), ("mixed_guard", G.MIXED)):
+        e = pt.PTEnsemble(ev, inj, th0, widths, temps, n_chains, precision=prec)
+        e.run(3)
+        hs = max(20, steps // 4)
+        a = time.perf_counter()
+        e.run(hs)
+        out["host_step_us_" + name] = (time.perf_counter() - a) / hs * 1e6
+    d_ll = G.DeviceArray(n * 8)
+    lib = G.lib()
+    for name, prec in (("fp64", G.FP64), ("mixed", G.MIXED)):
+        g = G.LoglikeGraph(ev, d_params.ptr, n, d_ll.ptr, precision=prec)
+        for _ in range(10):
+            g.launch()
+        g.synchronize()
+        a = time.perf_counter()
+        for _ in range(steps):
+            g.launch()
+            g.synchronize()
+        out["graph_likelihood_step_us_" + name] = (time.perf_counter() - a) / steps * 1e6
+        a = time.perf_counter()
+        for _ in range(steps):
+            G.check(lib.gwh_loglike_batch_device(None, ev.id, G.IMRPHENOMD, prec, d_params.ptr, n, d_ll.ptr, None))
+            G.synchronize()
+        out["eager_likelihood_step_us_" + name] = (time.perf_counter() - a) / steps * 1e6
+        g.close()
+    d_ll.free()
+    log("device PT step: fp64 %.1f us, mixed+guard %.1f us (%d guard evals); host-driven %.1f / %.1f us; "
+        "likelihood step graph %.1f / eager %.1f us (mixed)"
+        % (out["device_step_us_fp64"], out["device_step_us_mixed_guard"], out["guard_fp64_evals_mixed_guard"],
+           out["host_step_us_fp64"], out["host_step_us_mixed_guard"], out["graph_likelihood_step_us_mixed"],
+           out["eager_likelihood_step_us_mixed"]))
+    return out
+
+
 def bench_serving(G, W, rates, seconds, fisher_fraction=0.1, clients=32):
     """Open-loop Poisson arrivals from `clients` threads at each aggregate rate: a request is a
     log-likelihood batch of U{16..1024} instances (IMRPhenomD or IMRPhenomPv2, 50/50, mixed precision) on
@@ -296,6 +352,7 @@ def main():
     ap.add_argument("--no-c2", action="store_true", help="skip the uneven multi-event batch leg")
     ap.add_argument("--no-fisher", action="store_true", help="skip the batched Fisher leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (broker) leg")
+    ap.add_argument("--no-pt", action="store_true", help="skip the device parallel-tempering leg")
     ap.add_argument("--serving-rates", default="50,100,200,500,1000,2000")
     ap.add_argument("--serving-seconds", type=float, default=2.0)
     ap.add_argument("--c2-steps", type=int, default=20)
@@ -484,6 +541,15 @@ def main():
             log("Fisher leg failed:", e)
             fisher = {"error": str(e)}
 
+    # ---- the whole PT step on the device, one graph replay per step ----
+    ptd = None
+    if not args.no_pt:
+        try:
+            ptd = bench_pt_device(G, W, ev, inj, cfg["temps"], cfg["n_chains"], d_params, n)
+        except Exception as e:  # reported, not fatal for the headline
+            log("device PT leg failed:", e)
+            ptd = {"error": str(e)}
+
     # ---- uneven batch: 256 events of 1e3..2.6e5 bins, 8 chains each, one launch per step ----
     c2 = None
     if not args.no_c2:
@@ -545,6 +611,7 @@ def main():
         "uneven_batch_c2": c2,
         "fisher_c3": fisher,
         "serving_c4": serving,
+        "pt_step_on_device_c1": ptd,
     }
     print(json.dumps(rec))
 

@@ -36,14 +36,26 @@ struct EventDesc {
     int det[GWH_MAXDET];
 };
 
+/* Loads through global-address-space pointers.  Pointers held in device structs are generic, and generic
+ * (flat) loads count against the scalar-memory counter as well as the vector one, so every wait for a
+ * scalar load of the instance coefficients would also wait for the bin data in flight. */
+#define GWH_GLOBAL __attribute__((address_space(1)))
+template <class T> __device__ __forceinline__ const GWH_GLOBAL T *gwh_g(const T *p) { return (const GWH_GLOBAL T *)p; }
+__device__ __forceinline__ double2 gwh_ld2(const double2 *p, size_t k)
+{
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    const v2 v = ((const GWH_GLOBAL v2 *)p)[k];
+    return make_double2(v.x, v.y);
+}
+
 __device__ __forceinline__ BinPow gwh_load_bin(const EventDesc &E, int k)
 {
     BinPow b;
-    b.lf = E.lgf[k];
-    b.f13 = E.f13[k];
-    b.fm13 = E.fm13[k];
-    b.fm76 = E.fm76[k];
-    b.f34 = E.f34[k];
+    b.lf = gwh_g(E.lgf)[k];
+    b.f13 = gwh_g(E.f13)[k];
+    b.fm13 = gwh_g(E.fm13)[k];
+    b.fm76 = gwh_g(E.fm76)[k];
+    b.f34 = gwh_g(E.f34)[k];
     return b;
 }
 

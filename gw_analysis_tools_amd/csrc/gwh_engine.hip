@@ -4,14 +4,16 @@
  *
  * Hot path (one MCMC step):
  *   1. gwh_setup_kernel      one lane per instance: FP64 InstCoef record
- *   2. loglike kernel        one block per (bin-tile, instance) -- a dense
- *                            tiles x instances grid for one event, flattened
- *                            (instance, tile) work lists for multi-event
- *                            batches; tiles above 0.2/M of an ascending grid
- *                            do nothing; BPL bins per lane, the instance record
- *                            uniform (scalar loads); per detector accumulate
- *                            |h|^2 w/S and h* d w/S in FP64; wave shuffle + LDS
- *                            tree per block -> per-item partials
+ *   2. loglike kernel        (instance, bin-tile) work items: a persistent grid
+ *                            of as many blocks as fit on the chip at once
+ *                            strides over the items in tile-major order (for
+ *                            one event, or events whose sizes are known only on
+ *                            the device), or one block per item of flattened
+ *                            work lists for multi-event batches; tiles above
+ *                            0.2/M of an ascending grid do nothing; BPL bins per
+ *                            lane, the instance record uniform (scalar loads);
+ *                            per detector accumulate |h|^2 w/S and h* d w/S in
+ *                            FP64; wave shuffle + LDS tree per item -> partials
  *   3. gwh_finalize_kernel   one wave per instance sums its items in a fixed
  *                            order (deterministic, batch-composition free)
  *
@@ -123,6 +125,8 @@ static void spline_c(const double *x, const double *y, int n, double *c)
     for (int i = m - 2; i >= 0; i--) c[i + 1] = (rh[i] - of[i] * c[i + 2]) / dg[i];
 }
 
+void gwh_persist_warmup();
+
 static const double k_geometric_factor[GWH_NDET_TABLE] = {1., 1., 1., 1., 1., 1., 0.86602540378, 0.86602540378,
                                                           0.86602540378};
 
@@ -170,6 +174,7 @@ int gwh_ensure_init()
     GWH_HIP(hipMalloc(&E.d_events, sizeof(EventDesc) * GWH_MAX_EVENTS));
     GWH_HIP(hipMemset(E.d_events, 0, sizeof(EventDesc) * GWH_MAX_EVENTS));
     E.events.assign(GWH_MAX_EVENTS, Event());
+    gwh_persist_warmup();
     E.ready = true;
     return GWH_OK;
 }
@@ -558,17 +563,17 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
 #pragma unroll
         for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) acc[j] = 0;
         /* skip tiles entirely above the 0.2/M cutoff (h = 0 there, IMRPhenomD.cpp:475) */
-        double fmin_tile = E.chunk_fmin[tile * BPL];
+        double fmin_tile = gwh_g(E.chunk_fmin)[tile * BPL];
 #pragma unroll
         for (int j = 1; j < BPL; j++)
-            if (tile * BPL + j < E.nchunk) fmin_tile = fmin(fmin_tile, E.chunk_fmin[tile * BPL + j]);
+            if (tile * BPL + j < E.nchunk) fmin_tile = fmin(fmin_tile, gwh_g(E.chunk_fmin)[tile * BPL + j]);
         if (!(fmin_tile > C.fcut)) {
             if (PREC == GWH_FP64) {
 #pragma unroll
                 for (int j = 0; j < BPL; j++) {
                     int k = tile * GWH_TPB * BPL + j * GWH_TPB + threadIdx.x;
                     if (k >= N) break;
-                    double f = E.f[k];
+                    double f = gwh_g(E.f)[k];
                     if (f > C.fcut) continue;
                     const BinPow bp = gwh_load_bin(E, k);
                     if (MODEL == GWH_IMRPHENOMD) {
@@ -579,8 +584,8 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                             if (d >= D) break;
                             double s, c;
                             sincos(phi + C.det[d][3] * f, &s, &c);
-                            double ws = E.wS[(size_t)d * N + k];
-                            double2 dw = E.dwS[(size_t)d * N + k];
+                            double ws = gwh_g(E.wS)[(size_t)d * N + k];
+                            double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
                             acc[3 * d] += A * A * ws;
                             acc[3 * d + 1] += A * (c * dw.x - s * dw.y);
                             acc[3 * d + 2] += A * (c * dw.y + s * dw.x);
@@ -595,8 +600,8 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                             double Gi = C.det[d][0] * hpi + C.det[d][1] * hci;
                             double s, c;
                             sincos(Phi + C.det[d][3] * f, &s, &c);
-                            double ws = E.wS[(size_t)d * N + k];
-                            double2 dw = E.dwS[(size_t)d * N + k];
+                            double ws = gwh_g(E.wS)[(size_t)d * N + k];
+                            double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
                             acc[3 * d] += A * A * (Gr * Gr + Gi * Gi) * ws;
                             acc[3 * d + 1] += A * (dw.x * (Gr * c + Gi * s) - dw.y * (Gr * s - Gi * c));
                         }
@@ -607,7 +612,7 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                 for (int j = 0; j < BPL; j++) {
                     int k = tile * GWH_TPB * BPL + j * GWH_TPB + threadIdx.x;
                     if (k >= N) break;
-                    double f = E.f[k];
+                    double f = gwh_g(E.f)[k];
                     if (f > C.fcut) continue;
                     const BinPow bp = gwh_load_bin(E, k);
                     if (MODEL == GWH_IMRPHENOMD) {
@@ -619,8 +624,8 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                             float sf, cf;
                             sincos_turns(phi + C.det[d][3] * f, sf, cf);
                             const double s = sf, c = cf;
-                            const double ws = E.wS[(size_t)d * N + k];
-                            const double2 dw = E.dwS[(size_t)d * N + k];
+                            const double ws = gwh_g(E.wS)[(size_t)d * N + k];
+                            const double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
                             acc[3 * d] += a * a * ws;
                             acc[3 * d + 1] += a * (c * dw.x - s * dw.y);
                             acc[3 * d + 2] += a * (c * dw.y + s * dw.x);
@@ -676,8 +681,8 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                             float sf, cf;
                             sincos_turns(Phi + C.det[d][3] * f, sf, cf);
                             const double s = sf, c = cf;
-                            const double ws = E.wS[(size_t)d * N + k];
-                            const double2 dw = E.dwS[(size_t)d * N + k];
+                            const double ws = gwh_g(E.wS)[(size_t)d * N + k];
+                            const double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
                             acc[3 * d] += a * a * (Gr * Gr + Gi * Gi) * ws;
                             acc[3 * d + 1] += a * (dw.x * (Gr * c + Gi * s) - dw.y * (Gr * s - Gi * c));
                         }
@@ -687,16 +692,6 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
         }
         block_reduce_store<GWH_PARTIAL_STRIDE>(acc, partial + (size_t)w * GWH_PARTIAL_STRIDE, lds, 3 * D);
     }
-}
-
-/* the dense grid: tiles x instances (one event, or events whose sizes are known only on the device) */
-template <int MODEL, int PREC, int BPL>
-__global__ __launch_bounds__(GWH_TPB) void gwh_loglike_kernel(WorkMap W, const EventDesc *__restrict__ events,
-                                                              const InstCoef *__restrict__ coefs,
-                                                              double *__restrict__ partial, const int *__restrict__ active)
-{
-    __shared__ double lds[GWH_TPB / 64][GWH_PARTIAL_STRIDE];
-    gwh_loglike_tile<MODEL, PREC, BPL>(W, events, coefs, partial, active, blockIdx.y, blockIdx.x, lds);
 }
 
 /* the flattened (instance, tile) work list of one bins-per-lane class: one block per live work item,
@@ -713,6 +708,29 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_loglike_wl_kernel(WorkMap W, cons
     if ((int)blockIdx.x >= *count) return;
     const int2 wi = work[blockIdx.x];
     gwh_loglike_tile<MODEL, PREC, BPL>(W, events, coefs, partial, active, wi.x, wi.y, lds);
+}
+
+/* The persistent grid: as many blocks as are resident on the chip at once stride over the (instance, tile)
+ * items in tile-major order (item w = tile * n + instance).  Consecutive blocks go to different XCDs, so
+ * the instances of every tile spread over all XCDs; a grid with one block per item in instance-major order
+ * would send tile t of every instance to the same XCD, and the tiles below an instance's 0.2/M cutoff --
+ * the only ones with work -- to a few XCDs.  Items above the cutoff cost one load.  count (optional,
+ * device): only instances i < *count are visited, so a launch sized for the largest possible count costs
+ * little when the count is small. */
+template <int MODEL, int PREC, int BPL>
+__global__ __launch_bounds__(GWH_TPB) void gwh_loglike_persist_kernel(WorkMap W, const EventDesc *__restrict__ events,
+                                                                      const InstCoef *__restrict__ coefs,
+                                                                      double *__restrict__ partial,
+                                                                      const int *__restrict__ active,
+                                                                      const int *__restrict__ count)
+{
+    __shared__ double lds[GWH_TPB / 64][GWH_PARTIAL_STRIDE];
+    const int nn = count ? min(*count, W.n) : W.n;
+    const int items = nn * W.tiles0;
+    for (int w = blockIdx.x; w < items; w += gridDim.x) {
+        gwh_loglike_tile<MODEL, PREC, BPL>(W, events, coefs, partial, active, w % nn, w / nn, lds);
+        __syncthreads(); /* the LDS of the block reduction is reused by the next item */
+    }
 }
 
 /* Work list of one bins-per-lane class (single block): instance i contributes active[i] items if its
@@ -754,11 +772,11 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const 
                                                                const InstCoef *__restrict__ coefs,
                                                                const double *__restrict__ partial,
                                                                const int *__restrict__ active, int model, int what,
-                                                               double *__restrict__ logl)
+                                                               double *__restrict__ logl, const int *__restrict__ count)
 {
     const int i = blockIdx.x * (GWH_TPB / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (i >= W.n) return; /* whole waves leave together */
+    if (i >= W.n || (count && i >= *count)) return; /* whole waves leave together */
     const int ev = W.inst_event ? W.inst_event[i] : W.event0;
     const EventDesc &E = events[ev];
     const int nv = 3 * E.D;
@@ -814,15 +832,55 @@ int gwh_choose_bpl(int N)
     return 1;
 }
 
+/* resident blocks of one persistent-kernel instantiation on the whole chip (cached per instantiation) */
+template <int MODEL, int PREC, int BPL> static int persist_grid()
+{
+    static int cached = 0;
+    if (!cached) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gwh_loglike_persist_kernel<MODEL, PREC, BPL>, GWH_TPB,
+                                                         0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 4;
+        cached = per_cu * gwh_engine().num_cu;
+    }
+    return cached;
+}
+
+/* queried once at initialisation, outside any stream capture */
+template <int MODEL, int PREC> static void persist_warm()
+{
+    persist_grid<MODEL, PREC, 1>();
+    persist_grid<MODEL, PREC, 2>();
+    persist_grid<MODEL, PREC, 4>();
+    persist_grid<MODEL, PREC, 8>();
+}
+void gwh_persist_warmup()
+{
+    persist_warm<GWH_IMRPHENOMD, GWH_FP64>();
+    persist_warm<GWH_IMRPHENOMD, GWH_MIXED>();
+    persist_warm<GWH_IMRPHENOMPV2, GWH_FP64>();
+    persist_warm<GWH_IMRPHENOMPV2, GWH_MIXED>();
+}
+
+template <int MODEL, int PREC, int BPL>
+static void launch_persist_b(long long items, hipStream_t st, const WorkMap &W, const EventDesc *ev, const InstCoef *c,
+                             double *part, const int *act, const int *count)
+{
+    const int blocks = (int)std::max(1LL, std::min<long long>(items, persist_grid<MODEL, PREC, BPL>()));
+    hipLaunchKernelGGL((gwh_loglike_persist_kernel<MODEL, PREC, BPL>), dim3(blocks), dim3(GWH_TPB), 0, st, W, ev, c,
+                       part, act, count);
+}
+
 template <int MODEL, int PREC>
-static void launch_bpl(int bpl, dim3 grid, hipStream_t st, const WorkMap &W, const EventDesc *ev, const InstCoef *c,
-                       double *part, const int *act)
+static void launch_persist(int bpl, long long items, hipStream_t st, const WorkMap &W, const EventDesc *ev,
+                           const InstCoef *c, double *part, const int *act, const int *count)
 {
     switch (bpl) {
-    case 1: hipLaunchKernelGGL((gwh_loglike_kernel<MODEL, PREC, 1>), grid, dim3(GWH_TPB), 0, st, W, ev, c, part, act); break;
-    case 2: hipLaunchKernelGGL((gwh_loglike_kernel<MODEL, PREC, 2>), grid, dim3(GWH_TPB), 0, st, W, ev, c, part, act); break;
-    case 4: hipLaunchKernelGGL((gwh_loglike_kernel<MODEL, PREC, 4>), grid, dim3(GWH_TPB), 0, st, W, ev, c, part, act); break;
-    default: hipLaunchKernelGGL((gwh_loglike_kernel<MODEL, PREC, 8>), grid, dim3(GWH_TPB), 0, st, W, ev, c, part, act); break;
+    case 1: launch_persist_b<MODEL, PREC, 1>(items, st, W, ev, c, part, act, count); break;
+    case 2: launch_persist_b<MODEL, PREC, 2>(items, st, W, ev, c, part, act, count); break;
+    case 4: launch_persist_b<MODEL, PREC, 4>(items, st, W, ev, c, part, act, count); break;
+    default: launch_persist_b<MODEL, PREC, 8>(items, st, W, ev, c, part, act, count); break;
     }
 }
 
@@ -936,18 +994,18 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     }
     for (int sl = 0; sl < 4 && !wl; sl++) {
         if (!tiles_b[sl]) continue;
-        dim3 grid(tiles_b[sl], n);
-        int bpl = kB[sl];
+        const long long items = (long long)n * maxtiles;
+        const int bpl = kB[sl];
         if (model == GWH_IMRPHENOMD) {
             if (precision == GWH_FP64)
-                launch_bpl<GWH_IMRPHENOMD, GWH_FP64>(bpl, grid, stream, W, evp, cp, pp, ap);
+                launch_persist<GWH_IMRPHENOMD, GWH_FP64>(bpl, items, stream, W, evp, cp, pp, ap, X.d_count);
             else
-                launch_bpl<GWH_IMRPHENOMD, GWH_MIXED>(bpl, grid, stream, W, evp, cp, pp, ap);
+                launch_persist<GWH_IMRPHENOMD, GWH_MIXED>(bpl, items, stream, W, evp, cp, pp, ap, X.d_count);
         } else {
             if (precision == GWH_FP64)
-                launch_bpl<GWH_IMRPHENOMPV2, GWH_FP64>(bpl, grid, stream, W, evp, cp, pp, ap);
+                launch_persist<GWH_IMRPHENOMPV2, GWH_FP64>(bpl, items, stream, W, evp, cp, pp, ap, X.d_count);
             else
-                launch_bpl<GWH_IMRPHENOMPV2, GWH_MIXED>(bpl, grid, stream, W, evp, cp, pp, ap);
+                launch_persist<GWH_IMRPHENOMPV2, GWH_MIXED>(bpl, items, stream, W, evp, cp, pp, ap, X.d_count);
         }
     }
     GWH_HIP(hipGetLastError());
@@ -955,7 +1013,7 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     if (live) GWH_HIP(hipEventRecord(E.kt_ev[2 * E.kt_used++ + 1], stream));
     hipLaunchKernelGGL(gwh_finalize_kernel, dim3((n + GWH_TPB / 64 - 1) / (GWH_TPB / 64)), dim3(GWH_TPB), 0, stream,
                        W, evp, cp,
-                       (const double *)pp, ap, model, what, d_logl);
+                       (const double *)pp, ap, model, what, d_logl, X.d_count);
     GWH_HIP(hipGetLastError());
     if (timed && kernel_ms) {
         GWH_HIP(hipEventSynchronize(E.ev1));
