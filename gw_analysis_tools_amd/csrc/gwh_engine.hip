@@ -415,11 +415,31 @@ int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_para
 /* reductions                                                                 */
 /* ------------------------------------------------------------------------- */
 
+/* lane move of an FP64 value by a DPP pattern (two 32-bit moves) */
+template <int CTRL> __device__ __forceinline__ double dpp_move(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+/* Sum over the 64 lanes of a wave in a fixed order, with DPP lane moves on the vector ALU instead of
+ * LDS permutes: pairs (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1), half rows (row_half_mirror), rows
+ * (row_mirror); then the four row sums, read from lanes 0, 16, 32, 48, added as (r0 + r1) + (r2 + r3).
+ * Every lane must be active; the result is uniform. */
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_move<0xB1>(v);
+    v += dpp_move<0x4E>(v);
+    v += dpp_move<0x141>(v);
+    v += dpp_move<0x140>(v);
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 /* Block-reduce NV values per thread (fixed order) and write them to out[0..NV) */
@@ -476,6 +496,14 @@ __device__ __forceinline__ double exp_mixed(double x)
 /* per-bin mixed-precision evaluators                                         */
 /* ------------------------------------------------------------------------- */
 
+/* 1/z in FP64: hardware reciprocal and one Newton step (no IEEE division sequence) */
+__device__ __forceinline__ double rcp_newton(double z)
+{
+    double r = __builtin_amdgcn_rcp(z);
+    r = fma(r, fma(-z, r, 1.0), r);
+    return r;
+}
+
 /* z^{-1/2} in FP64 from an FP32 v_rsq_f32 seed and two Newton steps */
 __device__ __forceinline__ double rsqrt_mixed(double z)
 {
@@ -503,7 +531,7 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, c
         ar = pn + u3 * u3 * u * (C.rho[0] + u * (C.rho[1] + u * C.rho[2]));
     } else if (f > C.f3) {
         double dl = f - C.fRD;
-        ar = C.mr_k * exp_mixed(-C.mr_g2 * dl) / (dl * dl + C.mr_w2);
+        ar = C.mr_k * exp_mixed(-C.mr_g2 * dl) * rcp_newton(dl * dl + C.mr_w2);
     } else {
         ar = C.delta[0] + Mf * (C.delta[1] + Mf * (C.delta[2] + Mf * (C.delta[3] + Mf * C.delta[4])));
     }
@@ -524,7 +552,7 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, c
               (C.sig[0] * Mf + C.sig[1] * Mf43 + C.sig[2] * Mf43 * u + C.sig[3] * Mf * Mf);
     } else if (f > C.f2p) {
         psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] * (r2 * r) + C.alpha[3] * (C.M34 * bp.f34) +
-              C.alpha[4] * atan((f - C.alpha[5]) / C.fdamp);
+              C.alpha[4] * atan((f - C.alpha[5]) * C.inv_fdamp);
     } else {
         double r3 = r2 * r;
         psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] * (r3 * r3 * r3);

@@ -62,7 +62,7 @@ struct InstCoef {
     double wf_a, wf_b; /* polarisation constants: PhenomD ((1+cos^2 i)/2, cos i); Pv2 (cos 2 zeta, sin 2 zeta) */
     /* powers of the total mass that pair with the tabulated powers of each bin: (Mf)^p = M^p f^p */
     double M13, Mm13, Mm76, M34, lgM, lgPiM;
-    double pad;
+    double inv_fdamp; /* 1 / fdamp */
 };
 
 /* Powers of one frequency bin that every instance needs (log f, f^(1/3), f^(-1/3), f^(-7/6), f^(3/4)).
@@ -705,7 +705,7 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
     const DetGeom &g0 = geo[dets[0]];
     C.wf_a = pv2 ? c2z : kplus;
     C.wf_b = pv2 ? s2z : ci;
-    C.pad = 0;
+    C.inv_fdamp = 1.0 / S.fdamp;
     /* detector_util.cpp:900-1016: polarisation basis (shared by every detector) */
     const double gha = p.gmst - p.RA;
     double cg, sg, cd, sd, cp, sp;
