@@ -440,7 +440,7 @@ def main():
             traffic = None
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_VALU_TFLOPS, "traffic": traffic,
-                "kernel": "gwh_loglike_kernel<IMRPhenomD, mixed>", "kernel_ms": kernel_ms,
+                "kernel": "gwh_loglike_persist_kernel<IMRPhenomD, mixed, 8>", "kernel_ms": kernel_ms,
                 "fe_per_launch": fe, "hbm_algorithmic_GBs": bytes_alg / (kernel_ms * 1e-3) / 1e9,
                 "hbm_peak_GBs": PEAK_HBM_GBS}
     if args.copy_calibration_mib:
