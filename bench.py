@@ -62,7 +62,7 @@ def bench_uneven(G, W, steps):
     for sp in specs:
         f, psd = sp["f"], sp["psd"]
         inj = sp["injection"]
-        pinj = G.make_params(**{k: (list(inj[k]) if k in ("spin1", "spin2") else inj[k].item())
+        pinj = G.make_params(**{k: (list(inj[k]) if inj[k].shape else inj[k].item())
                                 for k in inj.dtype.names})
         resp = G.create_coherent_GW_detection(f, pinj, sp["dets"])
         ev = G.Event(f, [r + W.noise(f, psd, g) for r in resp], [psd] * 3, sp["dets"])
@@ -250,7 +250,7 @@ def bench_serving(G, W, rates, seconds, fisher_fraction=0.1, clients=32):
     for sp in specs:
         f, psd = sp["f"], sp["psd"]
         inj = sp["injection"]
-        p = G.make_params(**{k: (list(inj[k]) if k in ("spin1", "spin2") else inj[k].item()) for k in inj.dtype.names})
+        p = G.make_params(**{k: (list(inj[k]) if inj[k].shape else inj[k].item()) for k in inj.dtype.names})
         data = [r + W.noise(f, psd, g) for r in G.create_coherent_GW_detection(f, p, sp["dets"])]
         events.append((G.Event(f, data, [psd] * 3, sp["dets"]), sp))
     rng = np.random.default_rng(20261204)
@@ -374,7 +374,7 @@ def main():
     cfg = W.config1()
     f, psd, dets = cfg["f"], cfg["psd"], cfg["dets"]
     inj = W.injection_params(1)[0]
-    pinj = G.make_params(**{k: (list(inj[k]) if k in ("spin1", "spin2") else inj[k].item())
+    pinj = G.make_params(**{k: (list(inj[k]) if inj[k].shape else inj[k].item())
                             for k in inj.dtype.names})
     resp = G.create_coherent_GW_detection(f, pinj, dets)
     g = W.rng(1, 0)
@@ -574,7 +574,7 @@ def main():
         try:
             import oracle as O
             cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-            plist = [O.make_params(**{k: (list(P[i][k]) if k in ("spin1", "spin2") else P[i][k].item())
+            plist = [O.make_params(**{k: (list(P[i][k]) if P[i][k].shape else P[i][k].item())
                                       for k in P.dtype.names}) for i in range(n)]
             a = time.perf_counter()
             O.network_loglike_batch(f, data, [psd] * 3, dets, plist[:8], nthreads=1)

@@ -17,6 +17,9 @@ GWH_OK = 1
 GWH_ERR_INVALID, GWH_ERR_HIP, GWH_ERR_UNSUPPORTED, GWH_ERR_NOMEM, GWH_ERR_BUSY = -1, -2, -3, -4, -5
 IMRPHENOMD = 0
 IMRPHENOMPV2 = 1
+PPE_IMRPHENOMD_INSPIRAL = 2
+PPE_IMRPHENOMD_IMR = 3
+PPE_MAX = 4
 FP64 = 0
 MIXED = 1
 SIMPSONS = 0
@@ -63,7 +66,9 @@ class Params(ctypes.Structure):
                 ("DEC", ctypes.c_double), ("psi", ctypes.c_double), ("gmst", ctypes.c_double),
                 ("chip", ctypes.c_double), ("phip", ctypes.c_double),
                 ("shift_time", ctypes.c_int32), ("shift_phase", ctypes.c_int32),
-                ("sky_average", ctypes.c_int32), ("dep_postmerger", ctypes.c_int32)]
+                ("sky_average", ctypes.c_int32), ("dep_postmerger", ctypes.c_int32),
+                ("ppe_nmod", ctypes.c_int32), ("ppe_pad", ctypes.c_int32),
+                ("ppe_b", ctypes.c_double * 4), ("ppe_beta", ctypes.c_double * 4)]
 
 
 class PTConfig(ctypes.Structure):
@@ -78,8 +83,9 @@ PARAMS_DTYPE = np.dtype([("mass1", "f8"), ("mass2", "f8"), ("spin1", "f8", 3), (
                          ("tc", "f8"), ("phiRef", "f8"), ("f_ref", "f8"), ("incl", "f8"), ("RA", "f8"),
                          ("DEC", "f8"), ("psi", "f8"), ("gmst", "f8"), ("chip", "f8"), ("phip", "f8"),
                          ("shift_time", "i4"), ("shift_phase", "i4"), ("sky_average", "i4"),
-                         ("dep_postmerger", "i4")])
-assert PARAMS_DTYPE.itemsize == ctypes.sizeof(Params) == 168
+                         ("dep_postmerger", "i4"), ("ppe_nmod", "i4"), ("ppe_pad", "i4"), ("ppe_b", "f8", 4),
+                         ("ppe_beta", "f8", 4)])
+assert PARAMS_DTYPE.itemsize == ctypes.sizeof(Params) == 240
 
 
 def make_params(**kw):
@@ -89,8 +95,8 @@ def make_params(**kw):
     p.shift_time = 1
     p.shift_phase = 1
     for k, v in kw.items():
-        if k in ("spin1", "spin2"):
-            getattr(p, k)[:] = list(v)
+        if k in ("spin1", "spin2", "ppe_b", "ppe_beta"):
+            getattr(p, k)[:len(v)] = list(v)
         else:
             setattr(p, k, v)
     return p

@@ -410,7 +410,7 @@ int gwh_broker_submit_loglike(int broker, const int *events, int event0, int mod
     Broker *B = get(broker);
     if (!B) return gwh_set_error(GWH_ERR_INVALID, "unknown broker");
     if (!params || !logl || !ticket || n < 1) return gwh_set_error(GWH_ERR_INVALID, "bad arguments");
-    if (model != GWH_IMRPHENOMD && model != GWH_IMRPHENOMPV2) return gwh_set_error(GWH_ERR_UNSUPPORTED, "bad model");
+    if (!gwh_model_valid(model)) return gwh_set_error(GWH_ERR_UNSUPPORTED, "bad model");
     if (precision != GWH_FP64 && precision != GWH_MIXED) return gwh_set_error(GWH_ERR_INVALID, "bad precision");
     if (n > B->max_n) return gwh_set_error(GWH_ERR_INVALID, "request larger than the broker's batch limit");
     Engine &E = gwh_engine();

@@ -351,7 +351,8 @@ __global__ __launch_bounds__(64) void gwh_setup_kernel(const gwh_params *__restr
                                                        const DetGeom *__restrict__ geo,
                                                        const EventDesc *__restrict__ events,
                                                        InstCoef *__restrict__ coef, int *__restrict__ active,
-                                                       const int *__restrict__ d_count, const int *__restrict__ d_mask)
+                                                       const int *__restrict__ d_count, const int *__restrict__ d_mask,
+                                                       int ppe)
 {
     const int model = MODEL;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(64) void gwh_setup_kernel(const gwh_params *__restr
     const EventDesc &E = events[ev];
     gwh_params p = params[i];
     InstCoef C;
-    gwh_setup_instance(p, model, fits, qnm, geo, E.det, E.D, C);
+    gwh_setup_instance(p, model, fits, qnm, geo, E.det, E.D, C, ppe);
     coef[i] = C;
     if (active) {
         /* tiles to visit: all of them, or on an ascending grid those starting at or below 0.2/M */
@@ -395,11 +396,11 @@ static void launch_setup(int model, int n, hipStream_t st, const gwh_params *par
     if (model == GWH_IMRPHENOMPV2)
         hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMPV2>, g, b, 0, st, params, inst_event, event0, n,
                            (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
-                           coef, active, d_count, d_mask);
+                           coef, active, d_count, d_mask, 0);
     else
         hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMD>, g, b, 0, st, params, inst_event, event0, n,
                            (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
-                           coef, active, d_count, d_mask);
+                           coef, active, d_count, d_mask, gwh_ppe_variant(model));
 }
 
 int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_params *d_params, int n, InstCoef *d_coef,
@@ -557,6 +558,7 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, c
         double r3 = r2 * r;
         psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] * (r3 * r3 * r3);
     }
+    if (C.ppe_n > 0 && (C.ppe_imr || f < C.f1p)) psi += gwh_ppe_bin(C, bp.lf); /* FP64: phase accuracy */
     phi = psi - (C.tc_total * (f - C.f_ref) + C.phic);
     u_out = u;
     lx_out = lx;
@@ -831,7 +833,7 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const 
     double ll = 0;
     for (int d = 0; d < E.D; d++) {
         double hh, dh;
-        if (model == GWH_IMRPHENOMD) {
+        if (model != GWH_IMRPHENOMPV2) {
             double ha = s[3 * d], zr = s[3 * d + 1], zi = s[3 * d + 2];
             hh = C.det[d][2] * ha;
             dh = C.det[d][0] * zr + C.det[d][1] * zi;
@@ -936,8 +938,8 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
            &s_active = scratch ? scratch->active : E.active, &s_work = scratch ? scratch->work : E.work,
            &s_wcount = scratch ? scratch->wcount : E.wcount;
     if (n <= 0) return GWH_OK;
-    if (model != GWH_IMRPHENOMD && model != GWH_IMRPHENOMPV2)
-        return gwh_set_error(GWH_ERR_UNSUPPORTED, "model must be GWH_IMRPHENOMD or GWH_IMRPHENOMPV2");
+    if (!gwh_model_valid(model))
+        return gwh_set_error(GWH_ERR_UNSUPPORTED, "model must be IMRPhenomD, IMRPhenomPv2 or a ppE IMRPhenomD");
     if (precision != GWH_FP64 && precision != GWH_MIXED) return gwh_set_error(GWH_ERR_INVALID, "bad precision");
     if (events0 < 0 || events0 >= GWH_MAX_EVENTS || (!d_events && !E.events[events0].live))
         return gwh_set_error(GWH_ERR_INVALID, "unknown event " + std::to_string(events0));
@@ -1006,7 +1008,7 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
             int *cnt = s_wcount.as<int>() + sl;
             hipLaunchKernelGGL(gwh_worklist_kernel, dim3(1), dim3(1024), 0, stream, W, evp, ap, kB[sl], wp, cnt,
                                (int)ub[sl]);
-            if (model == GWH_IMRPHENOMD) {
+            if (model != GWH_IMRPHENOMPV2) {
                 if (precision == GWH_FP64)
                     launch_wl<GWH_IMRPHENOMD, GWH_FP64>(kB[sl], (int)ub[sl], stream, W, evp, cp, pp, ap, wp, cnt);
                 else
@@ -1024,7 +1026,7 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
         if (!tiles_b[sl]) continue;
         const long long items = (long long)n * maxtiles;
         const int bpl = kB[sl];
-        if (model == GWH_IMRPHENOMD) {
+        if (model != GWH_IMRPHENOMPV2) {
             if (precision == GWH_FP64)
                 launch_persist<GWH_IMRPHENOMD, GWH_FP64>(bpl, items, stream, W, evp, cp, pp, ap, X.d_count);
             else
@@ -1530,8 +1532,8 @@ static int check_common(const double *f, int n, const gwh_params *p, int model)
 {
     if (!f || !p) return gwh_set_error(GWH_ERR_INVALID, "null argument");
     if (n < 1) return gwh_set_error(GWH_ERR_INVALID, "length must be positive");
-    if (model != GWH_IMRPHENOMD && model != GWH_IMRPHENOMPV2)
-        return gwh_set_error(GWH_ERR_UNSUPPORTED, "only IMRPhenomD and IMRPhenomPv2 are provided");
+    if (!gwh_model_valid(model))
+        return gwh_set_error(GWH_ERR_UNSUPPORTED, "only IMRPhenomD, IMRPhenomPv2 and ppE IMRPhenomD are provided");
     return GWH_OK;
 }
 
@@ -1546,7 +1548,7 @@ int gwh_fourier_waveform(const double *frequencies, int length, double *hplus, d
     if (!hplus || !hcross) return gwh_set_error(GWH_ERR_INVALID, "null output");
     if ((rc = upload_single(frequencies, length, params, model, 0)) != GWH_OK) return rc;
     dim3 g((length + 255) / 256);
-    if (model == GWH_IMRPHENOMD)
+    if (model != GWH_IMRPHENOMPV2)
         hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMD>, g, dim3(256), 0, E.stream, E.f.as<const double>(),
                            length, E.coef.as<const InstCoef>(), 0, E.out1.as<double2>(), E.out2.as<double2>());
     else
@@ -1572,7 +1574,7 @@ int gwh_fourier_detector_response(const double *frequencies, int length, double 
         return gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported detector (the reference exits here)");
     if ((rc = upload_single(frequencies, length, params, model, detector)) != GWH_OK) return rc;
     dim3 g((length + 255) / 256);
-    if (model == GWH_IMRPHENOMD)
+    if (model != GWH_IMRPHENOMPV2)
         hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMD>, g, dim3(256), 0, E.stream, E.f.as<const double>(),
                            length, E.coef.as<const InstCoef>(), 1, E.out1.as<double2>(), E.out2.as<double2>());
     else
@@ -1871,7 +1873,7 @@ int gwh_create_coherent_GW_detection(const int *detectors, int num_detectors, co
         return rc;
     dim3 g((length + 255) / 256);
     for (int d = 0; d < num_detectors; d++) {
-        if (model == GWH_IMRPHENOMD)
+        if (model != GWH_IMRPHENOMPV2)
             hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMD>, g, dim3(256), 0, E.stream, E.f.as<const double>(),
                                length, E.coef.as<const InstCoef>(), 2, E.out1.as<double2>(), E.out2.as<double2>(), d);
         else

@@ -373,6 +373,9 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
         std::lock_guard<std::recursive_mutex> lk(E.mu); /* snapshot: the event stays alive meanwhile */
         ev = E.events[event];
     }
+    if (model != GWH_IMRPHENOMD && model != GWH_IMRPHENOMPV2)
+        return gwh_set_error(GWH_ERR_UNSUPPORTED,
+                             "Fisher matrices of the ppE models (betappe in the basis) are not provided");
     int dim = basis_dim(basis);
     bool sky = (basis == GWH_BASIS_PHENOMD_SKYAVG);
     if (sky && model != GWH_IMRPHENOMD)
@@ -509,6 +512,9 @@ int gwh_fisher_batch_with(int event, int model, int basis, int precision, const 
     if (basis_dim(basis) < 0 || basis_dim(basis) != dim)
         return gwh_set_error(GWH_ERR_INVALID, "dimension does not match the parameter basis");
     if (precision != GWH_FP64 && precision != GWH_MIXED) return gwh_set_error(GWH_ERR_INVALID, "bad precision");
+    if (model != GWH_IMRPHENOMD && model != GWH_IMRPHENOMPV2)
+        return gwh_set_error(GWH_ERR_UNSUPPORTED,
+                             "Fisher matrices of the ppE models (betappe in the basis) are not provided");
     if ((basis == GWH_BASIS_PV2_FULL || basis == GWH_BASIS_PV2_MCMC) != (model == GWH_IMRPHENOMPV2))
         return gwh_set_error(GWH_ERR_INVALID, "basis does not belong to the model");
     /* the event's weights use the likelihood's df = f[N/2]-f[N/2-1]; the Fisher uses f[1]-f[0] */

@@ -7,8 +7,9 @@
  * for the functions on the hot path: parameter handles, fourier_waveform_py,
  * fourier_detector_response_py, MCMC_likelihood_extrinsic_py, repack_parameters_py and the two
  * geometry helpers.  The handle here is the engine's own record; parameters the engine does not model
- * (equatorial or horizon orientation, neutron-star flags, ppE / gIMR modifications) are recorded and
- * reported as GWH_ERR_UNSUPPORTED when the handle is used, never silently dropped.
+ * (equatorial or horizon orientation, neutron-star flags, more than GWH_PPE_MAX ppE terms) are recorded
+ * and reported as GWH_ERR_UNSUPPORTED when the handle is used, never silently dropped.  ppE terms are
+ * used by the ppE_IMRPhenomD methods; like the reference, other methods ignore them and the gIMR terms.
  */
 #include <cmath>
 #include <cstdint>
@@ -51,6 +52,8 @@ int model_of(const char *method)
     if (!method) return -1;
     if (!strcmp(method, "IMRPhenomD")) return GWH_IMRPHENOMD;
     if (!strcmp(method, "IMRPhenomPv2")) return GWH_IMRPHENOMPV2;
+    if (!strcmp(method, "ppE_IMRPhenomD_Inspiral")) return GWH_PPE_IMRPHENOMD_INSPIRAL;
+    if (!strcmp(method, "ppE_IMRPhenomD_IMR")) return GWH_PPE_IMRPHENOMD_IMR;
     return -1;
 }
 
@@ -84,7 +87,7 @@ void *gen_params_base_py(double mass1, double mass2, double *spin1, double *spin
 {
     (void)theta_l, (void)phi_l, (void)theta, (void)phi, (void)cosmology, (void)LISA_alpha0, (void)LISA_phi0;
     (void)phii, (void)sigmai, (void)betai, (void)alphai, (void)delta_phi, (void)delta_sigma, (void)delta_beta;
-    (void)delta_alpha, (void)bppe, (void)betappe;
+    (void)delta_alpha;
     PyParams *q = new PyParams;
     q->magic = kMagic;
     gwh_params &p = q->p;
@@ -113,7 +116,19 @@ void *gen_params_base_py(double mass1, double mass2, double *spin1, double *spin
     if (equatorial_orientation) q->unsupported = "equatorial_orientation";
     else if (horizon_coord) q->unsupported = "horizon_coord";
     else if (NSflag1 || NSflag2) q->unsupported = "NSflag1/NSflag2";
-    else if (Nmod || Nmod_phi || Nmod_sigma || Nmod_beta || Nmod_alpha) q->unsupported = "ppE/gIMR modifications";
+    else if (Nmod > GWH_PPE_MAX)
+        q->unsupported = "more than " + std::to_string(GWH_PPE_MAX) + " ppE terms";
+    else if (Nmod > 0 && (!bppe || !betappe))
+        q->unsupported = "ppE terms without bppe / betappe arrays";
+    /* ppE terms are read by the ppE methods only, gIMR terms by none of the engine's methods; both are
+     * ignored by the others, as in prep_source_parameters (waveform_generator.cpp:1296-1313) */
+    if (q->unsupported.empty() && Nmod > 0) {
+        p.ppe_nmod = Nmod;
+        for (int i = 0; i < Nmod; i++) {
+            p.ppe_b[i] = bppe[i];
+            p.ppe_beta[i] = betappe[i];
+        }
+    }
     std::lock_guard<std::mutex> lk(g_handles_mu);
     g_handles.insert(q);
     return q;

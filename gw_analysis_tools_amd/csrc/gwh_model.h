@@ -30,6 +30,14 @@
 #define GWH_LOG_64 4.15888308336 /* the reference's truncated ln 64 (IMRPhenomD.cpp:25) */
 #define GWH_SQRT_6 2.44948974278317788
 
+/* API model ids -> the kernel family (IMRPhenomD or IMRPhenomPv2) and the ppE variant of IMRPhenomD
+ * (0 none, 1 inspiral phase only, 2 all regions) */
+__host__ __device__ inline bool gwh_model_valid(int m) { return m >= GWH_IMRPHENOMD && m <= GWH_PPE_IMRPHENOMD_IMR; }
+__host__ __device__ inline int gwh_ppe_variant(int m)
+{
+    return m == GWH_PPE_IMRPHENOMD_INSPIRAL ? 1 : (m == GWH_PPE_IMRPHENOMD_IMR ? 2 : 0);
+}
+
 /* Frequency-independent constants of one instance (one parameter vector on
  * one event).  All dimensionful quantities in seconds / Hz. */
 struct InstCoef {
@@ -63,6 +71,10 @@ struct InstCoef {
     /* powers of the total mass that pair with the tabulated powers of each bin: (Mf)^p = M^p f^p */
     double M13, Mm13, Mm76, M34, lgM, lgPiM;
     double inv_fdamp; /* 1 / fdamp */
+    /* ppE phase terms: sum_i ppe_c[i] f^ppe_e[i], ppe_c = beta (pi Mc)^(b/3), ppe_e = b/3; added to the
+     * inspiral phase (ppe_imr = 0) or to every region (ppe_imr = 1) */
+    int ppe_n, ppe_imr;
+    double ppe_e[GWH_PPE_MAX], ppe_c[GWH_PPE_MAX];
 };
 
 /* Powers of one frequency bin that every instance needs (log f, f^(1/3), f^(-1/3), f^(-7/6), f^(3/4)).
@@ -159,7 +171,24 @@ struct SetupState {
     double pa[7];  /* amplitude PN coefficients (reference numbering) */
     double pp[12]; /* phase PN coefficients (reference numbering), [5],[6] dynamic */
     double delta_c[5];
+    int ppe_n, ppe_imr;
+    double ppe_e[GWH_PPE_MAX], ppe_c[GWH_PPE_MAX];
 };
+
+/* ppE phase terms at f and their frequency derivative (ppE_IMRPhenomD.cpp:22-48): beta (pi Mc f)^(b/3) and
+ * (b/3) beta (pi Mc f)^(b/3) / f */
+__device__ __forceinline__ double gwh_setup_ppe(const SetupState &S, double f, double *deriv)
+{
+    double t = 0, d = 0;
+    const double lf = log(f);
+    for (int i = 0; i < S.ppe_n; i++) {
+        const double v = S.ppe_c[i] * exp(S.ppe_e[i] * lf);
+        t += v;
+        d += S.ppe_e[i] * v;
+    }
+    if (deriv) *deriv = d / f;
+    return t;
+}
 
 /* TF2 + sigma inspiral phase at f (IMRPhenomD.cpp:1364-1399) */
 __device__ __forceinline__ double gwh_setup_phase_ins(const SetupState &S, double f)
@@ -174,9 +203,11 @@ __device__ __forceinline__ double gwh_setup_phase_ins(const SetupState &S, doubl
     double Mf = S.M * f;
     double u = cbrt(Mf);
     double Mf43 = Mf * u, Mf53 = Mf43 * u;
-    return -GWH_PI / 4. + 3. / (128. * S.eta) * poly / (v2 * v2 * v) +
-           (1. / S.eta) * (S.sigma[1] * Mf + 0.75 * S.sigma[2] * Mf43 + 0.6 * S.sigma[3] * Mf53 +
-                           0.5 * S.sigma[4] * Mf * Mf);
+    double ph = -GWH_PI / 4. + 3. / (128. * S.eta) * poly / (v2 * v2 * v) +
+                (1. / S.eta) * (S.sigma[1] * Mf + 0.75 * S.sigma[2] * Mf43 + 0.6 * S.sigma[3] * Mf53 +
+                                0.5 * S.sigma[4] * Mf * Mf);
+    if (S.ppe_n > 0) ph += gwh_setup_ppe(S, f, nullptr);
+    return ph;
 }
 
 /* d/df of the inspiral phase (IMRPhenomD.cpp:1426-1460) */
@@ -202,26 +233,44 @@ __device__ __forceinline__ double gwh_setup_dphase_ins(const SetupState &S, doub
     /* M^(4/3) f^(1/3) = M (Mf)^(1/3), M^(5/3) f^(2/3) = M (Mf)^(2/3) */
     double u = cbrt(M * f);
     double sg = S.sigma[1] * M + S.sigma[2] * M * u + S.sigma[3] * M * u * u + S.sigma[4] * M * M * f;
-    return d + sg / S.eta;
+    double r = d + sg / S.eta;
+    if (S.ppe_n > 0) {
+        double dt;
+        gwh_setup_ppe(S, f, &dt);
+        r += dt;
+    }
+    return r;
+}
+
+/* the ppE_IMRPhenomD_IMR terms of the intermediate and merger-ringdown phases (ppE_IMRPhenomD.cpp:54-103) */
+__device__ __forceinline__ double gwh_setup_ppe_imr(const SetupState &S, double f, bool deriv)
+{
+    if (S.ppe_n <= 0 || !S.ppe_imr) return 0;
+    double d;
+    double t = gwh_setup_ppe(S, f, &d);
+    return deriv ? d : t;
 }
 
 __device__ __forceinline__ double gwh_setup_phase_int(const SetupState &S, double f)
 {
     double Mf = S.M * f;
-    return (1. / S.eta) * (S.beta[0] + S.beta[1] * Mf + S.beta[2] * log(Mf) - S.beta[3] / 3. / (Mf * Mf * Mf));
+    return (1. / S.eta) * (S.beta[0] + S.beta[1] * Mf + S.beta[2] * log(Mf) - S.beta[3] / 3. / (Mf * Mf * Mf)) +
+           gwh_setup_ppe_imr(S, f, false);
 }
 
 __device__ __forceinline__ double gwh_setup_dphase_int(const SetupState &S, double f)
 {
     double M = S.M;
-    return (1. / S.eta) * (S.beta[1] * M + S.beta[2] / f + S.beta[3] / (M * M * M * f * f * f * f));
+    return (1. / S.eta) * (S.beta[1] * M + S.beta[2] / f + S.beta[3] / (M * M * M * f * f * f * f)) +
+           gwh_setup_ppe_imr(S, f, true);
 }
 
 __device__ __forceinline__ double gwh_setup_phase_mr(const SetupState &S, double f)
 {
     double Mf = S.M * f;
     return (1. / S.eta) * (S.alpha[0] + S.alpha[1] * Mf - S.alpha[2] / Mf + (4. / 3.) * S.alpha[3] * sqrt(sqrt(Mf * Mf * Mf)) +
-                           S.alpha[4] * atan((f - S.alpha[5] * S.fRD) / S.fdamp));
+                           S.alpha[4] * atan((f - S.alpha[5] * S.fRD) / S.fdamp)) +
+           gwh_setup_ppe_imr(S, f, false);
 }
 
 __device__ __forceinline__ double gwh_setup_dphase_mr(const SetupState &S, double f)
@@ -229,7 +278,8 @@ __device__ __forceinline__ double gwh_setup_dphase_mr(const SetupState &S, doubl
     double M = S.M;
     double t = (f - S.alpha[5] * S.fRD) / S.fdamp;
     return (S.alpha[4] / (S.fdamp * (1. + t * t)) + S.alpha[2] / (f * f * M) + M * (S.alpha[1] + S.alpha[3] / sqrt(sqrt(f * M)))) /
-           S.eta;
+               S.eta +
+           gwh_setup_ppe_imr(S, f, true);
 }
 
 __device__ __forceinline__ double gwh_setup_build_phase(const SetupState &S, double f)
@@ -342,7 +392,7 @@ __device__ __forceinline__ double gwh_euler_eval(const double *c, double v, doub
 /* ------------------------------------------------------------------------- */
 
 __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const double *fits, const QnmTable &qnm,
-                                         const DetGeom *geo, const int *dets, int ndet, InstCoef &C)
+                                         const DetGeom *geo, const int *dets, int ndet, InstCoef &C, int ppe = 0)
 {
     SetupState S;
     /* util.cpp:996-1028 */
@@ -362,6 +412,20 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
     double DL = p.DL * (GWH_MPC_SI / GWH_C_SI);
     double kA = p.sky_average ? sqrt(GWH_PI / 30.) : sqrt(GWH_PI * 40. / 192.);
     S.A0 = kA * S.Mc * S.Mc / DL * pow(GWH_PI * S.Mc, -7. / 6.);
+    /* ppE terms (waveform_generator.cpp:1296-1300 copies them for the ppE methods only); more than
+     * GWH_PPE_MAX terms cannot be represented: the waveform below 0.2/M and the likelihood are NaN */
+    S.ppe_n = 0;
+    S.ppe_imr = (ppe == 2);
+    if (ppe) {
+        const int nm = p.ppe_nmod;
+        if (nm > GWH_PPE_MAX) S.A0 = __longlong_as_double(0x7ff8000000000000LL); /* NaN amplitude */
+        S.ppe_n = nm < 0 ? 0 : (nm > GWH_PPE_MAX ? GWH_PPE_MAX : nm);
+        const double lpm = log(GWH_PI * S.Mc);
+        for (int i = 0; i < GWH_PPE_MAX; i++) {
+            S.ppe_e[i] = i < S.ppe_n ? p.ppe_b[i] / 3.0 : 0.0;
+            S.ppe_c[i] = i < S.ppe_n ? p.ppe_beta[i] * exp(S.ppe_e[i] * lpm) : 0.0;
+        }
+    }
 
     /* Pv2 frame (IMRPhenomP.cpp:821-938, :1081-1211) */
     double chip = -1, chil = 0, SP = 0, SL = 0, thetaJN = 0, alpha0 = 0, zeta = 0, phi_aligned = 0;
@@ -706,6 +770,12 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
     C.wf_a = pv2 ? c2z : kplus;
     C.wf_b = pv2 ? s2z : ci;
     C.inv_fdamp = 1.0 / S.fdamp;
+    C.ppe_n = S.ppe_n;
+    C.ppe_imr = S.ppe_imr;
+    for (int i = 0; i < GWH_PPE_MAX; i++) {
+        C.ppe_e[i] = S.ppe_e[i];
+        C.ppe_c[i] = S.ppe_c[i];
+    }
     /* detector_util.cpp:900-1016: polarisation basis (shared by every detector) */
     const double gha = p.gmst - p.RA;
     double cg, sg, cd, sd, cp, sp;
@@ -749,6 +819,14 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
 
 /* IMRPhenomD amplitude A(f) and phase phi(f) = Psi(f) - tc_total (f - f_ref) - phic
  * (IMRPhenomD.cpp:471-501); caller guarantees 0 < f <= fcut. */
+/* ppE phase terms at one bin from its tabulated log f: sum_i beta_i (pi Mc f)^(b_i/3) */
+__device__ __forceinline__ double gwh_ppe_bin(const InstCoef &C, double lf)
+{
+    double t = 0;
+    for (int i = 0; i < C.ppe_n; i++) t += C.ppe_c[i] * exp(C.ppe_e[i] * lf);
+    return t;
+}
+
 __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, const BinPow &bp, double &A, double &phi,
                                                  double &u_out, double &lx_out)
 {
@@ -786,6 +864,7 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, co
     } else {
         psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] / (Mf * Mf * Mf);
     }
+    if (C.ppe_n > 0 && (C.ppe_imr || f < C.f1p)) psi += gwh_ppe_bin(C, bp.lf);
     phi = psi - (C.tc_total * (f - C.f_ref) + C.phic);
     u_out = u;
     lx_out = lx;

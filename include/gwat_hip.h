@@ -37,6 +37,12 @@ extern "C" {
  * fourier_waveform, src/waveform_generator.cpp:129-266) */
 #define GWH_IMRPHENOMD 0
 #define GWH_IMRPHENOMPV2 1
+/* IMRPhenomD with ppE phase terms sum_i betappe[i] (pi Mc f)^(bppe[i]/3) (src/ppE_IMRPhenomD.cpp:22-103):
+ * in the inspiral phase only ("ppE_IMRPhenomD_Inspiral"; the C1 connections carry it into the later
+ * regions) or in all three regions ("ppE_IMRPhenomD_IMR"). */
+#define GWH_PPE_IMRPHENOMD_INSPIRAL 2
+#define GWH_PPE_IMRPHENOMD_IMR 3
+#define GWH_PPE_MAX 4 /* ppE terms per parameter set */
 
 /* arithmetic of the batched kernels */
 #define GWH_FP64 0  /* everything in FP64 */
@@ -78,6 +84,10 @@ typedef struct gwh_params {
     double gmst;
     double chip, phip; /* chip <= -1 + 1e-10: precessing spins are taken from spin1/spin2 vectors */
     int32_t shift_time, shift_phase, sky_average, dep_postmerger;
+    /* ppE terms (gen_params_base: Nmod, bppe, betappe), read only by the ppE models */
+    int32_t ppe_nmod, ppe_pad;
+    double ppe_b[GWH_PPE_MAX];
+    double ppe_beta[GWH_PPE_MAX];
 } gwh_params;
 
 /* ---- runtime ---- */

@@ -26,11 +26,14 @@
 
 namespace gwh {
 
-/* "IMRPhenomD" / "IMRPhenomPv2" (waveform_generator.cpp:103-294 dispatch); -1 otherwise */
+/* "IMRPhenomD" / "IMRPhenomPv2" / "ppE_IMRPhenomD_Inspiral" / "ppE_IMRPhenomD_IMR"
+ * (waveform_generator.cpp:103-294 dispatch); -1 otherwise */
 inline int model_id(const std::string &method)
 {
     if (method == "IMRPhenomD") return GWH_IMRPHENOMD;
     if (method == "IMRPhenomPv2") return GWH_IMRPHENOMPV2;
+    if (method == "ppE_IMRPhenomD_Inspiral") return GWH_PPE_IMRPHENOMD_INSPIRAL;
+    if (method == "ppE_IMRPhenomD_IMR") return GWH_PPE_IMRPHENOMD_IMR;
     return -1;
 }
 
@@ -47,10 +50,12 @@ inline int basis_id(const std::string &method, bool sky_average)
     return -1;
 }
 
-/* the hot-path fields of a gen_params_base<double>-shaped object; false if it asks for physics the
- * engine does not model (equatorial / horizon orientation, neutron-star flags, ppE or gIMR terms) */
+/* the hot-path fields of a gen_params_base<double>-shaped object for the given model; false if it asks
+ * for physics the engine does not model (equatorial / horizon orientation, neutron-star flags, more ppE
+ * terms than GWH_PPE_MAX).  As in prep_source_parameters (waveform_generator.cpp:1296-1313), the ppE
+ * fields (Nmod, bppe, betappe) are read for the ppE models only and the gIMR fields for none. */
 template <class Params>
-inline bool to_params(const Params &g, gwh_params &p)
+inline bool to_params(const Params &g, gwh_params &p, int model = GWH_IMRPHENOMD)
 {
     p = gwh_params();
     p.mass1 = g.mass1;
@@ -74,8 +79,15 @@ inline bool to_params(const Params &g, gwh_params &p)
     p.shift_phase = g.shift_phase;
     p.sky_average = g.sky_average;
     p.dep_postmerger = g.dep_postmerger;
-    return !(g.equatorial_orientation || g.horizon_coord || g.NSflag1 || g.NSflag2 || g.Nmod || g.Nmod_phi ||
-             g.Nmod_sigma || g.Nmod_beta || g.Nmod_alpha);
+    if (model == GWH_PPE_IMRPHENOMD_INSPIRAL || model == GWH_PPE_IMRPHENOMD_IMR) {
+        if (g.Nmod < 0 || g.Nmod > GWH_PPE_MAX || (g.Nmod > 0 && (!g.bppe || !g.betappe))) return false;
+        p.ppe_nmod = g.Nmod;
+        for (int i = 0; i < g.Nmod; i++) {
+            p.ppe_b[i] = g.bppe[i];
+            p.ppe_beta[i] = g.betappe[i];
+        }
+    }
+    return !(g.equatorial_orientation || g.horizon_coord || g.NSflag1 || g.NSflag2);
 }
 
 /* write the basis fields back into the caller's object (repack_parameters, fisher.cpp:2162-2511) */
@@ -108,7 +120,7 @@ inline int fourier_waveform(const double *f, int n, std::complex<double> *hplus,
 {
     gwh_params p;
     int model = model_id(method);
-    if (model < 0 || !to_params(*g, p)) return GWH_ERR_UNSUPPORTED;
+    if (model < 0 || !to_params(*g, p, model)) return GWH_ERR_UNSUPPORTED;
     return gwh_fourier_waveform(f, n, reinterpret_cast<double *>(hplus), reinterpret_cast<double *>(hcross), model,
                                 &p);
 }
@@ -120,7 +132,7 @@ inline int fourier_detector_response(const double *f, int n, std::complex<double
 {
     gwh_params p;
     int model = model_id(method), d = gwh_detector_id(det.c_str());
-    if (model < 0 || d < 0 || !to_params(*g, p)) return GWH_ERR_UNSUPPORTED;
+    if (model < 0 || d < 0 || !to_params(*g, p, model)) return GWH_ERR_UNSUPPORTED;
     return gwh_fourier_detector_response(f, n, reinterpret_cast<double *>(response), d, model, &p);
 }
 
@@ -131,7 +143,7 @@ inline double Log_Likelihood(const std::complex<double> *data, const double *psd
 {
     gwh_params p;
     int model = model_id(method), d = gwh_detector_id(det.c_str());
-    if (model < 0 || d < 0 || !to_params(*g, p)) return nan_with(GWH_ERR_UNSUPPORTED);
+    if (model < 0 || d < 0 || !to_params(*g, p, model)) return nan_with(GWH_ERR_UNSUPPORTED);
     return gwh_log_likelihood(reinterpret_cast<const double *>(data), psd, f, n, &p, d, model);
 }
 
@@ -156,7 +168,8 @@ inline double MCMC_likelihood_extrinsic(bool save_waveform, const Params *g, con
 {
     gwh_params p;
     int model = model_id(method);
-    if (model < 0 || !to_params(*g, p) || num_detectors < 1 || num_detectors > 4) return nan_with(GWH_ERR_UNSUPPORTED);
+    if (model < 0 || !to_params(*g, p, model) || num_detectors < 1 || num_detectors > 4)
+        return nan_with(GWH_ERR_UNSUPPORTED);
     int dets[4];
     std::vector<const double *> dp(num_detectors);
     for (int i = 0; i < num_detectors; i++) {

@@ -14,6 +14,8 @@ _LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
 
 IMRPHENOMD = 0
 IMRPHENOMPV2 = 1
+PPE_IMRPHENOMD_INSPIRAL = 2
+PPE_IMRPHENOMD_IMR = 3
 
 DETECTORS = {"Hanford": 0, "Livingston": 1, "Virgo": 2, "Kagra": 3, "Indigo": 4,
              "CE": 5, "ET1": 6, "ET2": 7, "ET3": 8}
@@ -28,7 +30,9 @@ class Params(ctypes.Structure):
                 ("DEC", ctypes.c_double), ("psi", ctypes.c_double), ("gmst", ctypes.c_double),
                 ("chip", ctypes.c_double), ("phip", ctypes.c_double),
                 ("shift_time", ctypes.c_int), ("shift_phase", ctypes.c_int),
-                ("sky_average", ctypes.c_int), ("dep_postmerger", ctypes.c_int)]
+                ("sky_average", ctypes.c_int), ("dep_postmerger", ctypes.c_int),
+                ("ppe_nmod", ctypes.c_int), ("ppe_pad", ctypes.c_int),
+                ("ppe_b", ctypes.c_double * 4), ("ppe_beta", ctypes.c_double * 4)]
 
 
 def make_params(**kw):
@@ -38,8 +42,8 @@ def make_params(**kw):
     p.shift_time = 1
     p.shift_phase = 1
     for k, v in kw.items():
-        if k in ("spin1", "spin2"):
-            getattr(p, k)[:] = list(v)
+        if k in ("spin1", "spin2", "ppe_b", "ppe_beta"):
+            getattr(p, k)[:len(v)] = list(v)
         else:
             setattr(p, k, v)
     return p

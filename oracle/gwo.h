@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-enum { GWO_IMRPHENOMD = 0, GWO_IMRPHENOMPV2 = 1 };
+enum { GWO_IMRPHENOMD = 0, GWO_IMRPHENOMPV2 = 1, GWO_PPE_IMRPHENOMD_INSPIRAL = 2, GWO_PPE_IMRPHENOMD_IMR = 3 };
 
 /* Flat view of gen_params_base<double> (include/gwat/util.h:121-389): the
  * fields the hot path reads. */
@@ -39,6 +39,8 @@ typedef struct {
     double chip, phip;            /* Pv2 reduced spin parameters; chip=-1 => use spin vectors */
     int shift_time, shift_phase, sky_average;
     int dep_postmerger;           /* closed-form ringdown fits instead of the QNM spline (util.h:197) */
+    int ppe_nmod, ppe_pad;        /* ppE terms: Nmod, bppe, betappe (util.h:249-255) */
+    double ppe_b[4], ppe_beta[4];
 } gwo_params;
 
 void gwo_default_params(gwo_params *p);

@@ -27,6 +27,9 @@ typedef struct {
     int shift_phase, shift_time, sky_average, precessing, dep_postmerger;
     double chil, chip, phip, f_ref, phi_aligned, incl_angle, phiRef;
     double alpha0, thetaJN, zeta_polariz, SP, SL;
+    int ppe;                      /* 0 GR, 1 ppE_IMRPhenomD_Inspiral, 2 ppE_IMRPhenomD_IMR */
+    int Nmod;
+    double bppe[4], betappe[4];
 } gwo_src;
 
 /* lambda_parameters<double> (IMRPhenomD.h:16-26) */

@@ -78,6 +78,15 @@ void gwo_populate_source(gwo_src *s, const gwo_params *p, int model)
     s->dep_postmerger = p->dep_postmerger;
     s->chip = -1;
     s->phip = -1;
+    /* waveform_generator.cpp:1296-1300: the ppE fields are copied for the ppE methods only */
+    if (model == GWO_PPE_IMRPHENOMD_INSPIRAL || model == GWO_PPE_IMRPHENOMD_IMR) {
+        s->ppe = (model == GWO_PPE_IMRPHENOMD_INSPIRAL) ? 1 : 2;
+        s->Nmod = p->ppe_nmod < 0 ? 0 : (p->ppe_nmod > 4 ? 4 : p->ppe_nmod);
+        for (int i = 0; i < 4 && i < p->ppe_nmod; i++) {
+            s->bppe[i] = p->ppe_b[i];
+            s->betappe[i] = p->ppe_beta[i];
+        }
+    }
     if (model == GWO_IMRPHENOMPV2) {
         s->precessing = 1;
         if ((p->chip + 1) > GWO_DOUBLE_COMP_THRESH) {
@@ -362,6 +371,26 @@ static double fpeak(const gwo_src *s, const gwo_lambda *l)
     return sqrt(v * v);
 }
 
+/* ppE phase terms (ppE_IMRPhenomD.cpp:25-35): sum_i betappe_i ((pi Mc f)^(1/3))^bppe_i, and their frequency
+ * derivative (:40-48): sum_i bppe_i/3 f^(bppe_i/3 - 1) (Mc pi)^(bppe_i/3) betappe_i.  Added to the inspiral
+ * phase by ppE_IMRPhenomD_Inspiral and to all three regions by ppE_IMRPhenomD_IMR (:54-103). */
+static double ppe_phase(double f, const gwo_src *s)
+{
+    double pimfcube = pow(M_PI * s->chirpmass * f, 1. / 3.);
+    double out = 0;
+    for (int i = 0; i < s->Nmod; i++) out = out + pow(pimfcube, s->bppe[i]) * s->betappe[i];
+    return out;
+}
+
+static double ppe_dphase(double f, const gwo_src *s)
+{
+    double out = 0;
+    for (int i = 0; i < s->Nmod; i++)
+        out = out + s->bppe[i] / 3. * pow(f, s->bppe[i] / 3. - 1.) * pow(s->chirpmass * M_PI, s->bppe[i] / 3.) *
+                        s->betappe[i];
+    return out;
+}
+
 /* IMRPhenomD.cpp:1335-1355 */
 double gwo_amp_ins(double f, gwo_src *s, double *pn, gwo_lambda *lam, gwo_pows *pw)
 {
@@ -385,8 +414,10 @@ double gwo_phase_ins(double f, gwo_src *s, double *pn, gwo_lambda *lam, gwo_pows
     double phase_TF2 = -M_PI / 4. + 3. / (128. * eta) * pw->PIminus_5third * pw->MFminus_5third * pn_phase;
     double sigma0 = 0, sigma1 = lam->sigma[1], sigma2 = lam->sigma[2], sigma3 = lam->sigma[3],
            sigma4 = lam->sigma[4];
-    return phase_TF2 + (1. / eta) * (sigma0 + sigma1 * M * f + (3. / 4.) * sigma2 * pw->MF4third +
-                                     (3. / 5) * sigma3 * pw->MF5third + (1. / 2.) * sigma4 * pw->MFsquare);
+    double out = phase_TF2 + (1. / eta) * (sigma0 + sigma1 * M * f + (3. / 4.) * sigma2 * pw->MF4third +
+                                           (3. / 5) * sigma3 * pw->MF5third + (1. / 2.) * sigma4 * pw->MFsquare);
+    if (s->ppe) out += ppe_phase(f, s);
+    return out;
 }
 
 /* Frequency derivative of the reduced inspiral amplitude (IMRPhenomD.cpp:1406-1420),
@@ -416,7 +447,9 @@ double gwo_dphase_ins(double f, gwo_src *s, double *pn, gwo_lambda *lam)
     d += K * (pn[9] - pn[11] * pow(x, 1. / 3.)) / f;
     double sig = lam->sigma[1] * M + lam->sigma[2] * pow(M, 4. / 3.) * pow(f, 1. / 3) +
                  lam->sigma[3] * pow(M, 5. / 3.) * pow(f, 2. / 3) + lam->sigma[4] * pow(M, 2.) * f;
-    return d + (1. / eta) * sig;
+    double out = d + (1. / eta) * sig;
+    if (s->ppe) out += ppe_dphase(f, s);
+    return out;
 }
 
 /* IMRPhenomD.cpp:1467-1478 */
@@ -437,8 +470,11 @@ double gwo_phase_mr(double f, gwo_src *s, gwo_lambda *lam)
     double Mf = M * f;
     double Mfcube = Mf * Mf * Mf;
     double Mf3fourths = sqrt(sqrt(Mfcube));
-    return (1. / eta) * (lam->alpha[0] + lam->alpha[1] * (Mf) - lam->alpha[2] * (1. / (Mf)) +
-                         (4. / 3) * lam->alpha[3] * Mf3fourths + lam->alpha[4] * atan((f - lam->alpha[5] * fRD) / fdamp));
+    double out = (1. / eta) * (lam->alpha[0] + lam->alpha[1] * (Mf) - lam->alpha[2] * (1. / (Mf)) +
+                               (4. / 3) * lam->alpha[3] * Mf3fourths +
+                               lam->alpha[4] * atan((f - lam->alpha[5] * fRD) / fdamp));
+    if (s->ppe == 2) out += ppe_phase(f, s);
+    return out;
 }
 
 /* d/df of the Lorentzian-times-exponential merger-ringdown amplitude (IMRPhenomD.cpp:1514-1532) */
@@ -456,8 +492,10 @@ double gwo_dphase_mr(double f, gwo_src *s, gwo_lambda *lam)
 {
     double M = s->M, eta = s->eta, fdamp = s->fdamp, fRD = s->fRD;
     double a1 = lam->alpha[1], a2 = lam->alpha[2], a3 = lam->alpha[3], a4 = lam->alpha[4], a5 = lam->alpha[5];
-    return (a4 / (fdamp * (1. + pow(f - a5 * fRD, 2.) / pow(fdamp, 2.))) + a2 / (pow(f, 2.) * M) +
-            M * (a1 + a3 / pow(f * M, 0.25))) / eta;
+    double out = (a4 / (fdamp * (1. + pow(f - a5 * fRD, 2.) / pow(fdamp, 2.))) + a2 / (pow(f, 2.) * M) +
+                  M * (a1 + a3 / pow(f * M, 0.25))) / eta;
+    if (s->ppe == 2) out += ppe_dphase(f, s);
+    return out;
 }
 
 /* IMRPhenomD.cpp:1559-1570 */
@@ -472,14 +510,18 @@ double gwo_amp_int(double f, gwo_src *s, double *deltas)
 double gwo_phase_int(double f, gwo_src *s, gwo_lambda *lam)
 {
     double M = s->M, Mf = M * f, Mf3 = Mf * Mf * Mf, eta = s->eta;
-    return (1. / eta) * (lam->beta[0] + lam->beta[1] * (Mf) + lam->beta[2] * log(Mf) - lam->beta[3] / 3. * 1. / Mf3);
+    double out = (1. / eta) * (lam->beta[0] + lam->beta[1] * (Mf) + lam->beta[2] * log(Mf) - lam->beta[3] / 3. * 1. / Mf3);
+    if (s->ppe == 2) out += ppe_phase(f, s);
+    return out;
 }
 
 /* IMRPhenomD.cpp:1596-1607 */
 double gwo_dphase_int(double f, gwo_src *s, gwo_lambda *lam)
 {
     double M = s->M, eta = s->eta;
-    return (1. / eta) * (lam->beta[1] * (M) + lam->beta[2] / f + lam->beta[3] * pow(M, -3.) * pow(f, -4.));
+    double out = (1. / eta) * (lam->beta[1] * (M) + lam->beta[2] / f + lam->beta[3] * pow(M, -3.) * pow(f, -4.));
+    if (s->ppe == 2) out += ppe_dphase(f, s);
+    return out;
 }
 
 /* IMRPhenomD.cpp:1613-1674: C^1 phase connection (beta1, beta0, alpha1, alpha0 in that order) */
@@ -686,7 +728,7 @@ int gwo_fourier_waveform(const double *f, int n, double *hplus, double *hcross, 
 {
     gwo_src s;
     gwo_populate_source(&s, p, model);
-    if (model == GWO_IMRPHENOMD) {
+    if (model == GWO_IMRPHENOMD || model == GWO_PPE_IMRPHENOMD_INSPIRAL || model == GWO_PPE_IMRPHENOMD_IMR) {
         int status = phenomd_construct_waveform(f, n, hplus, &s);
         double ci = cos(s.incl_angle);
         double kplus = .5 * (1 + ci * ci);
@@ -718,7 +760,7 @@ int gwo_fourier_waveform(const double *f, int n, double *hplus, double *hcross, 
 
 int gwo_fourier_amplitude(const double *f, int n, double *amp, int model, const gwo_params *p)
 {
-    if (model != GWO_IMRPHENOMD) return 0;
+    if (model == GWO_IMRPHENOMPV2 || model < 0 || model > GWO_PPE_IMRPHENOMD_IMR) return 0;
     gwo_src s;
     gwo_populate_source(&s, p, model);
     return phenomd_construct_amplitude(f, n, amp, &s);
@@ -727,7 +769,7 @@ int gwo_fourier_amplitude(const double *f, int n, double *amp, int model, const 
 int gwo_fourier_phase(const double *f, int n, double *phase_plus, double *phase_cross, int model,
                       const gwo_params *p)
 {
-    if (model != GWO_IMRPHENOMD) return 0;
+    if (model == GWO_IMRPHENOMPV2 || model < 0 || model > GWO_PPE_IMRPHENOMD_IMR) return 0;
     gwo_src s;
     gwo_populate_source(&s, p, model);
     int status = phenomd_construct_phase(f, n, phase_plus, &s);

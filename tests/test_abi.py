@@ -29,10 +29,25 @@ def test_library_exports_every_declared_symbol():
     assert sorted(G.EXPORTS) == names
 
 
-def test_params_layout_matches_header():
-    # 19 doubles then 4 int32, no padding
-    assert ctypes.sizeof(G.Params) == 19 * 8 + 4 * 4
-    assert G.Params.shift_time.offset == 19 * 8
+def test_params_layout_matches_header(tmp_path):
+    """every field of the ctypes / numpy view at the offset the C compiler gives it in include/gwat_hip.h"""
+    import shutil
+    import subprocess
+    assert ctypes.sizeof(G.Params) == G.PARAMS_DTYPE.itemsize
+    for name, _ in G.Params._fields_:
+        assert getattr(G.Params, name).offset == G.PARAMS_DTYPE.fields[name][1], name
+    if shutil.which("gcc") is None:
+        return
+    names = [n for n, _ in G.Params._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "gwat_hip.h"\nint main(void){\n'
+                   'printf("%zu\\n", sizeof(gwh_params));\n' +
+                   "".join('printf("%%zu\\n", offsetof(gwh_params, %s));\n' % n for n in names) + "return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(G.Params)
+    assert out[1:] == [getattr(G.Params, n).offset for n in names]
 
 
 def test_version_string_without_gpu():
@@ -111,14 +126,15 @@ def test_gwatpy_host_helpers_and_errors():
     assert np.isnan(L.DTOA_DETECTOR_py(1.2, -0.3, 0.7, b"Hanford", b"H1"))
     f = np.linspace(20, 30, 5)
     # fields the engine does not model are refused at use, before any device work
-    for kw in (dict(equatorial_orientation=True), dict(horizon_coord=True), dict(NSflag1=True), dict(Nmod=1)):
+    for kw in (dict(equatorial_orientation=True), dict(horizon_coord=True), dict(NSflag1=True), dict(Nmod=1),
+               dict(Nmod=5, bppe=[-7.] * 5, betappe=[1e-5] * 5)):
         p = P.gen_params(mass1=30, mass2=20, **kw)
         st, _, _ = P.fourier_waveform(f, "IMRPhenomD", p)
         assert st == G.GWH_ERR_UNSUPPORTED, kw
         assert b"does not model" in G.lib().gwh_last_error()
         p.close()
     p = P.gen_params(mass1=30, mass2=20)
-    st, _, _ = P.fourier_waveform(f, "ppE_IMRPhenomD_Inspiral", p)
+    st, _, _ = P.fourier_waveform(f, "gIMRPhenomD", p)
     assert st == G.GWH_ERR_UNSUPPORTED
     st, _ = P.fourier_detector_response(f, "LIGO", "IMRPhenomD", p)
     assert st == G.GWH_ERR_UNSUPPORTED

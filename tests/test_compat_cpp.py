@@ -24,6 +24,7 @@ struct params_like {
     bool shift_time = true, shift_phase = true, sky_average = false, dep_postmerger = false,
          equatorial_orientation = false, horizon_coord = false, NSflag1 = false, NSflag2 = false;
     int Nmod = 0, Nmod_phi = 0, Nmod_sigma = 0, Nmod_beta = 0, Nmod_alpha = 0;
+    double *bppe = nullptr, *betappe = nullptr;
 };
 
 int main()
@@ -35,7 +36,20 @@ int main()
     g.NSflag2 = true;
     printf("ns %d\n", (int)gwh::to_params(g, p));
     g.NSflag2 = false;
-    printf("models %d %d %d\n", gwh::model_id("IMRPhenomD"), gwh::model_id("IMRPhenomPv2"), gwh::model_id("PhenomD"));
+    printf("models %d %d %d %d %d\n", gwh::model_id("IMRPhenomD"), gwh::model_id("IMRPhenomPv2"),
+           gwh::model_id("PhenomD"), gwh::model_id("ppE_IMRPhenomD_Inspiral"), gwh::model_id("ppE_IMRPhenomD_IMR"));
+    /* ppE fields: read for the ppE models only (a GR method ignores them, as prep_source_parameters does) */
+    double b[2] = {-7, -3}, be[2] = {1e-5, -2e-3};
+    g.Nmod = 2;
+    g.bppe = b;
+    g.betappe = be;
+    int okgr = gwh::to_params(g, p, GWH_IMRPHENOMD);
+    int ngr = p.ppe_nmod;
+    int okppe = gwh::to_params(g, p, GWH_PPE_IMRPHENOMD_IMR);
+    printf("ppe %d %d %d %d %.17g %.17g\n", okgr, ngr, okppe, p.ppe_nmod, p.ppe_b[0], p.ppe_beta[1]);
+    g.Nmod = GWH_PPE_MAX + 1;
+    printf("ppemax %d\n", (int)gwh::to_params(g, p, GWH_PPE_IMRPHENOMD_INSPIRAL));
+    g.Nmod = 0;
     printf("bases %d %d %d %d %d\n", gwh::basis_id("IMRPhenomD", true), gwh::basis_id("IMRPhenomD", false),
            gwh::basis_id("MCMC_IMRPhenomD", false), gwh::basis_id("IMRPhenomPv2", false),
            gwh::basis_id("MCMC_IMRPhenomPv2", false));
@@ -65,7 +79,10 @@ def test_compat_header_with_reference_shaped_params(tmp_path):
     assert f[:2] == ["ok", "1"] and float(f[2]) == 36 and float(f[3]) == 410 and float(f[4]) == -0.2
     assert float(f[5]) == 2.9 and float(f[6]) == 2.45 and f[7] == "1"
     assert out[1] == "ns 0"
-    assert out[2] == "models 0 1 -1"
+    assert out[2] == "models 0 1 -1 %d %d" % (G.PPE_IMRPHENOMD_INSPIRAL, G.PPE_IMRPHENOMD_IMR)
+    assert out[3].split() == ["ppe", "1", "0", "1", "2", "-7", "-0.002"]
+    assert out[4] == "ppemax 0"
+    out = out[:3] + out[5:]
     assert out[3] == "bases %d %d %d %d %d" % (G.BASIS_PHENOMD_SKYAVG, G.BASIS_PHENOMD_FULL, G.BASIS_PHENOMD_MCMC,
                                              G.BASIS_PV2_FULL, G.BASIS_PV2_MCMC)
     r = out[4].split()

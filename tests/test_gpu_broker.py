@@ -19,7 +19,7 @@ def _events(k=6):
     for sp in specs:
         f, psd = sp["f"], sp["psd"]
         inj = sp["injection"]
-        p = G.make_params(**{n: (list(inj[n]) if n in ("spin1", "spin2") else inj[n].item())
+        p = G.make_params(**{n: (list(inj[n]) if inj[n].shape else inj[n].item())
                              for n in inj.dtype.names})
         data = [r + W.noise(f, psd, g) for r in G.create_coherent_GW_detection(f, p, sp["dets"])]
         out.append((G.Event(f, data, [psd] * 3, sp["dets"]), sp))

@@ -150,7 +150,7 @@ def _points(model, n):
 
 def _oracle_params(rec):
     kw = {k: (list(rec[k]) if rec[k].shape else float(rec[k])) for k in rec.dtype.names}
-    for k in ("shift_time", "shift_phase", "sky_average", "dep_postmerger"):
+    for k in ("shift_time", "shift_phase", "sky_average", "dep_postmerger", "ppe_nmod", "ppe_pad"):
         kw[k] = int(rec[k])
     return O.make_params(**kw)
 

@@ -90,7 +90,7 @@ def test_log_likelihood_single_detector():
     f, psd, data = _c0_data()
     P = W.config0(32)["params"]
     for i in range(0, 32, 4):
-        kw = {k: (list(P[i][k]) if k in ("spin1", "spin2") else P[i][k].item()) for k in P.dtype.names}
+        kw = {k: (list(P[i][k]) if P[i][k].shape else P[i][k].item()) for k in P.dtype.names}
         g, o = both(**kw)
         a = G.log_likelihood(data, psd, f, g, 0)
         b = O.log_likelihood(data, psd, f, o, 0)
@@ -112,7 +112,7 @@ def _net_setup(n_bins_df=0.125, fmax=2048.):
 
 
 def _oracle_params(rec):
-    return O.make_params(**{k: (list(rec[k]) if k in ("spin1", "spin2") else rec[k].item())
+    return O.make_params(**{k: (list(rec[k]) if rec[k].shape else rec[k].item())
                             for k in rec.dtype.names})
 
 
@@ -163,7 +163,7 @@ def test_batch_result_independent_of_batch_composition():
 def test_mcmc_likelihood_extrinsic_T_eff():
     f, psd, dets, data = _net_setup(0.25, 1024.)
     p = W.injection_params(1)[0]
-    g = G.make_params(**{k: (list(p[k]) if k in ("spin1", "spin2") else p[k].item()) for k in p.dtype.names})
+    g = G.make_params(**{k: (list(p[k]) if p[k].shape else p[k].item()) for k in p.dtype.names})
     o = _oracle_params(p)
     for T in (0.0, 4.0):
         a = G.mcmc_likelihood_extrinsic(f, data, [psd] * 3, dets, g, T_eff=T)

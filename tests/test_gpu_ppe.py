@@ -1,0 +1,130 @@
+"""ppE phase terms on IMRPhenomD (ppE_IMRPhenomD_Inspiral / ppE_IMRPhenomD_IMR, src/ppE_IMRPhenomD.cpp) on the GPU
+against the CPU oracle: waveforms <= 1e-10 relative per bin, network log-likelihood <= 1e-9 relative (FP64),
+mixed within 1e-3 of FP64, zero coefficients equal to IMRPhenomD bit for bit, and the refusals (more than
+GWH_PPE_MAX terms, ppE Fisher bases)."""
+import numpy as np
+import pytest
+
+import gw_analysis_tools_amd as G
+import oracle as O
+from gw_analysis_tools_amd import gwatpy_compat as P
+from gw_analysis_tools_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+WF_TOL = 1e-10
+LL_TOL = 1e-9
+EXTR = dict(DL=410., RA=1.37, DEC=-1.26, psi=2.66, incl=2.9, f_ref=20., gmst=W.GMST, tc=0.01, phiRef=0.3)
+MODELS = [G.PPE_IMRPHENOMD_INSPIRAL, G.PPE_IMRPHENOMD_IMR]
+TERMS = [dict(ppe_nmod=1, ppe_b=[-7.], ppe_beta=[3e-5]),
+         dict(ppe_nmod=2, ppe_b=[-7., -3.], ppe_beta=[3e-5, -3e-3]),
+         dict(ppe_nmod=3, ppe_b=[-5., -1., 1.5], ppe_beta=[-1e-3, 2e-2, 0.3])]
+CASES = [dict(mass1=36, mass2=29, spin1=[0, 0, 0.3], spin2=[0, 0, -0.1]),
+         dict(mass1=80, mass2=12, spin1=[0, 0, -0.8], spin2=[0, 0, 0.5]),
+         dict(mass1=5.5, mass2=4.2, spin1=[0, 0, 0.9], spin2=[0, 0, 0.9])]
+
+
+def both(**kw):
+    return G.make_params(**kw), O.make_params(**kw)
+
+
+def wf_close(a, b, tol=WF_TOL):
+    nz = np.abs(b) > 0
+    assert np.all(a[~nz] == 0)
+    rel = np.abs(a[nz] - b[nz]) / np.abs(b[nz])
+    assert rel.max() < tol, rel.max()
+
+
+@pytest.mark.parametrize("model", MODELS)
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("terms", TERMS)
+def test_ppe_waveform_matches_oracle(model, case, terms):
+    f = W.uniform_grid(20., 1024., 0.25)
+    g, o = both(**case, **EXTR, **terms)
+    hp, hc = G.fourier_waveform(f, g, model)
+    op, oc = O.fourier_waveform(f, o, model)
+    wf_close(hp, op)
+    wf_close(hc, oc)
+    # and it is not GR
+    gp, _ = G.fourier_waveform(f, g, G.IMRPHENOMD)
+    assert np.max(np.abs(np.angle(hp[np.abs(op) > 0] / gp[np.abs(op) > 0]))) > 1e-3
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_ppe_zero_terms_equal_gr_and_limits(model):
+    f = W.uniform_grid(20., 1024., 0.25)
+    kw = dict(CASES[0], **EXTR)
+    gr, _ = G.fourier_waveform(f, G.make_params(**kw), G.IMRPHENOMD)
+    z, _ = G.fourier_waveform(f, G.make_params(**kw, ppe_nmod=2, ppe_b=[-7., -3.], ppe_beta=[0., 0.]), model)
+    assert np.array_equal(gr, z)
+    neg, _ = G.fourier_waveform(f, G.make_params(**kw, ppe_nmod=-1, ppe_b=[-7.], ppe_beta=[1.]), model)
+    assert np.array_equal(gr, neg)  # Nmod < 0: no terms, as the reference's loop
+    # GR models ignore the ppE fields (prep_source_parameters copies them for the ppE methods only)
+    ign, _ = G.fourier_waveform(f, G.make_params(**kw, **TERMS[1]), G.IMRPHENOMD)
+    assert np.array_equal(gr, ign)
+    # more terms than the record holds: NaN, never a silently truncated model
+    over, _ = G.fourier_waveform(f, G.make_params(**kw, ppe_nmod=G.PPE_MAX + 1, ppe_b=[-7.] * 4,
+                                                  ppe_beta=[1e-5] * 4), model)
+    assert np.all(np.isnan(over[:100]))
+
+
+def _net():
+    f = W.uniform_grid(20., 512., 0.125)
+    psd = W.analytic_psd(f)
+    dets = [0, 1, 2]
+    inj_g, inj_o = both(**W.INJECTION, ppe_nmod=1, ppe_b=[-7.], ppe_beta=[2e-5])
+    data = []
+    g = W.rng(1, 3)
+    for d in dets:
+        h = O.detector_response(f, inj_o, d, O.PPE_IMRPHENOMD_INSPIRAL)
+        dt = O.dtoa(inj_o.RA, inj_o.DEC, inj_o.gmst, dets[0], d)
+        data.append(h * np.exp(-2j * np.pi * dt * f) + W.noise(f, psd, g))
+    return f, psd, dets, data
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_ppe_network_loglike_batch(model):
+    f, psd, dets, data = _net()
+    ev = G.Event(f, data, [psd] * 3, dets)
+    P_ = W.perturbed_params(W.injection_params(1)[0], 48, step=7)
+    rng = np.random.default_rng(11)
+    P_["ppe_nmod"] = rng.integers(0, 3, len(P_))
+    P_["ppe_b"][:, 0], P_["ppe_b"][:, 1] = -7., -3.
+    P_["ppe_beta"][:, 0] = rng.normal(0, 3e-5, len(P_))
+    P_["ppe_beta"][:, 1] = rng.normal(0, 3e-3, len(P_))
+    ll = G.loglike_batch(ev, P_, model, G.FP64)
+    for i in range(0, len(P_), 6):
+        rec = P_[i]
+        kw = {k: (list(rec[k]) if rec[k].shape else rec[k].item()) for k in rec.dtype.names}
+        ref = O.network_loglike(f, data, [psd] * 3, dets, O.make_params(**kw), model)
+        assert abs(ll[i] - ref) <= LL_TOL * max(abs(ref), 1), (i, ll[i], ref)
+    mx = G.loglike_batch(ev, P_, model, G.MIXED)
+    assert np.max(np.abs(mx - ll)) <= 1e-3
+    # each instance alone gives the same FP64 value (batch independence with mixed term counts)
+    for i in (0, 5, 17):
+        assert G.loglike_batch(ev, P_[i:i + 1], model, G.FP64)[0] == ll[i]
+    ev.close()
+
+
+def test_ppe_refusals_and_gwatpy():
+    f = W.uniform_grid(20., 512., 0.25)
+    psd = W.analytic_psd(f)
+    ev = G.Event(f, [np.zeros(len(f), complex)], [psd], [0])
+    P_ = W.perturbed_params(W.injection_params(1)[0], 2, step=1)
+    P_["ppe_nmod"] = 1
+    P_["ppe_b"][:, 0] = -7.
+    # the reference's ppE Fisher bases carry betappe as parameters: not provided, refused
+    with pytest.raises(G.GWHError) as e:
+        G.fisher_batch(ev, P_, G.PPE_IMRPHENOMD_IMR, G.BASIS_PHENOMD_FULL, 11)
+    assert "ppE" in str(e.value)
+    ev.close()
+    # gwatpy handles carry Nmod / bppe / betappe to the ppE methods
+    p = P.gen_params(mass1=36, mass2=29, spin1=(0, 0, .3), spin2=(0, 0, -.1), Luminosity_Distance=410.,
+                     incl_angle=2.9, RA=1.37, DEC=-1.26, psi=2.66, gmst=W.GMST, tc=0.01, phiRef=0.3, f_ref=20.,
+                     Nmod=2, bppe=[-7., -3.], betappe=[3e-5, -3e-3])
+    st, hp, hc = P.fourier_waveform(f, "ppE_IMRPhenomD_IMR", p)
+    assert st == G.GWH_OK
+    o = O.make_params(**CASES[0], **EXTR, **TERMS[1])
+    op, oc = O.fourier_waveform(f, o, O.PPE_IMRPHENOMD_IMR)
+    wf_close(hp, op)
+    p.close()

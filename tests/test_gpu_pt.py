@@ -15,7 +15,7 @@ def _setup():
     cfg = W.config1()
     f, psd, dets = cfg["f"], cfg["psd"], cfg["dets"]
     inj = W.injection_params(1)[0]
-    pinj = G.make_params(**{k: (list(inj[k]) if k in ("spin1", "spin2") else inj[k].item())
+    pinj = G.make_params(**{k: (list(inj[k]) if inj[k].shape else inj[k].item())
                             for k in inj.dtype.names})
     resp = G.create_coherent_GW_detection(f, pinj, dets)
     g = W.rng(1, 0)

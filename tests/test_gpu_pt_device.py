@@ -31,7 +31,7 @@ def _small_event(df=0.25, fmax=512.):
     psd = W.analytic_psd(f)
     dets = [0, 1, 2]
     inj = W.injection_params(1)[0]
-    pinj = G.make_params(**{k: (list(inj[k]) if k in ("spin1", "spin2") else inj[k].item())
+    pinj = G.make_params(**{k: (list(inj[k]) if inj[k].shape else inj[k].item())
                             for k in inj.dtype.names})
     resp = G.create_coherent_GW_detection(f, pinj, dets)
     g = W.rng(1, 0)
@@ -40,7 +40,7 @@ def _small_event(df=0.25, fmax=512.):
 
 
 def _oracle_ll(f, data, psd, dets, inj):
-    tmpl = O.make_params(**{k: (list(inj[k]) if k in ("spin1", "spin2") else inj[k].item())
+    tmpl = O.make_params(**{k: (list(inj[k]) if inj[k].shape else inj[k].item())
                             for k in inj.dtype.names})
 
     def ll(thetas):
@@ -92,7 +92,7 @@ def _c1_event():
     cfg = W.config1()
     f, psd, dets = cfg["f"], cfg["psd"], cfg["dets"]
     inj = W.injection_params(1)[0]
-    pinj = G.make_params(**{k: (list(inj[k]) if k in ("spin1", "spin2") else inj[k].item())
+    pinj = G.make_params(**{k: (list(inj[k]) if inj[k].shape else inj[k].item())
                             for k in inj.dtype.names})
     resp = G.create_coherent_GW_detection(f, pinj, dets)
     g = W.rng(1, 0)
@@ -201,7 +201,7 @@ def test_loglike_graph_multi_event():
     evs, P, ids = [], [], []
     for sp in specs:
         f, psd, inj = sp["f"], sp["psd"], sp["injection"]
-        pinj = G.make_params(**{k: (list(inj[k]) if k in ("spin1", "spin2") else inj[k].item())
+        pinj = G.make_params(**{k: (list(inj[k]) if inj[k].shape else inj[k].item())
                                 for k in inj.dtype.names})
         data = [r + W.noise(f, psd, g) for r in G.create_coherent_GW_detection(f, pinj, sp["dets"])]
         e = G.Event(f, data, [psd] * 3, sp["dets"])

@@ -20,7 +20,7 @@ def _events(n_events=16, chains=3):
     for k, sp in enumerate(specs):
         f, psd = sp["f"], sp["psd"]
         inj = sp["injection"]
-        pinj = G.make_params(**{n: (list(inj[n]) if n in ("spin1", "spin2") else inj[n].item())
+        pinj = G.make_params(**{n: (list(inj[n]) if inj[n].shape else inj[n].item())
                                 for n in inj.dtype.names})
         resp = G.create_coherent_GW_detection(f, pinj, sp["dets"])
         data = [r + W.noise(f, psd, g) for r in resp]
@@ -33,7 +33,7 @@ def _events(n_events=16, chains=3):
 
 def _oracle_params(rec):
     kw = {k: (list(rec[k]) if rec[k].shape else float(rec[k])) for k in rec.dtype.names}
-    for k in ("shift_time", "shift_phase", "sky_average", "dep_postmerger"):
+    for k in ("shift_time", "shift_phase", "sky_average", "dep_postmerger", "ppe_nmod", "ppe_pad"):
         kw[k] = int(rec[k])
     return O.make_params(**kw)
 

@@ -284,3 +284,74 @@ def test_unpack_repack_round_trip_and_closed_forms(method, dim, sky):
             assert abs(getattr(q, name) - getattr(p, name)) < 1e-12, name
     if method == "MCMC_IMRPhenomPv2":
         assert np.allclose(list(q.spin1), kw["spin1"], atol=1e-12) and np.allclose(list(q.spin2), kw["spin2"], atol=1e-12)
+
+
+# ---- ppE phase terms (ppE_IMRPhenomD.cpp) --------------------------------------------------------------
+MSOL_SEC = 4.925491025543575903411922162094833998e-6
+
+
+def _ppe_case(model, nmod=2, b=(-7.0, -3.0), beta=(3e-5, -3e-3)):
+    kw = dict(mass1=36., mass2=29., spin1=[0, 0, .3], spin2=[0, 0, -.1], DL=410., incl=.4, f_ref=20., phiRef=.3,
+              tc=0.)
+    gr = O.make_params(**kw)
+    pp = O.make_params(ppe_nmod=nmod, ppe_b=list(b), ppe_beta=list(beta), **kw)
+    mc = (36. * 29.) ** 0.6 / (65.) ** 0.2 * MSOL_SEC
+    f = np.linspace(20., 0.2 / (65. * MSOL_SEC) * 0.999, 3001)
+
+    def T(x):
+        return sum(be * (np.pi * mc * x) ** (bb / 3.) for bb, be in zip(b[:nmod], beta[:nmod]))
+
+    def dT(x):
+        return sum(be * (bb / 3.) * (np.pi * mc * x) ** (bb / 3.) / x for bb, be in zip(b[:nmod], beta[:nmod]))
+    return gr, pp, f, T, dT
+
+
+def _linear_residual(f, d):
+    A = np.vstack([np.ones_like(f), f]).T
+    c, *_ = np.linalg.lstsq(A, d, rcond=None)
+    return np.max(np.abs(d - A @ c))
+
+
+def test_ppe_zero_coefficients_reduce_to_gr():
+    for model in (O.PPE_IMRPHENOMD_INSPIRAL, O.PPE_IMRPHENOMD_IMR):
+        gr, pp, f, T, dT = _ppe_case(model, beta=(0., 0.))
+        a = O.fourier_waveform(f, gr, O.IMRPHENOMD)
+        b = O.fourier_waveform(f, pp, model)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_ppe_imr_adds_the_term_in_every_region():
+    """ppE_IMRPhenomD_IMR: the term enters all three phases, so the C1 connections are unchanged and the
+    phase differs from GR by T(f) plus a line (the phase and time alignment shifts)"""
+    gr, pp, f, T, dT = _ppe_case(O.PPE_IMRPHENOMD_IMR)
+    g = O.lib()
+    n = len(f)
+    pg, pp_ = np.zeros(n), np.zeros(n)
+    cross = np.zeros(n)
+    g.gwo_fourier_phase(O._dp(f), n, O._dp(pg), O._dp(cross), O.IMRPHENOMD, O.ctypes.byref(gr))
+    g.gwo_fourier_phase(O._dp(f), n, O._dp(pp_), O._dp(cross), O.PPE_IMRPHENOMD_IMR, O.ctypes.byref(pp))
+    # fourier_phase returns -(Psi - tc (f - f_ref) - phic)
+    d = -(pp_ - pg) - T(f)
+    assert np.max(np.abs(T(f))) > 1.0  # the term is not negligible on this grid
+    assert _linear_residual(f, d) < 1e-8 * np.max(np.abs(pg))
+    # a wrong sign or exponent leaves a curved residual
+    assert _linear_residual(f, -(pp_ - pg) + T(f)) > 1e-2
+    assert _linear_residual(f, -(pp_ - pg) - 1.01 * T(f)) > 1e-4  # a 1 % coefficient error
+
+
+def test_ppe_inspiral_term_is_carried_by_the_connections():
+    """ppE_IMRPhenomD_Inspiral: the term enters the inspiral phase only; the C1 connection at f1 = 0.018/M
+    carries its value and slope, so above f1 the phase differs from GR by the tangent of T at f1 (plus the
+    common alignment line)"""
+    gr, pp, f, T, dT = _ppe_case(O.PPE_IMRPHENOMD_INSPIRAL)
+    g = O.lib()
+    n = len(f)
+    pg, pp_, cross = np.zeros(n), np.zeros(n), np.zeros(n)
+    g.gwo_fourier_phase(O._dp(f), n, O._dp(pg), O._dp(cross), O.IMRPHENOMD, O.ctypes.byref(gr))
+    g.gwo_fourier_phase(O._dp(f), n, O._dp(pp_), O._dp(cross), O.PPE_IMRPHENOMD_INSPIRAL, O.ctypes.byref(pp))
+    f1 = 0.018 / (65. * MSOL_SEC)
+    model_diff = np.where(f < f1, T(f), T(f1) + dT(f1) * (f - f1))
+    d = -(pp_ - pg) - model_diff
+    assert _linear_residual(f, d) < 1e-8 * np.max(np.abs(pg))
+    # the IMR variant's form does not fit this one
+    assert _linear_residual(f, -(pp_ - pg) - T(f)) > 1e-3
