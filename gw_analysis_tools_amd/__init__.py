@@ -50,6 +50,8 @@ EXPORTS = [
     "gwh_loglike_graph_create", "gwh_graph_launch", "gwh_graph_synchronize", "gwh_graph_destroy",
     "gwh_pt_create", "gwh_pt_run", "gwh_pt_synchronize", "gwh_pt_state", "gwh_pt_last_step", "gwh_pt_counters",
     "gwh_pt_destroy",
+    "gwh_maximized_log_likelihood_aligned_internal", "gwh_maximized_log_likelihood_unaligned_internal",
+    "gwh_maximized_coal_log_likelihood_internal", "gwh_maximized_loglike_batch",
 ]
 
 
@@ -173,6 +175,10 @@ def lib():
         "gwh_pt_counters": ([I, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                              ctypes.POINTER(ctypes.c_longlong)], I),
         "gwh_pt_destroy": ([I], I),
+        "gwh_maximized_log_likelihood_aligned_internal": ([dp, dp, dp, dp, ctypes.c_size_t], D),
+        "gwh_maximized_log_likelihood_unaligned_internal": ([dp, dp, dp, dp, dp, ctypes.c_size_t], D),
+        "gwh_maximized_coal_log_likelihood_internal": ([dp, dp, dp, dp, ctypes.c_size_t, dp, dp], D),
+        "gwh_maximized_loglike_batch": ([I, I, V, I, dp], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

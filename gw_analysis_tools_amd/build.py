@@ -45,7 +45,9 @@ def build_engine(verbose=False, jobs=8):
     for p in procs:
         _wait(p)
     tmp = LIB + ".tmp"
-    cmd = [hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    cmd = [hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + \
+        ["-L" + rocm_lib, "-lrocfft", "-Wl,-rpath," + rocm_lib, "-lpthread"]
     if verbose:
         print(" ".join(cmd))
     subprocess.check_call(cmd)

@@ -59,6 +59,40 @@ __device__ __forceinline__ BinPow gwh_load_bin(const EventDesc &E, int k)
     return b;
 }
 
+/* w_k of simpsons_sum (util.h:841-856): 1,4,2,4,...,(2|4),1 by index parity for any n */
+__device__ __forceinline__ double simpson_w(int k, int n)
+{
+    if (k == 0 || k == n - 1) return (n == 1) ? 2.0 : 1.0;
+    return (k & 1) ? 4.0 : 2.0;
+}
+
+/* lane move of an FP64 value by a DPP pattern (two 32-bit moves) */
+template <int CTRL> __device__ __forceinline__ double dpp_move(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane)
+{
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+/* Sum over the 64 lanes of a wave in a fixed order, with DPP lane moves on the vector ALU instead of
+ * LDS permutes: pairs (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1), half rows (row_half_mirror), rows
+ * (row_mirror); then the four row sums, read from lanes 0, 16, 32, 48, added as (r0 + r1) + (r2 + r3).
+ * Every lane must be active; the result is uniform. */
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_move<0xB1>(v);
+    v += dpp_move<0x4E>(v);
+    v += dpp_move<0x141>(v);
+    v += dpp_move<0x140>(v);
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
 struct Event {
     bool live = false;
     int N = 0, D = 0;
@@ -71,6 +105,7 @@ struct Event {
     double df_fisher = 0;  /* f[1] - f[0]: the Fisher's Simpson step (fisher.cpp:2746) */
     uint64_t last_use = 0;
     int pins = 0;          /* step graphs holding this event's buffers: no destroy, eviction or re-tiling */
+    int method = 0, log10F = 0; /* integration rule of the stored weights */
 };
 
 /* grow-only device buffer */
@@ -127,6 +162,7 @@ struct Engine {
     DevBuf params, coef, partial, active, logl, inst_event, f, data, psd, wts, out1, out2, misc;
     DevBuf work, wcount; /* flattened (instance, tile) lists of multi-event batches */
     FisherScratch fisher; /* stream = the engine stream */
+    DevBuf fft, fftwork, mxpart, mxres, mxin; /* FFT-maximised likelihoods */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     /* live timing of the per-bin kernel inside a caller's timed region: event pairs around each launch */
     std::vector<hipEvent_t> kt_ev;

@@ -183,13 +183,6 @@ int gwh_ensure_init()
 /* event preparation                                                          */
 /* ------------------------------------------------------------------------- */
 
-/* w_k of simpsons_sum (util.h:841-856): 1,4,2,4,...,(2|4),1 by index parity for any n */
-__device__ __forceinline__ double simpson_w(int k, int n)
-{
-    if (k == 0 || k == n - 1) return (n == 1) ? 2.0 : 1.0;
-    return (k & 1) ? 4.0 : 2.0;
-}
-
 /* the tabulated powers of every bin (five arrays of `stride` doubles) */
 __global__ void gwh_bin_powers_kernel(const double *__restrict__ f, int N, double *__restrict__ out, size_t stride)
 {
@@ -300,6 +293,8 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     ev.block = blk;
     ev.bytes = total;
     ev.key = key;
+    ev.method = method;
+    ev.log10F = log10F;
     ev.last_use = ++E.use_clock;
     /* the stored weights carry the step of df_mode (df_like below); the Fisher rescales to f[1]-f[0] */
     double df_l = f[N / 2] - f[N / 2 - 1], df_f = f[1] - f[0];
@@ -415,33 +410,6 @@ int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_para
 /* ------------------------------------------------------------------------- */
 /* reductions                                                                 */
 /* ------------------------------------------------------------------------- */
-
-/* lane move of an FP64 value by a DPP pattern (two 32-bit moves) */
-template <int CTRL> __device__ __forceinline__ double dpp_move(double v)
-{
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ double read_lane(double v, int lane)
-{
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
-                            __builtin_amdgcn_readlane(__double2loint(v), lane));
-}
-
-/* Sum over the 64 lanes of a wave in a fixed order, with DPP lane moves on the vector ALU instead of
- * LDS permutes: pairs (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1), half rows (row_half_mirror), rows
- * (row_mirror); then the four row sums, read from lanes 0, 16, 32, 48, added as (r0 + r1) + (r2 + r3).
- * Every lane must be active; the result is uniform. */
-__device__ __forceinline__ double wave_sum(double v)
-{
-    v += dpp_move<0xB1>(v);
-    v += dpp_move<0x4E>(v);
-    v += dpp_move<0x141>(v);
-    v += dpp_move<0x140>(v);
-    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
-}
 
 /* Block-reduce NV values per thread (fixed order) and write them to out[0..NV) */
 template <int NV>

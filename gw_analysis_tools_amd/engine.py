@@ -15,7 +15,9 @@ __all__ = ["fourier_waveform", "fourier_detector_response", "create_coherent_GW_
            "log_likelihood", "log_likelihood_internal", "mcmc_likelihood_extrinsic", "fisher_numerical",
            "fisher_derivatives", "unpack_parameters", "repack_parameters", "detector_id", "dtoa",
            "gps_to_gmst_radian", "calculate_snr_internal", "data_snr", "snr_batch", "Broker", "Event", "loglike_batch",
-           "fisher_batch", "DeviceArray", "set_bins_per_lane", "synchronize", "LoglikeGraph", "DevicePT"]
+           "fisher_batch", "DeviceArray", "set_bins_per_lane", "synchronize", "LoglikeGraph", "DevicePT",
+           "maximized_log_likelihood_aligned_internal", "maximized_log_likelihood_unaligned_internal",
+           "maximized_coal_log_likelihood_internal", "maximized_loglike_batch"]
 
 
 def _dp(a):
@@ -454,3 +456,37 @@ class DevicePT:
             self.close()
         except Exception:
             pass
+
+
+def maximized_log_likelihood_aligned_internal(data, psd, f, response):
+    """maximized_Log_Likelihood_aligned_spin_internal (src/mcmc_gw.cpp:595-654)"""
+    f = _f64(f)
+    v = lib().gwh_maximized_log_likelihood_aligned_internal(_dp(_interleave(data)), _dp(_f64(psd)), _dp(f),
+                                                            _dp(_interleave(response)), len(f))
+    return _nan_check(v)
+
+
+def maximized_log_likelihood_unaligned_internal(data, psd, f, hplus, hcross):
+    """maximized_Log_Likelihood_unaligned_spin_internal (src/mcmc_gw.cpp:660-800)"""
+    f = _f64(f)
+    v = lib().gwh_maximized_log_likelihood_unaligned_internal(_dp(_interleave(data)), _dp(_f64(psd)), _dp(f),
+                                                              _dp(_interleave(hplus)), _dp(_interleave(hcross)), len(f))
+    return _nan_check(v)
+
+
+def maximized_coal_log_likelihood_internal(data, psd, f, response):
+    """maximized_coal_Log_Likelihood_internal (src/mcmc_gw.cpp:493-556): (logL, tc, phiRef)"""
+    f = _f64(f)
+    tc, ph = np.zeros(1), np.zeros(1)
+    v = lib().gwh_maximized_coal_log_likelihood_internal(_dp(_interleave(data)), _dp(_f64(psd)), _dp(f),
+                                                         _dp(_interleave(response)), len(f), _dp(tc), _dp(ph))
+    return _nan_check(v), float(tc[0]), float(ph[0])
+
+
+def maximized_loglike_batch(event, params, model=IMRPHENOMD):
+    """the intrinsic MCMC likelihood of every parameter set on a resident event (src/mcmc_gw.cpp:2607-2700)"""
+    arr = _as_params_array(params)
+    out = np.zeros(len(arr))
+    ev0 = event.id if isinstance(event, Event) else int(event)
+    check(lib().gwh_maximized_loglike_batch(ev0, model, arr.ctypes.data_as(ctypes.c_void_p), len(arr), _dp(out)))
+    return out

@@ -262,6 +262,34 @@ int gwh_broker_wait(int broker, long long ticket, double timeout_ms, double *lat
  * peak device bytes (resident events + broker scratch), tickets not yet waited for */
 int gwh_broker_stats(int broker, double *stats);
 
+/* ---- likelihoods maximised over extrinsic parameters (FFT of the matched-filter integrand, rocFFT) ---- */
+
+/* maximized_Log_Likelihood_aligned_spin_internal(std::complex<double>* data, double* psd, double* frequencies,
+ * std::complex<double>* detector_response, size_t length, fftw_outline*) -- src/mcmc_gw.cpp:595-654:
+ * 0.5 max_k |FFT(4 d* h / S)_k|^2 df^2 / (h|h), df = f[1] - f[0] (maximum over time, phase and distance) */
+double gwh_maximized_log_likelihood_aligned_internal(const double *data, const double *psd, const double *frequencies,
+                                                     const double *response, size_t length);
+
+/* maximized_Log_Likelihood_unaligned_spin_internal(data, psd, frequencies, hplus, hcross, length, plan)
+ * -- src/mcmc_gw.cpp:660-800: also maximised over the polarisation, 0.25 max_k lambda_k */
+double gwh_maximized_log_likelihood_unaligned_internal(const double *data, const double *psd,
+                                                       const double *frequencies, const double *hplus,
+                                                       const double *hcross, size_t length);
+
+/* maximized_coal_Log_Likelihood_internal(data, psd, frequencies, detector_response, length, plan, double* tc,
+ * double* phiRef) -- src/mcmc_gw.cpp:493-556: -0.5 ((h|h) - 2 max_k |FFT_k| df), df = f[N/2] - f[N/2-1];
+ * tc = k_max / N / df and phiRef = arg FFT_{k_max} at the first maximum */
+double gwh_maximized_coal_log_likelihood_internal(const double *data, const double *psd, const double *frequencies,
+                                                  const double *response, size_t length, double *tc, double *phiRef);
+
+/* The intrinsic MCMC likelihood (src/mcmc_gw.cpp:2607-2700): for every parameter set, with the extrinsic
+ * values fixed as there (psi = incl = 0, phiRef = 1, tc = 1, f_ref = 10 Hz, or 20 Hz for IMRPhenomPv2), the
+ * maximised statistic of each detector of a resident Simpson-rule event, summed over its detectors: the
+ * aligned statistic on the plus polarisation for the IMRPhenomD models (the reference's horizon response at
+ * theta = phi = psi = 0 is proportional to it, and the statistic is invariant under that factor), the
+ * unaligned one on (h+, hx) for IMRPhenomPv2.  FP64; host parameters in, host values out. */
+int gwh_maximized_loglike_batch(int event, int model, const gwh_params *params, int n, double *out);
+
 /* ---- sampler steps as replayable HIP graphs ---- */
 
 /* Likelihood step of a sampler that makes its own proposals: the launch sequence of
