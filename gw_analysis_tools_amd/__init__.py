@@ -20,6 +20,7 @@ IMRPHENOMPV2 = 1
 PPE_IMRPHENOMD_INSPIRAL = 2
 PPE_IMRPHENOMD_IMR = 3
 PPE_MAX = 4
+GIMRPHENOMD = 4
 FP64 = 0
 MIXED = 1
 SIMPSONS = 0
@@ -70,7 +71,9 @@ class Params(ctypes.Structure):
                 ("shift_time", ctypes.c_int32), ("shift_phase", ctypes.c_int32),
                 ("sky_average", ctypes.c_int32), ("dep_postmerger", ctypes.c_int32),
                 ("ppe_nmod", ctypes.c_int32), ("ppe_pad", ctypes.c_int32),
-                ("ppe_b", ctypes.c_double * 4), ("ppe_beta", ctypes.c_double * 4)]
+                ("ppe_b", ctypes.c_double * 4), ("ppe_beta", ctypes.c_double * 4),
+                ("gimr_dphi", ctypes.c_double * 14), ("gimr_dsigma", ctypes.c_double * 3),
+                ("gimr_dbeta", ctypes.c_double * 2), ("gimr_dalpha", ctypes.c_double * 4)]
 
 
 class PTConfig(ctypes.Structure):
@@ -86,8 +89,9 @@ PARAMS_DTYPE = np.dtype([("mass1", "f8"), ("mass2", "f8"), ("spin1", "f8", 3), (
                          ("DEC", "f8"), ("psi", "f8"), ("gmst", "f8"), ("chip", "f8"), ("phip", "f8"),
                          ("shift_time", "i4"), ("shift_phase", "i4"), ("sky_average", "i4"),
                          ("dep_postmerger", "i4"), ("ppe_nmod", "i4"), ("ppe_pad", "i4"), ("ppe_b", "f8", 4),
-                         ("ppe_beta", "f8", 4)])
-assert PARAMS_DTYPE.itemsize == ctypes.sizeof(Params) == 240
+                         ("ppe_beta", "f8", 4), ("gimr_dphi", "f8", 14), ("gimr_dsigma", "f8", 3),
+                         ("gimr_dbeta", "f8", 2), ("gimr_dalpha", "f8", 4)])
+assert PARAMS_DTYPE.itemsize == ctypes.sizeof(Params) == 424
 
 
 def make_params(**kw):
@@ -97,7 +101,7 @@ def make_params(**kw):
     p.shift_time = 1
     p.shift_phase = 1
     for k, v in kw.items():
-        if k in ("spin1", "spin2", "ppe_b", "ppe_beta"):
+        if k in ("spin1", "spin2", "ppe_b", "ppe_beta", "gimr_dphi", "gimr_dsigma", "gimr_dbeta", "gimr_dalpha"):
             getattr(p, k)[:len(v)] = list(v)
         else:
             setattr(p, k, v)

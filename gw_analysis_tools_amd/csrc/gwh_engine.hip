@@ -519,6 +519,7 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, c
         double Mf43 = Mf * u;
         psi = -GWH_PI / 4. + C.K * poly * (iv2 * iv2 * iv) +
               (C.sig[0] * Mf + C.sig[1] * Mf43 + C.sig[2] * Mf43 * u + C.sig[3] * Mf * Mf);
+        if (C.gimr) psi += gwh_gimr_bin(C, iv);
     } else if (f > C.f2p) {
         psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] * (r2 * r) + C.alpha[3] * (C.M34 * bp.f34) +
               C.alpha[4] * atan((f - C.alpha[5]) * C.inv_fdamp);

@@ -9,7 +9,8 @@
  * geometry helpers.  The handle here is the engine's own record; parameters the engine does not model
  * (equatorial or horizon orientation, neutron-star flags, more than GWH_PPE_MAX ppE terms) are recorded
  * and reported as GWH_ERR_UNSUPPORTED when the handle is used, never silently dropped.  ppE terms are
- * used by the ppE_IMRPhenomD methods; like the reference, other methods ignore them and the gIMR terms.
+ * used by the ppE_IMRPhenomD methods and gIMR terms by gIMRPhenomD; like the reference, other methods
+ * ignore them.
  */
 #include <cmath>
 #include <cstdint>
@@ -54,6 +55,7 @@ int model_of(const char *method)
     if (!strcmp(method, "IMRPhenomPv2")) return GWH_IMRPHENOMPV2;
     if (!strcmp(method, "ppE_IMRPhenomD_Inspiral")) return GWH_PPE_IMRPHENOMD_INSPIRAL;
     if (!strcmp(method, "ppE_IMRPhenomD_IMR")) return GWH_PPE_IMRPHENOMD_IMR;
+    if (!strcmp(method, "gIMRPhenomD")) return GWH_GIMRPHENOMD;
     return -1;
 }
 
@@ -86,8 +88,7 @@ void *gen_params_base_py(double mass1, double mass2, double *spin1, double *spin
                          double *delta_beta, double *delta_alpha, int Nmod, double *bppe, double *betappe)
 {
     (void)theta_l, (void)phi_l, (void)theta, (void)phi, (void)cosmology, (void)LISA_alpha0, (void)LISA_phi0;
-    (void)phii, (void)sigmai, (void)betai, (void)alphai, (void)delta_phi, (void)delta_sigma, (void)delta_beta;
-    (void)delta_alpha;
+
     PyParams *q = new PyParams;
     q->magic = kMagic;
     gwh_params &p = q->p;
@@ -120,8 +121,22 @@ void *gen_params_base_py(double mass1, double mass2, double *spin1, double *spin
         q->unsupported = "more than " + std::to_string(GWH_PPE_MAX) + " ppE terms";
     else if (Nmod > 0 && (!bppe || !betappe))
         q->unsupported = "ppE terms without bppe / betappe arrays";
-    /* ppE terms are read by the ppE methods only, gIMR terms by none of the engine's methods; both are
-     * ignored by the others, as in prep_source_parameters (waveform_generator.cpp:1296-1313) */
+    /* ppE terms are read by the ppE methods only, gIMR terms by gIMRPhenomD only; the other methods ignore
+     * them, as prep_source_parameters does (waveform_generator.cpp:1296-1313).  gIMR index lists become
+     * index -> delta tables (the first entry of an index wins, as check_list_id finds it). */
+    auto table = [](int n, const int *idx, const double *delta, int lo, int hi, double *out) {
+        bool seen[16] = {false};
+        for (int j = 0; j < n && idx && delta; j++) {
+            const int i = idx[j];
+            if (i < lo || i > hi || seen[i - lo]) continue;
+            seen[i - lo] = true;
+            out[i - lo] = delta[j];
+        }
+    };
+    table(Nmod_phi, phii, delta_phi, -4, 9, p.gimr_dphi);
+    table(Nmod_sigma, sigmai, delta_sigma, 2, 4, p.gimr_dsigma);
+    table(Nmod_beta, betai, delta_beta, 2, 3, p.gimr_dbeta);
+    table(Nmod_alpha, alphai, delta_alpha, 2, 5, p.gimr_dalpha);
     if (q->unsupported.empty() && Nmod > 0) {
         p.ppe_nmod = Nmod;
         for (int i = 0; i < Nmod; i++) {

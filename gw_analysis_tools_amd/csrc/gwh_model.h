@@ -32,10 +32,11 @@
 
 /* API model ids -> the kernel family (IMRPhenomD or IMRPhenomPv2) and the ppE variant of IMRPhenomD
  * (0 none, 1 inspiral phase only, 2 all regions) */
-__host__ __device__ inline bool gwh_model_valid(int m) { return m >= GWH_IMRPHENOMD && m <= GWH_PPE_IMRPHENOMD_IMR; }
+__host__ __device__ inline bool gwh_model_valid(int m) { return m >= GWH_IMRPHENOMD && m <= GWH_GIMRPHENOMD; }
+/* modifications of IMRPhenomD: 0 none, 1 ppE inspiral, 2 ppE all regions, 3 gIMR */
 __host__ __device__ inline int gwh_ppe_variant(int m)
 {
-    return m == GWH_PPE_IMRPHENOMD_INSPIRAL ? 1 : (m == GWH_PPE_IMRPHENOMD_IMR ? 2 : 0);
+    return m == GWH_PPE_IMRPHENOMD_INSPIRAL ? 1 : (m == GWH_PPE_IMRPHENOMD_IMR ? 2 : (m == GWH_GIMRPHENOMD ? 3 : 0));
 }
 
 /* Frequency-independent constants of one instance (one parameter vector on
@@ -75,6 +76,9 @@ struct InstCoef {
      * inspiral phase (ppe_imr = 0) or to every region (ppe_imr = 1) */
     int ppe_n, ppe_imr;
     double ppe_e[GWH_PPE_MAX], ppe_c[GWH_PPE_MAX];
+    /* gIMR: inspiral phase terms K (cp[1] v^-4 + sum_i gimr_neg[i + 4] v^(i - 5), i = -4..-1), v = (pi M f)^(1/3) */
+    int gimr;
+    double gimr_neg[4];
 };
 
 /* Powers of one frequency bin that every instance needs (log f, f^(1/3), f^(-1/3), f^(-7/6), f^(3/4)).
@@ -173,7 +177,27 @@ struct SetupState {
     double delta_c[5];
     int ppe_n, ppe_imr;
     double ppe_e[GWH_PPE_MAX], ppe_c[GWH_PPE_MAX];
+    int gimr;
+    double gimr_neg[4]; /* delta_phi of PN orders -4 .. -1 */
 };
+
+/* gIMR inspiral phase terms beyond the IMRPhenomD polynomial (gIMRPhenomD.cpp:148-184): the order-1 coefficient
+ * (zero in IMRPhenomD) and 3/(128 eta) delta_i (pi M f)^((i - 5)/3) for i = -4..-1, with their f derivative */
+__device__ __forceinline__ double gwh_setup_gimr(const SetupState &S, double f, double *deriv)
+{
+    const double K = 3. / (128. * S.eta);
+    const double x = GWH_PI * S.M * f, iv = 1.0 / cbrt(x);
+    double iv2 = iv * iv, iv4 = iv2 * iv2;
+    double t = S.pp[1] * iv4, d = S.pp[1] * (-4. / 3.) * iv4;
+    double p = iv4 * iv; /* iv^5 */
+    for (int i = -1; i >= -4; i--) {
+        p *= iv; /* iv^(5 - i) */
+        t += S.gimr_neg[i + 4] * p;
+        d += S.gimr_neg[i + 4] * ((i - 5.) / 3.) * p;
+    }
+    if (deriv) *deriv = K * d / f;
+    return K * t;
+}
 
 /* ppE phase terms at f and their frequency derivative (ppE_IMRPhenomD.cpp:22-48): beta (pi Mc f)^(b/3) and
  * (b/3) beta (pi Mc f)^(b/3) / f */
@@ -207,6 +231,7 @@ __device__ __forceinline__ double gwh_setup_phase_ins(const SetupState &S, doubl
                 (1. / S.eta) * (S.sigma[1] * Mf + 0.75 * S.sigma[2] * Mf43 + 0.6 * S.sigma[3] * Mf53 +
                                 0.5 * S.sigma[4] * Mf * Mf);
     if (S.ppe_n > 0) ph += gwh_setup_ppe(S, f, nullptr);
+    if (S.gimr) ph += gwh_setup_gimr(S, f, nullptr);
     return ph;
 }
 
@@ -238,6 +263,12 @@ __device__ __forceinline__ double gwh_setup_dphase_ins(const SetupState &S, doub
         double dt;
         gwh_setup_ppe(S, f, &dt);
         r += dt;
+    }
+    if (S.gimr) { /* the order-1 coefficient is already in the polynomial derivative above */
+        double dt;
+        gwh_setup_gimr(S, f, &dt);
+        const double x = GWH_PI * S.M * f, iv = 1.0 / cbrt(x), iv4 = iv * iv * iv * iv;
+        r += dt - (3. / (128. * S.eta)) * S.pp[1] * (-4. / 3.) * iv4 / f;
     }
     return r;
 }
@@ -416,7 +447,9 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
      * GWH_PPE_MAX terms cannot be represented: the waveform below 0.2/M and the likelihood are NaN */
     S.ppe_n = 0;
     S.ppe_imr = (ppe == 2);
-    if (ppe) {
+    S.gimr = (ppe == 3);
+    for (int i = 0; i < 4; i++) S.gimr_neg[i] = S.gimr ? p.gimr_dphi[i] : 0.0;
+    if (ppe == 1 || ppe == 2) {
         const int nm = p.ppe_nmod;
         if (nm > GWH_PPE_MAX) S.A0 = __longlong_as_double(0x7ff8000000000000LL); /* NaN amplitude */
         S.ppe_n = nm < 0 ? 0 : (nm > GWH_PPE_MAX ? GWH_PPE_MAX : nm);
@@ -503,6 +536,11 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
     S.alpha[0] = 0;
     for (int i = 0; i < 5; i++) S.alpha[i + 1] = gwh_fit(fits, 14 + i, S.eta, xfit);
     double alpha1_fit = S.alpha[1];
+    if (S.gimr) { /* gIMRPhenomD::assign_lambda_param (gIMRPhenomD.cpp:56-87) */
+        for (int i = 2; i < 5; i++) S.sigma[i] *= (1. + p.gimr_dsigma[i - 2]);
+        for (int i = 2; i < 4; i++) S.beta[i] *= (1. + p.gimr_dbeta[i - 2]);
+        for (int i = 2; i < 6; i++) S.alpha[i] *= (1. + p.gimr_dalpha[i - 2]);
+    }
 
     /* remnant spin -> ringdown and damping frequencies */
     double af;
@@ -583,6 +621,15 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
                    2270. * dm * ca * GWH_PI / 3 + (2270. * GWH_PI / 3 - 520. * GWH_PI * eta) * cs -
                    6848. * (GWH_LOG_64) / 63;
         S.pp[11] = 6848. / 63.;
+        if (S.gimr) { /* gIMRPhenomD::assign_static_pn_phase_coeff (gIMRPhenomD.cpp:89-143) */
+            const int direct[5] = {0, 2, 3, 4, 7};
+            for (int j = 0; j < 5; j++) S.pp[direct[j]] *= (1. + p.gimr_dphi[direct[j] + 4]);
+            S.pp[1] = p.gimr_dphi[1 + 4];
+            S.pp[8] *= (1. + p.gimr_dphi[5 + 4]);
+            S.pp[10] *= (1. + p.gimr_dphi[6 + 4]);
+            S.pp[9] *= (1. + p.gimr_dphi[8 + 4]);
+            S.pp[11] *= (1. + p.gimr_dphi[9 + 4]);
+        }
     }
 
     /* amplitude C1 connection: Hermite quartic through (f1, v1, v1'), (f2, v2), (f3, v3, v3') */
@@ -772,6 +819,8 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
     C.inv_fdamp = 1.0 / S.fdamp;
     C.ppe_n = S.ppe_n;
     C.ppe_imr = S.ppe_imr;
+    C.gimr = S.gimr;
+    for (int i = 0; i < 4; i++) C.gimr_neg[i] = S.gimr_neg[i];
     for (int i = 0; i < GWH_PPE_MAX; i++) {
         C.ppe_e[i] = S.ppe_e[i];
         C.ppe_c[i] = S.ppe_c[i];
@@ -819,6 +868,14 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
 
 /* IMRPhenomD amplitude A(f) and phase phi(f) = Psi(f) - tc_total (f - f_ref) - phic
  * (IMRPhenomD.cpp:471-501); caller guarantees 0 < f <= fcut. */
+/* gIMR inspiral phase terms at one bin from 1/v, v = (pi M f)^(1/3) */
+__device__ __forceinline__ double gwh_gimr_bin(const InstCoef &C, double iv)
+{
+    const double iv2 = iv * iv, iv4 = iv2 * iv2, iv6 = iv4 * iv2;
+    return C.K * (C.cp[1] * iv4 + iv6 * (C.gimr_neg[3] + iv * (C.gimr_neg[2] + iv * (C.gimr_neg[1] +
+                                                                                      iv * C.gimr_neg[0]))));
+}
+
 /* ppE phase terms at one bin from its tabulated log f: sum_i beta_i (pi Mc f)^(b_i/3) */
 __device__ __forceinline__ double gwh_ppe_bin(const InstCoef &C, double lf)
 {
@@ -858,6 +915,7 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, co
         double Mf43 = Mf * u;
         psi = -GWH_PI / 4. + C.K * poly / (v2 * v2 * v) +
               (C.sig[0] * Mf + C.sig[1] * Mf43 + C.sig[2] * Mf43 * u + C.sig[3] * Mf * Mf);
+        if (C.gimr) psi += gwh_gimr_bin(C, 1.0 / v);
     } else if (f > C.f2p) {
         psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] / Mf + C.alpha[3] * (C.M34 * bp.f34) +
               C.alpha[4] * atan((f - C.alpha[5]) / C.fdamp);

@@ -53,17 +53,34 @@ class gen_params:
                  incl_angle=0., RA=0., DEC=0., psi=0., gmst=0., tc=0., phiRef=0., f_ref=0., theta_l=0., phi_l=0.,
                  theta=0., phi=0., cosmology="PLANCK15", equatorial_orientation=False, horizon_coord=False,
                  NSflag1=False, NSflag2=False, dep_postmerger=False, shift_time=True, shift_phase=True,
-                 sky_average=False, LISA_alpha0=0., LISA_phi0=0., Nmod=0, bppe=None, betappe=None):
+                 sky_average=False, LISA_alpha0=0., LISA_phi0=0., Nmod=0, bppe=None, betappe=None,
+                 phii=None, delta_phi=None, sigmai=None, delta_sigma=None, betai=None, delta_beta=None,
+                 alphai=None, delta_alpha=None):
         self._s1 = np.ascontiguousarray(spin1, dtype=np.float64)
         self._s2 = np.ascontiguousarray(spin2, dtype=np.float64)
         null_i, null_d = ctypes.cast(None, _IP), ctypes.cast(None, _DP)
+        self._g = {}
+        for key, ii, dd in (("phi", phii, delta_phi), ("sigma", sigmai, delta_sigma), ("beta", betai, delta_beta),
+                            ("alpha", alphai, delta_alpha)):
+            if ii is not None:
+                self._g[key] = (np.ascontiguousarray(ii, dtype=np.int32), np.ascontiguousarray(dd, dtype=np.float64))
+
+        def lst(key):
+            if key not in self._g:
+                return 0, null_i, null_d
+            ii, dd = self._g[key]
+            return len(ii), ii.ctypes.data_as(_IP), _dp(dd)
+        nph, ph_i, ph_d = lst("phi")
+        nsg, sg_i, sg_d = lst("sigma")
+        nbt, bt_i, bt_d = lst("beta")
+        nal, al_i, al_d = lst("alpha")
         self._b = None if bppe is None else np.ascontiguousarray(bppe, dtype=np.float64)
         self._be = None if betappe is None else np.ascontiguousarray(betappe, dtype=np.float64)
         self.handle = lib().gen_params_base_py(
             mass1, mass2, _dp(self._s1), _dp(self._s2), Luminosity_Distance, incl_angle, RA, DEC, psi, gmst, tc,
             phiRef, f_ref, theta_l, phi_l, theta, phi, cosmology.encode(), equatorial_orientation, horizon_coord,
             NSflag1, NSflag2, dep_postmerger, shift_time, shift_phase, sky_average, LISA_alpha0, LISA_phi0,
-            0, 0, 0, 0, null_i, null_i, null_i, null_i, null_d, null_d, null_d, null_d, Nmod,
+            nph, nsg, nbt, nal, ph_i, sg_i, bt_i, al_i, ph_d, sg_d, bt_d, al_d, Nmod,
             null_d if self._b is None else _dp(self._b), null_d if self._be is None else _dp(self._be))
 
     def close(self):

@@ -43,6 +43,9 @@ extern "C" {
 #define GWH_PPE_IMRPHENOMD_INSPIRAL 2
 #define GWH_PPE_IMRPHENOMD_IMR 3
 #define GWH_PPE_MAX 4 /* ppE terms per parameter set */
+/* gIMRPhenomD (src/gIMRPhenomD.cpp): IMRPhenomD with the static PN phase coefficients, the sigma / beta /
+ * alpha calibration fits and the negative-order inspiral phase terms scaled by (1 + delta) */
+#define GWH_GIMRPHENOMD 4
 
 /* arithmetic of the batched kernels */
 #define GWH_FP64 0  /* everything in FP64 */
@@ -88,6 +91,13 @@ typedef struct gwh_params {
     int32_t ppe_nmod, ppe_pad;
     double ppe_b[GWH_PPE_MAX];
     double ppe_beta[GWH_PPE_MAX];
+    /* gIMR modifications (gen_params_base: phii / delta_phi, sigmai / delta_sigma, betai / delta_beta,
+     * alphai / delta_alpha, as index -> delta tables; 0 leaves a coefficient as in IMRPhenomD), read only by
+     * GWH_GIMRPHENOMD */
+    double gimr_dphi[14];  /* PN order i = -4 .. 9 at [i + 4] (8, 9: the log terms of orders 5 and 6) */
+    double gimr_dsigma[3]; /* sigma_2 .. sigma_4 */
+    double gimr_dbeta[2];  /* beta_2, beta_3 */
+    double gimr_dalpha[4]; /* alpha_2 .. alpha_5 */
 } gwh_params;
 
 /* ---- runtime ---- */

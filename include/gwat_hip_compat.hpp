@@ -34,7 +34,21 @@ inline int model_id(const std::string &method)
     if (method == "IMRPhenomPv2") return GWH_IMRPHENOMPV2;
     if (method == "ppE_IMRPhenomD_Inspiral") return GWH_PPE_IMRPHENOMD_INSPIRAL;
     if (method == "ppE_IMRPhenomD_IMR") return GWH_PPE_IMRPHENOMD_IMR;
+    if (method == "gIMRPhenomD") return GWH_GIMRPHENOMD;
     return -1;
+}
+
+/* gIMR index lists -> the index tables of gwh_params; the first entry of an index wins, as check_list_id
+ * (util.cpp:810-817) finds it; indices outside the modified ranges are ignored, as the reference's loops do */
+inline void gimr_table(int n, const int *idx, const double *delta, int lo, int hi, double *out)
+{
+    bool seen[16] = {false};
+    for (int j = 0; j < n && idx && delta; j++) {
+        const int i = idx[j];
+        if (i < lo || i > hi || seen[i - lo]) continue;
+        seen[i - lo] = true;
+        out[i - lo] = delta[j];
+    }
 }
 
 /* "SIMPSONS" / "GAUSSLEG" (mcmc_gw.cpp:811-856) */
@@ -53,7 +67,7 @@ inline int basis_id(const std::string &method, bool sky_average)
 /* the hot-path fields of a gen_params_base<double>-shaped object for the given model; false if it asks
  * for physics the engine does not model (equatorial / horizon orientation, neutron-star flags, more ppE
  * terms than GWH_PPE_MAX).  As in prep_source_parameters (waveform_generator.cpp:1296-1313), the ppE
- * fields (Nmod, bppe, betappe) are read for the ppE models only and the gIMR fields for none. */
+ * fields (Nmod, bppe, betappe) are read for the ppE models only and the gIMR lists for gIMRPhenomD only. */
 template <class Params>
 inline bool to_params(const Params &g, gwh_params &p, int model = GWH_IMRPHENOMD)
 {
@@ -86,6 +100,12 @@ inline bool to_params(const Params &g, gwh_params &p, int model = GWH_IMRPHENOMD
             p.ppe_b[i] = g.bppe[i];
             p.ppe_beta[i] = g.betappe[i];
         }
+    }
+    if (model == GWH_GIMRPHENOMD) {
+        gimr_table(g.Nmod_phi, g.phii, g.delta_phi, -4, 9, p.gimr_dphi);
+        gimr_table(g.Nmod_sigma, g.sigmai, g.delta_sigma, 2, 4, p.gimr_dsigma);
+        gimr_table(g.Nmod_beta, g.betai, g.delta_beta, 2, 3, p.gimr_dbeta);
+        gimr_table(g.Nmod_alpha, g.alphai, g.delta_alpha, 2, 5, p.gimr_dalpha);
     }
     return !(g.equatorial_orientation || g.horizon_coord || g.NSflag1 || g.NSflag2);
 }

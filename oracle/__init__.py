@@ -16,6 +16,7 @@ IMRPHENOMD = 0
 IMRPHENOMPV2 = 1
 PPE_IMRPHENOMD_INSPIRAL = 2
 PPE_IMRPHENOMD_IMR = 3
+GIMRPHENOMD = 4
 
 DETECTORS = {"Hanford": 0, "Livingston": 1, "Virgo": 2, "Kagra": 3, "Indigo": 4,
              "CE": 5, "ET1": 6, "ET2": 7, "ET3": 8}
@@ -32,7 +33,9 @@ class Params(ctypes.Structure):
                 ("shift_time", ctypes.c_int), ("shift_phase", ctypes.c_int),
                 ("sky_average", ctypes.c_int), ("dep_postmerger", ctypes.c_int),
                 ("ppe_nmod", ctypes.c_int), ("ppe_pad", ctypes.c_int),
-                ("ppe_b", ctypes.c_double * 4), ("ppe_beta", ctypes.c_double * 4)]
+                ("ppe_b", ctypes.c_double * 4), ("ppe_beta", ctypes.c_double * 4),
+                ("gimr_dphi", ctypes.c_double * 14), ("gimr_dsigma", ctypes.c_double * 3),
+                ("gimr_dbeta", ctypes.c_double * 2), ("gimr_dalpha", ctypes.c_double * 4)]
 
 
 def make_params(**kw):
@@ -42,7 +45,7 @@ def make_params(**kw):
     p.shift_time = 1
     p.shift_phase = 1
     for k, v in kw.items():
-        if k in ("spin1", "spin2", "ppe_b", "ppe_beta"):
+        if k in ("spin1", "spin2", "ppe_b", "ppe_beta", "gimr_dphi", "gimr_dsigma", "gimr_dbeta", "gimr_dalpha"):
             getattr(p, k)[:len(v)] = list(v)
         else:
             setattr(p, k, v)
@@ -102,6 +105,7 @@ def lib():
         L.gwo_phenomd_setup_summary.argtypes = [P(Params), ctypes.c_int, dp]
         L.gwo_phenomd_pieces.argtypes = [P(Params), D, dp]
         L.gwo_pv2_setup_summary.argtypes = [P(Params), dp]
+        L.gwo_pn_phase_coefficients.argtypes = [P(Params), ctypes.c_int, dp]
         L.gwo_fcut.argtypes = [P(Params)]
         L.gwo_fcut.restype = D
         _lib = L
@@ -317,3 +321,9 @@ def hanford_o1_fitted_asd(f):
     return np.sqrt(.8464) * np.exp(avec[0] + avec[1] * x + avec[2] * x * x + avec[3] * x * x * x +
                                    avec[4] * x * x * x * x + avec[5] * x * x * x * x * x +
                                    avec[6] * x * x * x * x * x * x)
+
+
+def pn_phase_coefficients(p, model=IMRPHENOMD):
+    out = np.zeros(12)
+    lib().gwo_pn_phase_coefficients(ctypes.byref(p), model, _dp(out))
+    return out

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-enum { GWO_IMRPHENOMD = 0, GWO_IMRPHENOMPV2 = 1, GWO_PPE_IMRPHENOMD_INSPIRAL = 2, GWO_PPE_IMRPHENOMD_IMR = 3 };
+enum { GWO_IMRPHENOMD = 0, GWO_IMRPHENOMPV2 = 1, GWO_PPE_IMRPHENOMD_INSPIRAL = 2, GWO_PPE_IMRPHENOMD_IMR = 3, GWO_GIMRPHENOMD = 4 };
 
 /* Flat view of gen_params_base<double> (include/gwat/util.h:121-389): the
  * fields the hot path reads. */
@@ -41,6 +41,7 @@ typedef struct {
     int dep_postmerger;           /* closed-form ringdown fits instead of the QNM spline (util.h:197) */
     int ppe_nmod, ppe_pad;        /* ppE terms: Nmod, bppe, betappe (util.h:249-255) */
     double ppe_b[4], ppe_beta[4];
+    double gimr_dphi[14], gimr_dsigma[3], gimr_dbeta[2], gimr_dalpha[4]; /* gIMR deltas (0: unmodified) */
 } gwo_params;
 
 void gwo_default_params(gwo_params *p);
@@ -100,6 +101,7 @@ int gwo_unpack_parameters(double *vec, int *log_factors, const gwo_params *p,
 int gwo_repack_parameters(const double *vec, gwo_params *out, const char *method, int dim,
                           const gwo_params *orig);
 int gwo_lu_invert(const double *in, double *out, int dim);
+int gwo_pn_phase_coefficients(const gwo_params *p, int model, double *out);
 
 /* --- internals exposed for pinning tests --- */
 /* natural cubic spline through (x,y), evaluated (deriv=0) or differentiated (deriv=1) at xq */

@@ -30,6 +30,8 @@ typedef struct {
     int ppe;                      /* 0 GR, 1 ppE_IMRPhenomD_Inspiral, 2 ppE_IMRPhenomD_IMR */
     int Nmod;
     double bppe[4], betappe[4];
+    int gimr;                     /* gIMRPhenomD: the deltas below apply */
+    double dphi[14], dsigma[3], dbeta[2], dalpha[4];
 } gwo_src;
 
 /* lambda_parameters<double> (IMRPhenomD.h:16-26) */

@@ -87,6 +87,14 @@ void gwo_populate_source(gwo_src *s, const gwo_params *p, int model)
             s->betappe[i] = p->ppe_beta[i];
         }
     }
+    /* waveform_generator.cpp:1301-1313: the gIMR fields are copied for the gIMR methods only */
+    if (model == GWO_GIMRPHENOMD) {
+        s->gimr = 1;
+        memcpy(s->dphi, p->gimr_dphi, sizeof(s->dphi));
+        memcpy(s->dsigma, p->gimr_dsigma, sizeof(s->dsigma));
+        memcpy(s->dbeta, p->gimr_dbeta, sizeof(s->dbeta));
+        memcpy(s->dalpha, p->gimr_dalpha, sizeof(s->dalpha));
+    }
     if (model == GWO_IMRPHENOMPV2) {
         s->precessing = 1;
         if ((p->chip + 1) > GWO_DOUBLE_COMP_THRESH) {
@@ -121,6 +129,12 @@ static void assign_lambda(const gwo_src *s, gwo_lambda *l)
     for (int i = 0; i < 4; i++) l->sigma[i + 1] = gwo_lambda_element(s, i + 7);
     for (int i = 0; i < 3; i++) l->beta[i + 1] = gwo_lambda_element(s, i + 11);
     for (int i = 0; i < 5; i++) l->alpha[i + 1] = gwo_lambda_element(s, i + 14);
+    /* gIMRPhenomD::assign_lambda_param (gIMRPhenomD.cpp:56-87): sigma_2..4, beta_2..3, alpha_2..5 scaled */
+    if (s->gimr) {
+        for (int i = 2; i < 5; i++) l->sigma[i] *= (1. + s->dsigma[i - 2]);
+        for (int i = 2; i < 4; i++) l->beta[i] *= (1. + s->dbeta[i - 2]);
+        for (int i = 2; i < 6; i++) l->alpha[i] *= (1. + s->dalpha[i - 2]);
+    }
 }
 
 /* IMRPhenomD.cpp:874-946 */
@@ -197,6 +211,17 @@ static void pn_phase_coeff(const gwo_src *s, double *c)
             (-15737765635. / 3048192 + 2255. * pi2 / 12) * eta + 76055. * eta2 / 1728 - 127825. * eta3 / 1296 +
             2270. * dl * ca * M_PI / 3 + (2270. * M_PI / 3 - 520. * M_PI * eta) * cs - 6848. * (GWO_LOG_64) / 63;
     c[11] = 6848. / 63.;
+    /* gIMRPhenomD::assign_static_pn_phase_coeff (gIMRPhenomD.cpp:89-143): orders 0, 2, 3, 4, 7 scaled; order 1
+     * set to its delta; orders 5, 6 scale the log-free parts c[8], c[10]; "orders" 8, 9 the log parts c[9], c[11] */
+    if (s->gimr) {
+        const int direct[5] = {0, 2, 3, 4, 7};
+        for (int j = 0; j < 5; j++) c[direct[j]] *= (1. + s->dphi[direct[j] + 4]);
+        c[1] = s->dphi[1 + 4];
+        c[8] *= (1. + s->dphi[5 + 4]);
+        c[10] *= (1. + s->dphi[6 + 4]);
+        c[9] *= (1. + s->dphi[8 + 4]);
+        c[11] *= (1. + s->dphi[9 + 4]);
+    }
 }
 
 /* IMRPhenomD.cpp:1039-1060: log-dependent 2.5PN/3PN coefficients at frequency f */
@@ -374,6 +399,15 @@ static double fpeak(const gwo_src *s, const gwo_lambda *l)
 /* ppE phase terms (ppE_IMRPhenomD.cpp:25-35): sum_i betappe_i ((pi Mc f)^(1/3))^bppe_i, and their frequency
  * derivative (:40-48): sum_i bppe_i/3 f^(bppe_i/3 - 1) (Mc pi)^(bppe_i/3) betappe_i.  Added to the inspiral
  * phase by ppE_IMRPhenomD_Inspiral and to all three regions by ppE_IMRPhenomD_IMR (:54-103). */
+/* pow_int (util.cpp:1585-1597) for any sign of the exponent */
+static double pow_int_n(double b, int e)
+{
+    double prod = 1;
+    int n = e < 0 ? -e : e;
+    for (int i = 0; i < n; i++) prod = prod * b;
+    return e < 0 ? 1. / prod : prod;
+}
+
 static double ppe_phase(double f, const gwo_src *s)
 {
     double pimfcube = pow(M_PI * s->chirpmass * f, 1. / 3.);
@@ -417,6 +451,12 @@ double gwo_phase_ins(double f, gwo_src *s, double *pn, gwo_lambda *lam, gwo_pows
     double out = phase_TF2 + (1. / eta) * (sigma0 + sigma1 * M * f + (3. / 4.) * sigma2 * pw->MF4third +
                                            (3. / 5) * sigma3 * pw->MF5third + (1. / 2.) * sigma4 * pw->MFsquare);
     if (s->ppe) out += ppe_phase(f, s);
+    if (s->gimr) {
+        /* gIMRPhenomD::phase_ins (gIMRPhenomD.cpp:148-163): 3/(128 eta) delta_i (pi M f)^((i - 5)/3), i = -4..-1 */
+        double pimfcube = pow(M_PI * M * f, 1. / 3.);
+        for (int i = -4; i < 0; i++)
+            if (s->dphi[i + 4] != 0) out += 3. / (128. * eta) * s->dphi[i + 4] * pow_int_n(pimfcube, i - 5);
+    }
     return out;
 }
 
@@ -449,6 +489,14 @@ double gwo_dphase_ins(double f, gwo_src *s, double *pn, gwo_lambda *lam)
                  lam->sigma[3] * pow(M, 5. / 3.) * pow(f, 2. / 3) + lam->sigma[4] * pow(M, 2.) * f;
     double out = d + (1. / eta) * sig;
     if (s->ppe) out += ppe_dphase(f, s);
+    if (s->gimr) {
+        /* gIMRPhenomD::Dphase_ins (gIMRPhenomD.cpp:165-184) */
+        double pimcube = pow(M_PI * M, 1. / 3.);
+        for (int i = -4; i < 0; i++)
+            if (s->dphi[i + 4] != 0)
+                out += 3. / (128. * eta) * s->dphi[i + 4] * pow_int_n(pimcube, i - 5) * (i - 5.) / 3. *
+                       pow(f, ((i - 5.) / 3. - 1));
+    }
     return out;
 }
 
@@ -728,7 +776,8 @@ int gwo_fourier_waveform(const double *f, int n, double *hplus, double *hcross, 
 {
     gwo_src s;
     gwo_populate_source(&s, p, model);
-    if (model == GWO_IMRPHENOMD || model == GWO_PPE_IMRPHENOMD_INSPIRAL || model == GWO_PPE_IMRPHENOMD_IMR) {
+    if (model == GWO_IMRPHENOMD || model == GWO_PPE_IMRPHENOMD_INSPIRAL || model == GWO_PPE_IMRPHENOMD_IMR ||
+        model == GWO_GIMRPHENOMD) {
         int status = phenomd_construct_waveform(f, n, hplus, &s);
         double ci = cos(s.incl_angle);
         double kplus = .5 * (1 + ci * ci);
@@ -760,7 +809,7 @@ int gwo_fourier_waveform(const double *f, int n, double *hplus, double *hcross, 
 
 int gwo_fourier_amplitude(const double *f, int n, double *amp, int model, const gwo_params *p)
 {
-    if (model == GWO_IMRPHENOMPV2 || model < 0 || model > GWO_PPE_IMRPHENOMD_IMR) return 0;
+    if (model == GWO_IMRPHENOMPV2 || model < 0 || model > GWO_GIMRPHENOMD) return 0;
     gwo_src s;
     gwo_populate_source(&s, p, model);
     return phenomd_construct_amplitude(f, n, amp, &s);
@@ -769,7 +818,7 @@ int gwo_fourier_amplitude(const double *f, int n, double *amp, int model, const 
 int gwo_fourier_phase(const double *f, int n, double *phase_plus, double *phase_cross, int model,
                       const gwo_params *p)
 {
-    if (model == GWO_IMRPHENOMPV2 || model < 0 || model > GWO_PPE_IMRPHENOMD_IMR) return 0;
+    if (model == GWO_IMRPHENOMPV2 || model < 0 || model > GWO_GIMRPHENOMD) return 0;
     gwo_src s;
     gwo_populate_source(&s, p, model);
     int status = phenomd_construct_phase(f, n, phase_plus, &s);
@@ -815,4 +864,14 @@ int gwo_phenomd_pieces(const gwo_params *p, double f, double *out)
     out[9] = gwo_dphase_int(f, &s, &st.lam);
     out[10] = gwo_dphase_mr(f, &s, &st.lam);
     return 11;
+}
+
+/* static TaylorF2 phase coefficients of a parameter set (IMRPhenomD.cpp:991-1032 and, for the gIMR model,
+ * their modification), for pinning tests */
+int gwo_pn_phase_coefficients(const gwo_params *p, int model, double *out)
+{
+    gwo_src s;
+    gwo_populate_source(&s, p, model);
+    pn_phase_coeff(&s, out);
+    return 12;
 }

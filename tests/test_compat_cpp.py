@@ -25,6 +25,8 @@ struct params_like {
          equatorial_orientation = false, horizon_coord = false, NSflag1 = false, NSflag2 = false;
     int Nmod = 0, Nmod_phi = 0, Nmod_sigma = 0, Nmod_beta = 0, Nmod_alpha = 0;
     double *bppe = nullptr, *betappe = nullptr;
+    int *phii = nullptr, *sigmai = nullptr, *betai = nullptr, *alphai = nullptr;
+    double *delta_phi = nullptr, *delta_sigma = nullptr, *delta_beta = nullptr, *delta_alpha = nullptr;
 };
 
 int main()
@@ -50,6 +52,23 @@ int main()
     g.Nmod = GWH_PPE_MAX + 1;
     printf("ppemax %d\n", (int)gwh::to_params(g, p, GWH_PPE_IMRPHENOMD_INSPIRAL));
     g.Nmod = 0;
+    /* gIMR lists -> tables: first entry of an index wins, out-of-range indices ignored */
+    int ph[4] = {2, -3, 2, 12};
+    double dph[4] = {0.1, 0.2, 0.3, 0.4};
+    int al[1] = {5};
+    double dal[1] = {-0.5};
+    g.Nmod_phi = 4;
+    g.phii = ph;
+    g.delta_phi = dph;
+    g.Nmod_alpha = 1;
+    g.alphai = al;
+    g.delta_alpha = dal;
+    gwh::to_params(g, p, GWH_GIMRPHENOMD);
+    printf("gimr %d %g %g %g %g\n", gwh::model_id("gIMRPhenomD"), p.gimr_dphi[2 + 4], p.gimr_dphi[-3 + 4],
+           p.gimr_dalpha[5 - 2], p.gimr_dphi[0]);
+    gwh::to_params(g, p, GWH_IMRPHENOMD);
+    printf("gimrgr %g\n", p.gimr_dphi[2 + 4]);
+    g.Nmod_phi = g.Nmod_alpha = 0;
     printf("bases %d %d %d %d %d\n", gwh::basis_id("IMRPhenomD", true), gwh::basis_id("IMRPhenomD", false),
            gwh::basis_id("MCMC_IMRPhenomD", false), gwh::basis_id("IMRPhenomPv2", false),
            gwh::basis_id("MCMC_IMRPhenomPv2", false));
@@ -82,7 +101,9 @@ def test_compat_header_with_reference_shaped_params(tmp_path):
     assert out[2] == "models 0 1 -1 %d %d" % (G.PPE_IMRPHENOMD_INSPIRAL, G.PPE_IMRPHENOMD_IMR)
     assert out[3].split() == ["ppe", "1", "0", "1", "2", "-7", "-0.002"]
     assert out[4] == "ppemax 0"
-    out = out[:3] + out[5:]
+    assert out[5].split() == ["gimr", str(G.GIMRPHENOMD), "0.1", "0.2", "-0.5", "0"]
+    assert out[6] == "gimrgr 0"
+    out = out[:3] + out[7:]
     assert out[3] == "bases %d %d %d %d %d" % (G.BASIS_PHENOMD_SKYAVG, G.BASIS_PHENOMD_FULL, G.BASIS_PHENOMD_MCMC,
                                              G.BASIS_PV2_FULL, G.BASIS_PV2_MCMC)
     r = out[4].split()

@@ -1,4 +1,5 @@
-"""ppE phase terms on IMRPhenomD (ppE_IMRPhenomD_Inspiral / ppE_IMRPhenomD_IMR, src/ppE_IMRPhenomD.cpp) on the GPU
+"""ppE phase terms on IMRPhenomD (ppE_IMRPhenomD_Inspiral / ppE_IMRPhenomD_IMR, src/ppE_IMRPhenomD.cpp) and the gIMR
+coefficient modifications (gIMRPhenomD, src/gIMRPhenomD.cpp) on the GPU
 against the CPU oracle: waveforms <= 1e-10 relative per bin, network log-likelihood <= 1e-9 relative (FP64),
 mixed within 1e-3 of FP64, zero coefficients equal to IMRPhenomD bit for bit, and the refusals (more than
 GWH_PPE_MAX terms, ppE Fisher bases)."""
@@ -126,5 +127,62 @@ def test_ppe_refusals_and_gwatpy():
     assert st == G.GWH_OK
     o = O.make_params(**CASES[0], **EXTR, **TERMS[1])
     op, oc = O.fourier_waveform(f, o, O.PPE_IMRPHENOMD_IMR)
+    wf_close(hp, op)
+    p.close()
+
+
+# ---- gIMR (gIMRPhenomD.cpp) --------------------------------------------------------------------------------
+GIMR = [dict(gimr_dphi=[2e-4, 0, -1e-4, 0, 0, 0.01, 0.05, -0.03, 0.02, 0.1, -0.05, 0.04, 0.2, -0.1]),
+        dict(gimr_dsigma=[0.1, -0.2, 0.3], gimr_dbeta=[0.05, -0.1], gimr_dalpha=[0.1, 0.2, -0.1, 0.05]),
+        dict(gimr_dphi=[0] * 4 + [0, 0, 0.03, 0, 0, 0, 0, 0, 0, 0], gimr_dalpha=[0, 0, 0, 0.3])]
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("mods", GIMR)
+def test_gimr_waveform_matches_oracle(case, mods):
+    f = W.uniform_grid(20., 1024., 0.25)
+    g, o = both(**case, **EXTR, **mods)
+    hp, hc = G.fourier_waveform(f, g, G.GIMRPHENOMD)
+    op, oc = O.fourier_waveform(f, o, O.GIMRPHENOMD)
+    wf_close(hp, op)
+    wf_close(hc, oc)
+    gp, _ = G.fourier_waveform(f, g, G.IMRPHENOMD)
+    assert not np.array_equal(gp, hp)
+    # IMRPhenomD ignores the gIMR fields
+    gr, _ = G.fourier_waveform(f, G.make_params(**case, **EXTR), G.IMRPHENOMD)
+    assert np.array_equal(gr, gp)
+
+
+def test_gimr_network_loglike_batch():
+    f, psd, dets, data = _net()
+    ev = G.Event(f, data, [psd] * 3, dets)
+    P_ = W.perturbed_params(W.injection_params(1)[0], 24, step=5)
+    rng = np.random.default_rng(12)
+    P_["gimr_dphi"][:, 6] = rng.normal(0, 0.02, len(P_))
+    P_["gimr_dphi"][:, 2] = rng.normal(0, 1e-4, len(P_))
+    P_["gimr_dbeta"][:, 0] = rng.normal(0, 0.05, len(P_))
+    P_["gimr_dalpha"][:, 3] = rng.normal(0, 0.05, len(P_))
+    ll = G.loglike_batch(ev, P_, G.GIMRPHENOMD, G.FP64)
+    for i in range(0, len(P_), 4):
+        rec = P_[i]
+        kw = {k: (list(rec[k]) if rec[k].shape else rec[k].item()) for k in rec.dtype.names}
+        ref = O.network_loglike(f, data, [psd] * 3, dets, O.make_params(**kw), O.GIMRPHENOMD)
+        assert abs(ll[i] - ref) <= LL_TOL * max(abs(ref), 1), (i, ll[i], ref)
+    mx = G.loglike_batch(ev, P_, G.GIMRPHENOMD, G.MIXED)
+    assert np.max(np.abs(mx - ll)) <= 1e-3
+    ev.close()
+
+
+def test_gimr_through_gwatpy_lists():
+    f = W.uniform_grid(20., 512., 0.25)
+    p = P.gen_params(mass1=36, mass2=29, spin1=(0, 0, .3), spin2=(0, 0, -.1), Luminosity_Distance=410.,
+                     incl_angle=2.9, RA=1.37, DEC=-1.26, psi=2.66, gmst=W.GMST, tc=0.01, phiRef=0.3, f_ref=20.,
+                     phii=[2, -2], delta_phi=[0.05, -1e-4], betai=[3], delta_beta=[0.1])
+    st, hp, hc = P.fourier_waveform(f, "gIMRPhenomD", p)
+    assert st == G.GWH_OK
+    dphi = [0.] * 14
+    dphi[2 + 4], dphi[-2 + 4] = 0.05, -1e-4
+    o = O.make_params(**CASES[0], **EXTR, gimr_dphi=dphi, gimr_dbeta=[0., 0.1])
+    op, oc = O.fourier_waveform(f, o, O.GIMRPHENOMD)
     wf_close(hp, op)
     p.close()

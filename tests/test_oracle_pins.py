@@ -355,3 +355,71 @@ def test_ppe_inspiral_term_is_carried_by_the_connections():
     assert _linear_residual(f, d) < 1e-8 * np.max(np.abs(pg))
     # the IMR variant's form does not fit this one
     assert _linear_residual(f, -(pp_ - pg) - T(f)) > 1e-3
+
+
+# ---- gIMR modifications (gIMRPhenomD.cpp) ----------------------------------------------------------------
+def _gimr_phase(f, kw, model, **extra):
+    p = O.make_params(**kw, **extra)
+    n = len(f)
+    ph, cross = np.zeros(n), np.zeros(n)
+    O.lib().gwo_fourier_phase(O._dp(f), n, O._dp(ph), O._dp(cross), model, O.ctypes.byref(p))
+    return ph
+
+
+def _gimr_base():
+    kw = dict(mass1=36., mass2=29., spin1=[0, 0, .3], spin2=[0, 0, -.1], DL=410., incl=.4, f_ref=20., phiRef=.3,
+              tc=0.)
+    f = np.linspace(20., 0.2 / (65. * MSOL_SEC) * 0.999, 3001)
+    return kw, f
+
+
+def test_gimr_zero_deltas_reduce_to_gr():
+    kw, f = _gimr_base()
+    a = O.fourier_waveform(f, O.make_params(**kw), O.IMRPHENOMD)
+    b = O.fourier_waveform(f, O.make_params(**kw), O.GIMRPHENOMD)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("order", [-4, -3, -2, -1, 0, 1, 2, 3, 4, 7])
+def test_gimr_phase_deltas_equal_the_ppe_mapping(order):
+    """gIMRPhenomD::ppE_gIMR_mapping (gIMRPhenomD.cpp:24-52): delta_phi of PN order i is the ppE inspiral term
+    b = i - 5, beta = (3/128) eta^(-i/5) phi_i delta (phi_i the GR coefficient, 1 for i < 0 and for i = 1,
+    whose coefficient is set to delta) -- two independent code paths of the oracle must agree"""
+    kw, f = _gimr_base()
+    delta = 0.05 if order >= 0 else 2e-4
+    dphi = [0.] * 14
+    dphi[order + 4] = delta
+    g = _gimr_phase(f, kw, O.GIMRPHENOMD, gimr_dphi=dphi)
+    eta = 36. * 29. / 65. ** 2
+    coeff = 1.0 if order < 0 or order == 1 else O.pn_phase_coefficients(O.make_params(**kw))[order]
+    beta = 3. / 128. * eta ** (-order / 5.) * coeff * delta
+    p = _gimr_phase(f, kw, O.PPE_IMRPHENOMD_INSPIRAL, ppe_nmod=1, ppe_b=[order - 5.], ppe_beta=[beta])
+    gr = _gimr_phase(f, kw, O.IMRPHENOMD)
+    assert np.max(np.abs(g - gr)) > 1e-2  # the modification is visible
+    assert np.max(np.abs(g - p)) < 1e-9 * np.max(np.abs(gr)), np.max(np.abs(g - p))
+
+
+@pytest.mark.parametrize("field,idx,region", [("gimr_dsigma", 0, "ins"), ("gimr_dsigma", 2, "ins"),
+                                               ("gimr_dbeta", 0, "int"), ("gimr_dbeta", 1, "int"),
+                                               ("gimr_dalpha", 0, "mr"), ("gimr_dalpha", 3, "mr")])
+def test_gimr_fit_deltas_change_only_their_region(field, idx, region):
+    """a calibration fit scales only its own region's phase; the C1 connections (and the time/phase
+    alignment) add a line elsewhere, so the difference from GR is linear outside that region and not in it"""
+    kw, f = _gimr_base()
+    n = {"gimr_dsigma": 3, "gimr_dbeta": 2, "gimr_dalpha": 4}[field]
+    d = [0.] * n
+    d[idx] = 0.2
+    diff = _gimr_phase(f, kw, O.GIMRPHENOMD, **{field: d}) - _gimr_phase(f, kw, O.IMRPHENOMD)
+    M = 65. * MSOL_SEC
+    summ = np.zeros(32)
+    O.lib().gwo_phenomd_setup_summary(O.ctypes.byref(O.make_params(**kw)), O.IMRPHENOMD, O._dp(summ))
+    f1p, f2p = summ[4], summ[5]
+    regions = {"ins": f < f1p, "int": (f > f1p) & (f < f2p), "mr": f > f2p}
+    for name, m in regions.items():
+        if m.sum() < 20:
+            continue
+        res = _linear_residual(f[m], diff[m])
+        if name == region:
+            assert res > 1e-6, (name, res)
+        else:
+            assert res < 1e-9 * np.max(np.abs(diff)) + 1e-10, (name, res)
