@@ -149,7 +149,6 @@ struct Engine {
     int device = 0;
     int num_cu = 256;
     hipStream_t stream = nullptr;
-    double *d_fits = nullptr;
     DetGeom *d_geo = nullptr;
     double *d_qnm = nullptr;
     QnmTable qnm;

@@ -146,8 +146,6 @@ int gwh_ensure_init()
     E.fisher.stream = E.stream;
     GWH_HIP(hipEventCreate(&E.ev0));
     GWH_HIP(hipEventCreate(&E.ev1));
-    GWH_HIP(hipMalloc(&E.d_fits, sizeof(gwh_phenomd_fit)));
-    GWH_HIP(hipMemcpy(E.d_fits, gwh_phenomd_fit, sizeof(gwh_phenomd_fit), hipMemcpyHostToDevice));
     DetGeom geo[GWH_NDET_TABLE];
     for (int d = 0; d < GWH_NDET_TABLE; d++) {
         for (int i = 0; i < 9; i++) geo[d].tensor[i] = gwh_det_tensor[d][i];
@@ -171,6 +169,8 @@ int gwh_ensure_init()
     E.qnm.c_ring = E.d_qnm + 3 * n;
     E.qnm.c_damp = E.d_qnm + 4 * n;
     E.qnm.n = n;
+    E.qnm.x0 = gwh_qnm_spin[0];
+    E.qnm.x1 = gwh_qnm_spin[n - 1];
     GWH_HIP(hipMalloc(&E.d_events, sizeof(EventDesc) * GWH_MAX_EVENTS));
     GWH_HIP(hipMemset(E.d_events, 0, sizeof(EventDesc) * GWH_MAX_EVENTS));
     E.events.assign(GWH_MAX_EVENTS, Event());
@@ -337,48 +337,302 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
 /* setup                                                                      */
 /* ------------------------------------------------------------------------- */
 
-/* One lane per instance: a long serial FP64 chain per lane, so the launch bound of 64 lets the
- * compiler keep the whole working set in registers instead of spilling it to scratch memory. */
+/* 64 instances per block, four waves (roles) per instance group: each lane computes one part of its
+ * instance's record, the parts meet in LDS (see gwh_model.h for the split).  256 threads per block leave
+ * each lane the whole register file of its SIMD, so the FP64 working set needs no scratch memory.  Every
+ * wave reaches both barriers; lanes without an instance only skip the work. */
 template <int MODEL>
-__global__ __launch_bounds__(64) void gwh_setup_kernel(const gwh_params *__restrict__ params,
-                                                       const int *__restrict__ inst_event, int event0, int n,
-                                                       const double *__restrict__ fits, QnmTable qnm,
-                                                       const DetGeom *__restrict__ geo,
-                                                       const EventDesc *__restrict__ events,
-                                                       InstCoef *__restrict__ coef, int *__restrict__ active,
-                                                       const int *__restrict__ d_count, const int *__restrict__ d_mask,
-                                                       int ppe)
+__global__ __launch_bounds__(256) void gwh_setup_kernel(const gwh_params *__restrict__ params,
+                                                        const int *__restrict__ inst_event, int event0, int n,
+                                                        QnmTable qnm,
+                                                        const DetGeom *__restrict__ geo,
+                                                        const EventDesc *__restrict__ events,
+                                                        InstCoef *__restrict__ coef, int *__restrict__ active,
+                                                        const int *__restrict__ d_count, const int *__restrict__ d_mask,
+                                                        int ppe)
 {
-    const int model = MODEL;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if ((d_count && i >= *d_count) || (d_mask && !d_mask[i])) {
-        if (active) active[i] = -1; /* not evaluated: no tiles, NaN result */
-        return;
+    __shared__ SetupShared sh[64];
+    constexpr bool pv2 = (MODEL == GWH_IMRPHENOMPV2);
+    const int lane = threadIdx.x & 63;
+    const int role = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = blockIdx.x * 64 + lane;
+    bool live = i < n;
+    if (live && ((d_count && i >= *d_count) || (d_mask && !d_mask[i]))) {
+        if (active && role == 2) active[i] = -1; /* not evaluated: no tiles, NaN result */
+        live = false;
     }
-    int ev = inst_event ? inst_event[i] : event0;
-    const EventDesc &E = events[ev];
-    gwh_params p = params[i];
-    InstCoef C;
-    gwh_setup_instance(p, model, fits, qnm, geo, E.det, E.D, C, ppe);
-    coef[i] = C;
-    if (active) {
-        /* tiles to visit: all of them, or on an ascending grid those starting at or below 0.2/M */
-        const int tile = GWH_TPB * E.bpl;
-        int nt = (E.N + tile - 1) / tile;
-        int cpt = E.bpl; /* 256-bin chunks per tile */
-        if (E.sorted) {
-            int lo = 0, hi = nt; /* first tile whose smallest frequency exceeds the cutoff */
-            while (lo < hi) {
-                int mid = (lo + hi) >> 1;
-                if (E.chunk_fmin[mid * cpt] > C.fcut)
-                    hi = mid;
-                else
-                    lo = mid + 1;
+    SetupShared &X = sh[lane];
+    InstCoef *C = coef + (live ? i : 0);
+    gwh_params p;
+    SetupState S;
+    int ev = event0;
+    double eul_a = 0, eul_e = 0; /* Pv2: Euler angles at f_ref (role 3) */
+
+    /* ---- phase 1 ---- */
+    if (live) {
+        if (inst_event) ev = inst_event[i];
+        p = params[i];
+        gwh_setup_base(p, ppe, S);
+        if (role == 0) {
+            const double chip = pv2 ? gwh_setup_chip(p, S) : -1;
+            gwh_setup_ringdown(p, pv2, chip, &gwh_phenomd_fit[0][0], qnm, S);
+            gwh_setup_amp_mr(S, S.f3, &X.v3, &X.d3);
+            SetupState &T = X.S; /* base, ringdown fits and frequencies */
+            T.m1 = S.m1;
+            T.m2 = S.m2;
+            T.M = S.M;
+            T.eta = S.eta;
+            T.chi1 = S.chi1;
+            T.chi2 = S.chi2;
+            T.chi_s = S.chi_s;
+            T.chi_a = S.chi_a;
+            T.delta = S.delta;
+            T.chi_pn = S.chi_pn;
+            T.chi_eff = S.chi_eff;
+            T.fRD = S.fRD;
+            T.fdamp = S.fdamp;
+            T.f1 = S.f1;
+            T.f3 = S.f3;
+            T.f1p = S.f1p;
+            T.f2p = S.f2p;
+            for (int k = 0; k < 3; k++) T.g[k] = S.g[k];
+            T.ppe_n = S.ppe_n;
+            T.ppe_imr = S.ppe_imr;
+            T.gimr = S.gimr;
+            for (int k = 0; k < 4; k++) T.gimr_neg[k] = S.gimr_neg[k];
+            C->M = S.M;
+            C->eta = S.eta;
+            C->fcut = .2 / S.M;
+            C->f1 = S.f1;
+            C->f3 = S.f3;
+            C->f1p = S.f1p;
+            C->f2p = S.f2p;
+            C->fRD = S.fRD;
+            C->fdamp = S.fdamp;
+            C->inv_fdamp = 1.0 / S.fdamp;
+            C->mr_k = S.g[0] * S.g[2] * S.fdamp / S.M;
+            C->mr_g2 = S.g[1] / (S.g[2] * S.fdamp);
+            C->mr_w2 = (S.g[2] * S.fdamp) * (S.g[2] * S.fdamp);
+        } else if (role == 1) {
+            gwh_setup_pn(p, S);
+            const double xfit = S.chi_pn - 1;
+            for (int k = 0; k < 3; k++) S.rho[k] = gwh_fit(&gwh_phenomd_fit[0][0], k, S.eta, xfit);
+            gwh_setup_amp_ins(S, 0.014 / S.M, &X.v1, &X.d1);
+            for (int k = 0; k < 3; k++) X.S.rho[k] = S.rho[k];
+            for (int k = 0; k < 7; k++) X.S.pa[k] = S.pa[k];
+            for (int k = 0; k < 12; k++) X.S.pp[k] = S.pp[k];
+            double pi13 = cbrt(GWH_PI), pw = 1.0;
+            for (int k = 0; k < 7; k++) {
+                C->ca[k] = S.pa[k] * pw;
+                pw *= pi13;
             }
-            nt = (C.fcut == C.fcut) ? lo : nt; /* NaN cutoff: visit everything, NaN propagates */
+            for (int k = 0; k < 3; k++) C->rho[k] = S.rho[k];
+            C->K = 3. / (128. * S.eta);
+            for (int k = 0; k < 8; k++) C->cp[k] = S.pp[k];
+            C->c8 = S.pp[8];
+            C->c9 = S.pp[9];
+            C->c10 = S.pp[10];
+            C->c11 = S.pp[11];
+            C->gimr = S.gimr;
+            for (int k = 0; k < 4; k++) C->gimr_neg[k] = S.gimr_neg[k];
+        } else if (role == 2) {
+            double zeta = 0;
+            if (pv2) {
+                const Pv2Frame F = gwh_setup_frame(p, S, gwh_setup_chip(p, S));
+                X.alpha0 = F.alpha0;
+                X.phi_aligned = F.phi_aligned;
+                zeta = F.zeta;
+                double ct = cos(F.thetaJN), st = sin(F.thetaJN);
+                C->ylm[0] = sqrt(5.0 / (64.0 * GWH_PI)) * (1.0 - ct) * (1.0 - ct);
+                C->ylm[1] = sqrt(5.0 / (16.0 * GWH_PI)) * st * (1.0 - ct);
+                C->ylm[2] = sqrt(15.0 / (32.0 * GWH_PI)) * st * st;
+                C->ylm[3] = sqrt(5.0 / (16.0 * GWH_PI)) * st * (1.0 + ct);
+                C->ylm[4] = sqrt(5.0 / (64.0 * GWH_PI)) * (1.0 + ct) * (1.0 + ct);
+            }
+            const EventDesc &E = events[ev];
+            gwh_setup_detectors(p, pv2, zeta, geo, E.det, E.D, *C);
+            if (active) {
+                /* tiles to visit: all of them, or on an ascending grid those starting at or below 0.2/M */
+                const double fcut = .2 / S.M;
+                const int tile = GWH_TPB * E.bpl;
+                int nt = (E.N + tile - 1) / tile;
+                const int cpt = E.bpl; /* 256-bin chunks per tile */
+                if (E.sorted) {
+                    /* the first tile whose smallest frequency exceeds the cutoff; up to 16 tiles the count of
+                     * tiles at or below it (same answer on an ascending grid), from loads issued together */
+                    int lo = 0;
+                    if (nt <= 0) {
+                        lo = 0;
+                    } else if (nt <= 16) {
+                        double fm[16];
+#pragma unroll
+                        for (int t = 0; t < 16; t++) fm[t] = E.chunk_fmin[(t < nt ? t : nt - 1) * cpt];
+#pragma unroll
+                        for (int t = 0; t < 16; t++) lo += (t < nt && !(fm[t] > fcut));
+                    } else {
+                        int hi = nt;
+                        while (lo < hi) {
+                            int mid = (lo + hi) >> 1;
+                            if (E.chunk_fmin[mid * cpt] > fcut)
+                                hi = mid;
+                            else
+                                lo = mid + 1;
+                        }
+                    }
+                    nt = (fcut == fcut) ? lo : nt; /* NaN cutoff: visit everything, NaN propagates */
+                }
+                active[i] = nt;
+            }
+        } else {
+            X.alpha1_fit = gwh_setup_phase_fits(p, &gwh_phenomd_fit[0][0], S);
+            {
+                SetupState &T = X.S; /* phase and intermediate-amplitude fits */
+                T.v2 = S.v2;
+                for (int k = 0; k < 5; k++) T.sigma[k] = S.sigma[k];
+                for (int k = 0; k < 4; k++) T.beta[k] = S.beta[k];
+                for (int k = 0; k < 6; k++) T.alpha[k] = S.alpha[k];
+            }
+            C->sig[0] = S.sigma[1] / S.eta;
+            C->sig[1] = 0.75 * S.sigma[2] / S.eta;
+            C->sig[2] = 0.6 * S.sigma[3] / S.eta;
+            C->sig[3] = 0.5 * S.sigma[4] / S.eta;
+            gwh_setup_mass(p, ppe, S);
+            X.S.Mc = S.Mc;
+            X.S.A0 = S.A0;
+            for (int k = 0; k < GWH_PPE_MAX; k++) {
+                X.S.ppe_e[k] = S.ppe_e[k];
+                X.S.ppe_c[k] = S.ppe_c[k];
+            }
+            const double M76 = pow(S.M, 7. / 6.);
+            C->amp_pref = pv2 ? S.A0 * M76 / (2. * sqrt(5. / (64. * GWH_PI))) / 2. : S.A0 * M76;
+            C->M13 = cbrt(S.M);
+            C->Mm13 = 1.0 / C->M13;
+            C->Mm76 = 1.0 / (S.M * sqrt(C->M13));
+            C->M34 = sqrt(sqrt(S.M * S.M * S.M));
+            C->lgM = log(S.M);
+            C->lgPiM = log(GWH_PI * S.M);
+            C->ppe_n = S.ppe_n;
+            C->ppe_imr = S.ppe_imr;
+            for (int k = 0; k < GWH_PPE_MAX; k++) {
+                C->ppe_e[k] = S.ppe_e[k];
+                C->ppe_c[k] = S.ppe_c[k];
+            }
+            if (pv2) { /* precession constants (IMRPhenomP.cpp:64-144, :1081-1211) */
+                const double chip = gwh_setup_chip(p, S);
+                const double q = S.m1 / S.m2;
+                const double chil = (1.0 + q) / q * S.chi_eff;
+                const double mm1 = q / (1 + q), mm2 = 1. / (1 + q);
+                C->SP = chip * mm1 * mm1;
+                C->SL = S.chi1 * mm1 * mm1 + S.chi2 * mm2 * mm2;
+                C->L_c1 = 1.5 + S.eta / 6.0;
+                C->L_c2 = 3.375 - (19.0 * S.eta) / 8. - S.eta * S.eta / 24.0;
+                double a_c[5], e_c[5];
+                gwh_euler_coeffs(q, chil, chip, a_c, e_c);
+                for (int k = 0; k < 5; k++) {
+                    C->a_c[k] = a_c[k];
+                    C->e_c[k] = e_c[k];
+                }
+                double vr = cbrt(GWH_PI * S.M * p.f_ref);
+                double lw = log(vr * vr * vr);
+                eul_a = gwh_euler_eval(a_c, vr, lw);
+                eul_e = gwh_euler_eval(e_c, vr, lw);
+            }
         }
-        active[i] = nt;
+    }
+    __syncthreads();
+
+    /* ---- phase 2: connection pieces with beta0 = beta1 = alpha0 = alpha1 = 0 ---- */
+    if (live) {
+        S = X.S;
+        S.beta[0] = S.beta[1] = 0;
+        S.alpha[0] = S.alpha[1] = 0;
+        if (role == 0) {
+            X.p_ins = gwh_setup_phase_ins(S, S.f1p);
+            X.l1 = log(S.M * S.f1p);
+        } else if (role == 1) {
+            X.d_ins = gwh_setup_dphase_ins(S, S.f1p);
+            X.l2 = log(S.M * S.f2p);
+        } else if (role == 2) {
+            X.q_mr = gwh_setup_phase_mr(S, S.f2p);
+            X.dq_mr = gwh_setup_dphase_mr(S, S.f2p);
+            double dl[5];
+            gwh_setup_amp_connection(S.M, S.f1, S.f3, X.v1, X.d1, S.v2, X.v3, X.d3, dl);
+            for (int k = 0; k < 5; k++) C->delta[k] = dl[k];
+        } else {
+            if (!pv2) {
+                if (p.shift_phase) {
+                    const double fr = p.f_ref;
+                    X.ref = fr < S.f1p ? gwh_setup_phase_ins(S, fr)
+                                       : (fr > S.f2p ? gwh_setup_phase_mr(S, fr) : log(S.M * fr));
+                }
+                if (p.shift_time) X.dq_mr3 = gwh_setup_dphase_mr(S, S.f3);
+            } else {
+                C->alpha_add = X.alpha0 - eul_a;
+                C->eps_add = -eul_e;
+            }
+        }
+        if (pv2 && p.shift_time)
+            for (int j = role; j < 10; j += 4) X.tq[j] = gwh_setup_phase_mr(S, gwh_setup_knot(S.fRD, j));
+    }
+    __syncthreads();
+
+    /* ---- phase 3: the phase connection (IMRPhenomD.cpp:1614-1674) and the alignment ---- */
+    if (live && role == 0) {
+        const double eta = S.eta, M = S.M;
+        S.beta[1] = (eta / M) * X.d_ins - (eta / M) * gwh_setup_dphase_int(S, S.f1p);
+        S.beta[0] = eta * X.p_ins - eta * gwh_setup_phase_int_l(S, S.f1p, X.l1);
+        S.alpha[1] = (eta / M) * gwh_setup_dphase_int(S, S.f2p) - (eta / M) * X.dq_mr;
+        S.alpha[0] = eta * gwh_setup_phase_int_l(S, S.f2p, X.l2) -
+                     eta * (X.q_mr + (1. / eta) * (S.alpha[1] * (M * S.f2p)));
+        C->beta[0] = S.beta[0] / eta;
+        C->beta[1] = S.beta[1] / eta;
+        C->beta[2] = S.beta[2] / eta;
+        C->beta[3] = S.beta[3] / (3. * eta);
+        C->alpha[0] = S.alpha[0] / eta;
+        C->alpha[1] = S.alpha[1] / eta;
+        C->alpha[2] = S.alpha[2] / eta;
+        C->alpha[3] = (4. / 3.) * S.alpha[3] / eta;
+        C->alpha[4] = S.alpha[4] / eta;
+        C->alpha[5] = S.alpha[5] * S.fRD;
+        if (!pv2) {
+            /* phase and time alignment (IMRPhenomD.cpp:441-466) */
+            double phic, fref;
+            if (p.shift_phase) {
+                fref = p.f_ref;
+                double b;
+                if (fref < S.f1p)
+                    b = X.ref;
+                else if (fref > S.f2p)
+                    b = X.ref + (1. / eta) * (S.alpha[0] + S.alpha[1] * (M * fref));
+                else
+                    b = gwh_setup_phase_int_l(S, fref, X.ref);
+                phic = 2 * p.phiRef + b;
+            } else {
+                fref = 0;
+                phic = p.phiRef;
+            }
+            double tc_shift = 0;
+            if (p.shift_time)
+                tc_shift = (X.dq_mr3 + (1. / eta) * (M * S.alpha[1])) + (-S.alpha[1] + X.alpha1_fit) * M / eta;
+            C->tc_total = GWH_2PI * p.tc + tc_shift;
+            C->phic = phic;
+            C->f_ref = fref;
+            C->t_corr_2pi = 0;
+        } else {
+            C->phic = 2 * X.phi_aligned;
+            C->tc_total = GWH_2PI * p.tc;
+            C->f_ref = p.f_ref;
+            /* t_c alignment: derivative at fRD of a 10-knot natural spline of -Psi_MR (IMRPhenomP.cpp:550-606) */
+            double t_corr = 0;
+            if (p.shift_time) {
+                double ph[10];
+#pragma unroll
+                for (int j = 0; j < 10; j++)
+                    ph[j] = -(X.tq[j] + (1. / eta) * (S.alpha[0] + S.alpha[1] * (M * gwh_setup_knot(S.fRD, j))));
+                t_corr = gwh_setup_tcorr(S.fRD, ph);
+            }
+            C->t_corr_2pi = GWH_2PI * t_corr;
+        }
     }
 }
 
@@ -387,14 +641,14 @@ static void launch_setup(int model, int n, hipStream_t st, const gwh_params *par
                          const int *d_mask = nullptr)
 {
     Engine &E = gwh_engine();
-    dim3 g((n + 63) / 64), b(64);
+    dim3 g((n + 63) / 64), b(256);
     if (model == GWH_IMRPHENOMPV2)
         hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMPV2>, g, b, 0, st, params, inst_event, event0, n,
-                           (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
+                           E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
                            coef, active, d_count, d_mask, 0);
     else
         hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMD>, g, b, 0, st, params, inst_event, event0, n,
-                           (const double *)E.d_fits, E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
+                           E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
                            coef, active, d_count, d_mask, gwh_ppe_variant(model));
 }
 

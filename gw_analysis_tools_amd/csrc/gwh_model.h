@@ -102,6 +102,7 @@ __device__ __forceinline__ BinPow gwh_bin_powers(double f)
 struct QnmTable {
     const double *spin, *fring, *fdamp, *c_ring, *c_damp;
     int n;
+    double x0, x1; /* first and last spin knot */
 };
 
 /* detector geometry in device memory */
@@ -130,6 +131,60 @@ __device__ __forceinline__ double gwh_spline_eval(const double *x, const double 
     double d = (c[lo + 1] - c[lo]) / (3.0 * h);
     double dx = xq - x[lo];
     return y[lo] + dx * (b + dx * (c[lo] + dx * d));
+}
+
+/* both QNM splines (M f_ring and M f_damp) at spin a, with the interval x[lo] <= a < x[lo + 1] (lo = n - 2 at
+ * the last knot) that a bisection finds.  On the table's near-uniform spin grid that interval lies within one
+ * knot of the uniform-spacing guess, so the knots and coefficients of the four-knot window around the guess
+ * are loaded together (one memory round trip) and the interval is picked among them; a window that misses it
+ * falls back to the walk of gwh_spline_eval.  Same arithmetic as gwh_spline_eval. */
+__device__ __forceinline__ void gwh_qnm_eval(const QnmTable &q, double a, double &fring, double &fdamp)
+{
+    const int n = q.n;
+    if (!(a >= q.x0 && a <= q.x1)) {
+        fring = fdamp = __longlong_as_double(0x7ff8000000000000LL);
+        return;
+    }
+    int g = (int)((a - q.x0) / (q.x1 - q.x0) * (n - 1));
+    g = g < 0 ? 0 : (g > n - 2 ? n - 2 : g);
+    const int b = (g - 1 < 0) ? 0 : (g - 1 > n - 4 ? n - 4 : g - 1);
+    double xs[4], yr[4], cr[4], yd[4], cd[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        xs[k] = q.spin[b + k];
+        yr[k] = q.fring[b + k];
+        cr[k] = q.c_ring[b + k];
+        yd[k] = q.fdamp[b + k];
+        cd[k] = q.c_damp[b + k];
+    }
+    if (!(xs[0] <= a && (a < xs[3] || b == n - 4))) {
+        fring = gwh_spline_eval(q.spin, q.fring, q.c_ring, n, a);
+        fdamp = gwh_spline_eval(q.spin, q.fdamp, q.c_damp, n, a);
+        return;
+    }
+    double x0 = xs[0], x1 = xs[1], r0 = yr[0], r1 = yr[1], rc0 = cr[0], rc1 = cr[1], d0 = yd[0], d1 = yd[1],
+           dc0 = cd[0], dc1 = cd[1];
+#pragma unroll
+    for (int k = 1; k < 3; k++)
+        if (xs[k] <= a) {
+            x0 = xs[k];
+            x1 = xs[k + 1];
+            r0 = yr[k];
+            r1 = yr[k + 1];
+            rc0 = cr[k];
+            rc1 = cr[k + 1];
+            d0 = yd[k];
+            d1 = yd[k + 1];
+            dc0 = cd[k];
+            dc1 = cd[k + 1];
+        }
+    const double h = x1 - x0, dx = a - x0;
+    double bb = (r1 - r0) / h - h * (rc1 + 2.0 * rc0) / 3.0;
+    double dd = (rc1 - rc0) / (3.0 * h);
+    fring = r0 + dx * (bb + dx * (rc0 + dx * dd));
+    bb = (d1 - d0) / h - h * (dc1 + 2.0 * dc0) / 3.0;
+    dd = (dc1 - dc0) / (3.0 * h);
+    fdamp = d0 + dx * (bb + dx * (dc0 + dx * dd));
 }
 
 /* 19 calibration fits (IMRPhenomD.cpp:852-870) */
@@ -169,7 +224,7 @@ __device__ __forceinline__ double gwh_erad0815(double eta, double chi1, double c
 
 /* Everything the setup needs while building the record */
 struct SetupState {
-    double m1, m2, M, eta, chi1, chi2, chi_s, chi_a, delta, chi_pn, Mc, A0;
+    double m1, m2, M, eta, chi1, chi2, chi_s, chi_a, delta, chi_pn, chi_eff, Mc, A0;
     double fRD, fdamp, f1, f3, f1p, f2p;
     double rho[3], v2, g[3], sigma[5], beta[4], alpha[6];
     double pa[7];  /* amplitude PN coefficients (reference numbering) */
@@ -205,7 +260,10 @@ __device__ __forceinline__ double gwh_setup_ppe(const SetupState &S, double f, d
 {
     double t = 0, d = 0;
     const double lf = log(f);
-    for (int i = 0; i < S.ppe_n; i++) {
+    /* a fixed trip count with a guard keeps every index a compile-time constant (registers, no stack) */
+#pragma unroll
+    for (int i = 0; i < GWH_PPE_MAX; i++) {
+        if (i >= S.ppe_n) break;
         const double v = S.ppe_c[i] * exp(S.ppe_e[i] * lf);
         t += v;
         d += S.ppe_e[i] * v;
@@ -282,11 +340,17 @@ __device__ __forceinline__ double gwh_setup_ppe_imr(const SetupState &S, double 
     return deriv ? d : t;
 }
 
-__device__ __forceinline__ double gwh_setup_phase_int(const SetupState &S, double f)
+/* intermediate phase at f, given lMf = ln(M f) */
+__device__ __forceinline__ double gwh_setup_phase_int_l(const SetupState &S, double f, double lMf)
 {
     double Mf = S.M * f;
-    return (1. / S.eta) * (S.beta[0] + S.beta[1] * Mf + S.beta[2] * log(Mf) - S.beta[3] / 3. / (Mf * Mf * Mf)) +
+    return (1. / S.eta) * (S.beta[0] + S.beta[1] * Mf + S.beta[2] * lMf - S.beta[3] / 3. / (Mf * Mf * Mf)) +
            gwh_setup_ppe_imr(S, f, false);
+}
+
+__device__ __forceinline__ double gwh_setup_phase_int(const SetupState &S, double f)
+{
+    return gwh_setup_phase_int_l(S, f, log(S.M * f));
 }
 
 __device__ __forceinline__ double gwh_setup_dphase_int(const SetupState &S, double f)
@@ -419,130 +483,112 @@ __device__ __forceinline__ double gwh_euler_eval(const double *c, double v, doub
 }
 
 /* ------------------------------------------------------------------------- */
-/* the setup proper                                                          */
+/* the setup proper, in parts that run on four waves                         */
 /* ------------------------------------------------------------------------- */
+/*
+ * The frequency-independent constants of one instance are a long chain of FP64 work, and a wave issues it
+ * at the same speed whatever the number of its active lanes.  The setup kernel therefore gives every
+ * instance one lane in each of four waves (roles) that compute independent parts and exchange them through
+ * one LDS record per instance (SetupShared):
+ *
+ *   phase 1  role 0  calibration fits, remnant spin, ringdown / damping and transition frequencies, the
+ *                    merger-ringdown amplitude and its derivative at f3
+ *            role 1  PN coefficients, the inspiral amplitude and its derivative at f1
+ *            role 2  IMRPhenomPv2 frame (thetaJN, alpha0, zeta), detector projections, tiles to visit
+ *            role 3  chirp mass, amplitude scale, ppE coefficients, powers of M; Pv2 precession constants
+ *   phase 2  role 0  inspiral phase at f1p                 role 1  its derivative at f1p
+ *            role 2  merger-ringdown phase and derivative at f2p with alpha0 = alpha1 = 0, ln(M f1p),
+ *                    ln(M f2p), amplitude connection
+ *            role 3  phase piece at f_ref, merger-ringdown derivative at f3 (alpha1 = 0); Pv2 Euler offsets
+ *            Pv2 with shift_time: the ten knots of the t_c spline (merger-ringdown phase, alpha0 = alpha1 = 0)
+ *   phase 3  role 0  the C1 phase connection (beta0, beta1, alpha0, alpha1), phase and time alignment
+ *
+ * The merger-ringdown phase is linear in alpha0 and alpha1: its values at f2p, at f_ref and at the spline
+ * knots are evaluated before the connection fixes those two and completed afterwards (a change of the
+ * rounding only).  Every other quantity follows the reference's own sequence of operations.
+ */
 
-__device__ inline int gwh_setup_instance(const gwh_params &p, int model, const double *fits, const QnmTable &qnm,
-                                         const DetGeom *geo, const int *dets, int ndet, InstCoef &C, int ppe = 0)
-{
+/* what the four roles of one instance exchange */
+struct SetupShared {
     SetupState S;
-    /* util.cpp:996-1028 */
+    double alpha1_fit;        /* alpha1 of the fits (the time shift uses it) */
+    double v1, d1, v3, d3;    /* inspiral amplitude / derivative at f1, merger-ringdown at f3 */
+    double alpha0, phi_aligned; /* Pv2 frame */
+    double p_ins, d_ins;      /* inspiral phase and derivative at f1p */
+    double q_mr, dq_mr;       /* merger-ringdown phase and derivative at f2p, alpha0 = alpha1 = 0 */
+    double l1, l2;            /* ln(M f1p), ln(M f2p) */
+    double ref;               /* at f_ref: inspiral phase, merger-ringdown phase (alpha0 = alpha1 = 0) or ln(M f_ref) */
+    double dq_mr3;            /* merger-ringdown derivative at f3, alpha1 = 0 */
+    double tq[10];            /* Pv2 t_c spline knots: merger-ringdown phase, alpha0 = alpha1 = 0 */
+};
+
+/* masses, spins and modification flags (util.cpp:996-1028); every role computes these */
+__device__ __forceinline__ void gwh_setup_base(const gwh_params &p, int ppe, SetupState &S)
+{
     S.m1 = p.mass1 * GWH_MSOL_SEC;
     S.m2 = p.mass2 * GWH_MSOL_SEC;
-    double s1x = p.spin1[0], s1y = p.spin1[1], s2x = p.spin2[0], s2y = p.spin2[1];
     S.chi1 = p.spin1[2];
     S.chi2 = p.spin2[2];
     S.M = S.m1 + S.m2;
     S.eta = (S.m1 * S.m2) / (S.M * S.M);
-    S.Mc = pow(S.m1 * S.m2, 0.6) / pow(S.M, 0.2);
     S.chi_s = 0.5 * (S.chi1 + S.chi2);
     S.chi_a = 0.5 * (S.chi1 - S.chi2);
     S.delta = sqrt(1. - 4. * S.eta);
-    double chi_eff = (S.m1 * S.chi1 + S.m2 * S.chi2) / S.M;
-    S.chi_pn = chi_eff - (38. * S.eta / 113.) * (2. * S.chi_s);
-    double DL = p.DL * (GWH_MPC_SI / GWH_C_SI);
-    double kA = p.sky_average ? sqrt(GWH_PI / 30.) : sqrt(GWH_PI * 40. / 192.);
-    S.A0 = kA * S.Mc * S.Mc / DL * pow(GWH_PI * S.Mc, -7. / 6.);
-    /* ppE terms (waveform_generator.cpp:1296-1300 copies them for the ppE methods only); more than
-     * GWH_PPE_MAX terms cannot be represented: the waveform below 0.2/M and the likelihood are NaN */
+    S.chi_eff = (S.m1 * S.chi1 + S.m2 * S.chi2) / S.M;
+    S.chi_pn = S.chi_eff - (38. * S.eta / 113.) * (2. * S.chi_s);
+    /* ppE terms (waveform_generator.cpp:1296-1300 copies them for the ppE methods only) */
     S.ppe_n = 0;
+    if (ppe == 1 || ppe == 2) {
+        const int nm = p.ppe_nmod;
+        S.ppe_n = nm < 0 ? 0 : (nm > GWH_PPE_MAX ? GWH_PPE_MAX : nm);
+    }
     S.ppe_imr = (ppe == 2);
     S.gimr = (ppe == 3);
     for (int i = 0; i < 4; i++) S.gimr_neg[i] = S.gimr ? p.gimr_dphi[i] : 0.0;
-    if (ppe == 1 || ppe == 2) {
-        const int nm = p.ppe_nmod;
-        if (nm > GWH_PPE_MAX) S.A0 = __longlong_as_double(0x7ff8000000000000LL); /* NaN amplitude */
-        S.ppe_n = nm < 0 ? 0 : (nm > GWH_PPE_MAX ? GWH_PPE_MAX : nm);
-        const double lpm = log(GWH_PI * S.Mc);
-        for (int i = 0; i < GWH_PPE_MAX; i++) {
-            S.ppe_e[i] = i < S.ppe_n ? p.ppe_b[i] / 3.0 : 0.0;
-            S.ppe_c[i] = i < S.ppe_n ? p.ppe_beta[i] * exp(S.ppe_e[i] * lpm) : 0.0;
-        }
-    }
+}
 
-    /* Pv2 frame (IMRPhenomP.cpp:821-938, :1081-1211) */
-    double chip = -1, chil = 0, SP = 0, SL = 0, thetaJN = 0, alpha0 = 0, zeta = 0, phi_aligned = 0;
-    bool pv2 = (model == GWH_IMRPHENOMPV2);
-    double q = S.m1 / S.m2;
-    if (pv2) {
-        bool reduced = (p.chip + 1) > 1e-10;
-        chil = (1.0 + q) / q * chi_eff;
-        double m1_2 = S.m1 * S.m1, m2_2 = S.m2 * S.m2;
-        if (reduced) {
-            chip = p.chip;
-        } else {
-            double S1p = m1_2 * sqrt(s1y * s1y + s1x * s1x), S2p = m2_2 * sqrt(s2y * s2y + s2x * s2x);
-            double A1 = 2 + (3 * S.m2) / (2 * S.m1), A2 = 2 + (3 * S.m1) / (2 * S.m2);
-            double num = (A2 * S2p > A1 * S1p) ? A2 * S2p : A1 * S1p;
-            double den = (S.m2 > S.m1) ? A2 * m2_2 : A1 * m1_2;
-            chip = num / den;
-        }
-        double mm1 = q / (1 + q), mm2 = 1. / (1 + q);
-        SP = chip * mm1 * mm1;
-        SL = S.chi1 * mm1 * mm1 + S.chi2 * mm2 * mm2;
-        double xr = cbrt(GWH_PI * S.M * p.f_ref);
-        double L0 = S.M * S.M * gwh_l2pn(S.eta, xr * xr);
-        double J0x, J0y;
-        if (reduced) {
-            J0x = m1_2 * chip * cos(p.phip);
-            J0y = m1_2 * chip * sin(p.phip);
-        } else {
-            J0x = m1_2 * s1x + m2_2 * s2x;
-            J0y = m1_2 * s1y + m2_2 * s2y;
-        }
-        double J0z = L0 + m1_2 * S.chi1 + m2_2 * S.chi2;
-        double J0 = sqrt(J0x * J0x + J0y * J0y + J0z * J0z);
-        double thetaJ = acos(J0z / J0), phiJ = atan2(J0y, J0x);
-        phi_aligned = -phiJ;
-        double incl = p.incl, phiRef = p.phiRef;
-        double Nx = sin(incl) * cos(GWH_PI / 2. - phiRef), Ny = sin(incl) * sin(GWH_PI / 2. - phiRef), Nz = cos(incl);
-        double tx = Nx, ty = Ny, tz = Nz;
-        gwh_rotz(-phiJ, tx, ty);
-        gwh_roty(-thetaJ, tx, tz);
-        double kappa = -atan2(ty, tx);
-        tx = 0.;
-        ty = 0.;
-        tz = 1.;
-        gwh_rotz(-phiJ, tx, ty);
-        gwh_roty(-thetaJ, tx, tz);
-        gwh_rotz(kappa, tx, ty);
-        alpha0 = atan2(ty, tx);
-        tx = Nx;
-        ty = Ny;
-        tz = Nz;
-        gwh_rotz(-phiJ, tx, ty);
-        gwh_roty(-thetaJ, tx, tz);
-        gwh_rotz(kappa, tx, ty);
-        double NxJ = tx, NzJ = tz;
-        thetaJN = acos(NzJ);
-        tx = -cos(incl) * sin(phiRef);
-        ty = -cos(incl) * cos(phiRef);
-        tz = sin(incl);
-        gwh_rotz(-phiJ, tx, ty);
-        gwh_roty(-thetaJ, tx, tz);
-        gwh_rotz(kappa, tx, ty);
-        zeta = atan2(tx * NzJ - tz * NxJ, -ty);
-    }
+/* Pv2 in-plane spin parameter (IMRPhenomP.cpp:1085-1110): the given chi_p, or the one of the in-plane spins */
+__device__ __forceinline__ double gwh_setup_chip(const gwh_params &p, const SetupState &S)
+{
+    if ((p.chip + 1) > 1e-10) return p.chip;
+    const double s1x = p.spin1[0], s1y = p.spin1[1], s2x = p.spin2[0], s2y = p.spin2[1];
+    const double m1_2 = S.m1 * S.m1, m2_2 = S.m2 * S.m2;
+    double S1p = m1_2 * sqrt(s1y * s1y + s1x * s1x), S2p = m2_2 * sqrt(s2y * s2y + s2x * s2x);
+    double A1 = 2 + (3 * S.m2) / (2 * S.m1), A2 = 2 + (3 * S.m1) / (2 * S.m2);
+    double num = (A2 * S2p > A1 * S1p) ? A2 * S2p : A1 * S1p;
+    double den = (S.m2 > S.m1) ? A2 * m2_2 : A1 * m1_2;
+    return num / den;
+}
 
-    /* calibration fits */
-    double xfit = S.chi_pn - 1;
-    for (int i = 0; i < 3; i++) S.rho[i] = gwh_fit(fits, i, S.eta, xfit);
+/* role 3, phase 1: the calibration fits of the phase and of the intermediate amplitude (IMRPhenomD.cpp:852-870)
+ * with the gIMR deltas (gIMRPhenomD::assign_lambda_param, gIMRPhenomD.cpp:56-87).  Returns alpha1 of the fits. */
+__device__ __forceinline__ double gwh_setup_phase_fits(const gwh_params &p, const double *fits, SetupState &S)
+{
+    const double xfit = S.chi_pn - 1;
     S.v2 = gwh_fit(fits, 3, S.eta, xfit);
-    for (int i = 0; i < 3; i++) S.g[i] = gwh_fit(fits, 4 + i, S.eta, xfit);
     S.sigma[0] = 0;
     for (int i = 0; i < 4; i++) S.sigma[i + 1] = gwh_fit(fits, 7 + i, S.eta, xfit);
     S.beta[0] = 0;
     for (int i = 0; i < 3; i++) S.beta[i + 1] = gwh_fit(fits, 11 + i, S.eta, xfit);
     S.alpha[0] = 0;
     for (int i = 0; i < 5; i++) S.alpha[i + 1] = gwh_fit(fits, 14 + i, S.eta, xfit);
-    double alpha1_fit = S.alpha[1];
-    if (S.gimr) { /* gIMRPhenomD::assign_lambda_param (gIMRPhenomD.cpp:56-87) */
+    const double alpha1_fit = S.alpha[1];
+    if (S.gimr) {
         for (int i = 2; i < 5; i++) S.sigma[i] *= (1. + p.gimr_dsigma[i - 2]);
         for (int i = 2; i < 4; i++) S.beta[i] *= (1. + p.gimr_dbeta[i - 2]);
         for (int i = 2; i < 6; i++) S.alpha[i] *= (1. + p.gimr_dalpha[i - 2]);
     }
+    return alpha1_fit;
+}
 
-    /* remnant spin -> ringdown and damping frequencies */
+/* role 0, phase 1: the merger-ringdown amplitude fits gamma1..3, remnant spin and the ringdown / damping
+ * frequencies (IMRPhenomD.cpp:1091-1179; IMRPhenomP.cpp:1232-1254 for the precessing remnant), transition
+ * frequencies (IMRPhenomD.cpp:1312) */
+__device__ __forceinline__ void gwh_setup_ringdown(const gwh_params &p, bool pv2, double chip, const double *fits,
+                                                   const QnmTable &qnm, SetupState &S)
+{
+    const double xfit = S.chi_pn - 1;
+    for (int i = 0; i < 3; i++) S.g[i] = gwh_fit(fits, 4 + i, S.eta, xfit);
     double af;
     if (pv2) {
         double Mt = S.m1 + S.m2;
@@ -574,8 +620,10 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
         S.fdamp = (1. / GWH_2PI) * MWd / (S.M * (1. - Er));
     } else {
         double Er = gwh_erad0815(S.eta, S.chi1, S.chi2);
-        S.fRD = gwh_spline_eval(qnm.spin, qnm.fring, qnm.c_ring, qnm.n, af) / (1.0 - Er) / S.M;
-        S.fdamp = gwh_spline_eval(qnm.spin, qnm.fdamp, qnm.c_damp, qnm.n, af) / (1.0 - Er) / S.M;
+        double fr, fd;
+        gwh_qnm_eval(qnm, af, fr, fd);
+        S.fRD = fr / (1.0 - Er) / S.M;
+        S.fdamp = fd / (1.0 - Er) / S.M;
     }
     S.f1p = 0.018 / S.M;
     S.f2p = S.fRD / 2.;
@@ -585,247 +633,148 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
         double v = (g2 > 1) ? S.fRD + (S.fdamp * (-1.) * g3) / g2 : S.fRD + S.fdamp * g3 * (sqrt(1 - g2 * g2) - 1) / g2;
         S.f3 = fabs(v);
     }
+}
 
-    /* PN coefficients (IMRPhenomD.cpp:951-1032) */
-    {
-        double dm = S.delta, eta = S.eta, ca = S.chi_a, cs = S.chi_s;
-        double eta2 = eta * eta, eta3 = eta2 * eta, cs2 = cs * cs, ca2 = ca * ca, pi2 = GWH_PI * GWH_PI;
-        S.pa[0] = 1.;
-        S.pa[1] = 0.;
-        S.pa[2] = (-323. / 224 + 451 * eta / 168);
-        S.pa[3] = (27. * dm * ca / 8 + (27. / 8 - 11. * eta / 6) * cs);
-        S.pa[4] = (-27312085. / 8128512 - 1975055 * eta / 338688 + 105271 * eta2 / 24192 + (-81. / 32 + 8 * eta) * ca2 -
-                   81 * dm * ca * cs / 16 + (-81. / 32 + 17 * eta / 8) * cs2);
-        S.pa[5] = (-85 * GWH_PI / 64 + 85 * GWH_PI * eta / 16 + dm * (285197. / 16128 - 1579 * eta / 4032) * ca +
-                   (285197. / 16128 - 15317 * eta / 672 - 2227 * eta2 / 1008) * cs);
-        S.pa[6] = (-177520268561. / 8583708672 + (545384828789. / 5007163392 - 205 * pi2 / 48) * eta -
-                   3248849057 * eta2 / 178827264 + 34473079 * eta3 / 6386688 +
-                   (1614569. / 64512 - 1873643. * eta / 16128 + 2167 * eta2 / 42) * ca2 +
-                   (31 * GWH_PI / 12 - 7 * GWH_PI * eta / 3) * cs +
-                   (1614569. / 64512 - 61391 * eta / 1344 + 57451 * eta2 / 4032) * cs2 +
-                   dm * ca * (31 * GWH_PI / 12 + (1614569. / 32256 - 165961 * eta / 2688) * cs));
-        for (int i = 0; i < 12; i++) S.pp[i] = 0;
-        S.pp[0] = 1.;
-        S.pp[2] = 3715. / 756 + 55. * eta / 9;
-        S.pp[3] = -16. * GWH_PI + 113. * dm * ca / 3 + (113. / 3 - 76. * eta / 3) * cs;
-        S.pp[4] = 15293365. / 508032 + 27145. * eta / 504 + 3085. * eta2 / 72 + (-405. / 8 + 200. * eta) * ca2 -
-                  (405. / 4) * dm * ca * cs + (-405. / 8 + 5. * eta / 2) * cs2;
-        S.pp[7] = 77096675. * GWH_PI / 254016 + 378515. * GWH_PI * eta / 1512 - 74045. * GWH_PI * eta2 / 756 +
-                  dm * (-25150083775. / 3048192 + 26804935. * eta / 6048 - 1985. * eta2 / 48) * ca +
-                  (-25150083775. / 3048192 + 10566655595. * eta / 762048 - 1042165. * eta2 / 3024 + 5345. * eta3 / 36) * cs;
-        S.pp[8] = (38645. * GWH_PI / 756 - 65. * GWH_PI * eta / 9 + dm * (-732985. / 2268 - 140. * eta / 9) * ca +
-                   (-732985. / 2268 + 24260. * eta / 81 + 340. * eta2 / 9) * cs);
-        S.pp[9] = S.pp[8];
-        S.pp[10] = 11583231236531. / 4694215680 - 6848. * GWH_GAMMA_E / 21 - 640. * pi2 / 3 +
-                   (-15737765635. / 3048192 + 2255. * pi2 / 12) * eta + 76055. * eta2 / 1728 - 127825. * eta3 / 1296 +
-                   2270. * dm * ca * GWH_PI / 3 + (2270. * GWH_PI / 3 - 520. * GWH_PI * eta) * cs -
-                   6848. * (GWH_LOG_64) / 63;
-        S.pp[11] = 6848. / 63.;
-        if (S.gimr) { /* gIMRPhenomD::assign_static_pn_phase_coeff (gIMRPhenomD.cpp:89-143) */
-            const int direct[5] = {0, 2, 3, 4, 7};
-            for (int j = 0; j < 5; j++) S.pp[direct[j]] *= (1. + p.gimr_dphi[direct[j] + 4]);
-            S.pp[1] = p.gimr_dphi[1 + 4];
-            S.pp[8] *= (1. + p.gimr_dphi[5 + 4]);
-            S.pp[10] *= (1. + p.gimr_dphi[6 + 4]);
-            S.pp[9] *= (1. + p.gimr_dphi[8 + 4]);
-            S.pp[11] *= (1. + p.gimr_dphi[9 + 4]);
+/* role 1, phase 1: PN coefficients (IMRPhenomD.cpp:955-1032) with the gIMR deltas
+ * (gIMRPhenomD::assign_static_pn_phase_coeff, gIMRPhenomD.cpp:89-143) */
+__device__ __forceinline__ void gwh_setup_pn(const gwh_params &p, SetupState &S)
+{
+    double dm = S.delta, eta = S.eta, ca = S.chi_a, cs = S.chi_s;
+    double eta2 = eta * eta, eta3 = eta2 * eta, cs2 = cs * cs, ca2 = ca * ca, pi2 = GWH_PI * GWH_PI;
+    S.pa[0] = 1.;
+    S.pa[1] = 0.;
+    S.pa[2] = (-323. / 224 + 451 * eta / 168);
+    S.pa[3] = (27. * dm * ca / 8 + (27. / 8 - 11. * eta / 6) * cs);
+    S.pa[4] = (-27312085. / 8128512 - 1975055 * eta / 338688 + 105271 * eta2 / 24192 + (-81. / 32 + 8 * eta) * ca2 -
+               81 * dm * ca * cs / 16 + (-81. / 32 + 17 * eta / 8) * cs2);
+    S.pa[5] = (-85 * GWH_PI / 64 + 85 * GWH_PI * eta / 16 + dm * (285197. / 16128 - 1579 * eta / 4032) * ca +
+               (285197. / 16128 - 15317 * eta / 672 - 2227 * eta2 / 1008) * cs);
+    S.pa[6] = (-177520268561. / 8583708672 + (545384828789. / 5007163392 - 205 * pi2 / 48) * eta -
+               3248849057 * eta2 / 178827264 + 34473079 * eta3 / 6386688 +
+               (1614569. / 64512 - 1873643. * eta / 16128 + 2167 * eta2 / 42) * ca2 +
+               (31 * GWH_PI / 12 - 7 * GWH_PI * eta / 3) * cs +
+               (1614569. / 64512 - 61391 * eta / 1344 + 57451 * eta2 / 4032) * cs2 +
+               dm * ca * (31 * GWH_PI / 12 + (1614569. / 32256 - 165961 * eta / 2688) * cs));
+    for (int i = 0; i < 12; i++) S.pp[i] = 0;
+    S.pp[0] = 1.;
+    S.pp[2] = 3715. / 756 + 55. * eta / 9;
+    S.pp[3] = -16. * GWH_PI + 113. * dm * ca / 3 + (113. / 3 - 76. * eta / 3) * cs;
+    S.pp[4] = 15293365. / 508032 + 27145. * eta / 504 + 3085. * eta2 / 72 + (-405. / 8 + 200. * eta) * ca2 -
+              (405. / 4) * dm * ca * cs + (-405. / 8 + 5. * eta / 2) * cs2;
+    S.pp[7] = 77096675. * GWH_PI / 254016 + 378515. * GWH_PI * eta / 1512 - 74045. * GWH_PI * eta2 / 756 +
+              dm * (-25150083775. / 3048192 + 26804935. * eta / 6048 - 1985. * eta2 / 48) * ca +
+              (-25150083775. / 3048192 + 10566655595. * eta / 762048 - 1042165. * eta2 / 3024 + 5345. * eta3 / 36) * cs;
+    S.pp[8] = (38645. * GWH_PI / 756 - 65. * GWH_PI * eta / 9 + dm * (-732985. / 2268 - 140. * eta / 9) * ca +
+               (-732985. / 2268 + 24260. * eta / 81 + 340. * eta2 / 9) * cs);
+    S.pp[9] = S.pp[8];
+    S.pp[10] = 11583231236531. / 4694215680 - 6848. * GWH_GAMMA_E / 21 - 640. * pi2 / 3 +
+               (-15737765635. / 3048192 + 2255. * pi2 / 12) * eta + 76055. * eta2 / 1728 - 127825. * eta3 / 1296 +
+               2270. * dm * ca * GWH_PI / 3 + (2270. * GWH_PI / 3 - 520. * GWH_PI * eta) * cs -
+               6848. * (GWH_LOG_64) / 63;
+    S.pp[11] = 6848. / 63.;
+    if (S.gimr) {
+        const int direct[5] = {0, 2, 3, 4, 7};
+        for (int j = 0; j < 5; j++) S.pp[direct[j]] *= (1. + p.gimr_dphi[direct[j] + 4]);
+        S.pp[1] = p.gimr_dphi[1 + 4];
+        S.pp[8] *= (1. + p.gimr_dphi[5 + 4]);
+        S.pp[10] *= (1. + p.gimr_dphi[6 + 4]);
+        S.pp[9] *= (1. + p.gimr_dphi[8 + 4]);
+        S.pp[11] *= (1. + p.gimr_dphi[9 + 4]);
+    }
+}
+
+/* role 3, phase 1: chirp mass and amplitude scale (util.cpp:996-1028), ppE coefficients beta (pi Mc)^(b/3).
+ * More than GWH_PPE_MAX ppE terms cannot be represented: the amplitude (and so the likelihood) is NaN. */
+__device__ __forceinline__ void gwh_setup_mass(const gwh_params &p, int ppe, SetupState &S)
+{
+    S.Mc = pow(S.m1 * S.m2, 0.6) / pow(S.M, 0.2);
+    double DL = p.DL * (GWH_MPC_SI / GWH_C_SI);
+    double kA = p.sky_average ? sqrt(GWH_PI / 30.) : sqrt(GWH_PI * 40. / 192.);
+    S.A0 = kA * S.Mc * S.Mc / DL * pow(GWH_PI * S.Mc, -7. / 6.);
+    for (int i = 0; i < GWH_PPE_MAX; i++) S.ppe_e[i] = S.ppe_c[i] = 0.0;
+    if (ppe == 1 || ppe == 2) {
+        if (p.ppe_nmod > GWH_PPE_MAX) S.A0 = __longlong_as_double(0x7ff8000000000000LL); /* NaN amplitude */
+        const double lpm = log(GWH_PI * S.Mc);
+        for (int i = 0; i < GWH_PPE_MAX; i++) {
+            S.ppe_e[i] = i < S.ppe_n ? p.ppe_b[i] / 3.0 : 0.0;
+            S.ppe_c[i] = i < S.ppe_n ? p.ppe_beta[i] * exp(S.ppe_e[i] * lpm) : 0.0;
         }
     }
+}
 
-    /* amplitude C1 connection: Hermite quartic through (f1, v1, v1'), (f2, v2), (f3, v3, v3') */
-    {
-        double f1 = S.f1, f3 = S.f3, f2 = (f1 + f3) / 2.;
-        double v1, d1, v3, d3;
-        gwh_setup_amp_ins(S, f1, &v1, &d1);
-        gwh_setup_amp_mr(S, f3, &v3, &d3);
-        double v2 = S.v2;
-        double q12 = (v2 - v1) / (f2 - f1), q23 = (v3 - v2) / (f3 - f2);
-        double q012 = (q12 - d1) / (f2 - f1), q123 = (q23 - q12) / (f3 - f1), q234 = (d3 - q23) / (f3 - f2);
-        double q0123 = (q123 - q012) / (f3 - f1), q1234 = (q234 - q123) / (f3 - f1);
-        double q01234 = (q1234 - q0123) / (f3 - f1);
-        double a[5] = {v1, d1, q012, q0123, q01234};
-        double z[4] = {f1, f1, f2, f3};
-        double poly[5] = {a[4], 0, 0, 0, 0};
-        for (int k = 3; k >= 0; k--) {
-            double nw[5] = {0, 0, 0, 0, 0};
-            for (int j = 0; j < 4 - k; j++) {
-                nw[j + 1] += poly[j];
-                nw[j] -= z[k] * poly[j];
-            }
-            nw[0] += a[k];
-            for (int j = 0; j < 5; j++) poly[j] = nw[j];
-        }
-        double Mk = 1.0;
-        for (int k = 0; k < 5; k++) {
-            S.delta_c[k] = poly[k] / Mk;
-            Mk *= S.M;
-        }
-    }
-
-    /* phase C1 connection (IMRPhenomD.cpp:1613-1674) */
-    S.beta[0] = 0;
-    S.beta[1] = 0;
-    S.alpha[0] = 0;
-    S.alpha[1] = 0;
-    S.beta[1] = (S.eta / S.M) * gwh_setup_dphase_ins(S, S.f1p) - (S.eta / S.M) * gwh_setup_dphase_int(S, S.f1p);
-    S.beta[0] = S.eta * gwh_setup_phase_ins(S, S.f1p) - S.eta * gwh_setup_phase_int(S, S.f1p);
-    S.alpha[1] = (S.eta / S.M) * gwh_setup_dphase_int(S, S.f2p) - (S.eta / S.M) * gwh_setup_dphase_mr(S, S.f2p);
-    S.alpha[0] = S.eta * gwh_setup_phase_int(S, S.f2p) - S.eta * gwh_setup_phase_mr(S, S.f2p);
-
-    /* ---- record ---- */
-    C.M = S.M;
-    C.eta = S.eta;
-    C.fcut = .2 / S.M;
-    C.f1 = S.f1;
-    C.f3 = S.f3;
-    C.f1p = S.f1p;
-    C.f2p = S.f2p;
-    C.fRD = S.fRD;
-    C.fdamp = S.fdamp;
-    C.M13 = cbrt(S.M);
-    C.Mm13 = 1.0 / C.M13;
-    C.Mm76 = 1.0 / (S.M * sqrt(C.M13));
-    C.M34 = sqrt(sqrt(S.M * S.M * S.M));
-    C.lgM = log(S.M);
-    C.lgPiM = log(GWH_PI * S.M);
-    {
-        double pi13 = cbrt(GWH_PI), pw = 1.0;
-        for (int i = 0; i < 7; i++) {
-            C.ca[i] = S.pa[i] * pw;
-            pw *= pi13;
-        }
-    }
-    for (int i = 0; i < 3; i++) C.rho[i] = S.rho[i];
-    for (int i = 0; i < 5; i++) C.delta[i] = S.delta_c[i];
-    C.mr_k = S.g[0] * S.g[2] * S.fdamp / S.M;
-    C.mr_g2 = S.g[1] / (S.g[2] * S.fdamp);
-    C.mr_w2 = (S.g[2] * S.fdamp) * (S.g[2] * S.fdamp);
-    C.K = 3. / (128. * S.eta);
-    for (int i = 0; i < 8; i++) C.cp[i] = S.pp[i];
-    C.c8 = S.pp[8];
-    C.c9 = S.pp[9];
-    C.c10 = S.pp[10];
-    C.c11 = S.pp[11];
-    C.sig[0] = S.sigma[1] / S.eta;
-    C.sig[1] = 0.75 * S.sigma[2] / S.eta;
-    C.sig[2] = 0.6 * S.sigma[3] / S.eta;
-    C.sig[3] = 0.5 * S.sigma[4] / S.eta;
-    C.beta[0] = S.beta[0] / S.eta;
-    C.beta[1] = S.beta[1] / S.eta;
-    C.beta[2] = S.beta[2] / S.eta;
-    C.beta[3] = S.beta[3] / (3. * S.eta);
-    C.alpha[0] = S.alpha[0] / S.eta;
-    C.alpha[1] = S.alpha[1] / S.eta;
-    C.alpha[2] = S.alpha[2] / S.eta;
-    C.alpha[3] = (4. / 3.) * S.alpha[3] / S.eta;
-    C.alpha[4] = S.alpha[4] / S.eta;
-    C.alpha[5] = S.alpha[5] * S.fRD;
-
-    if (!pv2) {
-        /* phase and time alignment (IMRPhenomD.cpp:441-466) */
-        double phic, fref;
-        if (p.shift_phase) {
-            fref = p.f_ref;
-            phic = 2 * p.phiRef + gwh_setup_build_phase(S, fref);
-        } else {
-            fref = 0;
-            phic = p.phiRef;
-        }
-        double tc_shift = 0;
-        if (p.shift_time) tc_shift = gwh_setup_dphase_mr(S, S.f3) + (-S.alpha[1] + alpha1_fit) * S.M / S.eta;
-        C.tc_total = GWH_2PI * p.tc + tc_shift;
-        C.phic = phic;
-        C.f_ref = fref;
-        C.t_corr_2pi = 0;
-        C.amp_pref = S.A0 * pow(S.M, 7. / 6.);
+/* role 2, phase 1: the Pv2 frame (IMRPhenomP.cpp:821-938, :1081-1211): thetaJN, alpha0, zeta and the
+ * aligned-spin phase offset -phiJ */
+struct Pv2Frame {
+    double thetaJN, alpha0, zeta, phi_aligned;
+};
+__device__ __forceinline__ Pv2Frame gwh_setup_frame(const gwh_params &p, const SetupState &S, double chip)
+{
+    const double s1x = p.spin1[0], s1y = p.spin1[1], s2x = p.spin2[0], s2y = p.spin2[1];
+    const bool reduced = (p.chip + 1) > 1e-10;
+    const double m1_2 = S.m1 * S.m1, m2_2 = S.m2 * S.m2;
+    double xr = cbrt(GWH_PI * S.M * p.f_ref);
+    double L0 = S.M * S.M * gwh_l2pn(S.eta, xr * xr);
+    double J0x, J0y;
+    if (reduced) {
+        J0x = m1_2 * chip * cos(p.phip);
+        J0y = m1_2 * chip * sin(p.phip);
     } else {
-        C.phic = 2 * phi_aligned;
-        C.tc_total = GWH_2PI * p.tc;
-        C.f_ref = p.f_ref;
-        C.amp_pref = S.A0 * pow(S.M, 7. / 6.) / (2. * sqrt(5. / (64. * GWH_PI))) / 2.;
-        C.SP = SP;
-        C.SL = SL;
-        C.L_c1 = 1.5 + S.eta / 6.0;
-        C.L_c2 = 3.375 - (19.0 * S.eta) / 8. - S.eta * S.eta / 24.0;
-        gwh_euler_coeffs(q, chil, chip, C.a_c, C.e_c);
-        double vr = cbrt(GWH_PI * S.M * p.f_ref);
-        double lw = log(vr * vr * vr);
-        C.alpha_add = alpha0 - gwh_euler_eval(C.a_c, vr, lw);
-        C.eps_add = -gwh_euler_eval(C.e_c, vr, lw);
-        double ct = cos(thetaJN), st = sin(thetaJN);
-        C.ylm[0] = sqrt(5.0 / (64.0 * GWH_PI)) * (1.0 - ct) * (1.0 - ct);
-        C.ylm[1] = sqrt(5.0 / (16.0 * GWH_PI)) * st * (1.0 - ct);
-        C.ylm[2] = sqrt(15.0 / (32.0 * GWH_PI)) * st * st;
-        C.ylm[3] = sqrt(5.0 / (16.0 * GWH_PI)) * st * (1.0 + ct);
-        C.ylm[4] = sqrt(5.0 / (64.0 * GWH_PI)) * (1.0 + ct) * (1.0 + ct);
-        /* t_c alignment: derivative at fRD of a 10-knot natural spline of -Psi_MR (IMRPhenomP.cpp:550-606) */
-        double t_corr = 0;
-        if (p.shift_time) {
-            const int n = 10;
-            double start = .8 * S.fRD, stop = 1.2 * S.fRD;
-            double df = (stop - start) / (n - 1);
-            if (df > 0) {
-                double fs[10], ph[10], c[10], dg[10], rh[10], of[10];
-                for (int j = 0; j < n; j++) {
-                    fs[j] = start + j * df;
-                    ph[j] = -gwh_setup_build_phase(S, fs[j]);
-                }
-                int m = n - 2;
-                for (int i = 0; i < m; i++) {
-                    double h0 = fs[i + 1] - fs[i], h1 = fs[i + 2] - fs[i + 1];
-                    dg[i] = 2.0 * (h0 + h1);
-                    of[i] = h1;
-                    rh[i] = 3.0 * ((ph[i + 2] - ph[i + 1]) / h1 - (ph[i + 1] - ph[i]) / h0);
-                }
-                for (int i = 1; i < m; i++) {
-                    double w = of[i - 1] / dg[i - 1];
-                    dg[i] -= w * of[i - 1];
-                    rh[i] -= w * rh[i - 1];
-                }
-                c[0] = 0;
-                c[n - 1] = 0;
-                c[m] = rh[m - 1] / dg[m - 1];
-                for (int i = m - 2; i >= 0; i--) c[i + 1] = (rh[i] - of[i] * c[i + 2]) / dg[i];
-                /* interval holding fRD (the bisection's answer, found by an unrolled scan so that every
-                 * array index is a compile-time constant and the arrays stay in registers) */
-                double x0 = fs[0], x1 = fs[1], y0 = ph[0], y1 = ph[1], c0 = c[0], c1 = c[1];
-#pragma unroll
-                for (int j = 1; j < n - 1; j++)
-                    if (fs[j] <= S.fRD) {
-                        x0 = fs[j];
-                        x1 = fs[j + 1];
-                        y0 = ph[j];
-                        y1 = ph[j + 1];
-                        c0 = c[j];
-                        c1 = c[j + 1];
-                    }
-                double h = x1 - x0;
-                double b = (y1 - y0) / h - h * (c1 + 2.0 * c0) / 3.0;
-                double dd = (c1 - c0) / (3.0 * h);
-                double dx = S.fRD - x0;
-                t_corr = (b + dx * (2.0 * c0 + 3.0 * dd * dx)) / GWH_2PI;
-            }
-        }
-        C.t_corr_2pi = GWH_2PI * t_corr;
+        J0x = m1_2 * s1x + m2_2 * s2x;
+        J0y = m1_2 * s1y + m2_2 * s2y;
     }
+    double J0z = L0 + m1_2 * S.chi1 + m2_2 * S.chi2;
+    double J0 = sqrt(J0x * J0x + J0y * J0y + J0z * J0z);
+    double thetaJ = acos(J0z / J0), phiJ = atan2(J0y, J0x);
+    Pv2Frame F;
+    F.phi_aligned = -phiJ;
+    double incl = p.incl, phiRef = p.phiRef;
+    double Nx = sin(incl) * cos(GWH_PI / 2. - phiRef), Ny = sin(incl) * sin(GWH_PI / 2. - phiRef), Nz = cos(incl);
+    double tx = Nx, ty = Ny, tz = Nz;
+    gwh_rotz(-phiJ, tx, ty);
+    gwh_roty(-thetaJ, tx, tz);
+    double kappa = -atan2(ty, tx);
+    tx = 0.;
+    ty = 0.;
+    tz = 1.;
+    gwh_rotz(-phiJ, tx, ty);
+    gwh_roty(-thetaJ, tx, tz);
+    gwh_rotz(kappa, tx, ty);
+    F.alpha0 = atan2(ty, tx);
+    tx = Nx;
+    ty = Ny;
+    tz = Nz;
+    gwh_rotz(-phiJ, tx, ty);
+    gwh_roty(-thetaJ, tx, tz);
+    gwh_rotz(kappa, tx, ty);
+    double NxJ = tx, NzJ = tz;
+    F.thetaJN = acos(NzJ);
+    tx = -cos(incl) * sin(phiRef);
+    ty = -cos(incl) * cos(phiRef);
+    tz = sin(incl);
+    gwh_rotz(-phiJ, tx, ty);
+    gwh_roty(-thetaJ, tx, tz);
+    gwh_rotz(kappa, tx, ty);
+    F.zeta = atan2(tx * NzJ - tz * NxJ, -ty);
+    return F;
+}
 
-    /* detector projection constants and time-of-flight phase (waveform_util.cpp:152-184) */
+/* role 2, phase 1: polarisation constants and, per detector, the projection constants and the time-of-flight
+ * phase slope (waveform_util.cpp:152-184, detector_util.cpp:768-790 and :900-1016) */
+__device__ __forceinline__ void gwh_setup_detectors(const gwh_params &p, bool pv2, double zeta, const DetGeom *geo,
+                                                    const int *dets, int ndet, InstCoef &C)
+{
+    /* the detectors' geometry is loaded before the trigonometry, which then covers the load latency */
+    DetGeom gd[GWH_MAXDET];
+#pragma unroll
+    for (int d = 0; d < GWH_MAXDET; d++)
+        if (d < ndet) gd[d] = geo[dets[d]];
     double ci = cos(p.incl);
     double kplus = .5 * (1. + ci * ci);
     double c2z = cos(2. * zeta), s2z = sin(2. * zeta);
-    const DetGeom &g0 = geo[dets[0]];
+    const DetGeom &g0 = gd[0];
     C.wf_a = pv2 ? c2z : kplus;
     C.wf_b = pv2 ? s2z : ci;
-    C.inv_fdamp = 1.0 / S.fdamp;
-    C.ppe_n = S.ppe_n;
-    C.ppe_imr = S.ppe_imr;
-    C.gimr = S.gimr;
-    for (int i = 0; i < 4; i++) C.gimr_neg[i] = S.gimr_neg[i];
-    for (int i = 0; i < GWH_PPE_MAX; i++) {
-        C.ppe_e[i] = S.ppe_e[i];
-        C.ppe_c[i] = S.ppe_c[i];
-    }
-    /* detector_util.cpp:900-1016: polarisation basis (shared by every detector) */
+    /* polarisation basis (shared by every detector) */
     const double gha = p.gmst - p.RA;
     double cg, sg, cd, sd, cp, sp;
     sincos(gha, &sg, &cg);
@@ -833,33 +782,129 @@ __device__ inline int gwh_setup_instance(const gwh_params &p, int model, const d
     sincos(p.psi, &sp, &cp);
     const double X[3] = {-cp * sg - sp * cg * sd, -cp * cg + sp * sg * sd, sp * cd};
     const double Y[3] = {sp * sg - cp * cg * sd, sp * cg + cp * sg * sd, cp * cd};
-    /* detector_util.cpp:768-790: propagation direction for dt = (r0 - rd).n / c (hour angle gmst - RA) */
+    /* propagation direction for dt = (r0 - rd).n / c (hour angle gmst - RA) */
     const double e0 = cd * cg, e1 = cd * -sg, e2 = sd;
+#pragma unroll
     for (int d = 0; d < GWH_MAXDET; d++) {
-        C.det[d][0] = C.det[d][1] = C.det[d][2] = C.det[d][3] = 0;
-        if (d >= ndet) continue;
-        const DetGeom &g = geo[dets[d]];
-        double fp = 0, fx = 0;
-        for (int i = 0; i < 3; i++) {
-            double DX = g.tensor[3 * i] * X[0] + g.tensor[3 * i + 1] * X[1] + g.tensor[3 * i + 2] * X[2];
-            double DY = g.tensor[3 * i] * Y[0] + g.tensor[3 * i + 1] * Y[1] + g.tensor[3 * i + 2] * Y[2];
-            fp += X[i] * DX - Y[i] * DY;
-            fx += X[i] * DY + Y[i] * DX;
+        double k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+        if (d < ndet) {
+            const DetGeom &g = gd[d];
+            double fp = 0, fx = 0;
+            for (int i = 0; i < 3; i++) {
+                double DX = g.tensor[3 * i] * X[0] + g.tensor[3 * i + 1] * X[1] + g.tensor[3 * i + 2] * X[2];
+                double DY = g.tensor[3 * i] * Y[0] + g.tensor[3 * i + 1] * Y[1] + g.tensor[3 * i + 2] * Y[2];
+                fp += X[i] * DX - Y[i] * DY;
+                fx += X[i] * DY + Y[i] * DX;
+            }
+            fp *= g.gfac;
+            fx *= g.gfac;
+            double dt =
+                ((g0.loc[0] - g.loc[0]) * e0 + (g0.loc[1] - g.loc[1]) * e1 + (g0.loc[2] - g.loc[2]) * e2) / GWH_C_SI;
+            if (!pv2) {
+                k0 = fp * kplus;
+                k1 = -fx * ci;
+                k2 = k0 * k0 + k1 * k1;
+            } else {
+                k0 = fp * c2z - fx * s2z;
+                k1 = fp * s2z + fx * c2z;
+            }
+            k3 = GWH_2PI * dt;
         }
-        fp *= g.gfac;
-        fx *= g.gfac;
-        double dt = ((g0.loc[0] - g.loc[0]) * e0 + (g0.loc[1] - g.loc[1]) * e1 + (g0.loc[2] - g.loc[2]) * e2) / GWH_C_SI;
-        if (!pv2) {
-            C.det[d][0] = fp * kplus;
-            C.det[d][1] = -fx * ci;
-            C.det[d][2] = C.det[d][0] * C.det[d][0] + C.det[d][1] * C.det[d][1];
-        } else {
-            C.det[d][0] = fp * c2z - fx * s2z;
-            C.det[d][1] = fp * s2z + fx * c2z;
-        }
-        C.det[d][3] = GWH_2PI * dt;
+        C.det[d][0] = k0;
+        C.det[d][1] = k1;
+        C.det[d][2] = k2;
+        C.det[d][3] = k3;
     }
-    return 1;
+}
+
+/* role 2, phase 2: amplitude C1 connection, the Hermite quartic through (f1, v1, v1'), (f2, v2), (f3, v3, v3')
+ * with f2 = (f1 + f3) / 2 (IMRPhenomD.cpp:1679-1702), as coefficients of powers of f */
+__device__ __forceinline__ void gwh_setup_amp_connection(double M, double f1, double f3, double v1, double d1,
+                                                         double v2, double v3, double d3, double *out)
+{
+    double f2 = (f1 + f3) / 2.;
+    double q12 = (v2 - v1) / (f2 - f1), q23 = (v3 - v2) / (f3 - f2);
+    double q012 = (q12 - d1) / (f2 - f1), q123 = (q23 - q12) / (f3 - f1), q234 = (d3 - q23) / (f3 - f2);
+    double q0123 = (q123 - q012) / (f3 - f1), q1234 = (q234 - q123) / (f3 - f1);
+    double q01234 = (q1234 - q0123) / (f3 - f1);
+    double a[5] = {v1, d1, q012, q0123, q01234};
+    double z[4] = {f1, f1, f2, f3};
+    double poly[5] = {a[4], 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 3; k >= 0; k--) {
+        double nw[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 4 - k; j++) {
+            nw[j + 1] += poly[j];
+            nw[j] -= z[k] * poly[j];
+        }
+        nw[0] += a[k];
+#pragma unroll
+        for (int j = 0; j < 5; j++) poly[j] = nw[j];
+    }
+    double Mk = 1.0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        out[k] = poly[k] / Mk;
+        Mk *= M;
+    }
+}
+
+/* phase 3: derivative at fRD of the natural cubic spline through the 10 knots (f_j, ph_j) on [0.8, 1.2] fRD
+ * (IMRPhenomP.cpp:550-606), divided by 2 pi; 0 when the knots collapse */
+__device__ __forceinline__ double gwh_setup_tcorr(double fRD, const double *ph)
+{
+    const int n = 10;
+    double start = .8 * fRD, stop = 1.2 * fRD;
+    double df = (stop - start) / (n - 1);
+    if (!(df > 0)) return 0;
+    double fs[10], c[10], dg[10], rh[10], of[10];
+#pragma unroll
+    for (int j = 0; j < n; j++) fs[j] = start + j * df;
+    const int m = n - 2;
+#pragma unroll
+    for (int i = 0; i < m; i++) {
+        double h0 = fs[i + 1] - fs[i], h1 = fs[i + 2] - fs[i + 1];
+        dg[i] = 2.0 * (h0 + h1);
+        of[i] = h1;
+        rh[i] = 3.0 * ((ph[i + 2] - ph[i + 1]) / h1 - (ph[i + 1] - ph[i]) / h0);
+    }
+#pragma unroll
+    for (int i = 1; i < m; i++) {
+        double w = of[i - 1] / dg[i - 1];
+        dg[i] -= w * of[i - 1];
+        rh[i] -= w * rh[i - 1];
+    }
+    c[0] = 0;
+    c[n - 1] = 0;
+    c[m] = rh[m - 1] / dg[m - 1];
+#pragma unroll
+    for (int i = m - 2; i >= 0; i--) c[i + 1] = (rh[i] - of[i] * c[i + 2]) / dg[i];
+    /* interval holding fRD (the bisection's answer, found by an unrolled scan so that every array index is a
+     * compile-time constant and the arrays stay in registers) */
+    double x0 = fs[0], x1 = fs[1], y0 = ph[0], y1 = ph[1], c0 = c[0], c1 = c[1];
+#pragma unroll
+    for (int j = 1; j < n - 1; j++)
+        if (fs[j] <= fRD) {
+            x0 = fs[j];
+            x1 = fs[j + 1];
+            y0 = ph[j];
+            y1 = ph[j + 1];
+            c0 = c[j];
+            c1 = c[j + 1];
+        }
+    double h = x1 - x0;
+    double b = (y1 - y0) / h - h * (c1 + 2.0 * c0) / 3.0;
+    double dd = (c1 - c0) / (3.0 * h);
+    double dx = fRD - x0;
+    return (b + dx * (2.0 * c0 + 3.0 * dd * dx)) / GWH_2PI;
+}
+
+/* the t_c spline's knot j: 0.8 fRD + j (0.4 fRD / 9) */
+__device__ __forceinline__ double gwh_setup_knot(double fRD, int j)
+{
+    double start = .8 * fRD, stop = 1.2 * fRD;
+    return start + j * ((stop - start) / 9);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -105,27 +105,32 @@ struct PtArgs {
     double *gll, *lll;     /* FP64 log-likelihoods of gparams, lparams */
 };
 
-__device__ __forceinline__ gwh_params pt_record(const PtArgs &a, const double *v)
+/* the record of basis vector v written straight into dst: the template's fields copied word by word
+ * (global to global, no private copy of the 424-byte record), then the sampled fields repacked over them */
+static_assert(sizeof(gwh_params) % 8 == 0, "parameter record is a whole number of 8-byte words");
+__device__ __forceinline__ void pt_record(const PtArgs &a, const double *v, gwh_params *dst)
 {
-    gwh_params p = *a.tmpl;
-    repack(v, a.basis, p);
-    return p;
+    const unsigned long long *src = (const unsigned long long *)a.tmpl;
+    unsigned long long *out = (unsigned long long *)dst;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(gwh_params) / 8); k++) out[k] = src[k];
+    repack(v, a.basis, *dst);
 }
 
 __device__ __forceinline__ double pt_T(const PtArgs &a, int i) { return a.temps[i / a.nc]; }
 
 /* records of the initial states (for their likelihoods) */
-__global__ void gwh_pt_init_kernel(PtArgs a)
+__global__ __launch_bounds__(256) void gwh_pt_init_kernel(PtArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    a.params[i] = pt_record(a, a.theta + (size_t)i * a.dim);
+    pt_record(a, a.theta + (size_t)i * a.dim, a.params + i);
     a.mask[i] = 1;
 }
 
 /* gaussian_step (mcmc_sampler_internals.cpp:364-409): one dimension chosen uniformly, moved by
  * width * N(0, 1); the prior box decides whether the likelihood is needed at all (mcmc_step :83-92) */
-__global__ void gwh_pt_propose_kernel(PtArgs a)
+__global__ __launch_bounds__(256) void gwh_pt_propose_kernel(PtArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
@@ -143,17 +148,30 @@ __global__ void gwh_pt_propose_kernel(PtArgs a)
     a.logu[i] = log(u53(B.y, B.z));
     const double *th = a.theta + (size_t)i * a.dim;
     double *pr = a.prop + (size_t)i * a.dim;
-    double v[GWH_PT_MAXDIM];
+    /* fixed trip count: every index of v is a compile-time constant, so v lives in registers */
+    double v[GWH_PT_MAXDIM], t_beta = 0;
     int ok = 1;
-    for (int d = 0; d < a.dim; d++) {
-        double x = th[d];
-        if (d == beta) x = __dadd_rn(__dmul_rn(a.width[(size_t)i * a.dim + d], z), x);
-        v[d] = x;
-        pr[d] = x;
-        ok &= (x >= a.lower[d]) & (x <= a.upper[d]);
+#pragma unroll
+    for (int d = 0; d < GWH_PT_MAXDIM; d++) {
+        v[d] = 0;
+        if (d < a.dim) {
+            double x = th[d];
+            if (d == beta) {
+                t_beta = x;
+                x = __dadd_rn(__dmul_rn(a.width[(size_t)i * a.dim + d], z), x);
+            }
+            v[d] = x;
+            pr[d] = x;
+            ok &= (x >= a.lower[d]) & (x <= a.upper[d]);
+        }
     }
     a.mask[i] = ok;
-    a.params[i] = pt_record(a, ok ? v : th);
+    if (!ok) { /* outside the prior box: the record of the current state (its likelihood is not used) */
+#pragma unroll
+        for (int d = 0; d < GWH_PT_MAXDIM; d++)
+            if (d == beta) v[d] = t_beta;
+    }
+    pt_record(a, v, a.params + i);
 }
 
 /* Metropolis-Hastings (mcmc_sampler_internals.cpp:96-142): reject iff (ll' - ll) / T < log U; a proposal
@@ -172,7 +190,7 @@ __device__ __forceinline__ bool mh_accept(double ratio, double logu)
     return ratio > -INFINITY && !(ratio < logu);
 }
 
-__global__ void gwh_pt_accept_kernel(PtArgs a)
+__global__ __launch_bounds__(256) void gwh_pt_accept_kernel(PtArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -188,14 +206,14 @@ __global__ void gwh_pt_accept_kernel(PtArgs a)
         const int slot = atomicAdd(&a.gcount[0], 2);
         a.gslot[i] = slot;
         a.gparams[slot] = a.params[i];
-        a.gparams[slot + 1] = pt_record(a, a.theta + (size_t)i * a.dim);
+        pt_record(a, a.theta + (size_t)i * a.dim, a.gparams + slot + 1);
         a.acc[i] = 2;
         return;
     }
     pt_apply(a, i, mh_accept(ratio, lu));
 }
 
-__global__ void gwh_pt_accept_guard_kernel(PtArgs a)
+__global__ __launch_bounds__(256) void gwh_pt_accept_guard_kernel(PtArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n || a.acc[i] != 2) return;
@@ -273,7 +291,7 @@ __global__ __launch_bounds__(1024) void gwh_pt_swap_kernel(PtArgs a, int last)
                 const int slot = atomicAdd(&a.gcount[1], a.nt);
                 a.lslot[c] = slot;
                 for (int k = 0; k < a.nt; k++)
-                    a.lparams[slot + k] = pt_record(a, a.theta + (size_t)(k * a.nc + c) * a.dim);
+                    pt_record(a, a.theta + (size_t)(k * a.nc + c) * a.dim, a.lparams + slot + k);
             }
         }
     }
