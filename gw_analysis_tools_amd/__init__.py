@@ -21,6 +21,9 @@ PPE_IMRPHENOMD_INSPIRAL = 2
 PPE_IMRPHENOMD_IMR = 3
 PPE_MAX = 4
 GIMRPHENOMD = 4
+PPE_IMRPHENOMPV2_INSPIRAL = 5
+PPE_IMRPHENOMPV2_IMR = 6
+GIMRPHENOMPV2 = 7
 FP64 = 0
 MIXED = 1
 SIMPSONS = 0

@@ -642,10 +642,10 @@ static void launch_setup(int model, int n, hipStream_t st, const gwh_params *par
 {
     Engine &E = gwh_engine();
     dim3 g((n + 63) / 64), b(256);
-    if (model == GWH_IMRPHENOMPV2)
+    if (gwh_is_pv2(model))
         hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMPV2>, g, b, 0, st, params, inst_event, event0, n,
                            E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
-                           coef, active, d_count, d_mask, 0);
+                           coef, active, d_count, d_mask, gwh_ppe_variant(model));
     else
         hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMD>, g, b, 0, st, params, inst_event, event0, n,
                            E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
@@ -1056,7 +1056,7 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const 
     double ll = 0;
     for (int d = 0; d < E.D; d++) {
         double hh, dh;
-        if (model != GWH_IMRPHENOMPV2) {
+        if (!gwh_is_pv2(model)) {
             double ha = s[3 * d], zr = s[3 * d + 1], zi = s[3 * d + 2];
             hh = C.det[d][2] * ha;
             dh = C.det[d][0] * zr + C.det[d][1] * zi;
@@ -1231,7 +1231,7 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
             int *cnt = s_wcount.as<int>() + sl;
             hipLaunchKernelGGL(gwh_worklist_kernel, dim3(1), dim3(1024), 0, stream, W, evp, ap, kB[sl], wp, cnt,
                                (int)ub[sl]);
-            if (model != GWH_IMRPHENOMPV2) {
+            if (!gwh_is_pv2(model)) {
                 if (precision == GWH_FP64)
                     launch_wl<GWH_IMRPHENOMD, GWH_FP64>(kB[sl], (int)ub[sl], stream, W, evp, cp, pp, ap, wp, cnt);
                 else
@@ -1249,7 +1249,7 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
         if (!tiles_b[sl]) continue;
         const long long items = (long long)n * maxtiles;
         const int bpl = kB[sl];
-        if (model != GWH_IMRPHENOMPV2) {
+        if (!gwh_is_pv2(model)) {
             if (precision == GWH_FP64)
                 launch_persist<GWH_IMRPHENOMD, GWH_FP64>(bpl, items, stream, W, evp, cp, pp, ap, X.d_count);
             else
@@ -1771,7 +1771,7 @@ int gwh_fourier_waveform(const double *frequencies, int length, double *hplus, d
     if (!hplus || !hcross) return gwh_set_error(GWH_ERR_INVALID, "null output");
     if ((rc = upload_single(frequencies, length, params, model, 0)) != GWH_OK) return rc;
     dim3 g((length + 255) / 256);
-    if (model != GWH_IMRPHENOMPV2)
+    if (!gwh_is_pv2(model))
         hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMD>, g, dim3(256), 0, E.stream, E.f.as<const double>(),
                            length, E.coef.as<const InstCoef>(), 0, E.out1.as<double2>(), E.out2.as<double2>());
     else
@@ -1797,7 +1797,7 @@ int gwh_fourier_detector_response(const double *frequencies, int length, double 
         return gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported detector (the reference exits here)");
     if ((rc = upload_single(frequencies, length, params, model, detector)) != GWH_OK) return rc;
     dim3 g((length + 255) / 256);
-    if (model != GWH_IMRPHENOMPV2)
+    if (!gwh_is_pv2(model))
         hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMD>, g, dim3(256), 0, E.stream, E.f.as<const double>(),
                            length, E.coef.as<const InstCoef>(), 1, E.out1.as<double2>(), E.out2.as<double2>());
     else
@@ -2096,7 +2096,7 @@ int gwh_create_coherent_GW_detection(const int *detectors, int num_detectors, co
         return rc;
     dim3 g((length + 255) / 256);
     for (int d = 0; d < num_detectors; d++) {
-        if (model != GWH_IMRPHENOMPV2)
+        if (!gwh_is_pv2(model))
             hipLaunchKernelGGL(gwh_waveform_kernel<GWH_IMRPHENOMD>, g, dim3(256), 0, E.stream, E.f.as<const double>(),
                                length, E.coef.as<const InstCoef>(), 2, E.out1.as<double2>(), E.out2.as<double2>(), d);
         else

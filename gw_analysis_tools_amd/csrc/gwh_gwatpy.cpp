@@ -56,6 +56,9 @@ int model_of(const char *method)
     if (!strcmp(method, "ppE_IMRPhenomD_Inspiral")) return GWH_PPE_IMRPHENOMD_INSPIRAL;
     if (!strcmp(method, "ppE_IMRPhenomD_IMR")) return GWH_PPE_IMRPHENOMD_IMR;
     if (!strcmp(method, "gIMRPhenomD")) return GWH_GIMRPHENOMD;
+    if (!strcmp(method, "ppE_IMRPhenomPv2_Inspiral")) return GWH_PPE_IMRPHENOMPV2_INSPIRAL;
+    if (!strcmp(method, "ppE_IMRPhenomPv2_IMR")) return GWH_PPE_IMRPHENOMPV2_IMR;
+    if (!strcmp(method, "gIMRPhenomPv2")) return GWH_GIMRPHENOMPV2;
     return -1;
 }
 
@@ -121,7 +124,7 @@ void *gen_params_base_py(double mass1, double mass2, double *spin1, double *spin
         q->unsupported = "more than " + std::to_string(GWH_PPE_MAX) + " ppE terms";
     else if (Nmod > 0 && (!bppe || !betappe))
         q->unsupported = "ppE terms without bppe / betappe arrays";
-    /* ppE terms are read by the ppE methods only, gIMR terms by gIMRPhenomD only; the other methods ignore
+    /* ppE terms are read by the ppE methods only, gIMR terms by the gIMR methods only; the other methods ignore
      * them, as prep_source_parameters does (waveform_generator.cpp:1296-1313).  gIMR index lists become
      * index -> delta tables (the first entry of an index wins, as check_list_id finds it). */
     auto table = [](int n, const int *idx, const double *delta, int lo, int hi, double *out) {

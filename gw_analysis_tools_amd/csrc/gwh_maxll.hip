@@ -392,7 +392,7 @@ int gwh_maximized_loglike_batch(int event, int model, const gwh_params *params, 
     if (ev.method != GWH_SIMPSONS || ev.log10F)
         return gwh_set_error(GWH_ERR_UNSUPPORTED, "the maximised likelihoods need a Simpson-rule event");
     const int N = ev.N, D = ev.D;
-    const bool pv2 = (model == GWH_IMRPHENOMPV2);
+    const bool pv2 = gwh_is_pv2(model);
     const int P = pv2 ? 2 : 1;
     const int nblk = (N + GWH_MX_TPB - 1) / GWH_MX_TPB;
     const size_t per_inst = (size_t)D * P * N * 16;

@@ -30,13 +30,21 @@
 #define GWH_LOG_64 4.15888308336 /* the reference's truncated ln 64 (IMRPhenomD.cpp:25) */
 #define GWH_SQRT_6 2.44948974278317788
 
-/* API model ids -> the kernel family (IMRPhenomD or IMRPhenomPv2) and the ppE variant of IMRPhenomD
- * (0 none, 1 inspiral phase only, 2 all regions) */
-__host__ __device__ inline bool gwh_model_valid(int m) { return m >= GWH_IMRPHENOMD && m <= GWH_GIMRPHENOMD; }
-/* modifications of IMRPhenomD: 0 none, 1 ppE inspiral, 2 ppE all regions, 3 gIMR */
+/* API model ids -> the kernel family (IMRPhenomD or IMRPhenomPv2) and the modification of its aligned-spin
+ * phase */
+__host__ __device__ inline bool gwh_model_valid(int m) { return m >= GWH_IMRPHENOMD && m <= GWH_GIMRPHENOMPV2; }
+__host__ __device__ inline bool gwh_is_pv2(int m)
+{
+    return m == GWH_IMRPHENOMPV2 || m == GWH_PPE_IMRPHENOMPV2_INSPIRAL || m == GWH_PPE_IMRPHENOMPV2_IMR ||
+           m == GWH_GIMRPHENOMPV2;
+}
+/* modification: 0 none, 1 ppE inspiral, 2 ppE all regions, 3 gIMR */
 __host__ __device__ inline int gwh_ppe_variant(int m)
 {
-    return m == GWH_PPE_IMRPHENOMD_INSPIRAL ? 1 : (m == GWH_PPE_IMRPHENOMD_IMR ? 2 : (m == GWH_GIMRPHENOMD ? 3 : 0));
+    if (m == GWH_PPE_IMRPHENOMD_INSPIRAL || m == GWH_PPE_IMRPHENOMPV2_INSPIRAL) return 1;
+    if (m == GWH_PPE_IMRPHENOMD_IMR || m == GWH_PPE_IMRPHENOMPV2_IMR) return 2;
+    if (m == GWH_GIMRPHENOMD || m == GWH_GIMRPHENOMPV2) return 3;
+    return 0;
 }
 
 /* Frequency-independent constants of one instance (one parameter vector on

@@ -46,6 +46,12 @@ extern "C" {
 /* gIMRPhenomD (src/gIMRPhenomD.cpp): IMRPhenomD with the static PN phase coefficients, the sigma / beta /
  * alpha calibration fits and the negative-order inspiral phase terms scaled by (1 + delta) */
 #define GWH_GIMRPHENOMD 4
+/* the same three modifications on IMRPhenomPv2 (src/ppE_IMRPhenomP.cpp, src/gIMRPhenomP.cpp): the aligned-spin
+ * phase of the precessing model carries the terms ("ppE_IMRPhenomPv2_Inspiral", "ppE_IMRPhenomPv2_IMR",
+ * "gIMRPhenomPv2"; dispatch src/waveform_generator.cpp:227-246) */
+#define GWH_PPE_IMRPHENOMPV2_INSPIRAL 5
+#define GWH_PPE_IMRPHENOMPV2_IMR 6
+#define GWH_GIMRPHENOMPV2 7
 
 /* arithmetic of the batched kernels */
 #define GWH_FP64 0  /* everything in FP64 */

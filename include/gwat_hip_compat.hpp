@@ -26,7 +26,8 @@
 
 namespace gwh {
 
-/* "IMRPhenomD" / "IMRPhenomPv2" / "ppE_IMRPhenomD_Inspiral" / "ppE_IMRPhenomD_IMR"
+/* "IMRPhenomD" / "IMRPhenomPv2", their ppE ("ppE_IMRPhenomD_Inspiral", "ppE_IMRPhenomD_IMR",
+ * "ppE_IMRPhenomPv2_Inspiral", "ppE_IMRPhenomPv2_IMR") and gIMR ("gIMRPhenomD", "gIMRPhenomPv2") variants
  * (waveform_generator.cpp:103-294 dispatch); -1 otherwise */
 inline int model_id(const std::string &method)
 {
@@ -35,6 +36,9 @@ inline int model_id(const std::string &method)
     if (method == "ppE_IMRPhenomD_Inspiral") return GWH_PPE_IMRPHENOMD_INSPIRAL;
     if (method == "ppE_IMRPhenomD_IMR") return GWH_PPE_IMRPHENOMD_IMR;
     if (method == "gIMRPhenomD") return GWH_GIMRPHENOMD;
+    if (method == "ppE_IMRPhenomPv2_Inspiral") return GWH_PPE_IMRPHENOMPV2_INSPIRAL;
+    if (method == "ppE_IMRPhenomPv2_IMR") return GWH_PPE_IMRPHENOMPV2_IMR;
+    if (method == "gIMRPhenomPv2") return GWH_GIMRPHENOMPV2;
     return -1;
 }
 
@@ -67,7 +71,7 @@ inline int basis_id(const std::string &method, bool sky_average)
 /* the hot-path fields of a gen_params_base<double>-shaped object for the given model; false if it asks
  * for physics the engine does not model (equatorial / horizon orientation, neutron-star flags, more ppE
  * terms than GWH_PPE_MAX).  As in prep_source_parameters (waveform_generator.cpp:1296-1313), the ppE
- * fields (Nmod, bppe, betappe) are read for the ppE models only and the gIMR lists for gIMRPhenomD only. */
+ * fields (Nmod, bppe, betappe) are read for the ppE models only and the gIMR lists for the gIMR models only. */
 template <class Params>
 inline bool to_params(const Params &g, gwh_params &p, int model = GWH_IMRPHENOMD)
 {
@@ -93,7 +97,8 @@ inline bool to_params(const Params &g, gwh_params &p, int model = GWH_IMRPHENOMD
     p.shift_phase = g.shift_phase;
     p.sky_average = g.sky_average;
     p.dep_postmerger = g.dep_postmerger;
-    if (model == GWH_PPE_IMRPHENOMD_INSPIRAL || model == GWH_PPE_IMRPHENOMD_IMR) {
+    if (model == GWH_PPE_IMRPHENOMD_INSPIRAL || model == GWH_PPE_IMRPHENOMD_IMR ||
+        model == GWH_PPE_IMRPHENOMPV2_INSPIRAL || model == GWH_PPE_IMRPHENOMPV2_IMR) {
         if (g.Nmod < 0 || g.Nmod > GWH_PPE_MAX || (g.Nmod > 0 && (!g.bppe || !g.betappe))) return false;
         p.ppe_nmod = g.Nmod;
         for (int i = 0; i < g.Nmod; i++) {
@@ -101,7 +106,7 @@ inline bool to_params(const Params &g, gwh_params &p, int model = GWH_IMRPHENOMD
             p.ppe_beta[i] = g.betappe[i];
         }
     }
-    if (model == GWH_GIMRPHENOMD) {
+    if (model == GWH_GIMRPHENOMD || model == GWH_GIMRPHENOMPV2) {
         gimr_table(g.Nmod_phi, g.phii, g.delta_phi, -4, 9, p.gimr_dphi);
         gimr_table(g.Nmod_sigma, g.sigmai, g.delta_sigma, 2, 4, p.gimr_dsigma);
         gimr_table(g.Nmod_beta, g.betai, g.delta_beta, 2, 3, p.gimr_dbeta);

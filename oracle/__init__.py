@@ -17,6 +17,10 @@ IMRPHENOMPV2 = 1
 PPE_IMRPHENOMD_INSPIRAL = 2
 PPE_IMRPHENOMD_IMR = 3
 GIMRPHENOMD = 4
+PPE_IMRPHENOMPV2_INSPIRAL = 5
+PPE_IMRPHENOMPV2_IMR = 6
+GIMRPHENOMPV2 = 7
+PV2_FAMILY = (IMRPHENOMPV2, PPE_IMRPHENOMPV2_INSPIRAL, PPE_IMRPHENOMPV2_IMR, GIMRPHENOMPV2)
 
 DETECTORS = {"Hanford": 0, "Livingston": 1, "Virgo": 2, "Kagra": 3, "Indigo": 4,
              "CE": 5, "ET1": 6, "ET2": 7, "ET3": 8}

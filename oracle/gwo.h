@@ -23,7 +23,20 @@
 extern "C" {
 #endif
 
-enum { GWO_IMRPHENOMD = 0, GWO_IMRPHENOMPV2 = 1, GWO_PPE_IMRPHENOMD_INSPIRAL = 2, GWO_PPE_IMRPHENOMD_IMR = 3, GWO_GIMRPHENOMD = 4 };
+enum {
+    GWO_IMRPHENOMD = 0,
+    GWO_IMRPHENOMPV2 = 1,
+    GWO_PPE_IMRPHENOMD_INSPIRAL = 2,
+    GWO_PPE_IMRPHENOMD_IMR = 3,
+    GWO_GIMRPHENOMD = 4,
+    GWO_PPE_IMRPHENOMPV2_INSPIRAL = 5,
+    GWO_PPE_IMRPHENOMPV2_IMR = 6,
+    GWO_GIMRPHENOMPV2 = 7
+};
+/* the precessing family (waveform_generator.cpp:227: local_method.find("IMRPhenomPv2")) */
+#define GWO_IS_PV2(m) \
+    ((m) == GWO_IMRPHENOMPV2 || (m) == GWO_PPE_IMRPHENOMPV2_INSPIRAL || (m) == GWO_PPE_IMRPHENOMPV2_IMR || \
+     (m) == GWO_GIMRPHENOMPV2)
 
 /* Flat view of gen_params_base<double> (include/gwat/util.h:121-389): the
  * fields the hot path reads. */

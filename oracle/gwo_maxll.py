@@ -13,7 +13,7 @@ The FFT is numpy's (pocketfft) forward transform, the direction of the reference
 """
 import numpy as np
 
-from . import IMRPHENOMD, IMRPHENOMPV2, PPE_IMRPHENOMD_INSPIRAL, PPE_IMRPHENOMD_IMR, fourier_waveform, make_params, \
+from . import IMRPHENOMD, IMRPHENOMPV2, PV2_FAMILY, PPE_IMRPHENOMD_INSPIRAL, PPE_IMRPHENOMD_IMR, fourier_waveform, make_params, \
     simpsons_sum
 
 
@@ -67,19 +67,19 @@ def intrinsic_params(p, model):
         else:
             setattr(q, name, v)
     q.psi, q.phiRef, q.incl, q.tc = 0., 1., 0., 1.
-    q.f_ref = 20. if model == IMRPHENOMPV2 else 10.
+    q.f_ref = 20. if model in PV2_FAMILY else 10.
     return q
 
 
 def intrinsic_network(f, data, psd, p, model):
-    """network sum of the maximised statistic: aligned (IMRPhenomD and its ppE variants, on the plus
+    """network sum of the maximised statistic: aligned (IMRPhenomD and its ppE / gIMR variants, on the plus
     polarisation, to which the horizon response at theta = phi = psi = 0 is proportional) or unaligned
-    (IMRPhenomPv2)"""
+    (IMRPhenomPv2 and its variants)"""
     q = intrinsic_params(p, model)
     hp, hc = fourier_waveform(f, q, model)
     total = 0.
     for d in range(len(data)):
-        if model == IMRPHENOMPV2:
+        if model in PV2_FAMILY:
             total += unaligned(data[d], psd[d], f, hp, hc)
         else:
             total += aligned(data[d], psd[d], f, hp)

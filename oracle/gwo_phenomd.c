@@ -79,8 +79,10 @@ void gwo_populate_source(gwo_src *s, const gwo_params *p, int model)
     s->chip = -1;
     s->phip = -1;
     /* waveform_generator.cpp:1296-1300: the ppE fields are copied for the ppE methods only */
-    if (model == GWO_PPE_IMRPHENOMD_INSPIRAL || model == GWO_PPE_IMRPHENOMD_IMR) {
-        s->ppe = (model == GWO_PPE_IMRPHENOMD_INSPIRAL) ? 1 : 2;
+    if (model == GWO_PPE_IMRPHENOMD_INSPIRAL || model == GWO_PPE_IMRPHENOMD_IMR ||
+        model == GWO_PPE_IMRPHENOMPV2_INSPIRAL || model == GWO_PPE_IMRPHENOMPV2_IMR) {
+        /* the Pv2 variants take the IMRPhenomD variants' phase terms (ppE_IMRPhenomP.cpp:12-60) */
+        s->ppe = (model == GWO_PPE_IMRPHENOMD_INSPIRAL || model == GWO_PPE_IMRPHENOMPV2_INSPIRAL) ? 1 : 2;
         s->Nmod = p->ppe_nmod < 0 ? 0 : (p->ppe_nmod > 4 ? 4 : p->ppe_nmod);
         for (int i = 0; i < 4 && i < p->ppe_nmod; i++) {
             s->bppe[i] = p->ppe_b[i];
@@ -88,14 +90,14 @@ void gwo_populate_source(gwo_src *s, const gwo_params *p, int model)
         }
     }
     /* waveform_generator.cpp:1301-1313: the gIMR fields are copied for the gIMR methods only */
-    if (model == GWO_GIMRPHENOMD) {
+    if (model == GWO_GIMRPHENOMD || model == GWO_GIMRPHENOMPV2) { /* gIMRPhenomP.cpp:12-41 */
         s->gimr = 1;
         memcpy(s->dphi, p->gimr_dphi, sizeof(s->dphi));
         memcpy(s->dsigma, p->gimr_dsigma, sizeof(s->dsigma));
         memcpy(s->dbeta, p->gimr_dbeta, sizeof(s->dbeta));
         memcpy(s->dalpha, p->gimr_dalpha, sizeof(s->dalpha));
     }
-    if (model == GWO_IMRPHENOMPV2) {
+    if (GWO_IS_PV2(model)) {
         s->precessing = 1;
         if ((p->chip + 1) > GWO_DOUBLE_COMP_THRESH) {
             s->chip = p->chip;
@@ -791,7 +793,7 @@ int gwo_fourier_waveform(const double *f, int n, double *hplus, double *hcross, 
         }
         return status;
     }
-    if (model == GWO_IMRPHENOMPV2) {
+    if (GWO_IS_PV2(model)) {
         int status = gwo_pv2_waveform(f, n, hplus, hcross, &s);
         double c2z = cos(2. * s.zeta_polariz), s2z = sin(2. * s.zeta_polariz);
         for (int i = 0; i < n; i++) {
@@ -809,7 +811,7 @@ int gwo_fourier_waveform(const double *f, int n, double *hplus, double *hcross, 
 
 int gwo_fourier_amplitude(const double *f, int n, double *amp, int model, const gwo_params *p)
 {
-    if (model == GWO_IMRPHENOMPV2 || model < 0 || model > GWO_GIMRPHENOMD) return 0;
+    if (GWO_IS_PV2(model) || model < 0 || model > GWO_GIMRPHENOMD) return 0;
     gwo_src s;
     gwo_populate_source(&s, p, model);
     return phenomd_construct_amplitude(f, n, amp, &s);
@@ -818,7 +820,7 @@ int gwo_fourier_amplitude(const double *f, int n, double *amp, int model, const 
 int gwo_fourier_phase(const double *f, int n, double *phase_plus, double *phase_cross, int model,
                       const gwo_params *p)
 {
-    if (model == GWO_IMRPHENOMPV2 || model < 0 || model > GWO_GIMRPHENOMD) return 0;
+    if (GWO_IS_PV2(model) || model < 0 || model > GWO_GIMRPHENOMD) return 0;
     gwo_src s;
     gwo_populate_source(&s, p, model);
     int status = phenomd_construct_phase(f, n, phase_plus, &s);

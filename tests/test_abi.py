@@ -134,7 +134,7 @@ def test_gwatpy_host_helpers_and_errors():
         assert b"does not model" in G.lib().gwh_last_error()
         p.close()
     p = P.gen_params(mass1=30, mass2=20)
-    st, _, _ = P.fourier_waveform(f, "gIMRPhenomPv2", p)
+    st, _, _ = P.fourier_waveform(f, "IMRPhenomPv3", p)
     assert st == G.GWH_ERR_UNSUPPORTED
     st, _ = P.fourier_detector_response(f, "LIGO", "IMRPhenomD", p)
     assert st == G.GWH_ERR_UNSUPPORTED

@@ -44,34 +44,40 @@ def test_single_call_statistics_match_restatement():
     assert a > 0.4 * 0.5 * 4 * O.simpsons_sum(f[1] - f[0], np.abs(r) ** 2 / psd)  # a real peak, not noise
 
 
+PV2 = (G.IMRPHENOMPV2, G.PPE_IMRPHENOMPV2_IMR, G.GIMRPHENOMPV2)
+
+
 def _network(model):
     f = W.uniform_grid(20., 512., 0.125)
     psd = W.analytic_psd(f)
     dets = [0, 1, 2]
     kw = dict(W.INJECTION)
-    if model == G.IMRPHENOMPV2:
+    if model in PV2:
         kw.update(spin1=[0.3, 0.1, 0.2], spin2=[-0.1, 0.2, 0.1])
     inj = O.make_params(**kw)
     data = []
     g = W.rng(1, 5)
     for d in dets:
-        h = O.detector_response(f, inj, d, O.IMRPHENOMPV2 if model == G.IMRPHENOMPV2 else O.IMRPHENOMD)
+        h = O.detector_response(f, inj, d, O.IMRPHENOMPV2 if model in PV2 else O.IMRPHENOMD)
         dt = O.dtoa(inj.RA, inj.DEC, inj.gmst, dets[0], d)
         data.append(h * np.exp(-2j * np.pi * dt * f) + W.noise(f, psd, g))
     return f, psd, dets, data
 
 
-@pytest.mark.parametrize("model", [G.IMRPHENOMD, G.IMRPHENOMPV2, G.PPE_IMRPHENOMD_IMR])
+@pytest.mark.parametrize("model", [G.IMRPHENOMD, G.IMRPHENOMPV2, G.PPE_IMRPHENOMD_IMR, G.PPE_IMRPHENOMPV2_IMR,
+                                   G.GIMRPHENOMPV2])
 def test_intrinsic_network_batch_matches_restatement(model):
     f, psd, dets, data = _network(model)
     ev = G.Event(f, data, [psd] * 3, dets)
     P = W.perturbed_params(W.injection_params(1)[0], 12, step=2)
-    if model == G.IMRPHENOMPV2:
+    if model in PV2:
         P = W.precessing_params(12)
-    if model == G.PPE_IMRPHENOMD_IMR:
+    if model in (G.PPE_IMRPHENOMD_IMR, G.PPE_IMRPHENOMPV2_IMR):
         P["ppe_nmod"] = 1
         P["ppe_b"][:, 0] = -7.
         P["ppe_beta"][:, 0] = np.linspace(-3e-5, 3e-5, len(P))
+    if model == G.GIMRPHENOMPV2:
+        P["gimr_dphi"][:, 6] = np.linspace(-0.02, 0.02, len(P))
     out = G.maximized_loglike_batch(ev, P, model)
     assert np.all(np.isfinite(out))
     for i in range(0, len(P), 3):

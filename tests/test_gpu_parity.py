@@ -198,7 +198,7 @@ def test_errors_are_reported_not_fatal():
     with pytest.raises(G.GWHError):
         G.fourier_detector_response(f, g, 99)
     with pytest.raises(G.GWHError):
-        G.fourier_waveform(f, g, model=7)
+        G.fourier_waveform(f, g, model=99)
     # eta > 1/4 style garbage propagates as NaN like the reference, without faulting
     bad = G.make_params(**dict(CASES[0], **EXTR))
     bad.mass2 = -3

@@ -1,5 +1,6 @@
 """ppE phase terms on IMRPhenomD (ppE_IMRPhenomD_Inspiral / ppE_IMRPhenomD_IMR, src/ppE_IMRPhenomD.cpp) and the gIMR
-coefficient modifications (gIMRPhenomD, src/gIMRPhenomD.cpp) on the GPU
+coefficient modifications (gIMRPhenomD, src/gIMRPhenomD.cpp), and the same on IMRPhenomPv2 (src/ppE_IMRPhenomP.cpp,
+src/gIMRPhenomP.cpp), on the GPU
 against the CPU oracle: waveforms <= 1e-10 relative per bin, network log-likelihood <= 1e-9 relative (FP64),
 mixed within 1e-3 of FP64, zero coefficients equal to IMRPhenomD bit for bit, and the refusals (more than
 GWH_PPE_MAX terms, ppE Fisher bases)."""
@@ -184,5 +185,76 @@ def test_gimr_through_gwatpy_lists():
     dphi[2 + 4], dphi[-2 + 4] = 0.05, -1e-4
     o = O.make_params(**CASES[0], **EXTR, gimr_dphi=dphi, gimr_dbeta=[0., 0.1])
     op, oc = O.fourier_waveform(f, o, O.GIMRPHENOMD)
+    wf_close(hp, op)
+    p.close()
+
+
+# ---- the modifications on IMRPhenomPv2 (ppE_IMRPhenomP.cpp, gIMRPhenomP.cpp) ---------------------------------
+PV2_CASES = [dict(mass1=36, mass2=29, spin1=[0.3, 0.1, 0.2], spin2=[-0.1, 0.2, 0.1]),
+             dict(mass1=50, mass2=12, spin1=[0.5, -0.3, -0.4], spin2=[0.1, 0.0, 0.3])]
+PV2_MODS = [(G.PPE_IMRPHENOMPV2_INSPIRAL, TERMS[1]), (G.PPE_IMRPHENOMPV2_IMR, TERMS[2]),
+            (G.GIMRPHENOMPV2, GIMR[0]), (G.GIMRPHENOMPV2, GIMR[1])]
+
+
+@pytest.mark.parametrize("case", PV2_CASES)
+@pytest.mark.parametrize("model,mods", PV2_MODS)
+def test_pv2_variants_waveform_matches_oracle(case, model, mods):
+    f = W.uniform_grid(20., 1024., 0.25)
+    g, o = both(**case, **EXTR, **mods)
+    hp, hc = G.fourier_waveform(f, g, model)
+    op, oc = O.fourier_waveform(f, o, model)
+    wf_close(hp, op)
+    wf_close(hc, oc)
+    gp, _ = G.fourier_waveform(f, g, G.IMRPHENOMPV2)
+    assert not np.array_equal(gp, hp)
+    # with zero modifications the variant is IMRPhenomPv2 bit for bit
+    z, _ = G.fourier_waveform(f, G.make_params(**case, **EXTR), model)
+    gz, _ = G.fourier_waveform(f, G.make_params(**case, **EXTR), G.IMRPHENOMPV2)
+    assert np.array_equal(z, gz)
+
+
+@pytest.mark.parametrize("model", [G.PPE_IMRPHENOMPV2_IMR, G.GIMRPHENOMPV2])
+def test_pv2_variants_network_loglike_batch(model):
+    f, psd, dets, data = _net()
+    ev = G.Event(f, data, [psd] * 3, dets)
+    P_ = W.precessing_params(24)
+    rng = np.random.default_rng(13)
+    if model == G.PPE_IMRPHENOMPV2_IMR:
+        P_["ppe_nmod"] = 1
+        P_["ppe_b"][:, 0] = -7.
+        P_["ppe_beta"][:, 0] = rng.normal(0, 3e-5, len(P_))
+    else:
+        P_["gimr_dphi"][:, 6] = rng.normal(0, 0.02, len(P_))
+        P_["gimr_dalpha"][:, 3] = rng.normal(0, 0.05, len(P_))
+    ll = G.loglike_batch(ev, P_, model, G.FP64)
+    for i in range(0, len(P_), 4):
+        rec = P_[i]
+        kw = {k: (list(rec[k]) if rec[k].shape else rec[k].item()) for k in rec.dtype.names}
+        ref = O.network_loglike(f, data, [psd] * 3, dets, O.make_params(**kw), model)
+        assert abs(ll[i] - ref) <= LL_TOL * max(abs(ref), 1), (i, ll[i], ref)
+    mx = G.loglike_batch(ev, P_, model, G.MIXED)
+    assert np.max(np.abs(mx - ll)) <= 1e-3
+    for i in (0, 7):
+        assert G.loglike_batch(ev, P_[i:i + 1], model, G.FP64)[0] == ll[i]
+    ev.close()
+
+
+def test_pv2_variants_through_gwatpy():
+    f = W.uniform_grid(20., 512., 0.25)
+    kw = dict(mass1=36, mass2=29, Luminosity_Distance=410., incl_angle=2.9, RA=1.37, DEC=-1.26, psi=2.66,
+              gmst=W.GMST, tc=0.01, phiRef=0.3, f_ref=20.)
+    p = P.gen_params(spin1=(.3, .1, .2), spin2=(-.1, .2, .1), Nmod=2, bppe=[-7., -3.], betappe=[3e-5, -3e-3],
+                     phii=[2], delta_phi=[0.05], **kw)
+    st, hp, hc = P.fourier_waveform(f, "ppE_IMRPhenomPv2_Inspiral", p)
+    assert st == G.GWH_OK
+    base = dict(PV2_CASES[0], **EXTR)
+    op, oc = O.fourier_waveform(f, O.make_params(**base, **TERMS[1]), O.PPE_IMRPHENOMPV2_INSPIRAL)
+    wf_close(hp, op)
+    wf_close(hc, oc)
+    st, hp, hc = P.fourier_waveform(f, "gIMRPhenomPv2", p)
+    assert st == G.GWH_OK
+    dphi = [0.] * 14
+    dphi[2 + 4] = 0.05
+    op, oc = O.fourier_waveform(f, O.make_params(**base, gimr_dphi=dphi), O.GIMRPHENOMPV2)
     wf_close(hp, op)
     p.close()

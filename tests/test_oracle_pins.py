@@ -423,3 +423,71 @@ def test_gimr_fit_deltas_change_only_their_region(field, idx, region):
             assert res > 1e-6, (name, res)
         else:
             assert res < 1e-9 * np.max(np.abs(diff)) + 1e-10, (name, res)
+
+
+# ---- the same modifications on IMRPhenomPv2 (ppE_IMRPhenomP.cpp, gIMRPhenomP.cpp) ---------------------------
+def _pv2_kw():
+    return dict(mass1=36., mass2=29., spin1=[.3, .1, .2], spin2=[-.1, .2, .1], DL=410., incl=.4, f_ref=20.,
+                phiRef=.3, tc=0.)
+
+
+def _pv2_phase_shift(f, mod, model, **extra):
+    """-arg(h_mod / h_Pv2) for both polarisations: the modifications change only the aligned-spin phase (and the
+    t_c spline's slope), so the ratio is a pure phase, the same for h+ and hx"""
+    kw = _pv2_kw()
+    gp, gc = O.fourier_waveform(f, O.make_params(**kw), O.IMRPHENOMPV2)
+    mp, mc = O.fourier_waveform(f, O.make_params(**kw, **extra), model)
+    rp, rc = mp / gp, mc / gc
+    assert np.max(np.abs(np.abs(rp) - 1)) < 1e-9 and np.max(np.abs(np.abs(rc) - 1)) < 1e-9
+    assert np.max(np.abs(np.angle(rp / rc))) < 1e-9
+    return -np.unwrap(np.angle(rp)), np.unwrap(np.angle(gp))
+
+
+def test_pv2_variants_with_zero_terms_reduce_to_pv2():
+    kw = _pv2_kw()
+    f = np.linspace(20., 0.2 / (65. * MSOL_SEC) * 0.999, 2001)
+    a = O.fourier_waveform(f, O.make_params(**kw), O.IMRPHENOMPV2)
+    for model, extra in ((O.PPE_IMRPHENOMPV2_INSPIRAL, dict(ppe_nmod=2, ppe_b=[-7., -3.], ppe_beta=[0., 0.])),
+                         (O.PPE_IMRPHENOMPV2_IMR, dict(ppe_nmod=1, ppe_b=[-7.], ppe_beta=[0.])),
+                         (O.GIMRPHENOMPV2, {})):
+        b = O.fourier_waveform(f, O.make_params(**kw, **extra), model)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # and the plain model ignores the modification fields
+    c = O.fourier_waveform(f, O.make_params(**kw, ppe_nmod=1, ppe_b=[-7.], ppe_beta=[1e-4], gimr_dalpha=[.1] * 4),
+                           O.IMRPHENOMPV2)
+    assert np.array_equal(a[0], c[0])
+
+
+def test_ppe_pv2_terms_and_connections():
+    """ppE_IMRPhenomPv2_IMR: the aligned-spin phase gains T(f) in every region; ppE_IMRPhenomPv2_Inspiral: T(f)
+    below f1 = 0.018/M and its tangent at f1 above (the C1 connections); both up to a line (alignment)"""
+    _, _, f, T, dT = _ppe_case(O.PPE_IMRPHENOMD_IMR)
+    extra = dict(ppe_nmod=2, ppe_b=[-7., -3.], ppe_beta=[3e-5, -3e-3])
+    d, gph = _pv2_phase_shift(f, None, O.PPE_IMRPHENOMPV2_IMR, **extra)
+    scale = np.max(np.abs(gph))
+    assert np.max(np.abs(T(f))) > 1.0
+    assert _linear_residual(f, d - T(f)) < 1e-8 * scale
+    assert _linear_residual(f, d + T(f)) > 1e-2  # a wrong sign leaves a curve
+    d, _ = _pv2_phase_shift(f, None, O.PPE_IMRPHENOMPV2_INSPIRAL, **extra)
+    f1 = 0.018 / (65. * MSOL_SEC)
+    assert _linear_residual(f, d - np.where(f < f1, T(f), T(f1) + dT(f1) * (f - f1))) < 1e-8 * scale
+    assert _linear_residual(f, d - T(f)) > 1e-3  # the IMR variant's form does not fit
+
+
+@pytest.mark.parametrize("order", [-3, 0, 2, 4, 7])
+def test_gimr_pv2_phase_deltas_equal_the_ppe_mapping(order):
+    """the gIMR <-> ppE mapping (gIMRPhenomD.cpp:24-52) holds on IMRPhenomPv2's aligned-spin phase too: the
+    gIMRPhenomPv2 waveform with delta_phi of order i equals ppE_IMRPhenomPv2_Inspiral with b = i - 5,
+    beta = (3/128) eta^(-i/5) phi_i delta"""
+    kw = _pv2_kw()
+    f = np.linspace(20., 0.2 / (65. * MSOL_SEC) * 0.999, 2001)
+    delta = 0.05 if order >= 0 else 2e-4
+    dphi = [0.] * 14
+    dphi[order + 4] = delta
+    eta = 36. * 29. / 65. ** 2
+    coeff = 1.0 if order < 0 or order == 1 else O.pn_phase_coefficients(O.make_params(**kw))[order]
+    beta = 3. / 128. * eta ** (-order / 5.) * coeff * delta
+    dg, gph = _pv2_phase_shift(f, None, O.GIMRPHENOMPV2, gimr_dphi=dphi)
+    dp, _ = _pv2_phase_shift(f, None, O.PPE_IMRPHENOMPV2_INSPIRAL, ppe_nmod=1, ppe_b=[order - 5.], ppe_beta=[beta])
+    assert np.max(np.abs(dg)) > 1e-2
+    assert np.max(np.abs(dg - dp)) < 1e-9 * np.max(np.abs(gph)), np.max(np.abs(dg - dp))
