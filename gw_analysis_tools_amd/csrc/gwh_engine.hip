@@ -971,11 +971,9 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_loglike_wl_kernel(WorkMap W, cons
  * device): only instances i < *count are visited, so a launch sized for the largest possible count costs
  * little when the count is small. */
 template <int MODEL, int PREC, int BPL>
-__global__ __launch_bounds__(GWH_TPB) void gwh_loglike_persist_kernel(WorkMap W, const EventDesc *__restrict__ events,
-                                                                      const InstCoef *__restrict__ coefs,
-                                                                      double *__restrict__ partial,
-                                                                      const int *__restrict__ active,
-                                                                      const int *__restrict__ count)
+__device__ __forceinline__ void gwh_loglike_persist(WorkMap W, const EventDesc *__restrict__ events,
+                                                    const InstCoef *__restrict__ coefs, double *__restrict__ partial,
+                                                    const int *__restrict__ active, const int *__restrict__ count)
 {
     __shared__ double lds[GWH_TPB / 64][GWH_PARTIAL_STRIDE];
     const int nn = count ? min(*count, W.n) : W.n;
@@ -985,6 +983,31 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_loglike_persist_kernel(WorkMap W,
         __syncthreads(); /* the LDS of the block reduction is reused by the next item */
     }
 }
+
+#define GWH_PERSIST_ARGS                                                                                             \
+    WorkMap W, const EventDesc *__restrict__ events, const InstCoef *__restrict__ coefs, double *__restrict__ partial, \
+        const int *__restrict__ active, const int *__restrict__ count
+template <int MODEL, int PREC, int BPL>
+__global__ __launch_bounds__(GWH_TPB) void gwh_loglike_persist_kernel(GWH_PERSIST_ARGS)
+{
+    gwh_loglike_persist<MODEL, PREC, BPL>(W, events, coefs, partial, active, count);
+}
+/* Occupancy bounds where they fit without scratch: the mixed IMRPhenomD kernel in 128 VGPRs (four waves per SIMD
+ * instead of three: the 512-instance step's kernel 34.8 -> 29 us), the FP64 IMRPhenomD kernel in 168 (three
+ * instead of two: 62 -> 54 us).  More waves hide more of the bin loads' latency; tighter bounds would spill
+ * (the precessing kernels spill at them). */
+#define GWH_PERSIST_OCC(M, P, B, WAVES)                                                                            \
+    template <>                                                                                                    \
+    __global__ __launch_bounds__(GWH_TPB) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void                 \
+    gwh_loglike_persist_kernel<M, P, B>(GWH_PERSIST_ARGS)                                                          \
+    {                                                                                                              \
+        gwh_loglike_persist<M, P, B>(W, events, coefs, partial, active, count);                                   \
+    }
+#define GWH_PERSIST_OCC_ALL(M, P, WAVES)                                                                           \
+    GWH_PERSIST_OCC(M, P, 1, WAVES) GWH_PERSIST_OCC(M, P, 2, WAVES) GWH_PERSIST_OCC(M, P, 4, WAVES)               \
+        GWH_PERSIST_OCC(M, P, 8, WAVES)
+GWH_PERSIST_OCC_ALL(GWH_IMRPHENOMD, GWH_MIXED, 4)
+GWH_PERSIST_OCC_ALL(GWH_IMRPHENOMD, GWH_FP64, 3)
 
 /* Work list of one bins-per-lane class (single block): instance i contributes active[i] items if its
  * event is tiled with `bpl`; items are ordered by instance then tile.  At most `cap` are written. */
