@@ -105,26 +105,21 @@ struct PtArgs {
     double *gll, *lll;     /* FP64 log-likelihoods of gparams, lparams */
 };
 
-/* the record of basis vector v written straight into dst: the template's fields copied word by word
- * (global to global, no private copy of the 424-byte record), then the sampled fields repacked over them */
-static_assert(sizeof(gwh_params) % 8 == 0, "parameter record is a whole number of 8-byte words");
-__device__ __forceinline__ void pt_record(const PtArgs &a, const double *v, gwh_params *dst)
+__device__ __forceinline__ gwh_params pt_record(const PtArgs &a, const double *v)
 {
-    const unsigned long long *src = (const unsigned long long *)a.tmpl;
-    unsigned long long *out = (unsigned long long *)dst;
-#pragma unroll
-    for (int k = 0; k < (int)(sizeof(gwh_params) / 8); k++) out[k] = src[k];
-    repack(v, a.basis, *dst);
+    gwh_params p = *a.tmpl;
+    repack(v, a.basis, p);
+    return p;
 }
 
 __device__ __forceinline__ double pt_T(const PtArgs &a, int i) { return a.temps[i / a.nc]; }
 
 /* records of the initial states (for their likelihoods) */
-__global__ __launch_bounds__(256) void gwh_pt_init_kernel(PtArgs a)
+__global__ void gwh_pt_init_kernel(PtArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    pt_record(a, a.theta + (size_t)i * a.dim, a.params + i);
+    a.params[i] = pt_record(a, a.theta + (size_t)i * a.dim);
     a.mask[i] = 1;
 }
 
@@ -171,7 +166,9 @@ __global__ __launch_bounds__(256) void gwh_pt_propose_kernel(PtArgs a)
         for (int d = 0; d < GWH_PT_MAXDIM; d++)
             if (d == beta) v[d] = t_beta;
     }
-    pt_record(a, v, a.params + i);
+    /* a.params[i] holds the template's other fields since the init kernel: only the sampled ones change (the
+     * sampling bases' repack reads no field of the record) */
+    repack(v, a.basis, a.params[i]);
 }
 
 /* Metropolis-Hastings (mcmc_sampler_internals.cpp:96-142): reject iff (ll' - ll) / T < log U; a proposal
@@ -190,7 +187,7 @@ __device__ __forceinline__ bool mh_accept(double ratio, double logu)
     return ratio > -INFINITY && !(ratio < logu);
 }
 
-__global__ __launch_bounds__(256) void gwh_pt_accept_kernel(PtArgs a)
+__global__ void gwh_pt_accept_kernel(PtArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -206,14 +203,14 @@ __global__ __launch_bounds__(256) void gwh_pt_accept_kernel(PtArgs a)
         const int slot = atomicAdd(&a.gcount[0], 2);
         a.gslot[i] = slot;
         a.gparams[slot] = a.params[i];
-        pt_record(a, a.theta + (size_t)i * a.dim, a.gparams + slot + 1);
+        a.gparams[slot + 1] = pt_record(a, a.theta + (size_t)i * a.dim);
         a.acc[i] = 2;
         return;
     }
     pt_apply(a, i, mh_accept(ratio, lu));
 }
 
-__global__ __launch_bounds__(256) void gwh_pt_accept_guard_kernel(PtArgs a)
+__global__ void gwh_pt_accept_guard_kernel(PtArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n || a.acc[i] != 2) return;
@@ -291,7 +288,7 @@ __global__ __launch_bounds__(1024) void gwh_pt_swap_kernel(PtArgs a, int last)
                 const int slot = atomicAdd(&a.gcount[1], a.nt);
                 a.lslot[c] = slot;
                 for (int k = 0; k < a.nt; k++)
-                    pt_record(a, a.theta + (size_t)(k * a.nc + c) * a.dim, a.lparams + slot + k);
+                    a.lparams[slot + k] = pt_record(a, a.theta + (size_t)(k * a.nc + c) * a.dim);
             }
         }
     }
