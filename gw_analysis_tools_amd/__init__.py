@@ -45,7 +45,7 @@ EXPORTS = [
     "gwh_loglike_batch_device", "gwh_set_bins_per_lane", "gwh_fisher_batch",
     "gwh_time_loglike_batch_device", "gwh_device_alloc", "gwh_device_free", "gwh_memcpy_h2d",
     "gwh_memcpy_d2h", "gwh_work_counts", "gwh_create_coherent_GW_detection",
-    "gwh_kernel_timing_start", "gwh_kernel_timing_stop", "gwh_device_copy",
+    "gwh_kernel_timing_start", "gwh_kernel_timing_every", "gwh_kernel_timing_stop", "gwh_device_copy",
     "gwh_fisher_derivatives", "gwh_unpack_parameters", "gwh_repack_parameters", "gwh_detector_id", "gwh_dtoa",
     "gwh_gps_to_gmst_radian", "gwh_loglike_multi_device", "gwh_autotune_bins_per_lane",
     "gwh_calculate_snr_internal", "gwh_data_snr", "gwh_snr_batch",
@@ -157,7 +157,8 @@ def lib():
         "gwh_memcpy_h2d": ([V, V, ctypes.c_size_t], I), "gwh_memcpy_d2h": ([V, V, ctypes.c_size_t], I),
         "gwh_work_counts": ([I, I, V, I, ip], I),
         "gwh_create_coherent_GW_detection": ([ip, I, dp, I, I, pp, dp], I),
-        "gwh_kernel_timing_start": ([I], I), "gwh_kernel_timing_stop": ([dp, ip], I),
+        "gwh_kernel_timing_start": ([I], I), "gwh_kernel_timing_every": ([I], I),
+        "gwh_kernel_timing_stop": ([dp, ip], I),
         "gwh_device_copy": ([V, V, ctypes.c_size_t], I),
         "gwh_fisher_derivatives": ([dp, I, I, I, I, I, pp, I, D, dp], I),
         "gwh_unpack_parameters": ([pp, I, dp, ip], I), "gwh_repack_parameters": ([dp, I, pp], I),

@@ -167,6 +167,8 @@ struct Engine {
     std::vector<hipEvent_t> kt_ev;
     int kt_used = 0;
     bool kt_on = false;
+    int kt_every = 1;        /* time every kt_every-th launch */
+    long long kt_seq = 0;    /* launches since timing started */
 };
 
 Engine &gwh_engine();

@@ -1239,7 +1239,8 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     GWH_HIP(hipGetLastError());
     if (X.capturing) timed = false;
     if (timed) GWH_HIP(hipEventRecord(E.ev0, stream));
-    const bool live = !X.capturing && E.kt_on && 2 * (size_t)E.kt_used + 1 < E.kt_ev.size();
+    const bool live = !X.capturing && E.kt_on && (E.kt_seq++ % E.kt_every) == 0 &&
+                      2 * (size_t)E.kt_used + 1 < E.kt_ev.size();
     if (live) GWH_HIP(hipEventRecord(E.kt_ev[2 * E.kt_used], stream));
     const EventDesc *evp = (const EventDesc *)E.d_events;
     const InstCoef *cp = s_coef.as<const InstCoef>();
@@ -1702,6 +1703,19 @@ int gwh_kernel_timing_start(int max_launches)
     }
     E.kt_used = 0;
     E.kt_on = true;
+    E.kt_every = 1;
+    E.kt_seq = 0;
+    return GWH_OK;
+}
+
+int gwh_kernel_timing_every(int k)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    if (!E.kt_on) return gwh_set_error(GWH_ERR_INVALID, "kernel timing was not started");
+    if (k < 1) return gwh_set_error(GWH_ERR_INVALID, "k must be >= 1");
+    E.kt_every = k;
+    E.kt_seq = 0;
     return GWH_OK;
 }
 

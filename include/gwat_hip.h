@@ -367,8 +367,11 @@ int gwh_memcpy_d2h(void *dst, const void *src, size_t bytes);
 
 /* Measurement helpers (no counterpart in the reference).  Between start and stop, every batched
  * likelihood launch records a HIP event pair around its per-bin kernel on the launch stream (at most
- * max_launches pairs); stop synchronises and returns the summed kernel time and the launch count. */
+ * max_launches pairs); stop synchronises and returns the summed kernel time and the launch count.
+ * gwh_kernel_timing_every(k) (after start) times only every k-th launch, so that the event markers
+ * leave the timed launches' wall clock nearly untouched. */
 int gwh_kernel_timing_start(int max_launches);
+int gwh_kernel_timing_every(int k);
 int gwh_kernel_timing_stop(double *total_ms, int *launches);
 /* A plain 16-byte-vector copy kernel of `bytes` (device to device, synchronous), the reference point
  * for scaling hardware traffic counters. */
