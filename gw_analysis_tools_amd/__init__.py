@@ -46,7 +46,8 @@ EXPORTS = [
     "gwh_time_loglike_batch_device", "gwh_device_alloc", "gwh_device_free", "gwh_memcpy_h2d",
     "gwh_memcpy_d2h", "gwh_work_counts", "gwh_create_coherent_GW_detection",
     "gwh_kernel_timing_start", "gwh_kernel_timing_every", "gwh_kernel_timing_stop", "gwh_device_copy",
-    "gwh_fisher_derivatives", "gwh_unpack_parameters", "gwh_repack_parameters", "gwh_detector_id", "gwh_dtoa",
+    "gwh_fisher_derivatives", "gwh_fisher_numerical_model", "gwh_fisher_derivatives_model",
+    "gwh_unpack_parameters", "gwh_repack_parameters", "gwh_detector_id", "gwh_dtoa",
     "gwh_gps_to_gmst_radian", "gwh_loglike_multi_device", "gwh_autotune_bins_per_lane",
     "gwh_calculate_snr_internal", "gwh_data_snr", "gwh_snr_batch",
     "gwh_broker_create", "gwh_broker_destroy", "gwh_broker_submit_loglike", "gwh_broker_submit_fisher",
@@ -161,6 +162,8 @@ def lib():
         "gwh_kernel_timing_stop": ([dp, ip], I),
         "gwh_device_copy": ([V, V, ctypes.c_size_t], I),
         "gwh_fisher_derivatives": ([dp, I, I, I, I, I, pp, I, D, dp], I),
+        "gwh_fisher_numerical_model": ([dp, I, I, I, I, I, dp, I, pp, I, dp, D], I),
+        "gwh_fisher_derivatives_model": ([dp, I, I, I, I, I, I, pp, I, D, dp], I),
         "gwh_unpack_parameters": ([pp, I, dp, ip], I), "gwh_repack_parameters": ([dp, I, pp], I),
         "gwh_detector_id": ([ctypes.c_char_p], I), "gwh_dtoa": ([D, D, D, I, I], D),
         "gwh_gps_to_gmst_radian": ([D], D),

@@ -45,7 +45,52 @@ static gwh_params non_parameter_options(const gwh_params &in)
     w.shift_time = 0; /* fisher.cpp:2527 */
     w.shift_phase = in.shift_phase;
     w.dep_postmerger = in.dep_postmerger;
+    /* fisher.cpp:2541-2551: the ppE terms travel with the parameters */
+    w.ppe_nmod = in.ppe_nmod;
+    for (int i = 0; i < GWH_PPE_MAX; i++) {
+        w.ppe_b[i] = in.ppe_b[i];
+        w.ppe_beta[i] = in.ppe_beta[i];
+    }
     return w;
+}
+
+/* Fisher basis of a model: the GR basis, with betappe[0..nmod) appended for the ppE methods
+ * (fisher.cpp:2083-2088 unpack, :2424-2429 repack) */
+static void unpack_mod(const gwh_params &p, int basis, int nmod, double *v, int *lf)
+{
+    unpack(p, basis, v, lf);
+    const int base = basis_dim(basis);
+    for (int i = 0; i < nmod; i++) {
+        v[base + i] = p.ppe_beta[i];
+        lf[base + i] = 0;
+    }
+}
+static void repack_mod(const double *v, int basis, int nmod, gwh_params &q)
+{
+    repack(v, basis, q);
+    const int base = basis_dim(basis);
+    for (int i = 0; i < nmod; i++) q.ppe_beta[i] = v[base + i];
+}
+
+/* appended parameters of a model's Fisher basis: ppE betas (every point of a batch with the same count);
+ * the gIMR deltas are refused (the engine keeps them as index tables, which cannot say which deltas are
+ * parameters at a zero value) */
+static int fisher_mods(int model, const gwh_params *params, int n, int *nmod)
+{
+    *nmod = 0;
+    if (!gwh_model_valid(model)) return gwh_set_error(GWH_ERR_UNSUPPORTED, "unknown model");
+    const int var = gwh_ppe_variant(model);
+    if (var == 3)
+        return gwh_set_error(GWH_ERR_UNSUPPORTED,
+                             "Fisher matrices of the gIMR methods (deltas in the basis) are not provided");
+    if (var == 0 || n <= 0) return GWH_OK;
+    const int m = params[0].ppe_nmod;
+    if (m < 0 || m > GWH_PPE_MAX) return gwh_set_error(GWH_ERR_UNSUPPORTED, "ppE term count outside 0..4");
+    for (int i = 1; i < n; i++)
+        if (params[i].ppe_nmod != m)
+            return gwh_set_error(GWH_ERR_INVALID, "every point of a ppE Fisher batch needs the same ppE term count");
+    *nmod = m;
+    return GWH_OK;
 }
 
 static double host_dtoa(double ra, double dec, double gmst, int d1, int d2)
@@ -65,13 +110,13 @@ struct PointStencil {
     double c[GWH_FROWS][GWH_MAXSTEN];
 };
 
-static int build_stencil(const gwh_params &p, int basis, int order, double eps, int det, int ref_det,
+static int build_stencil(const gwh_params &p, int basis, int nmod, int order, double eps, int det, int ref_det,
                          std::vector<gwh_params> &inst, std::vector<unsigned char> &no_tof, PointStencil &ps)
 {
-    int dim = basis_dim(basis);
+    int dim = basis_dim(basis) + nmod;
     double v[GWH_FROWS];
     int lf[GWH_FROWS];
-    unpack(p, basis, v, lf);
+    unpack_mod(p, basis, nmod, v, lf);
     gwh_params base = non_parameter_options(p);
     ps.S = (order >= 4) ? 4 : 2;
     for (int i = 0; i < GWH_FROWS; i++)
@@ -103,7 +148,7 @@ static int build_stencil(const gwh_params &p, int basis, int order, double eps, 
             for (int j = 0; j < dim; j++) w[j] = v[j];
             w[i] = v[i] + off[s];
             gwh_params q = base;
-            repack(w, basis, q);
+            repack_mod(w, basis, nmod, q);
             bool two = (S == 4) && (s == 0 || s == 3);
             if (two && det != ref_det) q.tc -= host_dtoa(q.RA, q.DEC, q.gmst, ref_det, det);
             inst.push_back(q);
@@ -373,12 +418,13 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
         std::lock_guard<std::recursive_mutex> lk(E.mu); /* snapshot: the event stays alive meanwhile */
         ev = E.events[event];
     }
-    if (model != GWH_IMRPHENOMD && model != GWH_IMRPHENOMPV2)
-        return gwh_set_error(GWH_ERR_UNSUPPORTED,
-                             "Fisher matrices of the ppE models (betappe in the basis) are not provided");
-    int dim = basis_dim(basis);
+    int nmod, rc0;
+    if ((rc0 = fisher_mods(model, params, n, &nmod)) != GWH_OK) return rc0;
+    const bool pv2 = gwh_is_pv2(model);
+    int dim = basis_dim(basis) + nmod;
+    if (dim > GWH_FROWS) return gwh_set_error(GWH_ERR_UNSUPPORTED, "at most 16 Fisher parameters");
     bool sky = (basis == GWH_BASIS_PHENOMD_SKYAVG);
-    if (sky && model != GWH_IMRPHENOMD)
+    if (sky && pv2)
         return gwh_set_error(GWH_ERR_UNSUPPORTED, "Sky averaged IMRPhenomP is not supported for regular fishers.");
     int S = (order >= 4) ? 4 : 2;
     int per_point = dim * S + (sky ? 1 : 0);
@@ -403,7 +449,7 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
         inst.reserve((size_t)np * per_point);
         for (int q = 0; q < np; q++) {
             PointStencil ps;
-            build_stencil(params[p0 + q], basis, order, eps, det_for_tof, ref_det, inst, no_tof, ps);
+            build_stencil(params[p0 + q], basis, nmod, order, eps, det_for_tof, ref_det, inst, no_tof, ps);
             if (sky) {
                 inst.push_back(params[p0 + q]); /* central amplitude: the caller's parameters */
                 no_tof.push_back(0);
@@ -443,7 +489,7 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
             a.raw = F.raw.as<double>();
             a.d_count = 1;
             dim3 g1((N + GWH_TPB - 1) / GWH_TPB, 1);
-            if (model == GWH_IMRPHENOMD)
+            if (!pv2)
                 hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, double>), g1, dim3(GWH_TPB), 0,
                                    F.stream, a, evp, event, F.deriv.as<double>());
             else
@@ -455,7 +501,7 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
             return GWH_OK;
         }
         if (precision == GWH_FP64) {
-            if (model == GWH_IMRPHENOMD)
+            if (!pv2)
                 hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, double>), g, dim3(GWH_TPB), 0, F.stream,
                                    a, evp, event, F.deriv.as<double>());
             else
@@ -465,7 +511,7 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
             hipLaunchKernelGGL(gwh_gram_kernel<double>, dim3(nblk, np), dim3(GWH_TPB), 0, F.stream,
                                F.deriv.as<const double>(), nrows, F.gram.as<double>(), nblk);
         } else {
-            if (model == GWH_IMRPHENOMD)
+            if (!pv2)
                 hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, float>), g, dim3(GWH_TPB), 0, F.stream,
                                    a, evp, event, F.deriv.as<float>());
             else
@@ -509,13 +555,12 @@ int gwh_fisher_batch_with(int event, int model, int basis, int precision, const 
             return gwh_set_error(GWH_ERR_INVALID, "unknown event");
         ev = E.events[event];
     }
-    if (basis_dim(basis) < 0 || basis_dim(basis) != dim)
-        return gwh_set_error(GWH_ERR_INVALID, "dimension does not match the parameter basis");
+    int nmod;
+    if ((rc = fisher_mods(model, params, n, &nmod)) != GWH_OK) return rc;
+    if (basis_dim(basis) < 0 || basis_dim(basis) + nmod != dim)
+        return gwh_set_error(GWH_ERR_INVALID, "dimension does not match the parameter basis (and ppE terms)");
     if (precision != GWH_FP64 && precision != GWH_MIXED) return gwh_set_error(GWH_ERR_INVALID, "bad precision");
-    if (model != GWH_IMRPHENOMD && model != GWH_IMRPHENOMPV2)
-        return gwh_set_error(GWH_ERR_UNSUPPORTED,
-                             "Fisher matrices of the ppE models (betappe in the basis) are not provided");
-    if ((basis == GWH_BASIS_PV2_FULL || basis == GWH_BASIS_PV2_MCMC) != (model == GWH_IMRPHENOMPV2))
+    if ((basis == GWH_BASIS_PV2_FULL || basis == GWH_BASIS_PV2_MCMC) != gwh_is_pv2(model))
         return gwh_set_error(GWH_ERR_INVALID, "basis does not belong to the model");
     /* the event's weights use the likelihood's df = f[N/2]-f[N/2-1]; the Fisher uses f[1]-f[0] */
     double scale = ev.df_fisher / ev.df_like;
@@ -605,21 +650,24 @@ int gwh_repack_parameters(const double *v, int basis, gwh_params *params)
 /* one-point Fisher on a temporary event holding the PSD for (reference detector, detector): the
  * time-of-flight phase of the second is relative to the first; the weights use df = f[1]-f[0]
  * (fisher.cpp:2746).  deriv != null: return the bare derivatives instead (calculate_derivatives). */
-static int single_point(const double *frequencies, int length, int basis, int detector, int reference_detector,
-                        double *out, int dim, const gwh_params *params, int order, const double *psd, double epsilon,
-                        double *deriv)
+static int single_point(const double *frequencies, int length, int model, int basis, int detector,
+                        int reference_detector, double *out, int dim, const gwh_params *params, int order,
+                        const double *psd, double epsilon, double *deriv)
 {
     Engine &E = gwh_engine();
     int rc = gwh_ensure_init();
     if (rc != GWH_OK) return rc;
     if (!frequencies || !(out || deriv) || !params || length < 2 || (!psd && !deriv))
         return gwh_set_error(GWH_ERR_INVALID, "null argument or length < 2");
-    if (basis_dim(basis) < 0 || basis_dim(basis) != dim)
-        return gwh_set_error(GWH_ERR_INVALID, "dimension does not match the parameter basis");
+    int nmod;
+    if ((rc = fisher_mods(model, params, 1, &nmod)) != GWH_OK) return rc;
+    if (basis_dim(basis) < 0 || basis_dim(basis) + nmod != dim)
+        return gwh_set_error(GWH_ERR_INVALID, "dimension does not match the parameter basis (and ppE terms)");
+    if ((basis == GWH_BASIS_PV2_FULL || basis == GWH_BASIS_PV2_MCMC) != gwh_is_pv2(model))
+        return gwh_set_error(GWH_ERR_INVALID, "basis does not belong to the model");
     if (order != 2 && order != 4) return gwh_set_error(GWH_ERR_INVALID, "order must be 2 or 4");
     if (detector < 0 || detector >= GWH_NDET_TABLE || reference_detector < 0 || reference_detector >= GWH_NDET_TABLE)
         return gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported detector");
-    int model = (basis == GWH_BASIS_PV2_FULL || basis == GWH_BASIS_PV2_MCMC) ? GWH_IMRPHENOMPV2 : GWH_IMRPHENOMD;
     if (basis == GWH_BASIS_PHENOMD_SKYAVG && !params->sky_average)
         return gwh_set_error(GWH_ERR_INVALID, "the sky-averaged basis needs params->sky_average");
     if (basis != GWH_BASIS_PHENOMD_SKYAVG && params->sky_average)
@@ -642,24 +690,45 @@ static int single_point(const double *frequencies, int length, int basis, int de
     return rc;
 }
 
-int gwh_fisher_derivatives(const double *frequencies, int length, int basis, int detector, int reference_detector,
-                           int dim, const gwh_params *params, int order, double epsilon, double *deriv)
+static int gr_model_of(int basis)
+{
+    return (basis == GWH_BASIS_PV2_FULL || basis == GWH_BASIS_PV2_MCMC) ? GWH_IMRPHENOMPV2 : GWH_IMRPHENOMD;
+}
+
+int gwh_fisher_derivatives_model(const double *frequencies, int length, int model, int basis, int detector,
+                                 int reference_detector, int dim, const gwh_params *params, int order, double epsilon,
+                                 double *deriv)
 {
     Engine &E = gwh_engine();
     std::lock_guard<std::recursive_mutex> lk(E.mu);
     if (!deriv) return gwh_set_error(GWH_ERR_INVALID, "null argument");
-    return single_point(frequencies, length, basis, detector, reference_detector, nullptr, dim, params, order,
+    return single_point(frequencies, length, model, basis, detector, reference_detector, nullptr, dim, params, order,
                         nullptr, epsilon, deriv);
+}
+
+int gwh_fisher_numerical_model(const double *frequencies, int length, int model, int basis, int detector,
+                               int reference_detector, double *out, int dim, const gwh_params *params, int order,
+                               const double *psd, double epsilon)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    if (!out || !psd) return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    return single_point(frequencies, length, model, basis, detector, reference_detector, out, dim, params, order, psd,
+                        epsilon, nullptr);
+}
+
+int gwh_fisher_derivatives(const double *frequencies, int length, int basis, int detector, int reference_detector,
+                           int dim, const gwh_params *params, int order, double epsilon, double *deriv)
+{
+    return gwh_fisher_derivatives_model(frequencies, length, gr_model_of(basis), basis, detector, reference_detector,
+                                        dim, params, order, epsilon, deriv);
 }
 
 int gwh_fisher_numerical(const double *frequencies, int length, int basis, int detector, int reference_detector,
                          double *out, int dim, const gwh_params *params, int order, const double *psd, double epsilon)
 {
-    Engine &E = gwh_engine();
-    std::lock_guard<std::recursive_mutex> lk(E.mu);
-    if (!out || !psd) return gwh_set_error(GWH_ERR_INVALID, "null argument");
-    return single_point(frequencies, length, basis, detector, reference_detector, out, dim, params, order, psd,
-                        epsilon, nullptr);
+    return gwh_fisher_numerical_model(frequencies, length, gr_model_of(basis), basis, detector, reference_detector,
+                                      out, dim, params, order, psd, epsilon);
 }
 
 int gwh_fisher_batch(int event, int model, int basis, int precision, const gwh_params *params, int n, int dim,

@@ -292,12 +292,27 @@ void repack_parameters_py(double *parameters, void *gen_param, char *generation_
         gwh_report(GWH_ERR_INVALID, "not a parameter handle from gen_params_base_py");
         return;
     }
-    int basis = basis_of(generation_method, dim, q->p.sky_average);
+    /* a ppE method's betas follow its GR basis (fisher.cpp:2083-2088, :2424-2429) */
+    const bool ppe = generation_method && strstr(generation_method, "ppE_");
+    const int nmod = ppe ? q->p.ppe_nmod : 0;
+    std::string gr = generation_method ? generation_method : "";
+    if (ppe) {
+        gr.erase(gr.find("ppE_"), 4);
+        for (const char *suf : {"_Inspiral", "_IMR"}) {
+            const std::string s(suf);
+            if (gr.size() > s.size() && gr.compare(gr.size() - s.size(), s.size(), s) == 0) {
+                gr.erase(gr.size() - s.size());
+                break;
+            }
+        }
+    }
+    int basis = basis_of(gr.c_str(), dim - nmod, q->p.sky_average);
     if (basis < 0) {
         gwh_report(GWH_ERR_UNSUPPORTED, "generation method / dimension not on the engine's path");
         return;
     }
     gwh_repack_parameters(parameters, basis, &q->p);
+    for (int i = 0; i < nmod; i++) q->p.ppe_beta[i] = parameters[dim - nmod + i];
 }
 
 double gps_to_GMST_radian_py(double gps) { return gwh_gps_to_gmst_radian(gps); }

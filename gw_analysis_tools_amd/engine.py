@@ -120,21 +120,30 @@ def mcmc_likelihood_extrinsic(f, data, psd, dets, p, model=IMRPHENOMD, T_eff=0.0
                                                0, di, D, T_eff)
 
 
-def fisher_numerical(f, psd, basis, det, ref_det, dim, p, order=2, epsilon=1e-8):
+def fisher_numerical(f, psd, basis, det, ref_det, dim, p, order=2, epsilon=1e-8, model=None):
+    """model None: the GR model of the basis; a ppE model appends p.ppe_nmod betas to the basis"""
     f = _f64(f)
     s = _f64(psd)
     out = np.zeros(dim * dim)
-    check(lib().gwh_fisher_numerical(_dp(f), len(f), basis, det, ref_det, _dp(out), dim, ctypes.byref(p), order,
-                                     _dp(s), epsilon))
+    if model is None:
+        check(lib().gwh_fisher_numerical(_dp(f), len(f), basis, det, ref_det, _dp(out), dim, ctypes.byref(p), order,
+                                         _dp(s), epsilon))
+    else:
+        check(lib().gwh_fisher_numerical_model(_dp(f), len(f), model, basis, det, ref_det, _dp(out), dim,
+                                               ctypes.byref(p), order, _dp(s), epsilon))
     return out.reshape(dim, dim)
 
 
-def fisher_derivatives(f, basis, det, ref_det, dim, p, order=2, epsilon=1e-8):
+def fisher_derivatives(f, basis, det, ref_det, dim, p, order=2, epsilon=1e-8, model=None):
     """bare derivatives dh/dtheta_i (dim x len(f) complex), as calculate_derivatives returns them"""
     f = _f64(f)
     out = np.zeros(2 * dim * len(f))
-    check(lib().gwh_fisher_derivatives(_dp(f), len(f), basis, det, ref_det, dim, ctypes.byref(p), order, epsilon,
-                                       _dp(out)))
+    if model is None:
+        check(lib().gwh_fisher_derivatives(_dp(f), len(f), basis, det, ref_det, dim, ctypes.byref(p), order, epsilon,
+                                           _dp(out)))
+    else:
+        check(lib().gwh_fisher_derivatives_model(_dp(f), len(f), model, basis, det, ref_det, dim, ctypes.byref(p),
+                                                 order, epsilon, _dp(out)))
     return _cplx(out).reshape(dim, len(f))
 
 

@@ -182,6 +182,17 @@ double gwh_data_snr(const double *frequencies, int length, const double *data, c
 int gwh_fisher_derivatives(const double *frequencies, int length, int basis, int detector, int reference_detector,
                            int dim, const gwh_params *params, int order, double epsilon, double *deriv);
 
+/* The same two with the waveform model given (fisher_numerical / calculate_derivatives with a ppE method string,
+ * src/fisher.cpp:42-60, :2083-2088, :2424-2429): for GWH_PPE_IMRPHENOMD_INSPIRAL / _IMR and the IMRPhenomPv2
+ * ppE ids the basis is the GR basis of `basis` with betappe[0 .. params->ppe_nmod) appended, so
+ * dim = basis dimension + ppe_nmod (at most 16 in all).  gIMR models are refused (GWH_ERR_UNSUPPORTED). */
+int gwh_fisher_numerical_model(const double *frequencies, int length, int model, int basis, int detector,
+                               int reference_detector, double *out, int dim, const gwh_params *params, int order,
+                               const double *psd, double epsilon);
+int gwh_fisher_derivatives_model(const double *frequencies, int length, int model, int basis, int detector,
+                                 int reference_detector, int dim, const gwh_params *params, int order, double epsilon,
+                                 double *deriv);
+
 /* unpack_parameters / repack_parameters (src/fisher.cpp:1841-2161, :2162-2511): the basis vector of a
  * parameter set and back.  log_factors (optional, dim ints) marks the parameters differentiated in log. */
 int gwh_unpack_parameters(const gwh_params *params, int basis, double *v, int *log_factors);

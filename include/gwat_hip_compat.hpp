@@ -58,9 +58,29 @@ inline void gimr_table(int n, const int *idx, const double *delta, int lo, int h
 /* "SIMPSONS" / "GAUSSLEG" (mcmc_gw.cpp:811-856) */
 inline int integration_id(const std::string &method) { return method == "GAUSSLEG" ? GWH_GAUSSLEG : GWH_SIMPSONS; }
 
-/* Fisher / MCMC bases of fisher_numerical's method strings (fisher.cpp:1841-2161) */
-inline int basis_id(const std::string &method, bool sky_average)
+/* the GR method under a ppE method string: "ppE_IMRPhenomD_Inspiral" -> "IMRPhenomD",
+ * "MCMC_ppE_IMRPhenomPv2_IMR" -> "MCMC_IMRPhenomPv2" (fisher.cpp:2083-2088 appends the betas to that basis) */
+inline std::string fisher_gr_method(const std::string &method)
 {
+    std::string m = method;
+    const size_t p = m.find("ppE_");
+    if (p == std::string::npos) return m;
+    m.erase(p, 4);
+    for (const char *suf : {"_Inspiral", "_IMR"}) {
+        const std::string s(suf);
+        if (m.size() > s.size() && m.compare(m.size() - s.size(), s.size(), s) == 0) {
+            m.erase(m.size() - s.size());
+            break;
+        }
+    }
+    return m;
+}
+
+/* Fisher / MCMC bases of fisher_numerical's method strings (fisher.cpp:1841-2161); a ppE method's betas are
+ * appended to the basis of its GR method */
+inline int basis_id(const std::string &method_in, bool sky_average)
+{
+    const std::string method = fisher_gr_method(method_in);
     if (method == "IMRPhenomD") return sky_average ? GWH_BASIS_PHENOMD_SKYAVG : GWH_BASIS_PHENOMD_FULL;
     if (method == "MCMC_IMRPhenomD") return GWH_BASIS_PHENOMD_MCMC;
     if (method == "IMRPhenomPv2") return GWH_BASIS_PV2_FULL;
@@ -213,11 +233,12 @@ inline void fisher_numerical(const double *f, int n, const std::string &method, 
                              const double *psd, double epsilon = 1e-8)
 {
     gwh_params p;
+    const std::string mm = method.compare(0, 5, "MCMC_") == 0 ? method.substr(5) : method;
     int basis = basis_id(method, g->sky_average), d = gwh_detector_id(det.c_str()),
-        r = gwh_detector_id(ref_det.c_str());
+        r = gwh_detector_id(ref_det.c_str()), model = model_id(mm);
     std::vector<double> out((size_t)dim * dim, std::nan(""));
-    if (basis >= 0 && d >= 0 && r >= 0 && to_params(*g, p))
-        gwh_fisher_numerical(f, n, basis, d, r, out.data(), dim, &p, order, psd, epsilon);
+    if (basis >= 0 && model >= 0 && d >= 0 && r >= 0 && to_params(*g, p, model))
+        gwh_fisher_numerical_model(f, n, model, basis, d, r, out.data(), dim, &p, order, psd, epsilon);
     for (int i = 0; i < dim; i++)
         for (int j = 0; j < dim; j++) output[i][j] = out[(size_t)i * dim + j];
 }
@@ -230,8 +251,9 @@ inline void repack_parameters(const double *v, Params *g, const std::string &met
     int basis = basis_id(method, g->sky_average);
     to_params(*g, p);
     if (basis < 0 || gwh_repack_parameters(v, basis, &p) != GWH_OK) return;
-    (void)dim;
     from_params(p, *g);
+    if (method.find("ppE") != std::string::npos) /* the betas follow the GR basis (fisher.cpp:2424-2429) */
+        for (int i = 0, base = dim - g->Nmod; i < g->Nmod; i++) g->betappe[i] = v[base + i];
 }
 
 } // namespace gwh

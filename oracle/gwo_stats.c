@@ -215,7 +215,17 @@ static void local_method(const char *m, char *out)
     }
 }
 
-static int model_of(const char *m) { return has(m, "IMRPhenomPv2") ? GWO_IMRPHENOMPV2 : GWO_IMRPHENOMD; }
+/* the waveform model of a (local) generation method; the ppE methods keep their variant */
+static int model_of(const char *m)
+{
+    const int pv2 = has(m, "IMRPhenomPv2");
+    if (has(m, "ppE")) {
+        const int ins = has(m, "Inspiral");
+        return pv2 ? (ins ? GWO_PPE_IMRPHENOMPV2_INSPIRAL : GWO_PPE_IMRPHENOMPV2_IMR)
+                   : (ins ? GWO_PPE_IMRPHENOMD_INSPIRAL : GWO_PPE_IMRPHENOMD_IMR);
+    }
+    return pv2 ? GWO_IMRPHENOMPV2 : GWO_IMRPHENOMD;
+}
 
 static void cart_sph(const double *c, double *s)
 {
@@ -232,7 +242,32 @@ static void sph_cart(const double *s, double *c)
 }
 
 /* fisher.cpp:1841-2161 (terrestrial, non-equatorial-orientation branches) */
+static int unpack_gr(double *v, int *lf, const gwo_params *p, const char *m, int dim);
+static int repack_gr(const double *v, gwo_params *a, const char *m);
+
+/* the ppE methods append betappe[0..Nmod) to the GR basis (fisher.cpp:2083-2088, :2424-2429) */
 int gwo_unpack_parameters(double *v, int *lf, const gwo_params *p, const char *m, int dim)
+{
+    if (!unpack_gr(v, lf, p, m, dim)) return 0;
+    if (has(m, "ppE")) {
+        const int base = dim - p->ppe_nmod;
+        for (int i = 0; i < p->ppe_nmod; i++) v[base + i] = p->ppe_beta[i];
+    }
+    return 1;
+}
+
+int gwo_repack_parameters(const double *v, gwo_params *a, const char *m, int dim, const gwo_params *orig)
+{
+    (void)orig;
+    if (!repack_gr(v, a, m)) return 0;
+    if (has(m, "ppE")) {
+        const int base = dim - a->ppe_nmod;
+        for (int i = 0; i < a->ppe_nmod; i++) a->ppe_beta[i] = v[base + i];
+    }
+    return 1;
+}
+
+static int unpack_gr(double *v, int *lf, const gwo_params *p, const char *m, int dim)
 {
     for (int i = 0; i < dim; i++) lf[i] = 0;
     int mcmc = has(m, "MCMC");
@@ -314,10 +349,8 @@ int gwo_unpack_parameters(double *v, int *lf, const gwo_params *p, const char *m
 }
 
 /* fisher.cpp:2167-2511 */
-int gwo_repack_parameters(const double *v, gwo_params *a, const char *m, int dim, const gwo_params *orig)
+static int repack_gr(const double *v, gwo_params *a, const char *m)
 {
-    (void)dim;
-    (void)orig;
     int mcmc = has(m, "MCMC");
     if (!a->sky_average) {
         if (has(m, "IMRPhenomPv2")) {
@@ -413,6 +446,12 @@ static void repack_non_parameter_options(gwo_params *w, const gwo_params *in)
     w->shift_time = 0;
     w->shift_phase = in->shift_phase;
     w->dep_postmerger = in->dep_postmerger;
+    /* fisher.cpp:2541-2551: the ppE terms travel with the parameters */
+    w->ppe_nmod = in->ppe_nmod;
+    for (int i = 0; i < 4; i++) {
+        w->ppe_b[i] = in->ppe_b[i];
+        w->ppe_beta[i] = in->ppe_beta[i];
+    }
 }
 
 static void add_tof_phase(double *r, const double *f, int n, double dt_phase)

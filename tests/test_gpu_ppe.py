@@ -115,10 +115,14 @@ def test_ppe_refusals_and_gwatpy():
     P_ = W.perturbed_params(W.injection_params(1)[0], 2, step=1)
     P_["ppe_nmod"] = 1
     P_["ppe_b"][:, 0] = -7.
-    # the reference's ppE Fisher bases carry betappe as parameters: not provided, refused
+    # the ppE Fisher bases carry betappe as parameters: the GR dimension alone is refused
     with pytest.raises(G.GWHError) as e:
         G.fisher_batch(ev, P_, G.PPE_IMRPHENOMD_IMR, G.BASIS_PHENOMD_FULL, 11)
     assert "ppE" in str(e.value)
+    # the gIMR deltas are not basis parameters here
+    with pytest.raises(G.GWHError) as e:
+        G.fisher_batch(ev, P_, G.GIMRPHENOMD, G.BASIS_PHENOMD_FULL, 11)
+    assert "gIMR" in str(e.value)
     ev.close()
     # gwatpy handles carry Nmod / bppe / betappe to the ppE methods
     p = P.gen_params(mass1=36, mass2=29, spin1=(0, 0, .3), spin2=(0, 0, -.1), Luminosity_Distance=410.,
@@ -258,3 +262,67 @@ def test_pv2_variants_through_gwatpy():
     op, oc = O.fourier_waveform(f, O.make_params(**base, gimr_dphi=dphi), O.GIMRPHENOMPV2)
     wf_close(hp, op)
     p.close()
+
+
+# ---- Fisher matrices with the ppE betas in the basis (fisher.cpp:42-60, :2083-2088, :2424-2429) --------------
+FISHER_TOL = 1e-6
+EPS = 1e-5  # the well-conditioned step of the GPU Fisher parity tests
+
+
+def _fnorm(a, b):
+    s = np.sqrt(np.abs(np.diag(b)))
+    return np.max(np.abs(a - b) / np.outer(s, s)), np.max(np.abs(np.diag(a) - np.diag(b)) / np.abs(np.diag(b)))
+
+
+FISHER_CASES = [  # (model, engine basis, oracle method, GR dim, sky, ppE terms)
+    (G.PPE_IMRPHENOMD_IMR, G.BASIS_PHENOMD_FULL, "ppE_IMRPhenomD_IMR", 11, 0, TERMS[1]),
+    (G.PPE_IMRPHENOMD_INSPIRAL, G.BASIS_PHENOMD_FULL, "ppE_IMRPhenomD_Inspiral", 11, 0, TERMS[0]),
+    (G.PPE_IMRPHENOMD_INSPIRAL, G.BASIS_PHENOMD_SKYAVG, "ppE_IMRPhenomD_Inspiral", 7, 1, TERMS[1]),
+    (G.PPE_IMRPHENOMD_IMR, G.BASIS_PHENOMD_MCMC, "MCMC_ppE_IMRPhenomD_IMR", 11, 0, TERMS[2]),
+    (G.PPE_IMRPHENOMPV2_IMR, G.BASIS_PV2_FULL, "ppE_IMRPhenomPv2_IMR", 13, 0, TERMS[1]),
+]
+
+
+@pytest.mark.parametrize("model,basis,method,gdim,sky,terms", FISHER_CASES)
+@pytest.mark.parametrize("order", [2, 4])
+def test_ppe_fisher_vs_oracle(model, basis, method, gdim, sky, terms, order):
+    f = W.uniform_grid(20., 1024., 0.25)
+    psd = W.analytic_psd(f)
+    kw = dict(mass1=36., mass2=29., spin1=[0, 0, 0.3], spin2=[0, 0, -0.1], DL=410., tc=0., phiRef=0., f_ref=20.,
+              RA=1.37, DEC=-1.26, psi=2.66, incl=2.9, gmst=W.GMST, sky_average=sky, **terms)
+    if gwh_pv2(model):
+        kw.update(spin1=[0.3, 0.1, 0.3], spin2=[-0.1, 0.2, -0.1], chip=0.35, phip=0.8)
+    dim = gdim + terms["ppe_nmod"]
+    gp, op = G.make_params(**kw), O.make_params(**kw)
+    a = G.fisher_numerical(f, psd, basis, 0, 0, dim, gp, order=order, epsilon=EPS, model=model)
+    b = O.fisher_numerical(f, psd, method, 0, 0, dim, op, order=order, epsilon=EPS)
+    e, ed = _fnorm(a, b)
+    assert e < FISHER_TOL and ed < FISHER_TOL, (e, ed)
+    assert np.allclose(a, a.T, rtol=0, atol=0)
+
+
+def gwh_pv2(model):
+    return model in (G.IMRPHENOMPV2, G.PPE_IMRPHENOMPV2_INSPIRAL, G.PPE_IMRPHENOMPV2_IMR, G.GIMRPHENOMPV2)
+
+
+def test_ppe_fisher_batch_and_two_detectors():
+    """the batched Fisher of ppE points on a resident network event against one-point Fishers summed over the
+    detectors; and one point on (H1 relative to L1) against the oracle"""
+    f = W.uniform_grid(20., 512., 0.25)
+    psd = W.analytic_psd(f)
+    dets = [0, 1, 2]
+    ev = G.Event(f, [np.zeros(len(f), complex)] * 3, [psd] * 3, dets)
+    P_ = W.perturbed_params(W.injection_params(1)[0], 6, step=3)
+    P_["ppe_nmod"] = 2
+    P_["ppe_b"][:, 0], P_["ppe_b"][:, 1] = -7., -3.
+    P_["ppe_beta"][:, 0] = np.linspace(-2e-5, 2e-5, len(P_))
+    F, sig = G.fisher_batch(ev, P_, G.PPE_IMRPHENOMD_IMR, G.BASIS_PHENOMD_FULL, 13, epsilon=EPS, with_sigma=True)
+    assert F.shape == (len(P_), 13, 13) and np.all(np.isfinite(sig))
+    for i in (0, 4):
+        rec = P_[i]
+        kw = {k: (list(rec[k]) if rec[k].shape else rec[k].item()) for k in rec.dtype.names}
+        op = O.make_params(**kw)
+        ref = sum(O.fisher_numerical(f, psd, "ppE_IMRPhenomD_IMR", d, dets[0], 13, op, epsilon=EPS) for d in dets)
+        e, ed = _fnorm(F[i], ref)
+        assert e < FISHER_TOL and ed < FISHER_TOL, (i, e, ed)
+    ev.close()

@@ -491,3 +491,49 @@ def test_gimr_pv2_phase_deltas_equal_the_ppe_mapping(order):
     dp, _ = _pv2_phase_shift(f, None, O.PPE_IMRPHENOMPV2_INSPIRAL, ppe_nmod=1, ppe_b=[order - 5.], ppe_beta=[beta])
     assert np.max(np.abs(dg)) > 1e-2
     assert np.max(np.abs(dg - dp)) < 1e-9 * np.max(np.abs(gph)), np.max(np.abs(dg - dp))
+
+
+# ---- Fisher bases of the ppE methods (fisher.cpp:2083-2088, :2424-2429: betappe appended to the GR basis) ----
+def _ppe_fisher_case(nmod=1, beta=(0.0,), b=(-7.0,)):
+    kw = dict(mass1=36., mass2=29., spin1=[0, 0, .3], spin2=[0, 0, -.1], DL=410., incl=.4, f_ref=20., phiRef=.3,
+              tc=0., RA=1.37, DEC=-1.26, psi=2.66, gmst=0.5)
+    f = np.linspace(20., 0.2 / (65. * MSOL_SEC) * 0.999, 4001)
+    psd = 1e-46 * (1 + (50. / f) ** 4 + (f / 300.) ** 2)
+    p = O.make_params(ppe_nmod=nmod, ppe_b=list(b) + [0.] * (4 - nmod), ppe_beta=list(beta) + [0.] * (4 - nmod), **kw)
+    mc = (36. * 29.) ** 0.6 / 65. ** 0.2 * MSOL_SEC
+    return kw, f, psd, p, mc
+
+
+def test_ppe_fisher_gr_block_at_beta_zero_is_the_gr_fisher():
+    """at beta = 0 the ppE waveform and all its GR-parameter perturbations equal IMRPhenomD's, so the first 11 x 11
+    block of the 12-dimensional ppE Fisher is the GR Fisher (an index or ordering mistake breaks it)"""
+    kw, f, psd, p, mc = _ppe_fisher_case()
+    g = O.fisher_numerical(f, psd, "IMRPhenomD", 0, 0, 11, O.make_params(**kw))
+    for m in ("ppE_IMRPhenomD_IMR", "ppE_IMRPhenomD_Inspiral"):
+        F = O.fisher_numerical(f, psd, m, 0, 0, 12, p)
+        assert np.allclose(F[:11, :11], g, rtol=1e-12, atol=0), m
+        assert F[11, 11] > 0
+
+
+def test_ppe_fisher_beta_beta_element_closed_form():
+    """d h / d beta = -i dphi/dbeta h with dphi/dbeta = T(f) - T(f_ref) for ppE_IMRPhenomD_IMR (the connections do
+    not move; the phase alignment subtracts the value at f_ref; the Fisher runs without the time shift), and for
+    ppE_IMRPhenomD_Inspiral the term below f1 = 0.018/M, its tangent at f1 above; so F_bb = 4 (|h|^2 dphi^2 / S)
+    with the same Simpson rule"""
+    kw, f, psd, p, mc = _ppe_fisher_case(beta=(2e-5,))
+    h = O.detector_response(f, p, 0, O.PPE_IMRPHENOMD_IMR)
+    T = lambda x: (np.pi * mc * x) ** (-7. / 3.)
+    dT = lambda x: (-7. / 3.) * (np.pi * mc * x) ** (-7. / 3.) / x
+    df = f[1] - f[0]
+    F = O.fisher_numerical(f, psd, "ppE_IMRPhenomD_IMR", 0, 0, 12, p)
+    d = T(f) - T(20.)
+    ref = 4 * O.simpsons_sum(df, np.abs(h) ** 2 * d ** 2 / psd)
+    assert abs(F[11, 11] - ref) <= 1e-6 * ref, (F[11, 11], ref)
+    hi = O.detector_response(f, p, 0, O.PPE_IMRPHENOMD_INSPIRAL)
+    f1 = 0.018 / (65. * MSOL_SEC)
+    d = np.where(f < f1, T(f), T(f1) + dT(f1) * (f - f1)) - T(20.)
+    ref = 4 * O.simpsons_sum(df, np.abs(hi) ** 2 * d ** 2 / psd)
+    Fi = O.fisher_numerical(f, psd, "ppE_IMRPhenomD_Inspiral", 0, 0, 12, p)
+    assert abs(Fi[11, 11] - ref) <= 1e-6 * ref, (Fi[11, 11], ref)
+    # the other variant's closed form does not fit
+    assert abs(Fi[11, 11] - F[11, 11]) > 1e-3 * ref
