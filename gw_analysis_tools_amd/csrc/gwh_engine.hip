@@ -3,7 +3,8 @@
  * log-likelihood of IMRPhenomD / IMRPhenomPv2 on MI355X (gfx950).
  *
  * Hot path (one MCMC step):
- *   1. gwh_setup_kernel      one lane per instance: FP64 InstCoef record
+ *   1. gwh_setup_kernel      FP64 InstCoef record of every instance: four waves per
+ *                            64 instances compute independent parts, joined in LDS
  *   2. loglike kernel        (instance, bin-tile) work items: a persistent grid
  *                            of as many blocks as fit on the chip at once
  *                            strides over the items in tile-major order (for

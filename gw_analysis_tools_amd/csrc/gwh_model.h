@@ -2,7 +2,7 @@
  * gwh_model.h -- per-instance setup and per-bin evaluation of IMRPhenomD and
  * IMRPhenomPv2 on the GPU (gfx950).
  *
- * The setup (one lane per instance, FP64) folds everything that does not
+ * The setup (FP64; one lane of each of four waves per instance) folds everything that does not
  * depend on frequency into one InstCoef record: the 19 calibration fits, the
  * ringdown/damping frequencies, the C1 connection coefficients of amplitude
  * and phase, the phase/time alignment, the precession constants, and for each
@@ -121,7 +121,7 @@ struct DetGeom {
 };
 
 /* ------------------------------------------------------------------------- */
-/* FP64 setup helpers (device, one lane per instance)                          */
+/* FP64 setup helpers (device)                                                 */
 /* ------------------------------------------------------------------------- */
 
 __device__ __forceinline__ double gwh_spline_eval(const double *x, const double *y, const double *c, int n,
