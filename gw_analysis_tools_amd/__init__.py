@@ -77,7 +77,8 @@ class Params(ctypes.Structure):
                 ("ppe_nmod", ctypes.c_int32), ("ppe_pad", ctypes.c_int32),
                 ("ppe_b", ctypes.c_double * 4), ("ppe_beta", ctypes.c_double * 4),
                 ("gimr_dphi", ctypes.c_double * 14), ("gimr_dsigma", ctypes.c_double * 3),
-                ("gimr_dbeta", ctypes.c_double * 2), ("gimr_dalpha", ctypes.c_double * 4)]
+                ("gimr_dbeta", ctypes.c_double * 2), ("gimr_dalpha", ctypes.c_double * 4),
+                ("gimr_nlist", ctypes.c_int32), ("gimr_pad", ctypes.c_int32), ("gimr_list", ctypes.c_int8 * 16)]
 
 
 class PTConfig(ctypes.Structure):
@@ -94,8 +95,9 @@ PARAMS_DTYPE = np.dtype([("mass1", "f8"), ("mass2", "f8"), ("spin1", "f8", 3), (
                          ("shift_time", "i4"), ("shift_phase", "i4"), ("sky_average", "i4"),
                          ("dep_postmerger", "i4"), ("ppe_nmod", "i4"), ("ppe_pad", "i4"), ("ppe_b", "f8", 4),
                          ("ppe_beta", "f8", 4), ("gimr_dphi", "f8", 14), ("gimr_dsigma", "f8", 3),
-                         ("gimr_dbeta", "f8", 2), ("gimr_dalpha", "f8", 4)])
-assert PARAMS_DTYPE.itemsize == ctypes.sizeof(Params) == 424
+                         ("gimr_dbeta", "f8", 2), ("gimr_dalpha", "f8", 4), ("gimr_nlist", "i4"),
+                         ("gimr_pad", "i4"), ("gimr_list", "i1", 16)])
+assert PARAMS_DTYPE.itemsize == ctypes.sizeof(Params) == 448
 
 
 def make_params(**kw):
@@ -105,7 +107,8 @@ def make_params(**kw):
     p.shift_time = 1
     p.shift_phase = 1
     for k, v in kw.items():
-        if k in ("spin1", "spin2", "ppe_b", "ppe_beta", "gimr_dphi", "gimr_dsigma", "gimr_dbeta", "gimr_dalpha"):
+        if k in ("spin1", "spin2", "ppe_b", "ppe_beta", "gimr_dphi", "gimr_dsigma", "gimr_dbeta", "gimr_dalpha",
+                 "gimr_list"):
             getattr(p, k)[:len(v)] = list(v)
         else:
             setattr(p, k, v)

@@ -45,50 +45,83 @@ static gwh_params non_parameter_options(const gwh_params &in)
     w.shift_time = 0; /* fisher.cpp:2527 */
     w.shift_phase = in.shift_phase;
     w.dep_postmerger = in.dep_postmerger;
-    /* fisher.cpp:2541-2551: the ppE terms travel with the parameters */
+    /* fisher.cpp:2541-2570: the ppE terms and the gIMR lists travel with the parameters */
     w.ppe_nmod = in.ppe_nmod;
     for (int i = 0; i < GWH_PPE_MAX; i++) {
         w.ppe_b[i] = in.ppe_b[i];
         w.ppe_beta[i] = in.ppe_beta[i];
     }
+    memcpy(w.gimr_dphi, in.gimr_dphi, sizeof(w.gimr_dphi));
+    memcpy(w.gimr_dsigma, in.gimr_dsigma, sizeof(w.gimr_dsigma));
+    memcpy(w.gimr_dbeta, in.gimr_dbeta, sizeof(w.gimr_dbeta));
+    memcpy(w.gimr_dalpha, in.gimr_dalpha, sizeof(w.gimr_dalpha));
+    w.gimr_nlist = in.gimr_nlist;
+    memcpy(w.gimr_list, in.gimr_list, sizeof(w.gimr_list));
     return w;
 }
 
+/* the table entry of a gIMR Fisher-basis code (GWH_GIMR_CODE_*), null for a parameter the waveform does not read */
+static const double *gimr_slot(const gwh_params &p, int code)
+{
+    if (code < 0 || code >= GWH_GIMR_NCODES) return nullptr;
+    if (code < GWH_GIMR_CODE_SIGMA(2)) return &p.gimr_dphi[code];
+    if (code < GWH_GIMR_CODE_BETA(2)) return &p.gimr_dsigma[code - GWH_GIMR_CODE_SIGMA(2)];
+    if (code < GWH_GIMR_CODE_ALPHA(2)) return &p.gimr_dbeta[code - GWH_GIMR_CODE_BETA(2)];
+    return &p.gimr_dalpha[code - GWH_GIMR_CODE_ALPHA(2)];
+}
+
 /* Fisher basis of a model: the GR basis, with betappe[0..nmod) appended for the ppE methods
- * (fisher.cpp:2083-2088 unpack, :2424-2429 repack) */
-static void unpack_mod(const gwh_params &p, int basis, int nmod, double *v, int *lf)
+ * (fisher.cpp:2083-2088 unpack, :2424-2429 repack) or the listed gIMR deltas for the gIMR methods
+ * (:2128-2158, :2473-2505; a repeated entry repacks into the table only through its first occurrence, the one
+ * the waveform reads) */
+static void unpack_mod(const gwh_params &p, int basis, int var, int nmod, double *v, int *lf)
 {
     unpack(p, basis, v, lf);
     const int base = basis_dim(basis);
     for (int i = 0; i < nmod; i++) {
-        v[base + i] = p.ppe_beta[i];
+        const double *s = (var == 3) ? gimr_slot(p, p.gimr_list[i]) : &p.ppe_beta[i];
+        v[base + i] = s ? *s : 0.0;
         lf[base + i] = 0;
     }
 }
-static void repack_mod(const double *v, int basis, int nmod, gwh_params &q)
+static void repack_mod(const double *v, int basis, int var, int nmod, gwh_params &q)
 {
     repack(v, basis, q);
     const int base = basis_dim(basis);
-    for (int i = 0; i < nmod; i++) q.ppe_beta[i] = v[base + i];
+    for (int i = 0; i < nmod; i++) {
+        if (var != 3) {
+            q.ppe_beta[i] = v[base + i];
+            continue;
+        }
+        bool first = true;
+        for (int j = 0; j < i; j++) first &= (q.gimr_list[j] != q.gimr_list[i]);
+        double *s = const_cast<double *>(gimr_slot(q, q.gimr_list[i]));
+        if (first && s) *s = v[base + i];
+    }
 }
 
-/* appended parameters of a model's Fisher basis: ppE betas (every point of a batch with the same count);
- * the gIMR deltas are refused (the engine keeps them as index tables, which cannot say which deltas are
- * parameters at a zero value) */
+/* appended parameters of a model's Fisher basis: the ppE betas or the listed gIMR deltas (every point of a
+ * batch with the same count) */
 static int fisher_mods(int model, const gwh_params *params, int n, int *nmod)
 {
     *nmod = 0;
     if (!gwh_model_valid(model)) return gwh_set_error(GWH_ERR_UNSUPPORTED, "unknown model");
     const int var = gwh_ppe_variant(model);
-    if (var == 3)
-        return gwh_set_error(GWH_ERR_UNSUPPORTED,
-                             "Fisher matrices of the gIMR methods (deltas in the basis) are not provided");
     if (var == 0 || n <= 0) return GWH_OK;
-    const int m = params[0].ppe_nmod;
-    if (m < 0 || m > GWH_PPE_MAX) return gwh_set_error(GWH_ERR_UNSUPPORTED, "ppE term count outside 0..4");
-    for (int i = 1; i < n; i++)
-        if (params[i].ppe_nmod != m)
-            return gwh_set_error(GWH_ERR_INVALID, "every point of a ppE Fisher batch needs the same ppE term count");
+    const int m = (var == 3) ? params[0].gimr_nlist : params[0].ppe_nmod;
+    if (var != 3 && (m < 0 || m > GWH_PPE_MAX))
+        return gwh_set_error(GWH_ERR_UNSUPPORTED, "ppE term count outside 0..4");
+    if (var == 3 && (m < 0 || m > GWH_GIMR_LIST_MAX))
+        return gwh_set_error(GWH_ERR_UNSUPPORTED, "at most 16 gIMR deltas in a Fisher basis");
+    for (int i = 0; i < n; i++) {
+        if (((var == 3) ? params[i].gimr_nlist : params[i].ppe_nmod) != m)
+            return gwh_set_error(GWH_ERR_INVALID, "every point of a modified-gravity Fisher batch needs the same "
+                                                  "number of ppE terms / gIMR deltas");
+        if (var == 3)
+            for (int k = 0; k < m; k++)
+                if (params[i].gimr_list[k] < -1 || params[i].gimr_list[k] >= GWH_GIMR_NCODES)
+                    return gwh_set_error(GWH_ERR_INVALID, "bad gIMR Fisher-basis code");
+    }
     *nmod = m;
     return GWH_OK;
 }
@@ -110,13 +143,14 @@ struct PointStencil {
     double c[GWH_FROWS][GWH_MAXSTEN];
 };
 
-static int build_stencil(const gwh_params &p, int basis, int nmod, int order, double eps, int det, int ref_det,
-                         std::vector<gwh_params> &inst, std::vector<unsigned char> &no_tof, PointStencil &ps)
+static int build_stencil(const gwh_params &p, int basis, int var, int nmod, int order, double eps, int det,
+                         int ref_det, std::vector<gwh_params> &inst, std::vector<unsigned char> &no_tof,
+                         PointStencil &ps)
 {
     int dim = basis_dim(basis) + nmod;
     double v[GWH_FROWS];
     int lf[GWH_FROWS];
-    unpack_mod(p, basis, nmod, v, lf);
+    unpack_mod(p, basis, var, nmod, v, lf);
     gwh_params base = non_parameter_options(p);
     ps.S = (order >= 4) ? 4 : 2;
     for (int i = 0; i < GWH_FROWS; i++)
@@ -148,7 +182,7 @@ static int build_stencil(const gwh_params &p, int basis, int nmod, int order, do
             for (int j = 0; j < dim; j++) w[j] = v[j];
             w[i] = v[i] + off[s];
             gwh_params q = base;
-            repack_mod(w, basis, nmod, q);
+            repack_mod(w, basis, var, nmod, q);
             bool two = (S == 4) && (s == 0 || s == 3);
             if (two && det != ref_det) q.tc -= host_dtoa(q.RA, q.DEC, q.gmst, ref_det, det);
             inst.push_back(q);
@@ -449,7 +483,8 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
         inst.reserve((size_t)np * per_point);
         for (int q = 0; q < np; q++) {
             PointStencil ps;
-            build_stencil(params[p0 + q], basis, nmod, order, eps, det_for_tof, ref_det, inst, no_tof, ps);
+            build_stencil(params[p0 + q], basis, gwh_ppe_variant(model), nmod, order, eps, det_for_tof, ref_det,
+                          inst, no_tof, ps);
             if (sky) {
                 inst.push_back(params[p0 + q]); /* central amplitude: the caller's parameters */
                 no_tof.push_back(0);

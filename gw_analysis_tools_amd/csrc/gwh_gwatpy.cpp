@@ -140,6 +140,20 @@ void *gen_params_base_py(double mass1, double mass2, double *spin1, double *spin
     table(Nmod_sigma, sigmai, delta_sigma, 2, 4, p.gimr_dsigma);
     table(Nmod_beta, betai, delta_beta, 2, 3, p.gimr_dbeta);
     table(Nmod_alpha, alphai, delta_alpha, 2, 5, p.gimr_dalpha);
+    /* the lists in order, as the Fisher basis of the gIMR methods (fisher.cpp:2128-2158); an index outside the
+     * modified range is a parameter the waveform does not read (code -1); more than 16 entries: no basis */
+    auto list = [&](int n, const int *idx, int lo, int hi, int code0) {
+        for (int j = 0; j < n && idx; j++) {
+            if (p.gimr_nlist < GWH_GIMR_LIST_MAX)
+                p.gimr_list[p.gimr_nlist] = (int8_t)((idx[j] < lo || idx[j] > hi) ? -1 : code0 + idx[j]);
+            p.gimr_nlist++;
+        }
+    };
+    list(Nmod_phi, phii, -4, 9, GWH_GIMR_CODE_PHI(0));
+    list(Nmod_sigma, sigmai, 2, 4, GWH_GIMR_CODE_SIGMA(0));
+    list(Nmod_beta, betai, 2, 3, GWH_GIMR_CODE_BETA(0));
+    list(Nmod_alpha, alphai, 2, 5, GWH_GIMR_CODE_ALPHA(0));
+    if (p.gimr_nlist > GWH_GIMR_LIST_MAX) p.gimr_nlist = -1;
     if (q->unsupported.empty() && Nmod > 0) {
         p.ppe_nmod = Nmod;
         for (int i = 0; i < Nmod; i++) {
@@ -292,10 +306,17 @@ void repack_parameters_py(double *parameters, void *gen_param, char *generation_
         gwh_report(GWH_ERR_INVALID, "not a parameter handle from gen_params_base_py");
         return;
     }
-    /* a ppE method's betas follow its GR basis (fisher.cpp:2083-2088, :2424-2429) */
+    /* a ppE method's betas, a gIMR method's listed deltas follow its GR basis (fisher.cpp:2083-2088, :2128-2158,
+     * :2424-2429, :2473-2505) */
     const bool ppe = generation_method && strstr(generation_method, "ppE_");
-    const int nmod = ppe ? q->p.ppe_nmod : 0;
+    const bool gimr = generation_method && !strncmp(generation_method, "gIMR", 4);
+    if (gimr && q->p.gimr_nlist < 0) {
+        gwh_report(GWH_ERR_UNSUPPORTED, "at most 16 gIMR deltas in a parameter vector");
+        return;
+    }
+    const int nmod = ppe ? q->p.ppe_nmod : (gimr ? q->p.gimr_nlist : 0);
     std::string gr = generation_method ? generation_method : "";
+    if (gimr) gr.erase(0, 1);
     if (ppe) {
         gr.erase(gr.find("ppE_"), 4);
         for (const char *suf : {"_Inspiral", "_IMR"}) {
@@ -312,7 +333,19 @@ void repack_parameters_py(double *parameters, void *gen_param, char *generation_
         return;
     }
     gwh_repack_parameters(parameters, basis, &q->p);
-    for (int i = 0; i < nmod; i++) q->p.ppe_beta[i] = parameters[dim - nmod + i];
+    for (int i = 0; i < nmod && ppe; i++) q->p.ppe_beta[i] = parameters[dim - nmod + i];
+    /* the waveform reads the first list entry of an index (check_list_id): later repeats are not written */
+    for (int i = 0; i < nmod && gimr; i++) {
+        const int c = q->p.gimr_list[i];
+        bool first = c >= 0;
+        for (int j = 0; j < i; j++) first &= (q->p.gimr_list[j] != c);
+        if (!first) continue;
+        const double v = parameters[dim - nmod + i];
+        if (c < GWH_GIMR_CODE_SIGMA(2)) q->p.gimr_dphi[c] = v;
+        else if (c < GWH_GIMR_CODE_BETA(2)) q->p.gimr_dsigma[c - GWH_GIMR_CODE_SIGMA(2)] = v;
+        else if (c < GWH_GIMR_CODE_ALPHA(2)) q->p.gimr_dbeta[c - GWH_GIMR_CODE_BETA(2)] = v;
+        else q->p.gimr_dalpha[c - GWH_GIMR_CODE_ALPHA(2)] = v;
+    }
 }
 
 double gps_to_GMST_radian_py(double gps) { return gwh_gps_to_gmst_radian(gps); }

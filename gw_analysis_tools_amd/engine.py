@@ -121,7 +121,8 @@ def mcmc_likelihood_extrinsic(f, data, psd, dets, p, model=IMRPHENOMD, T_eff=0.0
 
 
 def fisher_numerical(f, psd, basis, det, ref_det, dim, p, order=2, epsilon=1e-8, model=None):
-    """model None: the GR model of the basis; a ppE model appends p.ppe_nmod betas to the basis"""
+    """model None: the GR model of the basis; a ppE model appends p.ppe_nmod betas to the basis, a gIMR model the
+    p.gimr_nlist listed deltas"""
     f = _f64(f)
     s = _f64(psd)
     out = np.zeros(dim * dim)

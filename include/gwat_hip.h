@@ -46,6 +46,12 @@ extern "C" {
 /* gIMRPhenomD (src/gIMRPhenomD.cpp): IMRPhenomD with the static PN phase coefficients, the sigma / beta /
  * alpha calibration fits and the negative-order inspiral phase terms scaled by (1 + delta) */
 #define GWH_GIMRPHENOMD 4
+#define GWH_GIMR_LIST_MAX 16            /* gIMR deltas in one Fisher basis */
+#define GWH_GIMR_CODE_PHI(i) ((i) + 4)   /* PN order i = -4 .. 9 -> 0 .. 13 (gimr_dphi) */
+#define GWH_GIMR_CODE_SIGMA(i) (12 + (i)) /* sigma_2 .. sigma_4 -> 14 .. 16 (gimr_dsigma) */
+#define GWH_GIMR_CODE_BETA(i) (15 + (i))  /* beta_2, beta_3 -> 17, 18 (gimr_dbeta) */
+#define GWH_GIMR_CODE_ALPHA(i) (17 + (i)) /* alpha_2 .. alpha_5 -> 19 .. 22 (gimr_dalpha) */
+#define GWH_GIMR_NCODES 23
 /* the same three modifications on IMRPhenomPv2 (src/ppE_IMRPhenomP.cpp, src/gIMRPhenomP.cpp): the aligned-spin
  * phase of the precessing model carries the terms ("ppE_IMRPhenomPv2_Inspiral", "ppE_IMRPhenomPv2_IMR",
  * "gIMRPhenomPv2"; dispatch src/waveform_generator.cpp:227-246) */
@@ -104,6 +110,11 @@ typedef struct gwh_params {
     double gimr_dsigma[3]; /* sigma_2 .. sigma_4 */
     double gimr_dbeta[2];  /* beta_2, beta_3 */
     double gimr_dalpha[4]; /* alpha_2 .. alpha_5 */
+    /* the gIMR deltas that are Fisher parameters, in the reference's basis order (the phii list, then sigmai,
+     * betai, alphai: src/fisher.cpp:2128-2158), as codes into the tables above (GWH_GIMR_CODE_*); the first
+     * entry of a repeated code is the one the waveform reads, as check_list_id finds it */
+    int32_t gimr_nlist, gimr_pad;
+    int8_t gimr_list[GWH_GIMR_LIST_MAX];
 } gwh_params;
 
 /* ---- runtime ---- */
@@ -182,10 +193,11 @@ double gwh_data_snr(const double *frequencies, int length, const double *data, c
 int gwh_fisher_derivatives(const double *frequencies, int length, int basis, int detector, int reference_detector,
                            int dim, const gwh_params *params, int order, double epsilon, double *deriv);
 
-/* The same two with the waveform model given (fisher_numerical / calculate_derivatives with a ppE method string,
- * src/fisher.cpp:42-60, :2083-2088, :2424-2429): for GWH_PPE_IMRPHENOMD_INSPIRAL / _IMR and the IMRPhenomPv2
- * ppE ids the basis is the GR basis of `basis` with betappe[0 .. params->ppe_nmod) appended, so
- * dim = basis dimension + ppe_nmod (at most 16 in all).  gIMR models are refused (GWH_ERR_UNSUPPORTED). */
+/* The same two with the waveform model given (fisher_numerical / calculate_derivatives with a ppE or gIMR method
+ * string, src/fisher.cpp:42-60, :2083-2088, :2128-2158, :2424-2429, :2473-2505): for the ppE ids the basis is
+ * the GR basis of `basis` with betappe[0 .. params->ppe_nmod) appended, dim = basis dimension + ppe_nmod; for
+ * the gIMR ids the deltas named by params->gimr_list[0 .. gimr_nlist) are appended, dim = basis dimension +
+ * gimr_nlist; at most 16 in all.  Every point of a batch needs the same appended count. */
 int gwh_fisher_numerical_model(const double *frequencies, int length, int model, int basis, int detector,
                                int reference_detector, double *out, int dim, const gwh_params *params, int order,
                                const double *psd, double epsilon);
@@ -249,8 +261,9 @@ int gwh_autotune_bins_per_lane(const int *events, const int *d_events, int model
 /* Tuning of the bins-per-lane of the likelihood kernels (1, 2, 4 or 8; 0 = automatic). */
 int gwh_set_bins_per_lane(int bpl);
 
-/* Batched network Fisher: n points, order 2 central differences in the given basis, per-detector
- * projection summed over the event's detectors.  out: n*dim*dim (host).  precision GWH_FP64 uses an
+/* Batched network Fisher: n points, order 2 central differences in the given basis (with the ppE betas or
+ * listed gIMR deltas appended for those models, as gwh_fisher_numerical_model), per-detector projection
+ * summed over the event's detectors.  out: n*dim*dim (host).  precision GWH_FP64 uses an
  * FP64 matrix-core Gram, GWH_MIXED an FP32 matrix-core Gram with FP64 accumulation across tiles;
  * the stencil numerators are FP64 in both.  sigma (optional, n*dim): marginal standard deviations
  * from the FP64 inverse of each Fisher matrix. */

@@ -55,14 +55,28 @@ inline void gimr_table(int n, const int *idx, const double *delta, int lo, int h
     }
 }
 
+/* a gIMR index list appended to the Fisher-basis codes (an index outside the modified range is a parameter the
+ * waveform does not read: code -1); the count keeps growing past GWH_GIMR_LIST_MAX so that the caller sees it */
+inline void gimr_list(int n, const int *idx, int lo, int hi, int code0, gwh_params &p)
+{
+    for (int j = 0; j < n && idx; j++) {
+        if (p.gimr_nlist < GWH_GIMR_LIST_MAX)
+            p.gimr_list[p.gimr_nlist] = (int8_t)((idx[j] < lo || idx[j] > hi) ? -1 : code0 + idx[j]);
+        p.gimr_nlist++;
+    }
+}
+
 /* "SIMPSONS" / "GAUSSLEG" (mcmc_gw.cpp:811-856) */
 inline int integration_id(const std::string &method) { return method == "GAUSSLEG" ? GWH_GAUSSLEG : GWH_SIMPSONS; }
 
-/* the GR method under a ppE method string: "ppE_IMRPhenomD_Inspiral" -> "IMRPhenomD",
- * "MCMC_ppE_IMRPhenomPv2_IMR" -> "MCMC_IMRPhenomPv2" (fisher.cpp:2083-2088 appends the betas to that basis) */
+/* the GR method under a ppE or gIMR method string: "ppE_IMRPhenomD_Inspiral" -> "IMRPhenomD",
+ * "MCMC_ppE_IMRPhenomPv2_IMR" -> "MCMC_IMRPhenomPv2", "gIMRPhenomD" -> "IMRPhenomD" (fisher.cpp:2083-2088,
+ * :2128-2158 append the betas / deltas to that basis) */
 inline std::string fisher_gr_method(const std::string &method)
 {
     std::string m = method;
+    const size_t g = m.find("gIMR");
+    if (g != std::string::npos) return m.erase(g, 1);
     const size_t p = m.find("ppE_");
     if (p == std::string::npos) return m;
     m.erase(p, 4);
@@ -76,8 +90,8 @@ inline std::string fisher_gr_method(const std::string &method)
     return m;
 }
 
-/* Fisher / MCMC bases of fisher_numerical's method strings (fisher.cpp:1841-2161); a ppE method's betas are
- * appended to the basis of its GR method */
+/* Fisher / MCMC bases of fisher_numerical's method strings (fisher.cpp:1841-2161); a ppE method's betas and a
+ * gIMR method's listed deltas are appended to the basis of its GR method */
 inline int basis_id(const std::string &method_in, bool sky_average)
 {
     const std::string method = fisher_gr_method(method_in);
@@ -131,6 +145,12 @@ inline bool to_params(const Params &g, gwh_params &p, int model = GWH_IMRPHENOMD
         gimr_table(g.Nmod_sigma, g.sigmai, g.delta_sigma, 2, 4, p.gimr_dsigma);
         gimr_table(g.Nmod_beta, g.betai, g.delta_beta, 2, 3, p.gimr_dbeta);
         gimr_table(g.Nmod_alpha, g.alphai, g.delta_alpha, 2, 5, p.gimr_dalpha);
+        /* the lists in order, as the Fisher basis of the gIMR methods (fisher.cpp:2128-2158) */
+        gimr_list(g.Nmod_phi, g.phii, -4, 9, GWH_GIMR_CODE_PHI(0), p);
+        gimr_list(g.Nmod_sigma, g.sigmai, 2, 4, GWH_GIMR_CODE_SIGMA(0), p);
+        gimr_list(g.Nmod_beta, g.betai, 2, 3, GWH_GIMR_CODE_BETA(0), p);
+        gimr_list(g.Nmod_alpha, g.alphai, 2, 5, GWH_GIMR_CODE_ALPHA(0), p);
+        if (p.gimr_nlist > GWH_GIMR_LIST_MAX) p.gimr_nlist = -1; /* more than the record holds: no Fisher basis */
     }
     return !(g.equatorial_orientation || g.horizon_coord || g.NSflag1 || g.NSflag2);
 }
@@ -254,6 +274,13 @@ inline void repack_parameters(const double *v, Params *g, const std::string &met
     from_params(p, *g);
     if (method.find("ppE") != std::string::npos) /* the betas follow the GR basis (fisher.cpp:2424-2429) */
         for (int i = 0, base = dim - g->Nmod; i < g->Nmod; i++) g->betappe[i] = v[base + i];
+    if (method.find("gIMR") != std::string::npos) { /* the deltas in list order (fisher.cpp:2473-2505) */
+        int k = dim - (g->Nmod_phi + g->Nmod_sigma + g->Nmod_beta + g->Nmod_alpha);
+        for (int i = 0; i < g->Nmod_phi; i++) g->delta_phi[i] = v[k++];
+        for (int i = 0; i < g->Nmod_sigma; i++) g->delta_sigma[i] = v[k++];
+        for (int i = 0; i < g->Nmod_beta; i++) g->delta_beta[i] = v[k++];
+        for (int i = 0; i < g->Nmod_alpha; i++) g->delta_alpha[i] = v[k++];
+    }
 }
 
 } // namespace gwh

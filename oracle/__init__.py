@@ -39,7 +39,8 @@ class Params(ctypes.Structure):
                 ("ppe_nmod", ctypes.c_int), ("ppe_pad", ctypes.c_int),
                 ("ppe_b", ctypes.c_double * 4), ("ppe_beta", ctypes.c_double * 4),
                 ("gimr_dphi", ctypes.c_double * 14), ("gimr_dsigma", ctypes.c_double * 3),
-                ("gimr_dbeta", ctypes.c_double * 2), ("gimr_dalpha", ctypes.c_double * 4)]
+                ("gimr_dbeta", ctypes.c_double * 2), ("gimr_dalpha", ctypes.c_double * 4),
+                ("gimr_nlist", ctypes.c_int), ("gimr_pad", ctypes.c_int), ("gimr_list", ctypes.c_int8 * 16)]
 
 
 def make_params(**kw):
@@ -49,7 +50,8 @@ def make_params(**kw):
     p.shift_time = 1
     p.shift_phase = 1
     for k, v in kw.items():
-        if k in ("spin1", "spin2", "ppe_b", "ppe_beta", "gimr_dphi", "gimr_dsigma", "gimr_dbeta", "gimr_dalpha"):
+        if k in ("spin1", "spin2", "ppe_b", "ppe_beta", "gimr_dphi", "gimr_dsigma", "gimr_dbeta", "gimr_dalpha",
+                 "gimr_list"):
             getattr(p, k)[:len(v)] = list(v)
         else:
             setattr(p, k, v)

@@ -55,6 +55,10 @@ typedef struct {
     int ppe_nmod, ppe_pad;        /* ppE terms: Nmod, bppe, betappe (util.h:249-255) */
     double ppe_b[4], ppe_beta[4];
     double gimr_dphi[14], gimr_dsigma[3], gimr_dbeta[2], gimr_dalpha[4]; /* gIMR deltas (0: unmodified) */
+    /* the gIMR deltas in a Fisher basis, in the reference's list order (phii, sigmai, betai, alphai), as codes:
+     * PN order i -> i + 4, sigma_i -> 12 + i, beta_i -> 15 + i, alpha_i -> 17 + i */
+    int gimr_nlist, gimr_pad;
+    signed char gimr_list[16];
 } gwo_params;
 
 void gwo_default_params(gwo_params *p);

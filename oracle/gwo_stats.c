@@ -215,10 +215,11 @@ static void local_method(const char *m, char *out)
     }
 }
 
-/* the waveform model of a (local) generation method; the ppE methods keep their variant */
+/* the waveform model of a (local) generation method; the ppE and gIMR methods keep their variant */
 static int model_of(const char *m)
 {
     const int pv2 = has(m, "IMRPhenomPv2");
+    if (has(m, "gIMR")) return pv2 ? GWO_GIMRPHENOMPV2 : GWO_GIMRPHENOMD;
     if (has(m, "ppE")) {
         const int ins = has(m, "Inspiral");
         return pv2 ? (ins ? GWO_PPE_IMRPHENOMPV2_INSPIRAL : GWO_PPE_IMRPHENOMPV2_IMR)
@@ -245,11 +246,32 @@ static void sph_cart(const double *s, double *c)
 static int unpack_gr(double *v, int *lf, const gwo_params *p, const char *m, int dim);
 static int repack_gr(const double *v, gwo_params *a, const char *m);
 
-/* the ppE methods append betappe[0..Nmod) to the GR basis (fisher.cpp:2083-2088, :2424-2429) */
+/* the table entry of a gIMR list code: 0..13 delta_phi of PN order code - 4, 14..16 delta_sigma 2..4,
+ * 17..18 delta_beta 2..3, 19..22 delta_alpha 2..5; NULL (-1): an index the waveform does not read */
+static double *gimr_slot(gwo_params *p, int code)
+{
+    if (code < 0 || code > 22) return NULL;
+    if (code < 14) return &p->gimr_dphi[code];
+    if (code < 17) return &p->gimr_dsigma[code - 14];
+    if (code < 19) return &p->gimr_dbeta[code - 17];
+    return &p->gimr_dalpha[code - 19];
+}
+
+/* the ppE methods append betappe[0..Nmod) to the GR basis (fisher.cpp:2083-2088, :2424-2429); the gIMR methods
+ * append delta_phi, delta_sigma, delta_beta, delta_alpha in list order (:2128-2158, :2473-2505). The deltas are
+ * held here as tables indexed by PN order / coefficient; the waveform reads the first list entry of an index
+ * (check_list_id), so a repeated entry writes the table only through its first occurrence. */
 int gwo_unpack_parameters(double *v, int *lf, const gwo_params *p, const char *m, int dim)
 {
     if (!unpack_gr(v, lf, p, m, dim)) return 0;
-    if (has(m, "ppE")) {
+    if (has(m, "gIMR")) {
+        gwo_params q = *p;
+        const int base = dim - p->gimr_nlist;
+        for (int i = 0; i < p->gimr_nlist; i++) {
+            const double *s = gimr_slot(&q, p->gimr_list[i]);
+            v[base + i] = s ? *s : 0.0;
+        }
+    } else if (has(m, "ppE")) {
         const int base = dim - p->ppe_nmod;
         for (int i = 0; i < p->ppe_nmod; i++) v[base + i] = p->ppe_beta[i];
     }
@@ -260,7 +282,16 @@ int gwo_repack_parameters(const double *v, gwo_params *a, const char *m, int dim
 {
     (void)orig;
     if (!repack_gr(v, a, m)) return 0;
-    if (has(m, "ppE")) {
+    if (has(m, "gIMR")) {
+        const int base = dim - a->gimr_nlist;
+        for (int i = 0; i < a->gimr_nlist; i++) {
+            int first = 1;
+            for (int j = 0; j < i; j++)
+                if (a->gimr_list[j] == a->gimr_list[i]) first = 0;
+            double *s = gimr_slot(a, a->gimr_list[i]);
+            if (first && s) *s = v[base + i];
+        }
+    } else if (has(m, "ppE")) {
         const int base = dim - a->ppe_nmod;
         for (int i = 0; i < a->ppe_nmod; i++) a->ppe_beta[i] = v[base + i];
     }
@@ -446,12 +477,18 @@ static void repack_non_parameter_options(gwo_params *w, const gwo_params *in)
     w->shift_time = 0;
     w->shift_phase = in->shift_phase;
     w->dep_postmerger = in->dep_postmerger;
-    /* fisher.cpp:2541-2551: the ppE terms travel with the parameters */
+    /* fisher.cpp:2541-2570: the ppE terms and the gIMR index lists travel with the parameters */
     w->ppe_nmod = in->ppe_nmod;
     for (int i = 0; i < 4; i++) {
         w->ppe_b[i] = in->ppe_b[i];
         w->ppe_beta[i] = in->ppe_beta[i];
     }
+    memcpy(w->gimr_dphi, in->gimr_dphi, sizeof(w->gimr_dphi));
+    memcpy(w->gimr_dsigma, in->gimr_dsigma, sizeof(w->gimr_dsigma));
+    memcpy(w->gimr_dbeta, in->gimr_dbeta, sizeof(w->gimr_dbeta));
+    memcpy(w->gimr_dalpha, in->gimr_dalpha, sizeof(w->gimr_dalpha));
+    w->gimr_nlist = in->gimr_nlist;
+    memcpy(w->gimr_list, in->gimr_list, sizeof(w->gimr_list));
 }
 
 static void add_tof_phase(double *r, const double *f, int n, double dt_phase)

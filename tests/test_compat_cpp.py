@@ -79,6 +79,18 @@ int main()
     v[6] += 0.1; /* ln DL */
     gwh::repack_parameters(v, &g, "MCMC_IMRPhenomD", 11);
     printf("repack %.17g %.17g\n", g.Luminosity_Distance, g.mass1);
+    /* the gIMR lists as a Fisher basis: codes in list order (-1: an index the model does not modify), and a
+     * basis vector written back into the caller's delta arrays entry by entry */
+    g.Nmod_phi = 4;
+    g.Nmod_alpha = 1;
+    gwh::to_params(g, p, GWH_GIMRPHENOMD);
+    printf("gimrlist %d %d %d %d %d %d %d\n", p.gimr_nlist, p.gimr_list[0], p.gimr_list[1], p.gimr_list[2],
+           p.gimr_list[3], p.gimr_list[4], gwh::basis_id("MCMC_gIMRPhenomD", false));
+    double w[16];
+    gwh_unpack_parameters(&p, GWH_BASIS_PHENOMD_FULL, w, nullptr);
+    for (int i = 0; i < 5; i++) w[11 + i] = 0.5 + 0.1 * i;
+    gwh::repack_parameters(w, &g, "gIMRPhenomD", 16);
+    printf("gimrrepack %g %g %g %g %g\n", dph[0], dph[1], dph[2], dph[3], dal[0]);
     return 0;
 }
 '''
@@ -108,3 +120,6 @@ def test_compat_header_with_reference_shaped_params(tmp_path):
                                              G.BASIS_PV2_FULL, G.BASIS_PV2_MCMC)
     r = out[4].split()
     assert abs(float(r[1]) - 410 * 2.718281828459045 ** 0.1) < 1e-9 and abs(float(r[2]) - 36) < 1e-9
+    # phi 2, -3, 2, 12 (out of range), alpha 5 -> codes 6, 1, 6, -1, 22
+    assert out[5].split() == ["gimrlist", "5", "6", "1", "6", "-1", "22", str(G.BASIS_PHENOMD_MCMC)]
+    assert out[6].split() == ["gimrrepack", "0.5", "0.6", "0.7", "0.8", "0.9"]

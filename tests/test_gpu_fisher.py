@@ -149,10 +149,7 @@ def _points(model, n):
 
 
 def _oracle_params(rec):
-    kw = {k: (list(rec[k]) if rec[k].shape else float(rec[k])) for k in rec.dtype.names}
-    for k in ("shift_time", "shift_phase", "sky_average", "dep_postmerger", "ppe_nmod", "ppe_pad"):
-        kw[k] = int(rec[k])
-    return O.make_params(**kw)
+    return O.make_params(**{k: (list(rec[k]) if rec[k].shape else rec[k].item()) for k in rec.dtype.names})
 
 
 @pytest.mark.parametrize("basis,method,dim,model", [(G.BASIS_PHENOMD_MCMC, "MCMC_IMRPhenomD", 11, G.IMRPHENOMD),

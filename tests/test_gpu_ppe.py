@@ -119,10 +119,19 @@ def test_ppe_refusals_and_gwatpy():
     with pytest.raises(G.GWHError) as e:
         G.fisher_batch(ev, P_, G.PPE_IMRPHENOMD_IMR, G.BASIS_PHENOMD_FULL, 11)
     assert "ppE" in str(e.value)
-    # the gIMR deltas are not basis parameters here
+    # gIMR bases: more deltas than the record holds, unequal counts in a batch, a code outside the tables
+    Q = P_.copy()
+    Q["gimr_nlist"] = -1
     with pytest.raises(G.GWHError) as e:
-        G.fisher_batch(ev, P_, G.GIMRPHENOMD, G.BASIS_PHENOMD_FULL, 11)
+        G.fisher_batch(ev, Q, G.GIMRPHENOMD, G.BASIS_PHENOMD_FULL, 11)
     assert "gIMR" in str(e.value)
+    Q["gimr_nlist"] = [1, 2]
+    with pytest.raises(G.GWHError):
+        G.fisher_batch(ev, Q, G.GIMRPHENOMD, G.BASIS_PHENOMD_FULL, 12)
+    Q["gimr_nlist"] = 1
+    Q["gimr_list"][:, 0] = 23
+    with pytest.raises(G.GWHError):
+        G.fisher_batch(ev, Q, G.GIMRPHENOMD, G.BASIS_PHENOMD_FULL, 12)
     ev.close()
     # gwatpy handles carry Nmod / bppe / betappe to the ppE methods
     p = P.gen_params(mass1=36, mass2=29, spin1=(0, 0, .3), spin2=(0, 0, -.1), Luminosity_Distance=410.,
@@ -190,6 +199,17 @@ def test_gimr_through_gwatpy_lists():
     o = O.make_params(**CASES[0], **EXTR, gimr_dphi=dphi, gimr_dbeta=[0., 0.1])
     op, oc = O.fourier_waveform(f, o, O.GIMRPHENOMD)
     wf_close(hp, op)
+    # a gIMR parameter vector (fisher.cpp:2473-2505) written into the handle: the deltas follow the GR basis in
+    # list order (phi 2, phi -2, beta 3)
+    v, _ = O.unpack_parameters(o, "IMRPhenomD", 11)
+    v = np.concatenate([v, [0.07, -2e-4, 0.2]])
+    P.repack_parameters(v, p, "gIMRPhenomD", 14)
+    o2 = O.make_params(**{**CASES[0], **EXTR, "gimr_nlist": 3, "gimr_list": [6, 2, 18]})
+    o2 = O.repack_parameters(v, "gIMRPhenomD", 14, o2)
+    assert o2.gimr_dphi[6] == 0.07 and o2.gimr_dphi[2] == -2e-4 and o2.gimr_dbeta[1] == 0.2
+    st, hp, hc = P.fourier_waveform(f, "gIMRPhenomD", p)
+    assert st == G.GWH_OK
+    wf_close(hp, O.fourier_waveform(f, o2, O.GIMRPHENOMD)[0])
     p.close()
 
 
@@ -299,6 +319,42 @@ def test_ppe_fisher_vs_oracle(model, basis, method, gdim, sky, terms, order):
     e, ed = _fnorm(a, b)
     assert e < FISHER_TOL and ed < FISHER_TOL, (e, ed)
     assert np.allclose(a, a.T, rtol=0, atol=0)
+
+
+# gIMR bases: phi orders -2 and 2, sigma_3, beta_2, alpha_4 (GWH_GIMR_CODE_*), at nonzero deltas; a repeat and an
+# unread index (-1) give zero rows
+GIMR_FISHER = dict(gimr_nlist=5, gimr_list=[2, 6, 15, 17, 21], gimr_dphi=[0, 0, 2e-4] + [0] * 3 + [0.05] + [0] * 7,
+                   gimr_dsigma=[0, 0.1, 0], gimr_dbeta=[-0.05, 0], gimr_dalpha=[0, 0, 0.1, 0])
+GIMR_FISHER_CASES = [  # (model, engine basis, oracle method, GR dim, sky, list (None: the 5 above), zero rows)
+    (G.GIMRPHENOMD, G.BASIS_PHENOMD_FULL, "gIMRPhenomD", 11, 0, None, []),
+    (G.GIMRPHENOMD, G.BASIS_PHENOMD_SKYAVG, "gIMRPhenomD", 7, 1, None, []),
+    (G.GIMRPHENOMD, G.BASIS_PHENOMD_MCMC, "MCMC_gIMRPhenomD", 11, 0, [6, -1, 6, 21], [12, 13]),
+    (G.GIMRPHENOMPV2, G.BASIS_PV2_FULL, "gIMRPhenomPv2", 13, 0, [6, 15, 21], []),  # 16 parameters at most
+]
+
+
+@pytest.mark.parametrize("model,basis,method,gdim,sky,lst,zero", GIMR_FISHER_CASES)
+def test_gimr_fisher_vs_oracle(model, basis, method, gdim, sky, lst, zero):
+    f = W.uniform_grid(20., 1024., 0.25)
+    psd = W.analytic_psd(f)
+    kw = dict(mass1=36., mass2=29., spin1=[0, 0, 0.3], spin2=[0, 0, -0.1], DL=410., tc=0., phiRef=0., f_ref=20.,
+              RA=1.37, DEC=-1.26, psi=2.66, incl=2.9, gmst=W.GMST, sky_average=sky, **GIMR_FISHER)
+    if lst is not None:
+        kw.update(gimr_nlist=len(lst), gimr_list=lst)
+    if gwh_pv2(model):
+        kw.update(spin1=[0.3, 0.1, 0.3], spin2=[-0.1, 0.2, -0.1], chip=0.35, phip=0.8)
+    dim = gdim + kw["gimr_nlist"]
+    gp, op = G.make_params(**kw), O.make_params(**kw)
+    a = G.fisher_numerical(f, psd, basis, 0, 0, dim, gp, order=2, epsilon=EPS, model=model)
+    b = O.fisher_numerical(f, psd, method, 0, 0, dim, op, order=2, epsilon=EPS)
+    # the zero rows of a repeat / an unread index: the two stencil waveforms are equal, and the fused
+    # multiply-adds of the stencil sum (c h - c h) leave only the rounding of c h
+    keep = np.diag(b) != 0
+    assert np.all(b[~keep] == 0) and np.max(np.abs(a[~keep]), initial=0) <= 1e-12 * np.max(np.diag(b))
+    e, ed = _fnorm(a[np.ix_(keep, keep)], b[np.ix_(keep, keep)])
+    assert e < FISHER_TOL and ed < FISHER_TOL, (e, ed)
+    assert np.allclose(a, a.T, rtol=0, atol=0)
+    assert list(np.flatnonzero(~keep)) == zero
 
 
 def gwh_pv2(model):

@@ -32,10 +32,7 @@ def _events(n_events=16, chains=3):
 
 
 def _oracle_params(rec):
-    kw = {k: (list(rec[k]) if rec[k].shape else float(rec[k])) for k in rec.dtype.names}
-    for k in ("shift_time", "shift_phase", "sky_average", "dep_postmerger", "ppe_nmod", "ppe_pad"):
-        kw[k] = int(rec[k])
-    return O.make_params(**kw)
+    return O.make_params(**{k: (list(rec[k]) if rec[k].shape else rec[k].item()) for k in rec.dtype.names})
 
 
 def test_uneven_batch_equals_single_event_batches_and_oracle():

@@ -537,3 +537,72 @@ def test_ppe_fisher_beta_beta_element_closed_form():
     assert abs(Fi[11, 11] - ref) <= 1e-6 * ref, (Fi[11, 11], ref)
     # the other variant's closed form does not fit
     assert abs(Fi[11, 11] - F[11, 11]) > 1e-3 * ref
+
+
+# ---- Fisher bases of the gIMR methods (fisher.cpp:2128-2158, :2473-2505: the listed deltas appended) ----------
+def _gimr_fisher_case(codes, **tables):
+    kw, f, psd, _, mc = _ppe_fisher_case()
+    p = O.make_params(gimr_nlist=len(codes), gimr_list=codes, **tables, **kw)
+    return kw, f, psd, p
+
+
+# phi orders -2 and 2, sigma_3, beta_2, alpha_4 (codes: phi i -> i + 4, sigma i -> 12 + i, beta i -> 15 + i,
+# alpha i -> 17 + i)
+GIMR_CODES = [2, 6, 15, 17, 21]
+
+
+def test_gimr_fisher_gr_block_at_zero_deltas_is_the_gr_fisher():
+    """at zero deltas the gIMR waveform and its GR-parameter perturbations equal IMRPhenomD's: the first 11 x 11
+    block of the gIMR Fisher is the GR Fisher, and every delta row has a positive diagonal"""
+    kw, f, psd, p = _gimr_fisher_case(GIMR_CODES)
+    g = O.fisher_numerical(f, psd, "IMRPhenomD", 0, 0, 11, O.make_params(**kw))
+    F = O.fisher_numerical(f, psd, "gIMRPhenomD", 0, 0, 11 + len(GIMR_CODES), p)
+    assert np.allclose(F[:11, :11], g, rtol=1e-12, atol=0)
+    assert np.all(np.diag(F)[11:] > 0)
+
+
+@pytest.mark.parametrize("order", [-2, 0, 2])
+def test_gimr_fisher_phase_delta_is_the_scaled_ppe_beta(order):
+    """delta_phi of PN order i is the ppE inspiral beta of b = i - 5 scaled by c = (3/128) eta^(-i/5) phi_i
+    (gIMRPhenomD.cpp:24-52): at zero, the delta row of the gIMR Fisher is c times the beta row of the
+    ppE_IMRPhenomD_Inspiral Fisher, and the delta-delta element c^2 times the beta-beta element. A code that
+    addresses the wrong table entry, or a repack into the wrong order, breaks it."""
+    kw, f, psd, p = _gimr_fisher_case([order + 4])
+    F = O.fisher_numerical(f, psd, "gIMRPhenomD", 0, 0, 12, p)
+    pp = O.make_params(ppe_nmod=1, ppe_b=[order - 5.], ppe_beta=[0.], **kw)
+    Fp = O.fisher_numerical(f, psd, "ppE_IMRPhenomD_Inspiral", 0, 0, 12, pp)
+    eta = 36. * 29. / 65. ** 2
+    coeff = 1.0 if order < 0 else O.pn_phase_coefficients(O.make_params(**kw))[order]
+    c = 3. / 128. * eta ** (-order / 5.) * coeff
+    assert abs(F[11, 11] - c * c * Fp[11, 11]) <= 1e-6 * F[11, 11], (F[11, 11], c * c * Fp[11, 11])
+    scale = np.sqrt(F[11, 11] * np.diag(F)[:11])
+    assert np.max(np.abs(F[11, :11] - c * Fp[11, :11]) / scale) < 1e-6
+
+
+@pytest.mark.parametrize("code", [15, 17, 21])
+def test_gimr_fisher_fit_delta_against_a_direct_difference(code):
+    """the delta-delta element of a fit delta (sigma_3, beta_2, alpha_4) against 4 sum |dh/ddelta|^2 / S with
+    dh/ddelta a central difference of the waveform taken here, with the table entry the code names set directly
+    (no time shift, as the Fisher's fresh record)"""
+    kw, f, psd, p = _gimr_fisher_case([code])
+    F = O.fisher_numerical(f, psd, "gIMRPhenomD", 0, 0, 12, p)
+    field, idx, n = {15: ("gimr_dsigma", 1, 3), 17: ("gimr_dbeta", 0, 2), 21: ("gimr_dalpha", 2, 4)}[code]
+    eps = 1e-6
+
+    def resp(d):
+        t = [0.] * n
+        t[idx] = d
+        return O.detector_response(f, O.make_params(**{**kw, field: t, "shift_time": 0}), 0, O.GIMRPHENOMD)
+
+    dh = (resp(eps) - resp(-eps)) / (2 * eps)
+    ref = 4 * O.simpsons_sum(f[1] - f[0], np.abs(dh) ** 2 / psd)
+    assert abs(F[11, 11] - ref) <= 1e-5 * ref, (F[11, 11], ref)
+
+
+def test_gimr_fisher_repeats_and_unread_indices_are_zero_rows():
+    """the waveform reads the first list entry of an index (check_list_id), so a repeated entry and an index the
+    model does not modify (code -1) are parameters the waveform does not depend on: zero rows and columns"""
+    kw, f, psd, p = _gimr_fisher_case([6, -1, 6])
+    F = O.fisher_numerical(f, psd, "gIMRPhenomD", 0, 0, 14, p)
+    assert F[11, 11] > 0
+    assert np.all(F[12] == 0) and np.all(F[13] == 0) and np.all(F[:, 12] == 0) and np.all(F[:, 13] == 0)
