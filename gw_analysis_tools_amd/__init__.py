@@ -24,6 +24,11 @@ GIMRPHENOMD = 4
 PPE_IMRPHENOMPV2_INSPIRAL = 5
 PPE_IMRPHENOMPV2_IMR = 6
 GIMRPHENOMPV2 = 7
+# theories mapped onto the ppE models (ppe_theory; ppE_utilities.cpp:158-360) and the redshift cosmologies
+(THEORY_NONE, THEORY_DCS, THEORY_EDGB, THEORY_EDGB_HO_LO, THEORY_EDGB_GHOV1, THEORY_EDGB_GHOV2, THEORY_EDGB_GHOV3,
+ THEORY_EXTRADIMENSION, THEORY_BHEVAPORATION, THEORY_TVG, THEORY_DIPRAD, THEORY_NONCOMM, THEORY_PNSERIES,
+ THEORY_PPEALT) = range(14)
+COSMO_PLANCK15, COSMO_PLANCK13, COSMO_WMAP9, COSMO_WMAP7, COSMO_WMAP5 = range(5)
 FP64 = 0
 MIXED = 1
 SIMPSONS = 0
@@ -74,11 +79,11 @@ class Params(ctypes.Structure):
                 ("chip", ctypes.c_double), ("phip", ctypes.c_double),
                 ("shift_time", ctypes.c_int32), ("shift_phase", ctypes.c_int32),
                 ("sky_average", ctypes.c_int32), ("dep_postmerger", ctypes.c_int32),
-                ("ppe_nmod", ctypes.c_int32), ("ppe_pad", ctypes.c_int32),
+                ("ppe_nmod", ctypes.c_int32), ("ppe_theory", ctypes.c_int32),
                 ("ppe_b", ctypes.c_double * 4), ("ppe_beta", ctypes.c_double * 4),
                 ("gimr_dphi", ctypes.c_double * 14), ("gimr_dsigma", ctypes.c_double * 3),
                 ("gimr_dbeta", ctypes.c_double * 2), ("gimr_dalpha", ctypes.c_double * 4),
-                ("gimr_nlist", ctypes.c_int32), ("gimr_pad", ctypes.c_int32), ("gimr_list", ctypes.c_int8 * 16)]
+                ("gimr_nlist", ctypes.c_int32), ("cosmology", ctypes.c_int32), ("gimr_list", ctypes.c_int8 * 16)]
 
 
 class PTConfig(ctypes.Structure):
@@ -93,10 +98,10 @@ PARAMS_DTYPE = np.dtype([("mass1", "f8"), ("mass2", "f8"), ("spin1", "f8", 3), (
                          ("tc", "f8"), ("phiRef", "f8"), ("f_ref", "f8"), ("incl", "f8"), ("RA", "f8"),
                          ("DEC", "f8"), ("psi", "f8"), ("gmst", "f8"), ("chip", "f8"), ("phip", "f8"),
                          ("shift_time", "i4"), ("shift_phase", "i4"), ("sky_average", "i4"),
-                         ("dep_postmerger", "i4"), ("ppe_nmod", "i4"), ("ppe_pad", "i4"), ("ppe_b", "f8", 4),
+                         ("dep_postmerger", "i4"), ("ppe_nmod", "i4"), ("ppe_theory", "i4"), ("ppe_b", "f8", 4),
                          ("ppe_beta", "f8", 4), ("gimr_dphi", "f8", 14), ("gimr_dsigma", "f8", 3),
                          ("gimr_dbeta", "f8", 2), ("gimr_dalpha", "f8", 4), ("gimr_nlist", "i4"),
-                         ("gimr_pad", "i4"), ("gimr_list", "i1", 16)])
+                         ("cosmology", "i4"), ("gimr_list", "i1", 16)])
 assert PARAMS_DTYPE.itemsize == ctypes.sizeof(Params) == 448
 
 

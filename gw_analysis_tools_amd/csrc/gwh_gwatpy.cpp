@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "gwat_hip.h"
+#include "gwat_hip_compat.hpp"
 #include "gwatpy_abi.h"
 
 /* the engine's error recorder (gwh_engine.hip) */
@@ -59,7 +60,19 @@ int model_of(const char *method)
     if (!strcmp(method, "ppE_IMRPhenomPv2_Inspiral")) return GWH_PPE_IMRPHENOMPV2_INSPIRAL;
     if (!strcmp(method, "ppE_IMRPhenomPv2_IMR")) return GWH_PPE_IMRPHENOMPV2_IMR;
     if (!strcmp(method, "gIMRPhenomPv2")) return GWH_GIMRPHENOMPV2;
+    int m;
+    if (gwh::theory_id(method, &m) > 0) return m; /* a theory rides on the ppE method it maps to */
     return -1;
+}
+
+/* the handle's record as a method string reads it: a theory string maps the ppE fields as that theory's
+ * parameters (prep_source_parameters, waveform_generator.cpp:1379-1404) */
+gwh_params for_method(const gwh_params &p, const char *method)
+{
+    gwh_params r = p;
+    const int th = method ? gwh::theory_id(method) : 0;
+    r.ppe_theory = th > 0 ? th : GWH_THEORY_NONE;
+    return r;
 }
 
 /* Fisher / MCMC basis of a generation method string and dimension (fisher.cpp:1841-2161) */
@@ -90,7 +103,7 @@ void *gen_params_base_py(double mass1, double mass2, double *spin1, double *spin
                          int *sigmai, int *betai, int *alphai, double *delta_phi, double *delta_sigma,
                          double *delta_beta, double *delta_alpha, int Nmod, double *bppe, double *betappe)
 {
-    (void)theta_l, (void)phi_l, (void)theta, (void)phi, (void)cosmology, (void)LISA_alpha0, (void)LISA_phi0;
+    (void)theta_l, (void)phi_l, (void)theta, (void)phi, (void)LISA_alpha0, (void)LISA_phi0;
 
     PyParams *q = new PyParams;
     q->magic = kMagic;
@@ -122,8 +135,8 @@ void *gen_params_base_py(double mass1, double mass2, double *spin1, double *spin
     else if (NSflag1 || NSflag2) q->unsupported = "NSflag1/NSflag2";
     else if (Nmod > GWH_PPE_MAX)
         q->unsupported = "more than " + std::to_string(GWH_PPE_MAX) + " ppE terms";
-    else if (Nmod > 0 && (!bppe || !betappe))
-        q->unsupported = "ppE terms without bppe / betappe arrays";
+    else if (Nmod > 0 && !betappe)
+        q->unsupported = "ppE terms without a betappe array"; /* bppe may be absent: the theories fix it */
     /* ppE terms are read by the ppE methods only, gIMR terms by the gIMR methods only; the other methods ignore
      * them, as prep_source_parameters does (waveform_generator.cpp:1296-1313).  gIMR index lists become
      * index -> delta tables (the first entry of an index wins, as check_list_id finds it). */
@@ -157,10 +170,12 @@ void *gen_params_base_py(double mass1, double mass2, double *spin1, double *spin
     if (q->unsupported.empty() && Nmod > 0) {
         p.ppe_nmod = Nmod;
         for (int i = 0; i < Nmod; i++) {
-            p.ppe_b[i] = bppe[i];
+            p.ppe_b[i] = bppe ? bppe[i] : 0.0; /* a theory's parameters come without exponents */
             p.ppe_beta[i] = betappe[i];
         }
     }
+    /* the redshift of the theory maps (gen_params_base::cosmology); unknown names leave a theory unmappable */
+    p.cosmology = cosmology ? gwh::cosmology_id(cosmology) : GWH_COSMO_PLANCK15;
     std::lock_guard<std::mutex> lk(g_handles_mu);
     g_handles.insert(q);
     return q;
@@ -196,7 +211,8 @@ int fourier_waveform_py(double *frequencies, int length, double *wf_plus_real, d
     if (length < 1 || !wf_plus_real || !wf_plus_imaginary || !wf_cross_real || !wf_cross_imaginary)
         return gwh_report(GWH_ERR_INVALID, "null output or empty grid");
     std::vector<double> hp(2 * (size_t)length), hc(2 * (size_t)length);
-    rc = gwh_fourier_waveform(frequencies, length, hp.data(), hc.data(), model, &q->p);
+    const gwh_params pm = for_method(q->p, generation_method);
+    rc = gwh_fourier_waveform(frequencies, length, hp.data(), hc.data(), model, &pm);
     if (rc != GWH_OK) return rc;
     for (int i = 0; i < length; i++) {
         wf_plus_real[i] = hp[2 * i];
@@ -220,7 +236,8 @@ int fourier_detector_response_py(double *frequencies, int length, double *respon
     if (length < 1 || !response_real || !response_imaginary)
         return gwh_report(GWH_ERR_INVALID, "null output or empty grid");
     std::vector<double> r(2 * (size_t)length);
-    rc = gwh_fourier_detector_response(frequencies, length, r.data(), det, model, &q->p);
+    const gwh_params pm = for_method(q->p, generation_method);
+    rc = gwh_fourier_detector_response(frequencies, length, r.data(), det, model, &pm);
     if (rc != GWH_OK) return rc;
     for (int i = 0; i < length; i++) {
         response_real[i] = r[2 * i];
@@ -295,7 +312,8 @@ double MCMC_likelihood_extrinsic_py(bool save_waveform, void *parameters, char *
         pp[i] = psd + off;
         wp[i] = weights ? weights + off : nullptr;
     }
-    return gwh_mcmc_likelihood_extrinsic(save_waveform, &q->p, model, data_length, fp.data(), dp.data(), pp.data(),
+    const gwh_params pm = for_method(q->p, generation_method);
+    return gwh_mcmc_likelihood_extrinsic(save_waveform, &pm, model, data_length, fp.data(), dp.data(), pp.data(),
                                          weights ? wp.data() : nullptr, method, log10F, dets, num_detectors, 0.0);
 }
 
@@ -308,7 +326,8 @@ void repack_parameters_py(double *parameters, void *gen_param, char *generation_
     }
     /* a ppE method's betas, a gIMR method's listed deltas follow its GR basis (fisher.cpp:2083-2088, :2128-2158,
      * :2424-2429, :2473-2505) */
-    const bool ppe = generation_method && strstr(generation_method, "ppE_");
+    const int theory = generation_method ? gwh::theory_id(generation_method) : 0;
+    const bool ppe = generation_method && (strstr(generation_method, "ppE_") || theory > 0);
     const bool gimr = generation_method && !strncmp(generation_method, "gIMR", 4);
     if (gimr && q->p.gimr_nlist < 0) {
         gwh_report(GWH_ERR_UNSUPPORTED, "at most 16 gIMR deltas in a parameter vector");
@@ -317,7 +336,9 @@ void repack_parameters_py(double *parameters, void *gen_param, char *generation_
     const int nmod = ppe ? q->p.ppe_nmod : (gimr ? q->p.gimr_nlist : 0);
     std::string gr = generation_method ? generation_method : "";
     if (gimr) gr.erase(0, 1);
-    if (ppe) {
+    if (theory > 0) /* a theory's parameters follow its family's GR basis, as betappe does */
+        gr = gwh::fisher_gr_method(gr);
+    else if (ppe) {
         gr.erase(gr.find("ppE_"), 4);
         for (const char *suf : {"_Inspiral", "_IMR"}) {
             const std::string s(suf);

@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "gwh_tables.h"
 
 #include "../../include/gwat_hip.h"
 
@@ -45,6 +46,148 @@ __host__ __device__ inline int gwh_ppe_variant(int m)
     if (m == GWH_PPE_IMRPHENOMD_IMR || m == GWH_PPE_IMRPHENOMPV2_IMR) return 2;
     if (m == GWH_GIMRPHENOMD || m == GWH_GIMRPHENOMPV2) return 3;
     return 0;
+}
+
+/* ---- theories mapped onto the ppE terms (ppE_utilities.cpp:158-360 assign_mapping, the beta functions
+ * :408-735, applied in prep_source_parameters, waveform_generator.cpp:1379-1404) ---- */
+
+/* redshift of a luminosity distance in Mpc (Z_from_DL, util.cpp:356-382): the segment whose upper bound exceeds
+ * D_L holds a power series in sqrt(D_L) (cosmology_interpolation_function, util.cpp:502-512); -1 past the table */
+__host__ __device__ inline double gwh_z_from_dl(double dl_mpc, int cosmo)
+{
+    for (int s = 0; s < GWH_COSMO_SEG; s++) {
+        if (dl_mpc < gwh_cosmo_dl_bound[cosmo][s + 1]) {
+            const double r = sqrt(dl_mpc);
+            double z = gwh_cosmo_dz[cosmo][s][0], rk = 1.0;
+            for (int k = 1; k < GWH_COSMO_DEG; k++) {
+                rk *= r;
+                z += gwh_cosmo_dz[cosmo][s][k] * rk;
+            }
+            return z;
+        }
+    }
+    return -1.0;
+}
+
+/* ppE terms a theory maps to (the mapping's Nmod), and how many theory parameters it reads from ppe_beta */
+__host__ __device__ inline int gwh_theory_nterms(int theory, int nmod)
+{
+    if (theory >= GWH_THEORY_EDGB_GHOV1 && theory <= GWH_THEORY_EDGB_GHOV3) return 2;
+    if (theory == GWH_THEORY_PNSERIES || theory == GWH_THEORY_PPEALT) return nmod;
+    return 1;
+}
+__host__ __device__ inline int gwh_theory_ninputs(int theory, int nmod)
+{
+    return gwh_theory_nterms(theory, nmod);
+}
+
+/* the scalar charges of dCS (dCS_phase_factor, ppE_utilities.cpp:487-523) and EdGB (EdGB_phase_factor,
+ * :589-616) of a black hole of dimensionless spin chi; 0 below the reference's comparison threshold */
+__host__ __device__ inline double gwh_dcs_charge(double chi)
+{
+    if (fabs(chi) < 1e-10) return 0.0;
+    const double c2 = chi * chi, r = sqrt(1. - c2);
+    return (2. + 2. * c2 * c2 - 2. * r - c2 * (3. - 2. * r)) / (2. * c2 * chi);
+}
+__host__ __device__ inline double gwh_edgb_charge(double chi)
+{
+    if (fabs(chi) < 1e-10) return 0.0;
+    const double c2 = chi * chi;
+    return 2. * (sqrt(1. - c2) - 1. + c2) / c2;
+}
+
+/* The ppE exponents b[] and coefficients beta[] of a theory, from the source in seconds (total mass M, chirp mass
+ * Mc, eta as the waveform uses them) and the theory parameters ppe_beta[] (ppe_b[] for PNSeries / ppEAlt).
+ * Returns the term count, or -1 for an unknown theory / cosmology or too few theory parameters. */
+__host__ __device__ inline int gwh_theory_terms(const gwh_params &p, double M, double Mc, double eta, double *b,
+                                                double *beta)
+{
+    const int th = p.ppe_theory, nm = p.ppe_nmod;
+    if (th <= GWH_THEORY_NONE || th >= GWH_THEORY_COUNT || p.cosmology < 0 || p.cosmology >= GWH_NCOSMO) return -1;
+    const int n = gwh_theory_nterms(th, nm);
+    if (n < 1 || n > GWH_PPE_MAX || nm < gwh_theory_ninputs(th, nm)) return -1;
+    const double a0 = p.ppe_beta[0];
+    if (th == GWH_THEORY_PNSERIES || th == GWH_THEORY_PPEALT) { /* PNSeries_beta / ppEAlt_beta (:408-443) */
+        const double m1 = p.mass1 * GWH_MSOL_SEC, m2 = p.mass2 * GWH_MSOL_SEC;
+        const double ratio = (m1 + m2) / (pow(m1 * m2, 3. / 5.) / pow(m1 + m2, 1. / 5.));
+        for (int i = 0; i < n; i++) {
+            b[i] = p.ppe_b[i];
+            const double amp = (th == GWH_THEORY_PNSERIES && i > 0) ? a0 * p.ppe_beta[i] : p.ppe_beta[i];
+            beta[i] = amp * pow(ratio, b[i] / 3.);
+        }
+        return n;
+    }
+    const double z = gwh_z_from_dl(p.DL, p.cosmology);
+    const double Ms = M / (1. + z), Ms4 = Ms * Ms * Ms * Ms;
+    const double chi_s = 0.5 * (p.spin1[2] + p.spin2[2]), chi_a = 0.5 * (p.spin1[2] - p.spin2[2]);
+    const double chi1 = chi_s + chi_a, chi2 = chi_s - chi_a;
+    const double ep = pow(eta, 3. / 5.), sq = sqrt(1. - 4. * eta);
+    const double m1 = 0.5 * (Mc / ep + sq * Mc / ep), m2 = 0.5 * (Mc / ep - sq * Mc / ep); /* calculate_mass1/2 */
+    /* EdGB_phase_factor (:589-616) */
+    const double e1 = gwh_edgb_charge(chi1), e2 = gwh_edgb_charge(chi2), ed = m1 * m1 * e2 - m2 * m2 * e1;
+    const double edgb_f = (-5. / 7168.) * (ed * ed) / (M * M * M * M * pow(eta, 18. / 5.));
+    const double scale = 16. * GWH_PI * a0 / Ms4;
+    /* (3 - 26 eta + 34 eta^2) / (eta^(2/5) (1 - 2 eta)) of the mass-loss terms (:667-695) */
+    const double mdot_f = (25. / 851968.) * ((3. - 26. * eta + 34. * eta * eta) / (pow(eta, 2. / 5.) * (1 - 2 * eta)));
+    switch (th) {
+    case GWH_THEORY_DCS: { /* dCS_beta, dCS_phase_factor (:473-523) */
+        const double d1 = gwh_dcs_charge(chi1), d2 = gwh_dcs_charge(chi2), m = m1 + m2;
+        const double e145 = pow(eta, 14. / 5.), dd = m1 * d2 - m2 * d1;
+        double g = 0;
+        g += (-5. / 8192.) / e145 * pow(dd, 2.) / (m * m);
+        g += (15075. / 114688.) / e145 *
+             (m2 * m2 * chi1 * chi1 - 350. / 201. * m1 * m2 * chi1 * chi2 + m1 * m1 * chi2 * chi2) / (m * m);
+        b[0] = -1;
+        beta[0] = scale * g;
+        break;
+    }
+    case GWH_THEORY_EDGB: /* EdGB_beta (:573-582) */
+        b[0] = -7;
+        beta[0] = scale * edgb_f;
+        break;
+    case GWH_THEORY_EDGB_HO_LO: /* EdGB_HO_0PN_beta (:530-542) */
+        b[0] = -7;
+        beta[0] = -5. * (16. * GWH_PI * a0) / Ms4 / 7168. / pow(eta, 18. / 5.) * (4 * eta - 1);
+        break;
+    case GWH_THEORY_EDGB_GHOV1:
+    case GWH_THEORY_EDGB_GHOV2:
+    case GWH_THEORY_EDGB_GHOV3: { /* EdGB_beta and EdGB_GHO_betav1..3 (:622-664) */
+        b[0] = -7;
+        b[1] = -5;
+        beta[0] = scale * edgb_f;
+        double g = p.ppe_beta[1];
+        if (th != GWH_THEORY_EDGB_GHOV1) g = edgb_f * g;
+        if (th == GWH_THEORY_EDGB_GHOV3) g = g * (3715. / 756. + 55. * eta / 9.);
+        beta[1] = scale * g;
+        break;
+    }
+    case GWH_THEORY_EXTRADIMENSION: { /* ExtraDimension_beta (:667-679): m-dot of each hole in the brane world */
+        const double l10 = 10.e-6 / GWH_C_SI, year = 31557600.;
+        const double q1 = GWH_MSOL_SEC * (1 + z) / (p.mass1 * GWH_MSOL_SEC), q2 = GWH_MSOL_SEC * (1 + z) / (p.mass2 * GWH_MSOL_SEC);
+        const double md1 = -2.8e-7 * (q1 * q1) * a0 / (l10 * l10) * GWH_MSOL_SEC / year;
+        const double md2 = -2.8e-7 * (q2 * q2) * a0 / (l10 * l10) * GWH_MSOL_SEC / year;
+        b[0] = -13;
+        beta[0] = (md1 + md2) * mdot_f;
+        break;
+    }
+    case GWH_THEORY_BHEVAPORATION: /* BHEvaporation_beta (:685-692) */
+        b[0] = -13;
+        beta[0] = a0 * mdot_f;
+        break;
+    case GWH_THEORY_TVG: /* TVG_beta (:698-704) */
+        b[0] = -13;
+        beta[0] = (-25. / 65526.) * (a0 * Mc / (1 + z));
+        break;
+    case GWH_THEORY_DIPRAD: /* DipRad_beta (:710-716) */
+        b[0] = -7;
+        beta[0] = (-3. / 224.) * pow(eta, 2. / 5.) * a0;
+        break;
+    default: /* GWH_THEORY_NONCOMM, NonComm_beta (:721-726) */
+        b[0] = -1;
+        beta[0] = (-75. / 256.) * pow(eta, -4. / 5.) * (2. * eta - 1.) * a0;
+        break;
+    }
+    return n;
 }
 
 /* Frequency-independent constants of one instance (one parameter vector on
@@ -547,7 +690,7 @@ __device__ __forceinline__ void gwh_setup_base(const gwh_params &p, int ppe, Set
     /* ppE terms (waveform_generator.cpp:1296-1300 copies them for the ppE methods only) */
     S.ppe_n = 0;
     if (ppe == 1 || ppe == 2) {
-        const int nm = p.ppe_nmod;
+        const int nm = p.ppe_theory ? gwh_theory_nterms(p.ppe_theory, p.ppe_nmod) : p.ppe_nmod;
         S.ppe_n = nm < 0 ? 0 : (nm > GWH_PPE_MAX ? GWH_PPE_MAX : nm);
     }
     S.ppe_imr = (ppe == 2);
@@ -691,8 +834,9 @@ __device__ __forceinline__ void gwh_setup_pn(const gwh_params &p, SetupState &S)
     }
 }
 
-/* role 3, phase 1: chirp mass and amplitude scale (util.cpp:996-1028), ppE coefficients beta (pi Mc)^(b/3).
- * More than GWH_PPE_MAX ppE terms cannot be represented: the amplitude (and so the likelihood) is NaN. */
+/* role 3, phase 1: chirp mass and amplitude scale (util.cpp:996-1028), ppE coefficients beta (pi Mc)^(b/3), the
+ * terms mapped from a theory's parameters first where ppe_theory is set.  More than GWH_PPE_MAX ppE terms, and a
+ * theory that cannot be mapped, give a NaN amplitude (and so a NaN likelihood). */
 __device__ __forceinline__ void gwh_setup_mass(const gwh_params &p, int ppe, SetupState &S)
 {
     S.Mc = pow(S.m1 * S.m2, 0.6) / pow(S.M, 0.2);
@@ -701,11 +845,18 @@ __device__ __forceinline__ void gwh_setup_mass(const gwh_params &p, int ppe, Set
     S.A0 = kA * S.Mc * S.Mc / DL * pow(GWH_PI * S.Mc, -7. / 6.);
     for (int i = 0; i < GWH_PPE_MAX; i++) S.ppe_e[i] = S.ppe_c[i] = 0.0;
     if (ppe == 1 || ppe == 2) {
-        if (p.ppe_nmod > GWH_PPE_MAX) S.A0 = __longlong_as_double(0x7ff8000000000000LL); /* NaN amplitude */
+        double b[GWH_PPE_MAX], beta[GWH_PPE_MAX];
+        bool bad = p.ppe_nmod > GWH_PPE_MAX;
+        for (int i = 0; i < GWH_PPE_MAX; i++) {
+            b[i] = p.ppe_b[i];
+            beta[i] = p.ppe_beta[i];
+        }
+        if (p.ppe_theory) bad |= gwh_theory_terms(p, S.M, S.Mc, S.eta, b, beta) < 0;
+        if (bad) S.A0 = __longlong_as_double(0x7ff8000000000000LL); /* NaN amplitude */
         const double lpm = log(GWH_PI * S.Mc);
         for (int i = 0; i < GWH_PPE_MAX; i++) {
-            S.ppe_e[i] = i < S.ppe_n ? p.ppe_b[i] / 3.0 : 0.0;
-            S.ppe_c[i] = i < S.ppe_n ? p.ppe_beta[i] * exp(S.ppe_e[i] * lpm) : 0.0;
+            S.ppe_e[i] = i < S.ppe_n ? b[i] / 3.0 : 0.0;
+            S.ppe_c[i] = i < S.ppe_n ? beta[i] * exp(S.ppe_e[i] * lpm) : 0.0;
         }
     }
 }

@@ -59,6 +59,31 @@ extern "C" {
 #define GWH_PPE_IMRPHENOMPV2_IMR 6
 #define GWH_GIMRPHENOMPV2 7
 
+/* theories mapped onto the ppE models (src/ppE_utilities.cpp:84-135, :158-360); the model id is the ppE method the
+ * reference maps the theory's method string to (all of these the _Inspiral variant, PNSeries / ppEAlt either) */
+#define GWH_THEORY_NONE 0
+#define GWH_THEORY_DCS 1            /* dCS: b = -1, 16 pi alpha^2 / M_src^4 * dCS_phase_factor */
+#define GWH_THEORY_EDGB 2           /* EdGB (and EdGB_HO, whose terms the reference's mapping overwrites with EdGB's) */
+#define GWH_THEORY_EDGB_HO_LO 3     /* EdGB_HO_LO: b = -7, the 0PN higher-order beta */
+#define GWH_THEORY_EDGB_GHOV1 4     /* EdGB_GHOv1..v3: b = -7 (EdGB) and -5 (generic 1PN, betappe[1]) */
+#define GWH_THEORY_EDGB_GHOV2 5
+#define GWH_THEORY_EDGB_GHOV3 6
+#define GWH_THEORY_EXTRADIMENSION 7 /* b = -13, from the extra-dimension length squared (seconds^2) */
+#define GWH_THEORY_BHEVAPORATION 8  /* b = -13, from the evaporation rate */
+#define GWH_THEORY_TVG 9            /* b = -13, from G-dot */
+#define GWH_THEORY_DIPRAD 10        /* b = -7, from delta E-dot */
+#define GWH_THEORY_NONCOMM 11       /* b = -1, from Lambda^2 */
+#define GWH_THEORY_PNSERIES 12      /* b as given; beta_0 (M/Mc)^(b_0/3), beta_0 beta_i (M/Mc)^(b_i/3) */
+#define GWH_THEORY_PPEALT 13        /* b as given; beta_i (M/Mc)^(b_i/3) */
+#define GWH_THEORY_COUNT 14
+
+/* cosmologies of the redshift fits (src/util.cpp:356-382, include/gwat/D_Z_Config.h) */
+#define GWH_COSMO_PLANCK15 0
+#define GWH_COSMO_PLANCK13 1
+#define GWH_COSMO_WMAP9 2
+#define GWH_COSMO_WMAP7 3
+#define GWH_COSMO_WMAP5 4
+
 /* arithmetic of the batched kernels */
 #define GWH_FP64 0  /* everything in FP64 */
 #define GWH_MIXED 1 /* FP64 phase, amplitude, products and sums; FP32 sin/cos and merger-ringdown exp */
@@ -99,8 +124,11 @@ typedef struct gwh_params {
     double gmst;
     double chip, phip; /* chip <= -1 + 1e-10: precessing spins are taken from spin1/spin2 vectors */
     int32_t shift_time, shift_phase, sky_average, dep_postmerger;
-    /* ppE terms (gen_params_base: Nmod, bppe, betappe), read only by the ppE models */
-    int32_t ppe_nmod, ppe_pad;
+    /* ppE terms (gen_params_base: Nmod, bppe, betappe), read only by the ppE models. ppe_theory (GWH_THEORY_*,
+     * 0: none) maps theory parameters held in ppe_beta[0 .. ppe_nmod) (and ppe_b for PNSeries / ppEAlt) to the
+     * ppE terms, as the reference does for its theory method strings (dCS_IMRPhenomD -> ppE_IMRPhenomD_Inspiral,
+     * src/ppE_utilities.cpp:158-360, src/waveform_generator.cpp:1379-1404) */
+    int32_t ppe_nmod, ppe_theory;
     double ppe_b[GWH_PPE_MAX];
     double ppe_beta[GWH_PPE_MAX];
     /* gIMR modifications (gen_params_base: phii / delta_phi, sigmai / delta_sigma, betai / delta_beta,
@@ -113,7 +141,8 @@ typedef struct gwh_params {
     /* the gIMR deltas that are Fisher parameters, in the reference's basis order (the phii list, then sigmai,
      * betai, alphai: src/fisher.cpp:2128-2158), as codes into the tables above (GWH_GIMR_CODE_*); the first
      * entry of a repeated code is the one the waveform reads, as check_list_id finds it */
-    int32_t gimr_nlist, gimr_pad;
+    int32_t gimr_nlist;
+    int32_t cosmology; /* redshift from D_L for the theory maps (gen_params_base::cosmology): GWH_COSMO_*, 0 PLANCK15 */
     int8_t gimr_list[GWH_GIMR_LIST_MAX];
 } gwh_params;
 

@@ -29,6 +29,60 @@ namespace gwh {
 /* "IMRPhenomD" / "IMRPhenomPv2", their ppE ("ppE_IMRPhenomD_Inspiral", "ppE_IMRPhenomD_IMR",
  * "ppE_IMRPhenomPv2_Inspiral", "ppE_IMRPhenomPv2_IMR") and gIMR ("gIMRPhenomD", "gIMRPhenomPv2") variants
  * (waveform_generator.cpp:103-294 dispatch); -1 otherwise */
+/* A theory method string (check_theory_support, ppE_utilities.cpp:84-135) -> GWH_THEORY_*, and in *model the ppE
+ * method assign_mapping (:158-360) maps it to: the _Inspiral variant, or for PNSeries / ppEAlt the variant the
+ * string names.  0: not a theory string; -1: a theory the engine does not map (ModDispersion, polarization_test,
+ * the NRT / EA variants, another waveform family). */
+inline int theory_id(const std::string &m, int *model = nullptr)
+{
+    auto has = [&](const char *s) { return m.find(s) != std::string::npos; };
+    int th;
+    bool ins = true;
+    if (has("dCS")) th = GWH_THEORY_DCS;
+    else if (has("EdGB")) /* EdGB_HO keeps EdGB's terms: the reference's second if-chain overwrites them */
+        th = has("EdGB_HO_LO") ? GWH_THEORY_EDGB_HO_LO
+           : has("EdGB_GHOv1") ? GWH_THEORY_EDGB_GHOV1
+           : has("EdGB_GHOv2") ? GWH_THEORY_EDGB_GHOV2
+           : has("EdGB_GHOv3") ? GWH_THEORY_EDGB_GHOV3 : GWH_THEORY_EDGB;
+    else if (has("ExtraDimension")) th = GWH_THEORY_EXTRADIMENSION;
+    else if (has("BHEvaporation")) th = GWH_THEORY_BHEVAPORATION;
+    else if (has("TVG")) th = GWH_THEORY_TVG;
+    else if (has("DipRad")) th = GWH_THEORY_DIPRAD;
+    else if (has("NonComm")) th = GWH_THEORY_NONCOMM;
+    else if (has("ModDispersion") || has("polarization_test")) return -1;
+    else if (has("PNSeries_ppE")) th = GWH_THEORY_PNSERIES, ins = has("Inspiral");
+    else if (has("ppEAlt")) th = GWH_THEORY_PPEALT, ins = has("Inspiral");
+    else return 0;
+    if (has("NRT") || has("EA_")) return -1;
+    const bool pv2 = has("IMRPhenomPv2");
+    if (!pv2 && !has("IMRPhenomD")) return -1;
+    if (model)
+        *model = pv2 ? (ins ? GWH_PPE_IMRPHENOMPV2_INSPIRAL : GWH_PPE_IMRPHENOMPV2_IMR)
+                     : (ins ? GWH_PPE_IMRPHENOMD_INSPIRAL : GWH_PPE_IMRPHENOMD_IMR);
+    return th;
+}
+
+/* gen_params_base::cosmology (case-insensitive, util.cpp:356-382) -> GWH_COSMO_*; -1 if unknown */
+inline int cosmology_id(const std::string &c)
+{
+    std::string u;
+    for (char ch : c) u += (char)toupper((unsigned char)ch);
+    const char *names[5] = {"PLANCK15", "PLANCK13", "WMAP9", "WMAP7", "WMAP5"};
+    for (int i = 0; i < 5; i++)
+        if (u == names[i]) return i;
+    return -1;
+}
+template <class P>
+inline auto cosmology_of(const P &g, int) -> decltype(std::string(g.cosmology), int())
+{
+    return cosmology_id(std::string(g.cosmology));
+}
+template <class P>
+inline int cosmology_of(const P &, long)
+{
+    return GWH_COSMO_PLANCK15;
+}
+
 inline int model_id(const std::string &method)
 {
     if (method == "IMRPhenomD") return GWH_IMRPHENOMD;
@@ -39,6 +93,8 @@ inline int model_id(const std::string &method)
     if (method == "ppE_IMRPhenomPv2_Inspiral") return GWH_PPE_IMRPHENOMPV2_INSPIRAL;
     if (method == "ppE_IMRPhenomPv2_IMR") return GWH_PPE_IMRPHENOMPV2_IMR;
     if (method == "gIMRPhenomPv2") return GWH_GIMRPHENOMPV2;
+    int m;
+    if (theory_id(method, &m) > 0) return m;
     return -1;
 }
 
@@ -75,6 +131,9 @@ inline int integration_id(const std::string &method) { return method == "GAUSSLE
 inline std::string fisher_gr_method(const std::string &method)
 {
     std::string m = method;
+    if (theory_id(m) > 0) /* a theory's parameters follow the basis of its waveform family */
+        return std::string(m.compare(0, 5, "MCMC_") == 0 ? "MCMC_" : "") +
+               (m.find("IMRPhenomPv2") != std::string::npos ? "IMRPhenomPv2" : "IMRPhenomD");
     const size_t g = m.find("gIMR");
     if (g != std::string::npos) return m.erase(g, 1);
     const size_t p = m.find("ppE_");
@@ -105,9 +164,10 @@ inline int basis_id(const std::string &method_in, bool sky_average)
 /* the hot-path fields of a gen_params_base<double>-shaped object for the given model; false if it asks
  * for physics the engine does not model (equatorial / horizon orientation, neutron-star flags, more ppE
  * terms than GWH_PPE_MAX).  As in prep_source_parameters (waveform_generator.cpp:1296-1313), the ppE
- * fields (Nmod, bppe, betappe) are read for the ppE models only and the gIMR lists for the gIMR models only. */
+ * fields (Nmod, bppe, betappe) are read for the ppE models only and the gIMR lists for the gIMR models only.
+ * theory (theory_id of the method string): the ppE fields hold that theory's parameters, mapped on the device. */
 template <class Params>
-inline bool to_params(const Params &g, gwh_params &p, int model = GWH_IMRPHENOMD)
+inline bool to_params(const Params &g, gwh_params &p, int model = GWH_IMRPHENOMD, int theory = GWH_THEORY_NONE)
 {
     p = gwh_params();
     p.mass1 = g.mass1;
@@ -133,12 +193,16 @@ inline bool to_params(const Params &g, gwh_params &p, int model = GWH_IMRPHENOMD
     p.dep_postmerger = g.dep_postmerger;
     if (model == GWH_PPE_IMRPHENOMD_INSPIRAL || model == GWH_PPE_IMRPHENOMD_IMR ||
         model == GWH_PPE_IMRPHENOMPV2_INSPIRAL || model == GWH_PPE_IMRPHENOMPV2_IMR) {
-        if (g.Nmod < 0 || g.Nmod > GWH_PPE_MAX || (g.Nmod > 0 && (!g.bppe || !g.betappe))) return false;
+        if (g.Nmod < 0 || g.Nmod > GWH_PPE_MAX || (g.Nmod > 0 && ((!g.bppe && theory <= 0) || !g.betappe)))
+            return false;
         p.ppe_nmod = g.Nmod;
         for (int i = 0; i < g.Nmod; i++) {
-            p.ppe_b[i] = g.bppe[i];
+            p.ppe_b[i] = g.bppe ? g.bppe[i] : 0.0; /* the theories other than PNSeries / ppEAlt need no bppe */
             p.ppe_beta[i] = g.betappe[i];
         }
+        p.ppe_theory = theory > 0 ? theory : GWH_THEORY_NONE;
+        p.cosmology = cosmology_of(g, 0);
+        if (theory < 0 || (theory > 0 && p.cosmology < 0)) return false;
     }
     if (model == GWH_GIMRPHENOMD || model == GWH_GIMRPHENOMPV2) {
         gimr_table(g.Nmod_phi, g.phii, g.delta_phi, -4, 9, p.gimr_dphi);
@@ -185,7 +249,7 @@ inline int fourier_waveform(const double *f, int n, std::complex<double> *hplus,
 {
     gwh_params p;
     int model = model_id(method);
-    if (model < 0 || !to_params(*g, p, model)) return GWH_ERR_UNSUPPORTED;
+    if (model < 0 || !to_params(*g, p, model, theory_id(method))) return GWH_ERR_UNSUPPORTED;
     return gwh_fourier_waveform(f, n, reinterpret_cast<double *>(hplus), reinterpret_cast<double *>(hcross), model,
                                 &p);
 }
@@ -197,7 +261,7 @@ inline int fourier_detector_response(const double *f, int n, std::complex<double
 {
     gwh_params p;
     int model = model_id(method), d = gwh_detector_id(det.c_str());
-    if (model < 0 || d < 0 || !to_params(*g, p, model)) return GWH_ERR_UNSUPPORTED;
+    if (model < 0 || d < 0 || !to_params(*g, p, model, theory_id(method))) return GWH_ERR_UNSUPPORTED;
     return gwh_fourier_detector_response(f, n, reinterpret_cast<double *>(response), d, model, &p);
 }
 
@@ -208,7 +272,7 @@ inline double Log_Likelihood(const std::complex<double> *data, const double *psd
 {
     gwh_params p;
     int model = model_id(method), d = gwh_detector_id(det.c_str());
-    if (model < 0 || d < 0 || !to_params(*g, p, model)) return nan_with(GWH_ERR_UNSUPPORTED);
+    if (model < 0 || d < 0 || !to_params(*g, p, model, theory_id(method))) return nan_with(GWH_ERR_UNSUPPORTED);
     return gwh_log_likelihood(reinterpret_cast<const double *>(data), psd, f, n, &p, d, model);
 }
 
@@ -233,7 +297,7 @@ inline double MCMC_likelihood_extrinsic(bool save_waveform, const Params *g, con
 {
     gwh_params p;
     int model = model_id(method);
-    if (model < 0 || !to_params(*g, p, model) || num_detectors < 1 || num_detectors > 4)
+    if (model < 0 || !to_params(*g, p, model, theory_id(method)) || num_detectors < 1 || num_detectors > 4)
         return nan_with(GWH_ERR_UNSUPPORTED);
     int dets[4];
     std::vector<const double *> dp(num_detectors);
@@ -255,9 +319,9 @@ inline void fisher_numerical(const double *f, int n, const std::string &method, 
     gwh_params p;
     const std::string mm = method.compare(0, 5, "MCMC_") == 0 ? method.substr(5) : method;
     int basis = basis_id(method, g->sky_average), d = gwh_detector_id(det.c_str()),
-        r = gwh_detector_id(ref_det.c_str()), model = model_id(mm);
+        r = gwh_detector_id(ref_det.c_str()), model = model_id(mm), theory = theory_id(mm);
     std::vector<double> out((size_t)dim * dim, std::nan(""));
-    if (basis >= 0 && model >= 0 && d >= 0 && r >= 0 && to_params(*g, p, model))
+    if (basis >= 0 && model >= 0 && d >= 0 && r >= 0 && to_params(*g, p, model, theory))
         gwh_fisher_numerical_model(f, n, model, basis, d, r, out.data(), dim, &p, order, psd, epsilon);
     for (int i = 0; i < dim; i++)
         for (int j = 0; j < dim; j++) output[i][j] = out[(size_t)i * dim + j];

@@ -21,6 +21,11 @@ PPE_IMRPHENOMPV2_INSPIRAL = 5
 PPE_IMRPHENOMPV2_IMR = 6
 GIMRPHENOMPV2 = 7
 PV2_FAMILY = (IMRPHENOMPV2, PPE_IMRPHENOMPV2_INSPIRAL, PPE_IMRPHENOMPV2_IMR, GIMRPHENOMPV2)
+# theories mapped onto the ppE models (ppe_theory; ppE_utilities.cpp:158-360) and the redshift cosmologies
+(THEORY_NONE, THEORY_DCS, THEORY_EDGB, THEORY_EDGB_HO_LO, THEORY_EDGB_GHOV1, THEORY_EDGB_GHOV2, THEORY_EDGB_GHOV3,
+ THEORY_EXTRADIMENSION, THEORY_BHEVAPORATION, THEORY_TVG, THEORY_DIPRAD, THEORY_NONCOMM, THEORY_PNSERIES,
+ THEORY_PPEALT) = range(14)
+COSMO_PLANCK15, COSMO_PLANCK13, COSMO_WMAP9, COSMO_WMAP7, COSMO_WMAP5 = range(5)
 
 DETECTORS = {"Hanford": 0, "Livingston": 1, "Virgo": 2, "Kagra": 3, "Indigo": 4,
              "CE": 5, "ET1": 6, "ET2": 7, "ET3": 8}
@@ -36,11 +41,11 @@ class Params(ctypes.Structure):
                 ("chip", ctypes.c_double), ("phip", ctypes.c_double),
                 ("shift_time", ctypes.c_int), ("shift_phase", ctypes.c_int),
                 ("sky_average", ctypes.c_int), ("dep_postmerger", ctypes.c_int),
-                ("ppe_nmod", ctypes.c_int), ("ppe_pad", ctypes.c_int),
+                ("ppe_nmod", ctypes.c_int), ("ppe_theory", ctypes.c_int),
                 ("ppe_b", ctypes.c_double * 4), ("ppe_beta", ctypes.c_double * 4),
                 ("gimr_dphi", ctypes.c_double * 14), ("gimr_dsigma", ctypes.c_double * 3),
                 ("gimr_dbeta", ctypes.c_double * 2), ("gimr_dalpha", ctypes.c_double * 4),
-                ("gimr_nlist", ctypes.c_int), ("gimr_pad", ctypes.c_int), ("gimr_list", ctypes.c_int8 * 16)]
+                ("gimr_nlist", ctypes.c_int), ("cosmology", ctypes.c_int), ("gimr_list", ctypes.c_int8 * 16)]
 
 
 def make_params(**kw):
@@ -333,3 +338,28 @@ def pn_phase_coefficients(p, model=IMRPHENOMD):
     out = np.zeros(12)
     lib().gwo_pn_phase_coefficients(ctypes.byref(p), model, _dp(out))
     return out
+
+
+def z_from_dl(dl_mpc, cosmology=COSMO_PLANCK15):
+    """redshift of a luminosity distance in Mpc (Z_from_DL, util.cpp:356-382)"""
+    f = lib().gwo_z_from_dl
+    f.restype, f.argtypes = ctypes.c_double, [ctypes.c_double, ctypes.c_int]
+    return f(dl_mpc, cosmology)
+
+
+def theory_terms(p):
+    """the ppE exponents and coefficients p.ppe_theory maps p's theory parameters to (assign_mapping,
+    ppE_utilities.cpp:158-360), for the source as the waveform computes it; None if unmappable"""
+    ms = 4.925491025543575903411922162094833998e-6
+    L = lib()
+    for name in ("gwo_calculate_chirpmass", "gwo_calculate_eta"):
+        getattr(L, name).restype = ctypes.c_double
+        getattr(L, name).argtypes = [ctypes.c_double, ctypes.c_double]
+    m1, m2 = p.mass1 * ms, p.mass2 * ms
+    b, beta = np.zeros(4), np.zeros(4)
+    L.gwo_theory_terms.restype = ctypes.c_int
+    L.gwo_theory_terms.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    n = L.gwo_theory_terms(ctypes.addressof(p), m1 + m2, L.gwo_calculate_chirpmass(m1, m2),
+                           L.gwo_calculate_eta(m1, m2), _dp(b), _dp(beta))
+    return None if n < 0 else (b[:n], beta[:n])

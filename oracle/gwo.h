@@ -34,6 +34,14 @@ enum {
     GWO_GIMRPHENOMPV2 = 7
 };
 /* the precessing family (waveform_generator.cpp:227: local_method.find("IMRPhenomPv2")) */
+/* theories mapped onto ppE terms (ppE_utilities.cpp:158-360); EdGB_HO maps as EdGB */
+enum {
+    GWO_THEORY_NONE = 0, GWO_THEORY_DCS, GWO_THEORY_EDGB, GWO_THEORY_EDGB_HO_LO, GWO_THEORY_EDGB_GHOV1,
+    GWO_THEORY_EDGB_GHOV2, GWO_THEORY_EDGB_GHOV3, GWO_THEORY_EXTRADIMENSION, GWO_THEORY_BHEVAPORATION,
+    GWO_THEORY_TVG, GWO_THEORY_DIPRAD, GWO_THEORY_NONCOMM, GWO_THEORY_PNSERIES, GWO_THEORY_PPEALT, GWO_THEORY_COUNT
+};
+double gwo_z_from_dl(double dl_mpc, int cosmo);
+
 #define GWO_IS_PV2(m) \
     ((m) == GWO_IMRPHENOMPV2 || (m) == GWO_PPE_IMRPHENOMPV2_INSPIRAL || (m) == GWO_PPE_IMRPHENOMPV2_IMR || \
      (m) == GWO_GIMRPHENOMPV2)
@@ -52,12 +60,12 @@ typedef struct {
     double chip, phip;            /* Pv2 reduced spin parameters; chip=-1 => use spin vectors */
     int shift_time, shift_phase, sky_average;
     int dep_postmerger;           /* closed-form ringdown fits instead of the QNM spline (util.h:197) */
-    int ppe_nmod, ppe_pad;        /* ppE terms: Nmod, bppe, betappe (util.h:249-255) */
+    int ppe_nmod, ppe_theory; /* theory id (GWO_THEORY_*), 0: plain ppE */        /* ppE terms: Nmod, bppe, betappe (util.h:249-255) */
     double ppe_b[4], ppe_beta[4];
     double gimr_dphi[14], gimr_dsigma[3], gimr_dbeta[2], gimr_dalpha[4]; /* gIMR deltas (0: unmodified) */
     /* the gIMR deltas in a Fisher basis, in the reference's list order (phii, sigmai, betai, alphai), as codes:
      * PN order i -> i + 4, sigma_i -> 12 + i, beta_i -> 15 + i, alpha_i -> 17 + i */
-    int gimr_nlist, gimr_pad;
+    int gimr_nlist, cosmology; /* cosmology: GWH_COSMO_* (0: PLANCK15) */
     signed char gimr_list[16];
 } gwo_params;
 

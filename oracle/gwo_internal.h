@@ -55,6 +55,7 @@ typedef struct {
 
 double gwo_mpc_sec(void);
 void gwo_populate_source(gwo_src *s, const gwo_params *p, int model);
+int gwo_theory_terms(const gwo_params *p, double M, double mc, double eta, double *bppe, double *betappe);
 void gwo_phenomd_prepare(gwo_src *s, gwo_phenomd_state *st);
 void gwo_precalc_powers_pi(gwo_pows *pw);
 void gwo_precalc_powers_ins(double f, double M, gwo_pows *pw);

@@ -88,6 +88,14 @@ void gwo_populate_source(gwo_src *s, const gwo_params *p, int model)
             s->bppe[i] = p->ppe_b[i];
             s->betappe[i] = p->ppe_beta[i];
         }
+        /* waveform_generator.cpp:1379-1404: a theory's parameters become the mapped ppE terms */
+        if (p->ppe_theory) {
+            s->Nmod = gwo_theory_terms(p, s->M, s->chirpmass, s->eta, s->bppe, s->betappe);
+            if (s->Nmod < 0) {
+                s->Nmod = 0;
+                s->A0 = NAN;
+            }
+        }
     }
     /* waveform_generator.cpp:1301-1313: the gIMR fields are copied for the gIMR methods only */
     if (model == GWO_GIMRPHENOMD || model == GWO_GIMRPHENOMPV2) { /* gIMRPhenomP.cpp:12-41 */

@@ -91,6 +91,18 @@ int main()
     for (int i = 0; i < 5; i++) w[11 + i] = 0.5 + 0.1 * i;
     gwh::repack_parameters(w, &g, "gIMRPhenomD", 16);
     printf("gimrrepack %g %g %g %g %g\n", dph[0], dph[1], dph[2], dph[3], dal[0]);
+    /* theory method strings: the theory id and the ppE method the reference maps each to */
+    const char *th[8] = {"dCS_IMRPhenomD", "EdGB_HO_IMRPhenomPv2", "EdGB_HO_LO_IMRPhenomD", "EdGB_GHOv2_IMRPhenomD",
+                         "PNSeries_ppE_IMRPhenomD_IMR", "ppEAlt_ppE_IMRPhenomPv2_Inspiral", "ModDispersion_IMRPhenomD",
+                         "dCS_IMRPhenomD_NRT"};
+    printf("theories");
+    for (int i = 0; i < 8; i++) {
+        int m = -9;
+        int t = gwh::theory_id(th[i], &m);
+        printf(" %d:%d", t, m);
+    }
+    printf(" %d %s %d\n", gwh::model_id("TVG_IMRPhenomD"), gwh::fisher_gr_method("MCMC_dCS_IMRPhenomPv2").c_str(),
+           gwh::cosmology_id("wmap9"));
     return 0;
 }
 '''
@@ -123,3 +135,8 @@ def test_compat_header_with_reference_shaped_params(tmp_path):
     # phi 2, -3, 2, 12 (out of range), alpha 5 -> codes 6, 1, 6, -1, 22
     assert out[5].split() == ["gimrlist", "5", "6", "1", "6", "-1", "22", str(G.BASIS_PHENOMD_MCMC)]
     assert out[6].split() == ["gimrrepack", "0.5", "0.6", "0.7", "0.8", "0.9"]
+    D, DI, PI = G.PPE_IMRPHENOMD_INSPIRAL, G.PPE_IMRPHENOMD_IMR, G.PPE_IMRPHENOMPV2_INSPIRAL
+    assert out[7].split() == ["theories", "%d:%d" % (G.THEORY_DCS, D), "%d:%d" % (G.THEORY_EDGB, PI),
+                              "%d:%d" % (G.THEORY_EDGB_HO_LO, D), "%d:%d" % (G.THEORY_EDGB_GHOV2, D),
+                              "%d:%d" % (G.THEORY_PNSERIES, DI), "%d:%d" % (G.THEORY_PPEALT, PI), "-1:-9", "-1:-9",
+                              str(D), "MCMC_IMRPhenomPv2", str(G.COSMO_WMAP9)]

@@ -382,3 +382,102 @@ def test_ppe_fisher_batch_and_two_detectors():
         e, ed = _fnorm(F[i], ref)
         assert e < FISHER_TOL and ed < FISHER_TOL, (i, e, ed)
     ev.close()
+
+
+# ---- theories mapped onto the ppE terms (ppE_utilities.cpp:158-360) ------------------------------------------
+THEORIES = [  # (theory, theory parameters, exponents (PNSeries / ppEAlt), IMRPhenomD model)
+    (G.THEORY_DCS, [1e-17], [], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_EDGB, [1e-20], [], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_EDGB_HO_LO, [1e-20], [], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_EDGB_GHOV1, [1e-20, 1.5], [], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_EDGB_GHOV2, [1e-20, 1.5], [], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_EDGB_GHOV3, [1e-20, 1.5], [], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_EXTRADIMENSION, [1e-12], [], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_BHEVAPORATION, [1e-6], [], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_TVG, [1e-3], [], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_DIPRAD, [1e-4], [], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_NONCOMM, [0.05], [], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_PNSERIES, [1e-3, 0.5], [-7., -5.], G.PPE_IMRPHENOMD_IMR),
+    (G.THEORY_PPEALT, [3e-5, -3e-3], [-7., -3.], G.PPE_IMRPHENOMD_INSPIRAL),
+]
+TO_PV2 = {G.PPE_IMRPHENOMD_INSPIRAL: G.PPE_IMRPHENOMPV2_INSPIRAL, G.PPE_IMRPHENOMD_IMR: G.PPE_IMRPHENOMPV2_IMR}
+
+
+def _theory_kw(th, beta, b, **extra):
+    return dict(ppe_theory=th, ppe_nmod=len(beta), ppe_beta=beta, ppe_b=b or [0.] * len(beta), **extra)
+
+
+@pytest.mark.parametrize("th,beta,b,model", THEORIES)
+@pytest.mark.parametrize("pv2", [False, True])
+def test_theory_waveform_matches_oracle(th, beta, b, model, pv2):
+    f = W.uniform_grid(20., 1024., 0.25)
+    kw = dict(CASES[0], **EXTR, **_theory_kw(th, beta, b))
+    if pv2:
+        model = TO_PV2[model]
+        kw.update(spin1=[0.3, 0.1, 0.3], spin2=[-0.1, 0.2, -0.1], chip=0.35, phip=0.8)
+    g, o = both(**kw)
+    hp, hc = G.fourier_waveform(f, g, model)
+    op, oc = O.fourier_waveform(f, o, model)
+    wf_close(hp, op)
+    wf_close(hc, oc)
+    gr, _ = G.fourier_waveform(f, g, G.IMRPHENOMPV2 if pv2 else G.IMRPHENOMD)
+    nz = np.abs(op) > 0
+    assert np.max(np.abs(np.angle(hp[nz] / gr[nz]))) > 1e-3  # the theory term is visible
+
+
+def test_theory_network_loglike_batch_and_refusal():
+    """one batch of instances with different theories (and cosmologies), FP64 against the oracle, mixed within
+    1e-3; an instance whose theory cannot be mapped (a GHO variant given one parameter) is NaN, the others not"""
+    f, psd, dets, data = _net()
+    ev = G.Event(f, data, [psd] * 3, dets)
+    ins = [t for t in THEORIES if t[3] == G.PPE_IMRPHENOMD_INSPIRAL]
+    P_ = W.perturbed_params(W.injection_params(1)[0], 4 * len(ins), step=5)
+    for i in range(len(P_)):
+        th, beta, b, _ = ins[i % len(ins)]
+        P_["ppe_theory"][i], P_["ppe_nmod"][i] = th, len(beta)
+        P_["ppe_beta"][i, :len(beta)] = beta
+        P_["ppe_b"][i, :len(b)] = b
+        P_["cosmology"][i] = (i // len(ins)) % 5
+    bad = 3  # EDGB_GHOV1 with a single parameter
+    assert P_["ppe_theory"][bad] == G.THEORY_EDGB_GHOV1
+    P_["ppe_nmod"][bad] = 1
+    ll = G.loglike_batch(ev, P_, G.PPE_IMRPHENOMD_INSPIRAL, G.FP64)
+    assert np.isnan(ll[bad]) and np.all(np.isfinite(np.delete(ll, bad)))
+    for i in range(len(P_)):
+        if i == bad:
+            continue
+        rec = P_[i]
+        kw = {k: (list(rec[k]) if rec[k].shape else rec[k].item()) for k in rec.dtype.names}
+        ref = O.network_loglike(f, data, [psd] * 3, dets, O.make_params(**kw), G.PPE_IMRPHENOMD_INSPIRAL)
+        assert abs(ll[i] - ref) <= LL_TOL * max(abs(ref), 1), (i, ll[i], ref)
+    mx = G.loglike_batch(ev, P_, G.PPE_IMRPHENOMD_INSPIRAL, G.MIXED)
+    ok = np.isfinite(ll)
+    assert np.isnan(mx[bad]) and np.max(np.abs(mx[ok] - ll[ok])) <= 1e-3
+    ev.close()
+
+
+def test_theory_method_strings_through_gwatpy():
+    """"dCS_IMRPhenomD" and the like name the theory and, through the mapping, the ppE method; the handle's
+    cosmology string picks the redshift fit"""
+    f = W.uniform_grid(20., 512., 0.25)
+    for method, th, beta, cosmo, model in [("dCS_IMRPhenomD", G.THEORY_DCS, [1e-17], "PLANCK15",
+                                            G.PPE_IMRPHENOMD_INSPIRAL),
+                                           ("EdGB_HO_IMRPhenomD", G.THEORY_EDGB, [1e-20], "wmap9",
+                                            G.PPE_IMRPHENOMD_INSPIRAL),
+                                           ("EdGB_GHOv3_IMRPhenomPv2", G.THEORY_EDGB_GHOV3, [1e-20, 1.5], "WMAP7",
+                                            G.PPE_IMRPHENOMPV2_INSPIRAL),
+                                           ("TVG_IMRPhenomD", G.THEORY_TVG, [1e-3], "Planck13",
+                                            G.PPE_IMRPHENOMD_INSPIRAL)]:
+        p = P.gen_params(mass1=36, mass2=29, spin1=(0, 0, .3), spin2=(0, 0, -.1), Luminosity_Distance=410.,
+                         incl_angle=2.9, RA=1.37, DEC=-1.26, psi=2.66, gmst=W.GMST, tc=0.01, phiRef=0.3, f_ref=20.,
+                         cosmology=cosmo, Nmod=len(beta), betappe=beta)
+        st, hp, hc = P.fourier_waveform(f, method, p)
+        assert st == G.GWH_OK, method
+        o = O.make_params(**CASES[0], **EXTR, ppe_theory=th, ppe_nmod=len(beta), ppe_beta=beta,
+                          cosmology=["PLANCK15", "PLANCK13", "WMAP9", "WMAP7", "WMAP5"].index(cosmo.upper()))
+        wf_close(hp, O.fourier_waveform(f, o, model)[0])
+        p.close()
+    p = P.gen_params(mass1=36, mass2=29, Nmod=1, betappe=[1.])
+    st, _, _ = P.fourier_waveform(f, "ModDispersion_IMRPhenomD", p)
+    assert st == G.GWH_ERR_UNSUPPORTED
+    p.close()

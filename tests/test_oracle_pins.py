@@ -606,3 +606,102 @@ def test_gimr_fisher_repeats_and_unread_indices_are_zero_rows():
     F = O.fisher_numerical(f, psd, "gIMRPhenomD", 0, 0, 14, p)
     assert F[11, 11] > 0
     assert np.all(F[12] == 0) and np.all(F[13] == 0) and np.all(F[:, 12] == 0) and np.all(F[:, 13] == 0)
+
+
+# ---- theories mapped onto ppE terms (ppE_utilities.cpp:158-360, waveform_generator.cpp:1379-1404) ----------
+@pytest.mark.parametrize("cosmo,H0,Om", [(O.COSMO_PLANCK15, 67.74, 0.3075), (O.COSMO_PLANCK13, 67.77, 0.30712),
+                                         (O.COSMO_WMAP9, 69.32, 0.2865), (O.COSMO_WMAP7, 70.4, 0.272),
+                                         (O.COSMO_WMAP5, 70.2, 0.277)])
+def test_redshift_fits_against_a_flat_lcdm_integral(cosmo, H0, Om):
+    """Z_from_DL's fits (astropy cosmologies, include/gwat/D_Z_Config.h) against D_L(z) = (1+z) c/H0
+    int_0^z dz'/E(z'), E^2 = Om (1+z)^3 + 1 - Om, inverted by root finding: within 3e-3 from 0.01 Mpc to
+    100 Gpc (radiation and massive neutrinos, which the fits include, account for the rest); the next
+    cosmology's H0 differs by more than that, so a wrong cosmology index or segment fails"""
+    from scipy.integrate import quad
+    from scipy.optimize import brentq
+    ckm = 299792.458
+
+    def dl(z):
+        return (1 + z) * ckm / H0 * quad(lambda x: (Om * (1 + x) ** 3 + 1 - Om) ** -0.5, 0, z, epsrel=1e-12)[0]
+
+    for d in (0.01, 0.5, 1.0, 10., 100., 400., 1000., 5000., 20000., 100000.):
+        z = brentq(lambda x: dl(x) - d, 1e-12, 40)
+        assert abs(O.z_from_dl(d, cosmo) - z) <= 3e-3 * z, (d, O.z_from_dl(d, cosmo), z)
+    assert O.z_from_dl(1e6, cosmo) == -1  # past the table
+
+
+def _theory(th, beta, **extra):
+    kw = dict(mass1=36., mass2=29., spin1=[0, 0, .3], spin2=[0, 0, -.1], DL=410., incl=.4, f_ref=20., phiRef=.3,
+              tc=0.)
+    kw.update(extra)
+    return O.make_params(ppe_theory=th, ppe_nmod=len(beta), ppe_beta=list(beta), **kw)
+
+
+def test_theory_waveform_is_the_mapped_ppe_inspiral_waveform():
+    """a theory's waveform is the ppE_IMRPhenomD_Inspiral / IMR waveform of its mapped terms (the mapping
+    replaces the terms before the phase is built), bit for bit"""
+    kw, f = _gimr_base()
+    for th, beta, model in [(O.THEORY_DCS, [3e-22], O.PPE_IMRPHENOMD_INSPIRAL),
+                            (O.THEORY_EDGB_GHOV3, [2e-22, 1.5], O.PPE_IMRPHENOMD_INSPIRAL),
+                            (O.THEORY_TVG, [1e-11], O.PPE_IMRPHENOMPV2_INSPIRAL),
+                            (O.THEORY_PPEALT, [1e-3, 2e-2], O.PPE_IMRPHENOMD_IMR)]:
+        p = _theory(th, beta, ppe_b=[-7., -3.])
+        b, be = O.theory_terms(p)
+        q = O.make_params(**kw, ppe_nmod=len(b), ppe_b=list(b), ppe_beta=list(be))
+        a = O.fourier_waveform(f, p, model)
+        c = O.fourier_waveform(f, q, model)
+        assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1]), th
+
+
+def test_edgb_variants_against_each_other():
+    """EdGB_GHOv2 with a unit generic coefficient repeats EdGB's beta at b = -5; GHOv3 multiplies it by the GR
+    1PN phase coefficient 3715/756 + 55 eta / 9 (taken from the oracle's PN coefficients); in the small-spin
+    limit the EdGB charges tend to 1 and EdGB's beta to -(EdGB_HO_LO's), both reducing to (1 - 4 eta) /
+    eta^(18/5) (the reference's two conventions differ in sign)"""
+    a = 2e-22
+    bE, eE = O.theory_terms(_theory(O.THEORY_EDGB, [a]))
+    b2, e2 = O.theory_terms(_theory(O.THEORY_EDGB_GHOV2, [a, 1.0]))
+    b3, e3 = O.theory_terms(_theory(O.THEORY_EDGB_GHOV3, [a, 1.0]))
+    assert list(b2) == [-7, -5] and list(bE) == [-7]
+    assert e2[0] == eE[0] and abs(e2[1] - eE[0]) <= 1e-15 * abs(eE[0])
+    phi2 = O.pn_phase_coefficients(O.make_params(mass1=36., mass2=29.))[2]
+    assert abs(e3[1] - e2[1] * phi2) <= 1e-12 * abs(e3[1])
+    b1, e1 = O.theory_terms(_theory(O.THEORY_EDGB_GHOV1, [a, 0.37]))
+    assert e1[0] == eE[0] and e1[1] != 0
+    small = dict(spin1=[0, 0, 1e-4], spin2=[0, 0, -3e-4])
+    _, eS = O.theory_terms(_theory(O.THEORY_EDGB, [a], **small))
+    _, eH = O.theory_terms(_theory(O.THEORY_EDGB_HO_LO, [a], **small))
+    assert abs(eS[0] / eH[0] + 1) < 1e-6, eS[0] / eH[0]
+    # the charges vanish below the reference's threshold: EdGB of non-spinning holes has no term
+    assert O.theory_terms(_theory(O.THEORY_EDGB, [a], spin1=[0, 0, 0], spin2=[0, 0, 0]))[1][0] == 0
+
+
+def test_dcs_equal_mass_equal_spin_closed_form():
+    """equal masses and spins cancel the scalar-charge term (m1 s2 - m2 s1 = 0): beta = 16 pi alpha^2 / M_src^4
+    (15075/114688) 4^(14/5) chi^2 (2 - 350/201) / 4, with M_src = M / (1 + z) at the fitted redshift; spins of
+    both signs, and the redshift enters as (1 + z)^4"""
+    for chi in (0.4, -0.7):
+        p = _theory(O.THEORY_DCS, [1e-22], mass1=30., mass2=30., spin1=[0, 0, chi], spin2=[0, 0, chi], DL=900.)
+        b, be = O.theory_terms(p)
+        z = O.z_from_dl(900.)
+        Ms = 60. * MSOL_SEC / (1 + z)
+        ref = 16 * np.pi * 1e-22 / Ms ** 4 * (15075. / 114688.) * 4 ** 2.8 * chi ** 2 * (2 - 350. / 201.) / 4
+        assert list(b) == [-1] and abs(be[0] - ref) <= 1e-12 * abs(ref), (be[0], ref)
+
+
+def test_pnseries_is_ppealt_with_scaled_coefficients():
+    """PNSeries_beta's terms beyond the first carry beta_0 beta_i where ppEAlt_beta carries beta_i; both convert
+    the M f basis to Mc f by (M/Mc)^(b/3) (two code paths of the oracle)"""
+    b = [-7., -5., -3.]
+    _, s = O.theory_terms(_theory(O.THEORY_PNSERIES, [2e-3, 0.5, -4.], ppe_b=b))
+    _, t = O.theory_terms(_theory(O.THEORY_PPEALT, [2e-3, 2e-3 * 0.5, 2e-3 * -4.], ppe_b=b))
+    assert np.allclose(s, t, rtol=1e-15, atol=0)
+    eta = 36. * 29. / 65. ** 2
+    assert abs(t[0] - 2e-3 * eta ** (-0.6 * -7. / 3.)) <= 1e-12 * abs(t[0])  # M / Mc = eta^(-3/5)
+
+
+def test_theory_refusals():
+    """GHO variants read two theory parameters; a theory outside the list, an unknown cosmology: no terms"""
+    assert O.theory_terms(_theory(O.THEORY_EDGB_GHOV1, [1e-22])) is None
+    assert O.theory_terms(_theory(14, [1e-22])) is None
+    assert O.theory_terms(_theory(O.THEORY_DCS, [1e-22], cosmology=9)) is None
