@@ -341,8 +341,10 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
 /* 64 instances per block, four waves (roles) per instance group: each lane computes one part of its
  * instance's record, the parts meet in LDS (see gwh_model.h for the split).  256 threads per block leave
  * each lane the whole register file of its SIMD, so the FP64 working set needs no scratch memory.  Every
- * wave reaches both barriers; lanes without an instance only skip the work. */
-template <int MODEL>
+ * wave reaches both barriers; lanes without an instance only skip the work.  MOD: the ppE / gIMR / theory
+ * modifications are compiled in (the GR models run a kernel without them, which keeps its registers and
+ * code to what GR needs). */
+template <int MODEL, bool MOD>
 __global__ __launch_bounds__(256) void gwh_setup_kernel(const gwh_params *__restrict__ params,
                                                         const int *__restrict__ inst_event, int event0, int n,
                                                         QnmTable qnm,
@@ -350,8 +352,9 @@ __global__ __launch_bounds__(256) void gwh_setup_kernel(const gwh_params *__rest
                                                         const EventDesc *__restrict__ events,
                                                         InstCoef *__restrict__ coef, int *__restrict__ active,
                                                         const int *__restrict__ d_count, const int *__restrict__ d_mask,
-                                                        int ppe)
+                                                        int ppe_variant)
 {
+    const int ppe = MOD ? ppe_variant : 0;
     __shared__ SetupShared sh[64];
     constexpr bool pv2 = (MODEL == GWH_IMRPHENOMPV2);
     const int lane = threadIdx.x & 63;
@@ -643,14 +646,11 @@ static void launch_setup(int model, int n, hipStream_t st, const gwh_params *par
 {
     Engine &E = gwh_engine();
     dim3 g((n + 63) / 64), b(256);
-    if (gwh_is_pv2(model))
-        hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMPV2>, g, b, 0, st, params, inst_event, event0, n,
-                           E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
-                           coef, active, d_count, d_mask, gwh_ppe_variant(model));
-    else
-        hipLaunchKernelGGL(gwh_setup_kernel<GWH_IMRPHENOMD>, g, b, 0, st, params, inst_event, event0, n,
-                           E.qnm, (const DetGeom *)E.d_geo, (const EventDesc *)E.d_events,
-                           coef, active, d_count, d_mask, gwh_ppe_variant(model));
+    const int var = gwh_ppe_variant(model);
+    auto k = gwh_is_pv2(model) ? (var ? gwh_setup_kernel<GWH_IMRPHENOMPV2, true> : gwh_setup_kernel<GWH_IMRPHENOMPV2, false>)
+                               : (var ? gwh_setup_kernel<GWH_IMRPHENOMD, true> : gwh_setup_kernel<GWH_IMRPHENOMD, false>);
+    hipLaunchKernelGGL(k, g, b, 0, st, params, inst_event, event0, n, E.qnm, (const DetGeom *)E.d_geo,
+                       (const EventDesc *)E.d_events, coef, active, d_count, d_mask, var);
 }
 
 int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_params *d_params, int n, InstCoef *d_coef,

@@ -110,7 +110,9 @@ __host__ __device__ inline int gwh_theory_terms(const gwh_params &p, double M, d
     if (th == GWH_THEORY_PNSERIES || th == GWH_THEORY_PPEALT) { /* PNSeries_beta / ppEAlt_beta (:408-443) */
         const double m1 = p.mass1 * GWH_MSOL_SEC, m2 = p.mass2 * GWH_MSOL_SEC;
         const double ratio = (m1 + m2) / (pow(m1 * m2, 3. / 5.) / pow(m1 + m2, 1. / 5.));
-        for (int i = 0; i < n; i++) {
+#pragma unroll
+        for (int i = 0; i < GWH_PPE_MAX; i++) { /* constant indices: the terms stay in registers */
+            if (i >= n) break;
             b[i] = p.ppe_b[i];
             const double amp = (th == GWH_THEORY_PNSERIES && i > 0) ? a0 * p.ppe_beta[i] : p.ppe_beta[i];
             beta[i] = amp * pow(ratio, b[i] / 3.);
