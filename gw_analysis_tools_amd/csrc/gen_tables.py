@@ -15,6 +15,9 @@ text and re-emitted here in this project's own layout:
   include/gwat/D_Z_Config.h          cosmos, num_segments, interp_degree, boundaries_D, COEFF_VEC_DZ
                                      (redshift from luminosity distance: per cosmology, segments of a
                                      power series in sqrt(D_L / Mpc), fitted to astropy.cosmology)
+  include/gwat/D_Z_Config_modified_dispersion.h  MD_alphas, num_MD_segments, interp_MD_degree,
+                                     MD_boundaries_Z, MD_COEFF_VEC_ZD (the modified-dispersion distance
+                                     D_alpha(z) of Planck15: segments of a series in z^(k/2 - 7/2))
 
 Run from the repository root (only needed when the reference tables change):
     python gw_analysis_tools_amd/csrc/gen_tables.py /root/reference
@@ -81,6 +84,7 @@ def main(ref):
     phd = _strip_comments(open(os.path.join(inc, "IMRPhenomD.h")).read())
     det = _strip_comments(open(os.path.join(inc, "detector_util.h")).read())
     dz = _strip_comments(open(os.path.join(inc, "D_Z_Config.h")).read())
+    md = _strip_comments(open(os.path.join(inc, "D_Z_Config_modified_dispersion.h")).read())
 
     n = int(re.search(r"QNMData_length\s*=\s*(\d+)", qnm).group(1))
     spin = _numbers(_array_body(qnm, "QNMData_a"))
@@ -106,6 +110,15 @@ def main(ref):
     cdz = _numbers(_array_body(dz, "COEFF_VEC_DZ"))
     S, D = max(nseg), max(deg)
     assert len(nseg) == len(deg) == nc and len(bnd) == nc * (S + 1) and len(cdz) == nc * S * D, (nc, len(bnd), len(cdz))
+
+    md_alpha = _numbers(_array_body(md, "MD_alphas"))
+    na = len(md_alpha)
+    md_nseg = [int(v) for v in _numbers(_array_body(md, "num_MD_segments"))]
+    md_deg = [int(v) for v in _numbers(_array_body(md, "interp_MD_degree"))]
+    md_bnd = _numbers(_array_body(md, "MD_boundaries_Z"))
+    md_c = _numbers(_array_body(md, "MD_COEFF_VEC_ZD"))
+    MS, MD = max(md_nseg), max(md_deg)
+    assert len(md_nseg) == len(md_deg) == na and len(md_bnd) == na * (MS + 1) and len(md_c) == na * MS * MD
 
     lines = []
     lines.append("/* Generated by gen_tables.py -- do not edit by hand.")
@@ -166,6 +179,27 @@ def main(ref):
         for s in range(S):
             o = (c * S + s) * D
             lines.append("    {" + ", ".join(repr(v) for v in cdz[o:o + D]) + "},")
+        lines.append("  },")
+    lines.append("};")
+    lines.append("")
+    lines.append("/* modified-dispersion distance (Planck15): alpha gwh_md_alpha[a], segment s (z below gwh_md_z_bound[a][s + 1])")
+    lines.append(" * holds D_alpha / Mpc = sum_k gwh_md_dz[a][s][k] z^(k/2 - 7/2), k < gwh_md_degree[a] */")
+    lines.append("#define GWH_MD_NALPHA %d" % na)
+    lines.append("#define GWH_MD_SEG %d" % MS)
+    lines.append("#define GWH_MD_DEG %d" % MD)
+    lines.append("static GWH_FIT_CONST double gwh_md_alpha[GWH_MD_NALPHA] = {%s};" % ", ".join(repr(v) for v in md_alpha))
+    lines.append("static const int gwh_md_nseg[GWH_MD_NALPHA] = {%s};" % ", ".join(str(v) for v in md_nseg))
+    lines.append("static const int gwh_md_degree[GWH_MD_NALPHA] = {%s};" % ", ".join(str(v) for v in md_deg))
+    lines.append("static GWH_FIT_CONST double gwh_md_z_bound[GWH_MD_NALPHA][GWH_MD_SEG + 1] = {")
+    for a in range(na):
+        lines.append("  {" + ", ".join(repr(v) for v in md_bnd[(MS + 1) * a:(MS + 1) * (a + 1)]) + "},")
+    lines.append("};")
+    lines.append("static GWH_FIT_CONST double gwh_md_dz[GWH_MD_NALPHA][GWH_MD_SEG][GWH_MD_DEG] = {")
+    for a in range(na):
+        lines.append("  {")
+        for s in range(MS):
+            o = (a * MS + s) * MD
+            lines.append("    {" + ", ".join(repr(v) for v in md_c[o:o + MD]) + "},")
         lines.append("  },")
     lines.append("};")
     lines.append("")

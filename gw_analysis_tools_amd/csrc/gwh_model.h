@@ -184,6 +184,26 @@ __host__ __device__ inline int gwh_theory_terms(const gwh_params &p, double M, d
         b[0] = -7;
         beta[0] = (-3. / 224.) * pow(eta, 2. / 5.) * a0;
         break;
+    case GWH_THEORY_MODDISPERSION: { /* ModDispersion_beta (:757-772) with DL_from_Z_MD (:789-812) */
+        const double al = (p.ppe_b[0] + 3.) / 3.;
+        int ia = -1;
+        for (int a = 0; a < GWH_MD_NALPHA; a++)
+            if (ia < 0 && fabs(al - gwh_md_alpha[a]) < 1e-10) ia = a;
+        if (ia < 0) return -1; /* a dispersion power without a distance fit */
+        double D = -1;
+        for (int s = 0; s < GWH_MD_SEG; s++) {
+            if (z < gwh_md_z_bound[ia][s + 1]) {
+                D = 0;
+                for (int k = 0; k < GWH_MD_DEG; k++) D += gwh_md_dz[ia][s][k] * pow(z, -3.5 + k * 0.5);
+                break;
+            }
+        }
+        D *= GWH_MPC_SI / GWH_C_SI;
+        b[0] = p.ppe_b[0];
+        beta[0] = (pow(GWH_PI, 2. - al) / (1. - al)) * (D * a0 / pow(4.135667696e-15, 2. - al)) *
+                  (pow(Mc, 1. - al) / pow(1 + z, 1. - al));
+        break;
+    }
     default: /* GWH_THEORY_NONCOMM, NonComm_beta (:721-726) */
         b[0] = -1;
         beta[0] = (-75. / 256.) * pow(eta, -4. / 5.) * (2. * eta - 1.) * a0;

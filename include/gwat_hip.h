@@ -75,7 +75,8 @@ extern "C" {
 #define GWH_THEORY_NONCOMM 11       /* b = -1, from Lambda^2 */
 #define GWH_THEORY_PNSERIES 12      /* b as given; beta_0 (M/Mc)^(b_0/3), beta_0 beta_i (M/Mc)^(b_i/3) */
 #define GWH_THEORY_PPEALT 13        /* b as given; beta_i (M/Mc)^(b_i/3) */
-#define GWH_THEORY_COUNT 14
+#define GWH_THEORY_MODDISPERSION 14 /* b = ppe_b[0] (alpha = b/3 + 1 in 0, 0.5, .., 4), IMR variant; from A_alpha */
+#define GWH_THEORY_COUNT 15
 
 /* cosmologies of the redshift fits (src/util.cpp:356-382, include/gwat/D_Z_Config.h) */
 #define GWH_COSMO_PLANCK15 0

@@ -31,8 +31,8 @@ namespace gwh {
  * (waveform_generator.cpp:103-294 dispatch); -1 otherwise */
 /* A theory method string (check_theory_support, ppE_utilities.cpp:84-135) -> GWH_THEORY_*, and in *model the ppE
  * method assign_mapping (:158-360) maps it to: the _Inspiral variant, or for PNSeries / ppEAlt the variant the
- * string names.  0: not a theory string; -1: a theory the engine does not map (ModDispersion, polarization_test,
- * the NRT / EA variants, another waveform family). */
+ * string names, ModDispersion the _IMR one.  0: not a theory string; -1: a theory the engine does not map
+ * (polarization_test, the NRT / EA variants, another waveform family). */
 inline int theory_id(const std::string &m, int *model = nullptr)
 {
     auto has = [&](const char *s) { return m.find(s) != std::string::npos; };
@@ -49,7 +49,8 @@ inline int theory_id(const std::string &m, int *model = nullptr)
     else if (has("TVG")) th = GWH_THEORY_TVG;
     else if (has("DipRad")) th = GWH_THEORY_DIPRAD;
     else if (has("NonComm")) th = GWH_THEORY_NONCOMM;
-    else if (has("ModDispersion") || has("polarization_test")) return -1;
+    else if (has("ModDispersion")) th = GWH_THEORY_MODDISPERSION, ins = false;
+    else if (has("polarization_test")) return -1;
     else if (has("PNSeries_ppE")) th = GWH_THEORY_PNSERIES, ins = has("Inspiral");
     else if (has("ppEAlt")) th = GWH_THEORY_PPEALT, ins = has("Inspiral");
     else return 0;

@@ -24,7 +24,7 @@ PV2_FAMILY = (IMRPHENOMPV2, PPE_IMRPHENOMPV2_INSPIRAL, PPE_IMRPHENOMPV2_IMR, GIM
 # theories mapped onto the ppE models (ppe_theory; ppE_utilities.cpp:158-360) and the redshift cosmologies
 (THEORY_NONE, THEORY_DCS, THEORY_EDGB, THEORY_EDGB_HO_LO, THEORY_EDGB_GHOV1, THEORY_EDGB_GHOV2, THEORY_EDGB_GHOV3,
  THEORY_EXTRADIMENSION, THEORY_BHEVAPORATION, THEORY_TVG, THEORY_DIPRAD, THEORY_NONCOMM, THEORY_PNSERIES,
- THEORY_PPEALT) = range(14)
+ THEORY_PPEALT, THEORY_MODDISPERSION) = range(15)
 COSMO_PLANCK15, COSMO_PLANCK13, COSMO_WMAP9, COSMO_WMAP7, COSMO_WMAP5 = range(5)
 
 DETECTORS = {"Hanford": 0, "Livingston": 1, "Virgo": 2, "Kagra": 3, "Indigo": 4,

@@ -38,9 +38,12 @@ enum {
 enum {
     GWO_THEORY_NONE = 0, GWO_THEORY_DCS, GWO_THEORY_EDGB, GWO_THEORY_EDGB_HO_LO, GWO_THEORY_EDGB_GHOV1,
     GWO_THEORY_EDGB_GHOV2, GWO_THEORY_EDGB_GHOV3, GWO_THEORY_EXTRADIMENSION, GWO_THEORY_BHEVAPORATION,
-    GWO_THEORY_TVG, GWO_THEORY_DIPRAD, GWO_THEORY_NONCOMM, GWO_THEORY_PNSERIES, GWO_THEORY_PPEALT, GWO_THEORY_COUNT
+    GWO_THEORY_TVG, GWO_THEORY_DIPRAD, GWO_THEORY_NONCOMM, GWO_THEORY_PNSERIES, GWO_THEORY_PPEALT,
+    GWO_THEORY_MODDISPERSION, GWO_THEORY_COUNT
 };
 double gwo_z_from_dl(double dl_mpc, int cosmo);
+int gwo_md_index(double alpha);
+double gwo_md_distance(double z, double alpha);
 
 #define GWO_IS_PV2(m) \
     ((m) == GWO_IMRPHENOMPV2 || (m) == GWO_PPE_IMRPHENOMPV2_INSPIRAL || (m) == GWO_PPE_IMRPHENOMPV2_IMR || \

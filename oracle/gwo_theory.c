@@ -26,6 +26,29 @@ double gwo_z_from_dl(double dl_mpc, int cosmo)
     return -1;
 }
 
+/* dispersion_lookup (ppE_utilities.cpp:827-836) */
+int gwo_md_index(double alpha)
+{
+    for (int i = 0; i < GWH_MD_NALPHA; i++)
+        if (fabs(alpha - gwh_md_alpha[i]) < GWO_DOUBLE_COMP_THRESH) return i;
+    return -1;
+}
+
+/* DL_from_Z_MD (ppE_utilities.cpp:789-812) with cosmology_interpolation_function_MD (:814-821), Mpc */
+double gwo_md_distance(double z, double alpha)
+{
+    int a = gwo_md_index(alpha);
+    if (a < 0) return -1;
+    for (int i = 0; i < gwh_md_nseg[a]; i++) {
+        if (z < gwh_md_z_bound[a][i + 1]) {
+            double result = 0;
+            for (int k = 0; k < gwh_md_degree[a]; k++) result += gwh_md_dz[a][i][k] * pow(z, -3.5 + k * 0.5);
+            return result;
+        }
+    }
+    return -1;
+}
+
 static double mass1_of(double mc, double eta) /* calculate_mass1 (util.cpp:1516-1520) */
 {
     double etapow = pow(eta, 3. / 5);
@@ -130,6 +153,15 @@ int gwo_theory_terms(const gwo_params *p, double M, double mc, double eta, doubl
         bppe[0] = -1;
         betappe[0] = (-75. / 256.) * pow(eta, -4. / 5.) * (2. * eta - 1.) * in[0];
         return 1;
+    case GWO_THEORY_MODDISPERSION: { /* ModDispersion_beta (:757-772): alpha from the given exponent */
+        double alpha = (p->ppe_b[0] + 3.) / 3.;
+        double Dalpha = gwo_md_distance(Z, alpha) * (GWO_MPC_SI / GWO_C_SI);
+        if (gwo_md_index(alpha) < 0) return -1;
+        bppe[0] = p->ppe_b[0];
+        betappe[0] = (pow(M_PI, 2. - alpha) / (1. - alpha)) * (Dalpha * in[0] / pow(4.135667696e-15, 2. - alpha)) *
+                     (pow(mc, 1. - alpha) / pow(1 + Z, 1. - alpha));
+        return 1;
+    }
     default: { /* PNSeries_beta / ppEAlt_beta (:408-443): the given exponents, the M f basis converted to Mc f */
         double m1 = p->mass1 * GWO_MSOL_SEC, m2 = p->mass2 * GWO_MSOL_SEC;
         double chirp = gwo_calculate_chirpmass(m1, m2), total = m1 + m2;

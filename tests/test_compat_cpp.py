@@ -138,5 +138,6 @@ def test_compat_header_with_reference_shaped_params(tmp_path):
     D, DI, PI = G.PPE_IMRPHENOMD_INSPIRAL, G.PPE_IMRPHENOMD_IMR, G.PPE_IMRPHENOMPV2_INSPIRAL
     assert out[7].split() == ["theories", "%d:%d" % (G.THEORY_DCS, D), "%d:%d" % (G.THEORY_EDGB, PI),
                               "%d:%d" % (G.THEORY_EDGB_HO_LO, D), "%d:%d" % (G.THEORY_EDGB_GHOV2, D),
-                              "%d:%d" % (G.THEORY_PNSERIES, DI), "%d:%d" % (G.THEORY_PPEALT, PI), "-1:-9", "-1:-9",
+                              "%d:%d" % (G.THEORY_PNSERIES, DI), "%d:%d" % (G.THEORY_PPEALT, PI),
+                              "%d:%d" % (G.THEORY_MODDISPERSION, DI), "-1:-9",
                               str(D), "MCMC_IMRPhenomPv2", str(G.COSMO_WMAP9)]

@@ -399,6 +399,7 @@ THEORIES = [  # (theory, theory parameters, exponents (PNSeries / ppEAlt), IMRPh
     (G.THEORY_NONCOMM, [0.05], [], G.PPE_IMRPHENOMD_INSPIRAL),
     (G.THEORY_PNSERIES, [1e-3, 0.5], [-7., -5.], G.PPE_IMRPHENOMD_IMR),
     (G.THEORY_PPEALT, [3e-5, -3e-3], [-7., -3.], G.PPE_IMRPHENOMD_INSPIRAL),
+    (G.THEORY_MODDISPERSION, [3e-15], [4.5], G.PPE_IMRPHENOMD_IMR),
 ]
 TO_PV2 = {G.PPE_IMRPHENOMD_INSPIRAL: G.PPE_IMRPHENOMPV2_INSPIRAL, G.PPE_IMRPHENOMD_IMR: G.PPE_IMRPHENOMPV2_IMR}
 
@@ -478,6 +479,6 @@ def test_theory_method_strings_through_gwatpy():
         wf_close(hp, O.fourier_waveform(f, o, model)[0])
         p.close()
     p = P.gen_params(mass1=36, mass2=29, Nmod=1, betappe=[1.])
-    st, _, _ = P.fourier_waveform(f, "ModDispersion_IMRPhenomD", p)
+    st, _, _ = P.fourier_waveform(f, "polarization_test_IMRPhenomD", p)
     assert st == G.GWH_ERR_UNSUPPORTED
     p.close()

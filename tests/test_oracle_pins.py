@@ -703,5 +703,37 @@ def test_pnseries_is_ppealt_with_scaled_coefficients():
 def test_theory_refusals():
     """GHO variants read two theory parameters; a theory outside the list, an unknown cosmology: no terms"""
     assert O.theory_terms(_theory(O.THEORY_EDGB_GHOV1, [1e-22])) is None
-    assert O.theory_terms(_theory(14, [1e-22])) is None
+    assert O.theory_terms(_theory(O.THEORY_MODDISPERSION + 1, [1e-22])) is None
     assert O.theory_terms(_theory(O.THEORY_DCS, [1e-22], cosmology=9)) is None
+
+
+@pytest.mark.parametrize("alpha", [0., 0.5, 1., 2., 2.5, 4.])
+def test_modified_dispersion_distance_against_its_integral(alpha):
+    """DL_from_Z_MD's fits (Planck15) against D_alpha = (1+z)^(1-alpha) c/H0 int_0^z (1+z')^(alpha-2)/E(z') dz'
+    (Mirshekari, Yunes, Will 2012, arXiv:1110.2720) with flat LCDM, within 3e-3 for z = 1e-4 .. 10; a neighbouring
+    alpha's fit differs by far more at z = 2"""
+    from scipy.integrate import quad
+    L = O.lib()
+    L.gwo_md_distance.restype = O.ctypes.c_double
+    L.gwo_md_distance.argtypes = [O.ctypes.c_double, O.ctypes.c_double]
+    H0, Om, ckm = 67.74, 0.3075, 299792.458
+
+    def D(z, a):
+        I = quad(lambda x: (1 + x) ** (a - 2) / np.sqrt(Om * (1 + x) ** 3 + 1 - Om), 0, z, epsrel=1e-12)[0]
+        return (1 + z) ** (1 - a) * ckm / H0 * I
+
+    for z in (1e-4, 0.01, 0.087, 0.5, 2.0, 10.):
+        assert abs(L.gwo_md_distance(z, alpha) / D(z, alpha) - 1) < 3e-3, z
+    other = alpha + 0.5 if alpha < 4 else alpha - 0.5
+    assert abs(L.gwo_md_distance(2.0, other) / D(2.0, alpha) - 1) > 0.1
+    assert L.gwo_md_distance(0.5, 1.25) == -1  # no fit for that power
+
+
+def test_moddispersion_terms():
+    """ModDispersion keeps the given exponent b = 3 alpha - 3 and maps A_alpha to beta (ModDispersion_beta); the
+    beta is linear in A_alpha; powers without a distance fit are refused"""
+    p = _theory(O.THEORY_MODDISPERSION, [3e-15], ppe_b=[4.5])
+    b, be = O.theory_terms(p)
+    _, be2 = O.theory_terms(_theory(O.THEORY_MODDISPERSION, [6e-15], ppe_b=[4.5]))
+    assert list(b) == [4.5] and be[0] != 0 and abs(be2[0] / be[0] - 2) < 1e-14
+    assert O.theory_terms(_theory(O.THEORY_MODDISPERSION, [3e-15], ppe_b=[1.])) is None
