@@ -47,6 +47,8 @@ static gwh_params non_parameter_options(const gwh_params &in)
     w.dep_postmerger = in.dep_postmerger;
     /* fisher.cpp:2541-2570: the ppE terms and the gIMR lists travel with the parameters */
     w.ppe_nmod = in.ppe_nmod;
+    w.ppe_theory = in.ppe_theory; /* a theory's map and redshift travel with its parameters (:2541-2545) */
+    w.cosmology = in.cosmology;
     for (int i = 0; i < GWH_PPE_MAX; i++) {
         w.ppe_b[i] = in.ppe_b[i];
         w.ppe_beta[i] = in.ppe_beta[i];

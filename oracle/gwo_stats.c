@@ -479,6 +479,8 @@ static void repack_non_parameter_options(gwo_params *w, const gwo_params *in)
     w->dep_postmerger = in->dep_postmerger;
     /* fisher.cpp:2541-2570: the ppE terms and the gIMR index lists travel with the parameters */
     w->ppe_nmod = in->ppe_nmod;
+    w->ppe_theory = in->ppe_theory; /* fisher.cpp:2540-2545: check_theory_support keeps the theory's terms */
+    w->cosmology = in->cosmology;
     for (int i = 0; i < 4; i++) {
         w->ppe_b[i] = in->ppe_b[i];
         w->ppe_beta[i] = in->ppe_beta[i];

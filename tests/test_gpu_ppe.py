@@ -457,6 +457,28 @@ def test_theory_network_loglike_batch_and_refusal():
     ev.close()
 
 
+@pytest.mark.parametrize("model,basis,gdim,th,val", [
+    (G.PPE_IMRPHENOMD_INSPIRAL, G.BASIS_PHENOMD_FULL, 11, G.THEORY_NONCOMM, 0.05),
+    (G.PPE_IMRPHENOMPV2_INSPIRAL, G.BASIS_PV2_FULL, 13, G.THEORY_DIPRAD, 1e-4)])
+def test_theory_fisher_vs_oracle(model, basis, gdim, th, val):
+    """a theory's parameter appended to the GR basis as betappe (fisher.cpp:52-64), the theory map in every
+    perturbed record"""
+    f = W.uniform_grid(20., 1024., 0.25)
+    psd = W.analytic_psd(f)
+    kw = dict(mass1=36., mass2=29., spin1=[0, 0, 0.3], spin2=[0, 0, -0.1], DL=410., tc=0., phiRef=0., f_ref=20.,
+              RA=1.37, DEC=-1.26, psi=2.66, incl=2.9, gmst=W.GMST, ppe_theory=th, ppe_nmod=1, ppe_beta=[val],
+              cosmology=G.COSMO_WMAP9)
+    method = "ppE_IMRPhenomD_Inspiral"
+    if model == G.PPE_IMRPHENOMPV2_INSPIRAL:
+        kw.update(spin1=[0.3, 0.1, 0.3], spin2=[-0.1, 0.2, -0.1], chip=0.35, phip=0.8)
+        method = "ppE_IMRPhenomPv2_Inspiral"
+    gp, op = G.make_params(**kw), O.make_params(**kw)
+    a = G.fisher_numerical(f, psd, basis, 0, 0, gdim + 1, gp, order=2, epsilon=EPS, model=model)
+    b = O.fisher_numerical(f, psd, method, 0, 0, gdim + 1, op, order=2, epsilon=EPS)
+    e, ed = _fnorm(a, b)
+    assert e < FISHER_TOL and ed < FISHER_TOL, (e, ed)
+
+
 def test_theory_method_strings_through_gwatpy():
     """"dCS_IMRPhenomD" and the like name the theory and, through the mapping, the ppE method; the handle's
     cosmology string picks the redshift fit"""

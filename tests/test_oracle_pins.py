@@ -737,3 +737,22 @@ def test_moddispersion_terms():
     _, be2 = O.theory_terms(_theory(O.THEORY_MODDISPERSION, [6e-15], ppe_b=[4.5]))
     assert list(b) == [4.5] and be[0] != 0 and abs(be2[0] / be[0] - 2) < 1e-14
     assert O.theory_terms(_theory(O.THEORY_MODDISPERSION, [3e-15], ppe_b=[1.])) is None
+
+
+@pytest.mark.parametrize("th,val,b", [(O.THEORY_NONCOMM, 0.05, -1.), (O.THEORY_DIPRAD, 1e-4, -7.)])
+def test_theory_fisher_parameter_element(th, val, b):
+    """a theory parameter whose beta is linear in it (beta = c x): its Fisher diagonal is c^2 times the ppE beta-beta
+    element at the same waveform (the theory map and redshift travel into every perturbed record), while the
+    GR block differs from the ppE one (beta also moves with the masses)"""
+    kw, f, psd, _, mc = _ppe_fisher_case()
+    p = O.make_params(ppe_theory=th, ppe_nmod=1, ppe_beta=[val], **kw)
+    bb, be = O.theory_terms(p)
+    c = be[0] / val
+    q = O.make_params(ppe_nmod=1, ppe_b=[b], ppe_beta=[be[0]], **kw)
+    F = O.fisher_numerical(f, psd, "ppE_IMRPhenomD_Inspiral", 0, 0, 12, p, epsilon=1e-6)
+    # the same step in beta (c eps) for the beta-beta element, so both difference quotients match
+    Fp = O.fisher_numerical(f, psd, "ppE_IMRPhenomD_Inspiral", 0, 0, 12, q, epsilon=1e-6 * abs(c))
+    Fg = O.fisher_numerical(f, psd, "ppE_IMRPhenomD_Inspiral", 0, 0, 12, q, epsilon=1e-6)
+    assert list(bb) == [b]
+    assert abs(F[11, 11] - c * c * Fp[11, 11]) <= 1e-5 * F[11, 11], (F[11, 11], c * c * Fp[11, 11])
+    assert np.max(np.abs(F[:11, :11] - Fg[:11, :11]) / np.sqrt(np.outer(np.diag(Fg)[:11], np.diag(Fg)[:11]))) > 1e-8
