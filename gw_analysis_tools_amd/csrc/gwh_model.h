@@ -69,16 +69,12 @@ __host__ __device__ inline double gwh_z_from_dl(double dl_mpc, int cosmo)
     return -1.0;
 }
 
-/* ppE terms a theory maps to (the mapping's Nmod), and how many theory parameters it reads from ppe_beta */
+/* ppE terms a theory maps to (the mapping's Nmod); each reads as many theory parameters from ppe_beta */
 __host__ __device__ inline int gwh_theory_nterms(int theory, int nmod)
 {
     if (theory >= GWH_THEORY_EDGB_GHOV1 && theory <= GWH_THEORY_EDGB_GHOV3) return 2;
     if (theory == GWH_THEORY_PNSERIES || theory == GWH_THEORY_PPEALT) return nmod;
     return 1;
-}
-__host__ __device__ inline int gwh_theory_ninputs(int theory, int nmod)
-{
-    return gwh_theory_nterms(theory, nmod);
 }
 
 /* the scalar charges of dCS (dCS_phase_factor, ppE_utilities.cpp:487-523) and EdGB (EdGB_phase_factor,
@@ -105,7 +101,7 @@ __host__ __device__ inline int gwh_theory_terms(const gwh_params &p, double M, d
     const int th = p.ppe_theory, nm = p.ppe_nmod;
     if (th <= GWH_THEORY_NONE || th >= GWH_THEORY_COUNT || p.cosmology < 0 || p.cosmology >= GWH_NCOSMO) return -1;
     const int n = gwh_theory_nterms(th, nm);
-    if (n < 1 || n > GWH_PPE_MAX || nm < gwh_theory_ninputs(th, nm)) return -1;
+    if (n < 1 || n > GWH_PPE_MAX || nm < n) return -1;
     const double a0 = p.ppe_beta[0];
     if (th == GWH_THEORY_PNSERIES || th == GWH_THEORY_PPEALT) { /* PNSeries_beta / ppEAlt_beta (:408-443) */
         const double m1 = p.mass1 * GWH_MSOL_SEC, m2 = p.mass2 * GWH_MSOL_SEC;
