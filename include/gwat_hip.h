@@ -126,9 +126,9 @@ typedef struct gwh_params {
     double chip, phip; /* chip <= -1 + 1e-10: precessing spins are taken from spin1/spin2 vectors */
     int32_t shift_time, shift_phase, sky_average, dep_postmerger;
     /* ppE terms (gen_params_base: Nmod, bppe, betappe), read only by the ppE models. ppe_theory (GWH_THEORY_*,
-     * 0: none) maps theory parameters held in ppe_beta[0 .. ppe_nmod) (and ppe_b for PNSeries / ppEAlt) to the
-     * ppE terms, as the reference does for its theory method strings (dCS_IMRPhenomD -> ppE_IMRPhenomD_Inspiral,
-     * src/ppE_utilities.cpp:158-360, src/waveform_generator.cpp:1379-1404) */
+     * 0: none) maps theory parameters held in ppe_beta[0 .. ppe_nmod) (and ppe_b for PNSeries / ppEAlt /
+     * ModDispersion) to the ppE terms, as the reference does for its theory method strings (dCS_IMRPhenomD ->
+     * ppE_IMRPhenomD_Inspiral, src/ppE_utilities.cpp:158-360, src/waveform_generator.cpp:1379-1404) */
     int32_t ppe_nmod, ppe_theory;
     double ppe_b[GWH_PPE_MAX];
     double ppe_beta[GWH_PPE_MAX];
