@@ -3,7 +3,13 @@
  * The reference turns a theory method string ("dCS_IMRPhenomD", "EdGB_GHOv2_IMRPhenomPv2", ...) into a ppE
  * method and ppE terms in prep_source_parameters (waveform_generator.cpp:1379-1404) with the mapping of
  * assign_mapping (ppE_utilities.cpp:158-360) and the beta functions (ppE_utilities.cpp:408-735); the redshift
- * comes from Z_from_DL (util.cpp:356-382, :502-512) over the fits of include/gwat/D_Z_Config.h. */
+ * comes from Z_from_DL (util.cpp:356-382, :502-512) over the fits of include/gwat/D_Z_Config.h.
+ *
+ * Pinned by tests/test_oracle_pins.py: the redshift and modified-dispersion distance fits against their defining
+ * integrals; dCS (equal-mass closed form), the EdGB family (against each other, the GR 1PN coefficient and the
+ * small-spin limit), PNSeries against ppEAlt, ModDispersion's linearity and refusals.  Parity unpinned: the
+ * one-line coefficients of ExtraDimension_beta, BHEvaporation_beta, TVG_beta, DipRad_beta and NonComm_beta,
+ * which have no check beyond this restatement. */
 #include <math.h>
 #include "gwo_internal.h"
 #include "../gw_analysis_tools_amd/csrc/gwh_tables.h"
