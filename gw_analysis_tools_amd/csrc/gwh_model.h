@@ -405,6 +405,14 @@ struct SetupState {
     double gimr_neg[4]; /* delta_phi of PN orders -4 .. -1 */
 };
 
+/* 2.5PN phase coefficient of IMRPhenomD (IMRPhenomD.cpp:1016-1018; also the coefficient of its log term) */
+__device__ __forceinline__ double gwh_pn_c8_gr(double eta, double dm, double ca, double cs)
+{
+    const double eta2 = eta * eta;
+    return (38645. * GWH_PI / 756 - 65. * GWH_PI * eta / 9 + dm * (-732985. / 2268 - 140. * eta / 9) * ca +
+            (-732985. / 2268 + 24260. * eta / 81 + 340. * eta2 / 9) * cs);
+}
+
 /* gIMR inspiral phase terms beyond the IMRPhenomD polynomial (gIMRPhenomD.cpp:148-184): the order-1 coefficient
  * (zero in IMRPhenomD) and 3/(128 eta) delta_i (pi M f)^((i - 5)/3) for i = -4..-1, with their f derivative */
 __device__ __forceinline__ double gwh_setup_gimr(const SetupState &S, double f, double *deriv)
@@ -494,8 +502,14 @@ __device__ __forceinline__ double gwh_setup_dphase_ins(const SetupState &S, doub
     if (S.gimr) { /* the order-1 coefficient is already in the polynomial derivative above */
         double dt;
         gwh_setup_gimr(S, f, &dt);
-        const double x = GWH_PI * S.M * f, iv = 1.0 / cbrt(x), iv4 = iv * iv * iv * iv;
-        r += dt - (3. / (128. * S.eta)) * S.pp[1] * (-4. / 3.) * iv4 / f;
+        const double iv4 = iv * iv * iv * iv;
+        r += dt - K * S.pp[1] * (-4. / 3.) * iv4 / f;
+        /* The reference differentiates the log terms of c5 and c6 with their GR coefficients whatever gIMR did
+         * to them (assign_nonstatic_pn_phase_coeff_deriv, IMRPhenomD.cpp:1068-1084, reached from gIMRPhenomD's
+         * Dphase_ins through the GR class, gIMRPhenomD.cpp:167-184): with phi orders 8 and 9 set, the
+         * connection slope at f1 is not the inspiral phase's own slope. */
+        const double c9gr = gwh_pn_c8_gr(S.eta, S.delta, S.chi_a, S.chi_s), c11gr = 6848. / 63.;
+        r += K * ((c9gr - S.pp[9]) - (c11gr - S.pp[11]) * v) / f;
     }
     return r;
 }
@@ -833,8 +847,7 @@ __device__ __forceinline__ void gwh_setup_pn(const gwh_params &p, SetupState &S)
     S.pp[7] = 77096675. * GWH_PI / 254016 + 378515. * GWH_PI * eta / 1512 - 74045. * GWH_PI * eta2 / 756 +
               dm * (-25150083775. / 3048192 + 26804935. * eta / 6048 - 1985. * eta2 / 48) * ca +
               (-25150083775. / 3048192 + 10566655595. * eta / 762048 - 1042165. * eta2 / 3024 + 5345. * eta3 / 36) * cs;
-    S.pp[8] = (38645. * GWH_PI / 756 - 65. * GWH_PI * eta / 9 + dm * (-732985. / 2268 - 140. * eta / 9) * ca +
-               (-732985. / 2268 + 24260. * eta / 81 + 340. * eta2 / 9) * cs);
+    S.pp[8] = gwh_pn_c8_gr(eta, dm, ca, cs);
     S.pp[9] = S.pp[8];
     S.pp[10] = 11583231236531. / 4694215680 - 6848. * GWH_GAMMA_E / 21 - 640. * pi2 / 3 +
                (-15737765635. / 3048192 + 2255. * pi2 / 12) * eta + 76055. * eta2 / 1728 - 127825. * eta3 / 1296 +

@@ -198,8 +198,8 @@ static void pn_amplitude_coeff(const gwo_src *s, double *c)
                  dm * ca * (31 * M_PI / 12 + (1614569. / 32256 - 165961 * eta / 2688) * cs));
 }
 
-/* IMRPhenomD.cpp:991-1032: TaylorF2 phase coefficients; [5],[6] are set per frequency */
-static void pn_phase_coeff(const gwo_src *s, double *c)
+/* IMRPhenomD.cpp:991-1032: TaylorF2 phase coefficients of GR; [5],[6] are set per frequency */
+static void pn_phase_coeff_gr(const gwo_src *s, double *c)
 {
     double dl = s->delta_mass, eta = s->eta, ca = s->chi_a, cs = s->chi_s;
     double eta2 = eta * eta, eta3 = eta2 * eta, cs2 = cs * cs, ca2 = ca * ca;
@@ -221,6 +221,12 @@ static void pn_phase_coeff(const gwo_src *s, double *c)
             (-15737765635. / 3048192 + 2255. * pi2 / 12) * eta + 76055. * eta2 / 1728 - 127825. * eta3 / 1296 +
             2270. * dl * ca * M_PI / 3 + (2270. * M_PI / 3 - 520. * M_PI * eta) * cs - 6848. * (GWO_LOG_64) / 63;
     c[11] = 6848. / 63.;
+}
+
+/* the phase coefficients of the model: GR, or scaled by gIMR */
+static void pn_phase_coeff(const gwo_src *s, double *c)
+{
+    pn_phase_coeff_gr(s, c);
     /* gIMRPhenomD::assign_static_pn_phase_coeff (gIMRPhenomD.cpp:89-143): orders 0, 2, 3, 4, 7 scaled; order 1
      * set to its delta; orders 5, 6 scale the log-free parts c[8], c[10]; "orders" 8, 9 the log parts c[9], c[11] */
     if (s->gimr) {
@@ -493,8 +499,12 @@ double gwo_dphase_ins(double f, gwo_src *s, double *pn, gwo_lambda *lam)
     double sum = 0;
     for (int i = 0; i < 8; i++) sum += pn[i] * ((i - 5.) / 3.) * pow(x, (i - 8.) / 3.);
     double d = K * (M_PI * M) * sum;
-    /* d c5/df = c9/f multiplies x^0; d c6/df = -c11/f multiplies x^(1/3) */
-    d += K * (pn[9] - pn[11] * pow(x, 1. / 3.)) / f;
+    /* d c5/df = c9/f multiplies x^0; d c6/df = -c11/f multiplies x^(1/3).  The reference takes c9 and c11 at
+     * their GR values here (assign_nonstatic_pn_phase_coeff_deriv, IMRPhenomD.cpp:1068-1084), also when
+     * gIMRPhenomD has scaled them (its Dphase_ins calls the GR one, gIMRPhenomD.cpp:167-184). */
+    double gr[12];
+    pn_phase_coeff_gr(s, gr);
+    d += K * (gr[9] - gr[11] * pow(x, 1. / 3.)) / f;
     double sig = lam->sigma[1] * M + lam->sigma[2] * pow(M, 4. / 3.) * pow(f, 1. / 3) +
                  lam->sigma[3] * pow(M, 5. / 3.) * pow(f, 2. / 3) + lam->sigma[4] * pow(M, 2.) * f;
     double out = d + (1. / eta) * sig;

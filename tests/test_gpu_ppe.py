@@ -167,6 +167,28 @@ def test_gimr_waveform_matches_oracle(case, mods):
     assert np.array_equal(gr, gp)
 
 
+SINGLE_GIMR = ([("gimr_dphi", i + 4, 14, 2e-4 if i < 0 else 0.05) for i in range(-4, 10)] +
+               [("gimr_dsigma", i, 3, 0.2) for i in range(3)] + [("gimr_dbeta", i, 2, 0.1) for i in range(2)] +
+               [("gimr_dalpha", i, 4, 0.1) for i in range(4)])
+
+
+@pytest.mark.parametrize("model", [G.GIMRPHENOMD, G.GIMRPHENOMPV2])
+@pytest.mark.parametrize("field,idx,n,delta", SINGLE_GIMR)
+def test_gimr_every_single_coefficient_matches_oracle(model, field, idx, n, delta):
+    """each phi order -4..9 (8 and 9: the log parts, connected with the GR log derivative as the reference
+    does) and each sigma / beta / alpha index on its own"""
+    f = W.uniform_grid(20., 1024., 0.25)
+    d = [0.] * n
+    d[idx] = delta
+    case = dict(CASES[0]) if model == G.GIMRPHENOMD else dict(mass1=36, mass2=29, spin1=[.3, .1, .2],
+                                                               spin2=[-.1, .2, .1])
+    g, o = both(**case, **EXTR, **{field: d})
+    hp, hc = G.fourier_waveform(f, g, model)
+    op, oc = O.fourier_waveform(f, o, model)
+    wf_close(hp, op)
+    wf_close(hc, oc)
+
+
 def test_gimr_network_loglike_batch():
     f, psd, dets, data = _net()
     ev = G.Event(f, data, [psd] * 3, dets)

@@ -399,6 +399,82 @@ def test_gimr_phase_deltas_equal_the_ppe_mapping(order):
     assert np.max(np.abs(g - p)) < 1e-9 * np.max(np.abs(gr)), np.max(np.abs(g - p))
 
 
+def _gimr_f1p(kw):
+    summ = np.zeros(32)
+    O.lib().gwo_phenomd_setup_summary(O.ctypes.byref(O.make_params(**kw)), O.IMRPHENOMD, O._dp(summ))
+    return summ[4]
+
+
+def test_gimr_order5_is_a_constant_and_order6_the_ppe_term_of_b1():
+    """phi order 5 scales the log-free 2.5PN coefficient c8: K delta c8 (pi M f)^0 is a constant in the
+    inspiral, carried unchanged through the C1 connections, so the phase differs from GR by a line only.
+    Order 6 scales the log-free 3PN coefficient c10: the ppE inspiral term b = 1 with
+    beta = (3/128) eta^(-6/5) c10 delta (the ppE-gIMR mapping with phi_6 = c10)"""
+    kw, f = _gimr_base()
+    gr = _gimr_phase(f, kw, O.IMRPHENOMD)
+    dphi = [0.] * 14
+    dphi[5 + 4] = 0.1
+    g5 = _gimr_phase(f, kw, O.GIMRPHENOMD, gimr_dphi=dphi)
+    assert _linear_residual(f, g5 - gr) < 1e-9 * np.max(np.abs(gr))
+    dphi = [0.] * 14
+    dphi[6 + 4] = 0.05
+    g6 = _gimr_phase(f, kw, O.GIMRPHENOMD, gimr_dphi=dphi)
+    eta = 36. * 29. / 65. ** 2
+    c10 = O.pn_phase_coefficients(O.make_params(**kw))[10]
+    beta = 3. / 128. * eta ** (-6. / 5.) * c10 * 0.05
+    p = _gimr_phase(f, kw, O.PPE_IMRPHENOMD_INSPIRAL, ppe_nmod=1, ppe_b=[1.], ppe_beta=[beta])
+    assert np.max(np.abs(g6 - gr)) > 1e-2
+    assert np.max(np.abs(g6 - p)) < 1e-9 * np.max(np.abs(gr)), np.max(np.abs(g6 - p))
+
+
+@pytest.mark.parametrize("order", [8, 9])
+def test_gimr_log_orders_connect_with_the_gr_log_derivative(order):
+    """phi "orders" 8 and 9 scale the log parts c9, c11 of the 2.5PN and 3PN coefficients: the inspiral phase
+    gains T(f) = K delta c9 ln x (order 8) or -K delta c11 x^(1/3) ln x (order 9), x = pi M f, K = 3/(128 eta).
+    The reference's connection slope at f1 = 0.018/M differentiates the log terms with the GR coefficients
+    (IMRPhenomD.cpp:1068-1084), so above f1 the phase carries T(f1) and only the part of T's slope that treats
+    the log as a constant: 0 (order 8), -K delta c11 ln x1 x1^(-2/3) pi M / 3 (order 9).  Below f1 the phase
+    is GR + T; the whole difference minus that model is one line, continuous at f1.  The self-consistent
+    slope (T'(f1) itself) leaves a kink of K delta c9 / f1 (order 8) or K delta c11 x1^(1/3) / f1 (order 9)
+    and fails."""
+    kw, f = _gimr_base()
+    delta = 0.1
+    dphi = [0.] * 14
+    dphi[order + 4] = delta
+    g = _gimr_phase(f, kw, O.GIMRPHENOMD, gimr_dphi=dphi)
+    gr = _gimr_phase(f, kw, O.IMRPHENOMD)
+    M = 65. * MSOL_SEC
+    eta = 36. * 29. / 65. ** 2
+    K = 3. / (128. * eta)
+    c = O.pn_phase_coefficients(O.make_params(**kw))
+    f1 = _gimr_f1p(kw)
+    x, x1 = np.pi * M * f, np.pi * M * f1
+    if order == 8:
+        T = lambda xx: K * delta * c[9] * np.log(xx)
+        slope_ref, slope_true = 0.0, K * delta * c[9] / f1
+    else:
+        T = lambda xx: -K * delta * c[11] * xx ** (1. / 3.) * np.log(xx)
+        slope_ref = -K * delta * c[11] * np.log(x1) * x1 ** (-2. / 3.) * np.pi * M / 3.
+        slope_true = slope_ref - K * delta * c[11] * x1 ** (1. / 3.) / f1
+    # fourier_phase returns -(Psi - tc (f - f_ref) - phic)
+    below = f < f1
+    ref_model = np.where(below, T(x), T(x1) + slope_ref * (f - f1))
+    true_model = np.where(below, T(x), T(x1) + slope_true * (f - f1))
+    scale = np.max(np.abs(gr))
+    assert np.max(np.abs(T(x))) > 1e-2  # the term is visible
+    assert _linear_residual(f, -(g - gr) - ref_model) < 1e-9 * scale
+    assert _linear_residual(f, -(g - gr) - true_model) > 1e-4
+    # the slope jump at f1 measured on the phase itself: one-sided cubic fits on either side
+    h = 1e-3 * f1
+    fl, fr = f1 - h * np.arange(1, 5), f1 + h * np.arange(1, 5)
+    pl, pr = -_gimr_phase(fl, kw, O.GIMRPHENOMD, gimr_dphi=dphi), -_gimr_phase(fr, kw, O.GIMRPHENOMD, gimr_dphi=dphi)
+    cl, cr = np.polyfit(fl - f1, pl, 3), np.polyfit(fr - f1, pr, 3)
+    assert abs(cl[-1] - cr[-1]) < 1e-6 * (1 + abs(cl[-1]))  # C0 at f1
+    jump = cr[-2] - cl[-2]
+    expected = slope_ref - slope_true
+    assert abs(jump - expected) < 1e-4 * abs(expected), (jump, expected)
+
+
 @pytest.mark.parametrize("field,idx,region", [("gimr_dsigma", 0, "ins"), ("gimr_dsigma", 2, "ins"),
                                                ("gimr_dbeta", 0, "int"), ("gimr_dbeta", 1, "int"),
                                                ("gimr_dalpha", 0, "mr"), ("gimr_dalpha", 3, "mr")])
