@@ -116,6 +116,10 @@ def lib():
         L.gwo_phenomd_setup_summary.argtypes = [P(Params), ctypes.c_int, dp]
         L.gwo_phenomd_pieces.argtypes = [P(Params), D, dp]
         L.gwo_pv2_setup_summary.argtypes = [P(Params), dp]
+        L.gwo_pv2_pieces.argtypes = [P(Params), D, dp]
+        L.gwo_pv2_aligned_phase.argtypes = [P(Params), D]
+        L.gwo_pv2_aligned_phase.restype = D
+        L.gwo_pv2_waveform_unrotated.argtypes = [dp, ctypes.c_int, dp, dp, P(Params)]
         L.gwo_pn_phase_coefficients.argtypes = [P(Params), ctypes.c_int, dp]
         L.gwo_fcut.argtypes = [P(Params)]
         L.gwo_fcut.restype = D
@@ -317,8 +321,30 @@ def phenomd_pieces(p, f):
 def pv2_setup_summary(p):
     out = np.zeros(16)
     n = lib().gwo_pv2_setup_summary(ctypes.byref(p), _dp(out))
-    keys = ["chil", "chip", "SP", "SL", "thetaJN", "alpha0", "zeta", "phi_aligned", "t_corr", "fRD", "fdamp"]
+    keys = ["chil", "chip", "SP", "SL", "thetaJN", "alpha0", "zeta", "phi_aligned", "t_corr", "fRD", "fdamp",
+            "dphase_mr_at_fRD"]
     return dict(zip(keys, out[:n]))
+
+
+def pv2_pieces(p, f):
+    """twist-up pieces at one frequency (gwo_pv2_pieces)"""
+    out = np.zeros(23)
+    lib().gwo_pv2_pieces(ctypes.byref(p), f, _dp(out))
+    return dict(cos_beta=out[0], d2=out[1:6], dm2=out[6:11], ylm=out[11:16], alpha=out[16], epsilon=out[17],
+                hp_factor=out[18] + 1j * out[19], hc_factor=out[20] + 1j * out[21], L2PN=out[22])
+
+
+def pv2_aligned_phase(p, f):
+    """the aligned-spin (IMRPhenomD) phase of a precessing parameter set at f, as the t_c correction splines it"""
+    return lib().gwo_pv2_aligned_phase(ctypes.byref(p), f)
+
+
+def pv2_waveform_unrotated(f, p):
+    """IMRPhenomPv2 h+, hx before the rotation by the polarisation angle zeta"""
+    f = _c(f)
+    hp, hc = np.zeros(2 * len(f)), np.zeros(2 * len(f))
+    lib().gwo_pv2_waveform_unrotated(_dp(f), len(f), _dp(hp), _dp(hc), ctypes.byref(p))
+    return hp[0::2] + 1j * hp[1::2], hc[0::2] + 1j * hc[1::2]
 
 
 def fcut(p):

@@ -141,6 +141,9 @@ int gwo_phenomd_setup_summary(const gwo_params *p, int model, double *out);
 int gwo_phenomd_pieces(const gwo_params *p, double f, double *out);
 /* Pv2 setup summary: chil, chip, SP, SL, thetaJN, alpha0, zeta, phi_aligned, t_corr */
 int gwo_pv2_setup_summary(const gwo_params *p, double *out);
+int gwo_pv2_pieces(const gwo_params *p, double f, double *out);
+double gwo_pv2_aligned_phase(const gwo_params *p, double f);
+int gwo_pv2_waveform_unrotated(const double *f, int n, double *hp, double *hc, const gwo_params *p);
 double gwo_fcut(const gwo_params *p);
 
 #ifdef __cplusplus

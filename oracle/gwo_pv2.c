@@ -347,7 +347,77 @@ int gwo_pv2_setup_summary(const gwo_params *p, double *out)
     gwo_precalc_powers_ins(s.f_ref, s.M, &pw);
     double t_corr = time_shift(&s, &st, &pw);
     double v[] = {s.chil, s.chip, s.SP, s.SL, s.thetaJN, s.alpha0, s.zeta_polariz, s.phi_aligned, t_corr,
-                  s.fRD, s.fdamp};
+                  s.fRD, s.fdamp, gwo_dphase_mr(s.fRD, &s, &st.lam)};
     memcpy(out, v, sizeof(v));
     return (int)(sizeof(v) / sizeof(double));
+}
+
+/* Pieces of the twist-up at one frequency, for pinning tests: out = [cos beta, d2[5] (d^2_{m,2}, m = -2..2),
+ * dm2[5] (d^2_{m,-2}), -2Y_2m(thetaJN)[5], alpha, epsilon (with the f_ref offsets and alpha0 applied),
+ * hp factor (re, im), hc factor (re, im), L2PN] */
+int gwo_pv2_pieces(const gwo_params *p, double f, double *out)
+{
+    gwo_src s;
+    gwo_populate_source(&s, p, GWO_IMRPHENOMPV2);
+    s.q = s.mass1 / s.mass2;
+    euler_coeffs ac, ec;
+    euler_coeffs_calc(&ac, &ec, &s);
+    gwo_pows pw;
+    gwo_precalc_powers_pi(&pw);
+    pw.MFthird = pow(s.M * s.f_ref, 1. / 3.);
+    pw.MF2third = pw.MFthird * pw.MFthird;
+    double a_off, e_off;
+    euler_angles(&a_off, &e_off, &pw, &ac, &ec);
+    gwo_precalc_powers_ins(f, s.M, &pw);
+    double d2[5], dm2[5], y[5];
+    wigner_d(d2, dm2, &pw, &s);
+    ylm2(s.thetaJN, y);
+    double alpha, eps;
+    euler_angles(&alpha, &eps, &pw, &ac, &ec);
+    alpha = alpha + s.alpha0 - a_off;
+    eps = eps - e_off;
+    double hp[2], hc[2];
+    twistup(alpha, d2, dm2, y, hp, hc);
+    double L = L2PN(s.eta, &pw);
+    out[0] = 1. / sqrt(1.0 + (s.SP / (L + s.SL)) * (s.SP / (L + s.SL)));
+    for (int i = 0; i < 5; i++) {
+        out[1 + i] = d2[i];
+        out[6 + i] = dm2[i];
+        out[11 + i] = y[i];
+    }
+    out[16] = alpha;
+    out[17] = eps;
+    out[18] = hp[0];
+    out[19] = hp[1];
+    out[20] = hc[0];
+    out[21] = hc[1];
+    out[22] = L;
+    return 23;
+}
+
+/* the waveform before the polarisation rotation by zeta (waveform_generator.cpp:257-266 applies it) */
+int gwo_pv2_waveform_unrotated(const double *f, int n, double *hp, double *hc, const gwo_params *p)
+{
+    gwo_src s;
+    gwo_populate_source(&s, p, GWO_IMRPHENOMPV2);
+    return gwo_pv2_waveform(f, n, hp, hc, &s);
+}
+
+/* the aligned-spin phase the t_c correction splines (IMRPhenomD's build_phase with the precessing final spin),
+ * at one frequency, for pinning tests */
+double gwo_pv2_aligned_phase(const gwo_params *p, double f)
+{
+    gwo_src s;
+    gwo_populate_source(&s, p, GWO_IMRPHENOMPV2);
+    gwo_phenomd_state st;
+    s.shift_time = 0;
+    s.shift_phase = 0;
+    gwo_phenomd_prepare(&s, &st);
+    gwo_pows pw;
+    gwo_precalc_powers_pi(&pw);
+    if (f < s.f1_phase)
+        gwo_precalc_powers_ins(f, s.M, &pw);
+    else
+        pw.MFsixth = pow(s.M * f, 1. / 6);
+    return gwo_build_phase(f, &s, &st, &pw);
 }

@@ -832,3 +832,154 @@ def test_theory_fisher_parameter_element(th, val, b):
     assert list(bb) == [b]
     assert abs(F[11, 11] - c * c * Fp[11, 11]) <= 1e-5 * F[11, 11], (F[11, 11], c * c * Fp[11, 11])
     assert np.max(np.abs(F[:11, :11] - Fg[:11, :11]) / np.sqrt(np.outer(np.diag(Fg)[:11], np.diag(Fg)[:11]))) > 1e-8
+
+
+# ---- IMRPhenomPv2 precession pieces (IMRPhenomP.cpp) --------------------------------------------------------
+def _wigner_d2(beta):
+    """d^2_{m'm}(beta) = <2 m'| exp(-i beta J_y) |2 m> from the spin-2 matrices (a matrix exponential, no
+    closed forms), indexed [m' + 2, m + 2]"""
+    from scipy.linalg import expm
+    ms = np.arange(-2, 3)
+    jp = np.zeros((5, 5))
+    for a, m in enumerate(ms[:-1]):
+        jp[a + 1, a] = np.sqrt(6 - m * (m + 1))  # J+ |2 m> = sqrt(j(j+1) - m(m+1)) |2 m+1>
+    jy = (jp - jp.T) / 2j
+    return expm(-1j * beta * jy).real
+
+
+def _pv2_cases():
+    return [dict(mass1=36., mass2=29., spin1=[.3, .1, .2], spin2=[-.1, .2, .1], DL=410., incl=.4, f_ref=20.,
+                 phiRef=.3),
+            dict(mass1=50., mass2=8., spin1=[.7, -.5, -.3], spin2=[.2, .0, .4], DL=900., incl=2.2, f_ref=20.,
+                 phiRef=1.9),
+            dict(mass1=12., mass2=10., spin1=[-.05, .9, .1], spin2=[.6, .6, -.2], DL=200., incl=1.4, f_ref=30.,
+                 phiRef=4.0)]
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_pv2_wigner_d_against_the_spin2_rotation_matrix(case):
+    """IMRPhenomP.cpp:664-693: d2[m+2] = d^2_{m,2}(beta), dm2[m+2] = d^2_{m,-2}(beta) with cos beta from the
+    2PN angular momentum; checked against exp(-i beta J_y) of spin 2, and unitarity sum_m |d^2_{m,2}|^2 = 1"""
+    p = O.make_params(**_pv2_cases()[case])
+    for f in (20., 45., 120., 300.):
+        pc = O.pv2_pieces(p, f)
+        beta = np.arccos(pc["cos_beta"])
+        assert beta > 1e-3  # precessing: the test is not the trivial beta = 0
+        D = _wigner_d2(beta)
+        np.testing.assert_allclose(pc["d2"], D[:, 4], rtol=0, atol=1e-14)
+        np.testing.assert_allclose(pc["dm2"], D[:, 0], rtol=0, atol=1e-14)
+        assert abs(np.sum(pc["d2"] ** 2) - 1) < 1e-14 and abs(np.sum(pc["dm2"] ** 2) - 1) < 1e-14
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_pv2_spin_weighted_harmonics(case):
+    """util.cpp:1953-2000 at s = -2, l = 2, phi = 0: -2Y_2m(theta) = sqrt(5 / 4 pi) d^2_{m,2}(theta) (Goldberg
+    et al.), and each is normalised on the sphere"""
+    from scipy.integrate import quad
+    p = O.make_params(**_pv2_cases()[case])
+    th = O.pv2_setup_summary(p)["thetaJN"]
+    y = O.pv2_pieces(p, 50.)["ylm"]
+    np.testing.assert_allclose(y, np.sqrt(5 / (4 * np.pi)) * _wigner_d2(th)[:, 4], rtol=0, atol=1e-14)
+    for m in range(-2, 3):
+        norm = quad(lambda t: 2 * np.pi * np.sin(t) * (5 / (4 * np.pi)) * _wigner_d2(t)[m + 2, 4] ** 2, 0, np.pi)[0]
+        assert abs(norm - 1) < 1e-10
+
+
+def test_pv2_reduced_transform_equals_the_full_one():
+    """PhenomPv2_Param_Transform_reduced (IMRPhenomP.cpp:821-938) takes chi_p, phi_p; with in-plane spin on the
+    larger black hole only, chi_p = |chi_perp| and phi_p its azimuth give the full transform (:1081-1211)"""
+    f = np.arange(20, 1024.01, 0.25)
+    for s1 in ([.3, -.2, .2], [-.6, .1, -.4], [.0, .8, .1]):
+        kw = dict(mass1=36., mass2=29., spin1=s1, spin2=[0, 0, .1], DL=410., incl=.4, f_ref=20., phiRef=.3)
+        a = O.fourier_waveform(f, O.make_params(**kw), O.IMRPHENOMPV2)
+        chip, phip = np.hypot(s1[0], s1[1]), np.arctan2(s1[1], s1[0])
+        assert abs(O.pv2_setup_summary(O.make_params(**kw))["chip"] - chip) < 1e-15
+        b = O.fourier_waveform(f, O.make_params(**kw, chip=chip, phip=phip), O.IMRPHENOMPV2)
+        for x, y in zip(a, b):
+            assert np.max(np.abs(x - y)) < 1e-12 * np.max(np.abs(x))
+        # a wrong azimuth moves the waveform
+        c = O.fourier_waveform(f, O.make_params(**kw, chip=chip, phip=phip + 0.3), O.IMRPHENOMPV2)
+        assert np.max(np.abs(c[0] - a[0])) > 1e-3 * np.max(np.abs(a[0]))
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_pv2_zeta_rotation_is_a_polarisation_angle_shift(case):
+    """waveform_generator.cpp:257-266 rotates (h+, hx) by 2 zeta: the norm |h+|^2 + |hx|^2 is unchanged, and
+    the detector response of the rotated waveform at psi is that of the unrotated one at psi - zeta (the
+    antenna pattern's own polarisation-angle rotation); a reflection, a rotation by zeta or by -2 zeta fails"""
+    f = np.arange(20, 1024.01, 0.25)
+    kw = dict(_pv2_cases()[case], RA=1.0, DEC=0.3, psi=0.2, gmst=0.5)
+    p = O.make_params(**kw)
+    z = O.pv2_setup_summary(p)["zeta"]
+    assert abs(np.sin(2 * z)) > 0.05
+    hp, hc = O.fourier_waveform(f, p, O.IMRPHENOMPV2)
+    up, uc = O.pv2_waveform_unrotated(f, p)
+    n1, n0 = np.abs(hp) ** 2 + np.abs(hc) ** 2, np.abs(up) ** 2 + np.abs(uc) ** 2
+    nz = n0 > 0
+    assert np.max(np.abs(n1[nz] / n0[nz] - 1)) < 1e-12
+    for det in (0, 1, 2):
+        Gp, Gx = O.antenna(det, 1.0, 0.3, 0.2, 0.5)
+        Fp, Fx = O.antenna(det, 1.0, 0.3, 0.2 - z, 0.5)
+        a, b = Gp * hp + Gx * hc, Fp * up + Fx * uc
+        assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(a))
+        Fp2, Fx2 = O.antenna(det, 1.0, 0.3, 0.2 + z, 0.5)
+        assert np.max(np.abs(a - (Fp2 * up + Fx2 * uc))) > 1e-2 * np.max(np.abs(a))
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_pv2_time_correction_is_the_phase_slope_at_ringdown(case):
+    """calculate_time_shift (IMRPhenomP.cpp:550-606): t_corr = d(-Psi)/df / 2 pi at f_RD, from a 10-knot natural
+    spline on [0.8, 1.2] f_RD of the aligned-spin phase; it equals the analytic merger-ringdown slope and a
+    central difference of the phase to the spline's accuracy; a lost sign or 2 pi fails"""
+    p = O.make_params(**_pv2_cases()[case])
+    s = O.pv2_setup_summary(p)
+    fr = s["fRD"]
+    h = 1e-3 * fr
+    fd = -(O.pv2_aligned_phase(p, fr + h) - O.pv2_aligned_phase(p, fr - h)) / (2 * h) / (2 * np.pi)
+    an = -s["dphase_mr_at_fRD"] / (2 * np.pi)
+    assert abs(fd / an - 1) < 1e-5
+    assert abs(s["t_corr"] / an - 1) < 1e-3, (s["t_corr"], an)
+
+
+# ---- the order-4 Fisher stencil (fisher.cpp:403-453) -------------------------------------------------------
+def test_order4_stencil_is_the_five_point_rule_and_its_tc_shift_mix():
+    """Order 4: (-h(+2e) + 8 h(+e) - 8 h(-e) + h(-2e)) / 12 e.  With the detector equal to the reference one it
+    is the textbook five-point rule of the response.  Otherwise the +-e responses carry the time-of-flight
+    factor e^{-2 pi i dt f} while the +-2e ones shift t_c by -dt instead, which (h depends on t_c through
+    e^{2 pi i t_c (f - f_ref)}) is the same factor times e^{2 pi i dt f_ref}: the derivative differs from the
+    consistent five-point rule by (e^{i k} - 1)(h(-2e) - h(+2e)) / 12 e with k = 2 pi dt f_ref per point."""
+    f = np.arange(20., 512.01, 0.5)
+    kw = dict(mass1=36., mass2=29., spin1=[0, 0, .3], spin2=[0, 0, -.1], DL=410., incl=.4, f_ref=20., phiRef=.3,
+              RA=1.37, DEC=-1.26, psi=2.66, gmst=0.7, tc=0.01, shift_time=0)
+    p = O.make_params(**kw)
+    dim, eps, method = 11, 1e-5, "IMRPhenomD"
+    v, lf = O.unpack_parameters(p, method, dim)
+
+    def resp(vv, det, ref):
+        q = O.repack_parameters(vv, method, dim, p)
+        dt = O.dtoa(q.RA, q.DEC, q.gmst, ref, det)
+        r = O.detector_response(f, q, det)
+        q.tc -= dt
+        return r * np.exp(-2j * np.pi * dt * f), O.detector_response(f, q, det), dt
+
+    # the t_c shift equals the time-of-flight factor times e^{2 pi i dt f_ref}
+    a, b, dt = resp(v, 1, 0)
+    assert abs(dt) > 1e-3
+    assert np.max(np.abs(b - a * np.exp(2j * np.pi * dt * 20.))) < 1e-12 * np.max(np.abs(a))
+    for det, ref in ((0, 0), (1, 0), (2, 1)):
+        D = O.calculate_derivatives(f, method, det, ref, dim, p, order=4, epsilon=eps)
+        for i in range(dim):
+            e = np.zeros(dim)
+            e[i] = eps
+            (c_p, _, _), (c_m, _, _) = resp(v + e, det, ref), resp(v - e, det, ref)
+            (c_pp, s_pp, _), (c_mm, s_mm, _) = resp(v + 2 * e, det, ref), resp(v - 2 * e, det, ref)
+            consistent = (-c_pp + 8. * c_p - 8. * c_m + c_mm) / (12. * eps)
+            mix = ((c_mm - s_mm) - (c_pp - s_pp)) / (12. * eps) if det != ref else 0
+            scale = v[i] if lf[i] else 1.0
+            want = (consistent - mix) * scale
+            # rounding of the four responses (a few ulp each, time-of-flight factors from numpy's exp here and
+            # libm's sin/cos in the oracle) divided by 12 e, plus a relative floor
+            noise = 18 * 4.4e-16 * np.max(np.abs(c_p)) / (12. * eps) * abs(scale)
+            assert np.max(np.abs(D[i] - want)) < noise + 1e-12 * np.max(np.abs(want)), (det, i)
+            if det != ref and i == 5:  # the mix is not negligible (t_c derivative)
+                assert np.max(np.abs(mix)) > 1e-3 * np.max(np.abs(consistent))
