@@ -102,6 +102,10 @@ def lib():
         L.gwo_network_loglike.restype = D
         L.gwo_network_loglike_batch.argtypes = [P(Params), ctypes.c_int, ctypes.c_int, dp, ctypes.c_int,
                                                 P(dp), P(dp), P(ctypes.c_int), ctypes.c_int, ctypes.c_int, dp]
+        L.gwo_network_loglike_ex.argtypes = [P(Params), ctypes.c_int, dp, ctypes.c_int, P(dp), P(dp),
+                                             P(ctypes.c_int), ctypes.c_int, dp, ctypes.c_int, ctypes.c_int]
+        L.gwo_network_loglike_ex.restype = D
+        L.gwo_fisher_elements.argtypes = [dp, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int, dp, ctypes.c_int, dp]
         L.gwo_snr.argtypes = [dp, dp, dp, ctypes.c_int]
         L.gwo_snr.restype = D
         L.gwo_fisher_numerical.argtypes = [dp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, dp,
@@ -194,12 +198,15 @@ def simpsons_sum(dx, integrand):
     return lib().gwo_simpsons_sum(dx, len(x), _dp(x))
 
 
-def loglike_internal(data, psd, f, response):
+def loglike_internal(data, psd, f, response, weights=None, log10F=False, gaussleg=False):
+    """Log_Likelihood_internal (mcmc_gw.cpp:801-868): Simpson, or Gauss-Legendre weights (gaussleg)"""
     d = _cplx(data)
     r = _cplx(response)
     s = _c(psd)
     f = _c(f)
-    return lib().gwo_loglike_internal(_dp(d), _dp(s), _dp(f), None, _dp(r), len(f), 0, 0)
+    w = _c(weights) if weights is not None else None
+    return lib().gwo_loglike_internal(_dp(d), _dp(s), _dp(f), _dp(w) if w is not None else None, _dp(r), len(f),
+                                      int(log10F), int(gaussleg))
 
 
 def log_likelihood(data, psd, f, p, det, model=IMRPHENOMD):
@@ -213,13 +220,32 @@ def _ptr_array(arrs):
     return (ctypes.POINTER(ctypes.c_double) * len(arrs))(*[_dp(a) for a in arrs])
 
 
-def network_loglike(f, data, psd, dets, p, model=IMRPHENOMD):
+def network_loglike(f, data, psd, dets, p, model=IMRPHENOMD, weights=None, log10F=False, gaussleg=False):
     f = _c(f)
     dd = [_cplx(x) for x in data]
     ss = [_c(x) for x in psd]
     di = (ctypes.c_int * len(dets))(*dets)
-    return lib().gwo_network_loglike(ctypes.byref(p), model, _dp(f), len(f), _ptr_array(dd), _ptr_array(ss),
-                                     di, len(dets))
+    if weights is None and not log10F and not gaussleg:
+        return lib().gwo_network_loglike(ctypes.byref(p), model, _dp(f), len(f), _ptr_array(dd), _ptr_array(ss),
+                                         di, len(dets))
+    w = _c(weights) if weights is not None else None
+    return lib().gwo_network_loglike_ex(ctypes.byref(p), model, _dp(f), len(f), _ptr_array(dd), _ptr_array(ss), di,
+                                        len(dets), _dp(w) if w is not None else None, int(log10F), int(gaussleg))
+
+
+def fisher_elements(f, deriv, psd, weights=None, log10F=False, gaussleg=False):
+    """calculate_fisher_elements (fisher.cpp:2704-2781) of derivatives deriv[dim, n] (complex)"""
+    f = _c(f)
+    dim, n = deriv.shape
+    d = np.empty(2 * dim * n)
+    d[0::2] = deriv.real.ravel()
+    d[1::2] = deriv.imag.ravel()
+    s = _c(psd)
+    w = _c(weights) if weights is not None else None
+    out = np.zeros(dim * dim)
+    lib().gwo_fisher_elements(_dp(f), n, dim, _dp(d), _dp(s), int(gaussleg), _dp(w) if w is not None else None,
+                              int(log10F), _dp(out))
+    return out.reshape(dim, dim)
 
 
 def mcmc_likelihood_extrinsic(f, data, psd, dets, p, model=IMRPHENOMD, T_eff=0.0):

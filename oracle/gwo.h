@@ -107,6 +107,11 @@ double gwo_mcmc_likelihood_extrinsic(const gwo_params *p, int model, const doubl
 /* Same network likelihood with the waveform tc used as given (no T_eff mapping). */
 double gwo_network_loglike(const gwo_params *p, int model, const double *f, int n, const double *const *data,
                            const double *const *psd, const int *dets, int ndet);
+double gwo_network_loglike_ex(const gwo_params *p, int model, const double *f, int n, const double *const *data,
+                              const double *const *psd, const int *dets, int ndet, const double *weights, int log10f,
+                              int gaussleg);
+void gwo_fisher_elements(const double *f, int n, int dim, const double *deriv, const double *psd, int gaussleg,
+                         const double *weights, int log10f, double *out);
 /* Reference CPU path over a batch: one instance per call, OpenMP over instances
  * (mcmc_sampler.cpp:4583-4600 parallelises chains the same way). */
 int gwo_network_loglike_batch(const gwo_params *ps, int n_inst, int model, const double *f, int n,

@@ -152,6 +152,15 @@ double gwo_log_likelihood(const double *data, const double *psd, const double *f
 double gwo_network_loglike(const gwo_params *pp, int model, const double *f, int n, const double *const *data,
                            const double *const *psd, const int *dets, int ndet)
 {
+    return gwo_network_loglike_ex(pp, model, f, n, data, psd, dets, ndet, NULL, 0, 0);
+}
+
+/* the same with the integration rule of Log_Likelihood_internal (mcmc_gw.cpp:801-868): Simpson, or the caller's
+ * Gauss-Legendre weights (times f ln 10 on a log10 grid), as MCMC_likelihood_extrinsic passes them through */
+double gwo_network_loglike_ex(const gwo_params *pp, int model, const double *f, int n, const double *const *data,
+                              const double *const *psd, const int *dets, int ndet, const double *weights, int log10f,
+                              int gaussleg)
+{
     double *hp = (double *)malloc(sizeof(double) * 2 * n);
     double *hc = (double *)malloc(sizeof(double) * 2 * n);
     double *r = (double *)malloc(sizeof(double) * 2 * n);
@@ -170,7 +179,7 @@ double gwo_network_loglike(const gwo_params *pp, int model, const double *f, int
             double ph = tcd * f[j];
             gwo_cmul(rr, ri, cos(ph), sin(ph), &r[2 * j], &r[2 * j + 1]);
         }
-        ll += gwo_loglike_internal(data[d], psd[d], f, NULL, r, n, 0, 0);
+        ll += gwo_loglike_internal(data[d], psd[d], f, weights, r, n, log10f, gaussleg);
     }
     free(hp);
     free(hc);
@@ -629,6 +638,38 @@ static void fisher_elements(const double *f, int n, int dim, const double *deriv
             for (int i = 0; i < n; i++) integrand[i] = (a[2 * i] * b[2 * i] + a[2 * i + 1] * b[2 * i + 1]) / psd[i];
             out[j * dim + k] = 4 * gwo_simpsons_sum(df, n, integrand);
             out[k * dim + j] = out[j * dim + k];
+        }
+    }
+    free(integrand);
+}
+
+/* calculate_fisher_elements (fisher.cpp:2704-2781) with either rule: Simpson at f[1]-f[0], or the caller's
+ * Gauss-Legendre weights (times f ln 10 on a log10 grid); deriv as calculate_derivatives writes it */
+void gwo_fisher_elements(const double *f, int n, int dim, const double *deriv, const double *psd, int gaussleg,
+                         const double *weights, int log10f, double *out)
+{
+    if (!gaussleg && !log10f) {
+        fisher_elements(f, n, dim, deriv, out, psd);
+        return;
+    }
+    double *integrand = (double *)malloc(sizeof(double) * n);
+    for (int j = 0; j < dim; j++) {
+        for (int k = 0; k <= j; k++) {
+            const double *a = deriv + (size_t)2 * n * j, *b = deriv + (size_t)2 * n * k;
+            for (int i = 0; i < n; i++) {
+                integrand[i] = (a[2 * i] * b[2 * i] + a[2 * i + 1] * b[2 * i + 1]) / psd[i];
+                if (log10f) integrand[i] *= f[i] * GWO_LOG10;
+            }
+            double v;
+            if (gaussleg) {
+                double sum = 0;
+                for (int i = 0; i < n; i++) sum += weights[i] * integrand[i];
+                v = 4 * sum;
+            } else {
+                v = 4 * gwo_simpsons_sum(f[1] - f[0], n, integrand);
+            }
+            out[j * dim + k] = v;
+            out[k * dim + j] = v;
         }
     }
     free(integrand);

@@ -983,3 +983,50 @@ def test_order4_stencil_is_the_five_point_rule_and_its_tc_shift_mix():
             assert np.max(np.abs(D[i] - want)) < noise + 1e-12 * np.max(np.abs(want)), (det, i)
             if det != ref and i == 5:  # the mix is not negligible (t_c derivative)
                 assert np.max(np.abs(mix)) > 1e-3 * np.max(np.abs(consistent))
+
+
+# ---- Gauss-Legendre and log10-f integration rules (mcmc_gw.cpp:821-856, fisher.cpp:2736-2745) ------------------
+def _gl(x1, x2, n):
+    """Gauss-Legendre nodes and weights on [x1, x2] (numpy's leggauss: the same rule gauleg builds,
+    ortho_basis.cpp:14)"""
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (x2 - x1) * x + 0.5 * (x2 + x1), 0.5 * (x2 - x1) * w
+
+
+def test_gauss_legendre_inner_products_against_closed_forms():
+    """the GAUSSLEG branch of Log_Likelihood_internal: (a|b) = 4 sum w_i Re(a_i b_i*) / S_i, times f_i ln 10 on a
+    log10 grid (df = f ln 10 d log10 f).  With power laws h = f^a e^{i phi}, d = f^b e^{i phi}, S = f^c the inner
+    products are 4 int f^(2a - c) df and 4 int f^(a + b - c) df in closed form: exact for polynomial integrands on
+    a linear grid, to rounding on a log10 grid (smooth in log10 f); a dropped Jacobian or weight fails."""
+    f1, f2 = 20., 1024.
+    prim = lambda k: 4 * (f2 ** (k + 1) - f1 ** (k + 1)) / (k + 1)
+    # linear grid, polynomial integrands (a = 1.5, b = 0.5, c = 0 -> f^3 and f^2)
+    f, w = _gl(f1, f2, 32)
+    ph = np.exp(1j * 0.37 * f)
+    h, d, S = f ** 1.5 * ph, f ** 0.5 * ph, np.ones_like(f)
+    ll = O.loglike_internal(d, S, f, h, weights=w, gaussleg=True)
+    want = -0.5 * (prim(3.) - 2 * prim(2.))
+    assert abs(ll / want - 1) < 1e-13
+    # log10 grid, the amplitude and noise powers of a real signal (a = -7/6, c = -4.14)
+    u, wu = _gl(np.log10(f1), np.log10(f2), 64)
+    f = 10 ** u
+    ph = np.exp(1j * 3.1 * f)
+    h, d, S = f ** (-7 / 6) * ph, 0.7 * f ** (-1.0) * ph, f ** -4.14
+    ll = O.loglike_internal(d, S, f, h, weights=wu, log10F=True, gaussleg=True)
+    want = -0.5 * (prim(-7 / 3 + 4.14) - 2 * 0.7 * prim(-7 / 6 - 1.0 + 4.14))
+    assert abs(ll / want - 1) < 1e-12
+    assert abs(O.loglike_internal(d, S, f, h, weights=wu, log10F=False, gaussleg=True) / want - 1) > 0.1
+    # the Fisher elements of the same rule: F_ij = 4 int Re(h_i h_j*) / S df
+    D = np.vstack([f ** (-7 / 6) * ph, 1j * f ** (-1 / 2) * ph, f ** (-5 / 6) * np.exp(1j * 0.2) * ph])
+    F = O.fisher_elements(f, D, S, weights=wu, log10F=True, gaussleg=True)
+    pw = [-7 / 6, -1 / 2, -5 / 6]
+    ph0 = [1, 1j, np.exp(1j * 0.2)]
+    for i in range(3):
+        for j in range(3):
+            want = (ph0[i] * np.conj(ph0[j])).real * prim(pw[i] + pw[j] + 4.14)
+            assert abs(F[i, j] - want) < 1e-12 * np.sqrt(abs(F[i, i] * F[j, j]))
+    # Simpson on an equally spaced grid reproduces the engine-independent Fisher elements
+    fs = np.linspace(f1, f2, 2001)
+    Ds = np.vstack([fs ** (-7 / 6), fs ** (-1 / 2)]).astype(complex)
+    Fs = O.fisher_elements(fs, Ds, fs ** -4.14)
+    assert abs(Fs[0, 1] / prim(-7 / 6 - 1 / 2 + 4.14) - 1) < 1e-6
