@@ -47,7 +47,7 @@ EXPORTS = [
     "gwh_fourier_waveform", "gwh_fourier_detector_response", "gwh_log_likelihood",
     "gwh_log_likelihood_internal", "gwh_mcmc_likelihood_extrinsic", "gwh_fisher_numerical",
     "gwh_event_create", "gwh_event_destroy", "gwh_event_bytes", "gwh_loglike_batch",
-    "gwh_loglike_batch_device", "gwh_set_bins_per_lane", "gwh_fisher_batch",
+    "gwh_loglike_batch_device", "gwh_loglike_batch_device_noalloc", "gwh_set_bins_per_lane", "gwh_fisher_batch",
     "gwh_time_loglike_batch_device", "gwh_device_alloc", "gwh_device_free", "gwh_memcpy_h2d",
     "gwh_memcpy_d2h", "gwh_work_counts", "gwh_create_coherent_GW_detection",
     "gwh_kernel_timing_start", "gwh_kernel_timing_every", "gwh_kernel_timing_stop", "gwh_device_copy",
@@ -159,6 +159,7 @@ def lib():
         "gwh_event_destroy": ([I], I), "gwh_event_bytes": ([I], ctypes.c_size_t),
         "gwh_loglike_batch": ([ip, I, I, I, V, I, dp], I),
         "gwh_loglike_batch_device": ([V, I, I, I, V, I, V, V], I),
+        "gwh_loglike_batch_device_noalloc": ([V, I, I, I, V, I, V, V], I),
         "gwh_set_bins_per_lane": ([I], I),
         "gwh_fisher_batch": ([I, I, I, I, V, I, I, D, dp, dp], I),
         "gwh_time_loglike_batch_device": ([I, I, I, V, I, V, I, dp, dp], I),

@@ -108,10 +108,12 @@ struct Event {
     int method = 0, log10F = 0; /* integration rule of the stored weights */
 };
 
-/* grow-only device buffer */
+/* grow-only device buffer; a fixed one is a view of memory the buffer does not own (never grown or freed:
+ * a request beyond its capacity is an error) */
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool fixed = false;
     int ensure(size_t bytes);
     void release();
     template <class T> T *as() { return (T *)p; }
@@ -169,6 +171,9 @@ struct Engine {
     bool kt_on = false;
     int kt_every = 1;        /* time every kt_every-th launch */
     long long kt_seq = 0;    /* launches since timing started */
+    /* device address of the allocation-free entry point's scratch array on this engine's device
+     * (hipGetSymbolAddress at initialisation; the array's name is not a device address in host code) */
+    unsigned char *noalloc_scratch = nullptr;
 };
 
 Engine &gwh_engine();

@@ -63,6 +63,9 @@ Engine &gwh_engine()
 int DevBuf::ensure(size_t bytes)
 {
     if (bytes <= cap) return GWH_OK;
+    if (fixed)
+        return gwh_set_error(GWH_ERR_NOMEM, "launch needs " + std::to_string(bytes) + " bytes of a fixed scratch region of " +
+                                                std::to_string(cap));
     if (p) {
         hipError_t e = hipFree(p);
         (void)e;
@@ -80,7 +83,7 @@ int DevBuf::ensure(size_t bytes)
 
 void DevBuf::release()
 {
-    if (p) (void)hipFree(p);
+    if (p && !fixed) (void)hipFree(p);
     p = nullptr;
     cap = 0;
 }
@@ -131,6 +134,12 @@ void gwh_persist_warmup();
 static const double k_geometric_factor[GWH_NDET_TABLE] = {1., 1., 1., 1., 1., 1., 0.86602540378, 0.86602540378,
                                                           0.86602540378};
 
+/* Scratch of the allocation-free likelihood entry point (gwh_loglike_batch_device_noalloc): instance
+ * records, partial sums and tile counts of one launch, GWH_NOALLOC_SCRATCH_BYTES on every device the library
+ * runs on, held until the process ends.  Host code uses the address hipGetSymbolAddress returns for it
+ * (Engine::noalloc_scratch), never the name. */
+static __device__ __align__(256) unsigned char scratch[GWH_NOALLOC_SCRATCH_BYTES];
+
 int gwh_ensure_init()
 {
     Engine &E = gwh_engine();
@@ -175,6 +184,7 @@ int gwh_ensure_init()
     GWH_HIP(hipMalloc(&E.d_events, sizeof(EventDesc) * GWH_MAX_EVENTS));
     GWH_HIP(hipMemset(E.d_events, 0, sizeof(EventDesc) * GWH_MAX_EVENTS));
     E.events.assign(GWH_MAX_EVENTS, Event());
+    GWH_HIP(hipGetSymbolAddress((void **)&E.noalloc_scratch, HIP_SYMBOL(scratch)));
     gwh_persist_warmup();
     E.ready = true;
     return GWH_OK;
@@ -1534,6 +1544,35 @@ int gwh_loglike_batch_device(const int *d_events, int events0, int model, int pr
     hipStream_t s = stream ? (hipStream_t)stream : E.stream;
     return gwh_loglike_launch(d_events, events0, model, precision, (const gwh_params *)d_params, n, d_logl, s, false,
                               nullptr);
+}
+
+int gwh_loglike_batch_device_noalloc(const int *d_events, int events0, int model, int precision, const void *d_params,
+                                     int n, double *d_logl, void *stream)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    /* initialisation allocates the model tables: it must have happened before (gwh_init) */
+    if (!E.ready || !E.noalloc_scratch)
+        return gwh_set_error(GWH_ERR_INVALID, "gwh_loglike_batch_device_noalloc: call gwh_init first");
+    if (n <= 0) return GWH_OK;
+    /* the fixed array split: instance records, tile counts, then partial sums in the rest */
+    const size_t a_coef = ((size_t)n * sizeof(InstCoef) + 255) & ~(size_t)255;
+    const size_t a_act = ((size_t)n * sizeof(int) + 255) & ~(size_t)255;
+    if (a_coef + a_act >= (size_t)GWH_NOALLOC_SCRATCH_BYTES)
+        return gwh_set_error(GWH_ERR_NOMEM, std::to_string(n) + " instances exceed the allocation-free scratch of " +
+                                                std::to_string(GWH_NOALLOC_SCRATCH_BYTES) + " bytes");
+    Scratch S;
+    unsigned char *base = E.noalloc_scratch;
+    S.coef.p = base;
+    S.coef.cap = a_coef;
+    S.active.p = base + a_coef;
+    S.active.cap = a_act;
+    S.partial.p = base + a_coef + a_act;
+    S.partial.cap = (size_t)GWH_NOALLOC_SCRATCH_BYTES - a_coef - a_act;
+    for (DevBuf *b : {&S.coef, &S.active, &S.partial, &S.work, &S.wcount}) b->fixed = true;
+    hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+    return gwh_loglike_launch(d_events, events0, model, precision, (const gwh_params *)d_params, n, d_logl, s, false,
+                              nullptr, nullptr, 0, &S, nullptr);
 }
 
 int gwh_loglike_batch(const int *events, int events0, int model, int precision, const gwh_params *params, int n,

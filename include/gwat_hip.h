@@ -274,6 +274,18 @@ int gwh_snr_batch(const int *events, int events0, int model, int precision, cons
 int gwh_loglike_batch_device(const int *d_events, int events0, int model, int precision, const void *d_params,
                              int n, double *d_logl, void *stream);
 
+/* No counterpart in the reference.  gwh_loglike_batch_device for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same kernels and
+ * results, with the instance records, tile counts and partial sums in a fixed device array of
+ * GWH_NOALLOC_SCRATCH_BYTES that the library holds on the device until the process ends.  Calls no hipMalloc,
+ * hipFree or allocating library routine.  gwh_init must have been called first (initialisation allocates the
+ * model tables); a launch whose scratch exceeds the array fails with GWH_ERR_NOMEM before anything is enqueued
+ * (4096 instances of a 16225-bin event need ~7 MB; a 2.6e5-bin event ~13 kB per instance).  Every call on the
+ * device shares the one array: calls that overlap on the device (other streams, other threads' streams) race. */
+#define GWH_NOALLOC_SCRATCH_BYTES (32u << 20)
+int gwh_loglike_batch_device_noalloc(const int *d_events, int events0, int model, int precision,
+                                     const void *d_params, int n, double *d_logl, void *stream);
+
 /* Multi-event batch on device buffers with the instance -> event map also given on the host
  * (events: n ids, host; d_events: the same ids in device memory).  The host copy sizes one flattened
  * (instance, tile) work list per bins-per-lane class, so events of very different lengths share one
