@@ -122,7 +122,8 @@ double gwo_snr(const double *psd, const double *response, const double *f, int n
 /* --- Fisher (fisher.cpp:81-557, :1754-2781) --- */
 /* method: generation-method string as in the reference ("IMRPhenomD",
  * "MCMC_IMRPhenomD", "IMRPhenomPv2", "MCMC_IMRPhenomPv2"); sky_average in p
- * selects the sky-averaged basis.  out is dim*dim row major. */
+ * selects the sky-averaged basis.  out is dim*dim row major.  epsilon > 0: absolute step
+ * (fisher.cpp:158); epsilon < 0: step |epsilon|*|v_i| per basis coordinate (|epsilon| at v_i = 0). */
 int gwo_fisher_numerical(const double *f, int n, const char *method, int det, int ref_det,
                          double *out, int dim, const gwo_params *p, int order,
                          const double *psd, double epsilon);

@@ -511,9 +511,19 @@ static void add_tof_phase(double *r, const double *f, int n, double dt_phase)
     }
 }
 
+/* Finite-difference step of one Fisher-basis coordinate: step > 0 is the reference's absolute
+ * epsilon (fisher.cpp:158); step < 0 asks for a step relative to the coordinate, |step| * |v|, or
+ * |step| where v is exactly zero (tc and phiref at 0). The relative form is the benchmark
+ * configuration's "1e-7 relative" step, not a reference behaviour. */
+static double param_step(double step, double v)
+{
+    if (step > 0) return step;
+    return v != 0 ? -step * fabs(v) : -step;
+}
+
 /* fisher.cpp:148-557 */
 int gwo_calculate_derivatives(double *deriv, const double *f, int n, int dim, int det, int ref_det,
-                              const char *gen_method, const gwo_params *p, int order, double epsilon)
+                              const char *gen_method, const gwo_params *p, int order, double step)
 {
     char lm[128];
     local_method(gen_method, lm);
@@ -533,6 +543,7 @@ int gwo_calculate_derivatives(double *deriv, const double *f, int n, int dim, in
         double *tmp = (double *)malloc(sizeof(double) * n);
         gwo_fourier_amplitude(f, n, amp, model, p);
         for (int i = 0; i < dim; i++) {
+            double epsilon = param_step(step, vec[i]);
             memcpy(pp, vec, sizeof(double) * dim);
             memcpy(pm, vec, sizeof(double) * dim);
             pp[i] = vec[i] + epsilon;
@@ -581,6 +592,7 @@ int gwo_calculate_derivatives(double *deriv, const double *f, int n, int dim, in
         double *rp = (double *)malloc(sizeof(double) * 2 * n), *rm = (double *)malloc(sizeof(double) * 2 * n);
         double *rpp = (double *)malloc(sizeof(double) * 2 * n), *rmm = (double *)malloc(sizeof(double) * 2 * n);
         for (int i = 0; i < dim; i++) {
+            double epsilon = param_step(step, vec[i]);
             memcpy(pp, vec, sizeof(double) * dim);
             memcpy(pm, vec, sizeof(double) * dim);
             pp[i] = vec[i] + epsilon;

@@ -1,13 +1,16 @@
 /*
  * Frozen FLOP-equivalent (fe) table of the hot path, counted on the oracle (TEST AND MEASUREMENT
- * INFRASTRUCTURE): the reference algorithm as oracle/ restates it runs on the counting scalar of counted.h,
- * and the per-bin work of each frequency region is the difference between calls on n and n - 2 bins of that
- * region (per-call setup cancels).  Weights: add/sub/mul/cmp 1, div/sqrt 4, exp/log 16, pow 32, sin + cos
- * pair 24, atan 20.  Prints one JSON object.
+ * INFRASTRUCTURE).  The reference algorithm as oracle/ restates it runs on the counting scalar of counted.h;
+ * the per-bin work of a frequency region is the difference between waveform calls on n and n - 2 bins of that
+ * region (per-call setup cancels), split into the amplitude part (gwo_build_amp: region of f against f1 =
+ * 0.014/M and the amplitude peak f3), the phase part (gwo_build_phase and the powers of f the reference
+ * precomputes for it: region of f against 0.018/M and f_RD / 2) and the rest (alignment, complex exponential,
+ * polarisations), which the harness checks is the same in every region.  Weights: add/sub/mul/cmp 1,
+ * div/sqrt 4, exp/log 16, pow 32, sin + cos pair 24, atan 20.  Prints one JSON object.
  */
 #include "counted.h"
 #define double cd
-#include "../gwo.h"
+#include "../gwo_internal.h"
 #undef double
 #include <cstdio>
 #include <vector>
@@ -43,8 +46,6 @@ static gwo_params injection(int pv2)
     return p;
 }
 
-static OpTally g_last; /* operations of the last measured call */
-
 /* fe of a waveform call on n bins at frequencies f0, f0 + df, ... */
 static double waveform_fe(const gwo_params &p, int model, double f0, double df, int n)
 {
@@ -52,17 +53,32 @@ static double waveform_fe(const gwo_params &p, int model, double f0, double df, 
     for (int i = 0; i < n; i++) f[i] = cd(f0 + i * df);
     OpTally before = g_ops;
     gwo_fourier_waveform(f.data(), n, hp.data(), hc.data(), model, &p);
-    g_last = diff(g_ops, before);
-    return fe_of(g_last);
+    return fe_of(diff(g_ops, before));
 }
 
-static void print_ops(const OpTally &a, const OpTally &b, double per)
+static double per_bin(const gwo_params &p, int model, double lo, double hi)
 {
-    OpTally d = diff(a, b);
-    printf("{\"add\": %g, \"mul\": %g, \"div\": %g, \"sqrt\": %g, \"exp\": %g, \"log\": %g, \"pow\": %g, "
-           "\"sin_or_cos\": %g, \"atan\": %g, \"cmp\": %g}",
-           d.add / per, d.mul / per, d.div / per, d.sqrt / per, d.exp / per, d.log / per, d.pow / per, d.sincos / per,
-           d.atan / per, (d.cmp + d.misc) / per);
+    const int n = 9;
+    double df = (hi - lo) / (n + 2), f0 = lo + 0.5 * df;
+    return (waveform_fe(p, model, f0, df, n) - waveform_fe(p, model, f0, df, n - 2)) / 2;
+}
+
+/* fe of gwo_build_amp and gwo_build_phase at one frequency (the source prepared as the waveform call does) */
+static void parts_fe(const gwo_params &p, double f, double *amp, double *phase)
+{
+    gwo_src s;
+    gwo_populate_source(&s, &p, GWO_IMRPHENOMD);
+    gwo_phenomd_state st;
+    gwo_phenomd_prepare(&s, &st);
+    gwo_pows pw;
+    gwo_precalc_powers_pi(&pw);
+    gwo_precalc_powers_ins(cd(f), s.M, &pw);
+    OpTally b = g_ops;
+    gwo_build_amp(cd(f), &s, &st, &pw);
+    *amp = fe_of(diff(g_ops, b));
+    b = g_ops;
+    gwo_build_phase(cd(f), &s, &st, &pw);
+    *phase = fe_of(diff(g_ops, b));
 }
 
 static double network_fe(const gwo_params &p, int model, double f0, double df, int n, int ndet)
@@ -85,48 +101,53 @@ int main()
     const double M = (pd.mass1.v + pd.mass2.v) * MSOL;
     cd summ[32];
     gwo_phenomd_setup_summary(&pd, GWO_IMRPHENOMD, summ);
-    /* fRD, fdamp, f1, f3, f1_phase, f2_phase */
     const double f1 = summ[2].v, f3 = summ[3].v, f1p = summ[4].v, f2p = summ[5].v, fcut = 0.2 / M;
-    struct { const char *name; double lo, hi; } reg[5] = {
-        {"amp_ins_phase_ins", 20., f1}, {"amp_int_phase_ins", f1, f1p}, {"amp_int_phase_int", f1p, f2p},
-        {"amp_int_phase_mr", f2p, f3}, {"amp_mr_phase_mr", f3, fcut}};
-    const int n = 9;
-    printf("{\n \"workload\": \"m1 = 36, m2 = 29 Msun, chi1z = 0.3, chi2z = -0.1 (in-plane 0.3, 0.1 / -0.1, 0.2 for "
-           "IMRPhenomPv2); regions f1 = %.3f, f1_phase = %.3f, f2_phase = %.3f, f3 = %.3f, 0.2/M = %.3f Hz\",\n",
-           f1, f1p, f2p, f3, fcut);
-    printf(" \"weights\": \"add/sub/mul/cmp 1, div/sqrt 4, exp/log 16, pow 32, sin+cos pair 24, atan 20\",\n");
-    printf(" \"imrphenomd_waveform_per_bin\": {\n  ");
-    double pv2_add = 0;
+    /* the (amplitude region, phase region) pairs this source reaches, in ascending frequency */
+    struct { int a, p; double lo, hi; } reg[5] = {
+        {0, 0, 20., f1}, {1, 0, f1, f1p}, {1, 1, f1p, f2p}, {1, 2, f2p, f3}, {2, 2, f3, fcut}};
+    double amp[3] = {0, 0, 0}, phase[3] = {0, 0, 0}, rest[5], total[5];
     for (int r = 0; r < 5; r++) {
-        double df = (reg[r].hi - reg[r].lo) / (n + 2);
-        double f0 = reg[r].lo + 0.5 * df;
-        double a = waveform_fe(pd, GWO_IMRPHENOMD, f0, df, n);
-        OpTally ta = g_last;
-        a -= waveform_fe(pd, GWO_IMRPHENOMD, f0, df, n - 2);
-        OpTally tb = g_last;
-        double b = waveform_fe(pp, GWO_IMRPHENOMPV2, f0, df, n) - waveform_fe(pp, GWO_IMRPHENOMPV2, f0, df, n - 2);
-        printf("%s\"%s\": {\"fe\": %.1f, \"ops\": ", r ? ", " : "", reg[r].name, a / 2);
-        print_ops(ta, tb, 2.0);
-        printf("}");
-        if (r == 2) pv2_add = (b - a) / 2;
+        total[r] = per_bin(pd, GWO_IMRPHENOMD, reg[r].lo, reg[r].hi);
+        double a, ph;
+        parts_fe(pd, 0.5 * (reg[r].lo + reg[r].hi), &a, &ph);
+        amp[reg[r].a] = a;
+        phase[reg[r].p] = ph;
     }
-    printf("},\n");
-    {
-        double df = (fcut * 1.5 - fcut * 1.1) / n, f0 = fcut * 1.1;
-        double z = waveform_fe(pd, GWO_IMRPHENOMD, f0, df, n) - waveform_fe(pd, GWO_IMRPHENOMD, f0, df, n - 2);
-        printf(" \"imrphenomd_waveform_per_zero_bin\": %.1f,\n", z / 2);
-    }
-    printf(" \"imrphenompv2_extra_per_bin\": %.1f,\n", pv2_add);
-    /* per detector per non-zero bin: the network log-likelihood on 3 vs 2 detectors, n vs n - 2 bins */
+    /* the rest: powers of f (which depend on the phase region) and the common per-bin work */
+    double powers[3] = {0, 0, 0}, common = 0;
+    for (int r = 0; r < 5; r++) rest[r] = total[r] - amp[reg[r].a] - phase[reg[r].p];
+    /* regions 1 and 2 share the amplitude region and differ in the phase region; the inspiral phase region
+     * alone evaluates the full set of powers */
+    common = rest[2];
+    powers[1] = powers[2] = 0;
+    powers[0] = rest[0] - common;
+    int additive = 1;
+    for (int r = 0; r < 5; r++)
+        if (rest[r] != common + powers[reg[r].p]) additive = 0;
+    double pv2_add = per_bin(pp, GWO_IMRPHENOMPV2, f1p, f2p) - per_bin(pp, GWO_IMRPHENOMD, f1p, f2p);
+    const int n = 9;
     double df = (f2p - f1p) / (n + 2), f0 = f1p + 0.5 * df;
-    for (int model : {GWO_IMRPHENOMD, GWO_IMRPHENOMPV2}) {
-        const gwo_params &p = model == GWO_IMRPHENOMD ? pd : pp;
+    double det[2];
+    for (int m = 0; m < 2; m++) {
+        int model = m ? GWO_IMRPHENOMPV2 : GWO_IMRPHENOMD;
+        const gwo_params &p = m ? pp : pd;
         double d3 = network_fe(p, model, f0, df, n, 3) - network_fe(p, model, f0, df, n - 2, 3);
         double d2 = network_fe(p, model, f0, df, n, 2) - network_fe(p, model, f0, df, n - 2, 2);
-        printf(" \"%s_per_detector_per_bin\": %.1f,\n", model == GWO_IMRPHENOMD ? "imrphenomd" : "imrphenompv2",
-               (d3 - d2) / 2);
+        det[m] = (d3 - d2) / 2;
     }
-    printf(" \"note\": \"per-bin differences of calls on %d and %d bins of one region (setup cancels)\"\n}\n", n,
-           n - 2);
-    return 0;
+    printf("{\n \"source\": \"m1 = 36, m2 = 29 Msun, chi1z = 0.3, chi2z = -0.1 (in-plane 0.3, 0.1 / -0.1, 0.2 for "
+           "IMRPhenomPv2); f1 = %.3f, f1_phase = %.3f, f2_phase = %.3f, f3 = %.3f, 0.2/M = %.3f Hz\",\n",
+           f1, f1p, f2p, f3, fcut);
+    printf(" \"weights\": \"add/sub/mul/cmp 1, div/sqrt 4, exp/log 16, pow 32, sin+cos pair 24, atan 20\",\n");
+    printf(" \"amp_region_fe\": [%.1f, %.1f, %.1f],\n", amp[0], amp[1], amp[2]);
+    printf(" \"phase_region_fe\": [%.1f, %.1f, %.1f],\n", phase[0] + powers[0], phase[1] + powers[1],
+           phase[2] + powers[2]);
+    printf(" \"common_per_bin_fe\": %.1f,\n", common);
+    printf(" \"additive\": %s,\n", additive ? "true" : "false");
+    printf(" \"measured_region_totals\": [%.1f, %.1f, %.1f, %.1f, %.1f],\n", total[0], total[1], total[2], total[3],
+           total[4]);
+    printf(" \"imrphenompv2_extra_per_bin_fe\": %.1f,\n", pv2_add);
+    printf(" \"per_detector_per_bin_fe\": %.1f,\n", det[0]);
+    printf(" \"imrphenompv2_per_detector_per_bin_fe\": %.1f\n}\n", det[1]);
+    return additive ? 0 : 1;
 }

@@ -985,6 +985,39 @@ def test_order4_stencil_is_the_five_point_rule_and_its_tc_shift_mix():
                 assert np.max(np.abs(mix)) > 1e-3 * np.max(np.abs(consistent))
 
 
+def test_relative_step_is_the_absolute_step_of_each_coordinate():
+    """epsilon < 0 asks for the step |epsilon| * |v_i| per basis coordinate (|epsilon| where v_i is 0): each
+    derivative row equals the absolute-step row at that coordinate's step, and the Fisher closed forms of the
+    full basis hold with it (F(lnDL,lnDL) = (h|h), F(phi,phi) = 4 (h|h), F(tc,tc) at tc = 0)."""
+    f = np.arange(20, 1024.01, 0.25)
+    x = f / 215.
+    psd = 1e-49 * (x ** -4.14 - 5 * x ** -2 + 111 * (1 - x ** 2 + x ** 4 / 2) / (1 + x ** 2 / 2))
+    rel = 1e-7
+    for method, dim, sky in (("IMRPhenomD", 7, True), ("IMRPhenomD", 11, False), ("MCMC_IMRPhenomD", 11, False)):
+        p = O.make_params(mass1=36, mass2=29, spin1=[0, 0, .3], spin2=[0, 0, -.1], DL=410, incl=2.9, RA=1.37,
+                          DEC=-1.26, psi=2.66, f_ref=20, tc=0., phiRef=0., sky_average=sky)
+        v, _ = O.unpack_parameters(p, method, dim)
+        assert np.any(v == 0) and np.any(np.abs(v) > 1)
+        D = O.calculate_derivatives(f[:801], method, 1, 0, dim, p, order=2, epsilon=-rel)
+        for i in range(dim):
+            h = rel * abs(v[i]) if v[i] != 0 else rel
+            A = O.calculate_derivatives(f[:801], method, 1, 0, dim, p, order=2, epsilon=h)
+            assert np.array_equal(D[i], A[i]), (method, i)
+    p = O.make_params(mass1=36, mass2=29, spin1=[0, 0, .3], spin2=[0, 0, -.1], DL=410, incl=2.9, RA=1.37,
+                      DEC=-1.26, psi=2.66, f_ref=20, tc=0.)
+    F = O.fisher_numerical(f, psd, "IMRPhenomD", 0, 0, 11, p, epsilon=-rel)
+    r = O.detector_response(f, p, 0)
+    hh = 4 * O.simpsons_sum(f[1] - f[0], np.abs(r) ** 2 / psd)
+    assert abs(F[6, 6] / hh - 1) < 1e-5
+    assert abs(F[4, 4] / (4 * hh) - 1) < 1e-5
+    ftt = 4 * O.simpsons_sum(f[1] - f[0], (2 * np.pi * (f - 20.)) ** 2 * np.abs(r) ** 2 / psd)
+    assert abs(F[5, 5] / ftt - 1) < 1e-4
+    # a planted error: a step 10x the coordinate's in one row would move that row's derivative
+    D10 = O.calculate_derivatives(f[:801], "IMRPhenomD", 1, 0, 11, p, order=2, epsilon=10 * rel * abs(v[0]))
+    D1 = O.calculate_derivatives(f[:801], "IMRPhenomD", 1, 0, 11, p, order=2, epsilon=-rel)
+    assert not np.array_equal(D10[0], D1[0])
+
+
 # ---- Gauss-Legendre and log10-f integration rules (mcmc_gw.cpp:821-856, fisher.cpp:2736-2745) ------------------
 def _gl(x1, x2, n):
     """Gauss-Legendre nodes and weights on [x1, x2] (numpy's leggauss: the same rule gauleg builds,
