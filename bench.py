@@ -30,11 +30,16 @@ sys.path.insert(0, ROOT)
 METRIC = "IMRPhenomD 3-det logL evals/s, 1 GPU, mixed vs FP64 at |ΔlogL|≤1e-3; p99 ms"
 PEAK_VALU_TFLOPS = 78.6   # MI355X vector FMA peak without packed FP32 (FP64 and non-packed FP32 issue alike)
 PEAK_HBM_GBS = 8000.0
-# FLOP-equivalents per bin (frozen table): inspiral, intermediate, merger-ringdown, per detector
-FE_REGION = (121.0, 86.0, 124.0)
-FE_DET = 46.0
-FE_PV2_BIN = 280.0
-FE_PV2_DET = 54.0
+# FLOP-equivalents (fe) per non-zero bin, frozen from the operation count of the reference algorithm as the
+# oracle restates it (oracle/build/opcount; add/mul/cmp 1, div/sqrt 4, exp/log 16, pow 32, sin+cos 24, atan 20):
+# a bin in amplitude region a and phase region p costs FE_COMMON + FE_AMP[a] + FE_PHASE[p], plus FE_DET per
+# detector; IMRPhenomPv2 adds FE_PV2_BIN per bin and FE_PV2_DET per detector and bin.
+FE_COMMON = 77.0
+FE_AMP = (31.0, 19.0, 47.0)     # f < 0.014/M, up to the amplitude peak, above
+FE_PHASE = (82.0, 40.0, 57.0)   # f < 0.018/M (with the powers of f it needs), up to f_RD / 2, above
+FE_DET = 55.0
+FE_PV2_BIN = 302.0
+FE_PV2_DET = 0.0
 
 
 def log(*a):
@@ -42,9 +47,11 @@ def log(*a):
 
 
 def fe_per_launch(counts, ndet, pv2=False):
-    nz = counts[:, 0] + counts[:, 1] + counts[:, 2]
-    fe = counts[:, 0] * FE_REGION[0] + counts[:, 1] * FE_REGION[1] + counts[:, 2] * FE_REGION[2]
-    fe = fe + ndet * nz * FE_DET
+    """counts: (per-instance [3, 3] non-zero bins by amplitude / phase region, zero bins) from work_counts"""
+    c, _zero = counts
+    per = FE_COMMON + np.add.outer(np.array(FE_AMP), np.array(FE_PHASE))
+    nz = c.sum(axis=(1, 2))
+    fe = (c * per[None]).sum(axis=(1, 2)) + ndet * nz * FE_DET
     if pv2:
         fe = fe + nz * FE_PV2_BIN + ndet * nz * FE_PV2_DET
     return float(fe.sum()), float(nz.sum())
@@ -149,36 +156,139 @@ def bench_uneven(G, W, steps):
 
 def bench_fisher(G, W, ev, n_points=2048):
     """IMRPhenomPv2 13-parameter Fisher matrices at n_points precessing parameter points on the resident
-    3-detector event: order-2 central differences with FP64 numerators, Gram on the matrix cores (FP64, or
-    FP32 operands with FP64 accumulation across tiles), FP64 LU inversion -> marginal sigmas."""
+    3-detector event: order-2 central differences with FP64 numerators, Gram on the FP64 matrix cores (the
+    GWH_FP64 and GWH_MIXED modes), or FP32 operands on the FP32 matrix cores with FP64 accumulation across tiles
+    (GWH_FISHER_GRAM_FP32), FP64 LU inversion -> marginal sigmas."""
     P = W.precessing_params(n_points, seed_stream=7)
     g = W.rng(3, 0, stream=8)
     P["chip"] = g.uniform(0.05, 0.9, n_points)
     P["phip"] = g.uniform(0, 2 * np.pi, n_points)
-    G.fisher_batch(ev, P[:8], G.IMRPHENOMPV2, G.BASIS_PV2_FULL, 13, precision=G.MIXED)  # warm-up
+    G.fisher_batch(ev, P[:8], G.IMRPHENOMPV2, G.BASIS_PV2_FULL, 13, precision=G.FP64)  # warm-up
     out = {}
     res = {}
-    for name, prec in (("mixed", G.MIXED), ("fp64", G.FP64)):
+    for name, prec in (("fp64", G.FP64), ("fp32_gram", G.FISHER_GRAM_FP32)):
         a = time.perf_counter()
         F, sig = G.fisher_batch(ev, P, G.IMRPHENOMPV2, G.BASIS_PV2_FULL, 13, precision=prec, with_sigma=True)
         el = time.perf_counter() - a
         res[name] = (F, sig)
         out[name + "_points_per_s"] = n_points / el
         out[name + "_s"] = el
-    (Fm, sm), (Fd, sd) = res["mixed"], res["fp64"]
+    (Fd, sd), (Fs, ss) = res["fp64"], res["fp32_gram"]
+    # the mixed precision flag runs the FP64 Gram: its matrices are those of the FP64 mode
+    Fm, sm = G.fisher_batch(ev, P[:64], G.IMRPHENOMPV2, G.BASIS_PV2_FULL, 13, precision=G.MIXED, with_sigma=True)
     dg = np.sqrt(np.abs(np.einsum("nii->ni", Fd)))
-    norm = np.abs(Fm - Fd) / (dg[:, :, None] * dg[:, None, :])
-    rel = np.abs(Fm - Fd) / np.maximum(np.abs(Fd), 1e-300)
-    fin = np.isfinite(sm) & np.isfinite(sd)
-    srel = np.abs(sm[fin] / sd[fin] - 1)
+    norm = np.abs(Fs - Fd) / (dg[:, :, None] * dg[:, None, :])
+    fin = np.isfinite(ss) & np.isfinite(sd)
+    srel = np.abs(ss[fin] / sd[fin] - 1)
+    fin64 = np.isfinite(sm) & np.isfinite(sd[:64])
     out.update({"points": n_points, "dim": 13, "bins": int(ev.n), "detectors": int(ev.ndet),
-                "mixed_vs_fp64_max_normalised_element_error": float(norm.max()),
-                "mixed_vs_fp64_median_element_rel_error": float(np.median(rel)),
-                "mixed_vs_fp64_sigma_rel_error_median": float(np.median(srel)) if srel.size else None,
-                "mixed_vs_fp64_sigma_rel_error_max": float(srel.max()) if srel.size else None,
+                "mixed_equals_fp64": bool(np.array_equal(Fm, Fd[:64])),
+                "mixed_vs_fp64_sigma_rel_error_max": float(np.max(np.abs(sm[fin64] / sd[:64][fin64] - 1)))
+                if fin64.any() else None,
+                "fp32_gram_vs_fp64_max_normalised_element_error": float(norm.max()),
+                "fp32_gram_vs_fp64_sigma_rel_error_median": float(np.median(srel)) if srel.size else None,
+                "fp32_gram_vs_fp64_sigma_rel_error_max": float(srel.max()) if srel.size else None,
                 "finite_sigma_fraction_fp64": float(np.isfinite(sd).mean())})
-    log("Fisher (Pv2, 13 params, %d points): mixed %.1f points/s, fp64 %.1f points/s, max normalised element "
-        "error %.2e" % (n_points, out["mixed_points_per_s"], out["fp64_points_per_s"], norm.max()))
+    log("Fisher (Pv2, 13 params, %d points): fp64 Gram %.1f points/s, fp32 Gram %.1f points/s (max normalised "
+        "element error %.2e, sigma error median %.2e); mixed == fp64: %s"
+        % (n_points, out["fp64_points_per_s"], out["fp32_gram_points_per_s"], norm.max(),
+           out["fp32_gram_vs_fp64_sigma_rel_error_median"] or float("nan"), out["mixed_equals_fp64"]))
+    return out
+
+
+def bench_c0(G, W, O=None, cpu_threads=0, cpu_seconds=3.0):
+    """The single-detector workload (Hanford, 4017 bins 20-1024 Hz at 1/4 Hz, 1024 parameter vectors around the
+    injection) through the batched API, and its 7x7 sky-averaged IMRPhenomD Fisher at the injection as a single
+    call: the reference's absolute step 1e-8 and the 1e-7 relative step.  With O (the oracle) the same work is
+    timed on the host cores."""
+    c = W.config0()
+    f, psd = c["f"], c["psd"]
+    inj = c["injection"]
+    pinj = G.make_params(**{k: (list(inj[k]) if inj[k].shape else inj[k].item()) for k in inj.dtype.names})
+    data = [G.fourier_detector_response(f, pinj, 0) + W.noise(f, psd, c["noise_gen"])]
+    ev = G.Event(f, data, [psd], [0])
+    P = c["params"]
+    out = {"instances": len(P), "bins": len(f)}
+    for name, prec in (("fp64", G.FP64), ("mixed", G.MIXED)):
+        G.loglike_batch(ev, P, G.IMRPHENOMD, prec)
+        reps = 20
+        a = time.perf_counter()
+        for _ in range(reps):
+            ll = G.loglike_batch(ev, P, G.IMRPHENOMD, prec)
+        out[name + "_batched_evals_per_s"] = reps * len(P) / (time.perf_counter() - a)
+    sa = dict((k, (list(inj[k]) if inj[k].shape else inj[k].item())) for k in inj.dtype.names)
+    sa["sky_average"] = 1
+    gp = G.make_params(**sa)
+    for name, eps in (("abs_1e-8", 1e-8), ("rel_1e-7", -1e-7)):
+        G.fisher_numerical(f, psd, G.BASIS_PHENOMD_SKYAVG, 0, 0, 7, gp, epsilon=eps)
+        reps = 50
+        a = time.perf_counter()
+        for _ in range(reps):
+            F = G.fisher_numerical(f, psd, G.BASIS_PHENOMD_SKYAVG, 0, 0, 7, gp, epsilon=eps)
+        out["fisher_sa7_" + name + "_calls_per_s"] = reps / (time.perf_counter() - a)
+        out["fisher_sa7_" + name + "_sigma"] = [float(x) for x in np.sqrt(np.abs(np.diag(np.linalg.inv(F))))]
+    if O is not None:
+        op = O.make_params(**sa)
+        plist = [O.make_params(**{k: (list(P[i][k]) if P[i][k].shape else P[i][k].item()) for k in P.dtype.names})
+                 for i in range(len(P))]
+        a = time.perf_counter()
+        m = 0
+        while time.perf_counter() - a < cpu_seconds:
+            O.network_loglike_batch(f, data, [psd], [0], plist, nthreads=cpu_threads)
+            m += len(plist)
+        out["cpu_evals_per_s"] = m / (time.perf_counter() - a)
+        a = time.perf_counter()
+        k = 0
+        while time.perf_counter() - a < cpu_seconds / 3 or k < 2:
+            O.fisher_numerical(f, psd, "IMRPhenomD", 0, 0, 7, op, epsilon=1e-8)
+            k += 1
+        out["cpu_fisher_sa7_calls_per_s_1thread"] = k / (time.perf_counter() - a)
+    ev.close()
+    log("c0: batched fp64 %.3e, mixed %.3e evals/s; SA Fisher %.0f calls/s (abs step), %.0f (relative step)%s"
+        % (out["fp64_batched_evals_per_s"], out["mixed_batched_evals_per_s"], out["fisher_sa7_abs_1e-8_calls_per_s"],
+           out["fisher_sa7_rel_1e-7_calls_per_s"],
+           "; cpu %.3e evals/s, %.1f Fisher/s (1 thread)" % (out["cpu_evals_per_s"],
+                                                            out["cpu_fisher_sa7_calls_per_s_1thread"])
+           if O is not None else ""))
+    return out
+
+
+def bench_threaded_calls(G, W, f, data, psd, dets, inj, batched_rate, threads=(1, 32, 64), count=8192):
+    """The reference-signature C++ layer called from host threads of one process (the threaded single-call
+    driver, host/gwh_threaded_calls.cpp): FP64 MCMC_likelihood_extrinsic calls on the c1 arrays, coalesced by
+    the engine into shared launches.  Calls per second per thread count and cache mode; every thread count's
+    results must equal the one-thread results bit for bit."""
+    import subprocess
+    import tempfile
+    exe = G.THREADED_CALLS
+    if not os.path.exists(exe):
+        raise RuntimeError("threaded-call driver not built (python -m gw_analysis_tools_amd.build)")
+    P = W.perturbed_params(inj, count, step=7, scale=0.3)
+    out = {"calls": count, "precision": "fp64", "runs": []}
+    with tempfile.TemporaryDirectory() as tmp:
+        inp = os.path.join(tmp, "in.bin")
+        G.write_threaded_calls_input(inp, f, data, [psd] * len(dets), dets, P)
+        ref = None
+        for mode in ("pointer", "content"):
+            for t in threads:
+                if mode == "content" and t not in (1, max(threads)):
+                    continue
+                o = os.path.join(tmp, "out_%s_%d.bin" % (mode, t))
+                r = subprocess.run([exe, inp, o, str(t), mode, "IMRPhenomD"], capture_output=True, text=True,
+                                   timeout=300)
+                if r.returncode != 0:
+                    raise RuntimeError("driver failed: " + r.stderr.strip())
+                rec = json.loads(r.stdout)
+                res = np.fromfile(o, dtype=np.float64)
+                if ref is None:
+                    ref = res
+                rec["bitwise_equal_to_one_thread"] = bool(np.array_equal(res, ref))
+                rec["fraction_of_batched_fp64_evals_per_s"] = rec["calls_per_s"] / batched_rate
+                out["runs"].append(rec)
+                log("threaded single calls (%s keys, %d threads): %.3e calls/s (%.0f%% of batched FP64), %d shared "
+                    "launches, up to %d calls per launch, equal to one thread: %s"
+                    % (mode, t, rec["calls_per_s"], 100 * rec["fraction_of_batched_fp64_evals_per_s"],
+                       rec["batches"], rec["max_calls_per_batch"], rec["bitwise_equal_to_one_thread"]))
     return out
 
 
@@ -254,11 +364,13 @@ def bench_maxll(G, W, ev, inj, n=512, reps=10):
     return out
 
 
-def bench_serving(G, W, rates, seconds, fisher_fraction=0.1, clients=32):
-    """Open-loop Poisson arrivals from `clients` threads at each aggregate rate: a request is a
-    log-likelihood batch of U{16..1024} instances (IMRPhenomD or IMRPhenomPv2, 50/50, mixed precision) on
-    one of 64 resident events with the uneven-batch grid lengths, or (fisher_fraction) a 1..64-point
-    Fisher batch; admission by a 200 GB HBM budget.  Latency = submission to completion per request."""
+def bench_serving(G, W, rates, seconds, fisher_fraction=0.1, clients=32, min_requests=1000):
+    """Open-loop Poisson arrivals from `clients` threads at each aggregate rate, for max(seconds,
+    min_requests / rate) seconds: a request is a log-likelihood batch of U{16..1024} instances (IMRPhenomD or
+    IMRPhenomPv2, 50/50, mixed precision) on one of 64 resident events with the uneven-batch grid lengths, or
+    (fisher_fraction) a 1..64-point FP64 Fisher batch of either model (IMRPhenomD 11-parameter or IMRPhenomPv2
+    15-parameter MCMC basis); admission by a 200 GB HBM budget.  Latency = submission to completion per
+    request; served rate = completed requests / (last completion - first arrival)."""
     import threading
     specs = W.config2(n_events=64, chains=4)
     g = W.rng(4, 1)
@@ -277,7 +389,12 @@ def bench_serving(G, W, rates, seconds, fisher_fraction=0.1, clients=32):
         if rng.random() < fisher_fraction:
             npts = int(rng.integers(1, 65))
             P = W.perturbed_params(sp["injection"], npts, step=k, config=4, scale=0.1)
-            pool.append(("fisher", e, P, G.IMRPHENOMD, None))
+            if pv2:
+                P["spin1"][:, 0] = 0.2
+                P["spin2"][:, 1] = -0.1
+                pool.append(("fisher", e, P, G.IMRPHENOMPV2, (G.BASIS_PV2_MCMC, 15)))
+            else:
+                pool.append(("fisher", e, P, G.IMRPHENOMD, (G.BASIS_PHENOMD_MCMC, 11)))
         else:
             n = int(rng.integers(16, 1025))
             P = W.perturbed_params(sp["injection"], n, step=k, config=4, scale=0.1)
@@ -296,6 +413,7 @@ def bench_serving(G, W, rates, seconds, fisher_fraction=0.1, clients=32):
     for rate in rates:
         lock = threading.Lock()
         lats, done_t, kinds = [], [], []
+        span = max(seconds, min_requests / rate)
         t_start = time.perf_counter() + 0.05
 
         def client(cid):
@@ -304,14 +422,14 @@ def bench_serving(G, W, rates, seconds, fisher_fraction=0.1, clients=32):
             mine = []
             while True:
                 t += r2.exponential(clients / rate)
-                if t - t_start > seconds:
+                if t - t_start > span:
                     break
-                kind, e, P, model, _ = pool[r2.integers(len(pool))]
+                kind, e, P, model, fb = pool[r2.integers(len(pool))]
                 dt = t - time.perf_counter()
                 if dt > 0:
                     time.sleep(dt)
                 if kind == "fisher":
-                    tk, _, _ = br.submit_fisher(e, P, model, G.BASIS_PHENOMD_MCMC, 11, precision=G.FP64)
+                    tk, _, _ = br.submit_fisher(e, P, model, fb[0], fb[1], precision=G.FP64)
                 else:
                     tk, _ = br.submit_loglike(e, P, model, G.MIXED)
                 mine.append((tk, kind))
@@ -330,13 +448,20 @@ def bench_serving(G, W, rates, seconds, fisher_fraction=0.1, clients=32):
         wall = max(done_t) - t_start if done_t else float("nan")
         lat_ms = np.array(lats) / 1e3
         ll = lat_ms[np.array(kinds) == "loglike"]
-        rec = {"rate_req_per_s": rate, "requests": len(lats), "throughput_req_per_s": len(lats) / wall,
+        fi = lat_ms[np.array(kinds) == "fisher"]
+        rec = {"rate_req_per_s": rate, "seconds": span, "requests": len(lats),
+               "offered_req_per_s": len(lats) / span, "throughput_req_per_s": len(lats) / wall,
+               "fisher_requests": int(fi.size),
+               "fisher_p50_ms": float(np.percentile(fi, 50)) if fi.size else None,
+               "fisher_p99_ms": float(np.percentile(fi, 99)) if fi.size else None,
                "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
                "loglike_p50_ms": float(np.percentile(ll, 50)) if ll.size else None,
                "loglike_p99_ms": float(np.percentile(ll, 99)) if ll.size else None}
         out["rates"].append(rec)
-        log("serving %5d req/s offered: %7.1f req/s served, p50 %.2f ms, p99 %.2f ms (%d requests)"
-            % (rate, rec["throughput_req_per_s"], rec["p50_ms"], rec["p99_ms"], len(lats)))
+        log("serving %5d req/s offered (%.1f realised): %7.1f req/s served, p50 %.2f ms, p99 %.2f ms (%d requests, "
+            "%d Fisher, Fisher p99 %.1f ms)"
+            % (rate, rec["offered_req_per_s"], rec["throughput_req_per_s"], rec["p50_ms"], rec["p99_ms"], len(lats),
+               fi.size, rec["fisher_p99_ms"] or float("nan")))
     out["broker"] = br.stats()
     # mixed vs FP64 on a sample of the likelihood requests
     worst = 0.0
@@ -352,6 +477,89 @@ def bench_serving(G, W, rates, seconds, fisher_fraction=0.1, clients=32):
     br.close()
     for e, _ in events:
         e.close()
+    return out
+
+
+def cpu_threads():
+    """host threads for the CPU legs: the CPUs this process may run on, at most 16 (a one-GPU box's share of its
+    host cores)"""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(avail, 16, env if env > 0 else 16))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baselines(G, W, f, data, psd, dets, P, n, seconds):
+    """The oracle (the reference algorithm restated in C, one instance per call) on the host cores: the headline
+    workload (bounded sample, OpenMP over instances), a warm single-thread rate over at least one second,
+    IMRPhenomPv2 on the same grid, and 13-parameter IMRPhenomPv2 network Fisher matrices (threads over points)."""
+    import concurrent.futures
+    import oracle as O
+    cores = cpu_threads()
+
+    def oparams(rec):
+        return O.make_params(**{k: (list(rec[k]) if rec[k].shape else rec[k].item()) for k in rec.dtype.names})
+
+    plist = [oparams(P[i]) for i in range(n)]
+    O.network_loglike_batch(f, data, [psd] * 3, dets, plist[:2], nthreads=1)  # warm
+    a = time.perf_counter()
+    k = 0
+    while time.perf_counter() - a < 1.0:
+        O.network_loglike_batch(f, data, [psd] * 3, dets, plist[k % n:k % n + 1], nthreads=1)
+        k += 1
+    per_eval_1t = (time.perf_counter() - a) / k
+    m = int(max(cores, seconds * cores / per_eval_1t))
+    sample = [plist[i % n] for i in range(m)]
+    a = time.perf_counter()
+    _, used = O.network_loglike_batch(f, data, [psd] * 3, dets, sample, nthreads=cores)
+    el = time.perf_counter() - a
+    out = {"value": m / el, "unit": "logL evals/s", "cores": int(used), "kind": "port", "cpu_model": cpu_model(),
+           "host_cpus_visible": os.cpu_count(),
+           "sample": "%d evals of the headline workload (the first %d instances of the first batch, cycled), "
+                     "oracle network_loglike with OpenMP over instances on %d threads (a one-GPU box's share)"
+                     % (m, min(m, n), used),
+           "seconds": el, "single_thread_ms_per_eval": per_eval_1t * 1e3, "single_thread_evals_timed": k}
+    # IMRPhenomPv2 likelihood on the same grid
+    P3 = W.precessing_params(64)
+    p3 = [oparams(P3[i]) for i in range(len(P3))]
+    O.network_loglike_batch(f, data, [psd] * 3, dets, p3[:1], model=O.IMRPHENOMPV2, nthreads=1)
+    m3 = max(cores, int(seconds / 3 * cores / (3 * per_eval_1t)))
+    a = time.perf_counter()
+    O.network_loglike_batch(f, data, [psd] * 3, dets, [p3[i % 64] for i in range(m3)], model=O.IMRPHENOMPV2,
+                            nthreads=cores)
+    out["imrphenompv2_evals_per_s"] = m3 / (time.perf_counter() - a)
+    # 13-parameter IMRPhenomPv2 network Fisher (the GPU Fisher leg's basis and points), threads over points
+    PF = W.precessing_params(2048, seed_stream=7)
+    g = W.rng(3, 0, stream=8)
+    PF["chip"] = g.uniform(0.05, 0.9, 2048)
+    PF["phip"] = g.uniform(0, 2 * np.pi, 2048)
+
+    def fisher(i):
+        op = oparams(PF[i])
+        return sum(O.fisher_numerical(f, psd, "IMRPhenomPv2", d, dets[0], 13, op) for d in dets)
+
+    fisher(0)
+    npts = cores
+    a = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(fisher, range(1, 1 + npts)))
+    out["imrphenompv2_fisher13_points_per_s"] = npts / (time.perf_counter() - a)
+    log("cpu baseline (%s, %d threads): %.1f evals/s (single thread %.2f ms/eval over %d evals); Pv2 %.1f evals/s; "
+        "Pv2 13-parameter Fisher %.2f points/s"
+        % (out["cpu_model"], used, out["value"], per_eval_1t * 1e3, k, out["imrphenompv2_evals_per_s"],
+           out["imrphenompv2_fisher13_points_per_s"]))
     return out
 
 
@@ -372,6 +580,7 @@ def main():
     ap.add_argument("--no-fisher", action="store_true", help="skip the batched Fisher leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving (broker) leg")
     ap.add_argument("--no-pt", action="store_true", help="skip the device parallel-tempering leg")
+    ap.add_argument("--no-threads", action="store_true", help="skip the threaded single-call leg")
     ap.add_argument("--serving-rates", default="50,100,200,500,1000,2000")
     ap.add_argument("--serving-seconds", type=float, default=2.0)
     ap.add_argument("--c2-steps", type=int, default=20)
@@ -600,28 +809,31 @@ def main():
             log("serving leg failed:", e)
             serving = {"error": str(e)}
 
+    # ---- the single-detector workload and its sky-averaged Fisher (GPU; CPU beside it unless --no-cpu) ----
+    c0 = None
+    try:
+        O0 = None
+        if not args.no_cpu:
+            import oracle as O0
+        c0 = bench_c0(G, W, O0, cpu_threads=cpu_threads())
+    except Exception as e:  # reported, not fatal for the headline
+        log("c0 leg failed:", e)
+        c0 = {"error": str(e)}
+
+    # ---- the reference-signature C++ layer called from many host threads (coalesced single calls) ----
+    thr = None
+    if not args.no_threads:
+        try:
+            thr = bench_threaded_calls(G, W, f, data, psd, dets, inj, fp64_rate)
+        except Exception as e:  # reported, not fatal for the headline
+            log("threaded-call leg failed:", e)
+            thr = {"error": str(e)}
+
     # ---- CPU baseline: the oracle (reference algorithm, C, one instance per call) ----
     cpu = None
     if not args.no_cpu:
         try:
-            import oracle as O
-            cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-            plist = [O.make_params(**{k: (list(P[i][k]) if P[i][k].shape else P[i][k].item())
-                                      for k in P.dtype.names}) for i in range(n)]
-            a = time.perf_counter()
-            O.network_loglike_batch(f, data, [psd] * 3, dets, plist[:8], nthreads=1)
-            per_eval_1t = (time.perf_counter() - a) / 8
-            m = int(max(cores, args.cpu_seconds * cores / per_eval_1t))
-            sample = [plist[i % n] for i in range(m)]
-            a = time.perf_counter()
-            out, used = O.network_loglike_batch(f, data, [psd] * 3, dets, sample, nthreads=cores)
-            el = time.perf_counter() - a
-            cpu = {"value": m / el, "unit": "logL evals/s", "cores": int(used), "kind": "port",
-                   "sample": "%d evals of the same workload (first %d instances of the first batch, cycled), "
-                             "oracle/gwo network_loglike with OpenMP over instances; single-thread %.2f ms/eval"
-                             % (m, min(m, n), per_eval_1t * 1e3),
-                   "seconds": el}
-            log("cpu baseline: %.1f evals/s on %d threads (%.1f s)" % (m / el, used, el))
+            cpu = cpu_baselines(G, W, f, data, psd, dets, P, n, args.cpu_seconds)
         except Exception as e:
             log("cpu baseline failed:", e)
             cpu = {"error": str(e)}
@@ -645,6 +857,8 @@ def main():
         "serving_c4": serving,
         "pt_step_on_device_c1": ptd,
         "maximized_intrinsic_likelihood_c1": mxl,
+        "single_detector_c0": c0,
+        "threaded_single_calls_c1": thr,
     }
     print(json.dumps(rec))
 

@@ -31,6 +31,8 @@ GIMRPHENOMPV2 = 7
 COSMO_PLANCK15, COSMO_PLANCK13, COSMO_WMAP9, COSMO_WMAP7, COSMO_WMAP5 = range(5)
 FP64 = 0
 MIXED = 1
+FISHER_GRAM_FP32 = 2  # Fisher only: FP32 operands on the FP32 matrix cores
+CACHE_BY_CONTENT, CACHE_BY_POINTER = 0, 1  # how single calls find their cached event
 SIMPSONS = 0
 GAUSSLEG = 1
 DETECTORS = {"Hanford": 0, "Livingston": 1, "Virgo": 2, "Kagra": 3, "Indigo": 4, "CE": 5, "ET1": 6, "ET2": 7,
@@ -48,6 +50,9 @@ EXPORTS = [
     "gwh_log_likelihood_internal", "gwh_mcmc_likelihood_extrinsic", "gwh_fisher_numerical",
     "gwh_event_create", "gwh_event_destroy", "gwh_event_bytes", "gwh_loglike_batch",
     "gwh_loglike_batch_device", "gwh_loglike_batch_device_noalloc", "gwh_set_bins_per_lane", "gwh_fisher_batch",
+    "gwh_event_create_weighted", "gwh_gauleg", "gwh_set_single_call_cache", "gwh_host_data_register",
+    "gwh_host_data_release",
+    "gwh_single_call_stats",
     "gwh_time_loglike_batch_device", "gwh_device_alloc", "gwh_device_free", "gwh_memcpy_h2d",
     "gwh_memcpy_d2h", "gwh_work_counts", "gwh_create_coherent_GW_detection",
     "gwh_kernel_timing_start", "gwh_kernel_timing_every", "gwh_kernel_timing_stop", "gwh_device_copy",
@@ -156,6 +161,12 @@ def lib():
         "gwh_mcmc_likelihood_extrinsic": ([I, pp, I, ip, dpp, dpp, dpp, dpp, I, I, ip, I, D], D),
         "gwh_fisher_numerical": ([dp, I, I, I, I, dp, I, pp, I, dp, D], I),
         "gwh_event_create": ([dp, I, I, ip, dp, dp, ip], I),
+        "gwh_event_create_weighted": ([dp, I, I, ip, dp, dp, I, dp, I, ip], I),
+        "gwh_gauleg": ([D, D, dp, dp, I], I),
+        "gwh_set_single_call_cache": ([I], I),
+        "gwh_host_data_register": ([I, ip, dpp, dpp, dpp, dpp, ip], I),
+        "gwh_host_data_release": ([I], I),
+        "gwh_single_call_stats": ([dp], I),
         "gwh_event_destroy": ([I], I), "gwh_event_bytes": ([I], ctypes.c_size_t),
         "gwh_loglike_batch": ([ip, I, I, I, V, I, dp], I),
         "gwh_loglike_batch_device": ([V, I, I, I, V, I, V, V], I),

@@ -52,7 +52,22 @@ def build_engine(verbose=False, jobs=8):
         print(" ".join(cmd))
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_host_programs(verbose)
     return LIB
+
+
+def build_host_programs(verbose=False):
+    """host programs over the library (host/*.cpp -> package directory), e.g. the threaded single-call driver"""
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    for src in sorted(glob.glob(os.path.join(HERE, "host", "*.cpp"))):
+        exe = os.path.join(HERE, os.path.splitext(os.path.basename(src))[0])
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-pthread", "-Wall", "-I", os.path.join(ROOT, "include"),
+               src, "-o", exe + ".tmp", "-L" + HERE, "-lgwat_hip", "-Wl,-rpath,$ORIGIN", "-L" + rocm_lib,
+               "-Wl,-rpath," + rocm_lib]
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.check_call(cmd)
+        os.replace(exe + ".tmp", exe)
 
 
 def _wait(item):

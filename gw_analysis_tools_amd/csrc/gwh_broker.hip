@@ -14,12 +14,16 @@
  *  - admission is by an HBM budget: resident events + the broker's scratch + the new batch's needs must
  *    fit, otherwise the batch waits for in-flight work to drain (and a request that cannot fit even on
  *    an idle device fails with GWH_ERR_NOMEM);
- *  - Fisher requests run one at a time on a worker thread with its own stream and scratch, concurrently
- *    with the likelihood batches.
+ *  - Fisher requests go to two worker threads, each with its own stream and scratch, concurrently with the
+ *    likelihood batches; a worker merges the queued Fisher requests of the same event, model, basis,
+ *    precision and step into one batch, and sizes its passes (points per pass) so that resident events +
+ *    likelihood scratch + every Fisher scratch stay within the HBM budget (waiting for in-flight work when
+ *    the budget is taken; refusing with GWH_ERR_NOMEM what cannot fit on an idle device);
+ *  - every event a request reads is pinned from submission until its ticket is collected (gwh_broker_wait)
+ *    or the broker is destroyed: gwh_event_destroy returns GWH_ERR_BUSY meanwhile.
  *
- * An instance's result does not depend on what it is batched with (per-event tiling, fixed-order sums),
- * so a request's results are bit-identical alone and in the mix.  Events referenced by pending
- * requests must stay alive until those requests complete.
+ * An instance's result (and a Fisher point's) does not depend on what it is batched with (per-event tiling,
+ * fixed-order sums), so a request's results are bit-identical alone and in the mix.
  */
 #include <hip/hip_runtime.h>
 
@@ -52,8 +56,13 @@ struct Request {
     int basis = 0, dim = 0;
     double eps = 0;
     clk::time_point submitted, finished;
-    int status = 0; /* 0 pending, else the GWH status */
+    int status = 0;           /* 0 pending, else the GWH status */
+    std::vector<int> pinned;  /* distinct events held for this request */
+    std::condition_variable done; /* its waiter (under the broker lock): one wake-up per request */
 };
+
+constexpr int kFisherWorkers = 2;
+constexpr int kFisherMaxPoints = 256; /* points of one merged Fisher batch */
 
 struct Lane {
     hipStream_t stream = nullptr;
@@ -73,20 +82,23 @@ struct Broker {
     size_t budget = 0;
     int max_n = 0;
     std::mutex m;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work;
     std::deque<std::shared_ptr<Request>> queue;
     std::map<long long, std::shared_ptr<Request>> all;
     long long next_ticket = 1;
     bool stop = false;
     Lane lanes[2];
     std::thread th;
-    /* Fisher requests: their own queue, worker thread, stream and scratch */
+    /* Fisher requests: their own queue and workers, each with a stream and scratch */
     std::deque<std::shared_ptr<Request>> fisher_queue;
     std::condition_variable cv_fisher;
-    FisherScratch fisher;
-    std::thread fisher_th;
+    FisherScratch fisher[kFisherWorkers];
+    bool fisher_busy[kFisherWorkers] = {false, false};
+    size_t fisher_reserved[kFisherWorkers] = {0, 0}; /* bytes a worker's admitted batch may grow its scratch to */
+    std::thread fisher_th[kFisherWorkers];
     /* statistics */
     long long batches = 0, instances = 0, fishers = 0, max_merged = 0, admission_waits = 0;
+    long long fisher_batches = 0, max_fisher_merged = 0;
     size_t peak_bytes = 0;
 };
 
@@ -110,20 +122,59 @@ size_t lane_bytes(const Lane &L)
            L.d_logl.cap;
 }
 
-/* device bytes a batch of n instances over `events` needs in one lane's scratch */
-size_t batch_bytes(const std::vector<int> &events, int n)
+/* device bytes the broker holds: resident events, likelihood lanes and Fisher scratch (engine lock held) */
+size_t used_bytes(const Broker &B)
+{
+    size_t u = resident_bytes() + lane_bytes(B.lanes[0]) + lane_bytes(B.lanes[1]);
+    for (int w = 0; w < kFisherWorkers; w++) u += std::max(B.fisher[w].bytes(), B.fisher_reserved[w]);
+    return u;
+}
+
+/* pin the distinct events of a request (engine lock held) */
+int pin_events(Engine &E, Request &r, const std::vector<int> &evs)
+{
+    for (int e : evs)
+        if (e < 0 || e >= GWH_MAX_EVENTS || !E.events[e].live)
+            return gwh_set_error(GWH_ERR_INVALID, "unknown event in request");
+    for (int e : evs) {
+        bool seen = false;
+        for (int q : r.pinned) seen = seen || (q == e);
+        if (seen) continue;
+        r.pinned.push_back(e);
+        E.events[e].pins++;
+    }
+    return GWH_OK;
+}
+
+void unpin_events(Request &r)
 {
     Engine &E = gwh_engine();
-    size_t maxtiles = 1, tiles = 0;
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    for (int e : r.pinned)
+        if (E.events[e].pins > 0) E.events[e].pins--;
+    r.pinned.clear();
+}
+
+/* device bytes lane L holds after a batch of n instances over `events` (the buffers launch_batch and the
+ * batched launch ensure, grown as gwh_grown_capacity says; engine lock held) */
+size_t lane_bytes_after(const Lane &L, const std::vector<int> &events, int n)
+{
+    Engine &E = gwh_engine();
+    size_t maxtiles = 0, tiles = 0;
     for (int i = 0; i < n; i++) {
         const Event &ev = E.events[events[i]];
         size_t t = (ev.N + GWH_TPB * ev.desc.bpl - 1) / (GWH_TPB * ev.desc.bpl);
         tiles += t;
         if (t > maxtiles) maxtiles = t;
     }
-    return (size_t)n * (sizeof(InstCoef) + sizeof(gwh_params) + 2 * sizeof(int) + sizeof(double) +
-                        maxtiles * GWH_PARTIAL_STRIDE * sizeof(double)) +
-           tiles * sizeof(int2) + 64;
+    const Scratch &S = L.scratch;
+    const size_t m = (size_t)n;
+    return gwh_grown_capacity(m * sizeof(InstCoef), S.coef.cap) +
+           gwh_grown_capacity(m * maxtiles * GWH_PARTIAL_STRIDE * sizeof(double), S.partial.cap) +
+           gwh_grown_capacity(m * sizeof(int), S.active.cap) + gwh_grown_capacity(tiles * sizeof(int2), S.work.cap) +
+           gwh_grown_capacity(4 * sizeof(int), S.wcount.cap) +
+           gwh_grown_capacity(m * sizeof(gwh_params), L.d_params.cap) +
+           gwh_grown_capacity(m * sizeof(int), L.d_events.cap) + gwh_grown_capacity(m * sizeof(double), L.d_logl.cap);
 }
 
 void finish(Broker &B, const std::shared_ptr<Request> &r, int status)
@@ -131,7 +182,7 @@ void finish(Broker &B, const std::shared_ptr<Request> &r, int status)
     r->finished = clk::now();
     std::lock_guard<std::mutex> lk(B.m);
     r->status = status;
-    B.cv_done.notify_all();
+    r->done.notify_all();
 }
 
 /* launch one likelihood batch on lane L (engine lock held by the caller) */
@@ -174,7 +225,7 @@ int launch_batch(Broker &B, Lane &L)
     if (rc != GWH_OK) return rc;
     GWH_HIP(hipMemcpyAsync(L.h_logl, L.d_logl.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, L.stream));
     GWH_HIP(hipEventRecord(L.done, L.stream));
-    size_t used = resident_bytes() + lane_bytes(B.lanes[0]) + lane_bytes(B.lanes[1]);
+    size_t used = used_bytes(B);
     std::lock_guard<std::mutex> lk(B.m);
     B.batches++;
     B.instances += n;
@@ -195,25 +246,116 @@ void reap(Broker &B, Lane &L)
     L.busy = false;
 }
 
-void fisher_worker(Broker *bp)
+bool compatible_fisher(const Request &a, const Request &b)
+{
+    return a.events[0] == b.events[0] && a.model == b.model && a.basis == b.basis && a.precision == b.precision &&
+           a.dim == b.dim && a.eps == b.eps;
+}
+
+void fisher_worker(Broker *bp, int w)
 {
     Broker &B = *bp;
+    Engine &E = gwh_engine();
+    FisherScratch &F = B.fisher[w];
     for (;;) {
-        std::shared_ptr<Request> r;
+        std::vector<std::shared_ptr<Request>> batch;
         {
             std::unique_lock<std::mutex> lk(B.m);
             B.cv_fisher.wait(lk, [&] { return B.stop || !B.fisher_queue.empty(); });
             if (B.fisher_queue.empty()) return; /* stop requested and nothing left */
-            r = B.fisher_queue.front();
-            B.fisher_queue.pop_front();
+            /* the head and every compatible queued request, in arrival order, up to kFisherMaxPoints */
+            auto head = B.fisher_queue.front();
+            int n = 0;
+            for (auto it = B.fisher_queue.begin(); it != B.fisher_queue.end();) {
+                if (compatible_fisher(**it, *head) && (n == 0 || n + (*it)->n <= kFisherMaxPoints)) {
+                    n += (*it)->n;
+                    batch.push_back(*it);
+                    it = B.fisher_queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            B.fisher_busy[w] = true;
         }
-        int st = gwh_fisher_batch_with(r->events[0], r->model, r->basis, r->precision, r->params.data(), r->n, r->dim,
-                                       r->eps, r->out, r->sigma, B.fisher);
+        const Request &h = *batch[0];
+        std::vector<gwh_params> params;
+        bool want_sigma = false;
+        for (auto &r : batch) {
+            params.insert(params.end(), r->params.begin(), r->params.end());
+            want_sigma = want_sigma || (r->sigma != nullptr);
+        }
+        const int n = (int)params.size(), dim = h.dim;
+        /* admission: shrink the operand pass until this worker's scratch fits beside everything else */
+        int st = GWH_OK;
+        for (;;) {
+            bool others_busy, fits;
+            size_t mo = (size_t)1 << 30, need = 0;
+            {
+                /* size and reserve in one critical section: the dispatcher's admission, which holds the
+                   engine lock while it launches, sees the reservation before this worker allocates */
+                std::lock_guard<std::recursive_mutex> le(E.mu);
+                std::lock_guard<std::mutex> lk(B.m);
+                B.fisher_reserved[w] = 0;
+                const size_t other = used_bytes(B) - F.bytes();
+                const size_t avail = (B.budget > other) ? B.budget - other : 0;
+                others_busy = B.lanes[0].busy || B.lanes[1].busy;
+                for (int q = 0; q < kFisherWorkers; q++) others_busy = others_busy || (q != w && B.fisher_busy[q]);
+                for (;;) {
+                    need = gwh_fisher_batch_bytes(h.events[0], h.model, h.basis, h.precision, params.data(), n, mo,
+                                                  F, want_sigma);
+                    if (need == 0 || need <= avail || mo < 4096) break;
+                    mo /= 2;
+                }
+                fits = (need == 0 || need <= avail);
+                if (fits) B.fisher_reserved[w] = need;
+            }
+            if (!fits) {
+                if (others_busy) { /* wait for in-flight work, then size again */
+                    {
+                        std::lock_guard<std::mutex> lk(B.m);
+                        B.admission_waits++;
+                    }
+                    std::this_thread::sleep_for(std::chrono::microseconds(200));
+                    continue;
+                }
+                if (F.bytes() > 0) { /* give back this worker's own buffers and size from nothing */
+                    (void)hipStreamSynchronize(F.stream);
+                    std::lock_guard<std::recursive_mutex> le(E.mu);
+                    std::lock_guard<std::mutex> lk(B.m);
+                    for (DevBuf *d : {&F.params, &F.coef, &F.misc, &F.deriv, &F.gram, &F.out, &F.sig, &F.raw})
+                        d->release();
+                    continue;
+                }
+                st = gwh_set_error(GWH_ERR_NOMEM, "Fisher request exceeds the HBM budget");
+                break;
+            }
+            F.max_operand = mo;
+            break;
+        }
+        std::vector<double> out((size_t)n * dim * dim), sig(want_sigma ? (size_t)n * dim : 0);
+        if (st == GWH_OK)
+            st = gwh_fisher_batch_with(h.events[0], h.model, h.basis, h.precision, params.data(), n, dim, h.eps,
+                                       out.data(), want_sigma ? sig.data() : nullptr, F);
         {
+            std::lock_guard<std::recursive_mutex> le(E.mu);
             std::lock_guard<std::mutex> lk(B.m);
-            B.fishers++;
+            B.fishers += (long long)batch.size();
+            B.fisher_batches++;
+            if ((long long)batch.size() > B.max_fisher_merged) B.max_fisher_merged = (long long)batch.size();
+            const size_t used = used_bytes(B);
+            if (used > B.peak_bytes) B.peak_bytes = used;
+            B.fisher_reserved[w] = 0;
+            B.fisher_busy[w] = false;
         }
-        finish(B, r, st);
+        size_t k = 0;
+        for (auto &r : batch) {
+            if (st == GWH_OK) {
+                memcpy(r->out, out.data() + k * dim * dim, (size_t)r->n * dim * dim * sizeof(double));
+                if (r->sigma) memcpy(r->sigma, sig.data() + k * dim, (size_t)r->n * dim * sizeof(double));
+            }
+            k += r->n;
+            finish(B, r, st);
+        }
     }
 }
 
@@ -267,23 +409,40 @@ void dispatcher(Broker *bp)
             evs.insert(evs.end(), r->events.begin(), r->events.end());
             n += r->n;
         }
-        size_t need;
         {
             std::lock_guard<std::recursive_mutex> lk(E.mu);
-            need = batch_bytes(evs, n);
-            size_t used = resident_bytes() + lane_bytes(B.lanes[0]) + lane_bytes(B.lanes[1]);
+            size_t used = used_bytes(B);
             size_t mine = lane_bytes(*free_lane);
-            if (used - mine + need > B.budget) {
+            if (used - mine + lane_bytes_after(*free_lane, evs, n) > B.budget) {
                 bool other_busy = B.lanes[0].busy || B.lanes[1].busy;
+                {
+                    std::lock_guard<std::mutex> lq(B.m);
+                    for (int q = 0; q < kFisherWorkers; q++) other_busy = other_busy || B.fisher_busy[q];
+                }
                 if (other_busy) {
                     /* wait for in-flight work, then retry with the batch back at the queue head */
                     std::lock_guard<std::mutex> lq(B.m);
                     for (auto it = batch.rbegin(); it != batch.rend(); ++it) B.queue.push_front(*it);
                     B.admission_waits++;
                     batch.clear();
-                } else if (resident_bytes() + need > B.budget) {
-                    for (auto &r : batch) finish(B, r, gwh_set_error(GWH_ERR_NOMEM, "request exceeds the HBM budget"));
-                    batch.clear();
+                } else {
+                    /* nothing in flight: idle Fisher scratch is given back before refusing */
+                    {
+                        std::lock_guard<std::mutex> lq(B.m);
+                        for (int q = 0; q < kFisherWorkers; q++)
+                            if (!B.fisher_busy[q]) {
+                                (void)hipStreamSynchronize(B.fisher[q].stream);
+                                for (DevBuf *d : {&B.fisher[q].params, &B.fisher[q].coef, &B.fisher[q].misc,
+                                                  &B.fisher[q].deriv, &B.fisher[q].gram, &B.fisher[q].out,
+                                                  &B.fisher[q].sig, &B.fisher[q].raw})
+                                    d->release();
+                            }
+                    }
+                    if (used_bytes(B) - mine + lane_bytes_after(*free_lane, evs, n) > B.budget) {
+                        for (auto &r : batch)
+                            finish(B, r, gwh_set_error(GWH_ERR_NOMEM, "request exceeds the HBM budget"));
+                        batch.clear();
+                    }
                 }
             }
             if (!batch.empty()) {
@@ -299,12 +458,15 @@ void dispatcher(Broker *bp)
             }
         }
         if (batch.empty()) {
-            /* admission wait: block on the busy lanes' completion */
+            /* admission wait: block on the busy lanes' completion (or let a Fisher batch finish) */
+            bool any = false;
             for (Lane &L : B.lanes)
                 if (L.busy) {
                     (void)hipEventSynchronize(L.done);
                     reap(B, L);
+                    any = true;
                 }
+            if (!any) std::this_thread::sleep_for(std::chrono::microseconds(200));
         }
     }
 }
@@ -354,12 +516,13 @@ int gwh_broker_create(size_t hbm_budget_bytes, int max_batch_instances, int *bro
             return gwh_set_error(GWH_ERR_HIP, "stream creation failed");
         }
     }
-    if (hipStreamCreateWithFlags(&B->fisher.stream, hipStreamNonBlocking) != hipSuccess) {
-        delete B;
-        return gwh_set_error(GWH_ERR_HIP, "stream creation failed");
-    }
+    for (int w = 0; w < kFisherWorkers; w++)
+        if (hipStreamCreateWithFlags(&B->fisher[w].stream, hipStreamNonBlocking) != hipSuccess) {
+            delete B;
+            return gwh_set_error(GWH_ERR_HIP, "stream creation failed");
+        }
     B->th = std::thread(dispatcher, B);
-    B->fisher_th = std::thread(fisher_worker, B);
+    for (int w = 0; w < kFisherWorkers; w++) B->fisher_th[w] = std::thread(fisher_worker, B, w);
     std::lock_guard<std::mutex> lk(g_brokers_mu);
     *broker = g_next_broker++;
     g_brokers[*broker] = B;
@@ -383,12 +546,15 @@ int gwh_broker_destroy(int broker)
         B->cv_fisher.notify_all();
     }
     B->th.join(); /* drains the queue and the in-flight batches first */
-    B->fisher_th.join();
-    (void)hipStreamSynchronize(B->fisher.stream);
-    for (DevBuf *d : {&B->fisher.params, &B->fisher.coef, &B->fisher.misc, &B->fisher.deriv, &B->fisher.gram,
-                      &B->fisher.out, &B->fisher.sig, &B->fisher.raw})
-        if (d->p) (void)hipFree(d->p);
-    (void)hipStreamDestroy(B->fisher.stream);
+    for (int w = 0; w < kFisherWorkers; w++) {
+        B->fisher_th[w].join();
+        FisherScratch &F = B->fisher[w];
+        (void)hipStreamSynchronize(F.stream);
+        for (DevBuf *d : {&F.params, &F.coef, &F.misc, &F.deriv, &F.gram, &F.out, &F.sig, &F.raw}) d->release();
+        (void)hipStreamDestroy(F.stream);
+    }
+    /* tickets never collected release their events */
+    for (auto &kv : B->all) unpin_events(*kv.second);
     for (Lane &L : B->lanes) {
         (void)hipStreamSynchronize(L.stream);
         (void)hipEventDestroy(L.done);
@@ -421,17 +587,16 @@ int gwh_broker_submit_loglike(int broker, const int *events, int event0, int mod
     r->n = n;
     r->params.assign(params, params + n);
     r->events.resize(n);
+    for (int i = 0; i < n; i++) r->events[i] = events ? events[i] : event0;
     {
         std::lock_guard<std::recursive_mutex> lk(E.mu);
-        for (int i = 0; i < n; i++) {
-            int e = events ? events[i] : event0;
-            if (e < 0 || e >= GWH_MAX_EVENTS || !E.events[e].live)
-                return gwh_set_error(GWH_ERR_INVALID, "unknown event in request");
-            r->events[i] = e;
-        }
+        int rc = pin_events(E, *r, r->events);
+        if (rc != GWH_OK) return rc;
     }
     r->out = logl;
-    return submit(B, r, ticket);
+    int rc = submit(B, r, ticket);
+    if (rc != GWH_OK) unpin_events(*r);
+    return rc;
 }
 
 int gwh_broker_submit_fisher(int broker, int event, int model, int basis, int precision, const gwh_params *params,
@@ -452,7 +617,15 @@ int gwh_broker_submit_fisher(int broker, int event, int model, int basis, int pr
     r->events.assign(1, event);
     r->out = out;
     r->sigma = sigma;
-    return submit(B, r, ticket);
+    {
+        Engine &E = gwh_engine();
+        std::lock_guard<std::recursive_mutex> lk(E.mu);
+        int rc = pin_events(E, *r, r->events);
+        if (rc != GWH_OK) return rc;
+    }
+    int rc = submit(B, r, ticket);
+    if (rc != GWH_OK) unpin_events(*r);
+    return rc;
 }
 
 int gwh_broker_wait(int broker, long long ticket, double timeout_ms, double *latency_us)
@@ -465,16 +638,19 @@ int gwh_broker_wait(int broker, long long ticket, double timeout_ms, double *lat
     auto r = it->second;
     auto ready = [&] { return r->status != 0; };
     if (timeout_ms < 0)
-        B->cv_done.wait(lk, ready);
-    else if (!B->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
+        r->done.wait(lk, ready);
+    else if (!r->done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
         return gwh_set_error(GWH_ERR_BUSY, "ticket still pending");
     if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(r->finished - r->submitted).count();
     B->all.erase(ticket);
+    lk.unlock();
+    unpin_events(*r); /* the request no longer reads its events */
     return r->status;
 }
 
-/* stats[0..6]: batches, instances, Fisher requests, most requests merged into one batch, admission waits,
- * peak device bytes (resident events + broker scratch), requests pending */
+/* stats[0..8]: batches, instances, Fisher requests, most requests merged into one batch, admission waits,
+ * peak device bytes (resident events + broker scratch), requests pending, Fisher batches, most Fisher requests
+ * merged into one Fisher batch */
 int gwh_broker_stats(int broker, double *stats)
 {
     Broker *B = get(broker);
@@ -487,6 +663,8 @@ int gwh_broker_stats(int broker, double *stats)
     stats[4] = (double)B->admission_waits;
     stats[5] = (double)B->peak_bytes;
     stats[6] = (double)B->all.size();
+    stats[7] = (double)B->fisher_batches;
+    stats[8] = (double)B->max_fisher_merged;
     return GWH_OK;
 }
 

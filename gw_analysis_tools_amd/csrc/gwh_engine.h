@@ -100,7 +100,8 @@ struct Event {
     void *block = nullptr; /* one allocation holding every array of the event */
     size_t bytes = 0;
     EventDesc desc;
-    uint64_t key = 0;      /* content key for the single-call cache (0 = explicit event) */
+    uint64_t key = 0;      /* key of the single-call cache (0 = explicit event): content or pointer key */
+    bool ptr_key = false;  /* keyed by host pointers (pointer mode or registered arrays), not by content */
     double df_like = 0;    /* f[N/2] - f[N/2-1]: the likelihood's Simpson step (mcmc_gw.cpp:811) */
     double df_fisher = 0;  /* f[1] - f[0]: the Fisher's Simpson step (fisher.cpp:2746) */
     uint64_t last_use = 0;
@@ -110,6 +111,14 @@ struct Event {
 
 /* grow-only device buffer; a fixed one is a view of memory the buffer does not own (never grown or freed:
  * a request beyond its capacity is an error) */
+/* Capacity a growable device buffer of capacity `cap` holds after it is asked for `need` bytes: it keeps
+ * what it has when that suffices, else it is reallocated with a quarter of headroom (4 KiB at least). */
+static inline size_t gwh_grown_capacity(size_t need, size_t cap)
+{
+    if (need <= cap) return cap;
+    return need < 4096 ? 4096 : need + need / 4;
+}
+
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
@@ -143,6 +152,11 @@ struct LaunchExtra {
 struct FisherScratch {
     hipStream_t stream = nullptr;
     DevBuf params, coef, misc, deriv, gram, out, sig, raw;
+    size_t max_operand = (size_t)1 << 30; /* operand rows of one pass: points per pass follow from it */
+    size_t bytes() const
+    {
+        return params.cap + coef.cap + misc.cap + deriv.cap + gram.cap + out.cap + sig.cap + raw.cap;
+    }
 };
 
 struct Engine {
@@ -171,6 +185,11 @@ struct Engine {
     bool kt_on = false;
     int kt_every = 1;        /* time every kt_every-th launch */
     long long kt_seq = 0;    /* launches since timing started */
+    /* single calls: pointer-keyed cache mode, registered host arrays (handle -> arrays), the coalescing broker */
+    int cache_by_pointer = 0;
+    std::vector<std::pair<int, std::vector<const void *>>> host_regs;
+    int next_reg = 1;
+    int single_broker = -1;
     /* device address of the allocation-free entry point's scratch array on this engine's device
      * (hipGetSymbolAddress at initialisation; the array's name is not a device address in host code) */
     unsigned char *noalloc_scratch = nullptr;
@@ -181,10 +200,18 @@ int gwh_set_error(int code, const std::string &msg);
 int gwh_ensure_init();
 
 /* Create an event from host arrays. method: GWH_SIMPSONS / GWH_GAUSSLEG (weights used), log10F as in
- * Log_Likelihood_internal.  df_mode 0: the likelihood's f[N/2]-f[N/2-1]; 1: the Fisher's f[1]-f[0]. */
+ * Log_Likelihood_internal.  df_mode 0: the likelihood's conventions (Simpson step f[N/2]-f[N/2-1], the log10
+ * Jacobian on Gauss-Legendre weights only); 1: the SNR's and Fisher's (f[1]-f[0], the Jacobian under either rule). */
 int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const double *data, const double *psd,
                         int method, const double *weights, int log10F, int df_mode, uint64_t key, int *event);
 int gwh_choose_bpl(int N);
+
+/* Resident event for the host arrays of a single-call entry point (engine lock held): a registered or pointer-mode
+ * lookup by the arrays' addresses, else by a hash of their contents; created (and cached, LRU of 64) on a miss.
+ * data == NULL: zero data. df_mode as for gwh_event_create_ex. */
+extern "C" int gwh_single_call_event(const double *const *f, int n, int ndet, const int *dets,
+                                     const double *const *data, const double *const *psd, int method,
+                                     const double *weights, int log10F, int *event, int df_mode);
 
 /* the batched likelihood on device buffers (engine lock held by caller) */
 int gwh_loglike_launch(const int *d_events, int events0, int model, int precision, const gwh_params *d_params, int n,
@@ -196,6 +223,12 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
  * it runs; the event must stay alive).  The public entry points use the engine's scratch. */
 int gwh_fisher_batch_with(int event, int model, int basis, int precision, const gwh_params *params, int n, int dim,
                           double epsilon, double *out, double *sigma, FisherScratch &F);
+
+/* Device bytes scratch F holds after gwh_fisher_batch_with of these n points with at most max_operand bytes of
+ * operand rows per pass (its buffers grow as gwh_grown_capacity says; 0 when the arguments are invalid, the
+ * call itself reports why). */
+size_t gwh_fisher_batch_bytes(int event, int model, int basis, int precision, const gwh_params *params, int n,
+                              size_t max_operand, const FisherScratch &F, bool want_sigma);
 
 /* per-instance setup on device buffers */
 int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_params *d_params, int n, InstCoef *d_coef,

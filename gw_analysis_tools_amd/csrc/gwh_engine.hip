@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -72,7 +73,7 @@ int DevBuf::ensure(size_t bytes)
         p = nullptr;
         cap = 0;
     }
-    size_t nb = bytes < 4096 ? 4096 : bytes + bytes / 4;
+    size_t nb = gwh_grown_capacity(bytes, 0);
     if (hipMalloc(&p, nb) != hipSuccess) {
         p = nullptr;
         return gwh_set_error(GWH_ERR_NOMEM, "hipMalloc of " + std::to_string(nb) + " bytes failed");
@@ -90,22 +91,31 @@ void DevBuf::release()
 
 uint64_t gwh_hash_bytes(const void *p, size_t n, uint64_t seed)
 {
+    /* four independent multiply-xorshift chains over interleaved words (the single chain's multiply latency
+       bounded it to ~3 GB/s; content-keyed single calls hash every array on every call), folded at the end */
     const unsigned char *b = (const unsigned char *)p;
-    uint64_t h = seed ^ 0x9e3779b97f4a7c15ULL;
+    uint64_t h[4] = {seed ^ 0x9e3779b97f4a7c15ULL, seed ^ 0xc2b2ae3d27d4eb4fULL, seed ^ 0x165667b19e3779f9ULL,
+                     seed ^ 0x27d4eb2f165667c5ULL};
     size_t i = 0;
-    for (; i + 8 <= n; i += 8) {
-        uint64_t w;
-        memcpy(&w, b + i, 8);
-        h ^= w;
-        h *= 0xff51afd7ed558ccdULL;
-        h ^= h >> 29;
+    for (; i + 32 <= n; i += 32)
+        for (int k = 0; k < 4; k++) {
+            uint64_t w;
+            memcpy(&w, b + i + 8 * k, 8);
+            h[k] ^= w;
+            h[k] *= 0xff51afd7ed558ccdULL;
+            h[k] ^= h[k] >> 29;
+        }
+    uint64_t r = h[0];
+    for (int k = 1; k < 4; k++) {
+        r ^= h[k] + 0x9e3779b97f4a7c15ULL + (r << 6) + (r >> 2);
+        r *= 0xff51afd7ed558ccdULL;
     }
     for (; i < n; i++) {
-        h ^= b[i];
-        h *= 0x100000001b3ULL;
+        r ^= b[i];
+        r *= 0x100000001b3ULL;
     }
-    h ^= h >> 31;
-    return h;
+    r ^= r >> 31;
+    return r;
 }
 
 /* natural cubic spline second derivatives / 2 (the gsl_interp_cspline system), host side */
@@ -207,21 +217,24 @@ __global__ void gwh_bin_powers_kernel(const double *__restrict__ f, int N, doubl
     out[4 * stride + k] = b.f34;
 }
 
+/* weights folded with 4/S: Gauss-Legendre (the caller's) or Simpson by index parity; log10_jac multiplies by
+ * f ln 10 (a log10 frequency axis: the likelihood applies it to Gauss-Legendre weights only, mcmc_gw.cpp:821-856,
+ * calculate_snr_internal and calculate_fisher_elements under either rule, waveform_util.cpp:489-491,
+ * fisher.cpp:2726-2728) */
 __global__ void gwh_event_prep_kernel(const double *__restrict__ f, const double *__restrict__ psd,
                                       const double2 *__restrict__ data, const double *__restrict__ wts, int N, int D,
-                                      int method, int log10F, double dfac, double *__restrict__ wS,
+                                      int method, int log10_jac, double dfac, double *__restrict__ wS,
                                       double2 *__restrict__ dwS)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int d = blockIdx.y;
     if (k >= N || d >= D) return;
     double w;
-    if (method == GWH_GAUSSLEG) {
+    if (method == GWH_GAUSSLEG)
         w = 4.0 * wts[k];
-        if (log10F) w *= f[k] * 2.302585092994045684017991454684364208;
-    } else {
+    else
         w = 4.0 * dfac * simpson_w(k, N);
-    }
+    if (log10_jac) w *= f[k] * 2.302585092994045684017991454684364208;
     double s = w / psd[(size_t)d * N + k];
     wS[(size_t)d * N + k] = s;
     double2 x = data[(size_t)d * N + k];
@@ -287,9 +300,10 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     if (weights) GWH_HIP(hipMemcpyAsync(E.wts.p, weights, (size_t)N * 8, hipMemcpyHostToDevice, E.stream));
     double dfac = (df_mode == 0) ? (f[N / 2] - f[N / 2 - 1]) / 3.0 : (f[1] - f[0]) / 3.0;
     dim3 grid((N + GWH_TPB - 1) / GWH_TPB, ndet);
+    const int log10_jac = log10F && (method == GWH_GAUSSLEG || df_mode == 1);
     hipLaunchKernelGGL(gwh_event_prep_kernel, grid, dim3(GWH_TPB), 0, E.stream, (const double *)(blk + off_f),
                        E.psd.as<const double>(), E.data.as<const double2>(), E.wts.as<const double>(), N, ndet, method,
-                       log10F, dfac, (double *)(blk + off_wS), (double2 *)(blk + off_dwS));
+                       log10_jac, dfac, (double *)(blk + off_wS), (double2 *)(blk + off_dwS));
     GWH_HIP(hipGetLastError());
     hipLaunchKernelGGL(gwh_chunk_fmin_kernel, dim3(nchunk), dim3(GWH_TPB), 0, E.stream,
                        (const double *)(blk + off_f), N, (double *)(blk + off_cm));
@@ -818,7 +832,7 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
     {
         const int ev = W.inst_event ? W.inst_event[inst] : W.event0;
         const EventDesc &E = events[ev];
-        if (E.bpl != BPL) return;           /* the instance's event is tiled by another launch */
+        if (E.bpl != BPL) return;         /* the instance's event is tiled by another launch */
         if (tile >= active[inst]) return; /* above 0.2/M on an ascending grid: h = 0, nothing to add */
         const int w = inst * W.tiles0 + tile;
         const InstCoef &C = coefs[inst];
@@ -832,87 +846,67 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
         for (int j = 1; j < BPL; j++)
             if (tile * BPL + j < E.nchunk) fmin_tile = fmin(fmin_tile, gwh_g(E.chunk_fmin)[tile * BPL + j]);
         if (!(fmin_tile > C.fcut)) {
-            if (PREC == GWH_FP64) {
 #pragma unroll
-                for (int j = 0; j < BPL; j++) {
-                    int k = tile * GWH_TPB * BPL + j * GWH_TPB + threadIdx.x;
-                    if (k >= N) break;
-                    double f = gwh_g(E.f)[k];
-                    if (f > C.fcut) continue;
-                    const BinPow bp = gwh_load_bin(E, k);
-                    if (MODEL == GWH_IMRPHENOMD) {
-                        double A, phi, u, lx;
-                        gwh_phenomd_fp64(C, f, bp, A, phi, u, lx);
-#pragma unroll
-                        for (int d = 0; d < GWH_MAXDET; d++) {
-                            if (d >= D) break;
-                            double s, c;
-                            sincos(phi + C.det[d][3] * f, &s, &c);
-                            double ws = gwh_g(E.wS)[(size_t)d * N + k];
-                            double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
-                            acc[3 * d] += A * A * ws;
-                            acc[3 * d + 1] += A * (c * dw.x - s * dw.y);
-                            acc[3 * d + 2] += A * (c * dw.y + s * dw.x);
-                        }
-                    } else {
-                        double A, Phi, hpr, hpi, hcr, hci;
-                        gwh_pv2_fp64(C, f, bp, A, Phi, hpr, hpi, hcr, hci);
-#pragma unroll
-                        for (int d = 0; d < GWH_MAXDET; d++) {
-                            if (d >= D) break;
-                            double Gr = C.det[d][0] * hpr + C.det[d][1] * hcr;
-                            double Gi = C.det[d][0] * hpi + C.det[d][1] * hci;
-                            double s, c;
-                            sincos(Phi + C.det[d][3] * f, &s, &c);
-                            double ws = gwh_g(E.wS)[(size_t)d * N + k];
-                            double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
-                            acc[3 * d] += A * A * (Gr * Gr + Gi * Gi) * ws;
-                            acc[3 * d + 1] += A * (dw.x * (Gr * c + Gi * s) - dw.y * (Gr * s - Gi * c));
-                        }
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < BPL; j++) {
-                    int k = tile * GWH_TPB * BPL + j * GWH_TPB + threadIdx.x;
-                    if (k >= N) break;
-                    double f = gwh_g(E.f)[k];
-                    if (f > C.fcut) continue;
-                    const BinPow bp = gwh_load_bin(E, k);
-                    if (MODEL == GWH_IMRPHENOMD) {
-                        double a, phi, u, lx;
+            for (int j = 0; j < BPL; j++) {
+                /* Lanes past the end of the grid or above 0.2/M evaluate the last grid bin with a zero amplitude
+                 * (and phase), so the accumulators never sit under divergent control flow; a wave without any
+                 * bin at or below the cutoff skips the step on a uniform branch. */
+                const int k0 = tile * GWH_TPB * BPL + j * GWH_TPB + threadIdx.x;
+                const bool in = k0 < N;
+                const int k = in ? k0 : N - 1;
+                const double f = gwh_g(E.f)[k];
+                const bool live = in && f <= C.fcut;
+                if (!__ballot(live)) continue;
+                const BinPow bp = gwh_load_bin(E, k);
+                if (MODEL == GWH_IMRPHENOMD) {
+                    double a, phi, u, lx;
+                    if (PREC == GWH_FP64)
+                        gwh_phenomd_fp64(C, f, bp, a, phi, u, lx);
+                    else
                         gwh_phenomd_mixed(C, f, bp, a, phi, u, lx);
+                    a = live ? a : 0.0;
+                    phi = live ? phi : 0.0;
 #pragma unroll
-                        for (int d = 0; d < GWH_MAXDET; d++) {
-                            if (d >= D) break;
+                    for (int d = 0; d < GWH_MAXDET; d++) {
+                        if (d >= D) break;
+                        double s, c;
+                        if (PREC == GWH_FP64) {
+                            gwh_sincos_cw<15>(phi + C.det[d][3] * f, s, c);
+                        } else {
                             float sf, cf;
                             sincos_turns(phi + C.det[d][3] * f, sf, cf);
-                            const double s = sf, c = cf;
-                            const double ws = gwh_g(E.wS)[(size_t)d * N + k];
-                            const double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
-                            acc[3 * d] += a * a * ws;
-                            acc[3 * d + 1] += a * (c * dw.x - s * dw.y);
-                            acc[3 * d + 2] += a * (c * dw.y + s * dw.x);
+                            s = sf;
+                            c = cf;
                         }
+                        const double ws = gwh_g(E.wS)[(size_t)d * N + k];
+                        const double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
+                        acc[3 * d] += a * a * ws;
+                        acc[3 * d + 1] += a * (c * dw.x - s * dw.y);
+                        acc[3 * d + 2] += a * (c * dw.y + s * dw.x);
+                    }
+                } else {
+                    double A, Phi, hpr, hpi, hcr, hci;
+                    if (PREC == GWH_FP64) {
+                        gwh_pv2_fp64(C, f, bp, A, Phi, hpr, hpi, hcr, hci);
                     } else {
-                        double a, phi, u, lx;
-                        gwh_phenomd_mixed(C, f, bp, a, phi, u, lx);
-                        /* precession angles in FP64, sin/cos of the Euler angle in FP32 */
-                        double v = u * 1.4645918875615231;
-                        double logw = C.lgPiM + bp.lf; /* log(pi M f) */
-                        double x = v * v;
-                        double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) / v;
-                        double sv = C.SP / (L + C.SL);
-                        double cbd = rsqrt_mixed(1.0 + sv * sv);
-                        double cbhd = sqrt(0.5 * (1.0 + cbd)), sbhd = sqrt(0.5 * (1.0 - cbd));
-                        double c2d = cbhd * cbhd, s2d = sbhd * sbhd;
-                        double d2[5] = {s2d * s2d, 2. * cbhd * s2d * sbhd, GWH_SQRT_6 * s2d * c2d, 2. * c2d * cbhd * sbhd,
-                                        c2d * c2d};
-                        double alpha = gwh_euler_eval(C.a_c, v, logw) + C.alpha_add;
-                        double eps = gwh_euler_eval(C.e_c, v, logw) + C.eps_add;
-                        float saf, caf;
-                        sincos_turns(alpha, saf, caf);
-                        double sa = saf, ca = caf;
+                        double u, lx;
+                        gwh_phenomd_mixed(C, f, bp, A, Phi, u, lx);
+                        /* precession angles and the sin/cos of the Euler angle alpha in FP64 (alpha's rounding
+                         * would enter |h|^2 directly); the detector phases below take the FP32 sin/cos */
+                        const double v = u * 1.4645918875615231;
+                        const double logw = C.lgPiM + bp.lf; /* log(pi M f) */
+                        const double x = v * v;
+                        const double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) / v;
+                        const double sv = C.SP / (L + C.SL);
+                        const double cbd = rsqrt_mixed(1.0 + sv * sv);
+                        const double cbhd = sqrt(0.5 * (1.0 + cbd)), sbhd = sqrt(0.5 * (1.0 - cbd));
+                        const double c2d = cbhd * cbhd, s2d = sbhd * sbhd;
+                        const double d2[5] = {s2d * s2d, 2. * cbhd * s2d * sbhd, GWH_SQRT_6 * s2d * c2d,
+                                              2. * c2d * cbhd * sbhd, c2d * c2d};
+                        const double alpha = gwh_euler_eval(C.a_c, v, logw) + C.alpha_add;
+                        const double eps = gwh_euler_eval(C.e_c, v, logw) + C.eps_add;
+                        double sa, ca;
+                        gwh_sincos_cw<11>(alpha, sa, ca);
                         double er[5], ei[5];
                         er[2] = 1.;
                         ei[2] = 0.;
@@ -924,32 +918,41 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                         ei[4] = 2. * ca * sa;
                         er[0] = er[4];
                         ei[0] = -ei[4];
-                        double hpr = 0, hpi = 0, hcr = 0, hci = 0;
+                        hpr = hpi = hcr = hci = 0;
 #pragma unroll
                         for (int m = -2; m <= 2; m++) {
-                            double dm2 = (m == -2) ? d2[4] : (m == -1) ? -d2[3] : (m == 0) ? d2[2] : (m == 1) ? -d2[1] : d2[0];
-                            double y = C.ylm[m + 2];
-                            double t2r = er[-m + 2] * dm2 * y, t2i = ei[-m + 2] * dm2 * y;
-                            double tmr = er[m + 2] * d2[m + 2] * y, tmi = ei[m + 2] * d2[m + 2] * y;
+                            const double dm2 =
+                                (m == -2) ? d2[4] : (m == -1) ? -d2[3] : (m == 0) ? d2[2] : (m == 1) ? -d2[1] : d2[0];
+                            const double y = C.ylm[m + 2];
+                            const double t2r = er[-m + 2] * dm2 * y, t2i = ei[-m + 2] * dm2 * y;
+                            const double tmr = er[m + 2] * d2[m + 2] * y, tmi = ei[m + 2] * d2[m + 2] * y;
                             hpr += t2r + tmr;
                             hpi += t2i + tmi;
                             hcr -= t2i - tmi;
                             hci += t2r - tmr;
                         }
-                        double Phi = phi + 2. * eps + C.t_corr_2pi * f;
+                        Phi = Phi + 2. * eps + C.t_corr_2pi * f;
+                    }
+                    A = live ? A : 0.0;
+                    Phi = live ? Phi : 0.0;
 #pragma unroll
-                        for (int d = 0; d < GWH_MAXDET; d++) {
-                            if (d >= D) break;
-                            double P = C.det[d][0], Q = C.det[d][1];
-                            double Gr = P * hpr + Q * hcr, Gi = P * hpi + Q * hci;
+                    for (int d = 0; d < GWH_MAXDET; d++) {
+                        if (d >= D) break;
+                        const double P = C.det[d][0], Q = C.det[d][1];
+                        const double Gr = P * hpr + Q * hcr, Gi = P * hpi + Q * hci;
+                        double s, c;
+                        if (PREC == GWH_FP64) {
+                            gwh_sincos_cw<15>(Phi + C.det[d][3] * f, s, c);
+                        } else {
                             float sf, cf;
                             sincos_turns(Phi + C.det[d][3] * f, sf, cf);
-                            const double s = sf, c = cf;
-                            const double ws = gwh_g(E.wS)[(size_t)d * N + k];
-                            const double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
-                            acc[3 * d] += a * a * (Gr * Gr + Gi * Gi) * ws;
-                            acc[3 * d + 1] += a * (dw.x * (Gr * c + Gi * s) - dw.y * (Gr * s - Gi * c));
+                            s = sf;
+                            c = cf;
                         }
+                        const double ws = gwh_g(E.wS)[(size_t)d * N + k];
+                        const double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
+                        acc[3 * d] += A * A * (Gr * Gr + Gi * Gi) * ws;
+                        acc[3 * d + 1] += A * (dw.x * (Gr * c + Gi * s) - dw.y * (Gr * s - Gi * c));
                     }
                 }
             }
@@ -1312,28 +1315,32 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     return GWH_OK;
 }
 
-/* bins per IMRPhenomD phase region of every instance (for the algorithmic-work figure of benchmarks):
- * counts[4 i + 0..3] = inspiral (f < f1_phase), intermediate, merger-ringdown (f2_phase < f <= 0.2/M), zero */
+/* bins of every instance by IMRPhenomD region (for the algorithmic-work figure of benchmarks):
+ * counts[GWH_NREGION_COUNTS i + 3 a + p] for the non-zero bins (f <= 0.2/M) in amplitude region a (0: f < f1,
+ * 1: f1 <= f <= f3, 2: f > f3) and phase region p (0: f < f1_phase, 1: up to f2_phase, 2: above);
+ * counts[GWH_NREGION_COUNTS i + 9] = bins above 0.2/M */
 __global__ void gwh_region_count_kernel(const InstCoef *__restrict__ coefs, const EventDesc *__restrict__ events,
                                         int event0, int *__restrict__ counts)
 {
-    __shared__ int s[4][GWH_TPB];
+    __shared__ int s[GWH_NREGION_COUNTS][GWH_TPB];
     const InstCoef &C = coefs[blockIdx.x];
     const EventDesc &E = events[event0];
-    int c[4] = {0, 0, 0, 0};
+    int c[GWH_NREGION_COUNTS];
+    for (int j = 0; j < GWH_NREGION_COUNTS; j++) c[j] = 0;
     for (int k = threadIdx.x; k < E.N; k += GWH_TPB) {
-        double f = E.f[k];
-        int r = (f > C.fcut) ? 3 : (f < C.f1p) ? 0 : (f > C.f2p) ? 2 : 1;
-        c[r]++;
+        const double f = E.f[k];
+        const int a = (f < C.f1) ? 0 : (f > C.f3) ? 2 : 1;
+        const int p = (f < C.f1p) ? 0 : (f > C.f2p) ? 2 : 1;
+        c[(f > C.fcut) ? 9 : 3 * a + p]++;
     }
-    for (int j = 0; j < 4; j++) s[j][threadIdx.x] = c[j];
+    for (int j = 0; j < GWH_NREGION_COUNTS; j++) s[j][threadIdx.x] = c[j];
     __syncthreads();
     for (int w = GWH_TPB / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
-            for (int j = 0; j < 4; j++) s[j][threadIdx.x] += s[j][threadIdx.x + w];
+            for (int j = 0; j < GWH_NREGION_COUNTS; j++) s[j][threadIdx.x] += s[j][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < 4) counts[4 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+    if (threadIdx.x < GWH_NREGION_COUNTS) counts[GWH_NREGION_COUNTS * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1356,7 +1363,7 @@ __global__ void gwh_waveform_kernel(const double *__restrict__ f, int N, const I
             double A, phi, u, lx;
             gwh_phenomd_fp64(C, fk, bp, A, phi, u, lx);
             double s, c;
-            sincos(phi, &s, &c);
+            gwh_sincos_cw<15>(phi, s, c);
             double hr = A * c, hi = -A * s; /* h = A e^{-i phi} */
             hp = make_double2(hr * C.wf_a, hi * C.wf_a);
             hc = make_double2(hi * C.wf_b, -hr * C.wf_b); /* -i h cos(iota) */
@@ -1364,7 +1371,7 @@ __global__ void gwh_waveform_kernel(const double *__restrict__ f, int N, const I
             double A, Phi, hpr, hpi, hcr, hci;
             gwh_pv2_fp64(C, fk, bp, A, Phi, hpr, hpi, hcr, hci);
             double s, c;
-            sincos(Phi, &s, &c);
+            gwh_sincos_cw<15>(Phi, s, c);
             /* (A hpf) e^{-i Phi} */
             double pr = A * (hpr * c + hpi * s), pi = A * (hpi * c - hpr * s);
             double xr = A * (hcr * c + hci * s), xi = A * (hci * c - hcr * s);
@@ -1393,7 +1400,7 @@ __global__ void gwh_waveform_kernel(const double *__restrict__ f, int N, const I
         if (mode == 2) {
             /* time-of-flight phase e^{-2 pi i f dt} relative to the first detector (waveform_util.cpp:170-179) */
             double s, c;
-            sincos(-C.det[det][3] * fk, &s, &c);
+            gwh_sincos_cw<15>(-C.det[det][3] * fk, s, c);
             hp = make_double2(hp.x * c - hp.y * s, hp.x * s + hp.y * c);
         }
         out1[k] = hp;
@@ -1511,6 +1518,93 @@ int gwh_event_create(const double *frequencies, int length, int ndet, const int 
     Engine &E = gwh_engine();
     std::lock_guard<std::recursive_mutex> lk(E.mu);
     return gwh_event_create_ex(frequencies, length, ndet, detectors, data, psd, GWH_SIMPSONS, nullptr, 0, 0, 0, event);
+}
+
+/* drop the pointer-keyed cached events (their host arrays may change once no longer registered) */
+static int drop_pointer_keyed_events(Engine &E)
+{
+    for (int i = 0; i < GWH_MAX_EVENTS - 1; i++) {
+        Event &ev = E.events[i];
+        if (!ev.live || ev.key == 0 || !ev.ptr_key || ev.pins > 0) continue;
+        GWH_HIP(hipStreamSynchronize(E.stream));
+        GWH_HIP(hipFree(ev.block));
+        ev = Event();
+    }
+    return GWH_OK;
+}
+
+int gwh_set_single_call_cache(int mode)
+{
+    if (mode != GWH_CACHE_BY_CONTENT && mode != GWH_CACHE_BY_POINTER)
+        return gwh_set_error(GWH_ERR_INVALID, "cache mode must be GWH_CACHE_BY_CONTENT or GWH_CACHE_BY_POINTER");
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    if (E.cache_by_pointer && mode == GWH_CACHE_BY_CONTENT && (st = drop_pointer_keyed_events(E)) != GWH_OK) return st;
+    E.cache_by_pointer = (mode == GWH_CACHE_BY_POINTER);
+    return GWH_OK;
+}
+
+int gwh_single_call_stats(double *stats)
+{
+    if (!stats) return gwh_set_error(GWH_ERR_INVALID, "null stats");
+    Engine &E = gwh_engine();
+    int broker;
+    {
+        std::lock_guard<std::recursive_mutex> lk(E.mu);
+        broker = E.single_broker;
+    }
+    if (broker < 0) {
+        for (int i = 0; i < 9; i++) stats[i] = 0;
+        return GWH_OK;
+    }
+    return gwh_broker_stats(broker, stats);
+}
+
+int gwh_host_data_register(int num_detectors, const int *data_length, const double *const *frequencies,
+                           const double *const *data, const double *const *psd, const double *const *weights,
+                           int *handle)
+{
+    if (num_detectors < 1 || !data_length || !frequencies || !psd || !handle)
+        return gwh_set_error(GWH_ERR_INVALID, "null argument");
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    std::vector<const void *> ptrs;
+    for (int d = 0; d < num_detectors; d++) {
+        if (!frequencies[d] || !psd[d] || data_length[d] < 2) return gwh_set_error(GWH_ERR_INVALID, "bad arrays");
+        ptrs.push_back(frequencies[d]);
+        ptrs.push_back(psd[d]);
+        if (data && data[d]) ptrs.push_back(data[d]);
+        if (weights && weights[d]) ptrs.push_back(weights[d]);
+    }
+    *handle = E.next_reg++;
+    E.host_regs.emplace_back(*handle, ptrs);
+    return GWH_OK;
+}
+
+int gwh_host_data_release(int handle)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    for (size_t i = 0; i < E.host_regs.size(); i++)
+        if (E.host_regs[i].first == handle) {
+            E.host_regs.erase(E.host_regs.begin() + i);
+            return E.ready ? drop_pointer_keyed_events(E) : GWH_OK;
+        }
+    return gwh_set_error(GWH_ERR_INVALID, "unknown host data handle");
+}
+
+int gwh_event_create_weighted(const double *frequencies, int length, int ndet, const int *detectors,
+                              const double *data, const double *psd, int integration_method, const double *weights,
+                              int log10F, int *event)
+{
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    if (integration_method != GWH_SIMPSONS && integration_method != GWH_GAUSSLEG)
+        return gwh_set_error(GWH_ERR_INVALID, "integration method must be GWH_SIMPSONS or GWH_GAUSSLEG");
+    return gwh_event_create_ex(frequencies, length, ndet, detectors, data, psd, integration_method,
+                               integration_method == GWH_GAUSSLEG ? weights : nullptr, log10F ? 1 : 0, 0, 0, event);
 }
 
 int gwh_event_destroy(int event)
@@ -1886,34 +1980,63 @@ int gwh_fourier_detector_response(const double *frequencies, int length, double 
     return GWH_OK;
 }
 
-/* find or create a cached event for host arrays used by the single-call likelihoods */
-static int cached_event(const double *const *f, int n, int ndet, const int *dets, const double *const *data,
-                        const double *const *psd, int method, const double *weights, int log10F, int *event,
-                        int df_mode = 0)
+/* the host arrays of a call, all registered (gwh_host_data_register)? */
+static bool all_registered(const Engine &E, const std::vector<const void *> &ptrs)
+{
+    for (const void *p : ptrs) {
+        bool found = false;
+        for (auto &r : E.host_regs)
+            for (const void *q : r.second) found = found || (q == p);
+        if (!found) return false;
+    }
+    return true;
+}
+
+/* lk: the caller's hold on the engine lock, released while the arrays are hashed (content mode), so that
+ * concurrent callers hash in parallel; null: the lock stays held */
+static int single_call_event(const double *const *f, int n, int ndet, const int *dets, const double *const *data,
+                             const double *const *psd, int method, const double *weights, int log10F, int *event,
+                             int df_mode, std::unique_lock<std::recursive_mutex> *lk)
 {
     Engine &E = gwh_engine();
-    uint64_t key = 0x12345 + (uint64_t)df_mode;
-    key = gwh_hash_bytes(f[0], (size_t)n * 8, key);
+    std::vector<const void *> ptrs;
+    ptrs.push_back(f[0]);
     for (int d = 0; d < ndet; d++) {
-        key = gwh_hash_bytes(data[d], (size_t)n * 16, key);
-        key = gwh_hash_bytes(psd[d], (size_t)n * 8, key);
-        key = gwh_hash_bytes(&dets[d], sizeof(int), key);
+        if (data) ptrs.push_back(data[d]);
+        ptrs.push_back(psd[d]);
     }
+    if (weights) ptrs.push_back(weights);
+    /* pointer key: the caller keeps these arrays unchanged while they are cached (pointer mode) or registered */
+    const bool by_ptr = E.cache_by_pointer || (!E.host_regs.empty() && all_registered(E, ptrs));
+    uint64_t key = (by_ptr ? 0x9a7e0000ull : 0x12345ull) + (uint64_t)df_mode + (data ? 0 : 0x100);
+    if (by_ptr) {
+        key = gwh_hash_bytes(ptrs.data(), ptrs.size() * sizeof(void *), key);
+        key = gwh_hash_bytes(&n, sizeof(int), key);
+    } else {
+        if (lk) lk->unlock(); /* only the caller's arrays are read meanwhile */
+        key = gwh_hash_bytes(f[0], (size_t)n * 8, key);
+        for (int d = 0; d < ndet; d++) {
+            if (data) key = gwh_hash_bytes(data[d], (size_t)n * 16, key);
+            key = gwh_hash_bytes(psd[d], (size_t)n * 8, key);
+        }
+        if (weights) key = gwh_hash_bytes(weights, (size_t)n * 8, key);
+        if (lk) lk->lock();
+    }
+    for (int d = 0; d < ndet; d++) key = gwh_hash_bytes(&dets[d], sizeof(int), key);
     key = gwh_hash_bytes(&method, sizeof(int), key);
     key = gwh_hash_bytes(&log10F, sizeof(int), key);
-    if (weights) key = gwh_hash_bytes(weights, (size_t)n * 8, key);
     if (key == 0) key = 1;
     int live_cached = 0, oldest = -1;
     for (int i = 0; i < GWH_MAX_EVENTS - 1; i++) {
         Event &ev = E.events[i];
         if (!ev.live || ev.key == 0) continue;
-        if (ev.key == key && ev.N == n && ev.D == ndet) {
+        if (ev.key == key && ev.ptr_key == by_ptr && ev.N == n && ev.D == ndet) {
             ev.last_use = ++E.use_clock;
             *event = i;
             return GWH_OK;
         }
         live_cached++;
-        if (ev.pins > 0) continue; /* held by a step graph: never evicted */
+        if (ev.pins > 0) continue; /* held by a step graph or a pending request: never evicted */
         if (oldest < 0 || ev.last_use < E.events[oldest].last_use) oldest = i;
     }
     if (live_cached >= 64 && oldest >= 0) { /* LRU of implicit events */
@@ -1921,20 +2044,67 @@ static int cached_event(const double *const *f, int n, int ndet, const int *dets
         GWH_HIP(hipFree(E.events[oldest].block));
         E.events[oldest] = Event();
     }
-    std::vector<double> dat((size_t)ndet * n * 2), ps((size_t)ndet * n);
+    std::vector<double> dat((size_t)ndet * n * 2, 0.0), ps((size_t)ndet * n);
     for (int d = 0; d < ndet; d++) {
-        memcpy(&dat[(size_t)d * n * 2], data[d], (size_t)n * 16);
+        if (data) memcpy(&dat[(size_t)d * n * 2], data[d], (size_t)n * 16);
         memcpy(&ps[(size_t)d * n], psd[d], (size_t)n * 8);
     }
-    return gwh_event_create_ex(f[0], n, ndet, dets, dat.data(), ps.data(), method, weights, log10F, df_mode, key,
-                               event);
+    int rc = gwh_event_create_ex(f[0], n, ndet, dets, dat.data(), ps.data(), method, weights, log10F, df_mode, key,
+                                 event);
+    if (rc == GWH_OK) E.events[*event].ptr_key = by_ptr;
+    return rc;
+}
+
+int gwh_single_call_event(const double *const *f, int n, int ndet, const int *dets, const double *const *data,
+                          const double *const *psd, int method, const double *weights, int log10F, int *event,
+                          int df_mode)
+{
+    return single_call_event(f, n, ndet, dets, data, psd, method, weights, log10F, event, df_mode, nullptr);
+}
+
+static int cached_event(const double *const *f, int n, int ndet, const int *dets, const double *const *data,
+                        const double *const *psd, int method, const double *weights, int log10F, int *event,
+                        int df_mode = 0, std::unique_lock<std::recursive_mutex> *lk = nullptr)
+{
+    return single_call_event(f, n, ndet, dets, data, psd, method, weights, log10F, event, df_mode, lk);
+}
+
+/* One FP64 instance on a resident event for a single-call entry point (engine lock held on entry; it may be
+ * released on return, the caller touches no engine state afterwards).  With other single calls in flight the instance goes through the coalescing broker, which
+ * batches the concurrent calls of all threads into shared launches (the lock is released while waiting);
+ * alone it is launched directly.  The result is the same either way (an instance's result does not depend
+ * on its batch). */
+static std::atomic<int> g_single_calls{0};
+
+/* counts a single likelihood call in flight from entry (before the engine lock) to return */
+struct SingleCall {
+    bool alone;
+    SingleCall() : alone(g_single_calls.fetch_add(1) == 0) {}
+    ~SingleCall() { g_single_calls.fetch_sub(1); }
+};
+
+static int single_instance_loglike(Engine &E, std::unique_lock<std::recursive_mutex> &lk, const SingleCall &sc,
+                                   int ev, int model, const gwh_params &p, double *out)
+{
+    if (sc.alone) return gwh_loglike_batch(nullptr, ev, model, GWH_FP64, &p, 1, out);
+    if (E.single_broker < 0) {
+        int rc = gwh_broker_create((size_t)1 << 62, 4096, &E.single_broker);
+        if (rc != GWH_OK) return rc;
+    }
+    long long ticket = 0;
+    int rc = gwh_broker_submit_loglike(E.single_broker, nullptr, ev, model, GWH_FP64, &p, 1, out, &ticket);
+    if (rc != GWH_OK) return rc;
+    const int broker = E.single_broker;
+    lk.unlock(); /* the caller returns without the lock */
+    return gwh_broker_wait(broker, ticket, -1.0, nullptr);
 }
 
 double gwh_log_likelihood(const double *data, const double *psd, const double *frequencies, size_t length,
                           const gwh_params *params, int detector, int model)
 {
+    SingleCall sc;
     Engine &E = gwh_engine();
-    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    std::unique_lock<std::recursive_mutex> lk(E.mu);
     const double nan = std::nan("");
     if (gwh_ensure_init() != GWH_OK) return nan;
     int n = (int)length;
@@ -1945,9 +2115,9 @@ double gwh_log_likelihood(const double *data, const double *psd, const double *f
     }
     int ev;
     const double *fp[1] = {frequencies}, *dp[1] = {data}, *pp[1] = {psd};
-    if (cached_event(fp, n, 1, &detector, dp, pp, GWH_SIMPSONS, nullptr, 0, &ev) != GWH_OK) return nan;
+    if (cached_event(fp, n, 1, &detector, dp, pp, GWH_SIMPSONS, nullptr, 0, &ev, 0, &lk) != GWH_OK) return nan;
     double out = nan;
-    if (gwh_loglike_batch(nullptr, ev, model, GWH_FP64, params, 1, &out) != GWH_OK) return nan;
+    if (single_instance_loglike(E, lk, sc, ev, model, *params, &out) != GWH_OK) return nan;
     return out;
 }
 
@@ -1956,8 +2126,9 @@ double gwh_mcmc_likelihood_extrinsic(int save_waveform, const gwh_params *params
                                      const double *const *psd, const double *const *weights, int integration_method,
                                      int log10F, const int *detectors, int num_detectors, double T_eff)
 {
+    SingleCall sc;
     Engine &E = gwh_engine();
-    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    std::unique_lock<std::recursive_mutex> lk(E.mu);
     const double nan = std::nan("");
     if (gwh_ensure_init() != GWH_OK) return nan;
     if (!params || !data_length || !frequencies || !data || !psd || !detectors || num_detectors < 1) {
@@ -1971,7 +2142,8 @@ double gwh_mcmc_likelihood_extrinsic(int save_waveform, const gwh_params *params
     }
     int n = data_length[0];
     for (int d = 1; d < num_detectors; d++) {
-        if (data_length[d] != n || memcmp(frequencies[d], frequencies[0], (size_t)n * 8) != 0) {
+        if (data_length[d] != n ||
+            (frequencies[d] != frequencies[0] && memcmp(frequencies[d], frequencies[0], (size_t)n * 8) != 0)) {
             gwh_set_error(GWH_ERR_UNSUPPORTED, "all detectors must share one frequency grid");
             return nan;
         }
@@ -1982,12 +2154,13 @@ double gwh_mcmc_likelihood_extrinsic(int save_waveform, const gwh_params *params
     }
     int ev;
     if (cached_event(frequencies, n, num_detectors, detectors, data, psd, integration_method,
-                     (integration_method == GWH_GAUSSLEG && weights) ? weights[0] : nullptr, log10F, &ev) != GWH_OK)
+                     (integration_method == GWH_GAUSSLEG && weights) ? weights[0] : nullptr, log10F, &ev, 0,
+                     &lk) != GWH_OK)
         return nan;
     gwh_params p = *params;
     p.tc = T_eff - params->tc; /* mcmc_gw.cpp:2466-2468 */
     double out = nan;
-    if (gwh_loglike_batch(nullptr, ev, model, GWH_FP64, &p, 1, &out) != GWH_OK) return nan;
+    if (single_instance_loglike(E, lk, sc, ev, model, p, &out) != GWH_OK) return nan;
     return out;
 }
 
@@ -2143,7 +2316,7 @@ int gwh_work_counts(int event, int model, const gwh_params *params, int n, int *
     if (event < 0 || event >= GWH_MAX_EVENTS || !E.events[event].live) return gwh_set_error(GWH_ERR_INVALID, "unknown event");
     if ((rc = E.params.ensure((size_t)n * sizeof(gwh_params))) != GWH_OK) return rc;
     if ((rc = E.coef.ensure((size_t)n * sizeof(InstCoef))) != GWH_OK) return rc;
-    if ((rc = E.misc.ensure((size_t)n * 4 * sizeof(int))) != GWH_OK) return rc;
+    if ((rc = E.misc.ensure((size_t)n * GWH_NREGION_COUNTS * sizeof(int))) != GWH_OK) return rc;
     GWH_HIP(hipMemcpyAsync(E.params.p, params, (size_t)n * sizeof(gwh_params), hipMemcpyHostToDevice, E.stream));
     if ((rc = gwh_setup_launch(nullptr, event, model, E.params.as<const gwh_params>(), n, E.coef.as<InstCoef>(),
                                E.stream)) != GWH_OK)
@@ -2151,7 +2324,8 @@ int gwh_work_counts(int event, int model, const gwh_params *params, int n, int *
     hipLaunchKernelGGL(gwh_region_count_kernel, dim3(n), dim3(GWH_TPB), 0, E.stream, E.coef.as<const InstCoef>(),
                        (const EventDesc *)E.d_events, event, E.misc.as<int>());
     GWH_HIP(hipGetLastError());
-    GWH_HIP(hipMemcpyAsync(counts, E.misc.p, (size_t)n * 4 * sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    GWH_HIP(hipMemcpyAsync(counts, E.misc.p, (size_t)n * GWH_NREGION_COUNTS * sizeof(int), hipMemcpyDeviceToHost,
+                           E.stream));
     GWH_HIP(hipStreamSynchronize(E.stream));
     return GWH_OK;
 }

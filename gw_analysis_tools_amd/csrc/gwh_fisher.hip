@@ -145,7 +145,15 @@ struct PointStencil {
     double c[GWH_FROWS][GWH_MAXSTEN];
 };
 
-static int build_stencil(const gwh_params &p, int basis, int var, int nmod, int order, double eps, int det,
+/* Step of one basis coordinate v: step > 0 is the absolute epsilon of fisher.cpp:158; step < 0 is a
+ * step relative to the coordinate, |step| * |v| (|step| where v is exactly 0). */
+static double coordinate_step(double step, double v)
+{
+    if (step > 0) return step;
+    return v != 0 ? -step * fabs(v) : -step;
+}
+
+static int build_stencil(const gwh_params &p, int basis, int var, int nmod, int order, double step, int det,
                          int ref_det, std::vector<gwh_params> &inst, std::vector<unsigned char> &no_tof,
                          PointStencil &ps)
 {
@@ -158,6 +166,7 @@ static int build_stencil(const gwh_params &p, int basis, int var, int nmod, int 
     for (int i = 0; i < GWH_FROWS; i++)
         for (int s = 0; s < GWH_MAXSTEN; s++) ps.c[i][s] = 0;
     for (int i = 0; i < dim; i++) {
+        double eps = coordinate_step(step, v[i]);
         bool edge = (i == 8 && v[i] > .25 - eps); /* eta at the equal-mass boundary: one-sided */
         double fac = lf[i] ? v[i] : 1.0;
         double off[4];
@@ -275,7 +284,7 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
                         if (dd >= a.d_count) break;
                         int d = a.d_first + dd;
                         double sn, cs;
-                        sincos(phi + (tof ? Ci.det[d][3] : 0.0) * f, &sn, &cs);
+                        gwh_sincos_cw<15>(phi + (tof ? Ci.det[d][3] : 0.0) * f, sn, cs);
                         double er = A * cs, ei = -A * sn; /* A K e^{-i(phi + 2 pi dt f)} */
                         dr[dd] += c * (er * Ci.det[d][0] - ei * Ci.det[d][1]);
                         di[dd] += c * (er * Ci.det[d][1] + ei * Ci.det[d][0]);
@@ -289,7 +298,7 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
                         double Gr = Ci.det[d][0] * hpr + Ci.det[d][1] * hcr;
                         double Gi = Ci.det[d][0] * hpi + Ci.det[d][1] * hci;
                         double sn, cs;
-                        sincos(Phi + (tof ? Ci.det[d][3] : 0.0) * f, &sn, &cs);
+                        gwh_sincos_cw<15>(Phi + (tof ? Ci.det[d][3] : 0.0) * f, sn, cs);
                         dr[dd] += c * A * (Gr * cs + Gi * sn);
                         di[dd] += c * A * (Gi * cs - Gr * sn);
                     }
@@ -449,9 +458,13 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
                       double *sigma, FisherScratch &F, double *raw = nullptr)
 {
     Engine &E = gwh_engine();
+    if (!(eps != 0) || !std::isfinite(eps))
+        return gwh_set_error(GWH_ERR_INVALID, "the Fisher step must be finite and nonzero");
     Event ev;
     {
-        std::lock_guard<std::recursive_mutex> lk(E.mu); /* snapshot: the event stays alive meanwhile */
+        /* copy of the descriptor; the device block stays valid because broker requests pin the
+           event (gwh_event_destroy then refuses) and direct callers hold it for the call */
+        std::lock_guard<std::recursive_mutex> lk(E.mu);
         ev = E.events[event];
     }
     int nmod, rc0;
@@ -466,10 +479,10 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
     int per_point = dim * S + (sky ? 1 : 0);
     const int N = ev.N;
     const size_t nrows = (size_t)(sky ? 1 : d_count) * N * 2;
-    const size_t row_bytes = GWH_FROWS * ((precision == GWH_FP64) ? 8 : 4);
-    /* points per pass: bounded operand buffer (~1 GiB) */
+    const size_t row_bytes = GWH_FROWS * ((precision == GWH_FISHER_GRAM_FP32) ? 4 : 8);
+    /* points per pass: bounded operand buffer (F.max_operand, 1 GiB unless a caller's budget says less) */
     size_t per_pt_bytes = nrows * row_bytes;
-    int chunk = (int)std::max<size_t>(1, ((size_t)1 << 30) / per_pt_bytes);
+    int chunk = (int)std::max<size_t>(1, F.max_operand / per_pt_bytes);
     if (chunk > n) chunk = n;
     if (chunk > 65535) chunk = 65535;
     int nblk = (int)((nrows + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS);
@@ -537,7 +550,7 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
             GWH_HIP(hipStreamSynchronize(F.stream));
             return GWH_OK;
         }
-        if (precision == GWH_FP64) {
+        if (precision != GWH_FISHER_GRAM_FP32) {
             if (!pv2)
                 hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, double>), g, dim3(GWH_TPB), 0, F.stream,
                                    a, evp, event, F.deriv.as<double>());
@@ -596,13 +609,45 @@ int gwh_fisher_batch_with(int event, int model, int basis, int precision, const 
     if ((rc = fisher_mods(model, params, n, &nmod)) != GWH_OK) return rc;
     if (basis_dim(basis) < 0 || basis_dim(basis) + nmod != dim)
         return gwh_set_error(GWH_ERR_INVALID, "dimension does not match the parameter basis (and ppE terms)");
-    if (precision != GWH_FP64 && precision != GWH_MIXED) return gwh_set_error(GWH_ERR_INVALID, "bad precision");
+    if (precision != GWH_FP64 && precision != GWH_MIXED && precision != GWH_FISHER_GRAM_FP32)
+        return gwh_set_error(GWH_ERR_INVALID, "bad precision");
     if ((basis == GWH_BASIS_PV2_FULL || basis == GWH_BASIS_PV2_MCMC) != gwh_is_pv2(model))
         return gwh_set_error(GWH_ERR_INVALID, "basis does not belong to the model");
     /* the event's weights use the likelihood's df = f[N/2]-f[N/2-1]; the Fisher uses f[1]-f[0] */
     double scale = ev.df_fisher / ev.df_like;
     return fisher_run(event, model, basis, precision, params, n, 2, epsilon, 0, ev.D, ev.det[0], ev.det[0], scale, out,
                       sigma, F);
+}
+
+size_t gwh_fisher_batch_bytes(int event, int model, int basis, int precision, const gwh_params *params, int n,
+                              size_t max_operand, const FisherScratch &F, bool want_sigma)
+{
+    Engine &E = gwh_engine();
+    Event ev;
+    {
+        std::lock_guard<std::recursive_mutex> lk(E.mu);
+        if (event < 0 || event >= GWH_MAX_EVENTS || !E.events[event].live) return 0;
+        ev = E.events[event];
+    }
+    int nmod;
+    if (n <= 0 || !params || fisher_mods(model, params, n, &nmod) != GWH_OK || basis_dim(basis) < 0) return 0;
+    /* the buffers fisher_run ensures (order 2) */
+    const int dim = basis_dim(basis) + nmod;
+    const bool sky = (basis == GWH_BASIS_PHENOMD_SKYAVG);
+    const size_t per_point = (size_t)dim * 2 + (sky ? 1 : 0);
+    const size_t nrows = (size_t)(sky ? 1 : ev.D) * ev.N * 2;
+    const size_t per_pt_bytes = nrows * GWH_FROWS * ((precision == GWH_FISHER_GRAM_FP32) ? 4 : 8);
+    size_t chunk = std::max<size_t>(1, max_operand / per_pt_bytes);
+    chunk = std::min<size_t>(chunk, std::min(n, 65535));
+    const size_t nblk = (nrows + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS;
+    const size_t ni = chunk * per_point;
+    return gwh_grown_capacity(chunk * per_pt_bytes, F.deriv.cap) +
+           gwh_grown_capacity(chunk * nblk * 256 * 8, F.gram.cap) +
+           gwh_grown_capacity((size_t)n * dim * dim * 8, F.out.cap) +
+           (want_sigma ? gwh_grown_capacity((size_t)n * dim * 8, F.sig.cap) : F.sig.cap) +
+           gwh_grown_capacity(ni * sizeof(gwh_params), F.params.cap) +
+           gwh_grown_capacity(ni * sizeof(InstCoef), F.coef.cap) +
+           gwh_grown_capacity(chunk * GWH_FROWS * GWH_MAXSTEN * 8 + ni + 256, F.misc.cap) + F.raw.cap;
 }
 
 extern "C" {
@@ -712,18 +757,23 @@ static int single_point(const double *frequencies, int length, int model, int ba
     int dets[2] = {reference_detector, detector};
     int nd = (detector != reference_detector) ? 2 : 1;
     if (nd == 1) dets[0] = detector;
-    std::vector<double> zeros((size_t)2 * length * nd, 0.0), ps((size_t)length * nd, 1.0);
-    if (psd)
-        for (int d = 0; d < nd; d++) memcpy(&ps[(size_t)d * length], psd, (size_t)length * 8);
+    /* the PSD of the Fisher detector on both rows (the reference detector's row only carries the arrival-time
+     * reference); derivatives alone need no PSD: a unit one */
+    std::vector<double> unit;
+    const double *ps = psd;
+    if (!ps) {
+        unit.assign((size_t)length, 1.0);
+        ps = unit.data();
+    }
+    const double *fp[2] = {frequencies, frequencies}, *pp[2] = {ps, ps};
     int ev;
-    if ((rc = gwh_event_create_ex(frequencies, length, nd, dets, zeros.data(), ps.data(), GWH_SIMPSONS, nullptr, 0, 1,
-                                  0, &ev)) != GWH_OK)
+    /* a cached event (no allocation after the first call on these arrays) */
+    if ((rc = gwh_single_call_event(fp, length, nd, dets, nullptr, pp, GWH_SIMPSONS, nullptr, 0, &ev, 1)) != GWH_OK)
         return rc;
+    E.events[ev].pins++;
     rc = fisher_run(ev, model, basis, GWH_FP64, params, 1, order, epsilon, nd - 1, 1, detector, reference_detector,
                     1.0, out, nullptr, E.fisher, deriv);
-    (void)hipStreamSynchronize(E.stream);
-    (void)hipFree(E.events[ev].block);
-    E.events[ev] = Event();
+    E.events[ev].pins--;
     return rc;
 }
 

@@ -90,6 +90,57 @@ int basis_of(const char *method, int dim, int sky_average)
 /* failures found here are recorded like the engine's own (gwh_last_error) */
 int gwh_report(int code, const char *msg) { return gwh_set_error(code, msg); }
 
+/* MCMC_modification_struct_py's handle: the modification lists a sampler's parameter vector carries */
+const uint64_t kModMagic = 0x6777617470793032ull; /* "gwatpy02" */
+struct PyMod {
+    uint64_t magic;
+    int ppe_nmod = 0;
+    std::vector<double> bppe;
+    std::vector<int> phii, sigmai, betai, alphai;
+    bool ns = false;
+};
+std::unordered_set<void *> g_mods; /* guarded by g_handles_mu */
+
+/* the gIMR index lists in basis order (fisher.cpp:2128-2158) as table codes; more than 16: no basis (-1) */
+void fill_gimr_list(gwh_params &p, const std::vector<int> &phii, const std::vector<int> &sigmai,
+                    const std::vector<int> &betai, const std::vector<int> &alphai)
+{
+    auto list = [&](const std::vector<int> &idx, int lo, int hi, int code0) {
+        for (int i : idx) {
+            if (p.gimr_nlist < GWH_GIMR_LIST_MAX) p.gimr_list[p.gimr_nlist] = (int8_t)((i < lo || i > hi) ? -1 : code0 + i);
+            p.gimr_nlist++;
+        }
+    };
+    list(phii, -4, 9, GWH_GIMR_CODE_PHI(0));
+    list(sigmai, 2, 4, GWH_GIMR_CODE_SIGMA(0));
+    list(betai, 2, 3, GWH_GIMR_CODE_BETA(0));
+    list(alphai, 2, 5, GWH_GIMR_CODE_ALPHA(0));
+    if (p.gimr_nlist > GWH_GIMR_LIST_MAX) p.gimr_nlist = -1;
+}
+
+/* the modification parameters after a GR basis block: the ppE betas, or the listed gIMR deltas (the first entry
+ * of an index is the one the waveform reads, as check_list_id finds it) */
+void set_mod_params(gwh_params &p, const double *v, int nmod, bool ppe, bool gimr)
+{
+    for (int i = 0; i < nmod && ppe; i++) p.ppe_beta[i] = v[i];
+    for (int i = 0; i < nmod && gimr; i++) {
+        const int c = p.gimr_list[i];
+        bool first = c >= 0;
+        for (int j = 0; j < i; j++) first &= (p.gimr_list[j] != c);
+        if (!first) continue;
+        if (c < GWH_GIMR_CODE_SIGMA(2)) p.gimr_dphi[c] = v[i];
+        else if (c < GWH_GIMR_CODE_BETA(2)) p.gimr_dsigma[c - GWH_GIMR_CODE_SIGMA(2)] = v[i];
+        else if (c < GWH_GIMR_CODE_ALPHA(2)) p.gimr_dbeta[c - GWH_GIMR_CODE_BETA(2)] = v[i];
+        else p.gimr_dalpha[c - GWH_GIMR_CODE_ALPHA(2)] = v[i];
+    }
+}
+
+/* the GR family of a method string: IMRPhenomD or IMRPhenomPv2 (ppE, gIMR and theory variants alike) */
+std::string gr_family(const char *method)
+{
+    return (method && strstr(method, "IMRPhenomPv2")) ? "IMRPhenomPv2" : "IMRPhenomD";
+}
+
 } // namespace
 
 extern "C" {
@@ -250,14 +301,12 @@ int fourier_detector_response_py(double *frequencies, int length, double *respon
  * Hanford, Livingston, Virgo, replaced only where the suffix starting at letter i is exactly one of
  * "H", "L", "V", "K", "C" (so in practice only the last letter is ever matched).  The wrapper's T is a
  * pointer difference (mcmc_gw.cpp:2466); the engine uses T_eff = 0, i.e. t_c enters as -tc. */
-double MCMC_likelihood_extrinsic_py(bool save_waveform, void *parameters, char *generation_method, int *data_length,
-                                    double *frequencies, double *dataREAL, double *dataIMAG, double *psd,
-                                    double *weights, char *integration_method, bool log10F, char *detectors,
-                                    int num_detectors)
+/* the body shared by the two wrappers: flattened arrays, detector letters, the engine's network likelihood */
+static double extrinsic_flat(bool save_waveform, const gwh_params &params, char *generation_method, int *data_length,
+                             double *frequencies, double *dataREAL, double *dataIMAG, double *psd, double *weights,
+                             char *integration_method, bool log10F, char *detectors, int num_detectors)
 {
     const double nan = std::nan("");
-    PyParams *q;
-    if (usable(parameters, &q) != GWH_OK) return nan;
     int model = model_of(generation_method);
     if (model < 0) {
         gwh_report(GWH_ERR_UNSUPPORTED, "generation method not on the engine's path");
@@ -312,9 +361,119 @@ double MCMC_likelihood_extrinsic_py(bool save_waveform, void *parameters, char *
         pp[i] = psd + off;
         wp[i] = weights ? weights + off : nullptr;
     }
-    const gwh_params pm = for_method(q->p, generation_method);
+    const gwh_params pm = for_method(params, generation_method);
     return gwh_mcmc_likelihood_extrinsic(save_waveform, &pm, model, data_length, fp.data(), dp.data(), pp.data(),
                                          weights ? wp.data() : nullptr, method, log10F, dets, num_detectors, 0.0);
+}
+
+double MCMC_likelihood_extrinsic_py(bool save_waveform, void *parameters, char *generation_method, int *data_length,
+                                    double *frequencies, double *dataREAL, double *dataIMAG, double *psd,
+                                    double *weights, char *integration_method, bool log10F, char *detectors,
+                                    int num_detectors)
+{
+    PyParams *q;
+    if (usable(parameters, &q) != GWH_OK) return std::nan("");
+    return extrinsic_flat(save_waveform, q->p, generation_method, data_length, frequencies, dataREAL, dataIMAG, psd,
+                          weights, integration_method, log10F, detectors, num_detectors);
+}
+
+void *MCMC_modification_struct_py(int ppE_Nmod, double *bppe, int gIMR_Nmod_phi, int *gIMR_phii, int gIMR_Nmod_sigma,
+                                  int *gIMR_sigmai, int gIMR_Nmod_beta, int *gIMR_betai, int gIMR_Nmod_alpha,
+                                  int *gIMR_alphai, bool NSflag1, bool NSflag2)
+{
+    PyMod *m = new PyMod;
+    m->magic = kModMagic;
+    m->ppe_nmod = ppE_Nmod;
+    if (bppe && ppE_Nmod > 0) m->bppe.assign(bppe, bppe + ppE_Nmod);
+    auto copy = [](int n, const int *v, std::vector<int> &out) {
+        if (v && n > 0) out.assign(v, v + n);
+    };
+    copy(gIMR_Nmod_phi, gIMR_phii, m->phii);
+    copy(gIMR_Nmod_sigma, gIMR_sigmai, m->sigmai);
+    copy(gIMR_Nmod_beta, gIMR_betai, m->betai);
+    copy(gIMR_Nmod_alpha, gIMR_alphai, m->alphai);
+    m->ns = NSflag1 || NSflag2;
+    std::lock_guard<std::mutex> lk(g_handles_mu);
+    g_mods.insert(m);
+    return m;
+}
+
+void MCMC_modification_struct_py_destructor(void *mod_struct)
+{
+    std::lock_guard<std::mutex> lk(g_handles_mu);
+    if (!mod_struct || !g_mods.erase(mod_struct)) return;
+    PyMod *m = static_cast<PyMod *>(mod_struct);
+    m->magic = 0;
+    delete m;
+}
+
+double MCMC_likelihood_extrinsic_pyv2(bool save_waveform, double *parameters, void *mod_struct, int dimension,
+                                      char *generation_method, int *data_length, double *frequencies,
+                                      double *dataREAL, double *dataIMAG, double *psd, double *weights,
+                                      char *integration_method, bool log10F, char *detectors, int num_detectors)
+{
+    const double nan = std::nan("");
+    PyMod *m = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_handles_mu);
+        if (mod_struct && g_mods.count(mod_struct) && static_cast<PyMod *>(mod_struct)->magic == kModMagic)
+            m = static_cast<PyMod *>(mod_struct);
+    }
+    if (!m || !parameters || !generation_method) {
+        gwh_report(GWH_ERR_INVALID, "not a modification handle from MCMC_modification_struct_py");
+        return nan;
+    }
+    if (m->ns) {
+        gwh_report(GWH_ERR_UNSUPPORTED, "the engine does not model NSflag1/NSflag2");
+        return nan;
+    }
+    /* MCMC_prep_params (mcmc_gw.cpp:2492-2567): f_ref 20 Hz, both alignments on, the sampler's gmst (the
+     * module's mcmc_gmst, zero until a sampler sets it), then the MCMC basis vector with the modification
+     * parameters after the GR block */
+    gwh_params p;
+    memset(&p, 0, sizeof(p));
+    p.f_ref = 20.;
+    p.shift_time = 1;
+    p.shift_phase = 1;
+    p.chip = -1;
+    p.phip = -1;
+    p.cosmology = GWH_COSMO_PLANCK15;
+    const int theory = gwh::theory_id(generation_method);
+    const bool ppe = strstr(generation_method, "ppE_") || theory > 0;
+    const bool gimr = !strncmp(generation_method, "gIMR", 4);
+    int nmod = 0;
+    if (ppe) {
+        if (m->ppe_nmod > GWH_PPE_MAX) {
+            gwh_report(GWH_ERR_UNSUPPORTED, "more than GWH_PPE_MAX ppE terms");
+            return nan;
+        }
+        p.ppe_nmod = nmod = m->ppe_nmod;
+        for (int i = 0; i < nmod; i++) p.ppe_b[i] = i < (int)m->bppe.size() ? m->bppe[i] : 0.0;
+    } else if (gimr) {
+        fill_gimr_list(p, m->phii, m->sigmai, m->betai, m->alphai);
+        if (p.gimr_nlist < 0) {
+            gwh_report(GWH_ERR_UNSUPPORTED, "at most 16 gIMR deltas in a parameter vector");
+            return nan;
+        }
+        nmod = p.gimr_nlist;
+    }
+    std::vector<double> v(parameters, parameters + dimension);
+    const int base = dimension - nmod;
+    /* dCS and EdGB sample the length scale in km: the theory's parameter is (l / (c / 1000))^4 */
+    if (nmod > 0 && base >= 0 && (strstr(generation_method, "dCS") || strstr(generation_method, "EdGB"))) {
+        const double x = v[base] / (299792458. / 1000.);
+        v[base] = x * x * x * x;
+    }
+    std::string gr = std::string("MCMC_") + gr_family(generation_method);
+    const int basis = basis_of(gr.c_str(), base, 0);
+    if (basis < 0) {
+        gwh_report(GWH_ERR_UNSUPPORTED, "generation method / dimension not on the engine's path");
+        return nan;
+    }
+    gwh_repack_parameters(v.data(), basis, &p);
+    set_mod_params(p, v.data() + base, nmod, ppe, gimr);
+    return extrinsic_flat(save_waveform, p, generation_method, data_length, frequencies, dataREAL, dataIMAG, psd,
+                          weights, integration_method, log10F, detectors, num_detectors);
 }
 
 void repack_parameters_py(double *parameters, void *gen_param, char *generation_method, int dim)
@@ -354,19 +513,7 @@ void repack_parameters_py(double *parameters, void *gen_param, char *generation_
         return;
     }
     gwh_repack_parameters(parameters, basis, &q->p);
-    for (int i = 0; i < nmod && ppe; i++) q->p.ppe_beta[i] = parameters[dim - nmod + i];
-    /* the waveform reads the first list entry of an index (check_list_id): later repeats are not written */
-    for (int i = 0; i < nmod && gimr; i++) {
-        const int c = q->p.gimr_list[i];
-        bool first = c >= 0;
-        for (int j = 0; j < i; j++) first &= (q->p.gimr_list[j] != c);
-        if (!first) continue;
-        const double v = parameters[dim - nmod + i];
-        if (c < GWH_GIMR_CODE_SIGMA(2)) q->p.gimr_dphi[c] = v;
-        else if (c < GWH_GIMR_CODE_BETA(2)) q->p.gimr_dsigma[c - GWH_GIMR_CODE_SIGMA(2)] = v;
-        else if (c < GWH_GIMR_CODE_ALPHA(2)) q->p.gimr_dbeta[c - GWH_GIMR_CODE_BETA(2)] = v;
-        else q->p.gimr_dalpha[c - GWH_GIMR_CODE_ALPHA(2)] = v;
-    }
+    set_mod_params(q->p, parameters + dim - nmod, nmod, ppe, gimr);
 }
 
 double gps_to_GMST_radian_py(double gps) { return gwh_gps_to_gmst_radian(gps); }

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "gwh_tables.h"
+#include "gwh_trig.h"
 
 #include "../../include/gwat_hip.h"
 
@@ -1183,7 +1184,7 @@ __device__ __forceinline__ void gwh_pv2_fp64(const InstCoef &C, double f, const 
     double alpha = gwh_euler_eval(C.a_c, v, logw) + C.alpha_add;
     double eps = gwh_euler_eval(C.e_c, v, logw) + C.eps_add;
     double sa, ca;
-    sincos(alpha, &sa, &ca);
+    gwh_sincos_cw<15>(alpha, sa, ca);
     /* e^{i m alpha}, m=-2..2 */
     double er[5], ei[5];
     er[2] = 1;

@@ -1,0 +1,57 @@
+/*
+ * gwh_trig.h -- sine and cosine of FP64 phases for the per-bin kernels.
+ *
+ * Phases of the hot path reach ~1e5-1e6 rad (2 pi f t_c plus the inspiral phase).  The library sincos of
+ * FP64 carries a large-argument (Payne-Hanek) reduction for every call; here the argument is reduced by
+ * Cody and Waite's method (W. J. Cody, W. Waite, Software Manual for the Elementary Functions, 1980):
+ * q = rint(x 2/pi), r = (x - q p1) - q p2 with pi/2 = p1 + p2 split into two doubles and each step one
+ * fused multiply-add (x - q p1 is exact by Sterbenz's lemma, the neglected part of pi/2 is 1e-33, so r is
+ * within half an ulp for |x| < 2^30), then sin r and cos r on [-pi/4, pi/4] by their Taylor series and the
+ * quadrant by q mod 4.  Arguments beyond 2^30 rad take the library routine.
+ */
+#ifndef GWH_TRIG_H
+#define GWH_TRIG_H
+
+#include <hip/hip_runtime.h>
+
+/* NT: highest odd power of the sine series (15: truncation below 5e-17 on [-pi/4, pi/4]; 11: below 7e-12) */
+template <int NT> __device__ __forceinline__ void gwh_sincos_cw(double x, double &s, double &c)
+{
+    static_assert(NT == 15 || NT == 11, "series length");
+    if (!(fabs(x) < 1073741824.0)) { /* |x| >= 2^30 or NaN */
+        sincos(x, &s, &c);
+        return;
+    }
+    const double q = rint(x * 0.63661977236758134308);
+    double r = fma(-q, 1.5707963267948966192, x);
+    r = fma(-q, 6.123233995736766036e-17, r);
+    const double r2 = r * r;
+    /* sin r = r + r^3 sum_k (-1)^k r^(2k) / (2k+3)!,  cos r = 1 - r^2/2 + r^4 sum_k (-1)^k r^(2k) / (2k+4)! */
+    double ps, pc;
+    if (NT == 15) {
+        ps = fma(-7.6471637318198164759e-13, r2, 1.6059043836821614599e-10); /* -1/15!, 1/13! */
+        ps = fma(ps, r2, -2.5052108385441718775e-08);                        /* -1/11! */
+        ps = fma(ps, r2, 2.7557319223985890653e-06);                         /* 1/9! */
+        pc = fma(4.7794773323873852974e-14, r2, -1.1470745597729724714e-11); /* 1/16!, -1/14! */
+        pc = fma(pc, r2, 2.0876756987868098979e-09);                         /* 1/12! */
+        pc = fma(pc, r2, -2.7557319223985890653e-07);                        /* -1/10! */
+    } else {
+        ps = fma(-2.5052108385441718775e-08, r2, 2.7557319223985890653e-06); /* -1/11!, 1/9! */
+        pc = fma(2.0876756987868098979e-09, r2, -2.7557319223985890653e-07); /* 1/12!, -1/10! */
+    }
+    ps = fma(ps, r2, -1.9841269841269841270e-04); /* -1/7! */
+    ps = fma(ps, r2, 8.3333333333333333333e-03);  /* 1/5! */
+    ps = fma(ps, r2, -1.6666666666666666667e-01); /* -1/3! */
+    pc = fma(pc, r2, 2.4801587301587301587e-05);  /* 1/8! */
+    pc = fma(pc, r2, -1.3888888888888888889e-03); /* -1/6! */
+    pc = fma(pc, r2, 4.1666666666666666667e-02);  /* 1/4! */
+    pc = fma(pc, r2, -0.5);
+    const double sr = fma(ps * r2, r, r);
+    const double cr = fma(pc, r2, 1.0);
+    const int iq = (int)q;
+    const double ss = (iq & 1) ? cr : sr, cc = (iq & 1) ? sr : cr;
+    s = (iq & 2) ? -ss : ss;
+    c = ((iq + 1) & 2) ? -cc : cc;
+}
+
+#endif

@@ -6,10 +6,11 @@ src/mcmc_gw.cpp Log_Likelihood / MCMC_likelihood_extrinsic, src/fisher.cpp
 fisher_numerical); all compute happens in libgwat_hip.so on the GPU.
 """
 import ctypes
+import os
 
 import numpy as np
 
-from . import GWHError, Params, PTConfig, check, lib, PARAMS_DTYPE, IMRPHENOMD, FP64, BASIS_PHENOMD_MCMC
+from . import GWHError, Params, PTConfig, check, lib, LIB_PATH, PARAMS_DTYPE, IMRPHENOMD, FP64, BASIS_PHENOMD_MCMC
 
 __all__ = ["fourier_waveform", "fourier_detector_response", "create_coherent_GW_detection", "work_counts",
            "log_likelihood", "log_likelihood_internal", "mcmc_likelihood_extrinsic", "fisher_numerical",
@@ -17,7 +18,8 @@ __all__ = ["fourier_waveform", "fourier_detector_response", "create_coherent_GW_
            "gps_to_gmst_radian", "calculate_snr_internal", "data_snr", "snr_batch", "Broker", "Event", "loglike_batch",
            "fisher_batch", "DeviceArray", "set_bins_per_lane", "synchronize", "LoglikeGraph", "DevicePT",
            "maximized_log_likelihood_aligned_internal", "maximized_log_likelihood_unaligned_internal",
-           "maximized_coal_log_likelihood_internal", "maximized_loglike_batch"]
+           "maximized_coal_log_likelihood_internal", "maximized_loglike_batch", "gauleg", "THREADED_CALLS",
+           "write_threaded_calls_input"]
 
 
 def _dp(a):
@@ -70,13 +72,18 @@ def create_coherent_GW_detection(f, p, dets, model=IMRPHENOMD):
     return [_cplx(out[2 * len(f) * d:2 * len(f) * (d + 1)]) for d in range(D)]
 
 
+NREGION_COUNTS = 10
+
+
 def work_counts(event, params, model=IMRPHENOMD):
+    """bins of every instance by (amplitude region a, phase region p): [n, 3, 3] non-zero bins and [n] zero bins"""
     arr = _as_params_array(params)
-    c = np.zeros(4 * len(arr), dtype=np.int32)
+    c = np.zeros(NREGION_COUNTS * len(arr), dtype=np.int32)
     ev = event.id if hasattr(event, "id") else int(event)
     check(lib().gwh_work_counts(ev, model, arr.ctypes.data_as(ctypes.c_void_p), len(arr),
                                 c.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
-    return c.reshape(-1, 4)
+    c = c.reshape(-1, NREGION_COUNTS)
+    return c[:, :9].reshape(-1, 3, 3), c[:, 9]
 
 
 def _nan_check(v):
@@ -104,7 +111,9 @@ def log_likelihood_internal(data, psd, f, response, weights=None, log10F=False, 
                                              len(f), int(log10F), method)
 
 
-def mcmc_likelihood_extrinsic(f, data, psd, dets, p, model=IMRPHENOMD, T_eff=0.0, save_waveform=True):
+def mcmc_likelihood_extrinsic(f, data, psd, dets, p, model=IMRPHENOMD, T_eff=0.0, save_waveform=True, weights=None,
+                              log10F=False, method=0):
+    """weights (Gauss-Legendre, method GAUSSLEG) are shared by every detector, as the detectors share one grid"""
     f = _f64(f)
     n = len(f)
     D = len(dets)
@@ -114,15 +123,18 @@ def mcmc_likelihood_extrinsic(f, data, psd, dets, p, model=IMRPHENOMD, T_eff=0.0
     fl = (DP * D)(*[_dp(f)] * D)
     dl = (DP * D)(*[_dp(x) for x in dd])
     sl = (DP * D)(*[_dp(x) for x in ss])
+    w = _f64(weights) if weights is not None else None
+    wl = (DP * D)(*[_dp(w)] * D) if w is not None else None
     lens = (ctypes.c_int * D)(*[n] * D)
     di = (ctypes.c_int * D)(*dets)
-    return lib().gwh_mcmc_likelihood_extrinsic(int(save_waveform), ctypes.byref(p), model, lens, fl, dl, sl, None, 0,
-                                               0, di, D, T_eff)
+    return lib().gwh_mcmc_likelihood_extrinsic(int(save_waveform), ctypes.byref(p), model, lens, fl, dl, sl, wl,
+                                               method, int(log10F), di, D, T_eff)
 
 
 def fisher_numerical(f, psd, basis, det, ref_det, dim, p, order=2, epsilon=1e-8, model=None):
     """model None: the GR model of the basis; a ppE model appends p.ppe_nmod betas to the basis, a gIMR model the
-    p.gimr_nlist listed deltas"""
+    p.gimr_nlist listed deltas.  epsilon > 0 is the absolute step of fisher.cpp:158; epsilon < 0 a step of
+    |epsilon| * |theta_i| for each basis coordinate (|epsilon| where theta_i is 0)"""
     f = _f64(f)
     s = _f64(psd)
     out = np.zeros(dim * dim)
@@ -161,6 +173,13 @@ def repack_parameters(v, basis, template):
     v = _f64(v)
     check(lib().gwh_repack_parameters(_dp(v), basis, ctypes.byref(out)))
     return out
+
+
+def gauleg(x1, x2, n):
+    """Gauss-Legendre nodes (ascending) and weights of the n-point rule on [x1, x2] (gauleg)"""
+    x, w = np.zeros(n), np.zeros(n)
+    check(lib().gwh_gauleg(x1, x2, _dp(x), _dp(w), n))
+    return x, w
 
 
 def detector_id(name):
@@ -203,14 +222,20 @@ def snr_batch(event, params, model=IMRPHENOMD, precision=FP64, events=None):
 class Event:
     """A resident event: shared grid, per-detector data and PSD kept in HBM."""
 
-    def __init__(self, f, data, psd, dets):
+    def __init__(self, f, data, psd, dets, weights=None, log10F=False):
+        """weights: Gauss-Legendre weights of the grid (GAUSSLEG rule; None: Simpson's rule)"""
         f = _f64(f)
         D = len(dets)
         dat = np.concatenate([_interleave(x) for x in data])
         ps = np.concatenate([_f64(x) for x in psd])
         ev = ctypes.c_int(-1)
         di = (ctypes.c_int * D)(*dets)
-        check(lib().gwh_event_create(_dp(f), len(f), D, di, _dp(dat), _dp(ps), ctypes.byref(ev)))
+        if weights is None and not log10F:
+            check(lib().gwh_event_create(_dp(f), len(f), D, di, _dp(dat), _dp(ps), ctypes.byref(ev)))
+        else:
+            w = _f64(weights) if weights is not None else None
+            check(lib().gwh_event_create_weighted(_dp(f), len(f), D, di, _dp(dat), _dp(ps), 1 if w is not None else 0,
+                                                  _dp(w) if w is not None else None, int(log10F), ctypes.byref(ev)))
         self.id = ev.value
         self.n = len(f)
         self.ndet = D
@@ -345,10 +370,10 @@ class Broker:
         return float(lat[0])
 
     def stats(self):
-        s = np.zeros(7)
+        s = np.zeros(9)
         check(lib().gwh_broker_stats(self.id, _dp(s)))
         keys = ("batches", "instances", "fisher_requests", "max_requests_per_batch", "admission_waits",
-                "peak_device_bytes", "tickets_outstanding")
+                "peak_device_bytes", "tickets_outstanding", "fisher_batches", "max_fisher_requests_per_batch")
         return {k: (int(v) if k != "peak_device_bytes" else int(v)) for k, v in zip(keys, s)}
 
     def close(self):
@@ -500,3 +525,20 @@ def maximized_loglike_batch(event, params, model=IMRPHENOMD):
     ev0 = event.id if isinstance(event, Event) else int(event)
     check(lib().gwh_maximized_loglike_batch(ev0, model, arr.ctypes.data_as(ctypes.c_void_p), len(arr), _dp(out)))
     return out
+
+
+THREADED_CALLS = os.path.join(os.path.dirname(LIB_PATH), "gwh_threaded_calls")
+
+
+def write_threaded_calls_input(path, f, data, psd, dets, params):
+    """Input file of the threaded single-call driver (host/gwh_threaded_calls.cpp): int32 n, ndet, det[ndet],
+    count, then f[n], per detector data[n] (complex) and psd[n], then `count` gwh_params records."""
+    f = _f64(f)
+    P = _as_params_array(params)
+    with open(path, "wb") as fh:
+        np.array([len(f), len(dets)] + list(dets) + [len(P)], dtype=np.int32).tofile(fh)
+        f.tofile(fh)
+        for d in range(len(dets)):
+            np.ascontiguousarray(data[d], dtype=np.complex128).tofile(fh)
+            _f64(psd[d]).tofile(fh)
+        P.tofile(fh)

@@ -32,6 +32,12 @@ def lib():
         L.fourier_detector_response_py.restype = _I
         L.MCMC_likelihood_extrinsic_py.argtypes = [_B, _V, _S, _IP, _DP, _DP, _DP, _DP, _DP, _S, _B, _S, _I]
         L.MCMC_likelihood_extrinsic_py.restype = _D
+        L.MCMC_modification_struct_py.argtypes = [_I, _DP, _I, _IP, _I, _IP, _I, _IP, _I, _IP, _B, _B]
+        L.MCMC_modification_struct_py.restype = _V
+        L.MCMC_modification_struct_py_destructor.argtypes = [_V]
+        L.MCMC_modification_struct_py_destructor.restype = None
+        L.MCMC_likelihood_extrinsic_pyv2.argtypes = [_B, _DP, _V, _I, _S, _IP, _DP, _DP, _DP, _DP, _DP, _S, _B, _S, _I]
+        L.MCMC_likelihood_extrinsic_pyv2.restype = _D
         L.repack_parameters_py.argtypes = [_DP, _V, _S, _I]
         L.repack_parameters_py.restype = None
         L.gps_to_GMST_radian_py.argtypes = [_D]
@@ -131,3 +137,44 @@ def mcmc_likelihood_extrinsic(params, method, f, data, psd, detectors, weights=N
 def repack_parameters(v, params, method, dim):
     v = np.ascontiguousarray(v, dtype=np.float64)
     lib().repack_parameters_py(_dp(v), params.handle, method.encode(), dim)
+
+
+class modification_struct:
+    """gwatpy.mcmc_routines.MCMC_modification_struct_py: the modification lists of a sampler's parameter vector"""
+
+    def __init__(self, ppE_Nmod=0, bppe=(), gIMR_phii=(), gIMR_sigmai=(), gIMR_betai=(), gIMR_alphai=(),
+                 NSflag1=False, NSflag2=False):
+        def arr(v, t):
+            return (t * max(len(v), 1))(*v) if len(v) else None
+        self.handle = lib().MCMC_modification_struct_py(
+            ppE_Nmod, arr(list(bppe), ctypes.c_double), len(gIMR_phii), arr(list(gIMR_phii), ctypes.c_int),
+            len(gIMR_sigmai), arr(list(gIMR_sigmai), ctypes.c_int), len(gIMR_betai), arr(list(gIMR_betai), ctypes.c_int),
+            len(gIMR_alphai), arr(list(gIMR_alphai), ctypes.c_int), NSflag1, NSflag2)
+
+    def close(self):
+        if self.handle:
+            lib().MCMC_modification_struct_py_destructor(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mcmc_likelihood_extrinsic_v2(parameters, mod, method, f, data, psd, detectors, weights=None,
+                                 integration="SIMPSONS", log10F=False, save_waveform=True):
+    """MCMC_likelihood_extrinsic_pyv2: a parameter vector in the method's MCMC basis (mcmc_routines.py:126-158)"""
+    D = len(data)
+    n = len(f)
+    lens = (ctypes.c_int * D)(*[n] * D)
+    ff = np.ascontiguousarray(np.tile(f, D), dtype=np.float64)
+    dr = np.ascontiguousarray(np.concatenate([np.real(x) for x in data]))
+    di = np.ascontiguousarray(np.concatenate([np.imag(x) for x in data]))
+    ps = np.ascontiguousarray(np.concatenate(psd), dtype=np.float64)
+    w = np.ascontiguousarray(np.tile(weights, D), dtype=np.float64) if weights is not None else np.ones(n * D)
+    v = np.ascontiguousarray(parameters, dtype=np.float64)
+    return lib().MCMC_likelihood_extrinsic_pyv2(save_waveform, _dp(v), mod.handle, len(v), method.encode(), lens,
+                                                _dp(ff), _dp(dr), _dp(di), _dp(ps), _dp(w), integration.encode(),
+                                                log10F, detectors.encode(), D)

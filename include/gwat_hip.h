@@ -173,6 +173,28 @@ int gwh_fourier_detector_response(const double *frequencies, int length, double 
 int gwh_create_coherent_GW_detection(const int *detectors, int num_detectors, const double *frequencies, int length,
                                      int model, const gwh_params *params, double *responses);
 
+/* ---- the single-call entry points below (the reference's signatures) ----
+ * Their host arrays become resident events, cached (LRU of 64).  By default a call finds its event by a hash
+ * of the arrays' contents (safe for any caller, ~0.3 ms per MB hashed).  Arrays registered with
+ * gwh_host_data_register (the sampler's data, PSD, frequency and weight arrays, e.g. the module-level MCMC arrays
+ * of mcmc_gw.cpp:1466-1474), or every array after gwh_set_single_call_cache(GWH_CACHE_BY_POINTER), are found by
+ * their addresses instead: the caller keeps them unchanged while registered (or in pointer mode).  Likelihood
+ * calls made concurrently from several threads are coalesced into shared launches (the reference's samplers call
+ * them from OpenMP and std::thread workers); each result is the same as the call made alone. */
+#define GWH_CACHE_BY_CONTENT 0
+#define GWH_CACHE_BY_POINTER 1
+int gwh_set_single_call_cache(int mode);
+/* No counterpart in the reference (its arrays are globals of mcmc_gw.cpp).  weights may be NULL; data entries
+ * may be NULL (arrays that are not likelihood data). */
+int gwh_host_data_register(int num_detectors, const int *data_length, const double *const *frequencies,
+                           const double *const *data, const double *const *psd, const double *const *weights,
+                           int *handle);
+int gwh_host_data_release(int handle);
+/* No counterpart in the reference.  Counters of the coalescing of concurrent single calls, in the layout of
+ * gwh_broker_stats (9 doubles: shared launches, instances, ..., most calls merged into one launch); zeros
+ * before the first concurrent call. */
+int gwh_single_call_stats(double *stats);
+
 /* Log_Likelihood(std::complex<double>*, double* psd, double* freqs, size_t, gen_params_base<double>*,
  * std::string det, std::string method) -- include/gwat/mcmc_gw.h:210-217 */
 double gwh_log_likelihood(const double *data, const double *psd, const double *frequencies, size_t length,
@@ -199,8 +221,11 @@ double gwh_mcmc_likelihood_extrinsic(int save_waveform, const gwh_params *params
 
 /* fisher_numerical(double*, int, string method, string det, string ref_det, double** out, int dim,
  * gen_params_base<double>*, int order, int*, int*, double* noise) -- include/gwat/fisher.h:35-48.
- * out is dim*dim row major; psd is the one-sided PSD on the same grid; epsilon is the
- * absolute step of the unpacked basis (the reference uses 1e-8). */
+ * out is dim*dim row major; psd is the one-sided PSD on the same grid; epsilon > 0 is the
+ * absolute step of the unpacked basis (the reference uses 1e-8).  epsilon < 0 selects a relative step
+ * instead, |epsilon| * |theta_i| for each basis coordinate theta_i (|epsilon| where theta_i is 0);
+ * this form has no counterpart in the reference.  epsilon 0 or non-finite: GWH_ERR_INVALID.  The same
+ * convention holds for every Fisher entry point below. */
 int gwh_fisher_numerical(const double *frequencies, int length, int basis, int detector, int reference_detector,
                          double *out, int dim, const gwh_params *params, int order, const double *psd,
                          double epsilon);
@@ -263,6 +288,20 @@ size_t gwh_event_bytes(int event);
 int gwh_loglike_batch(const int *events, int events0, int model, int precision, const gwh_params *params, int n,
                       double *logl);
 
+/* gwh_event_create with the integration rule of Log_Likelihood_internal (mcmc_gw.cpp:801-868) and
+ * MCMC_likelihood_extrinsic's weights argument: integration_method GWH_SIMPSONS (weights ignored) or
+ * GWH_GAUSSLEG (weights: length doubles shared by every detector, e.g. from gwh_gauleg); log10F: the grid is
+ * a log10 frequency axis, so Gauss-Legendre weights are multiplied by f ln 10 (the likelihood ignores it under
+ * Simpson's rule, as the reference does).  Batched likelihoods, SNRs and Fisher matrices on the event use
+ * its rule. */
+int gwh_event_create_weighted(const double *frequencies, int length, int ndet, const int *detectors,
+                              const double *data, const double *psd, int integration_method, const double *weights,
+                              int log10F, int *event);
+
+/* gauleg(double x1, double x2, double x[], double w[], int n) -- src/ortho_basis.cpp:14-46: Gauss-Legendre
+ * nodes (ascending) and weights of the n-point rule on [x1, x2].  Host code; no GPU needed. */
+int gwh_gauleg(double x1, double x2, double *x, double *w, int n);
+
 /* Network optimal SNR sqrt(sum_d (h_d|h_d)) of n instances on resident events (the SNR convention of
  * calculate_snr_internal, src/waveform_util.cpp:479-510, summed over the event's detectors). */
 int gwh_snr_batch(const int *events, int events0, int model, int precision, const gwh_params *params, int n,
@@ -305,10 +344,13 @@ int gwh_set_bins_per_lane(int bpl);
 
 /* Batched network Fisher: n points, order 2 central differences in the given basis (with the ppE betas or
  * listed gIMR deltas appended for those models, as gwh_fisher_numerical_model), per-detector projection
- * summed over the event's detectors.  out: n*dim*dim (host).  precision GWH_FP64 uses an
- * FP64 matrix-core Gram, GWH_MIXED an FP32 matrix-core Gram with FP64 accumulation across tiles;
- * the stencil numerators are FP64 in both.  sigma (optional, n*dim): marginal standard deviations
- * from the FP64 inverse of each Fisher matrix. */
+ * summed over the event's detectors.  out: n*dim*dim (host).  The stencil numerators are FP64 always;
+ * precision GWH_FP64 and GWH_MIXED contract them on the FP64 matrix cores (the marginal sigmas of these
+ * matrices, normalised condition numbers 1e8-1e10, need FP64 elements), GWH_FISHER_GRAM_FP32 feeds FP32
+ * operands to the FP32 matrix cores with FP64 accumulation across tiles (elements to ~1e-6 of
+ * sqrt(F_ii F_jj); not for sigmas).  sigma (optional, n*dim): marginal standard deviations from the FP64
+ * inverse of each Fisher matrix. */
+#define GWH_FISHER_GRAM_FP32 2
 int gwh_fisher_batch(int event, int model, int basis, int precision, const gwh_params *params, int n, int dim,
                      double epsilon, double *out, double *sigma);
 
@@ -318,19 +360,24 @@ int gwh_fisher_batch(int event, int model, int basis, int precision, const gwh_p
 int gwh_time_loglike_batch_device(int events0, int model, int precision, const void *d_params, int n,
                                   double *d_logl, int reps, double *ms_per_launch, double *kernel_ms);
 
-/* Bins of every instance in each IMRPhenomD phase region on `event` (counts: n*4 ints: inspiral
- * f < 0.018/M, intermediate, merger-ringdown up to 0.2/M, and zero bins above 0.2/M).  Used to
- * price the algorithmic work of a batch. */
+/* Bins of every instance by IMRPhenomD region on `event` (counts: n * GWH_NREGION_COUNTS ints): entry 3 a + p
+ * counts the bins at or below 0.2/M in amplitude region a (0: f < 0.014/M, 1: up to the amplitude peak f3,
+ * 2: above) and phase region p (0: f < 0.018/M, 1: up to f_RD / 2, 2: above); entry 9 the bins above 0.2/M.
+ * Used to price the algorithmic work of a batch. */
+#define GWH_NREGION_COUNTS 10
 int gwh_work_counts(int event, int model, const gwh_params *params, int n, int *counts);
 
 /* ---- serving broker (no counterpart in the reference: its samplers call the likelihood from many
  * threads at once, mcmc_sampler.cpp:4585-4600; the broker coalesces such calls) ----
  * Requests from any number of host threads are batched continuously onto the GPU: likelihood requests
  * of the same model and precision are concatenated (each instance keeps its event) up to
- * max_batch_instances, two streams keep one batch in flight while the next is formed, and a batch is
- * admitted only if resident events + broker scratch + its own needs fit in hbm_budget_bytes.  Results
- * are bit-identical to the same request run alone.  Parameters are copied at submission; outputs are
- * written before the ticket completes; events must outlive their pending requests. */
+ * max_batch_instances, two streams keep one batch in flight while the next is formed; Fisher requests go to
+ * two workers that merge queued requests of the same event, model, basis, precision and step.  Work is
+ * admitted only if resident events + likelihood scratch + Fisher scratch fit in hbm_budget_bytes (Fisher
+ * passes shrink to fit; what cannot fit on an idle device fails with GWH_ERR_NOMEM).  Results are bit-identical
+ * to the same request run alone.  Parameters are copied at submission; outputs are written before the ticket
+ * completes; the events a request reads are held from submission until its ticket is waited for (or the broker
+ * destroyed): gwh_event_destroy returns GWH_ERR_BUSY meanwhile. */
 int gwh_broker_create(size_t hbm_budget_bytes, int max_batch_instances, int *broker);
 int gwh_broker_destroy(int broker); /* drains pending work first */
 int gwh_broker_submit_loglike(int broker, const int *events, int event0, int model, int precision,
@@ -340,8 +387,9 @@ int gwh_broker_submit_fisher(int broker, int event, int model, int basis, int pr
 /* waits for a ticket (timeout_ms < 0: no limit); returns the request's status (GWH_ERR_BUSY on timeout);
  * latency_us (optional): submission to completion */
 int gwh_broker_wait(int broker, long long ticket, double timeout_ms, double *latency_us);
-/* stats[7]: batches, instances, Fisher requests, most requests merged into one batch, admission waits,
- * peak device bytes (resident events + broker scratch), tickets not yet waited for */
+/* stats[9]: batches, instances, Fisher requests, most requests merged into one batch, admission waits,
+ * peak device bytes (resident events + broker scratch), tickets not yet waited for, Fisher batches, most Fisher
+ * requests merged into one Fisher batch */
 int gwh_broker_stats(int broker, double *stats);
 
 /* ---- likelihoods maximised over extrinsic parameters (FFT of the matched-filter integrand, rocFFT) ---- */

@@ -255,6 +255,24 @@ inline int fourier_waveform(const double *f, int n, std::complex<double> *hplus,
                                 &p);
 }
 
+/* fourier_waveform (waveform_generator.h:70-74, waveform_generator.cpp:407-509): the bare (2,2) waveform
+ * A e^{-i(Psi - tc (f - f_ref) - phic)} of IMRPhenomD and its two ppE variants, without polarisation or inclination
+ * factors (the reference's overload reads no theory mapping and builds no other model; other method strings are
+ * refused here, where the reference returns 1 and leaves the array unwritten) */
+template <class Params>
+inline int fourier_waveform(const double *f, int n, std::complex<double> *waveform, const std::string &method,
+                            const Params *g)
+{
+    gwh_params p;
+    int model = model_id(method);
+    if ((model != GWH_IMRPHENOMD && model != GWH_PPE_IMRPHENOMD_INSPIRAL && model != GWH_PPE_IMRPHENOMD_IMR) ||
+        !to_params(*g, p, model, GWH_THEORY_NONE))
+        return GWH_ERR_UNSUPPORTED;
+    p.incl = 0; /* h+ = h (1 + cos^2 0) / 2 = h, exactly */
+    std::vector<double> hcross(2 * (size_t)(n > 0 ? n : 1));
+    return gwh_fourier_waveform(f, n, reinterpret_cast<double *>(waveform), hcross.data(), model, &p);
+}
+
 /* fourier_detector_response (waveform_util.h:472-479), terrestrial detectors */
 template <class Params>
 inline int fourier_detector_response(const double *f, int n, std::complex<double> *response, const std::string &det,

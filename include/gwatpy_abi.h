@@ -16,9 +16,11 @@
 extern "C" {
 #endif
 
-/* gwatpy_wrapping.h:178-222 -- gen_params_base_py.  Fields the engine does not model
- * (equatorial_orientation, horizon_coord, NSflag1/2, ppE and gIMR modifications) make every later
- * use of the handle fail with GWH_ERR_UNSUPPORTED. */
+/* gwatpy_wrapping.h:178-222 -- gen_params_base_py.  The ppE terms (Nmod, bppe, betappe; at most GWH_PPE_MAX)
+ * and the gIMR index lists travel with the handle: the ppE / theory and gIMR methods read them, the other methods
+ * ignore them as prep_source_parameters does.  Fields the engine does not model (equatorial_orientation,
+ * horizon_coord, NSflag1/2), more than GWH_PPE_MAX ppE terms, or ppE terms without betappe make every later use
+ * of the handle fail with GWH_ERR_UNSUPPORTED. */
 void *gen_params_base_py(double mass1, double mass2, double *spin1, double *spin2, double Luminosity_Distance,
                          double incl_angle, double RA, double DEC, double psi, double gmst, double tc, double phiRef,
                          double f_ref, double theta_l, double phi_l, double theta, double phi, char *cosmology,
@@ -47,6 +49,24 @@ double MCMC_likelihood_extrinsic_py(bool save_waveform, void *parameters, char *
                                     double *frequencies, double *dataREAL, double *dataIMAG, double *psd,
                                     double *weights, char *integration_method, bool log10F, char *detectors,
                                     int num_detectors);
+
+/* gwatpy_wrapping.h:224-240: a handle holding the modification lists of a sampler's parameter vector (ppE
+ * exponents, gIMR index lists; the NS flags make the likelihood below fail with GWH_ERR_UNSUPPORTED), and its
+ * destructor (which here also frees the handle) */
+void *MCMC_modification_struct_py(int ppE_Nmod, double *bppe, int gIMR_Nmod_phi, int *gIMR_phii, int gIMR_Nmod_sigma,
+                                  int *gIMR_sigmai, int gIMR_Nmod_beta, int *gIMR_betai, int gIMR_Nmod_alpha,
+                                  int *gIMR_alphai, bool NSflag1, bool NSflag2);
+void MCMC_modification_struct_py_destructor(void *mod_struct);
+
+/* gwatpy_wrapping.h:41-56: the network likelihood of a sampler's parameter vector in the "MCMC_" basis of the
+ * method (dimension = GR block + modification parameters), as MCMC_prep_params (mcmc_gw.cpp:2492-2567) fills it:
+ * f_ref = 20 Hz, shift_time and shift_phase on, gmst = 0 (the reference reads its module's mcmc_gmst, zero
+ * unless a sampler run has set it), dCS / EdGB length scales in km raised to the theory's parameter; then as
+ * MCMC_likelihood_extrinsic_py. */
+double MCMC_likelihood_extrinsic_pyv2(bool save_waveform, double *parameters, void *mod_struct, int dimension,
+                                      char *generation_method, int *data_length, double *frequencies,
+                                      double *dataREAL, double *dataIMAG, double *psd, double *weights,
+                                      char *integration_method, bool log10F, char *detectors, int num_detectors);
 
 /* gwatpy_wrapping.h:115-119 (the reference calls repack_parameters with original_params = NULL) */
 void repack_parameters_py(double *parameters, void *gen_param, char *generation_method, int dim);

@@ -146,3 +146,18 @@ def test_gwatpy_host_helpers_and_errors():
     ll = P.mcmc_likelihood_extrinsic(p, "IMRPhenomD", f, [np.zeros(5, complex)], [np.ones(5)], "K")
     assert np.isnan(ll)
     p.close()
+
+
+def test_gauleg_nodes_and_weights():
+    """gwh_gauleg (gauleg, ortho_basis.cpp:14) against numpy's Gauss-Legendre rule mapped to [x1, x2], ascending,
+    and exact for polynomials of degree 2n - 1.  The reference stops Newton's method once a step moves a root by
+    at most 1e-10 and takes the weight from that step's derivative, so weights next to the ends of a long rule
+    (small 1 - z^2) carry up to ~1e-6 relative of that last step; short rules agree to 1e-8."""
+    for n, (x1, x2) in ((1, (0., 1.)), (7, (20., 1024.)), (64, (np.log10(20.), np.log10(2048.))), (801, (-1., 1.))):
+        x, w = G.gauleg(x1, x2, n)
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        np.testing.assert_allclose(x, 0.5 * (x2 - x1) * xr + 0.5 * (x2 + x1), rtol=0, atol=1e-12 * max(1, abs(x2)))
+        np.testing.assert_allclose(w, 0.5 * (x2 - x1) * wr, rtol=1e-8 if n <= 64 else 2e-6, atol=0)
+        assert np.all(np.diff(x) > 0)
+        k = 2 * n - 1
+        assert abs(np.sum(w * x ** k) - (x2 ** (k + 1) - x1 ** (k + 1)) / (k + 1)) < 1e-9 * max(1, abs(x2) ** (k + 1))

@@ -83,8 +83,11 @@ def test_fisher_requests_and_admission_budget():
     evs = _events(3)
     e0, sp = evs[0]
     resident = sum(e.nbytes for e, _ in evs)
-    # a budget that leaves room for small batches only: big requests wait for in-flight work or fail
-    br = G.Broker(hbm_budget_bytes=resident + (8 << 20), max_batch_instances=4096)
+    # a budget that leaves room for small batches and a one-point Fisher pass only: big requests wait for
+    # in-flight work, Fisher passes shrink, what cannot fit fails
+    one_point = 3 * e0.n * 2 * 16 * 8 * 2  # FP64 operand rows of one point, with room for the Gram partials
+    budget = resident + one_point + (8 << 20)
+    br = G.Broker(hbm_budget_bytes=budget, max_batch_instances=4096)
     P = W.perturbed_params(sp["injection"], 4, step=3, config=4, scale=0.1)
     tf, F, _ = br.submit_fisher(e0, P, G.IMRPHENOMD, G.BASIS_PHENOMD_MCMC, 11)
     tickets = [br.submit_loglike(e0, W.perturbed_params(sp["injection"], 64, step=s, config=4, scale=0.1))
@@ -97,12 +100,45 @@ def test_fisher_requests_and_admission_budget():
         assert np.all(np.isfinite(out))
     st = br.stats()
     print("small-budget broker:", st)
-    assert st["peak_device_bytes"] <= resident + (8 << 20) + (64 << 20)
-    # a request that cannot fit even on an idle device is refused
-    huge = W.perturbed_params(sp["injection"], 4000, step=99, config=4, scale=0.1)
+    assert st["peak_device_bytes"] <= budget
+    br.close()
+    # a request that cannot fit even on an idle device is refused: 60000 instances on this 75k-bin event hold
+    # ~5 kB of instance record and tile partials each, more than the budget leaves beside the events
+    assert e0.n > 50000
+    br = G.Broker(hbm_budget_bytes=budget, max_batch_instances=60000)
+    huge = W.perturbed_params(sp["injection"], 60000, step=99, config=4, scale=0.1)
     t, _ = br.submit_loglike(e0, huge)
     with pytest.raises(G.GWHError):
         br.wait(t)
     br.close()
     for e, _ in evs:
         e.close()
+
+
+def test_events_held_until_tickets_are_collected_and_fisher_merging():
+    """an event a request reads cannot be destroyed until the ticket is waited for; concurrent Fisher requests of
+    the same event, model and basis are merged into shared batches, each result equal to the request alone"""
+    evs = _events(2)
+    e0, sp = evs[0]
+    br = G.Broker(hbm_budget_bytes=200 << 30, max_batch_instances=4096)
+    reqs = []
+    for s in range(8):
+        P = W.perturbed_params(sp["injection"], 1 + s % 3, step=40 + s, config=4, scale=0.1)
+        t, F, _ = br.submit_fisher(e0, P, G.IMRPHENOMD, G.BASIS_PHENOMD_MCMC, 11)
+        reqs.append((t, F, P))
+    tl, _ = br.submit_loglike(evs[1][0], W.perturbed_params(evs[1][1]["injection"], 16, step=1, config=4, scale=0.1))
+    rc = G.lib().gwh_event_destroy(e0.id)
+    assert rc == G.GWH_ERR_BUSY
+    assert G.lib().gwh_event_destroy(evs[1][0].id) == G.GWH_ERR_BUSY
+    for t, F, P in reqs:
+        br.wait(t)
+        assert np.array_equal(F, G.fisher_batch(e0, P, G.IMRPHENOMD, G.BASIS_PHENOMD_MCMC, 11))
+    st = br.stats()
+    print("Fisher merging:", st)
+    assert st["fisher_requests"] == 8 and st["fisher_batches"] <= 8
+    # collected: the Fisher requests' event is free again; the likelihood request still holds the other
+    e0.close()
+    assert G.lib().gwh_event_destroy(evs[1][0].id) == G.GWH_ERR_BUSY
+    br.wait(tl)
+    evs[1][0].close()
+    br.close()

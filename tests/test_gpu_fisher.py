@@ -184,10 +184,15 @@ def test_fisher_batch_network_vs_oracle(basis, method, dim, model):
 @pytest.mark.parametrize("basis,dim,model", [(G.BASIS_PHENOMD_MCMC, 11, G.IMRPHENOMD),
                                              (G.BASIS_PV2_FULL, 13, G.IMRPHENOMPV2)])
 def test_fisher_batch_mixed_vs_fp64(basis, dim, model):
+    """GWH_MIXED contracts FP64 numerators on the FP64 matrix cores (sigmas need FP64 elements): identical to FP64;
+    the FP32 Gram (GWH_FISHER_GRAM_FP32) agrees on the elements to MIXED_FISHER_TOL of sqrt(F_ii F_jj)"""
     f, psd, dets, ev = _network()
     P = _points(model, 16)
     a, sa = G.fisher_batch(ev, P, model, basis, dim, precision=G.FP64, with_sigma=True)
-    b, sb = G.fisher_batch(ev, P, model, basis, dim, precision=G.MIXED, with_sigma=True)
+    m, sm = G.fisher_batch(ev, P, model, basis, dim, precision=G.MIXED, with_sigma=True)
+    assert np.array_equal(m, a) and np.array_equal(np.isfinite(sm), np.isfinite(sa))
+    assert np.array_equal(sm[np.isfinite(sm)], sa[np.isfinite(sa)])
+    b, sb = G.fisher_batch(ev, P, model, basis, dim, precision=G.FISHER_GRAM_FP32, with_sigma=True)
     e = max(norm_err(b[i], a[i]) for i in range(len(P)))
     rel = np.abs(b - a) / np.maximum(np.abs(a), 1e-300)
     fin = np.isfinite(sa) & np.isfinite(sb)
@@ -210,9 +215,33 @@ def test_fisher_batch_composition_independent():
         assert np.array_equal(one[0], full[i])
 
 
+@pytest.mark.parametrize("basis,method,dim,sky,model", [BASES[0], BASES[2], BASES[3]])
+def test_fisher_relative_step_vs_oracle(basis, method, dim, sky, model):
+    """The benchmark configuration's "1e-7 relative" step (epsilon < 0: |epsilon| * |theta_i| per basis
+    coordinate, |epsilon| where theta_i is 0) on the 20 + 0.25 k Hz, k < 4017 grid; the bar is the
+    parity bar, or 10x the oracle's own rounding jitter at that step where that is larger."""
+    f = 20. + 0.25 * np.arange(4017)
+    psd = W.analytic_psd(f)
+    kw = _kw(model, basis, sky_average=sky)
+    gp, op = G.make_params(**kw), O.make_params(**kw)
+    a = G.fisher_numerical(f, psd, basis, 1 - sky, 0, dim, gp, epsilon=-1e-7)
+    b = O.fisher_numerical(f, psd, method, 1 - sky, 0, dim, op, epsilon=-1e-7)
+    jitter = max(norm_err(O.fisher_numerical(f, psd, method, 1 - sky, 0, dim, op, epsilon=-1e-7 * (1 + j * 1e-7)),
+                          b) for j in (1, 2, 3))
+    e = norm_err(a, b)
+    print("relative step %s sky=%d: max normalised %.3e (oracle jitter %.3e)" % (method, sky, e, jitter))
+    assert e <= max(FISHER_TOL, 10 * jitter)
+    # and it is not the absolute step of the same size
+    c = G.fisher_numerical(f, psd, basis, 1 - sky, 0, dim, gp, epsilon=1e-7)
+    assert not np.array_equal(a, c)
+
+
 def test_fisher_errors():
     f, psd = _grid()
     p = G.make_params(**INJ)
+    for bad in (0., float("nan"), float("inf")):
+        with pytest.raises(G.GWHError):
+            G.fisher_numerical(f, psd, G.BASIS_PHENOMD_MCMC, 0, 0, 11, p, epsilon=bad)
     with pytest.raises(G.GWHError):
         G.fisher_numerical(f, psd, G.BASIS_PHENOMD_MCMC, 0, 0, 7, p)  # wrong dimension
     with pytest.raises(G.GWHError):

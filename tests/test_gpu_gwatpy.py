@@ -71,3 +71,41 @@ def test_repack_parameters_py():
     assert st == 1
     assert rel(r, O.detector_response(f, o2, 0)) < 1e-10
     p.close()
+
+
+@pytest.mark.parametrize("method,model,dim,extra", [
+    ("IMRPhenomD", O.IMRPHENOMD, 11, None),
+    ("ppE_IMRPhenomD_Inspiral", O.PPE_IMRPHENOMD_INSPIRAL, 12, dict(ppE_Nmod=1, bppe=[-7.])),
+    ("gIMRPhenomD", O.GIMRPHENOMD, 13, dict(gIMR_phii=[2], gIMR_betai=[3]))])
+def test_mcmc_likelihood_extrinsic_pyv2(method, model, dim, extra):
+    """the parameter-vector form (MCMC_prep_params + the MCMC basis, mcmc_gw.cpp:2492-2567): equal to the oracle's
+    network likelihood at the repacked parameters (f_ref 20 Hz, gmst 0, both alignments on)"""
+    f = W.uniform_grid(20., 1024., 0.25)
+    psd = W.analytic_psd(f)
+    kw = dict(KW, gmst=0.0, f_ref=20.)
+    o = _oracle(kw)
+    g = W.rng(0, 6)
+    data = [O.detector_response(f, o, d) + W.noise(f, psd, g) for d in range(3)]
+    v, _ = O.unpack_parameters(o, "MCMC_IMRPhenomD", 11)
+    v = v * (1 + 2e-3 * np.arange(11) / 11)
+    okw = {}
+    if method.startswith("ppE"):
+        v = np.append(v, [4e-5])
+        okw = dict(ppe_nmod=1, ppe_b=[-7.], ppe_beta=[4e-5])
+    elif method.startswith("gIMR"):
+        v = np.append(v, [0.02, -0.05])
+        dphi = [0.] * 14
+        dphi[2 + 4] = 0.02
+        okw = dict(gimr_dphi=dphi, gimr_dbeta=[0., -0.05])
+    mod = P.modification_struct(**(extra or {}))
+    a = P.mcmc_likelihood_extrinsic_v2(v, mod, method, f, data, [psd] * 3, "HLV")
+    ob = O.repack_parameters(v[:11], "MCMC_IMRPhenomD", 11, O.make_params(**dict(_kw_oracle(kw), **okw)))
+    b = O.mcmc_likelihood_extrinsic(f, data, [psd] * 3, [0, 1, 2], ob, model, T_eff=0.0)
+    assert abs(a - b) <= 1e-9 * max(abs(b), 1.0), (a, b)
+    mod.close()
+
+
+def _kw_oracle(kw):
+    return dict(mass1=kw["mass1"], mass2=kw["mass2"], spin1=list(kw["spin1"]), spin2=list(kw["spin2"]),
+                DL=kw["Luminosity_Distance"], incl=kw["incl_angle"], RA=kw["RA"], DEC=kw["DEC"], psi=kw["psi"],
+                gmst=kw["gmst"], tc=kw["tc"], phiRef=kw["phiRef"], f_ref=kw["f_ref"])
