@@ -185,11 +185,10 @@ struct Engine {
     bool kt_on = false;
     int kt_every = 1;        /* time every kt_every-th launch */
     long long kt_seq = 0;    /* launches since timing started */
-    /* single calls: pointer-keyed cache mode, registered host arrays (handle -> arrays), the coalescing broker */
+    /* single calls: pointer-keyed cache mode, registered host arrays (handle -> arrays) */
     int cache_by_pointer = 0;
     std::vector<std::pair<int, std::vector<const void *>>> host_regs;
     int next_reg = 1;
-    int single_broker = -1;
     /* device address of the allocation-free entry point's scratch array on this engine's device
      * (hipGetSymbolAddress at initialisation; the array's name is not a device address in host code) */
     unsigned char *noalloc_scratch = nullptr;

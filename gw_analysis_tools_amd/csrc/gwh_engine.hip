@@ -34,10 +34,16 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gwh_engine.h"
@@ -88,6 +94,12 @@ void DevBuf::release()
     p = nullptr;
     cap = 0;
 }
+
+/* Bumped (under the engine lock) whenever a cached key may stop naming its event: an event destroyed or evicted,
+ * the cache mode or the registered arrays changed.  Per-thread lookups taken at one generation stay valid
+ * while it holds. */
+static std::atomic<uint64_t> g_event_gen{1};
+static void bump_event_generation() { g_event_gen.fetch_add(1, std::memory_order_acq_rel); }
 
 uint64_t gwh_hash_bytes(const void *p, size_t n, uint64_t seed)
 {
@@ -1529,6 +1541,7 @@ static int drop_pointer_keyed_events(Engine &E)
         GWH_HIP(hipStreamSynchronize(E.stream));
         GWH_HIP(hipFree(ev.block));
         ev = Event();
+        bump_event_generation();
     }
     return GWH_OK;
 }
@@ -1543,24 +1556,10 @@ int gwh_set_single_call_cache(int mode)
     if (st != GWH_OK) return st;
     if (E.cache_by_pointer && mode == GWH_CACHE_BY_CONTENT && (st = drop_pointer_keyed_events(E)) != GWH_OK) return st;
     E.cache_by_pointer = (mode == GWH_CACHE_BY_POINTER);
+    bump_event_generation();
     return GWH_OK;
 }
 
-int gwh_single_call_stats(double *stats)
-{
-    if (!stats) return gwh_set_error(GWH_ERR_INVALID, "null stats");
-    Engine &E = gwh_engine();
-    int broker;
-    {
-        std::lock_guard<std::recursive_mutex> lk(E.mu);
-        broker = E.single_broker;
-    }
-    if (broker < 0) {
-        for (int i = 0; i < 9; i++) stats[i] = 0;
-        return GWH_OK;
-    }
-    return gwh_broker_stats(broker, stats);
-}
 
 int gwh_host_data_register(int num_detectors, const int *data_length, const double *const *frequencies,
                            const double *const *data, const double *const *psd, const double *const *weights,
@@ -1580,6 +1579,7 @@ int gwh_host_data_register(int num_detectors, const int *data_length, const doub
     }
     *handle = E.next_reg++;
     E.host_regs.emplace_back(*handle, ptrs);
+    bump_event_generation();
     return GWH_OK;
 }
 
@@ -1590,6 +1590,7 @@ int gwh_host_data_release(int handle)
     for (size_t i = 0; i < E.host_regs.size(); i++)
         if (E.host_regs[i].first == handle) {
             E.host_regs.erase(E.host_regs.begin() + i);
+            bump_event_generation();
             return E.ready ? drop_pointer_keyed_events(E) : GWH_OK;
         }
     return gwh_set_error(GWH_ERR_INVALID, "unknown host data handle");
@@ -1617,6 +1618,7 @@ int gwh_event_destroy(int event)
     GWH_HIP(hipStreamSynchronize(E.stream));
     GWH_HIP(hipFree(E.events[event].block));
     E.events[event] = Event();
+    bump_event_generation();
     return GWH_OK;
 }
 
@@ -1992,40 +1994,77 @@ static bool all_registered(const Engine &E, const std::vector<const void *> &ptr
     return true;
 }
 
-/* lk: the caller's hold on the engine lock, released while the arrays are hashed (content mode), so that
- * concurrent callers hash in parallel; null: the lock stays held */
-static int single_call_event(const double *const *f, int n, int ndet, const int *dets, const double *const *data,
-                             const double *const *psd, int method, const double *weights, int log10F, int *event,
-                             int df_mode, std::unique_lock<std::recursive_mutex> *lk)
-{
-    Engine &E = gwh_engine();
-    std::vector<const void *> ptrs;
-    ptrs.push_back(f[0]);
-    for (int d = 0; d < ndet; d++) {
-        if (data) ptrs.push_back(data[d]);
-        ptrs.push_back(psd[d]);
+/* the arrays of one single call */
+struct CallArrays {
+    const double *const *f;
+    int n, ndet;
+    const int *dets;
+    const double *const *data;
+    const double *const *psd;
+    int method;
+    const double *weights;
+    int log10F, df_mode;
+    void pointers(std::vector<const void *> &ptrs) const
+    {
+        ptrs.clear();
+        ptrs.push_back(f[0]);
+        for (int d = 0; d < ndet; d++) {
+            if (data) ptrs.push_back(data[d]);
+            ptrs.push_back(psd[d]);
+        }
+        if (weights) ptrs.push_back(weights);
     }
-    if (weights) ptrs.push_back(weights);
-    /* pointer key: the caller keeps these arrays unchanged while they are cached (pointer mode) or registered */
-    const bool by_ptr = E.cache_by_pointer || (!E.host_regs.empty() && all_registered(E, ptrs));
-    uint64_t key = (by_ptr ? 0x9a7e0000ull : 0x12345ull) + (uint64_t)df_mode + (data ? 0 : 0x100);
-    if (by_ptr) {
+    /* what the key depends on besides the arrays */
+    uint64_t tail(uint64_t key) const
+    {
+        for (int d = 0; d < ndet; d++) key = gwh_hash_bytes(&dets[d], sizeof(int), key);
+        key = gwh_hash_bytes(&method, sizeof(int), key);
+        key = gwh_hash_bytes(&log10F, sizeof(int), key);
+        return key == 0 ? 1 : key;
+    }
+    uint64_t pointer_key() const
+    {
+        std::vector<const void *> ptrs;
+        pointers(ptrs);
+        uint64_t key = 0x9a7e0000ull + (uint64_t)df_mode + (data ? 0 : 0x100);
         key = gwh_hash_bytes(ptrs.data(), ptrs.size() * sizeof(void *), key);
         key = gwh_hash_bytes(&n, sizeof(int), key);
-    } else {
-        if (lk) lk->unlock(); /* only the caller's arrays are read meanwhile */
+        return tail(key);
+    }
+    uint64_t content_key() const
+    {
+        uint64_t key = 0x12345ull + (uint64_t)df_mode + (data ? 0 : 0x100);
         key = gwh_hash_bytes(f[0], (size_t)n * 8, key);
         for (int d = 0; d < ndet; d++) {
             if (data) key = gwh_hash_bytes(data[d], (size_t)n * 16, key);
             key = gwh_hash_bytes(psd[d], (size_t)n * 8, key);
         }
         if (weights) key = gwh_hash_bytes(weights, (size_t)n * 8, key);
+        return tail(key);
+    }
+};
+
+/* lk: the caller's hold on the engine lock, released while the arrays are hashed (content mode), so that
+ * concurrent callers hash in parallel; null: the lock stays held */
+static int single_call_event(const CallArrays &A, int *event, uint64_t *key_out, bool *by_ptr_out,
+                             std::unique_lock<std::recursive_mutex> *lk)
+{
+    Engine &E = gwh_engine();
+    std::vector<const void *> ptrs;
+    A.pointers(ptrs);
+    /* pointer key: the caller keeps these arrays unchanged while they are cached (pointer mode) or registered */
+    const bool by_ptr = E.cache_by_pointer || (!E.host_regs.empty() && all_registered(E, ptrs));
+    uint64_t key;
+    if (by_ptr) {
+        key = A.pointer_key();
+    } else {
+        if (lk) lk->unlock(); /* only the caller's arrays are read meanwhile */
+        key = A.content_key();
         if (lk) lk->lock();
     }
-    for (int d = 0; d < ndet; d++) key = gwh_hash_bytes(&dets[d], sizeof(int), key);
-    key = gwh_hash_bytes(&method, sizeof(int), key);
-    key = gwh_hash_bytes(&log10F, sizeof(int), key);
-    if (key == 0) key = 1;
+    if (key_out) *key_out = key;
+    if (by_ptr_out) *by_ptr_out = by_ptr;
+    const int n = A.n, ndet = A.ndet;
     int live_cached = 0, oldest = -1;
     for (int i = 0; i < GWH_MAX_EVENTS - 1; i++) {
         Event &ev = E.events[i];
@@ -2036,21 +2075,22 @@ static int single_call_event(const double *const *f, int n, int ndet, const int 
             return GWH_OK;
         }
         live_cached++;
-        if (ev.pins > 0) continue; /* held by a step graph or a pending request: never evicted */
+        if (ev.pins > 0) continue; /* held by a step graph or a call in flight: never evicted */
         if (oldest < 0 || ev.last_use < E.events[oldest].last_use) oldest = i;
     }
     if (live_cached >= 64 && oldest >= 0) { /* LRU of implicit events */
         GWH_HIP(hipStreamSynchronize(E.stream));
         GWH_HIP(hipFree(E.events[oldest].block));
         E.events[oldest] = Event();
+        bump_event_generation();
     }
     std::vector<double> dat((size_t)ndet * n * 2, 0.0), ps((size_t)ndet * n);
     for (int d = 0; d < ndet; d++) {
-        if (data) memcpy(&dat[(size_t)d * n * 2], data[d], (size_t)n * 16);
-        memcpy(&ps[(size_t)d * n], psd[d], (size_t)n * 8);
+        if (A.data) memcpy(&dat[(size_t)d * n * 2], A.data[d], (size_t)n * 16);
+        memcpy(&ps[(size_t)d * n], A.psd[d], (size_t)n * 8);
     }
-    int rc = gwh_event_create_ex(f[0], n, ndet, dets, dat.data(), ps.data(), method, weights, log10F, df_mode, key,
-                                 event);
+    int rc = gwh_event_create_ex(A.f[0], n, ndet, A.dets, dat.data(), ps.data(), A.method, A.weights, A.log10F,
+                                 A.df_mode, key, event);
     if (rc == GWH_OK) E.events[*event].ptr_key = by_ptr;
     return rc;
 }
@@ -2059,66 +2099,349 @@ int gwh_single_call_event(const double *const *f, int n, int ndet, const int *de
                           const double *const *psd, int method, const double *weights, int log10F, int *event,
                           int df_mode)
 {
-    return single_call_event(f, n, ndet, dets, data, psd, method, weights, log10F, event, df_mode, nullptr);
+    CallArrays A{f, n, ndet, dets, data, psd, method, weights, log10F, df_mode};
+    return single_call_event(A, event, nullptr, nullptr, nullptr);
 }
 
 static int cached_event(const double *const *f, int n, int ndet, const int *dets, const double *const *data,
                         const double *const *psd, int method, const double *weights, int log10F, int *event,
-                        int df_mode = 0, std::unique_lock<std::recursive_mutex> *lk = nullptr)
+                        int df_mode = 0)
 {
-    return single_call_event(f, n, ndet, dets, data, psd, method, weights, log10F, event, df_mode, lk);
+    CallArrays A{f, n, ndet, dets, data, psd, method, weights, log10F, df_mode};
+    return single_call_event(A, event, nullptr, nullptr, nullptr);
 }
 
-/* One FP64 instance on a resident event for a single-call entry point (engine lock held on entry; it may be
- * released on return, the caller touches no engine state afterwards).  With other single calls in flight the instance goes through the coalescing broker, which
- * batches the concurrent calls of all threads into shared launches (the lock is released while waiting);
- * alone it is launched directly.  The result is the same either way (an instance's result does not depend
- * on its batch). */
-static std::atomic<int> g_single_calls{0};
+/* ---- single likelihood calls from many host threads ----
+ * The reference's samplers call the likelihood from OpenMP and std::thread workers, one parameter set per call.
+ * A call finds its resident event through a per-thread cache (no engine lock while the generation holds) and
+ * queues its instance; a caller that finds a launch lane free leads: it takes every queued instance of the head's
+ * model into one batch, launches it on its lane (engine lock held for the enqueue only), waits for its stream and
+ * hands the results out; callers still queued meanwhile poll briefly, then block.  Two lanes let a second batch
+ * start while the first is in flight.  An instance's result does not depend on its batch, so every call returns
+ * what it would return alone. */
+namespace {
 
-/* counts a single likelihood call in flight from entry (before the engine lock) to return */
-struct SingleCall {
-    bool alone;
-    SingleCall() : alone(g_single_calls.fetch_add(1) == 0) {}
-    ~SingleCall() { g_single_calls.fetch_sub(1); }
+struct EventLookup {
+    uint64_t ptr_hash = 0, key = 0, gen = 0;
+    int ev = -1;
+    bool by_ptr = false;
+};
+constexpr int kLookups = 4;
+thread_local EventLookup t_lookup[kLookups];
+thread_local int t_lookup_next = 0;
+
+std::atomic<bool> g_single_ready{false};
+
+struct SinglePending {
+    int ev;
+    uint64_t key;
+    int model;
+    gwh_params p;
+    double out = 0;
+    std::atomic<int> status{0}; /* 0 queued, 1 taken by a leader, 2 done (out and rc written before) */
+    int rc = GWH_OK;
+    bool sleeping = false; /* blocked on cv (coalescer lock): only such callers need a wake-up */
+    std::condition_variable cv;
 };
 
-static int single_instance_loglike(Engine &E, std::unique_lock<std::recursive_mutex> &lk, const SingleCall &sc,
-                                   int ev, int model, const gwh_params &p, double *out)
+constexpr int kSingleLanes = 2;
+constexpr int kSingleMaxBatch = 4096;
+constexpr int kStale = 1000; /* internal: the instance's event changed under a per-thread lookup; look up again */
+
+/* A launch lane: its stream and scratch, and the batch's parameters, instance events and results in coherent pinned
+ * host memory that the kernels read and write directly (no copies to enqueue for a batch of a few hundred bytes
+ * per instance). */
+struct SingleLane {
+    hipStream_t stream = nullptr;
+    Scratch scratch;
+    gwh_params *h_params = nullptr, *g_params = nullptr; /* host / device view of the same memory */
+    int *h_events = nullptr, *g_events = nullptr;
+    double *h_logl = nullptr, *g_logl = nullptr;
+    int cap = 0;
+    bool busy = false;
+};
+
+using sclk = std::chrono::steady_clock;
+/* how long a queued caller polls for its result before it blocks: batches turn over in ~50-100 us, and a wake-up
+ * through the kernel costs more than the poll when dozens of callers wait on a few cores */
+constexpr auto kSpin = std::chrono::microseconds(200);
+
+struct Coalescer {
+    std::mutex m;
+    std::deque<SinglePending *> queue;
+    SingleLane lanes[kSingleLanes];
+    std::atomic<int> free_lanes{0}; /* mirrors of lane and queue state for callers polling without the lock */
+    std::atomic<int> queued{0};
+    long long launches = 0, instances = 0, max_merged = 0;
+};
+Coalescer g_single;
+
+/* wake a queued caller if it blocked (coalescer lock held) */
+void wake(SinglePending *q)
 {
-    if (sc.alone) return gwh_loglike_batch(nullptr, ev, model, GWH_FP64, &p, 1, out);
-    if (E.single_broker < 0) {
-        int rc = gwh_broker_create((size_t)1 << 62, 4096, &E.single_broker);
-        if (rc != GWH_OK) return rc;
+    if (q->sleeping) q->cv.notify_one();
+}
+
+int pinned_alloc(void **host, void **dev, size_t bytes)
+{
+    if (hipHostMalloc(host, bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        *host = nullptr;
+        return gwh_set_error(GWH_ERR_NOMEM, "pinned staging allocation failed");
     }
-    long long ticket = 0;
-    int rc = gwh_broker_submit_loglike(E.single_broker, nullptr, ev, model, GWH_FP64, &p, 1, out, &ticket);
-    if (rc != GWH_OK) return rc;
-    const int broker = E.single_broker;
-    lk.unlock(); /* the caller returns without the lock */
-    return gwh_broker_wait(broker, ticket, -1.0, nullptr);
+    if (hipHostGetDevicePointer(dev, *host, 0) != hipSuccess)
+        return gwh_set_error(GWH_ERR_HIP, "no device address for pinned staging");
+    return GWH_OK;
+}
+
+uint64_t pointer_tuple_hash(const CallArrays &A)
+{
+    std::vector<const void *> ptrs;
+    A.pointers(ptrs);
+    uint64_t h = gwh_hash_bytes(ptrs.data(), ptrs.size() * sizeof(void *), 0x51u + (uint64_t)A.df_mode);
+    h = gwh_hash_bytes(&A.n, sizeof(int), h);
+    h = gwh_hash_bytes(&A.ndet, sizeof(int), h);
+    uint64_t flags = (A.data ? 1u : 0u);
+    h = gwh_hash_bytes(&flags, sizeof(flags), h);
+    return A.tail(h);
+}
+
+/* the resident event of a single call: per-thread cache at the current generation, else the engine's lookup */
+int lookup_event(const CallArrays &A, int *ev, uint64_t *key)
+{
+    Engine &E = gwh_engine();
+    const uint64_t ph = pointer_tuple_hash(A);
+    const uint64_t gen = g_event_gen.load(std::memory_order_acquire);
+    if (g_single_ready.load(std::memory_order_acquire)) {
+        for (auto &L : t_lookup) {
+            if (L.gen != gen || L.ptr_hash != ph || L.ev < 0) continue;
+            if (L.by_ptr || A.content_key() == L.key) {
+                *ev = L.ev;
+                *key = L.key;
+                return GWH_OK;
+            }
+        }
+    }
+    std::unique_lock<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    bool by_ptr = false;
+    if ((st = single_call_event(A, ev, key, &by_ptr, &lk)) != GWH_OK) return st;
+    EventLookup &L = t_lookup[t_lookup_next];
+    t_lookup_next = (t_lookup_next + 1) % kLookups;
+    L.ptr_hash = ph;
+    L.key = *key;
+    L.gen = g_event_gen.load(std::memory_order_acquire); /* read under the lock that every bump holds */
+    L.ev = *ev;
+    L.by_ptr = by_ptr;
+    g_single_ready.store(true, std::memory_order_release);
+    return GWH_OK;
+}
+
+void forget_lookups()
+{
+    for (auto &L : t_lookup) L = EventLookup();
+}
+
+/* enqueue and run one batch of queued instances on lane L (no coalescer lock held) */
+void run_single_batch(SingleLane &L, std::vector<SinglePending *> &batch)
+{
+    Engine &E = gwh_engine();
+    std::vector<int> evs;
+    int n = 0, rc = GWH_OK;
+    {
+        std::lock_guard<std::recursive_mutex> lk(E.mu);
+        rc = hipSetDevice(E.device) == hipSuccess ? GWH_OK : gwh_set_error(GWH_ERR_HIP, "hipSetDevice failed");
+        if (rc == GWH_OK && !L.stream && hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess)
+            rc = gwh_set_error(GWH_ERR_HIP, "hipStreamCreate failed");
+        /* drop instances whose event changed since their lookup; the rest are pinned until the results are out */
+        std::vector<SinglePending *> live;
+        for (SinglePending *q : batch) {
+            if (q->ev < 0 || q->ev >= GWH_MAX_EVENTS || !E.events[q->ev].live || E.events[q->ev].key != q->key) {
+                q->rc = kStale;
+                continue;
+            }
+            live.push_back(q);
+        }
+        batch.swap(live);
+        n = (int)batch.size();
+        if (rc == GWH_OK && n > 0 && n > L.cap) {
+            if (L.h_params) (void)hipHostFree(L.h_params);
+            if (L.h_events) (void)hipHostFree(L.h_events);
+            if (L.h_logl) (void)hipHostFree(L.h_logl);
+            L.h_params = L.g_params = nullptr;
+            L.h_events = L.g_events = nullptr;
+            L.h_logl = L.g_logl = nullptr;
+            L.cap = 0;
+            const int cap = std::max(n, 256);
+            if ((rc = pinned_alloc((void **)&L.h_params, (void **)&L.g_params, (size_t)cap * sizeof(gwh_params))) ==
+                    GWH_OK &&
+                (rc = pinned_alloc((void **)&L.h_events, (void **)&L.g_events, (size_t)cap * sizeof(int))) == GWH_OK &&
+                (rc = pinned_alloc((void **)&L.h_logl, (void **)&L.g_logl, (size_t)cap * sizeof(double))) == GWH_OK)
+                L.cap = cap;
+        }
+        bool multi = false;
+        if (rc == GWH_OK && n > 0) {
+            for (int i = 0; i < n; i++) {
+                L.h_params[i] = batch[i]->p;
+                L.h_events[i] = batch[i]->ev;
+                multi = multi || (batch[i]->ev != batch[0]->ev);
+                bool seen = false;
+                for (int e : evs) seen = seen || (e == batch[i]->ev);
+                if (!seen) evs.push_back(batch[i]->ev);
+            }
+            for (int e : evs) E.events[e].pins++;
+            rc = gwh_loglike_launch(multi ? L.g_events : nullptr, batch[0]->ev, batch[0]->model, GWH_FP64,
+                                    L.g_params, n, L.g_logl, L.stream, false, nullptr, multi ? L.h_events : nullptr, 0,
+                                    &L.scratch);
+        }
+    }
+    if (n > 0 && L.stream && hipStreamSynchronize(L.stream) != hipSuccess && rc == GWH_OK)
+        rc = gwh_set_error(GWH_ERR_HIP, "stream synchronisation failed");
+    for (int i = 0; i < n; i++) {
+        batch[i]->rc = rc;
+        batch[i]->out = (rc == GWH_OK) ? L.h_logl[i] : std::nan("");
+    }
+    if (!evs.empty()) {
+        std::lock_guard<std::recursive_mutex> lk(E.mu);
+        for (int e : evs)
+            if (E.events[e].pins > 0) E.events[e].pins--;
+    }
+}
+
+/* one FP64 instance through the coalescer; returns GWH_OK, an error, or kStale */
+int coalesced_loglike(int ev, uint64_t key, int model, const gwh_params &p, double *out)
+{
+    Coalescer &C = g_single;
+    SinglePending me;
+    me.ev = ev;
+    me.key = key;
+    me.model = model;
+    me.p = p;
+    const int nl = kSingleLanes;
+    std::unique_lock<std::mutex> lk(C.m);
+    C.queue.push_back(&me);
+    C.queued.store((int)C.queue.size(), std::memory_order_relaxed);
+    auto lane_free = [&]() -> SingleLane * {
+        for (int l = 0; l < nl; l++)
+            if (!C.lanes[l].busy) return &C.lanes[l];
+        return nullptr;
+    };
+    auto set_busy = [&](SingleLane *L, bool b) {
+        L->busy = b;
+        int f = 0;
+        for (int l = 0; l < nl; l++) f += C.lanes[l].busy ? 0 : 1;
+        C.free_lanes.store(f, std::memory_order_release);
+    };
+    for (;;) {
+        if (me.status.load(std::memory_order_acquire) == 2) break;
+        SingleLane *lane = lane_free();
+        if (lane && !C.queue.empty()) {
+            set_busy(lane, true);
+            /* lead: the head's model, every queued instance of it in arrival order */
+            std::vector<SinglePending *> batch, all;
+            const int model0 = C.queue.front()->model;
+            for (auto it = C.queue.begin(); it != C.queue.end();) {
+                if ((*it)->model == model0 && (int)batch.size() < kSingleMaxBatch) {
+                    (*it)->status.store(1, std::memory_order_relaxed);
+                    batch.push_back(*it);
+                    it = C.queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            C.queued.store((int)C.queue.size(), std::memory_order_relaxed);
+            all = batch;
+            /* another lane still free and instances of another model queued: their head leads there */
+            if (lane_free() && !C.queue.empty()) wake(C.queue.front());
+            lk.unlock();
+            run_single_batch(*lane, batch);
+            lk.lock();
+            set_busy(lane, false);
+            C.launches++;
+            C.instances += (long long)batch.size();
+            if ((long long)batch.size() > C.max_merged) C.max_merged = (long long)batch.size();
+            for (SinglePending *q : all) {
+                const bool sl = q->sleeping; /* read before the release: the owner may return right after */
+                q->status.store(2, std::memory_order_release);
+                if (sl && q != &me) q->cv.notify_one();
+            }
+            if (!C.queue.empty()) wake(C.queue.front()); /* a lane is free: the next head leads */
+            continue;
+        }
+        /* wait without the lock first (no system call on either side while batches turn over quickly) */
+        lk.unlock();
+        const auto until = sclk::now() + kSpin;
+        bool done = false, try_lead = false;
+        while (sclk::now() < until) {
+            if (me.status.load(std::memory_order_acquire) == 2) {
+                done = true;
+                break;
+            }
+            if (C.free_lanes.load(std::memory_order_acquire) > 0 && C.queued.load(std::memory_order_relaxed) > 0) {
+                try_lead = true;
+                break;
+            }
+            std::this_thread::yield();
+        }
+        lk.lock();
+        if (done || try_lead) continue;
+        if (me.status.load(std::memory_order_acquire) == 2 || (lane_free() && !C.queue.empty())) continue;
+        me.sleeping = true;
+        me.cv.wait(lk);
+        me.sleeping = false;
+    }
+    *out = me.out;
+    return me.rc;
+}
+
+} // namespace
+
+/* one FP64 instance of a single-call entry point: lookup, coalesced launch, one more lookup if the event changed */
+static double single_call_loglike(const CallArrays &A, int model, const gwh_params &p)
+{
+    const double nan = std::nan("");
+    for (int attempt = 0; attempt < 3; attempt++) {
+        int ev;
+        uint64_t key;
+        if (lookup_event(A, &ev, &key) != GWH_OK) return nan;
+        double out = nan;
+        int rc = coalesced_loglike(ev, key, model, p, &out);
+        if (rc == GWH_OK) return out;
+        if (rc != kStale) return nan;
+        forget_lookups();
+    }
+    gwh_set_error(GWH_ERR_BUSY, "the call's cached event kept changing");
+    return nan;
+}
+
+int gwh_single_call_stats(double *stats)
+{
+    if (!stats) return gwh_set_error(GWH_ERR_INVALID, "null stats");
+    std::lock_guard<std::mutex> lk(g_single.m);
+    for (int i = 0; i < 9; i++) stats[i] = 0;
+    stats[0] = (double)g_single.launches;
+    stats[1] = (double)g_single.instances;
+    stats[3] = (double)g_single.max_merged;
+    stats[6] = (double)g_single.queue.size();
+    return GWH_OK;
 }
 
 double gwh_log_likelihood(const double *data, const double *psd, const double *frequencies, size_t length,
                           const gwh_params *params, int detector, int model)
 {
-    SingleCall sc;
-    Engine &E = gwh_engine();
-    std::unique_lock<std::recursive_mutex> lk(E.mu);
     const double nan = std::nan("");
-    if (gwh_ensure_init() != GWH_OK) return nan;
     int n = (int)length;
-    if (check_common(frequencies, n, params, model) != GWH_OK || !data || !psd) return nan;
+    if (check_common(frequencies, n, params, model) != GWH_OK || !data || !psd) {
+        if (!data || !psd) gwh_set_error(GWH_ERR_INVALID, "null argument");
+        return nan;
+    }
     if (detector < 0 || detector >= GWH_NDET_TABLE) {
         gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported detector");
         return nan;
     }
-    int ev;
     const double *fp[1] = {frequencies}, *dp[1] = {data}, *pp[1] = {psd};
-    if (cached_event(fp, n, 1, &detector, dp, pp, GWH_SIMPSONS, nullptr, 0, &ev, 0, &lk) != GWH_OK) return nan;
-    double out = nan;
-    if (single_instance_loglike(E, lk, sc, ev, model, *params, &out) != GWH_OK) return nan;
-    return out;
+    CallArrays A{fp, n, 1, &detector, dp, pp, GWH_SIMPSONS, nullptr, 0, 0};
+    return single_call_loglike(A, model, *params);
 }
 
 double gwh_mcmc_likelihood_extrinsic(int save_waveform, const gwh_params *params, int model, const int *data_length,
@@ -2126,13 +2449,13 @@ double gwh_mcmc_likelihood_extrinsic(int save_waveform, const gwh_params *params
                                      const double *const *psd, const double *const *weights, int integration_method,
                                      int log10F, const int *detectors, int num_detectors, double T_eff)
 {
-    SingleCall sc;
-    Engine &E = gwh_engine();
-    std::unique_lock<std::recursive_mutex> lk(E.mu);
     const double nan = std::nan("");
-    if (gwh_ensure_init() != GWH_OK) return nan;
     if (!params || !data_length || !frequencies || !data || !psd || !detectors || num_detectors < 1) {
         gwh_set_error(GWH_ERR_INVALID, "null argument");
+        return nan;
+    }
+    if (!gwh_model_valid(model)) {
+        gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported model");
         return nan;
     }
     if (!save_waveform) {
@@ -2141,6 +2464,10 @@ double gwh_mcmc_likelihood_extrinsic(int save_waveform, const gwh_params *params
         return nan;
     }
     int n = data_length[0];
+    if (n < 1) {
+        gwh_set_error(GWH_ERR_INVALID, "length must be positive");
+        return nan;
+    }
     for (int d = 1; d < num_detectors; d++) {
         if (data_length[d] != n ||
             (frequencies[d] != frequencies[0] && memcmp(frequencies[d], frequencies[0], (size_t)n * 8) != 0)) {
@@ -2152,16 +2479,16 @@ double gwh_mcmc_likelihood_extrinsic(int save_waveform, const gwh_params *params
         gwh_set_error(GWH_ERR_UNSUPPORTED, "at most 4 detectors");
         return nan;
     }
-    int ev;
-    if (cached_event(frequencies, n, num_detectors, detectors, data, psd, integration_method,
-                     (integration_method == GWH_GAUSSLEG && weights) ? weights[0] : nullptr, log10F, &ev, 0,
-                     &lk) != GWH_OK)
-        return nan;
+    for (int d = 0; d < num_detectors; d++)
+        if (detectors[d] < 0 || detectors[d] >= GWH_NDET_TABLE) {
+            gwh_set_error(GWH_ERR_UNSUPPORTED, "unsupported detector");
+            return nan;
+        }
+    const double *w = (integration_method == GWH_GAUSSLEG && weights) ? weights[0] : nullptr;
+    CallArrays A{frequencies, n, num_detectors, detectors, data, psd, integration_method, w, log10F, 0};
     gwh_params p = *params;
     p.tc = T_eff - params->tc; /* mcmc_gw.cpp:2466-2468 */
-    double out = nan;
-    if (single_instance_loglike(E, lk, sc, ev, model, p, &out) != GWH_OK) return nan;
-    return out;
+    return single_call_loglike(A, model, p);
 }
 
 double gwh_log_likelihood_internal(const double *data, const double *psd, const double *frequencies,

@@ -175,12 +175,14 @@ int gwh_create_coherent_GW_detection(const int *detectors, int num_detectors, co
 
 /* ---- the single-call entry points below (the reference's signatures) ----
  * Their host arrays become resident events, cached (LRU of 64).  By default a call finds its event by a hash
- * of the arrays' contents (safe for any caller, ~0.3 ms per MB hashed).  Arrays registered with
+ * of the arrays' contents (safe for any caller, ~0.1 ms per MB hashed, outside the engine lock).  Arrays registered with
  * gwh_host_data_register (the sampler's data, PSD, frequency and weight arrays, e.g. the module-level MCMC arrays
  * of mcmc_gw.cpp:1466-1474), or every array after gwh_set_single_call_cache(GWH_CACHE_BY_POINTER), are found by
- * their addresses instead: the caller keeps them unchanged while registered (or in pointer mode).  Likelihood
- * calls made concurrently from several threads are coalesced into shared launches (the reference's samplers call
- * them from OpenMP and std::thread workers); each result is the same as the call made alone. */
+ * their addresses instead: the caller keeps them unchanged while registered (or in pointer mode).  A thread that
+ * repeats a call on the same arrays finds the event again without taking the engine lock.  Likelihood calls made
+ * concurrently from several threads are coalesced into shared launches (the reference's samplers call them from
+ * OpenMP and std::thread workers): the calling thread that finds a launch lane free (two exist) launches every
+ * queued call of its model together and hands the results out.  Each result is the same as the call made alone. */
 #define GWH_CACHE_BY_CONTENT 0
 #define GWH_CACHE_BY_POINTER 1
 int gwh_set_single_call_cache(int mode);
@@ -190,9 +192,8 @@ int gwh_host_data_register(int num_detectors, const int *data_length, const doub
                            const double *const *data, const double *const *psd, const double *const *weights,
                            int *handle);
 int gwh_host_data_release(int handle);
-/* No counterpart in the reference.  Counters of the coalescing of concurrent single calls, in the layout of
- * gwh_broker_stats (9 doubles: shared launches, instances, ..., most calls merged into one launch); zeros
- * before the first concurrent call. */
+/* No counterpart in the reference.  Counters of the single likelihood calls, 9 doubles in the layout of
+ * gwh_broker_stats: [0] launches, [1] calls launched, [3] most calls in one launch, [6] calls queued now (others 0). */
 int gwh_single_call_stats(double *stats);
 
 /* Log_Likelihood(std::complex<double>*, double* psd, double* freqs, size_t, gen_params_base<double>*,
