@@ -46,6 +46,7 @@
 #include <thread>
 #include <vector>
 
+#include "gwh_coalesce.h"
 #include "gwh_engine.h"
 #include "gwh_tables.h"
 
@@ -2132,16 +2133,13 @@ thread_local int t_lookup_next = 0;
 
 std::atomic<bool> g_single_ready{false};
 
-struct SinglePending {
+/* one queued call; its group is the likelihood model (one model per launch) */
+struct SinglePending : gwh::CoalesceItem {
     int ev;
     uint64_t key;
-    int model;
     gwh_params p;
     double out = 0;
-    std::atomic<int> status{0}; /* 0 queued, 1 taken by a leader, 2 done (out and rc written before) */
     int rc = GWH_OK;
-    bool sleeping = false; /* blocked on cv (coalescer lock): only such callers need a wake-up */
-    std::condition_variable cv;
 };
 
 constexpr int kSingleLanes = 2;
@@ -2158,29 +2156,13 @@ struct SingleLane {
     int *h_events = nullptr, *g_events = nullptr;
     double *h_logl = nullptr, *g_logl = nullptr;
     int cap = 0;
-    bool busy = false;
 };
 
-using sclk = std::chrono::steady_clock;
-/* how long a queued caller polls for its result before it blocks: batches turn over in ~50-100 us, and a wake-up
- * through the kernel costs more than the poll when dozens of callers wait on a few cores */
-constexpr auto kSpin = std::chrono::microseconds(200);
-
-struct Coalescer {
-    std::mutex m;
-    std::deque<SinglePending *> queue;
+struct SingleCalls {
+    gwh::Coalescer<kSingleLanes> co{kSingleMaxBatch};
     SingleLane lanes[kSingleLanes];
-    std::atomic<int> free_lanes{0}; /* mirrors of lane and queue state for callers polling without the lock */
-    std::atomic<int> queued{0};
-    long long launches = 0, instances = 0, max_merged = 0;
 };
-Coalescer g_single;
-
-/* wake a queued caller if it blocked (coalescer lock held) */
-void wake(SinglePending *q)
-{
-    if (q->sleeping) q->cv.notify_one();
-}
+SingleCalls g_single;
 
 int pinned_alloc(void **host, void **dev, size_t bytes)
 {
@@ -2242,10 +2224,13 @@ void forget_lookups()
     for (auto &L : t_lookup) L = EventLookup();
 }
 
-/* enqueue and run one batch of queued instances on lane L (no coalescer lock held) */
-void run_single_batch(SingleLane &L, std::vector<SinglePending *> &batch)
+/* enqueue and run one batch of queued calls on lane L (no coalescer lock held); `items` shrinks to the calls
+ * launched (the others are marked stale) */
+void run_single_batch(SingleLane &L, std::vector<gwh::CoalesceItem *> &items)
 {
     Engine &E = gwh_engine();
+    std::vector<SinglePending *> batch;
+    for (auto *it : items) batch.push_back(static_cast<SinglePending *>(it));
     std::vector<int> evs;
     int n = 0, rc = GWH_OK;
     {
@@ -2263,6 +2248,7 @@ void run_single_batch(SingleLane &L, std::vector<SinglePending *> &batch)
             live.push_back(q);
         }
         batch.swap(live);
+        items.assign(batch.begin(), batch.end());
         n = (int)batch.size();
         if (rc == GWH_OK && n > 0 && n > L.cap) {
             if (L.h_params) (void)hipHostFree(L.h_params);
@@ -2290,7 +2276,7 @@ void run_single_batch(SingleLane &L, std::vector<SinglePending *> &batch)
                 if (!seen) evs.push_back(batch[i]->ev);
             }
             for (int e : evs) E.events[e].pins++;
-            rc = gwh_loglike_launch(multi ? L.g_events : nullptr, batch[0]->ev, batch[0]->model, GWH_FP64,
+            rc = gwh_loglike_launch(multi ? L.g_events : nullptr, batch[0]->ev, batch[0]->group, GWH_FP64,
                                     L.g_params, n, L.g_logl, L.stream, false, nullptr, multi ? L.h_events : nullptr, 0,
                                     &L.scratch);
         }
@@ -2311,85 +2297,14 @@ void run_single_batch(SingleLane &L, std::vector<SinglePending *> &batch)
 /* one FP64 instance through the coalescer; returns GWH_OK, an error, or kStale */
 int coalesced_loglike(int ev, uint64_t key, int model, const gwh_params &p, double *out)
 {
-    Coalescer &C = g_single;
     SinglePending me;
+    me.group = model;
     me.ev = ev;
     me.key = key;
-    me.model = model;
     me.p = p;
-    const int nl = kSingleLanes;
-    std::unique_lock<std::mutex> lk(C.m);
-    C.queue.push_back(&me);
-    C.queued.store((int)C.queue.size(), std::memory_order_relaxed);
-    auto lane_free = [&]() -> SingleLane * {
-        for (int l = 0; l < nl; l++)
-            if (!C.lanes[l].busy) return &C.lanes[l];
-        return nullptr;
-    };
-    auto set_busy = [&](SingleLane *L, bool b) {
-        L->busy = b;
-        int f = 0;
-        for (int l = 0; l < nl; l++) f += C.lanes[l].busy ? 0 : 1;
-        C.free_lanes.store(f, std::memory_order_release);
-    };
-    for (;;) {
-        if (me.status.load(std::memory_order_acquire) == 2) break;
-        SingleLane *lane = lane_free();
-        if (lane && !C.queue.empty()) {
-            set_busy(lane, true);
-            /* lead: the head's model, every queued instance of it in arrival order */
-            std::vector<SinglePending *> batch, all;
-            const int model0 = C.queue.front()->model;
-            for (auto it = C.queue.begin(); it != C.queue.end();) {
-                if ((*it)->model == model0 && (int)batch.size() < kSingleMaxBatch) {
-                    (*it)->status.store(1, std::memory_order_relaxed);
-                    batch.push_back(*it);
-                    it = C.queue.erase(it);
-                } else {
-                    ++it;
-                }
-            }
-            C.queued.store((int)C.queue.size(), std::memory_order_relaxed);
-            all = batch;
-            /* another lane still free and instances of another model queued: their head leads there */
-            if (lane_free() && !C.queue.empty()) wake(C.queue.front());
-            lk.unlock();
-            run_single_batch(*lane, batch);
-            lk.lock();
-            set_busy(lane, false);
-            C.launches++;
-            C.instances += (long long)batch.size();
-            if ((long long)batch.size() > C.max_merged) C.max_merged = (long long)batch.size();
-            for (SinglePending *q : all) {
-                const bool sl = q->sleeping; /* read before the release: the owner may return right after */
-                q->status.store(2, std::memory_order_release);
-                if (sl && q != &me) q->cv.notify_one();
-            }
-            if (!C.queue.empty()) wake(C.queue.front()); /* a lane is free: the next head leads */
-            continue;
-        }
-        /* wait without the lock first (no system call on either side while batches turn over quickly) */
-        lk.unlock();
-        const auto until = sclk::now() + kSpin;
-        bool done = false, try_lead = false;
-        while (sclk::now() < until) {
-            if (me.status.load(std::memory_order_acquire) == 2) {
-                done = true;
-                break;
-            }
-            if (C.free_lanes.load(std::memory_order_acquire) > 0 && C.queued.load(std::memory_order_relaxed) > 0) {
-                try_lead = true;
-                break;
-            }
-            std::this_thread::yield();
-        }
-        lk.lock();
-        if (done || try_lead) continue;
-        if (me.status.load(std::memory_order_acquire) == 2 || (lane_free() && !C.queue.empty())) continue;
-        me.sleeping = true;
-        me.cv.wait(lk);
-        me.sleeping = false;
-    }
+    g_single.co.submit(me, [](int lane, std::vector<gwh::CoalesceItem *> &batch) {
+        run_single_batch(g_single.lanes[lane], batch);
+    });
     *out = me.out;
     return me.rc;
 }
@@ -2417,12 +2332,14 @@ static double single_call_loglike(const CallArrays &A, int model, const gwh_para
 int gwh_single_call_stats(double *stats)
 {
     if (!stats) return gwh_set_error(GWH_ERR_INVALID, "null stats");
-    std::lock_guard<std::mutex> lk(g_single.m);
+    long long launches, items, most;
+    int queued;
+    g_single.co.stats(&launches, &items, &most, &queued);
     for (int i = 0; i < 9; i++) stats[i] = 0;
-    stats[0] = (double)g_single.launches;
-    stats[1] = (double)g_single.instances;
-    stats[3] = (double)g_single.max_merged;
-    stats[6] = (double)g_single.queue.size();
+    stats[0] = (double)launches;
+    stats[1] = (double)items;
+    stats[3] = (double)most;
+    stats[6] = (double)queued;
     return GWH_OK;
 }
 
