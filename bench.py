@@ -9,11 +9,14 @@ enqueue of the batched engine followed by a stream synchronisation (a sampler re
 step), so per-step latencies give p50/p99.
 
 `value` is the mixed-precision batched engine (FP64 phase, amplitude, products and sums; FP32 sin/cos
-of the detector phases and FP32 merger-ringdown exponential).  Reported beside it: the FP64 batched engine, the plain FP64
-one-instance-per-launch path, max |delta logL| mixed vs FP64 over every timed evaluation, the same
-for IMRPhenomPv2 on the same grid (1024 chains), kernel-level roofline from HIP events, and the CPU
-oracle (the reference algorithm restated in C, one instance per call, OpenMP over instances) timed on
-a bounded sample on the host cores.
+of the detector phases and FP32 merger-ringdown exponential).  Reported beside it: the FP64 batched engine, the
+plain FP64 one-instance-per-launch path, max |delta logL| mixed vs FP64 over every timed evaluation, the same for
+IMRPhenomPv2 on the same grid (1024 chains; its mixed mode keeps FP64 table-driven sin/cos and exponentials),
+kernel-level roofline from HIP events, the single-detector workload with its sky-averaged Fisher, batched
+13-parameter IMRPhenomPv2 Fisher matrices, the tempering step on the device, the FFT-maximised likelihood, the
+uneven multi-event batch, the serving broker, the reference-signature C++ layer called from many host threads,
+and the CPU oracle (the reference algorithm restated in C, one instance per call, OpenMP over instances) timed
+on bounded samples on the host cores.
 """
 import argparse
 import ctypes

@@ -319,11 +319,9 @@ void fisher_worker(Broker *bp, int w)
                     continue;
                 }
                 if (F.bytes() > 0) { /* give back this worker's own buffers and size from nothing */
-                    (void)hipStreamSynchronize(F.stream);
                     std::lock_guard<std::recursive_mutex> le(E.mu);
                     std::lock_guard<std::mutex> lk(B.m);
-                    for (DevBuf *d : {&F.params, &F.coef, &F.misc, &F.deriv, &F.gram, &F.out, &F.sig, &F.raw})
-                        d->release();
+                    F.release();
                     continue;
                 }
                 st = gwh_set_error(GWH_ERR_NOMEM, "Fisher request exceeds the HBM budget");
@@ -431,11 +429,7 @@ void dispatcher(Broker *bp)
                         std::lock_guard<std::mutex> lq(B.m);
                         for (int q = 0; q < kFisherWorkers; q++)
                             if (!B.fisher_busy[q]) {
-                                (void)hipStreamSynchronize(B.fisher[q].stream);
-                                for (DevBuf *d : {&B.fisher[q].params, &B.fisher[q].coef, &B.fisher[q].misc,
-                                                  &B.fisher[q].deriv, &B.fisher[q].gram, &B.fisher[q].out,
-                                                  &B.fisher[q].sig, &B.fisher[q].raw})
-                                    d->release();
+                                B.fisher[q].release();
                             }
                     }
                     if (used_bytes(B) - mine + lane_bytes_after(*free_lane, evs, n) > B.budget) {
@@ -549,8 +543,7 @@ int gwh_broker_destroy(int broker)
     for (int w = 0; w < kFisherWorkers; w++) {
         B->fisher_th[w].join();
         FisherScratch &F = B->fisher[w];
-        (void)hipStreamSynchronize(F.stream);
-        for (DevBuf *d : {&F.params, &F.coef, &F.misc, &F.deriv, &F.gram, &F.out, &F.sig, &F.raw}) d->release();
+        F.release(true);
         (void)hipStreamDestroy(F.stream);
     }
     /* tickets never collected release their events */

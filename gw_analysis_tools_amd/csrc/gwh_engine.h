@@ -153,9 +153,28 @@ struct FisherScratch {
     hipStream_t stream = nullptr;
     DevBuf params, coef, misc, deriv, gram, out, sig, raw;
     size_t max_operand = (size_t)1 << 30; /* operand rows of one pass: points per pass follow from it */
-    size_t bytes() const
+    /* pinned host staging of a pass's stencil records, weights and flags; two alternate, so the host builds the
+     * next pass while the device runs the current one (stage_done: the copy out of each has run) */
+    void *stage[2] = {nullptr, nullptr};
+    size_t stage_cap[2] = {0, 0};
+    hipEvent_t stage_done[2] = {nullptr, nullptr};
+    size_t bytes() const /* device bytes */
     {
         return params.cap + coef.cap + misc.cap + deriv.cap + gram.cap + out.cap + sig.cap + raw.cap;
+    }
+    /* give back the device buffers (the stream's work finished first); all_host: the host staging too */
+    void release(bool all_host = false)
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (DevBuf *d : {&params, &coef, &misc, &deriv, &gram, &out, &sig, &raw}) d->release();
+        if (!all_host) return;
+        for (int i = 0; i < 2; i++) {
+            if (stage[i]) (void)hipHostFree(stage[i]);
+            if (stage_done[i]) (void)hipEventDestroy(stage_done[i]);
+            stage[i] = nullptr;
+            stage_cap[i] = 0;
+            stage_done[i] = nullptr;
+        }
     }
 };
 

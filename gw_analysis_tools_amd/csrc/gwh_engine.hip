@@ -727,10 +727,11 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double *__
 }
 
 
-/* sin and cos of an FP64 phase: FP64 reduction to a fraction of a turn in [-1/2, 1/2], then the FP32
+/* sin and cos of an FP64 phase in FP32: FP64 reduction to a fraction of a turn in [-1/2, 1/2], then the FP32
  * sin(pi x), cos(pi x) of x = 2 * fraction (exact scaling; a short polynomial, no large-argument path).
- * The part of x lost in rounding it to FP32 (up to 6e-8 half-turns, i.e. 1.9e-7 rad, the largest single
- * error of the mixed path) is restored to first order: sin(a + d) = sin a + d cos a, cos(a + d) = cos a - d sin a. */
+ * The part of x lost in rounding it to FP32 (up to 6e-8 half-turns, i.e. 1.9e-7 rad) is restored to first
+ * order: sin(a + d) = sin a + d cos a, cos(a + d) = cos a - d sin a.  What remains is FP32 sinpi's own rounding
+ * (~6e-8), random from bin to bin. */
 __device__ __forceinline__ void sincos_turns(double phase, float &s, float &c)
 {
     double r = phase * 0.15915494309189533577;
@@ -744,8 +745,8 @@ __device__ __forceinline__ void sincos_turns(double phase, float &s, float &c)
     c = fmaf(-d, s0, c0);
 }
 
-/* e^x for x <= 0 of moderate size: FP64 split x log2(e) = n + t, FP32 2^t, exact scaling by 2^n */
-__device__ __forceinline__ double exp_mixed(double x)
+/* e^x for x <= 0 of moderate size: FP64 split x log2(e) = n + t, FP32 2^t (~6e-8 relative), exact scaling */
+__device__ __forceinline__ double exp_fp32(double x)
 {
     double y = x * 1.4426950408889634074;
     double n = rint(y);
@@ -774,9 +775,11 @@ __device__ __forceinline__ double rsqrt_mixed(double z)
     return y;
 }
 
-/* IMRPhenomD in mixed precision: the FP64 evaluator with an FP32 exponential in the merger-ringdown
- * amplitude (the phase, including its arctangent, stays FP64: an FP32 atan there was the largest error
- * of the path, ~7e-4 in logL for heavy binaries). */
+/* IMRPhenomD in mixed precision: the FP64 evaluator with the merger-ringdown amplitude's exponential in FP32
+ * (TABLE_EXP false: ~6e-8 relative per bin) or through the FP64 table (true: a few ulp); the phase, including its
+ * arctangent, stays FP64 (an FP32 atan there was the largest error of the path, ~7e-4 in logL for heavy
+ * binaries). */
+template <bool TABLE_EXP>
 __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, const BinPow &bp, double &a, double &phi,
                                                   double &u_out, double &lx_out)
 {
@@ -792,7 +795,7 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, c
         ar = pn + u3 * u3 * u * (C.rho[0] + u * (C.rho[1] + u * C.rho[2]));
     } else if (f > C.f3) {
         double dl = f - C.fRD;
-        ar = C.mr_k * exp_mixed(-C.mr_g2 * dl) * rcp_newton(dl * dl + C.mr_w2);
+        ar = C.mr_k * (TABLE_EXP ? gwh_exp_table(-C.mr_g2 * dl) : exp_fp32(-C.mr_g2 * dl)) * rcp_newton(dl * dl + C.mr_w2);
     } else {
         ar = C.delta[0] + Mf * (C.delta[1] + Mf * (C.delta[2] + Mf * (C.delta[3] + Mf * C.delta[4])));
     }
@@ -876,7 +879,7 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                     if (PREC == GWH_FP64)
                         gwh_phenomd_fp64(C, f, bp, a, phi, u, lx);
                     else
-                        gwh_phenomd_mixed(C, f, bp, a, phi, u, lx);
+                        gwh_phenomd_mixed<false>(C, f, bp, a, phi, u, lx);
                     a = live ? a : 0.0;
                     phi = live ? phi : 0.0;
 #pragma unroll
@@ -884,7 +887,7 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                         if (d >= D) break;
                         double s, c;
                         if (PREC == GWH_FP64) {
-                            gwh_sincos_cw<15>(phi + C.det[d][3] * f, s, c);
+                            gwh_sincos_table(phi + C.det[d][3] * f, s, c);
                         } else {
                             float sf, cf;
                             sincos_turns(phi + C.det[d][3] * f, sf, cf);
@@ -903,9 +906,10 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                         gwh_pv2_fp64(C, f, bp, A, Phi, hpr, hpi, hcr, hci);
                     } else {
                         double u, lx;
-                        gwh_phenomd_mixed(C, f, bp, A, Phi, u, lx);
+                        gwh_phenomd_mixed<true>(C, f, bp, A, Phi, u, lx);
                         /* precession angles and the sin/cos of the Euler angle alpha in FP64 (alpha's rounding
-                         * would enter |h|^2 directly); the detector phases below take the FP32 sin/cos */
+                         * would enter |h|^2 directly), like the detector phases below: table-driven sin/cos and
+                         * exponential (the FP32 ones left ~2.5e-4 in logL on short, loud precessing events) */
                         const double v = u * 1.4645918875615231;
                         const double logw = C.lgPiM + bp.lf; /* log(pi M f) */
                         const double x = v * v;
@@ -919,7 +923,7 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                         const double alpha = gwh_euler_eval(C.a_c, v, logw) + C.alpha_add;
                         const double eps = gwh_euler_eval(C.e_c, v, logw) + C.eps_add;
                         double sa, ca;
-                        gwh_sincos_cw<11>(alpha, sa, ca);
+                        gwh_sincos_table(alpha, sa, ca);
                         double er[5], ei[5];
                         er[2] = 1.;
                         ei[2] = 0.;
@@ -954,14 +958,10 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                         const double P = C.det[d][0], Q = C.det[d][1];
                         const double Gr = P * hpr + Q * hcr, Gi = P * hpi + Q * hci;
                         double s, c;
-                        if (PREC == GWH_FP64) {
-                            gwh_sincos_cw<15>(Phi + C.det[d][3] * f, s, c);
-                        } else {
-                            float sf, cf;
-                            sincos_turns(Phi + C.det[d][3] * f, sf, cf);
-                            s = sf;
-                            c = cf;
-                        }
+                        if (PREC == GWH_FP64)
+                            gwh_sincos_table(Phi + C.det[d][3] * f, s, c);
+                        else
+                            gwh_sincos_table(Phi + C.det[d][3] * f, s, c);
                         const double ws = gwh_g(E.wS)[(size_t)d * N + k];
                         const double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
                         acc[3 * d] += A * A * (Gr * Gr + Gi * Gi) * ws;
@@ -1376,7 +1376,7 @@ __global__ void gwh_waveform_kernel(const double *__restrict__ f, int N, const I
             double A, phi, u, lx;
             gwh_phenomd_fp64(C, fk, bp, A, phi, u, lx);
             double s, c;
-            gwh_sincos_cw<15>(phi, s, c);
+            gwh_sincos_table(phi, s, c);
             double hr = A * c, hi = -A * s; /* h = A e^{-i phi} */
             hp = make_double2(hr * C.wf_a, hi * C.wf_a);
             hc = make_double2(hi * C.wf_b, -hr * C.wf_b); /* -i h cos(iota) */
@@ -1384,7 +1384,7 @@ __global__ void gwh_waveform_kernel(const double *__restrict__ f, int N, const I
             double A, Phi, hpr, hpi, hcr, hci;
             gwh_pv2_fp64(C, fk, bp, A, Phi, hpr, hpi, hcr, hci);
             double s, c;
-            gwh_sincos_cw<15>(Phi, s, c);
+            gwh_sincos_table(Phi, s, c);
             /* (A hpf) e^{-i Phi} */
             double pr = A * (hpr * c + hpi * s), pi = A * (hpi * c - hpr * s);
             double xr = A * (hcr * c + hci * s), xi = A * (hci * c - hcr * s);
@@ -1413,7 +1413,7 @@ __global__ void gwh_waveform_kernel(const double *__restrict__ f, int N, const I
         if (mode == 2) {
             /* time-of-flight phase e^{-2 pi i f dt} relative to the first detector (waveform_util.cpp:170-179) */
             double s, c;
-            gwh_sincos_cw<15>(-C.det[det][3] * fk, s, c);
+            gwh_sincos_table(-C.det[det][3] * fk, s, c);
             hp = make_double2(hp.x * c - hp.y * s, hp.x * s + hp.y * c);
         }
         out1[k] = hp;

@@ -241,7 +241,6 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
         double Ac = 0, u, lx, ph;
         if (f <= Cc.fcut) gwh_phenomd_fp64(Cc, f, bp, Ac, ph, u, lx);
         double sw = sqrt(E.wS[(size_t)a.d_first * N + k]);
-        T *r0 = base + (size_t)(2 * k) * GWH_FROWS;
         for (int i = 0; i < GWH_FROWS; i++) {
             double dA = 0, dP = 0;
             if (i < a.dim) {
@@ -261,8 +260,8 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
                 }
                 continue;
             }
-            r0[i] = (T)(dA * sw);
-            r0[GWH_FROWS + i] = (T)(a.sa_sign * dP * Ac * sw);
+            base[(size_t)i * N + k] = (T)(dA * sw);
+            base[(size_t)(GWH_FROWS + i) * N + k] = (T)(a.sa_sign * dP * Ac * sw);
         }
         return;
     }
@@ -284,7 +283,7 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
                         if (dd >= a.d_count) break;
                         int d = a.d_first + dd;
                         double sn, cs;
-                        gwh_sincos_cw<15>(phi + (tof ? Ci.det[d][3] : 0.0) * f, sn, cs);
+                        gwh_sincos_table(phi + (tof ? Ci.det[d][3] : 0.0) * f, sn, cs);
                         double er = A * cs, ei = -A * sn; /* A K e^{-i(phi + 2 pi dt f)} */
                         dr[dd] += c * (er * Ci.det[d][0] - ei * Ci.det[d][1]);
                         di[dd] += c * (er * Ci.det[d][1] + ei * Ci.det[d][0]);
@@ -298,7 +297,7 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
                         double Gr = Ci.det[d][0] * hpr + Ci.det[d][1] * hcr;
                         double Gi = Ci.det[d][0] * hpi + Ci.det[d][1] * hci;
                         double sn, cs;
-                        gwh_sincos_cw<15>(Phi + (tof ? Ci.det[d][3] : 0.0) * f, sn, cs);
+                        gwh_sincos_table(Phi + (tof ? Ci.det[d][3] : 0.0) * f, sn, cs);
                         dr[dd] += c * A * (Gr * cs + Gi * sn);
                         di[dd] += c * A * (Gi * cs - Gr * sn);
                     }
@@ -314,9 +313,9 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
         }
         for (int dd = 0; dd < GWH_MAXDET; dd++) {
             if (dd >= a.d_count) break;
-            T *r0 = base + ((size_t)dd * N + k) * 2 * GWH_FROWS;
-            r0[i] = (T)(dr[dd] * sw[dd]);
-            r0[GWH_FROWS + i] = (T)(di[dd] * sw[dd]);
+            T *pl = base + (size_t)dd * 2 * GWH_FROWS * N; /* planes (detector, re/im), component-major */
+            pl[(size_t)i * N + k] = (T)(dr[dd] * sw[dd]);
+            pl[(size_t)(GWH_FROWS + i) * N + k] = (T)(di[dd] * sw[dd]);
         }
     }
 }
@@ -325,35 +324,49 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_fisher_stencil_kernel(StencilArgs
 /* device: Gram on the matrix cores                                           */
 /* ------------------------------------------------------------------------- */
 
-/* Each block: GWH_GRAM_ROWS operand rows of one point, 4 waves; each wave accumulates a 16x16 tile of
- * a a^T over its rows, 4 rows per MFMA (lane l supplies a[row0 + l/16][l%16] as both A and B operand,
- * coalesced 512 B per wave per instruction).  Output: per block 256 FP64 partial sums. */
+/* The operand rows of a point lie in planes (detector, real / imaginary part), component-major: component i of
+ * bin k at plane[i * N + k], so the stencil kernels write whole waves of consecutive bins.  Each block: up to
+ * GWH_GRAM_ROWS bins of one plane of one point, 4 waves; each wave accumulates a 16x16 tile of a a^T over its
+ * bins, 4 bins per MFMA (lane l supplies component l%16 of bin k0 + l/16 as both A and B operand; eight
+ * MFMAs' operands are loaded first, 32 consecutive bins of each component).  Output: per block 256 FP64
+ * partial sums. */
 template <typename T>
-__global__ __launch_bounds__(GWH_TPB) void gwh_gram_kernel(const T *__restrict__ rows, size_t nrows,
-                                                           double *__restrict__ partial, int nblk)
+__global__ __launch_bounds__(GWH_TPB) void gwh_gram_kernel(const T *__restrict__ rows, int N, int nplanes,
+                                                           int blk_per_plane, double *__restrict__ partial, int nblk)
 {
     __shared__ double lds[GWH_TPB / 64][256];
     const int pt = blockIdx.y, blk = blockIdx.x;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const T *a = rows + (size_t)pt * nrows * GWH_FROWS;
-    size_t r0 = (size_t)blk * GWH_GRAM_ROWS + (size_t)wv * (GWH_GRAM_ROWS / 4);
-    size_t r1 = r0 + GWH_GRAM_ROWS / 4;
-    if (r1 > nrows) r1 = nrows;
+    const int plane = blk / blk_per_plane;
+    const T *a = rows + ((size_t)pt * nplanes + plane) * GWH_FROWS * N + (size_t)(lane & 15) * N;
+    int r0 = (blk % blk_per_plane) * GWH_GRAM_ROWS + wv * (GWH_GRAM_ROWS / 4);
+    int r1 = r0 + GWH_GRAM_ROWS / 4;
+    if (r1 > N) r1 = N;
     double out[4];
     if (sizeof(T) == 8) {
         gwh_d4 acc = {0, 0, 0, 0};
-        for (size_t r = r0; r < r1; r += 4) {
-            size_t rr = r + (lane >> 4);
-            double x = (rr < r1) ? (double)a[rr * GWH_FROWS + (lane & 15)] : 0.0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, acc, 0, 0, 0);
+        for (int r = r0; r < r1; r += 32) {
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int rr = r + 4 * u + (lane >> 4);
+                x[u] = (rr < r1) ? (double)a[rr] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x[u], x[u], acc, 0, 0, 0);
         }
         for (int j = 0; j < 4; j++) out[j] = acc[j];
     } else {
         gwh_f4 acc = {0, 0, 0, 0};
-        for (size_t r = r0; r < r1; r += 4) {
-            size_t rr = r + (lane >> 4);
-            float x = (rr < r1) ? (float)a[rr * GWH_FROWS + (lane & 15)] : 0.0f;
-            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x, x, acc, 0, 0, 0);
+        for (int r = r0; r < r1; r += 32) {
+            float x[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int rr = r + 4 * u + (lane >> 4);
+                x[u] = (rr < r1) ? (float)a[rr] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x[u], x[u], acc, 0, 0, 0);
         }
         for (int j = 0; j < 4; j++) out[j] = acc[j];
     }
@@ -485,16 +498,22 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
     int chunk = (int)std::max<size_t>(1, F.max_operand / per_pt_bytes);
     if (chunk > n) chunk = n;
     if (chunk > 65535) chunk = 65535;
-    int nblk = (int)((nrows + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS);
+    const int nplanes = (int)(nrows / N); /* (detector, re/im) planes of a point's operand rows */
+    const int blk_per_plane = (N + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS;
+    int nblk = nplanes * blk_per_plane;
     int rc;
     if ((rc = F.deriv.ensure((size_t)chunk * per_pt_bytes)) != GWH_OK) return rc;
     if ((rc = F.gram.ensure((size_t)chunk * nblk * 256 * 8)) != GWH_OK) return rc;
     if ((rc = F.out.ensure((size_t)n * dim * dim * 8)) != GWH_OK) return rc;
-    for (int p0 = 0; p0 < n; p0 += chunk) {
+    std::vector<gwh_params> inst;
+    std::vector<unsigned char> no_tof;
+    std::vector<double> cw;
+    int slot = 0;
+    for (int p0 = 0; p0 < n; p0 += chunk, slot ^= 1) {
         int np = std::min(chunk, n - p0);
-        std::vector<gwh_params> inst;
-        std::vector<unsigned char> no_tof;
-        std::vector<double> cw((size_t)np * GWH_FROWS * GWH_MAXSTEN, 0.0);
+        inst.clear();
+        no_tof.clear();
+        cw.assign((size_t)np * GWH_FROWS * GWH_MAXSTEN, 0.0);
         inst.reserve((size_t)np * per_point);
         for (int q = 0; q < np; q++) {
             PointStencil ps;
@@ -511,11 +530,25 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
         if ((rc = F.params.ensure((size_t)ni * sizeof(gwh_params))) != GWH_OK) return rc;
         if ((rc = F.coef.ensure((size_t)ni * sizeof(InstCoef))) != GWH_OK) return rc;
         if ((rc = F.misc.ensure(cw.size() * 8 + no_tof.size() + 256)) != GWH_OK) return rc;
-        GWH_HIP(hipMemcpyAsync(F.params.p, inst.data(), (size_t)ni * sizeof(gwh_params), hipMemcpyHostToDevice,
-                               F.stream));
+        /* stage the pass in pinned memory (asynchronous copies; the device buffers are reused in stream order) */
+        const size_t rec_bytes = (size_t)ni * sizeof(gwh_params), stage_bytes = rec_bytes + cw.size() * 8 + no_tof.size();
+        if (F.stage_done[slot]) GWH_HIP(hipEventSynchronize(F.stage_done[slot])); /* its last copy has run */
+        else GWH_HIP(hipEventCreateWithFlags(&F.stage_done[slot], hipEventDisableTiming));
+        if (F.stage_cap[slot] < stage_bytes) {
+            if (F.stage[slot]) GWH_HIP(hipHostFree(F.stage[slot]));
+            F.stage[slot] = nullptr;
+            F.stage_cap[slot] = 0;
+            GWH_HIP(hipHostMalloc(&F.stage[slot], stage_bytes + stage_bytes / 4, hipHostMallocDefault));
+            F.stage_cap[slot] = stage_bytes + stage_bytes / 4;
+        }
+        char *st = (char *)F.stage[slot];
+        memcpy(st, inst.data(), rec_bytes);
+        memcpy(st + rec_bytes, cw.data(), cw.size() * 8);
+        memcpy(st + rec_bytes + cw.size() * 8, no_tof.data(), no_tof.size());
+        GWH_HIP(hipMemcpyAsync(F.params.p, st, rec_bytes, hipMemcpyHostToDevice, F.stream));
         char *misc = (char *)F.misc.p;
-        GWH_HIP(hipMemcpyAsync(misc, cw.data(), cw.size() * 8, hipMemcpyHostToDevice, F.stream));
-        GWH_HIP(hipMemcpyAsync(misc + cw.size() * 8, no_tof.data(), no_tof.size(), hipMemcpyHostToDevice, F.stream));
+        GWH_HIP(hipMemcpyAsync(misc, st + rec_bytes, cw.size() * 8 + no_tof.size(), hipMemcpyHostToDevice, F.stream));
+        GWH_HIP(hipEventRecord(F.stage_done[slot], F.stream));
         if ((rc = gwh_setup_launch(nullptr, event, model, F.params.as<const gwh_params>(), ni, F.coef.as<InstCoef>(),
                                    F.stream)) != GWH_OK)
             return rc;
@@ -559,7 +592,7 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
                                    F.stream, a, evp, event, F.deriv.as<double>());
             GWH_HIP(hipGetLastError());
             hipLaunchKernelGGL(gwh_gram_kernel<double>, dim3(nblk, np), dim3(GWH_TPB), 0, F.stream,
-                               F.deriv.as<const double>(), nrows, F.gram.as<double>(), nblk);
+                               F.deriv.as<const double>(), N, nplanes, blk_per_plane, F.gram.as<double>(), nblk);
         } else {
             if (!pv2)
                 hipLaunchKernelGGL((gwh_fisher_stencil_kernel<GWH_IMRPHENOMD, float>), g, dim3(GWH_TPB), 0, F.stream,
@@ -569,14 +602,12 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
                                    F.stream, a, evp, event, F.deriv.as<float>());
             GWH_HIP(hipGetLastError());
             hipLaunchKernelGGL(gwh_gram_kernel<float>, dim3(nblk, np), dim3(GWH_TPB), 0, F.stream,
-                               F.deriv.as<const float>(), nrows, F.gram.as<double>(), nblk);
+                               F.deriv.as<const float>(), N, nplanes, blk_per_plane, F.gram.as<double>(), nblk);
         }
         GWH_HIP(hipGetLastError());
         hipLaunchKernelGGL(gwh_gram_reduce_kernel, dim3(np), dim3(256), 0, F.stream, F.gram.as<const double>(),
                            nblk, dim, scale, F.out.as<double>() + (size_t)p0 * dim * dim);
         GWH_HIP(hipGetLastError());
-        /* the host staging vectors of this chunk die with the iteration */
-        GWH_HIP(hipStreamSynchronize(F.stream));
     }
     if (sigma) {
         if ((rc = F.sig.ensure((size_t)n * dim * 8)) != GWH_OK) return rc;
@@ -639,7 +670,7 @@ size_t gwh_fisher_batch_bytes(int event, int model, int basis, int precision, co
     const size_t per_pt_bytes = nrows * GWH_FROWS * ((precision == GWH_FISHER_GRAM_FP32) ? 4 : 8);
     size_t chunk = std::max<size_t>(1, max_operand / per_pt_bytes);
     chunk = std::min<size_t>(chunk, std::min(n, 65535));
-    const size_t nblk = (nrows + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS;
+    const size_t nblk = (nrows / ev.N) * ((ev.N + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS);
     const size_t ni = chunk * per_point;
     return gwh_grown_capacity(chunk * per_pt_bytes, F.deriv.cap) +
            gwh_grown_capacity(chunk * nblk * 256 * 8, F.gram.cap) +

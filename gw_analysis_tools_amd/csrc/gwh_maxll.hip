@@ -78,13 +78,13 @@ __global__ __launch_bounds__(GWH_MX_TPB) void gwh_maxll_fill_kernel(const EventD
                 double A, phi, u, lx;
                 gwh_phenomd_fp64(C, f, bp, A, phi, u, lx);
                 double s, c;
-                gwh_sincos_cw<15>(phi, s, c);
+                gwh_sincos_table(phi, s, c);
                 hp = make_double2(A * c * C.wf_a, -A * s * C.wf_a); /* h+ = A e^{-i phi} (1 + cos^2 i) / 2 */
             } else {
                 double A, Phi, hpr, hpi, hcr, hci;
                 gwh_pv2_fp64(C, f, bp, A, Phi, hpr, hpi, hcr, hci);
                 double s, c;
-                gwh_sincos_cw<15>(Phi, s, c);
+                gwh_sincos_table(Phi, s, c);
                 const double pr = A * (hpr * c + hpi * s), pi = A * (hpi * c - hpr * s);
                 const double xr = A * (hcr * c + hci * s), xi = A * (hci * c - hcr * s);
                 const double c2z = C.wf_a, s2z = C.wf_b;

@@ -1134,7 +1134,7 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, co
         ar = pn + nr;
     } else if (f > C.f3) {
         double dl = f - C.fRD;
-        ar = C.mr_k * exp(-C.mr_g2 * dl) / (dl * dl + C.mr_w2);
+        ar = C.mr_k * gwh_exp_table(-C.mr_g2 * dl) / (dl * dl + C.mr_w2);
     } else {
         ar = C.delta[0] + Mf * (C.delta[1] + Mf * (C.delta[2] + Mf * (C.delta[3] + Mf * C.delta[4])));
     }
@@ -1184,7 +1184,7 @@ __device__ __forceinline__ void gwh_pv2_fp64(const InstCoef &C, double f, const 
     double alpha = gwh_euler_eval(C.a_c, v, logw) + C.alpha_add;
     double eps = gwh_euler_eval(C.e_c, v, logw) + C.eps_add;
     double sa, ca;
-    gwh_sincos_cw<15>(alpha, sa, ca);
+    gwh_sincos_table(alpha, sa, ca);
     /* e^{i m alpha}, m=-2..2 */
     double er[5], ei[5];
     er[2] = 1;

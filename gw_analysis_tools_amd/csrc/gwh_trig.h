@@ -8,11 +8,17 @@
  * fused multiply-add (x - q p1 is exact by Sterbenz's lemma, the neglected part of pi/2 is 1e-33, so r is
  * within half an ulp for |x| < 2^30), then sin r and cos r on [-pi/4, pi/4] by their Taylor series and the
  * quadrant by q mod 4.  Arguments beyond 2^30 rad take the library routine.
+ *
+ * gwh_sincos_table and gwh_exp_table trade the long series for a table lookup (gwh_trig_tables.h, 4.5 kB,
+ * correctly rounded entries) and a short series on a tiny remainder: ~20 FP64 operations and one gather for
+ * sin and cos together.  Any index they form stays inside the tables, whatever the argument (NaN included).
  */
 #ifndef GWH_TRIG_H
 #define GWH_TRIG_H
 
 #include <hip/hip_runtime.h>
+
+#include "gwh_trig_tables.h"
 
 /* NT: highest odd power of the sine series (15: truncation below 5e-17 on [-pi/4, pi/4]; 11: below 7e-12) */
 template <int NT> __device__ __forceinline__ void gwh_sincos_cw(double x, double &s, double &c)
@@ -52,6 +58,41 @@ template <int NT> __device__ __forceinline__ void gwh_sincos_cw(double x, double
     const double ss = (iq & 1) ? cr : sr, cc = (iq & 1) ? sr : cr;
     s = (iq & 2) ? -ss : ss;
     c = ((iq + 1) & 2) ? -cc : cc;
+}
+
+/* sin and cos of an FP64 phase through the table: the phase in turns r = n / 256 + y with n the nearest integer
+ * to 256 r (rounded by the 1.5 * 2^52 addition, whose low mantissa bits then hold n mod 256) and y = r - n / 256
+ * exact, |y| <= 1/512; sin and cos of t = 2 pi y (|t| <= pi / 256) by their Taylor series to t^5 and t^6
+ * (truncation below 1e-17), rotated by the table entry.  Error a few ulp of 1 beyond the rounding of the phase
+ * to turns (the same reduction the FP32 path takes); valid for |phase| < 2^44 rad. */
+__device__ __forceinline__ void gwh_sincos_table(double phase, double &s, double &c)
+{
+    const double r = phase * 0.15915494309189533577;
+    const double kRound = 6755399441055744.0; /* 1.5 * 2^52 */
+    const double rb = fma(r, 256.0, kRound);
+    const double n = rb - kRound;
+    const int j = (int)(__double_as_longlong(rb) & 255);
+    const double y = fma(n, -0.00390625, r);
+    const double t = y * 6.283185307179586477;
+    const double t2 = t * t;
+    const double sl = t * fma(t2, fma(t2, 1.0 / 120.0, -1.0 / 6.0), 1.0);
+    const double cl = fma(t2, fma(t2, fma(t2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+    const double2 T = gwh_turn_table[j];
+    s = fma(T.y, cl, T.x * sl);
+    c = fma(T.x, cl, -T.y * sl);
+}
+
+/* e^x for |x| < 700 through the 2^(j/64) table: x = n ln2 / 64 + r with n = rint(64 x / ln2) and the reduction
+ * by a two-part ln2 / 64 (r exact to ~1e-19, |r| <= ln2 / 128), e^r by its Taylor series to r^5 (truncation
+ * below 4e-17), scaled by 2^(n mod 64 / 64) from the table and 2^(n div 64) exactly.  Error a few ulp. */
+__device__ __forceinline__ double gwh_exp_table(double x)
+{
+    const double n = rint(x * 92.332482616893656877); /* 64 / ln 2 */
+    double r = fma(-n, 0.01083042469326756, x);        /* ln2/64, high 32 bits (n * hi exact for |n| < 2^21) */
+    r = fma(-n, 2.9815858269852933e-12, r);            /* ln2/64, the rest */
+    const double p = fma(r, fma(r, fma(r, fma(r, fma(r, 1.0 / 120.0, 1.0 / 24.0), 1.0 / 6.0), 0.5), 1.0), 1.0);
+    const int ni = (int)n;
+    return ldexp(gwh_exp2_table[ni & 63] * p, ni >> 6);
 }
 
 #endif
