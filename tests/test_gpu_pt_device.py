@@ -224,3 +224,27 @@ def test_loglike_graph_multi_event():
         a.free()
     for e in evs:
         e.close()
+
+
+def test_device_pt_full_c1_shape_identical_decisions():
+    """The benchmark's tempering run at its full shape: 16 temperatures x 32 chains on the 16225-bin H/L/V event,
+    2000 steps, ladder swaps every 10th step, 10 steps per graph replay.  The mixed engine with its FP64 guard band
+    must take exactly the FP64 run's decisions: equal acceptance and swap counts per chain and a bit-identical
+    final state (parameters), with logL within the mixed tolerance."""
+    ev, inj, temps = _c1_event()
+    th0 = pt.theta_from_params(inj)
+    runs = {}
+    for prec in (G.FP64, G.MIXED):
+        d = G.DevicePT(ev, inj, th0, WIDTHS, temps, 32, pt.LOWER, pt.UPPER, precision=prec, seed=2026,
+                       swap_every=10, steps_per_graph=10)
+        d.run(2000)
+        runs[prec] = (d.state(), d.counters())
+        d.close()
+    (sa, ca), (sb, cb) = runs[G.FP64], runs[G.MIXED]
+    print("2000 steps x 512 chains: accepts %d, swaps %d, guard FP64 evaluations %d"
+          % (ca[0].sum(), ca[1].sum(), cb[2]))
+    assert np.array_equal(ca[0], cb[0]) and np.array_equal(ca[1], cb[1])
+    assert np.array_equal(sa[0], sb[0])
+    assert np.all(np.abs(sa[1] - sb[1]) <= 1e-3)
+    assert ca[0].sum() > 1000 and ca[1].sum() > 0
+    ev.close()
