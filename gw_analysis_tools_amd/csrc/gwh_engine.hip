@@ -492,38 +492,6 @@ __global__ __launch_bounds__(256) void gwh_setup_kernel(const gwh_params *__rest
             }
             const EventDesc &E = events[ev];
             gwh_setup_detectors(p, pv2, zeta, geo, E.det, E.D, *C);
-            if (active) {
-                /* tiles to visit: all of them, or on an ascending grid those starting at or below 0.2/M */
-                const double fcut = .2 / S.M;
-                const int tile = GWH_TPB * E.bpl;
-                int nt = (E.N + tile - 1) / tile;
-                const int cpt = E.bpl; /* 256-bin chunks per tile */
-                if (E.sorted) {
-                    /* the first tile whose smallest frequency exceeds the cutoff; up to 16 tiles the count of
-                     * tiles at or below it (same answer on an ascending grid), from loads issued together */
-                    int lo = 0;
-                    if (nt <= 0) {
-                        lo = 0;
-                    } else if (nt <= 16) {
-                        double fm[16];
-#pragma unroll
-                        for (int t = 0; t < 16; t++) fm[t] = E.chunk_fmin[(t < nt ? t : nt - 1) * cpt];
-#pragma unroll
-                        for (int t = 0; t < 16; t++) lo += (t < nt && !(fm[t] > fcut));
-                    } else {
-                        int hi = nt;
-                        while (lo < hi) {
-                            int mid = (lo + hi) >> 1;
-                            if (E.chunk_fmin[mid * cpt] > fcut)
-                                hi = mid;
-                            else
-                                lo = mid + 1;
-                        }
-                    }
-                    nt = (fcut == fcut) ? lo : nt; /* NaN cutoff: visit everything, NaN propagates */
-                }
-                active[i] = nt;
-            }
         } else {
             X.alpha1_fit = gwh_setup_phase_fits(p, &gwh_phenomd_fit[0][0], S);
             {
@@ -577,6 +545,43 @@ __global__ __launch_bounds__(256) void gwh_setup_kernel(const gwh_params *__rest
                 double lw = log(vr * vr * vr);
                 eul_a = gwh_euler_eval(a_c, vr, lw);
                 eul_e = gwh_euler_eval(e_c, vr, lw);
+            }
+        }
+        /* the tile count (a dependent chain of loads) on a role with a short chain: the detectors role for
+           IMRPhenomD, the ringdown role for IMRPhenomPv2, whose frame makes the detectors role the longest */
+        if (role == (pv2 ? 0 : 2)) {
+            if (active) {
+                const EventDesc &E = events[ev];
+                /* tiles to visit: all of them, or on an ascending grid those starting at or below 0.2/M */
+                const double fcut = .2 / S.M;
+                const int tile = GWH_TPB * E.bpl;
+                int nt = (E.N + tile - 1) / tile;
+                const int cpt = E.bpl; /* 256-bin chunks per tile */
+                if (E.sorted) {
+                    /* the first tile whose smallest frequency exceeds the cutoff; up to 16 tiles the count of
+                     * tiles at or below it (same answer on an ascending grid), from loads issued together */
+                    int lo = 0;
+                    if (nt <= 0) {
+                        lo = 0;
+                    } else if (nt <= 16) {
+                        double fm[16];
+#pragma unroll
+                        for (int t = 0; t < 16; t++) fm[t] = E.chunk_fmin[(t < nt ? t : nt - 1) * cpt];
+#pragma unroll
+                        for (int t = 0; t < 16; t++) lo += (t < nt && !(fm[t] > fcut));
+                    } else {
+                        int hi = nt;
+                        while (lo < hi) {
+                            int mid = (lo + hi) >> 1;
+                            if (E.chunk_fmin[mid * cpt] > fcut)
+                                hi = mid;
+                            else
+                                lo = mid + 1;
+                        }
+                    }
+                    nt = (fcut == fcut) ? lo : nt; /* NaN cutoff: visit everything, NaN propagates */
+                }
+                active[i] = nt;
             }
         }
     }

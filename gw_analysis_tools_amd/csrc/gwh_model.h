@@ -598,19 +598,6 @@ __device__ __forceinline__ void gwh_setup_amp_mr(const SetupState &S, double f, 
 }
 
 /* ---- Pv2 precession helpers (IMRPhenomP.cpp) ---- */
-__device__ __forceinline__ void gwh_rotz(double a, double &x, double &y)
-{
-    double t1 = x * cos(a) - y * sin(a), t2 = x * sin(a) + y * cos(a);
-    x = t1;
-    y = t2;
-}
-__device__ __forceinline__ void gwh_roty(double a, double &x, double &z)
-{
-    double t1 = x * cos(a) + z * sin(a), t2 = -x * sin(a) + z * cos(a);
-    x = t1;
-    z = t2;
-}
-
 __device__ __forceinline__ double gwh_l2pn(double eta, double x)
 {
     double eta2 = eta * eta;
@@ -898,6 +885,21 @@ __device__ __forceinline__ void gwh_setup_mass(const gwh_params &p, int ppe, Set
 struct Pv2Frame {
     double thetaJN, alpha0, zeta, phi_aligned;
 };
+/* rotation by an angle given as its cosine and sine (the frame applies each of its three rotations several
+ * times: one sincos per angle) */
+__device__ __forceinline__ void gwh_rotz_cs(double c, double s, double &x, double &y)
+{
+    double t1 = x * c - y * s, t2 = x * s + y * c;
+    x = t1;
+    y = t2;
+}
+__device__ __forceinline__ void gwh_roty_cs(double c, double s, double &x, double &z)
+{
+    double t1 = x * c + z * s, t2 = -x * s + z * c;
+    x = t1;
+    z = t2;
+}
+
 __device__ __forceinline__ Pv2Frame gwh_setup_frame(const gwh_params &p, const SetupState &S, double chip)
 {
     const double s1x = p.spin1[0], s1y = p.spin1[1], s2x = p.spin2[0], s2y = p.spin2[1];
@@ -907,8 +909,10 @@ __device__ __forceinline__ Pv2Frame gwh_setup_frame(const gwh_params &p, const S
     double L0 = S.M * S.M * gwh_l2pn(S.eta, xr * xr);
     double J0x, J0y;
     if (reduced) {
-        J0x = m1_2 * chip * cos(p.phip);
-        J0y = m1_2 * chip * sin(p.phip);
+        double sp, cp;
+        sincos(p.phip, &sp, &cp);
+        J0x = m1_2 * chip * cp;
+        J0y = m1_2 * chip * sp;
     } else {
         J0x = m1_2 * s1x + m2_2 * s2x;
         J0y = m1_2 * s1y + m2_2 * s2y;
@@ -916,35 +920,46 @@ __device__ __forceinline__ Pv2Frame gwh_setup_frame(const gwh_params &p, const S
     double J0z = L0 + m1_2 * S.chi1 + m2_2 * S.chi2;
     double J0 = sqrt(J0x * J0x + J0y * J0y + J0z * J0z);
     double thetaJ = acos(J0z / J0), phiJ = atan2(J0y, J0x);
+    double sJ, cJ, sP, cP; /* rotations by -thetaJ and -phiJ */
+    sincos(thetaJ, &sJ, &cJ);
+    sincos(phiJ, &sP, &cP);
+    sJ = -sJ;
+    sP = -sP;
     Pv2Frame F;
     F.phi_aligned = -phiJ;
-    double incl = p.incl, phiRef = p.phiRef;
-    double Nx = sin(incl) * cos(GWH_PI / 2. - phiRef), Ny = sin(incl) * sin(GWH_PI / 2. - phiRef), Nz = cos(incl);
+    double si, ci, sr, cr;
+    sincos(p.incl, &si, &ci);
+    sincos(GWH_PI / 2. - p.phiRef, &sr, &cr);
+    double Nx = si * cr, Ny = si * sr, Nz = ci;
     double tx = Nx, ty = Ny, tz = Nz;
-    gwh_rotz(-phiJ, tx, ty);
-    gwh_roty(-thetaJ, tx, tz);
+    gwh_rotz_cs(cP, sP, tx, ty);
+    gwh_roty_cs(cJ, sJ, tx, tz);
     double kappa = -atan2(ty, tx);
+    double sK, cK;
+    sincos(kappa, &sK, &cK);
     tx = 0.;
     ty = 0.;
     tz = 1.;
-    gwh_rotz(-phiJ, tx, ty);
-    gwh_roty(-thetaJ, tx, tz);
-    gwh_rotz(kappa, tx, ty);
+    gwh_rotz_cs(cP, sP, tx, ty);
+    gwh_roty_cs(cJ, sJ, tx, tz);
+    gwh_rotz_cs(cK, sK, tx, ty);
     F.alpha0 = atan2(ty, tx);
     tx = Nx;
     ty = Ny;
     tz = Nz;
-    gwh_rotz(-phiJ, tx, ty);
-    gwh_roty(-thetaJ, tx, tz);
-    gwh_rotz(kappa, tx, ty);
+    gwh_rotz_cs(cP, sP, tx, ty);
+    gwh_roty_cs(cJ, sJ, tx, tz);
+    gwh_rotz_cs(cK, sK, tx, ty);
     double NxJ = tx, NzJ = tz;
     F.thetaJN = acos(NzJ);
-    tx = -cos(incl) * sin(phiRef);
-    ty = -cos(incl) * cos(phiRef);
-    tz = sin(incl);
-    gwh_rotz(-phiJ, tx, ty);
-    gwh_roty(-thetaJ, tx, tz);
-    gwh_rotz(kappa, tx, ty);
+    double sph, cph;
+    sincos(p.phiRef, &sph, &cph);
+    tx = -ci * sph;
+    ty = -ci * cph;
+    tz = si;
+    gwh_rotz_cs(cP, sP, tx, ty);
+    gwh_roty_cs(cJ, sJ, tx, tz);
+    gwh_rotz_cs(cK, sK, tx, ty);
     F.zeta = atan2(tx * NzJ - tz * NxJ, -ty);
     return F;
 }
