@@ -8,8 +8,9 @@ and the event (data, PSD, weights) are resident in HBM before the timed region; 
 enqueue of the batched engine followed by a stream synchronisation (a sampler reads its logL every
 step), so per-step latencies give p50/p99.
 
-`value` is the mixed-precision batched engine (FP64 phase, amplitude, products and sums; FP32 sin/cos
-of the detector phases and FP32 merger-ringdown exponential).  Reported beside it: the FP64 batched engine, the
+`value` is the mixed-precision batched engine (FP64 except the merger-ringdown exponential, in FP32; sin/cos
+of the detector phases through the FP64 tables), on the tiling the autotuner picks for the event before the
+warm-up.  Reported beside it: the FP64 batched engine, the
 plain FP64 one-instance-per-launch path, max |delta logL| mixed vs FP64 over every timed evaluation, the same for
 IMRPhenomPv2 on the same grid (1024 chains; its mixed mode keeps FP64 table-driven sin/cos and exponentials),
 kernel-level roofline from HIP events, the single-detector workload with its sky-averaged Fisher, batched
@@ -62,7 +63,8 @@ def fe_per_launch(counts, ndet, pv2=False):
 
 def bench_uneven(G, W, steps):
     """256 events (total mass 5-200 Msun, durations 4-128 s, 20 Hz to min(2048, 0.3/M) Hz at df = 1/T, H/L/V),
-    8 chains each = 2048 instances per launch; the autotuner picks bins per lane per size bucket first."""
+    8 chains each = 2048 instances per launch; the autotuner picks the tiling (lanes per tile x bins per lane) per
+    size bucket first."""
     lib = G.lib()
     ip = ctypes.POINTER(ctypes.c_int)
     specs = W.config2()
@@ -88,10 +90,8 @@ def bench_uneven(G, W, steps):
     d_params, d_ids = G.DeviceArray.from_host(P), G.DeviceArray.from_host(ids)
     d_mixed, d_fp64 = G.DeviceArray(n * 8), G.DeviceArray(n * 8)
     resident = sum(ev.nbytes for ev in events)
-    table = np.zeros(32, dtype=np.int32)
     t0 = time.perf_counter()
-    G.check(lib.gwh_autotune_bins_per_lane(ids.ctypes.data_as(ip), d_ids.ptr, G.IMRPHENOMD, G.MIXED, d_params.ptr, n,
-                                           d_mixed.ptr, 3, table.ctypes.data_as(ip)))
+    tiles = G.autotune_tiles(ids, d_ids, G.IMRPHENOMD, G.MIXED, d_params, d_mixed, reps=3)
     t_tune = time.perf_counter() - t0
 
     def launch(prec, out, idx=None):
@@ -135,14 +135,14 @@ def bench_uneven(G, W, steps):
             G.synchronize()
         el = (time.perf_counter() - a) / reps
         buckets[int(b)] = {"instances": int(len(sel)), "bins_per_instance_mean": float(nbins[sel].mean()),
-                           "bins_per_lane": int(table[b]), "ms_per_launch": el * 1e3,
+                           "bins_per_lane": tiles[int(b)][0], "lanes": tiles[int(b)][1], "ms_per_launch": el * 1e3,
                            "bin_evals_per_s": float(nbins[sel].sum() / el)}
         for d in (dpb, dib, dob):
             d.free()
     rec = {"instances_per_launch": n, "events": len(events), "bins_min": int(nbins.min()),
            "bins_max": int(nbins.max()), "bins_total_per_launch": int(nbins.sum()),
            "resident_event_bytes": int(resident), "setup_s": t_setup, "autotune_s": t_tune,
-           "autotuned_bins_per_lane": {int(i): int(table[i]) for i in range(32) if table[i]},
+           "autotuned_tiling": {int(i): {"bins_per_lane": b, "lanes": l} for i, (b, l) in tiles.items()},
            "mixed_evals_per_s": steps * n / wall, "mixed_bin_evals_per_s": steps * float(nbins.sum()) / wall,
            "ms_per_launch": wall / steps * 1e3, "p50_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
            "p99_ms": float(np.percentile(np.array(lat) * 1e3, 99)), "fp64_evals_per_s": fp64_rate,
@@ -154,6 +154,7 @@ def bench_uneven(G, W, steps):
     for ev in events:
         ev.close()
     G.set_bins_per_lane(0)
+    G.set_tile_lanes(0)
     return rec
 
 
@@ -576,7 +577,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--bpl", type=int, default=0)
+    ap.add_argument("--bpl", type=int, default=0, help="bins per lane (0: autotuned)")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per tile (0: autotuned)")
     ap.add_argument("--legs", choices=["all", "mixed"], default="all",
                     help="mixed: only the timed mixed-precision leg (for counter runs)")
     ap.add_argument("--no-c2", action="store_true", help="skip the uneven multi-event batch leg")
@@ -601,6 +603,8 @@ def main():
     G.check(G.lib().gwh_init(0))
     if args.bpl:
         G.set_bins_per_lane(args.bpl)
+    if args.lanes:
+        G.set_tile_lanes(args.lanes)
     lib = G.lib()
     cfg = W.config1()
     f, psd, dets = cfg["f"], cfg["psd"], cfg["dets"]
@@ -623,6 +627,21 @@ def main():
     def enqueue(s, prec, out, model=G.IMRPHENOMD, dev_params=d_params, nn=n):
         G.check(lib.gwh_loglike_batch_device(None, ev.id, model, prec, dev_params.ptr + s * nn * psz, nn,
                                              out.ptr + s * nn * 8, None))
+
+    # tiling of the c1 event (lanes per tile x bins per lane) by the autotuner, before the warm-up
+    t0 = time.perf_counter()
+    tiling = {"autotuned": not (args.bpl or args.lanes)}
+    if tiling["autotuned"]:
+        ids1 = np.full(n, ev.id, dtype=np.int32)
+        d_ids1 = G.DeviceArray.from_host(ids1)
+        choice = G.autotune_tiles(ids1, d_ids1, G.IMRPHENOMD, G.MIXED, d_params, d_mixed, reps=5)
+        d_ids1.free()
+        (bpl_c1, lanes_c1), = choice.values()
+    else:
+        bpl_c1, lanes_c1 = args.bpl or 8, args.lanes or 256
+    tiling.update({"bins_per_lane": bpl_c1, "lanes": lanes_c1, "autotune_s": time.perf_counter() - t0})
+    log("c1 tiling: %d lanes x %d bins per lane per tile (%s)"
+        % (lanes_c1, bpl_c1, "autotuned" if tiling["autotuned"] else "set"))
 
     for s in range(Wm):
         enqueue(s, G.MIXED, d_mixed)
@@ -675,7 +694,8 @@ def main():
             traffic = None
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_VALU_TFLOPS, "traffic": traffic,
-                "kernel": "gwh_loglike_persist_kernel<IMRPhenomD, mixed, 8>", "kernel_ms": kernel_ms,
+                "kernel": "gwh_loglike_grid_kernel<IMRPhenomD, mixed, %d, %d>" % (bpl_c1, lanes_c1),
+                "kernel_ms": kernel_ms,
                 "fe_per_launch": fe, "hbm_algorithmic_GBs": bytes_alg / (kernel_ms * 1e-3) / 1e9,
                 "hbm_peak_GBs": PEAK_HBM_GBS}
     if args.copy_calibration_mib:
@@ -690,7 +710,7 @@ def main():
         print(json.dumps({"metric": METRIC, "value": value, "unit": "logL evals/s", "n_gpus": 1, "steps": K,
                           "warmup": Wm, "ms_per_step": wall / K * 1e3, "legs": "mixed only",
                           "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
-                          "roofline": roofline}))
+                          "tiling": tiling, "roofline": roofline}))
         return
     # ---- FP64 batched on the same batches (untimed for value) ----
     tf0 = time.perf_counter()
@@ -844,7 +864,7 @@ def main():
     rec = {
         "metric": METRIC, "value": value, "unit": "logL evals/s", "n_gpus": 1, "steps": K, "warmup": Wm,
         "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 with f32 sin/cos and exp (mixed)", "data": "synthetic",
+        "dtype": "f64 (mixed: f32 merger-ringdown exp)", "data": "synthetic",
         "config": {"workload": "PT ensemble step: IMRPhenomD, H1/L1/V1, 16225 bins 20-2048 Hz df=1/8 Hz, "
                                "16 temperatures x 32 chains = 512 instances per step",
                    "instances_per_step": n, "bins": len(f), "detectors": len(dets)},
@@ -852,6 +872,7 @@ def main():
         "fp64_batched_evals_per_s": fp64_rate, "fp64_one_instance_per_launch_evals_per_s": single_rate,
         "speedup_mixed_vs_fp64_one_instance_per_launch": value / single_rate,
         "max_abs_dlogL_mixed_vs_fp64": dmax,
+        "tiling": tiling,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "imrphenompv2_c3_likelihood": pv2,

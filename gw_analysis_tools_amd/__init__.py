@@ -58,7 +58,8 @@ EXPORTS = [
     "gwh_kernel_timing_start", "gwh_kernel_timing_every", "gwh_kernel_timing_stop", "gwh_device_copy",
     "gwh_fisher_derivatives", "gwh_fisher_numerical_model", "gwh_fisher_derivatives_model",
     "gwh_unpack_parameters", "gwh_repack_parameters", "gwh_detector_id", "gwh_dtoa",
-    "gwh_gps_to_gmst_radian", "gwh_loglike_multi_device", "gwh_autotune_bins_per_lane",
+    "gwh_gps_to_gmst_radian", "gwh_loglike_multi_device", "gwh_autotune_bins_per_lane", "gwh_autotune_tiles",
+    "gwh_set_tile_lanes",
     "gwh_calculate_snr_internal", "gwh_data_snr", "gwh_snr_batch",
     "gwh_broker_create", "gwh_broker_destroy", "gwh_broker_submit_loglike", "gwh_broker_submit_fisher",
     "gwh_broker_wait", "gwh_broker_stats",
@@ -189,6 +190,8 @@ def lib():
         "gwh_gps_to_gmst_radian": ([D], D),
         "gwh_loglike_multi_device": ([ip, V, I, I, V, I, V, V], I),
         "gwh_autotune_bins_per_lane": ([ip, V, I, I, V, I, V, I, ip], I),
+        "gwh_autotune_tiles": ([ip, V, I, I, V, I, V, I, ip, ip], I),
+        "gwh_set_tile_lanes": ([I], I),
         "gwh_calculate_snr_internal": ([dp, dp, dp, I, I, dp, I], D),
         "gwh_data_snr": ([dp, I, dp, dp, dp], D),
         "gwh_snr_batch": ([ip, I, I, I, V, I, dp], I),

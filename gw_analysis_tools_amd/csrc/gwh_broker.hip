@@ -163,7 +163,7 @@ size_t lane_bytes_after(const Lane &L, const std::vector<int> &events, int n)
     size_t maxtiles = 0, tiles = 0;
     for (int i = 0; i < n; i++) {
         const Event &ev = E.events[events[i]];
-        size_t t = (ev.N + GWH_TPB * ev.desc.bpl - 1) / (GWH_TPB * ev.desc.bpl);
+        size_t t = gwh_tiles_of(ev.desc);
         tiles += t;
         if (t > maxtiles) maxtiles = t;
     }
@@ -172,7 +172,7 @@ size_t lane_bytes_after(const Lane &L, const std::vector<int> &events, int n)
     return gwh_grown_capacity(m * sizeof(InstCoef), S.coef.cap) +
            gwh_grown_capacity(m * maxtiles * GWH_PARTIAL_STRIDE * sizeof(double), S.partial.cap) +
            gwh_grown_capacity(m * sizeof(int), S.active.cap) + gwh_grown_capacity(tiles * sizeof(int2), S.work.cap) +
-           gwh_grown_capacity(4 * sizeof(int), S.wcount.cap) +
+           gwh_grown_capacity(GWH_NCLASS * sizeof(int), S.wcount.cap) +
            gwh_grown_capacity(m * sizeof(gwh_params), L.d_params.cap) +
            gwh_grown_capacity(m * sizeof(int), L.d_events.cap) + gwh_grown_capacity(m * sizeof(double), L.d_logl.cap);
 }

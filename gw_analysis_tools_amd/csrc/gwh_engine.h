@@ -17,7 +17,9 @@
 
 #include "gwh_model.h"
 
-#define GWH_TPB 256          /* threads per block of the per-bin kernels (4 wave64) */
+#define GWH_TPB 256          /* threads per block of the auxiliary kernels; the default lanes per likelihood tile */
+#define GWH_NCLASS 20        /* tiling classes: bins per lane {1, 2, 4, 8} x lanes per tile {64, ..., 1024} */
+#define GWH_FMIN_LEVELS 8    /* frequency-minimum levels: chunks of 64 * 2^L bins, L = 0..7 (64..8192 bins) */
 #define GWH_MAX_EVENTS 4096
 #define GWH_PARTIAL_STRIDE (3 * GWH_MAXDET)
 
@@ -26,15 +28,31 @@ struct EventDesc {
     const double *f;        /* N */
     const double *wS;       /* D x N : integration weight * 4 / S  (FP64) */
     const double2 *dwS;     /* D x N : data * wS */
-    const double *chunk_fmin; /* ceil(N / GWH_TPB): smallest frequency of every 256-bin chunk */
+    const double *tile_fmin; /* smallest frequency of every chunk of 64 * 2^L bins, level L from fmin_off[L] */
     /* N each: log f, f^(1/3), f^(-1/3), f^(-7/6), f^(3/4) of every bin (gwh_bin_powers) */
     const double *lgf, *f13, *fm13, *fm76, *f34;
-    int N, D, nchunk, sorted; /* sorted: frequencies ascending */
+    int N, D, sorted; /* sorted: frequencies ascending */
     double snr_scale;         /* (f[1]-f[0]) / (weights' df): the SNR's Simpson step (waveform_util.cpp:497) */
     int bpl;                  /* bins per lane of this event's tiles (fixed per event: results never depend
                                  on which other events share a batch) */
+    int lanes;                /* lanes per tile (64, 128, 256, 512 or 1024): a tile holds lanes x bpl bins */
+    int fmin_off[GWH_FMIN_LEVELS];
     int det[GWH_MAXDET];
 };
+
+/* level of the frequency minima whose chunks are tiles of `bins` bins (a power of two in 64..8192) */
+__host__ __device__ constexpr int gwh_tile_level(int bins)
+{
+    int l = 0;
+    while ((64 << l) < bins) l++;
+    return l;
+}
+/* (instance, tile) items of one instance on an event */
+static inline int gwh_tiles_of(const EventDesc &d)
+{
+    const int t = d.lanes * d.bpl;
+    return (d.N + t - 1) / t;
+}
 
 /* Loads through global-address-space pointers.  Pointers held in device structs are generic, and generic
  * (flat) loads count against the scalar-memory counter as well as the vector one, so every wait for a
@@ -190,8 +208,10 @@ struct Engine {
     EventDesc *d_events = nullptr;
     std::vector<Event> events;
     uint64_t use_clock = 0;
-    int bpl = 0; /* 0 = automatic */
-    int tune[32] = {0}; /* autotuned bins per lane by floor(log2 N) of the event grid (0 = default rule) */
+    int bpl = 0;   /* 0 = automatic */
+    int lanes = 0; /* lanes per tile, 0 = automatic */
+    int tune[32] = {0};       /* autotuned bins per lane by floor(log2 N) of the event grid (0 = default rule) */
+    int tune_lanes[32] = {0}; /* autotuned lanes per tile by the same buckets (0 = default) */
     /* scratch */
     DevBuf params, coef, partial, active, logl, inst_event, f, data, psd, wts, out1, out2, misc;
     DevBuf work, wcount; /* flattened (instance, tile) lists of multi-event batches */
@@ -223,6 +243,7 @@ int gwh_ensure_init();
 int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const double *data, const double *psd,
                         int method, const double *weights, int log10F, int df_mode, uint64_t key, int *event);
 int gwh_choose_bpl(int N);
+int gwh_choose_lanes(int N);
 
 /* Resident event for the host arrays of a single-call entry point (engine lock held): a registered or pointer-mode
  * lookup by the arrays' addresses, else by a hash of their contents; created (and cached, LRU of 64) on a miss.

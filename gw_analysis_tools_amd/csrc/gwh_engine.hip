@@ -5,26 +5,26 @@
  * Hot path (one MCMC step):
  *   1. gwh_setup_kernel      FP64 InstCoef record of every instance: four waves per
  *                            64 instances compute independent parts, joined in LDS
- *   2. loglike kernel        (instance, bin-tile) work items: a persistent grid
- *                            of as many blocks as fit on the chip at once
- *                            strides over the items in tile-major order (for
- *                            one event, or events whose sizes are known only on
- *                            the device), or one block per item of flattened
- *                            work lists for multi-event batches; tiles above
- *                            0.2/M of an ascending grid do nothing; BPL bins per
- *                            lane, the instance record uniform (scalar loads);
- *                            per detector accumulate |h|^2 w/S and h* d w/S in
- *                            FP64; wave shuffle + LDS tree per item -> partials
+ *   2. loglike kernel        (instance, bin-tile) work items, one block each
+ *                            (gwh_tiles.h): in tile-major order for one event
+ *                            (or events whose sizes are known only on the
+ *                            device), or from flattened work lists for
+ *                            multi-event batches; a tile is lanes x BPL bins,
+ *                            both per event (autotuned); tiles above 0.2/M of
+ *                            an ascending grid do nothing; the instance record
+ *                            uniform (scalar loads); per detector accumulate
+ *                            |h|^2 w/S and h* d w/S in FP64; wave shuffle + LDS
+ *                            tree per item -> partials
  *   3. gwh_finalize_kernel   one wave per instance sums its items in a fixed
  *                            order (deterministic, batch-composition free)
  *
- * Precision modes: GWH_FP64 evaluates everything in FP64.  GWH_MIXED differs
- * only in the transcendentals that dominate the per-bin cost: sin/cos of the
- * per-detector phase (FP64 reduction to turns, FP32 sincospi, first-order
- * correction of the rounded argument) and the merger-ringdown exponential
- * (FP32 exp2 with an FP64 exponent split).  Phase, amplitude polynomials,
- * products and sums stay FP64: with packed FP32 disabled, FP32 and FP64 FMA
- * issue at the same rate on gfx950, so FP32 products would only add error.
+ * Precision modes: GWH_FP64 evaluates everything in FP64, sin/cos and exp through
+ * correctly rounded tables and short series (gwh_trig.h).  GWH_MIXED differs only
+ * where FP32 is faster on gfx950: with packed FP32 disabled, FP32 and FP64 FMAs
+ * issue at the same rate, so only FP32 hardware transcendentals can pay --
+ * the IMRPhenomD merger-ringdown exponential (FP32 exp2 with an FP64 exponent
+ * split) and the IMRPhenomPv2 reciprocal square root's seed.  Phases, products
+ * and sums stay FP64.
  *
  * Reference: src/mcmc_gw.cpp:2374-2489 (MCMC_likelihood_extrinsic),
  * :801-868 (Log_Likelihood_internal), src/waveform_util.cpp:152-184
@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
@@ -49,6 +50,7 @@
 #include "gwh_coalesce.h"
 #include "gwh_engine.h"
 #include "gwh_tables.h"
+#include "gwh_tiles.h"
 
 /* ------------------------------------------------------------------------- */
 /* runtime                                                                    */
@@ -152,8 +154,6 @@ static void spline_c(const double *x, const double *y, int n, double *c)
     for (int i = m - 2; i >= 0; i--) c[i + 1] = (rh[i] - of[i] * c[i + 2]) / dg[i];
 }
 
-void gwh_persist_warmup();
-
 static const double k_geometric_factor[GWH_NDET_TABLE] = {1., 1., 1., 1., 1., 1., 0.86602540378, 0.86602540378,
                                                           0.86602540378};
 
@@ -208,7 +208,9 @@ int gwh_ensure_init()
     GWH_HIP(hipMemset(E.d_events, 0, sizeof(EventDesc) * GWH_MAX_EVENTS));
     E.events.assign(GWH_MAX_EVENTS, Event());
     GWH_HIP(hipGetSymbolAddress((void **)&E.noalloc_scratch, HIP_SYMBOL(scratch)));
-    gwh_persist_warmup();
+    if (gwh_tiles_load<GWH_IMRPHENOMD, GWH_FP64>() || gwh_tiles_load<GWH_IMRPHENOMD, GWH_MIXED>() ||
+        gwh_tiles_load<GWH_IMRPHENOMPV2, GWH_FP64>() || gwh_tiles_load<GWH_IMRPHENOMPV2, GWH_MIXED>())
+        return gwh_set_error(GWH_ERR_HIP, "loading the likelihood kernels failed");
     E.ready = true;
     return GWH_OK;
 }
@@ -255,17 +257,51 @@ __global__ void gwh_event_prep_kernel(const double *__restrict__ f, const double
 }
 
 
-__global__ void gwh_chunk_fmin_kernel(const double *__restrict__ f, int N, double *__restrict__ out)
+/* element offsets of the frequency-minimum levels of an N-bin grid (level L: ceil(N / (64 * 2^L)) chunks) and
+ * their total */
+struct FminLevels {
+    int off[GWH_FMIN_LEVELS];
+};
+static int fmin_levels(int N, FminLevels &L)
 {
-    __shared__ double s[GWH_TPB];
-    int k = blockIdx.x * GWH_TPB + threadIdx.x;
-    s[threadIdx.x] = (k < N) ? f[k] : __longlong_as_double(0x7ff0000000000000LL);
-    __syncthreads();
-    for (int w = GWH_TPB / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + w]);
-        __syncthreads();
+    int total = 0;
+    for (int l = 0; l < GWH_FMIN_LEVELS; l++) {
+        L.off[l] = total;
+        total += (N + (64 << l) - 1) / (64 << l);
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+    return total;
+}
+
+/* Smallest frequency of every chunk of 64 * 2^L bins, L = 0..7: a tile of lanes x bins-per-lane bins is one chunk of
+ * its level, so a work item reads one value to know whether the tile starts above the 0.2/M cutoff.  One block of
+ * 128 lanes per 8192 bins: each lane takes the minimum of its 64 bins (level 0), then pairs are combined level by
+ * level in LDS. */
+__global__ __launch_bounds__(128) void gwh_fmin_levels_kernel(const double *__restrict__ f, int N,
+                                                              double *__restrict__ out, FminLevels L)
+{
+    __shared__ double s[2][128];
+    const int t = threadIdx.x;
+    const int c0 = blockIdx.x * 128 + t; /* level-0 chunk */
+    double m = __longlong_as_double(0x7ff0000000000000LL);
+    for (int i = 0; i < 64; i++) {
+        const long long k = (long long)c0 * 64 + i;
+        if (k < N) m = fmin(m, f[k]);
+    }
+    if (c0 < (N + 63) / 64) out[L.off[0] + c0] = m;
+    s[0][t] = m;
+    __syncthreads();
+    int src = 0;
+    for (int l = 1; l < GWH_FMIN_LEVELS; l++) {
+        const int width = 128 >> l;
+        if (t < width) {
+            const double v = fmin(s[src][2 * t], s[src][2 * t + 1]);
+            s[src ^ 1][t] = v;
+            const int c = blockIdx.x * width + t;
+            if (c < (N + (64 << l) - 1) / (64 << l)) out[L.off[l] + c] = v;
+        }
+        __syncthreads();
+        src ^= 1;
+    }
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -291,10 +327,12 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
             break;
         }
     if (slot < 0) return gwh_set_error(GWH_ERR_NOMEM, "event table full");
-    int nchunk = (N + GWH_TPB - 1) / GWH_TPB;
+    const int nchunk = (N + GWH_TPB - 1) / GWH_TPB;
+    FminLevels levels;
+    const int nfmin = fmin_levels(N, levels);
     size_t Nd = (size_t)N * ndet;
     size_t off_f = 0, off_wS = align256(off_f + N * 8), off_dwS = align256(off_wS + Nd * 8),
-           off_cm = align256(off_dwS + Nd * 16), off_pw = align256(off_cm + nchunk * 8),
+           off_cm = align256(off_dwS + Nd * 16), off_pw = align256(off_cm + (size_t)nfmin * 8),
            total = align256(off_pw + 5 * align256((size_t)N * 8));
     const size_t pw_stride = align256((size_t)N * 8);
     char *blk = nullptr;
@@ -318,8 +356,8 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
                        E.psd.as<const double>(), E.data.as<const double2>(), E.wts.as<const double>(), N, ndet, method,
                        log10_jac, dfac, (double *)(blk + off_wS), (double2 *)(blk + off_dwS));
     GWH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gwh_chunk_fmin_kernel, dim3(nchunk), dim3(GWH_TPB), 0, E.stream,
-                       (const double *)(blk + off_f), N, (double *)(blk + off_cm));
+    hipLaunchKernelGGL(gwh_fmin_levels_kernel, dim3((N + 8191) / 8192), dim3(128), 0, E.stream,
+                       (const double *)(blk + off_f), N, (double *)(blk + off_cm), levels);
     GWH_HIP(hipGetLastError());
     hipLaunchKernelGGL(gwh_bin_powers_kernel, dim3(nchunk), dim3(GWH_TPB), 0, E.stream,
                        (const double *)(blk + off_f), N, (double *)(blk + off_pw), pw_stride / 8);
@@ -343,7 +381,8 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     ds.f = (const double *)(blk + off_f);
     ds.wS = (const double *)(blk + off_wS);
     ds.dwS = (const double2 *)(blk + off_dwS);
-    ds.chunk_fmin = (const double *)(blk + off_cm);
+    ds.tile_fmin = (const double *)(blk + off_cm);
+    for (int l = 0; l < GWH_FMIN_LEVELS; l++) ds.fmin_off[l] = levels.off[l];
     ds.lgf = (const double *)(blk + off_pw);
     ds.f13 = (const double *)(blk + off_pw + pw_stride);
     ds.fm13 = (const double *)(blk + off_pw + 2 * pw_stride);
@@ -351,8 +390,8 @@ int gwh_event_create_ex(const double *f, int N, int ndet, const int *dets, const
     ds.f34 = (const double *)(blk + off_pw + 4 * pw_stride);
     ds.N = N;
     ds.D = ndet;
-    ds.nchunk = nchunk;
     ds.bpl = gwh_choose_bpl(N);
+    ds.lanes = gwh_choose_lanes(N);
     /* Gauss-Legendre weights carry no step: nothing to rescale */
     ds.snr_scale = (method == GWH_GAUSSLEG) ? 1.0 : df_f / ev.df_like;
     ds.sorted = 1;
@@ -554,9 +593,9 @@ __global__ __launch_bounds__(256) void gwh_setup_kernel(const gwh_params *__rest
                 const EventDesc &E = events[ev];
                 /* tiles to visit: all of them, or on an ascending grid those starting at or below 0.2/M */
                 const double fcut = .2 / S.M;
-                const int tile = GWH_TPB * E.bpl;
+                const int tile = E.lanes * E.bpl;
                 int nt = (E.N + tile - 1) / tile;
-                const int cpt = E.bpl; /* 256-bin chunks per tile */
+                const double *tf = E.tile_fmin + E.fmin_off[gwh_tile_level(tile)]; /* tile minima */
                 if (E.sorted) {
                     /* the first tile whose smallest frequency exceeds the cutoff; up to 16 tiles the count of
                      * tiles at or below it (same answer on an ascending grid), from loads issued together */
@@ -566,14 +605,14 @@ __global__ __launch_bounds__(256) void gwh_setup_kernel(const gwh_params *__rest
                     } else if (nt <= 16) {
                         double fm[16];
 #pragma unroll
-                        for (int t = 0; t < 16; t++) fm[t] = E.chunk_fmin[(t < nt ? t : nt - 1) * cpt];
+                        for (int t = 0; t < 16; t++) fm[t] = tf[t < nt ? t : nt - 1];
 #pragma unroll
                         for (int t = 0; t < 16; t++) lo += (t < nt && !(fm[t] > fcut));
                     } else {
                         int hi = nt;
                         while (lo < hi) {
                             int mid = (lo + hi) >> 1;
-                            if (E.chunk_fmin[mid * cpt] > fcut)
+                            if (tf[mid] > fcut)
                                 hi = mid;
                             else
                                 lo = mid + 1;
@@ -704,347 +743,11 @@ int gwh_setup_launch(const int *d_events, int events0, int model, const gwh_para
     return GWH_OK;
 }
 
-/* ------------------------------------------------------------------------- */
-/* reductions                                                                 */
-/* ------------------------------------------------------------------------- */
-
-/* Block-reduce NV values per thread (fixed order) and write them to out[0..NV) */
-template <int NV>
-__device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double *__restrict__ out, double (*lds)[NV],
-                                                   int nv = NV)
-{
-    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < NV; j++)
-        if (j < nv) acc[j] = wave_sum(acc[j]); /* nv is uniform: unused slots are skipped, not masked */
-    if (lane == 0) {
-#pragma unroll
-        for (int j = 0; j < NV; j++) lds[wv][j] = acc[j];
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < nv) {
-        int j = threadIdx.x;
-        double s = lds[0][j];
-        for (int w = 1; w < GWH_TPB / 64; w++) s += lds[w][j];
-        out[j] = s;
-    }
-    __syncthreads();
-}
-
-
-/* sin and cos of an FP64 phase in FP32: FP64 reduction to a fraction of a turn in [-1/2, 1/2], then the FP32
- * sin(pi x), cos(pi x) of x = 2 * fraction (exact scaling; a short polynomial, no large-argument path).
- * The part of x lost in rounding it to FP32 (up to 6e-8 half-turns, i.e. 1.9e-7 rad) is restored to first
- * order: sin(a + d) = sin a + d cos a, cos(a + d) = cos a - d sin a.  What remains is FP32 sinpi's own rounding
- * (~6e-8), random from bin to bin. */
-__device__ __forceinline__ void sincos_turns(double phase, float &s, float &c)
-{
-    double r = phase * 0.15915494309189533577;
-    r = r - rint(r);
-    const double x = 2.0 * r;
-    const float xf = (float)x;
-    const float d = (float)(x - (double)xf) * 3.14159265358979f;
-    float s0, c0;
-    sincospif(xf, &s0, &c0);
-    s = fmaf(d, c0, s0);
-    c = fmaf(-d, s0, c0);
-}
-
-/* e^x for x <= 0 of moderate size: FP64 split x log2(e) = n + t, FP32 2^t (~6e-8 relative), exact scaling */
-__device__ __forceinline__ double exp_fp32(double x)
-{
-    double y = x * 1.4426950408889634074;
-    double n = rint(y);
-    float t = (float)(y - n);
-    return ldexp((double)__builtin_amdgcn_exp2f(t), (int)n);
-}
-
-/* ------------------------------------------------------------------------- */
-/* per-bin mixed-precision evaluators                                         */
-/* ------------------------------------------------------------------------- */
-
-/* 1/z in FP64: hardware reciprocal and one Newton step (no IEEE division sequence) */
-__device__ __forceinline__ double rcp_newton(double z)
-{
-    double r = __builtin_amdgcn_rcp(z);
-    r = fma(r, fma(-z, r, 1.0), r);
-    return r;
-}
-
-/* z^{-1/2} in FP64 from an FP32 v_rsq_f32 seed and two Newton steps */
-__device__ __forceinline__ double rsqrt_mixed(double z)
-{
-    double y = (double)__builtin_amdgcn_rsqf((float)z);
-    y = y * (1.5 - 0.5 * z * y * y);
-    y = y * (1.5 - 0.5 * z * y * y);
-    return y;
-}
-
-/* IMRPhenomD in mixed precision: the FP64 evaluator with the merger-ringdown amplitude's exponential in FP32
- * (TABLE_EXP false: ~6e-8 relative per bin) or through the FP64 table (true: a few ulp); the phase, including its
- * arctangent, stays FP64 (an FP32 atan there was the largest error of the path, ~7e-4 in logL for heavy
- * binaries). */
-template <bool TABLE_EXP>
-__device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, const BinPow &bp, double &a, double &phi,
-                                                  double &u_out, double &lx_out)
-{
-    double Mf = C.M * f;
-    double r = C.Mm13 * bp.fm13;     /* (Mf)^{-1/3} */
-    double u = C.M13 * bp.f13;       /* (Mf)^{1/3} */
-    double r2 = r * r;
-    double inv76 = C.Mm76 * bp.fm76; /* (Mf)^{-7/6} */
-    double ar;
-    if (f < C.f1) {
-        double pn = C.ca[0] + u * u * (C.ca[2] + u * (C.ca[3] + u * (C.ca[4] + u * (C.ca[5] + u * C.ca[6]))));
-        double u3 = u * u * u;
-        ar = pn + u3 * u3 * u * (C.rho[0] + u * (C.rho[1] + u * C.rho[2]));
-    } else if (f > C.f3) {
-        double dl = f - C.fRD;
-        ar = C.mr_k * (TABLE_EXP ? gwh_exp_table(-C.mr_g2 * dl) : exp_fp32(-C.mr_g2 * dl)) * rcp_newton(dl * dl + C.mr_w2);
-    } else {
-        ar = C.delta[0] + Mf * (C.delta[1] + Mf * (C.delta[2] + Mf * (C.delta[3] + Mf * C.delta[4])));
-    }
-    a = C.amp_pref * ar * inv76;
-    double psi;
-    double lx = 0;
-    if (f < C.f1p) {
-        double v = u * 1.4645918875615231;
-        lx = C.lgPiM + bp.lf; /* log(pi M f) */
-        double c5 = C.c8 + lx * C.c9;
-        double c6 = C.c10 - C.c11 * lx;
-        double v2 = v * v;
-        double poly = C.cp[0] + v2 * (C.cp[2] + v * (C.cp[3] + v * (C.cp[4] + v * (c5 + v * (c6 + v * C.cp[7])))));
-        double iv = r * 0.68278406325529568; /* 1/v = pi^{-1/3} (Mf)^{-1/3} */
-        double iv2 = iv * iv;
-        double Mf43 = Mf * u;
-        psi = -GWH_PI / 4. + C.K * poly * (iv2 * iv2 * iv) +
-              (C.sig[0] * Mf + C.sig[1] * Mf43 + C.sig[2] * Mf43 * u + C.sig[3] * Mf * Mf);
-        if (C.gimr) psi += gwh_gimr_bin(C, iv);
-    } else if (f > C.f2p) {
-        psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] * (r2 * r) + C.alpha[3] * (C.M34 * bp.f34) +
-              C.alpha[4] * atan((f - C.alpha[5]) * C.inv_fdamp);
-    } else {
-        double r3 = r2 * r;
-        psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] * (r3 * r3 * r3);
-    }
-    if (C.ppe_n > 0 && (C.ppe_imr || f < C.f1p)) psi += gwh_ppe_bin(C, bp.lf); /* FP64: phase accuracy */
-    phi = psi - (C.tc_total * (f - C.f_ref) + C.phic);
-    u_out = u;
-    lx_out = lx;
-}
-
-/* ------------------------------------------------------------------------- */
-/* the batched likelihood kernels                                             */
-/* ------------------------------------------------------------------------- */
-
-struct WorkMap {
-    const int *inst_event; /* NULL: every instance uses event0 */
-    int event0, n, tiles0; /* tiles0: tile stride of the partial buffer */
-};
-
-/* One (instance, tile) work item: BPL x 256 bins of one instance's network log-likelihood, reduced
- * to the instance's partial sums for that tile.  Every early return is uniform over the block. */
-template <int MODEL, int PREC, int BPL>
-__device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDesc *__restrict__ events,
-                                                 const InstCoef *__restrict__ coefs, double *__restrict__ partial,
-                                                 const int *__restrict__ active, const int inst, const int tile,
-                                                 double (*lds)[GWH_PARTIAL_STRIDE])
-{
-    {
-        const int ev = W.inst_event ? W.inst_event[inst] : W.event0;
-        const EventDesc &E = events[ev];
-        if (E.bpl != BPL) return;         /* the instance's event is tiled by another launch */
-        if (tile >= active[inst]) return; /* above 0.2/M on an ascending grid: h = 0, nothing to add */
-        const int w = inst * W.tiles0 + tile;
-        const InstCoef &C = coefs[inst];
-        const int N = E.N, D = E.D;
-        double acc[GWH_PARTIAL_STRIDE];
-#pragma unroll
-        for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) acc[j] = 0;
-        /* skip tiles entirely above the 0.2/M cutoff (h = 0 there, IMRPhenomD.cpp:475) */
-        double fmin_tile = gwh_g(E.chunk_fmin)[tile * BPL];
-#pragma unroll
-        for (int j = 1; j < BPL; j++)
-            if (tile * BPL + j < E.nchunk) fmin_tile = fmin(fmin_tile, gwh_g(E.chunk_fmin)[tile * BPL + j]);
-        if (!(fmin_tile > C.fcut)) {
-#pragma unroll
-            for (int j = 0; j < BPL; j++) {
-                /* Lanes past the end of the grid or above 0.2/M evaluate the last grid bin with a zero amplitude
-                 * (and phase), so the accumulators never sit under divergent control flow; a wave without any
-                 * bin at or below the cutoff skips the step on a uniform branch. */
-                const int k0 = tile * GWH_TPB * BPL + j * GWH_TPB + threadIdx.x;
-                const bool in = k0 < N;
-                const int k = in ? k0 : N - 1;
-                const double f = gwh_g(E.f)[k];
-                const bool live = in && f <= C.fcut;
-                if (!__ballot(live)) continue;
-                const BinPow bp = gwh_load_bin(E, k);
-                if (MODEL == GWH_IMRPHENOMD) {
-                    double a, phi, u, lx;
-                    if (PREC == GWH_FP64)
-                        gwh_phenomd_fp64(C, f, bp, a, phi, u, lx);
-                    else
-                        gwh_phenomd_mixed<false>(C, f, bp, a, phi, u, lx);
-                    a = live ? a : 0.0;
-                    phi = live ? phi : 0.0;
-#pragma unroll
-                    for (int d = 0; d < GWH_MAXDET; d++) {
-                        if (d >= D) break;
-                        double s, c;
-                        if (PREC == GWH_FP64) {
-                            gwh_sincos_table(phi + C.det[d][3] * f, s, c);
-                        } else {
-                            float sf, cf;
-                            sincos_turns(phi + C.det[d][3] * f, sf, cf);
-                            s = sf;
-                            c = cf;
-                        }
-                        const double ws = gwh_g(E.wS)[(size_t)d * N + k];
-                        const double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
-                        acc[3 * d] += a * a * ws;
-                        acc[3 * d + 1] += a * (c * dw.x - s * dw.y);
-                        acc[3 * d + 2] += a * (c * dw.y + s * dw.x);
-                    }
-                } else {
-                    double A, Phi, hpr, hpi, hcr, hci;
-                    if (PREC == GWH_FP64) {
-                        gwh_pv2_fp64(C, f, bp, A, Phi, hpr, hpi, hcr, hci);
-                    } else {
-                        double u, lx;
-                        gwh_phenomd_mixed<true>(C, f, bp, A, Phi, u, lx);
-                        /* precession angles and the sin/cos of the Euler angle alpha in FP64 (alpha's rounding
-                         * would enter |h|^2 directly), like the detector phases below: table-driven sin/cos and
-                         * exponential (the FP32 ones left ~2.5e-4 in logL on short, loud precessing events) */
-                        const double v = u * 1.4645918875615231;
-                        const double logw = C.lgPiM + bp.lf; /* log(pi M f) */
-                        const double x = v * v;
-                        const double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) / v;
-                        const double sv = C.SP / (L + C.SL);
-                        const double cbd = rsqrt_mixed(1.0 + sv * sv);
-                        const double cbhd = sqrt(0.5 * (1.0 + cbd)), sbhd = sqrt(0.5 * (1.0 - cbd));
-                        const double c2d = cbhd * cbhd, s2d = sbhd * sbhd;
-                        const double d2[5] = {s2d * s2d, 2. * cbhd * s2d * sbhd, GWH_SQRT_6 * s2d * c2d,
-                                              2. * c2d * cbhd * sbhd, c2d * c2d};
-                        const double alpha = gwh_euler_eval(C.a_c, v, logw) + C.alpha_add;
-                        const double eps = gwh_euler_eval(C.e_c, v, logw) + C.eps_add;
-                        double sa, ca;
-                        gwh_sincos_table(alpha, sa, ca);
-                        double er[5], ei[5];
-                        er[2] = 1.;
-                        ei[2] = 0.;
-                        er[3] = ca;
-                        ei[3] = sa;
-                        er[1] = ca;
-                        ei[1] = -sa;
-                        er[4] = ca * ca - sa * sa;
-                        ei[4] = 2. * ca * sa;
-                        er[0] = er[4];
-                        ei[0] = -ei[4];
-                        hpr = hpi = hcr = hci = 0;
-#pragma unroll
-                        for (int m = -2; m <= 2; m++) {
-                            const double dm2 =
-                                (m == -2) ? d2[4] : (m == -1) ? -d2[3] : (m == 0) ? d2[2] : (m == 1) ? -d2[1] : d2[0];
-                            const double y = C.ylm[m + 2];
-                            const double t2r = er[-m + 2] * dm2 * y, t2i = ei[-m + 2] * dm2 * y;
-                            const double tmr = er[m + 2] * d2[m + 2] * y, tmi = ei[m + 2] * d2[m + 2] * y;
-                            hpr += t2r + tmr;
-                            hpi += t2i + tmi;
-                            hcr -= t2i - tmi;
-                            hci += t2r - tmr;
-                        }
-                        Phi = Phi + 2. * eps + C.t_corr_2pi * f;
-                    }
-                    A = live ? A : 0.0;
-                    Phi = live ? Phi : 0.0;
-#pragma unroll
-                    for (int d = 0; d < GWH_MAXDET; d++) {
-                        if (d >= D) break;
-                        const double P = C.det[d][0], Q = C.det[d][1];
-                        const double Gr = P * hpr + Q * hcr, Gi = P * hpi + Q * hci;
-                        double s, c;
-                        if (PREC == GWH_FP64)
-                            gwh_sincos_table(Phi + C.det[d][3] * f, s, c);
-                        else
-                            gwh_sincos_table(Phi + C.det[d][3] * f, s, c);
-                        const double ws = gwh_g(E.wS)[(size_t)d * N + k];
-                        const double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
-                        acc[3 * d] += A * A * (Gr * Gr + Gi * Gi) * ws;
-                        acc[3 * d + 1] += A * (dw.x * (Gr * c + Gi * s) - dw.y * (Gr * s - Gi * c));
-                    }
-                }
-            }
-        }
-        block_reduce_store<GWH_PARTIAL_STRIDE>(acc, partial + (size_t)w * GWH_PARTIAL_STRIDE, lds, 3 * D);
-    }
-}
-
-/* the flattened (instance, tile) work list of one bins-per-lane class: one block per live work item,
- * so batches mixing 10^3- and 10^5-bin events launch no empty blocks */
-template <int MODEL, int PREC, int BPL>
-__global__ __launch_bounds__(GWH_TPB) void gwh_loglike_wl_kernel(WorkMap W, const EventDesc *__restrict__ events,
-                                                                 const InstCoef *__restrict__ coefs,
-                                                                 double *__restrict__ partial,
-                                                                 const int *__restrict__ active,
-                                                                 const int2 *__restrict__ work,
-                                                                 const int *__restrict__ count)
-{
-    __shared__ double lds[GWH_TPB / 64][GWH_PARTIAL_STRIDE];
-    if ((int)blockIdx.x >= *count) return;
-    const int2 wi = work[blockIdx.x];
-    gwh_loglike_tile<MODEL, PREC, BPL>(W, events, coefs, partial, active, wi.x, wi.y, lds);
-}
-
-/* The persistent grid: as many blocks as are resident on the chip at once stride over the (instance, tile)
- * items in tile-major order (item w = tile * n + instance).  Consecutive blocks go to different XCDs, so
- * the instances of every tile spread over all XCDs; a grid with one block per item in instance-major order
- * would send tile t of every instance to the same XCD, and the tiles below an instance's 0.2/M cutoff --
- * the only ones with work -- to a few XCDs.  Items above the cutoff cost one load.  count (optional,
- * device): only instances i < *count are visited, so a launch sized for the largest possible count costs
- * little when the count is small. */
-template <int MODEL, int PREC, int BPL>
-__device__ __forceinline__ void gwh_loglike_persist(WorkMap W, const EventDesc *__restrict__ events,
-                                                    const InstCoef *__restrict__ coefs, double *__restrict__ partial,
-                                                    const int *__restrict__ active, const int *__restrict__ count)
-{
-    __shared__ double lds[GWH_TPB / 64][GWH_PARTIAL_STRIDE];
-    const int nn = count ? min(*count, W.n) : W.n;
-    const int items = nn * W.tiles0;
-    for (int w = blockIdx.x; w < items; w += gridDim.x) {
-        gwh_loglike_tile<MODEL, PREC, BPL>(W, events, coefs, partial, active, w % nn, w / nn, lds);
-        __syncthreads(); /* the LDS of the block reduction is reused by the next item */
-    }
-}
-
-#define GWH_PERSIST_ARGS                                                                                             \
-    WorkMap W, const EventDesc *__restrict__ events, const InstCoef *__restrict__ coefs, double *__restrict__ partial, \
-        const int *__restrict__ active, const int *__restrict__ count
-template <int MODEL, int PREC, int BPL>
-__global__ __launch_bounds__(GWH_TPB) void gwh_loglike_persist_kernel(GWH_PERSIST_ARGS)
-{
-    gwh_loglike_persist<MODEL, PREC, BPL>(W, events, coefs, partial, active, count);
-}
-/* Occupancy bounds where they fit without scratch: the mixed IMRPhenomD kernel in 128 VGPRs (four waves per SIMD
- * instead of three: the 512-instance step's kernel 34.8 -> 29 us), the FP64 IMRPhenomD kernel in 168 (three
- * instead of two: 62 -> 54 us).  More waves hide more of the bin loads' latency; tighter bounds would spill
- * (the precessing kernels spill at them). */
-#define GWH_PERSIST_OCC(M, P, B, WAVES)                                                                            \
-    template <>                                                                                                    \
-    __global__ __launch_bounds__(GWH_TPB) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void                 \
-    gwh_loglike_persist_kernel<M, P, B>(GWH_PERSIST_ARGS)                                                          \
-    {                                                                                                              \
-        gwh_loglike_persist<M, P, B>(W, events, coefs, partial, active, count);                                   \
-    }
-#define GWH_PERSIST_OCC_ALL(M, P, WAVES)                                                                           \
-    GWH_PERSIST_OCC(M, P, 1, WAVES) GWH_PERSIST_OCC(M, P, 2, WAVES) GWH_PERSIST_OCC(M, P, 4, WAVES)               \
-        GWH_PERSIST_OCC(M, P, 8, WAVES)
-GWH_PERSIST_OCC_ALL(GWH_IMRPHENOMD, GWH_MIXED, 4)
-GWH_PERSIST_OCC_ALL(GWH_IMRPHENOMD, GWH_FP64, 3)
-
-/* Work list of one bins-per-lane class (single block): instance i contributes active[i] items if its
- * event is tiled with `bpl`; items are ordered by instance then tile.  At most `cap` are written. */
+/* Work list of one tiling class (single block): instance i contributes active[i] items if its event is tiled
+ * with `bpl` bins per lane and `lanes` lanes per tile; items are ordered by instance then tile.  At most `cap` are
+ * written. */
 __global__ __launch_bounds__(1024) void gwh_worklist_kernel(WorkMap W, const EventDesc *__restrict__ events,
-                                                            const int *__restrict__ active, int bpl,
+                                                            const int *__restrict__ active, int bpl, int lanes,
                                                             int2 *__restrict__ work, int *__restrict__ count, int cap)
 {
     __shared__ int part[1024];
@@ -1054,7 +757,7 @@ __global__ __launch_bounds__(1024) void gwh_worklist_kernel(WorkMap W, const Eve
     int local = 0;
     for (int i = i0; i < i1; i++) {
         int ev = W.inst_event ? W.inst_event[i] : W.event0;
-        if (events[ev].bpl == bpl) local += max(active[i], 0);
+        if (events[ev].bpl == bpl && events[ev].lanes == lanes) local += max(active[i], 0);
     }
     part[t] = local;
     __syncthreads();
@@ -1067,7 +770,7 @@ __global__ __launch_bounds__(1024) void gwh_worklist_kernel(WorkMap W, const Eve
     int off = part[t] - local;
     for (int i = i0; i < i1; i++) {
         int ev = W.inst_event ? W.inst_event[i] : W.event0;
-        if (events[ev].bpl != bpl) continue;
+        if (events[ev].bpl != bpl || events[ev].lanes != lanes) continue;
         for (int k = 0; k < active[i]; k++, off++)
             if (off < cap) work[off] = make_int2(i, k);
     }
@@ -1140,68 +843,41 @@ int gwh_choose_bpl(int N)
     return 1;
 }
 
-/* resident blocks of one persistent-kernel instantiation on the whole chip (cached per instantiation) */
-template <int MODEL, int PREC, int BPL> static int persist_grid()
+/* lanes per tile: the engine-wide override, else the autotuned choice of the grid's size bucket, else 256 */
+int gwh_choose_lanes(int N)
 {
-    static int cached = 0;
-    if (!cached) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gwh_loglike_persist_kernel<MODEL, PREC, BPL>, GWH_TPB,
-                                                         0) != hipSuccess ||
-            per_cu < 1)
-            per_cu = 4;
-        cached = per_cu * gwh_engine().num_cu;
-    }
-    return cached;
+    Engine &E = gwh_engine();
+    if (E.lanes > 0) return E.lanes;
+    int lg = 0;
+    while ((2 << lg) <= N && lg < 31) lg++;
+    if (E.tune_lanes[lg] > 0) return E.tune_lanes[lg];
+    return GWH_TPB;
 }
 
-/* queried once at initialisation, outside any stream capture */
-template <int MODEL, int PREC> static void persist_warm()
+/* tiling classes: bins per lane {1, 2, 4, 8} x lanes per tile {64, 128, 256, 512, 1024} */
+static const int kClassBpl[4] = {1, 2, 4, 8}, kClassLanes[5] = {64, 128, 256, 512, 1024};
+static int gwh_class_of(const EventDesc &d)
 {
-    persist_grid<MODEL, PREC, 1>();
-    persist_grid<MODEL, PREC, 2>();
-    persist_grid<MODEL, PREC, 4>();
-    persist_grid<MODEL, PREC, 8>();
-}
-void gwh_persist_warmup()
-{
-    persist_warm<GWH_IMRPHENOMD, GWH_FP64>();
-    persist_warm<GWH_IMRPHENOMD, GWH_MIXED>();
-    persist_warm<GWH_IMRPHENOMPV2, GWH_FP64>();
-    persist_warm<GWH_IMRPHENOMPV2, GWH_MIXED>();
+    const int b = d.bpl == 1 ? 0 : d.bpl == 2 ? 1 : d.bpl == 4 ? 2 : 3;
+    const int l = d.lanes == 64 ? 0 : d.lanes == 128 ? 1 : d.lanes == 256 ? 2 : d.lanes == 512 ? 3 : 4;
+    return 5 * b + l;
 }
 
-template <int MODEL, int PREC, int BPL>
-static void launch_persist_b(long long items, hipStream_t st, const WorkMap &W, const EventDesc *ev, const InstCoef *c,
-                             double *part, const int *act, const int *count)
+static void launch_tiles(int model, int precision, int cls, long long items, const int2 *work, hipStream_t st,
+                         const WorkMap &W, const EventDesc *ev, const InstCoef *c, double *part, const int *act,
+                         const int *count)
 {
-    const int blocks = (int)std::max(1LL, std::min<long long>(items, persist_grid<MODEL, PREC, BPL>()));
-    hipLaunchKernelGGL((gwh_loglike_persist_kernel<MODEL, PREC, BPL>), dim3(blocks), dim3(GWH_TPB), 0, st, W, ev, c,
-                       part, act, count);
-}
-
-template <int MODEL, int PREC>
-static void launch_persist(int bpl, long long items, hipStream_t st, const WorkMap &W, const EventDesc *ev,
-                           const InstCoef *c, double *part, const int *act, const int *count)
-{
-    switch (bpl) {
-    case 1: launch_persist_b<MODEL, PREC, 1>(items, st, W, ev, c, part, act, count); break;
-    case 2: launch_persist_b<MODEL, PREC, 2>(items, st, W, ev, c, part, act, count); break;
-    case 4: launch_persist_b<MODEL, PREC, 4>(items, st, W, ev, c, part, act, count); break;
-    default: launch_persist_b<MODEL, PREC, 8>(items, st, W, ev, c, part, act, count); break;
-    }
-}
-
-template <int MODEL, int PREC>
-static void launch_wl(int bpl, int blocks, hipStream_t st, const WorkMap &W, const EventDesc *ev, const InstCoef *c,
-                      double *part, const int *act, const int2 *work, const int *count)
-{
-    dim3 g(blocks), b(GWH_TPB);
-    switch (bpl) {
-    case 1: hipLaunchKernelGGL((gwh_loglike_wl_kernel<MODEL, PREC, 1>), g, b, 0, st, W, ev, c, part, act, work, count); break;
-    case 2: hipLaunchKernelGGL((gwh_loglike_wl_kernel<MODEL, PREC, 2>), g, b, 0, st, W, ev, c, part, act, work, count); break;
-    case 4: hipLaunchKernelGGL((gwh_loglike_wl_kernel<MODEL, PREC, 4>), g, b, 0, st, W, ev, c, part, act, work, count); break;
-    default: hipLaunchKernelGGL((gwh_loglike_wl_kernel<MODEL, PREC, 8>), g, b, 0, st, W, ev, c, part, act, work, count); break;
+    const int bpl = kClassBpl[cls / 5], lanes = kClassLanes[cls % 5];
+    if (!gwh_is_pv2(model)) {
+        if (precision == GWH_FP64)
+            gwh_launch_tiles<GWH_IMRPHENOMD, GWH_FP64>(bpl, lanes, items, work, st, W, ev, c, part, act, count);
+        else
+            gwh_launch_tiles<GWH_IMRPHENOMD, GWH_MIXED>(bpl, lanes, items, work, st, W, ev, c, part, act, count);
+    } else {
+        if (precision == GWH_FP64)
+            gwh_launch_tiles<GWH_IMRPHENOMPV2, GWH_FP64>(bpl, lanes, items, work, st, W, ev, c, part, act, count);
+        else
+            gwh_launch_tiles<GWH_IMRPHENOMPV2, GWH_MIXED>(bpl, lanes, items, work, st, W, ev, c, part, act, count);
     }
 }
 
@@ -1222,27 +898,24 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     if (events0 < 0 || events0 >= GWH_MAX_EVENTS || (!d_events && !E.events[events0].live))
         return gwh_set_error(GWH_ERR_INVALID, "unknown event " + std::to_string(events0));
     if (n > 65535) return gwh_set_error(GWH_ERR_INVALID, "at most 65535 instances per launch");
-    /* each event carries its own bins per lane; one launch per tiling present (instances of other
-     * tilings leave at once), so an instance's partial sums never depend on the batch composition */
-    int tiles_b[4] = {0, 0, 0, 0}; /* bpl 1, 2, 4, 8 */
-    auto slot_of = [](int b) { return b == 1 ? 0 : b == 2 ? 1 : b == 4 ? 2 : 3; };
+    /* each event carries its own tiling (bins per lane x lanes per tile); one launch per tiling class present
+     * (instances of other classes leave at once), so an instance's partial sums never depend on the batch */
+    int tiles_b[GWH_NCLASS] = {0};
     auto note = [&](const Event &ev) {
-        int b = ev.desc.bpl, t = (ev.N + GWH_TPB * b - 1) / (GWH_TPB * b);
-        int sl = slot_of(b);
+        const int t = gwh_tiles_of(ev.desc), sl = gwh_class_of(ev.desc);
         if (t > tiles_b[sl]) tiles_b[sl] = t;
     };
     /* multi-event batch whose instance -> event map is known on the host: flattened work lists with
      * per-class upper bounds (sum over instances of the event's tile count) */
     const bool wl = d_events && h_events;
-    long long ub[4] = {0, 0, 0, 0};
+    long long ub[GWH_NCLASS] = {0};
     if (wl) {
         for (int i = 0; i < n; i++) {
             const Event &ev = E.events[h_events[i]];
             note(ev);
-            int b = ev.desc.bpl;
-            ub[slot_of(b)] += (ev.N + GWH_TPB * b - 1) / (GWH_TPB * b);
+            ub[gwh_class_of(ev.desc)] += gwh_tiles_of(ev.desc);
         }
-        for (int c = 0; c < 4; c++)
+        for (int c = 0; c < GWH_NCLASS; c++)
             if (ub[c] > (1ll << 30)) return gwh_set_error(GWH_ERR_INVALID, "work list too large");
     } else if (d_events) {
         for (auto &ev : E.events)
@@ -1251,14 +924,17 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
         note(E.events[events0]);
     }
     int maxtiles = 0;
-    for (int b = 0; b < 4; b++) maxtiles = std::max(maxtiles, tiles_b[b]);
+    for (int b = 0; b < GWH_NCLASS; b++) maxtiles = std::max(maxtiles, tiles_b[b]);
+    if ((long long)n * maxtiles > INT_MAX) return gwh_set_error(GWH_ERR_INVALID, "too many (instance, tile) items");
     int rc;
     if ((rc = s_coef.ensure((size_t)n * sizeof(InstCoef))) != GWH_OK) return rc;
     if ((rc = s_partial.ensure((size_t)n * maxtiles * GWH_PARTIAL_STRIDE * sizeof(double))) != GWH_OK) return rc;
     if ((rc = s_active.ensure((size_t)n * sizeof(int))) != GWH_OK) return rc;
     if (wl) {
-        if ((rc = s_work.ensure((size_t)(ub[0] + ub[1] + ub[2] + ub[3]) * sizeof(int2))) != GWH_OK) return rc;
-        if ((rc = s_wcount.ensure(4 * sizeof(int))) != GWH_OK) return rc;
+        long long total = 0;
+        for (int c = 0; c < GWH_NCLASS; c++) total += ub[c];
+        if ((rc = s_work.ensure((size_t)total * sizeof(int2))) != GWH_OK) return rc;
+        if ((rc = s_wcount.ensure(GWH_NCLASS * sizeof(int))) != GWH_OK) return rc;
     }
     if (X.reserve_only) return GWH_OK;
     WorkMap W;
@@ -1278,45 +954,21 @@ int gwh_loglike_launch(const int *d_events, int events0, int model, int precisio
     const InstCoef *cp = s_coef.as<const InstCoef>();
     double *pp = s_partial.as<double>();
     const int *ap = s_active.as<const int>();
-    static const int kB[4] = {1, 2, 4, 8};
     if (wl) {
         long long off = 0;
-        for (int sl = 0; sl < 4; sl++) {
+        for (int sl = 0; sl < GWH_NCLASS; sl++) {
             if (!ub[sl]) continue;
             int2 *wp = s_work.as<int2>() + off;
             int *cnt = s_wcount.as<int>() + sl;
-            hipLaunchKernelGGL(gwh_worklist_kernel, dim3(1), dim3(1024), 0, stream, W, evp, ap, kB[sl], wp, cnt,
-                               (int)ub[sl]);
-            if (!gwh_is_pv2(model)) {
-                if (precision == GWH_FP64)
-                    launch_wl<GWH_IMRPHENOMD, GWH_FP64>(kB[sl], (int)ub[sl], stream, W, evp, cp, pp, ap, wp, cnt);
-                else
-                    launch_wl<GWH_IMRPHENOMD, GWH_MIXED>(kB[sl], (int)ub[sl], stream, W, evp, cp, pp, ap, wp, cnt);
-            } else {
-                if (precision == GWH_FP64)
-                    launch_wl<GWH_IMRPHENOMPV2, GWH_FP64>(kB[sl], (int)ub[sl], stream, W, evp, cp, pp, ap, wp, cnt);
-                else
-                    launch_wl<GWH_IMRPHENOMPV2, GWH_MIXED>(kB[sl], (int)ub[sl], stream, W, evp, cp, pp, ap, wp, cnt);
-            }
+            hipLaunchKernelGGL(gwh_worklist_kernel, dim3(1), dim3(1024), 0, stream, W, evp, ap, kClassBpl[sl / 5],
+                               kClassLanes[sl % 5], wp, cnt, (int)ub[sl]);
+            launch_tiles(model, precision, sl, ub[sl], wp, stream, W, evp, cp, pp, ap, cnt);
             off += ub[sl];
         }
     }
-    for (int sl = 0; sl < 4 && !wl; sl++) {
-        if (!tiles_b[sl]) continue;
-        const long long items = (long long)n * maxtiles;
-        const int bpl = kB[sl];
-        if (!gwh_is_pv2(model)) {
-            if (precision == GWH_FP64)
-                launch_persist<GWH_IMRPHENOMD, GWH_FP64>(bpl, items, stream, W, evp, cp, pp, ap, X.d_count);
-            else
-                launch_persist<GWH_IMRPHENOMD, GWH_MIXED>(bpl, items, stream, W, evp, cp, pp, ap, X.d_count);
-        } else {
-            if (precision == GWH_FP64)
-                launch_persist<GWH_IMRPHENOMPV2, GWH_FP64>(bpl, items, stream, W, evp, cp, pp, ap, X.d_count);
-            else
-                launch_persist<GWH_IMRPHENOMPV2, GWH_MIXED>(bpl, items, stream, W, evp, cp, pp, ap, X.d_count);
-        }
-    }
+    for (int sl = 0; sl < GWH_NCLASS && !wl; sl++)
+        if (tiles_b[sl]) launch_tiles(model, precision, sl, (long long)n * maxtiles, nullptr, stream, W, evp, cp, pp, ap,
+                                      X.d_count);
     GWH_HIP(hipGetLastError());
     if (timed) GWH_HIP(hipEventRecord(E.ev1, stream));
     if (live) GWH_HIP(hipEventRecord(E.kt_ev[2 * E.kt_used++ + 1], stream));
@@ -1481,6 +1133,26 @@ int gwh_set_bins_per_lane(int bpl)
         Event &ev = E.events[i];
         if (!ev.live || ev.pins > 0) continue; /* events held by a step graph keep their tiling */
         ev.desc.bpl = gwh_choose_bpl(ev.N);
+        GWH_HIP(hipMemcpyAsync(E.d_events + i, &ev.desc, sizeof(EventDesc), hipMemcpyHostToDevice, E.stream));
+    }
+    GWH_HIP(hipStreamSynchronize(E.stream));
+    return GWH_OK;
+}
+
+int gwh_set_tile_lanes(int lanes)
+{
+    if (!(lanes == 0 || lanes == 64 || lanes == 128 || lanes == 256 || lanes == 512 || lanes == 1024))
+        return gwh_set_error(GWH_ERR_INVALID, "lanes per tile must be 0, 64, 128, 256, 512 or 1024");
+    Engine &E = gwh_engine();
+    std::lock_guard<std::recursive_mutex> lk(E.mu);
+    int st = gwh_ensure_init();
+    if (st != GWH_OK) return st;
+    E.lanes = lanes;
+    /* re-tile the resident events (later results may differ in the last bits from earlier ones) */
+    for (int i = 0; i < GWH_MAX_EVENTS; i++) {
+        Event &ev = E.events[i];
+        if (!ev.live || ev.pins > 0) continue; /* events held by a step graph keep their tiling */
+        ev.desc.lanes = gwh_choose_lanes(ev.N);
         GWH_HIP(hipMemcpyAsync(E.d_events + i, &ev.desc, sizeof(EventDesc), hipMemcpyHostToDevice, E.stream));
     }
     GWH_HIP(hipStreamSynchronize(E.stream));
@@ -1716,6 +1388,10 @@ int gwh_time_loglike_batch_device(int events0, int model, int precision, const v
     int st = gwh_ensure_init();
     if (st != GWH_OK) return st;
     if (reps < 1) reps = 1;
+    /* one untimed launch: first-touch of the batch's buffers */
+    int rc0 = gwh_loglike_launch(nullptr, events0, model, precision, (const gwh_params *)d_params, n, d_logl, E.stream,
+                                 false, nullptr);
+    if (rc0 != GWH_OK) return rc0;
     hipEvent_t a, b;
     GWH_HIP(hipEventCreate(&a));
     GWH_HIP(hipEventCreate(&b));
@@ -1756,12 +1432,14 @@ int gwh_loglike_multi_device(const int *events, const int *d_events, int model, 
                               nullptr, events);
 }
 
-/* Coordinate search over the bins per lane of each event-size bucket (floor(log2 N)): for every bucket
- * present, time the whole batch with that bucket's events tiled at 1, 2, 4 and 8 bins per lane (the
- * others held), keep the fastest (median of `reps`), and record it in the engine's table for later
- * events of the same size.  table (optional, 32 ints) receives the choice per bucket (0 = absent). */
-int gwh_autotune_bins_per_lane(const int *events, const int *d_events, int model, int precision, const void *d_params,
-                               int n, double *d_logl, int reps, int *table)
+/* Search over the tiling of each event-size bucket (floor(log2 N)): for every bucket present, time the whole batch
+ * with that bucket's events tiled at each candidate (lanes per tile x bins per lane; the other buckets held), keep
+ * the fastest (median of `reps` launches), and record it in the engine's tables for later events of the same size.
+ * lanes_cand == nullptr: bins per lane only, each event keeping its lanes.  The tables (optional, 32 ints) receive
+ * the choice per bucket (0 = absent). */
+static int autotune_tiling(const int *events, const int *d_events, int model, int precision, const void *d_params,
+                           int n, double *d_logl, int reps, const int *lanes_cand, int nlanes, int *bpl_table,
+                           int *lanes_table)
 {
     Engine &E = gwh_engine();
     std::lock_guard<std::recursive_mutex> lk(E.mu);
@@ -1781,21 +1459,27 @@ int gwh_autotune_bins_per_lane(const int *events, const int *d_events, int model
         bucket_of[e] = lg;
         present[lg] = true;
     }
-    auto retile = [&](int lg, int b) -> int {
+    /* lanes <= 0: keep each event's lanes */
+    auto retile = [&](int lg, int b, int lanes) -> int {
         for (int e = 0; e < GWH_MAX_EVENTS; e++) {
             if (bucket_of[e] != lg) continue;
             E.events[e].desc.bpl = b;
+            if (lanes > 0) E.events[e].desc.lanes = lanes;
             GWH_HIP(hipMemcpyAsync(E.d_events + e, &E.events[e].desc, sizeof(EventDesc), hipMemcpyHostToDevice,
                                    E.stream));
         }
         return GWH_OK;
     };
+    /* a batch on one event is timed as such a batch runs: the tile-major grid, not the work lists */
+    bool one_event = true;
+    for (int i = 1; i < n && one_event; i++) one_event = events[i] == events[0];
     auto time_batch = [&](double *ms) -> int {
         std::vector<float> t(reps);
         for (int r = 0; r < reps; r++) {
             GWH_HIP(hipEventRecord(E.ev0, E.stream));
-            int rc = gwh_loglike_launch(d_events, events[0], model, precision, (const gwh_params *)d_params, n, d_logl,
-                                        E.stream, false, nullptr, events);
+            int rc = gwh_loglike_launch(one_event ? nullptr : d_events, events[0], model, precision,
+                                        (const gwh_params *)d_params, n, d_logl, E.stream, false, nullptr,
+                                        one_event ? nullptr : events);
             if (rc != GWH_OK) return rc;
             GWH_HIP(hipEventRecord(E.ev1, E.stream));
             GWH_HIP(hipEventSynchronize(E.ev1));
@@ -1806,29 +1490,53 @@ int gwh_autotune_bins_per_lane(const int *events, const int *d_events, int model
         return GWH_OK;
     };
     static const int cand[4] = {1, 2, 4, 8};
+    static const int keep[1] = {0};
+    if (!lanes_cand) {
+        lanes_cand = keep;
+        nlanes = 1;
+    }
     int rc;
     double warm;
     if ((rc = time_batch(&warm)) != GWH_OK) return rc; /* first-touch and code-load warm-up */
     for (int lg = 0; lg < 32; lg++) {
         if (!present[lg]) continue;
         double best = 1e300;
-        int bestb = 8;
-        for (int b : cand) {
-            if ((rc = retile(lg, b)) != GWH_OK) return rc;
-            double ms;
-            if ((rc = time_batch(&ms)) != GWH_OK) return rc;
-            if (ms < best) {
-                best = ms;
-                bestb = b;
+        int bestb = 8, bestl = 0;
+        for (int li = 0; li < nlanes; li++)
+            for (int b : cand) {
+                if ((rc = retile(lg, b, lanes_cand[li])) != GWH_OK) return rc;
+                double ms;
+                if ((rc = time_batch(&ms)) != GWH_OK) return rc;
+                if (ms < best) {
+                    best = ms;
+                    bestb = b;
+                    bestl = lanes_cand[li];
+                }
             }
-        }
         E.tune[lg] = bestb;
-        if ((rc = retile(lg, bestb)) != GWH_OK) return rc;
+        if (bestl > 0) E.tune_lanes[lg] = bestl;
+        if ((rc = retile(lg, bestb, bestl)) != GWH_OK) return rc;
     }
     GWH_HIP(hipStreamSynchronize(E.stream));
-    if (table)
-        for (int lg = 0; lg < 32; lg++) table[lg] = present[lg] ? E.tune[lg] : 0;
+    for (int lg = 0; lg < 32; lg++) {
+        if (bpl_table) bpl_table[lg] = present[lg] ? E.tune[lg] : 0;
+        if (lanes_table) lanes_table[lg] = present[lg] ? (E.tune_lanes[lg] > 0 ? E.tune_lanes[lg] : GWH_TPB) : 0;
+    }
     return GWH_OK;
+}
+
+int gwh_autotune_bins_per_lane(const int *events, const int *d_events, int model, int precision, const void *d_params,
+                               int n, double *d_logl, int reps, int *table)
+{
+    return autotune_tiling(events, d_events, model, precision, d_params, n, d_logl, reps, nullptr, 0, table, nullptr);
+}
+
+int gwh_autotune_tiles(const int *events, const int *d_events, int model, int precision, const void *d_params, int n,
+                       double *d_logl, int reps, int *bpl_table, int *lanes_table)
+{
+    static const int lanes[5] = {64, 128, 256, 512, 1024};
+    return autotune_tiling(events, d_events, model, precision, d_params, n, d_logl, reps, lanes, 5, bpl_table,
+                           lanes_table);
 }
 
 int gwh_kernel_timing_start(int max_launches)
@@ -1922,6 +1630,7 @@ static int upload_single(const double *f, int n, const gwh_params *p, int model,
     ds.D = ndet;
     ds.N = n;
     ds.bpl = 1;
+    ds.lanes = GWH_TPB;
     for (int d = 0; d < GWH_MAXDET; d++) ds.det[d] = (dets && d < ndet) ? dets[d] : det;
     GWH_HIP(hipMemcpyAsync(E.d_events + (GWH_MAX_EVENTS - 1), &ds, sizeof(ds), hipMemcpyHostToDevice, E.stream));
     launch_setup(model, 1, E.stream, E.params.as<const gwh_params>(), nullptr, GWH_MAX_EVENTS - 1,

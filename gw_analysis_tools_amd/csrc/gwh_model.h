@@ -1135,6 +1135,7 @@ __device__ __forceinline__ double gwh_ppe_bin(const InstCoef &C, double lf)
     return t;
 }
 
+template <bool FP32_EXP = false>
 __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, const BinPow &bp, double &A, double &phi,
                                                  double &u_out, double &lx_out)
 {
@@ -1149,7 +1150,7 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, co
         ar = pn + nr;
     } else if (f > C.f3) {
         double dl = f - C.fRD;
-        ar = C.mr_k * gwh_exp_table(-C.mr_g2 * dl) / (dl * dl + C.mr_w2);
+        ar = C.mr_k * (FP32_EXP ? exp_fp32(-C.mr_g2 * dl) : gwh_exp_table(-C.mr_g2 * dl)) / (dl * dl + C.mr_w2);
     } else {
         ar = C.delta[0] + Mf * (C.delta[1] + Mf * (C.delta[2] + Mf * (C.delta[3] + Mf * C.delta[4])));
     }

@@ -95,4 +95,13 @@ __device__ __forceinline__ double gwh_exp_table(double x)
     return ldexp(gwh_exp2_table[ni & 63] * p, ni >> 6);
 }
 
+/* e^x for x <= 0 of moderate size: FP64 split x log2(e) = n + t, FP32 2^t (~6e-8 relative), exact scaling */
+__device__ __forceinline__ double exp_fp32(double x)
+{
+    double y = x * 1.4426950408889634074;
+    double n = rint(y);
+    float t = (float)(y - n);
+    return ldexp((double)__builtin_amdgcn_exp2f(t), (int)n);
+}
+
 #endif

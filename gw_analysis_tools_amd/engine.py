@@ -16,7 +16,7 @@ __all__ = ["fourier_waveform", "fourier_detector_response", "create_coherent_GW_
            "log_likelihood", "log_likelihood_internal", "mcmc_likelihood_extrinsic", "fisher_numerical",
            "fisher_derivatives", "unpack_parameters", "repack_parameters", "detector_id", "dtoa",
            "gps_to_gmst_radian", "calculate_snr_internal", "data_snr", "snr_batch", "Broker", "Event", "loglike_batch",
-           "fisher_batch", "DeviceArray", "set_bins_per_lane", "synchronize", "LoglikeGraph", "DevicePT",
+           "fisher_batch", "DeviceArray", "set_bins_per_lane", "set_tile_lanes", "autotune_tiles", "synchronize", "LoglikeGraph", "DevicePT",
            "maximized_log_likelihood_aligned_internal", "maximized_log_likelihood_unaligned_internal",
            "maximized_coal_log_likelihood_internal", "maximized_loglike_batch", "gauleg", "THREADED_CALLS",
            "write_threaded_calls_input"]
@@ -45,6 +45,24 @@ def synchronize():
 
 def set_bins_per_lane(bpl):
     check(lib().gwh_set_bins_per_lane(int(bpl)))
+
+
+def set_tile_lanes(lanes):
+    """lanes (threads) per likelihood tile: 64, 128, 256, 512 or 1024; 0 = automatic (autotuned, else 256)"""
+    check(lib().gwh_set_tile_lanes(int(lanes)))
+
+
+def autotune_tiles(event_ids, d_event_ids, model, precision, d_params, d_logl, reps=3):
+    """Tile the events of a batch by timing it over lanes per tile x bins per lane per event-size bucket
+    (gwh_autotune_tiles).  event_ids: the batch's host ids (int32, one per instance); d_event_ids, d_params,
+    d_logl: DeviceArrays of the same batch.  Returns {floor(log2 N): (bins_per_lane, lanes)}."""
+    ids = np.ascontiguousarray(event_ids, dtype=np.int32)
+    bpl, lanes = np.zeros(32, dtype=np.int32), np.zeros(32, dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    check(lib().gwh_autotune_tiles(ids.ctypes.data_as(ip), d_event_ids.ptr, int(model), int(precision),
+                                   d_params.ptr, len(ids), d_logl.ptr, int(reps), bpl.ctypes.data_as(ip),
+                                   lanes.ctypes.data_as(ip)))
+    return {i: (int(bpl[i]), int(lanes[i])) for i in range(32) if bpl[i]}
 
 
 def fourier_waveform(f, p, model=IMRPHENOMD):

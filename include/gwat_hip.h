@@ -87,7 +87,8 @@ extern "C" {
 
 /* arithmetic of the batched kernels */
 #define GWH_FP64 0  /* everything in FP64 */
-#define GWH_MIXED 1 /* FP64 phase, amplitude, products and sums; FP32 sin/cos and merger-ringdown exp */
+#define GWH_MIXED 1 /* FP64 except the FP32 operations that are faster on gfx950: the IMRPhenomD merger-ringdown
+                       exponential and the IMRPhenomPv2 reciprocal square root's seed */
 
 /* integration rules of Log_Likelihood_internal (mcmc_gw.cpp:801-868) */
 #define GWH_SIMPSONS 0
@@ -328,20 +329,33 @@ int gwh_loglike_batch_device_noalloc(const int *d_events, int events0, int model
 
 /* Multi-event batch on device buffers with the instance -> event map also given on the host
  * (events: n ids, host; d_events: the same ids in device memory).  The host copy sizes one flattened
- * (instance, tile) work list per bins-per-lane class, so events of very different lengths share one
- * launch without empty work items. */
+ * (instance, tile) work list per tiling class (bins per lane x lanes per tile), so events of very different
+ * lengths share one launch without empty work items. */
 int gwh_loglike_multi_device(const int *events, const int *d_events, int model, int precision, const void *d_params,
                              int n, double *d_logl, void *stream);
 
 /* Autotuner: for each event-size bucket (floor(log2 N)) in the batch, time the batch with that bucket's
- * events at 1, 2, 4 and 8 bins per lane and keep the fastest (median of reps); the choice is stored for
- * later events of that size and written to table[32] (0 = bucket absent).  Results computed after the
- * call may differ from earlier ones in the last bits (a different tiling sums in a different order). */
+ * events at 1, 2, 4 and 8 bins per lane (each event keeping its lanes per tile) and keep the fastest (median
+ * of reps); the choice is stored for later events of that size and written to table[32] (0 = bucket absent).
+ * Results computed after the call may differ from earlier ones in the last bits (a different tiling sums in a
+ * different order).  The project's own tuning step: the reference has no counterpart (its likelihood is one
+ * serial loop per call, mcmc_gw.cpp:801-868). */
 int gwh_autotune_bins_per_lane(const int *events, const int *d_events, int model, int precision, const void *d_params,
                                int n, double *d_logl, int reps, int *table);
 
+/* Autotuner over the whole tiling: as gwh_autotune_bins_per_lane, over lanes per tile {64, 128, 256, 512, 1024}
+ * (the kernels' workgroup size) x bins per lane {1, 2, 4, 8}, 20 candidates per bucket.  bpl_table[32] and
+ * lanes_table[32] (optional) receive the choices (0 = bucket absent). */
+int gwh_autotune_tiles(const int *events, const int *d_events, int model, int precision, const void *d_params, int n,
+                       double *d_logl, int reps, int *bpl_table, int *lanes_table);
+
 /* Tuning of the bins-per-lane of the likelihood kernels (1, 2, 4 or 8; 0 = automatic). */
 int gwh_set_bins_per_lane(int bpl);
+
+/* Tuning of the lanes per tile (workgroup size) of the likelihood kernels (64, 128, 256, 512 or 1024;
+ * 0 = automatic: the autotuned choice of the event's size bucket, else 256).  A tile holds lanes x bins-per-lane
+ * bins. */
+int gwh_set_tile_lanes(int lanes);
 
 /* Batched network Fisher: n points, order 2 central differences in the given basis (with the ppE betas or
  * listed gIMR deltas appended for those models, as gwh_fisher_numerical_model), per-detector projection
