@@ -167,7 +167,8 @@ def bench_fisher(G, W, ev, n_points=2048):
     g = W.rng(3, 0, stream=8)
     P["chip"] = g.uniform(0.05, 0.9, n_points)
     P["phip"] = g.uniform(0, 2 * np.pi, n_points)
-    G.fisher_batch(ev, P[:8], G.IMRPHENOMPV2, G.BASIS_PV2_FULL, 13, precision=G.FP64)  # warm-up
+    for prec in (G.FP64, G.FISHER_GRAM_FP32):  # warm-up: each mode's buffers reach their size outside the timing
+        G.fisher_batch(ev, P, G.IMRPHENOMPV2, G.BASIS_PV2_FULL, 13, precision=prec, with_sigma=True)
     out = {}
     res = {}
     for name, prec in (("fp64", G.FP64), ("fp32_gram", G.FISHER_GRAM_FP32)):

@@ -30,6 +30,11 @@ typedef float gwh_f4 __attribute__((ext_vector_type(4)));
 #define GWH_FROWS 16      /* parameters padded to the MFMA tile */
 #define GWH_MAXSTEN 4     /* order 4 uses 4 evaluations per parameter */
 #define GWH_GRAM_ROWS 4096 /* operand rows per Gram block */
+#define GWH_FUSED_BINS 64  /* bins per block of the fused stencil + Gram kernel: one per lane of each wave */
+#define GWH_FUSED_MAX_POINTS 128 /* points per pass of the fused path (passes pipeline with the host staging) */
+#ifndef GWH_FUSED_WAVES
+#define GWH_FUSED_WAVES 8 /* waves per block of the fused kernel: GWH_FROWS / GWH_FUSED_WAVES operand rows each */
+#endif
 
 #include "gwh_basis.h"
 
@@ -409,6 +414,104 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_gram_kernel(const T *__restrict__
     }
 }
 
+/* The stencil and the FP64 Gram in one kernel: one block of GWH_FROWS waves per (point, chunk of 64 bins).  Wave
+ * i evaluates operand row i -- the S stencil evaluations of component i, unrolled, projected on every detector and
+ * weighted by sqrt(4 w / S_n) -- for the chunk's bins, one per lane, into LDS planes (detector, re/im) x row x bin
+ * (zero for padding rows and bins past the grid); then the waves contract every plane on the FP64 matrix cores
+ * (v_mfma_f64_16x16x4f64: 4 bins per instruction, lane l supplying component l % 16 of bin l / 16 as both
+ * operands; wave w takes the (plane, 4-bin) slices w, w + 16, ...), and the block writes one 16x16 partial Gram,
+ * summed over its waves in a fixed order.  The operand rows never leave the CU: the two-kernel path writes and
+ * re-reads them through HBM (2 x 1.07 GB per 86-point pass of the 13-parameter IMRPhenomPv2 Fisher on the
+ * 16225-bin 3-detector event). */
+template <int MODEL, int S, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void gwh_fisher_fused_kernel(StencilArgs a,
+                                                                     const EventDesc *__restrict__ events,
+                                                                     int event0, double *__restrict__ partial, int nbc)
+{
+    __shared__ double lds[2 * GWH_MAXDET][GWH_FROWS][GWH_FUSED_BINS]; /* 64 KiB */
+    const int pt = blockIdx.y, bc = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const EventDesc &E = events[event0];
+    const int N = E.N;
+    const int k0 = bc * GWH_FUSED_BINS + lane;
+    const bool in = k0 < N;
+    const int k = in ? k0 : N - 1;
+    const InstCoef *C = a.coef + (size_t)pt * a.per_point;
+    const double *cw = a.cw + (size_t)pt * GWH_FROWS * GWH_MAXSTEN;
+    const unsigned char *nt = a.no_tof + (size_t)pt * a.per_point;
+    /* wave wv evaluates rows wv, wv + WAVES, ... (unrolled: no loop for the compiler to hoist values across) */
+#pragma unroll
+    for (int h = 0; h < GWH_FROWS / WAVES; h++) {
+        const int i = wv + WAVES * h;
+        double dr[GWH_MAXDET] = {0, 0, 0, 0}, di[GWH_MAXDET] = {0, 0, 0, 0};
+        if (i < a.dim) { /* uniform over the wave */
+            const double f = gwh_g(E.f)[k];
+            const BinPow bp = gwh_load_bin(E, k);
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                const InstCoef &Ci = C[i * S + s];
+                if (!(f <= Ci.fcut)) continue; /* h = 0 above 0.2/M of this evaluation */
+                const double c = cw[i * GWH_MAXSTEN + s];
+                const bool tof = !nt[i * S + s];
+                if (MODEL == GWH_IMRPHENOMD) {
+                    double A, phi, u, lx;
+                    gwh_phenomd_fp64(Ci, f, bp, A, phi, u, lx);
+                    for (int dd = 0; dd < GWH_MAXDET; dd++) {
+                        if (dd >= a.d_count) break;
+                        const int d = a.d_first + dd;
+                        double sn, cs;
+                        gwh_sincos_table(phi + (tof ? Ci.det[d][3] : 0.0) * f, sn, cs);
+                        const double er = A * cs, ei = -A * sn; /* A K e^{-i(phi + 2 pi dt f)} */
+                        dr[dd] += c * (er * Ci.det[d][0] - ei * Ci.det[d][1]);
+                        di[dd] += c * (er * Ci.det[d][1] + ei * Ci.det[d][0]);
+                    }
+                } else {
+                    double A, Phi, hpr, hpi, hcr, hci;
+                    gwh_pv2_fp64(Ci, f, bp, A, Phi, hpr, hpi, hcr, hci);
+                    for (int dd = 0; dd < GWH_MAXDET; dd++) {
+                        if (dd >= a.d_count) break;
+                        const int d = a.d_first + dd;
+                        const double Gr = Ci.det[d][0] * hpr + Ci.det[d][1] * hcr;
+                        const double Gi = Ci.det[d][0] * hpi + Ci.det[d][1] * hci;
+                        double sn, cs;
+                        gwh_sincos_table(Phi + (tof ? Ci.det[d][3] : 0.0) * f, sn, cs);
+                        dr[dd] += c * A * (Gr * cs + Gi * sn);
+                        di[dd] += c * A * (Gi * cs - Gr * sn);
+                    }
+                }
+            }
+            for (int dd = 0; dd < GWH_MAXDET; dd++) {
+                if (dd >= a.d_count) break;
+                const double sw = in ? sqrt(gwh_g(E.wS)[(size_t)(a.d_first + dd) * N + k]) : 0.0;
+                dr[dd] *= sw;
+                di[dd] *= sw;
+            }
+        }
+        for (int dd = 0; dd < GWH_MAXDET; dd++) {
+            if (dd >= a.d_count) break;
+            lds[2 * dd][i][lane] = dr[dd];
+            lds[2 * dd + 1][i][lane] = di[dd];
+        }
+    }
+    __syncthreads();
+    gwh_d4 acc = {0, 0, 0, 0};
+    const int nslice = 2 * a.d_count * (GWH_FUSED_BINS / 4);
+    for (int t = wv; t < nslice; t += WAVES) {
+        const int pl = t / (GWH_FUSED_BINS / 4), q = t % (GWH_FUSED_BINS / 4);
+        const double x = lds[pl][lane & 15][4 * q + (lane >> 4)];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, acc, 0, 0, 0);
+    }
+    __syncthreads(); /* the planes are read: the LDS now holds the waves' partial matrices */
+    double(*part)[256] = reinterpret_cast<double(*)[256]>(&lds[0][0][0]);
+    for (int j = 0; j < 4; j++) part[wv][((lane >> 4) + 4 * j) * 16 + (lane & 15)] = acc[j];
+    __syncthreads();
+    if (threadIdx.x < 256) {
+        double sum = part[0][threadIdx.x];
+        for (int w = 1; w < WAVES; w++) sum += part[w][threadIdx.x];
+        partial[((size_t)pt * nbc + bc) * 256 + threadIdx.x] = sum;
+    }
+}
+
 /* fixed-order FP64 sum of the Gram partials; symmetric output (the upper and lower triangles are
  * averaged so the result is exactly symmetric, as calculate_fisher_elements assigns out[k][j] = out[j][k]) */
 __global__ void gwh_gram_reduce_kernel(const double *__restrict__ partial, int nblk, int dim, double scale,
@@ -518,16 +621,22 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
     const int N = ev.N;
     const size_t nrows = (size_t)(sky ? 1 : d_count) * N * 2;
     const size_t row_bytes = GWH_FROWS * ((precision == GWH_FISHER_GRAM_FP32) ? 4 : 8);
-    /* points per pass: bounded operand buffer (F.max_operand, 1 GiB unless a caller's budget says less) */
+    /* the FP64 Gram of a network Fisher runs fused with the stencil (no operand rows in HBM); the sky-averaged
+       basis, the FP32 Gram and the bare derivatives go through operand rows */
+    const bool fused = !raw && !sky && precision != GWH_FISHER_GRAM_FP32;
+    /* points per pass: bounded operand buffer (F.max_operand, 1 GiB unless a caller's budget says less); fused,
+       the partial Gram matrices take its place */
     size_t per_pt_bytes = nrows * row_bytes;
+    const int nplanes = (int)(nrows / N); /* (detector, re/im) planes of a point's operand rows */
+    const int blk_per_plane = (N + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS;
+    int nblk = fused ? (N + GWH_FUSED_BINS - 1) / GWH_FUSED_BINS : nplanes * blk_per_plane;
+    if (fused) per_pt_bytes = (size_t)nblk * 256 * 8;
     int chunk = (int)std::max<size_t>(1, F.max_operand / per_pt_bytes);
     if (chunk > n) chunk = n;
     if (chunk > 65535) chunk = 65535;
-    const int nplanes = (int)(nrows / N); /* (detector, re/im) planes of a point's operand rows */
-    const int blk_per_plane = (N + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS;
-    int nblk = nplanes * blk_per_plane;
+    if (fused && chunk > GWH_FUSED_MAX_POINTS) chunk = GWH_FUSED_MAX_POINTS;
     int rc;
-    if ((rc = F.deriv.ensure((size_t)chunk * per_pt_bytes)) != GWH_OK) return rc;
+    if (!fused && (rc = F.deriv.ensure((size_t)chunk * per_pt_bytes)) != GWH_OK) return rc;
     if ((rc = F.gram.ensure((size_t)chunk * nblk * 256 * 8)) != GWH_OK) return rc;
     if ((rc = F.out.ensure((size_t)n * dim * dim * 8)) != GWH_OK) return rc;
     std::vector<gwh_params> inst;
@@ -603,7 +712,24 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
             GWH_HIP(hipStreamSynchronize(F.stream));
             return GWH_OK;
         }
-        if (precision != GWH_FISHER_GRAM_FP32) {
+        if (fused) {
+            const dim3 gf(nblk, np), bf(GWH_FUSED_WAVES * 64);
+            if (!pv2) {
+                if (S == 4)
+                    hipLaunchKernelGGL((gwh_fisher_fused_kernel<GWH_IMRPHENOMD, 4, GWH_FUSED_WAVES>), gf, bf, 0, F.stream, a, evp, event,
+                                       F.gram.as<double>(), nblk);
+                else
+                    hipLaunchKernelGGL((gwh_fisher_fused_kernel<GWH_IMRPHENOMD, 2, GWH_FUSED_WAVES>), gf, bf, 0, F.stream, a, evp, event,
+                                       F.gram.as<double>(), nblk);
+            } else {
+                if (S == 4)
+                    hipLaunchKernelGGL((gwh_fisher_fused_kernel<GWH_IMRPHENOMPV2, 4, GWH_FUSED_WAVES>), gf, bf, 0, F.stream, a, evp,
+                                       event, F.gram.as<double>(), nblk);
+                else
+                    hipLaunchKernelGGL((gwh_fisher_fused_kernel<GWH_IMRPHENOMPV2, 2, GWH_FUSED_WAVES>), gf, bf, 0, F.stream, a, evp,
+                                       event, F.gram.as<double>(), nblk);
+            }
+        } else if (precision != GWH_FISHER_GRAM_FP32) {
             launch_stencil<double>(pv2, g, F.stream, a, evp, event, F.deriv.as<double>());
             GWH_HIP(hipGetLastError());
             hipLaunchKernelGGL(gwh_gram_kernel<double>, dim3(nblk, np), dim3(GWH_TPB), 0, F.stream,
@@ -677,12 +803,15 @@ size_t gwh_fisher_batch_bytes(int event, int model, int basis, int precision, co
     const bool sky = (basis == GWH_BASIS_PHENOMD_SKYAVG);
     const size_t per_point = (size_t)dim * 2 + (sky ? 1 : 0);
     const size_t nrows = (size_t)(sky ? 1 : ev.D) * ev.N * 2;
-    const size_t per_pt_bytes = nrows * GWH_FROWS * ((precision == GWH_FISHER_GRAM_FP32) ? 4 : 8);
+    const bool fused = !sky && precision != GWH_FISHER_GRAM_FP32;
+    const size_t nblk = fused ? (ev.N + GWH_FUSED_BINS - 1) / GWH_FUSED_BINS
+                              : (nrows / ev.N) * ((ev.N + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS);
+    const size_t per_pt_bytes = fused ? nblk * 256 * 8 : nrows * GWH_FROWS * ((precision == GWH_FISHER_GRAM_FP32) ? 4 : 8);
     size_t chunk = std::max<size_t>(1, max_operand / per_pt_bytes);
     chunk = std::min<size_t>(chunk, std::min(n, 65535));
-    const size_t nblk = (nrows / ev.N) * ((ev.N + GWH_GRAM_ROWS - 1) / GWH_GRAM_ROWS);
+    if (fused) chunk = std::min<size_t>(chunk, GWH_FUSED_MAX_POINTS);
     const size_t ni = chunk * per_point;
-    return gwh_grown_capacity(chunk * per_pt_bytes, F.deriv.cap) +
+    return (fused ? F.deriv.cap : gwh_grown_capacity(chunk * per_pt_bytes, F.deriv.cap)) +
            gwh_grown_capacity(chunk * nblk * 256 * 8, F.gram.cap) +
            gwh_grown_capacity((size_t)n * dim * dim * 8, F.out.cap) +
            (want_sigma ? gwh_grown_capacity((size_t)n * dim * 8, F.sig.cap) : F.sig.cap) +
