@@ -785,39 +785,48 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const 
                                                                const int *__restrict__ active, int model, int what,
                                                                double *__restrict__ logl, const int *__restrict__ count)
 {
-    const int i = blockIdx.x * (GWH_TPB / 64) + (threadIdx.x >> 6);
+    /* the instance index is uniform over the wave: scalar loads, and the loads that do not depend on each other
+       (tile count, event id, the instance's detector constants) are issued together before the partial sums */
+    const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * (GWH_TPB / 64) + (threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     if (i >= W.n || (count && i >= *count)) return; /* whole waves leave together */
-    const int ev = W.inst_event ? W.inst_event[i] : W.event0;
-    const EventDesc &E = events[ev];
-    const int nv = 3 * E.D;
-    const size_t t0 = (size_t)i * W.tiles0;
     const int nt = active[i];
+    const int ev = W.inst_event ? W.inst_event[i] : W.event0;
+    const InstCoef &C = coefs[i];
+    double cdet[GWH_MAXDET][3];
+#pragma unroll
+    for (int d = 0; d < GWH_MAXDET; d++)
+        for (int q = 0; q < 3; q++) cdet[d][q] = C.det[d][q];
+    const EventDesc &E = events[ev];
+    const size_t t0 = (size_t)i * W.tiles0;
     if (nt < 0) { /* instance not evaluated (count or mask of the launch) */
         if (lane == 0) logl[i] = __longlong_as_double(0x7ff8000000000000ll);
         return;
     }
+    /* every slot of a tile's partial record is summed (slots past 3 D are unused), so the loads need not wait
+       for the event's detector count */
     double s[GWH_PARTIAL_STRIDE];
 #pragma unroll
     for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) s[j] = 0;
     for (int t = lane; t < nt; t += 64) {
         const double *p = partial + (t0 + t) * GWH_PARTIAL_STRIDE;
 #pragma unroll
-        for (int j = 0; j < GWH_PARTIAL_STRIDE; j++)
-            if (j < nv) s[j] += p[j];
+        for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) s[j] += p[j];
     }
+    const int D = E.D, nv = 3 * D;
 #pragma unroll
     for (int j = 0; j < GWH_PARTIAL_STRIDE; j++)
         if (j < nv) s[j] = wave_sum(s[j]);
     if (lane != 0) return;
-    const InstCoef &C = coefs[i];
     double ll = 0;
-    for (int d = 0; d < E.D; d++) {
+#pragma unroll
+    for (int d = 0; d < GWH_MAXDET; d++) {
+        if (d >= D) break;
         double hh, dh;
         if (!gwh_is_pv2(model)) {
             double ha = s[3 * d], zr = s[3 * d + 1], zi = s[3 * d + 2];
-            hh = C.det[d][2] * ha;
-            dh = C.det[d][0] * zr + C.det[d][1] * zi;
+            hh = cdet[d][2] * ha;
+            dh = cdet[d][0] * zr + cdet[d][1] * zi;
         } else {
             hh = s[3 * d];
             dh = s[3 * d + 1];
