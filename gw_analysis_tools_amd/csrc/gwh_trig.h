@@ -65,7 +65,7 @@ template <int NT> __device__ __forceinline__ void gwh_sincos_cw(double x, double
  * exact, |y| <= 1/512; sin and cos of t = 2 pi y (|t| <= pi / 256) by their Taylor series to t^5 and t^6
  * (truncation below 1e-17), rotated by the table entry.  Error a few ulp of 1 beyond the rounding of the phase
  * to turns (the same reduction the FP32 path takes); valid for |phase| < 2^44 rad. */
-__device__ __forceinline__ void gwh_sincos_table(double phase, double &s, double &c)
+template <class Tab> __device__ __forceinline__ void gwh_sincos_tab(double phase, double &s, double &c, const Tab *tab)
 {
     const double r = phase * 0.15915494309189533577;
     const double kRound = 6755399441055744.0; /* 1.5 * 2^52 */
@@ -77,9 +77,14 @@ __device__ __forceinline__ void gwh_sincos_table(double phase, double &s, double
     const double t2 = t * t;
     const double sl = t * fma(t2, fma(t2, 1.0 / 120.0, -1.0 / 6.0), 1.0);
     const double cl = fma(t2, fma(t2, fma(t2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
-    const double2 T = gwh_turn_table[j];
+    const double2 T = tab[j];
     s = fma(T.y, cl, T.x * sl);
     c = fma(T.x, cl, -T.y * sl);
+}
+/* the same through the table in global memory */
+__device__ __forceinline__ void gwh_sincos_table(double phase, double &s, double &c)
+{
+    gwh_sincos_tab(phase, s, c, gwh_turn_table);
 }
 
 /* e^x for |x| < 700 through the 2^(j/64) table: x = n ln2 / 64 + r with n = rint(64 x / ln2) and the reduction

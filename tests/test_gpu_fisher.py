@@ -205,14 +205,19 @@ def test_fisher_batch_mixed_vs_fp64(basis, dim, model):
     assert e <= MIXED_FISHER_TOL, e
 
 
-def test_fisher_batch_composition_independent():
-    """a point's Fisher matrix is bitwise the same alone and inside a larger batch"""
+@pytest.mark.parametrize("model,basis,dim", [(G.IMRPHENOMD, G.BASIS_PHENOMD_MCMC, 11),
+                                             (G.IMRPHENOMPV2, G.BASIS_PV2_MCMC, 15)])
+def test_fisher_batch_composition_independent(model, basis, dim):
+    """a point's Fisher matrix is bitwise the same alone and inside a larger batch, also in the later passes of a
+    batch (300 points run as passes of 128, 128 and 44), for the network Fisher of both models"""
     f, psd, dets, ev = _network()
-    P = _points(G.IMRPHENOMD, 8)
-    full = G.fisher_batch(ev, P, G.IMRPHENOMD, G.BASIS_PHENOMD_MCMC, 11)
-    for i in (0, 5):
-        one = G.fisher_batch(ev, P[i:i + 1], G.IMRPHENOMD, G.BASIS_PHENOMD_MCMC, 11)
-        assert np.array_equal(one[0], full[i])
+    P = _points(model, 300)
+    full, sig = G.fisher_batch(ev, P, model, basis, dim, with_sigma=True)
+    for i in (0, 5, 130, 255, 256, 299):
+        one, s1 = G.fisher_batch(ev, P[i:i + 1], model, basis, dim, with_sigma=True)
+        assert np.array_equal(one[0], full[i]), i
+        assert np.array_equal(s1[0], sig[i], equal_nan=True), i
+    assert np.all(np.isfinite(full))
 
 
 @pytest.mark.parametrize("basis,method,dim,sky,model", [BASES[0], BASES[2], BASES[3]])
