@@ -512,80 +512,118 @@ __global__ __launch_bounds__(WAVES * 64) void gwh_fisher_fused_kernel(StencilArg
     }
 }
 
-/* fixed-order FP64 sum of the Gram partials; symmetric output (the upper and lower triangles are
- * averaged so the result is exactly symmetric, as calculate_fisher_elements assigns out[k][j] = out[j][k]) */
-__global__ void gwh_gram_reduce_kernel(const double *__restrict__ partial, int nblk, int dim, double scale,
-                                       double *__restrict__ out)
+/* Fixed-order FP64 sum of the Gram partials, one block of 1024 threads per point: thread (q, e) sums element e of
+ * the 16x16 partials of blocks q, q + 4, ... in increasing order, the four sums are added as (s0 + s1) + (s2 + s3),
+ * and the output is symmetric (the upper and lower triangles are averaged, as calculate_fisher_elements assigns
+ * out[k][j] = out[j][k]). */
+__global__ __launch_bounds__(1024) void gwh_gram_reduce_kernel(const double *__restrict__ partial, int nblk, int dim,
+                                                               double scale, double *__restrict__ out)
 {
+    __shared__ double red[4][256];
     const int pt = blockIdx.x;
-    for (int e = threadIdx.x; e < dim * dim; e += blockDim.x) {
-        int i = e / dim, j = e % dim;
-        double s1 = 0, s2 = 0;
-        for (int b = 0; b < nblk; b++) {
-            s1 += partial[((size_t)pt * nblk + b) * 256 + i * 16 + j];
-            s2 += partial[((size_t)pt * nblk + b) * 256 + j * 16 + i];
-        }
-        out[(size_t)pt * dim * dim + e] = 0.5 * (s1 + s2) * scale;
+    const int e = threadIdx.x & 255, q = threadIdx.x >> 8;
+    const double *pp = partial + (size_t)pt * nblk * 256 + e;
+    double acc = 0;
+    for (int b = q; b < nblk; b += 4) acc += pp[(size_t)b * 256];
+    red[q][e] = acc;
+    __syncthreads();
+    if (q == 0) red[0][e] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+    __syncthreads();
+    if (q == 0) {
+        const int i = e >> 4, j = e & 15;
+        if (i < dim && j < dim) out[(size_t)pt * dim * dim + i * dim + j] = 0.5 * (red[0][e] + red[0][j * 16 + i]) * scale;
     }
 }
 
-/* FP64 LU inverse with partial pivoting per point -> marginal sigmas sqrt(Sigma_ii) */
-__global__ void gwh_fisher_invert_kernel(const double *__restrict__ F, int n, int dim, double *__restrict__ sigma)
+/* FP64 LU inverse with partial pivoting per point -> marginal sigmas sqrt(Sigma_ii): one wave per point.  The
+ * elimination of step k updates every element (i, j > k) once, spread over the lanes, in the same order of steps
+ * as a serial elimination; the pivot is the first row of largest |a_ik| (a NaN diagonal keeps its row); lane c then
+ * solves L U x = P e_c by forward and back substitution (loops of fixed length, so x stays in registers). */
+__global__ __launch_bounds__(64) void gwh_fisher_invert_kernel(const double *__restrict__ F, int n, int dim,
+                                                               double *__restrict__ sigma)
 {
-    int pt = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pt >= n) return;
-    double a[GWH_FROWS][GWH_FROWS], x[GWH_FROWS];
-    int perm[GWH_FROWS];
+    __shared__ double a[GWH_FROWS][GWH_FROWS + 1];
+    __shared__ int perm[GWH_FROWS];
+    __shared__ int piv_s;
+    const int pt = blockIdx.x, lane = threadIdx.x;
+    if (pt >= n) return; /* the whole block */
     const double *M = F + (size_t)pt * dim * dim;
-    for (int i = 0; i < dim; i++) {
-        perm[i] = i;
-        for (int j = 0; j < dim; j++) a[i][j] = M[i * dim + j];
-    }
+    for (int e = lane; e < dim * dim; e += 64) a[e / dim][e % dim] = M[e];
+    if (lane < dim) perm[lane] = lane;
+    __syncthreads();
     bool sing = false;
     for (int k = 0; k < dim; k++) {
-        int piv = k;
-        double mx = fabs(a[k][k]);
-        for (int i = k + 1; i < dim; i++)
-            if (fabs(a[i][k]) > mx) {
-                mx = fabs(a[i][k]);
-                piv = i;
-            }
-        if (mx == 0) {
+        /* pivot: the first row i >= k of largest |a_ik| */
+        if (lane == 0) {
+            int piv = k;
+            double mx = fabs(a[k][k]);
+            for (int i = k + 1; i < dim; i++)
+                if (fabs(a[i][k]) > mx) {
+                    mx = fabs(a[i][k]);
+                    piv = i;
+                }
+            piv_s = (mx == 0) ? -1 : piv;
+        }
+        __syncthreads();
+        const int piv = piv_s;
+        if (piv < 0) {
             sing = true;
             break;
         }
         if (piv != k) {
-            for (int j = 0; j < dim; j++) {
-                double t = a[k][j];
-                a[k][j] = a[piv][j];
-                a[piv][j] = t;
+            if (lane < dim) {
+                const double t = a[k][lane];
+                a[k][lane] = a[piv][lane];
+                a[piv][lane] = t;
             }
-            int t = perm[k];
-            perm[k] = perm[piv];
-            perm[piv] = t;
+            if (lane == 0) {
+                const int t = perm[k];
+                perm[k] = perm[piv];
+                perm[piv] = t;
+            }
+            __syncthreads();
         }
-        for (int i = k + 1; i < dim; i++) {
-            a[i][k] /= a[k][k];
-            for (int j = k + 1; j < dim; j++) a[i][j] -= a[i][k] * a[k][j];
+        if (lane > k && lane < dim) a[lane][k] /= a[k][k];
+        __syncthreads();
+        const int m = dim - k - 1;
+        for (int e = lane; e < m * m; e += 64) {
+            const int i = k + 1 + e / m, j = k + 1 + e % m;
+            a[i][j] -= a[i][k] * a[k][j];
         }
+        __syncthreads();
     }
-    for (int c = 0; c < dim; c++) {
-        if (sing) {
-            sigma[(size_t)pt * dim + c] = __longlong_as_double(0x7ff8000000000000LL);
-            continue;
-        }
-        for (int i = 0; i < dim; i++) {
+    const int c = lane;
+    if (c >= dim) return;
+    if (sing) {
+        sigma[(size_t)pt * dim + c] = __longlong_as_double(0x7ff8000000000000LL);
+        return;
+    }
+    double x[GWH_FROWS];
+#pragma unroll
+    for (int i = 0; i < GWH_FROWS; i++) {
+        x[i] = 0;
+        if (i < dim) {
             double s = (perm[i] == c) ? 1.0 : 0.0;
+#pragma unroll
             for (int j = 0; j < i; j++) s -= a[i][j] * x[j];
             x[i] = s;
         }
-        for (int i = dim - 1; i >= 0; i--) {
+    }
+#pragma unroll
+    for (int i = GWH_FROWS - 1; i >= 0; i--) {
+        if (i < dim) {
             double s = x[i];
-            for (int j = i + 1; j < dim; j++) s -= a[i][j] * x[j];
+#pragma unroll
+            for (int j = i + 1; j < GWH_FROWS; j++)
+                if (j < dim) s -= a[i][j] * x[j];
             x[i] = s / a[i][i];
         }
-        sigma[(size_t)pt * dim + c] = sqrt(x[c]);
     }
+    double xc = 0;
+#pragma unroll
+    for (int i = 0; i < GWH_FROWS; i++)
+        if (i == c) xc = x[i];
+    sigma[(size_t)pt * dim + c] = sqrt(xc);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -741,13 +779,13 @@ static int fisher_run(int event, int model, int basis, int precision, const gwh_
                                F.deriv.as<const float>(), N, nplanes, blk_per_plane, F.gram.as<double>(), nblk);
         }
         GWH_HIP(hipGetLastError());
-        hipLaunchKernelGGL(gwh_gram_reduce_kernel, dim3(np), dim3(256), 0, F.stream, F.gram.as<const double>(),
+        hipLaunchKernelGGL(gwh_gram_reduce_kernel, dim3(np), dim3(1024), 0, F.stream, F.gram.as<const double>(),
                            nblk, dim, scale, F.out.as<double>() + (size_t)p0 * dim * dim);
         GWH_HIP(hipGetLastError());
     }
     if (sigma) {
         if ((rc = F.sig.ensure((size_t)n * dim * 8)) != GWH_OK) return rc;
-        hipLaunchKernelGGL(gwh_fisher_invert_kernel, dim3((n + 63) / 64), dim3(64), 0, F.stream,
+        hipLaunchKernelGGL(gwh_fisher_invert_kernel, dim3(n), dim3(64), 0, F.stream,
                            F.out.as<const double>(), n, dim, F.sig.as<double>());
         GWH_HIP(hipGetLastError());
         GWH_HIP(hipMemcpyAsync(sigma, F.sig.p, (size_t)n * dim * 8, hipMemcpyDeviceToHost, F.stream));
