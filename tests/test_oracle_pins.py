@@ -941,6 +941,43 @@ def test_pv2_time_correction_is_the_phase_slope_at_ringdown(case):
     assert abs(s["t_corr"] / an - 1) < 1e-3, (s["t_corr"], an)
 
 
+@pytest.mark.parametrize("m1,m2,chil", [(36., 29., 0.), (50., 8., 0.), (12., 10., 0.3), (30., 30., 0.), (50., 8., 0.4)])
+def test_pv2_euler_angles_leading_order(m1, m2, chil):
+    """IMRPhenomP.cpp:64-144 (the NNLO single-spin Euler angles, spin on the larger body b, s the other): as
+    omega = pi M f -> 0 the angles diverge as c1 / omega, and c1 is fixed by leading-order physics.  L precesses
+    about J at Omega_p = (2 + 3 m_s / (2 m_b)) L / r^3 = (2 + 3 m_s / (2 m_b)) eta v^5 / M (spin-orbit precession,
+    Apostolatos et al. 1994, Kidder 1995) while the orbit chirps at d(M omega)/dt = (96/5) eta v^11 / M (v^3 = M
+    omega), so d alpha / d(M omega) = (5/96) (2 + 3 m_s / (2 m_b)) (M omega)^-2 and c1 = -(5/96)(2 + 3 m_s / (2 m_b));
+    epsilon' = alpha' cos(beta) with beta -> 0 at leading order has the same c1.  The omega^-2/3 coefficient c2
+    follows from the same precession rate with J = L + S_L instead of L (L = eta / v, S_L = chi_L m_b^2): the
+    relative correction chi_L m_b^2 v / eta integrates to c2 = (3/2) c1 chi_L m_b^2 / eta, for both angles (cos
+    beta differs from 1 only at relative order v^2).  The coefficients are read off the oracle's angles at five tiny
+    frequencies by a fit over 1/omega, omega^-2/3, omega^-1/3 and a constant (the f_ref offsets); the next terms are
+    suppressed by further powers of omega^1/3.  A wrong mass ratio, a swapped spinning body, a lost factor or sign
+    in either coefficient fails; c3 (omega^-1/3), c4 (log omega) and c5 (omega^1/3) stay unpinned."""
+    kw = dict(mass1=m1, mass2=m2, spin1=[.4, .2, chil], spin2=[0., 0., 0.], DL=410., incl=.4, f_ref=20., phiRef=.3)
+    p = O.make_params(**kw)
+    M = 0.2 / O.fcut(p)
+    fs = np.array([1e-6, 2e-6, 4e-6, 8e-6, 1.6e-5])
+    om = np.pi * M * fs
+    A = np.stack([1 / om, om ** (-2 / 3), om ** (-1 / 3), np.ones_like(om)], axis=1)
+    scale = np.abs(A).max(axis=0)
+    mb, ms = max(m1, m2) / (m1 + m2), min(m1, m2) / (m1 + m2)
+    eta = mb * ms
+    chi_l = O.pv2_setup_summary(p)["chil"]
+    assert abs(chi_l - chil) < 1e-12  # the in-plane spin sits on the larger body only
+    c1 = -(5 / 96) * (2 + 3 * ms / (2 * mb))
+    c2 = 1.5 * c1 * chi_l * mb ** 2 / eta
+    for key in ("alpha", "epsilon"):
+        y = np.array([O.pv2_pieces(p, f)[key] for f in fs])
+        coef = np.linalg.lstsq(A / scale, y, rcond=None)[0] / scale
+        assert abs(coef[0] / c1 - 1) < 1e-6, (key, coef[0], c1)
+        assert abs(coef[1] - c2) < 2e-4 + 1e-3 * abs(c2), (key, coef[1], c2)
+    # the same fit tells a planted mistake apart: the mass ratio inverted in the leading coefficient
+    wrong = -(5 / 96) * (2 + 3 * mb / (2 * ms))
+    assert m1 == m2 or abs(wrong / c1 - 1) > 1e-2
+
+
 # ---- the order-4 Fisher stencil (fisher.cpp:403-453) -------------------------------------------------------
 def test_order4_stencil_is_the_five_point_rule_and_its_tc_shift_mix():
     """Order 4: (-h(+2e) + 8 h(+e) - 8 h(-e) + h(-2e)) / 12 e.  With the detector equal to the reference one it
