@@ -978,6 +978,41 @@ def test_pv2_euler_angles_leading_order(m1, m2, chil):
     assert m1 == m2 or abs(wrong / c1 - 1) > 1e-2
 
 
+@pytest.mark.parametrize("m1,m2,chil", [(36., 29., 0.2), (50., 8., 0.4), (12., 10., -0.3)])
+def test_pv2_euler_angles_inplane_spin_term(m1, m2, chil):
+    """The in-plane spin's part of the Euler angles (IMRPhenomP.cpp:64-144).  The precession rate of L about J
+    scales with |J| = sqrt((L + S_L)^2 + S_p^2) = L + S_L + S_p^2 / (2 L) - S_p^2 S_L / (2 L^2) + ... (L = eta / v,
+    S_p = chi_p m_b^2, S_L = chi_L m_b^2): relative corrections chi_p^2 m_b^4 v^2 / (2 eta^2) and
+    -chi_p^2 chi_L m_b^6 v^3 / (2 eta^3), which integrate to an omega^-1/3 (c3) term (3/2) c1 chi_p^2 m_b^4 / eta^2
+    and a log omega (c4) term c1 chi_p^2 chi_L m_b^6 / (2 eta^3) in alpha.  In epsilon' = alpha' cos(beta) =
+    alpha' (L + S_L) / |J| they cancel.  Checked as the change of the fitted c3 and c4 between chi_p = 0.2 and 0.8
+    (the spin-independent 1PN and 1.5PN parts drop out of the difference and stay unpinned)."""
+    def fit(p, key):
+        M = 0.2 / O.fcut(p)
+        fs = np.geomspace(1e-6, 2e-4, 10)
+        om = np.pi * M * fs
+        A = np.stack([1 / om, om ** (-2 / 3), om ** (-1 / 3), np.log(om), np.ones_like(om), om ** (1 / 3)], axis=1)
+        sc = np.abs(A).max(axis=0)
+        y = np.array([O.pv2_pieces(p, f)[key] for f in fs])
+        return np.linalg.lstsq(A / sc, y, rcond=None)[0] / sc
+    co = {}
+    for chip in (0.2, 0.8):
+        p = O.make_params(mass1=m1, mass2=m2, spin1=[chip, 0., chil], spin2=[0., 0., 0.], DL=410., incl=.4,
+                          f_ref=20., phiRef=.3)
+        assert abs(O.pv2_setup_summary(p)["chip"] - chip) < 1e-12
+        co[chip] = (fit(p, "alpha"), fit(p, "epsilon"))
+    mb, ms = max(m1, m2) / (m1 + m2), min(m1, m2) / (m1 + m2)
+    eta = mb * ms
+    c1 = -(5 / 96) * (2 + 3 * ms / (2 * mb))
+    dchip2 = 0.8 ** 2 - 0.2 ** 2
+    want3 = 1.5 * c1 * dchip2 * mb ** 4 / eta ** 2
+    want4 = c1 * dchip2 * chil * mb ** 6 / (2 * eta ** 3)
+    da, de = co[0.8][0] - co[0.2][0], co[0.8][1] - co[0.2][1]
+    assert abs(da[2] / want3 - 1) < 1e-5, (da[2], want3)
+    assert abs(da[3] / want4 - 1) < 1e-3, (da[3], want4)
+    assert np.all(np.abs(de[:4]) < 1e-6 * abs(want3)), de
+
+
 # ---- the order-4 Fisher stencil (fisher.cpp:403-453) -------------------------------------------------------
 def test_order4_stencil_is_the_five_point_rule_and_its_tc_shift_mix():
     """Order 4: (-h(+2e) + 8 h(+e) - 8 h(-e) + h(-2e)) / 12 e.  With the detector equal to the reference one it
