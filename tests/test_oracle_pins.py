@@ -1013,6 +1013,41 @@ def test_pv2_euler_angles_inplane_spin_term(m1, m2, chil):
     assert np.all(np.abs(de[:4]) < 1e-6 * abs(want3)), de
 
 
+@pytest.mark.parametrize("m1,m2", [(36., 29.), (50., 8.), (12., 10.), (30., 30.), (40., 20.)])
+def test_pv2_euler_angles_1pn_and_tail_terms(m1, m2):
+    """The Euler angles' omega^-1/3 and log omega coefficients without aligned spin (IMRPhenomP.cpp:64-144).  With
+    the spinning body's precession rate at next-to-leading order, Omega = (v^5 / M) [A0 + A2 v^2], A0 = 3/4 + eta/2
+    - 3 D / 4, A2 = 9/16 + 5 eta / 4 - eta^2 / 24 + D (-9/16 + 5 eta / 8) (D = m_b - m_s; the NLO spin-orbit
+    precession frequency of the post-Newtonian literature, e.g. Blanchet, Living Rev. Relativ. 17, 2 (2014)), over
+    the chirp d(M omega)/dt = (96/5) eta v^11 [1 + b2 v^2 + b3 v^3] / M with b2 = -(743/336 + 11 eta / 4) and the
+    tail term b3 = 4 pi: the omega^-1/3 coefficient is 3 c1 (A2 / A0 - b2) (plus the in-plane spin's
+    (3/2) c1 chi_p^2 m_b^4 / eta^2 in alpha) and the log omega coefficient 4 pi c1.  The aligned-spin parts of the log
+    omega and omega^1/3 coefficients stay unpinned."""
+    kw = dict(mass1=m1, mass2=m2, spin1=[.5, 0., 0.], spin2=[0., 0., 0.], DL=410., incl=.4, f_ref=20., phiRef=.3)
+    p = O.make_params(**kw)
+    chip = O.pv2_setup_summary(p)["chip"]
+    M = 0.2 / O.fcut(p)
+    fs = np.geomspace(1e-6, 2e-4, 10)
+    om = np.pi * M * fs
+    A = np.stack([1 / om, om ** (-2 / 3), om ** (-1 / 3), np.log(om), np.ones_like(om), om ** (1 / 3)], axis=1)
+    sc = np.abs(A).max(axis=0)
+    mb, ms = max(m1, m2) / (m1 + m2), min(m1, m2) / (m1 + m2)
+    eta, D = mb * ms, max(m1, m2) / (m1 + m2) - min(m1, m2) / (m1 + m2)
+    c1 = -(5 / 96) * (2 + 3 * ms / (2 * mb))
+    a0 = 3 / 4 + eta / 2 - 3 * D / 4
+    a2 = 9 / 16 + 5 * eta / 4 - eta ** 2 / 24 + D * (-9 / 16 + 5 * eta / 8)
+    b2 = -(743 / 336 + 11 * eta / 4)
+    assert abs(a0 / eta - (2 + 3 * ms / (2 * mb))) < 1e-14  # the leading rate of the precession test above
+    c3 = 3 * c1 * (a2 / a0 - b2)
+    for key, extra in (("alpha", 1.5 * c1 * chip ** 2 * mb ** 4 / eta ** 2), ("epsilon", 0.0)):
+        y = np.array([O.pv2_pieces(p, f)[key] for f in fs])
+        coef = np.linalg.lstsq(A / sc, y, rcond=None)[0] / sc
+        assert abs(coef[2] / (c3 + extra) - 1) < 1e-5, (key, coef[2], c3 + extra)
+        assert abs(coef[3] / (4 * np.pi * c1) - 1) < 1e-3, (key, coef[3], 4 * np.pi * c1)
+    # a dropped 11 eta / 4 in the chirp's 1PN term moves c3 by more than the fit's tolerance
+    assert abs(3 * c1 * (a2 / a0 + 743 / 336) / c3 - 1) > 1e-3
+
+
 # ---- the order-4 Fisher stencil (fisher.cpp:403-453) -------------------------------------------------------
 def test_order4_stencil_is_the_five_point_rule_and_its_tc_shift_mix():
     """Order 4: (-h(+2e) + 8 h(+e) - 8 h(-e) + h(-2e)) / 12 e.  With the detector equal to the reference one it
