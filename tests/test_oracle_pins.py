@@ -985,18 +985,20 @@ def test_pv2_euler_angles_inplane_spin_term(m1, m2, chil):
     S_p = chi_p m_b^2, S_L = chi_L m_b^2): relative corrections chi_p^2 m_b^4 v^2 / (2 eta^2) and
     -chi_p^2 chi_L m_b^6 v^3 / (2 eta^3), which integrate to an omega^-1/3 (c3) term (3/2) c1 chi_p^2 m_b^4 / eta^2
     and a log omega (c4) term c1 chi_p^2 chi_L m_b^6 / (2 eta^3) in alpha.  In epsilon' = alpha' cos(beta) =
-    alpha' (L + S_L) / |J| they cancel.  Checked as the change of the fitted c3 and c4 between chi_p = 0.2 and 0.8
-    (the spin-independent 1PN and 1.5PN parts drop out of the difference and stay unpinned)."""
+    alpha' (L + S_L) / |J| they cancel.  The next term of the expansion, -S_p^4 / (8 L^3), gives the omega^1/3 (c5)
+    coefficient's chi_p^4 part (3/8) c1 m_b^8 / eta^4.  Checked from the fitted coefficients at chi_p = 0.2, 0.5 and
+    0.8: the changes of c3 and c4, and the chi_p^4 term of a quadratic in chi_p^2 through c5 (the spin-independent
+    parts drop out)."""
     def fit(p, key):
         M = 0.2 / O.fcut(p)
-        fs = np.geomspace(1e-6, 2e-4, 10)
+        fs = np.geomspace(1e-6, 2e-3, 14)
         om = np.pi * M * fs
         A = np.stack([1 / om, om ** (-2 / 3), om ** (-1 / 3), np.log(om), np.ones_like(om), om ** (1 / 3)], axis=1)
         sc = np.abs(A).max(axis=0)
         y = np.array([O.pv2_pieces(p, f)[key] for f in fs])
         return np.linalg.lstsq(A / sc, y, rcond=None)[0] / sc
     co = {}
-    for chip in (0.2, 0.8):
+    for chip in (0.2, 0.5, 0.8):
         p = O.make_params(mass1=m1, mass2=m2, spin1=[chip, 0., chil], spin2=[0., 0., 0.], DL=410., incl=.4,
                           f_ref=20., phiRef=.3)
         assert abs(O.pv2_setup_summary(p)["chip"] - chip) < 1e-12
@@ -1011,6 +1013,9 @@ def test_pv2_euler_angles_inplane_spin_term(m1, m2, chil):
     assert abs(da[2] / want3 - 1) < 1e-5, (da[2], want3)
     assert abs(da[3] / want4 - 1) < 1e-3, (da[3], want4)
     assert np.all(np.abs(de[:4]) < 1e-6 * abs(want3)), de
+    quartic = np.polyfit(np.array([0.2, 0.5, 0.8]) ** 2, [co[c][0][5] for c in (0.2, 0.5, 0.8)], 2)[0]
+    want5 = 3 / 8 * c1 * mb ** 8 / eta ** 4
+    assert abs(quartic / want5 - 1) < 1e-2, (quartic, want5)
 
 
 @pytest.mark.parametrize("m1,m2", [(36., 29.), (50., 8.), (12., 10.), (30., 30.), (40., 20.)])
