@@ -1150,7 +1150,7 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, co
         ar = pn + nr;
     } else if (f > C.f3) {
         double dl = f - C.fRD;
-        ar = C.mr_k * (FP32_EXP ? exp_fp32(-C.mr_g2 * dl) : gwh_exp_table(-C.mr_g2 * dl)) / (dl * dl + C.mr_w2);
+        ar = C.mr_k * (FP32_EXP ? exp_fp32(-C.mr_g2 * dl) : gwh_exp_table(-C.mr_g2 * dl)) * rcp_newton(dl * dl + C.mr_w2);
     } else {
         ar = C.delta[0] + Mf * (C.delta[1] + Mf * (C.delta[2] + Mf * (C.delta[3] + Mf * C.delta[4])));
     }
@@ -1165,14 +1165,14 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, co
         double v2 = v * v;
         double poly = C.cp[0] + v2 * (C.cp[2] + v * (C.cp[3] + v * (C.cp[4] + v * (c5 + v * (c6 + v * C.cp[7])))));
         double Mf43 = Mf * u;
-        psi = -GWH_PI / 4. + C.K * poly / (v2 * v2 * v) +
+        psi = -GWH_PI / 4. + C.K * poly * rcp_newton(v2 * v2 * v) +
               (C.sig[0] * Mf + C.sig[1] * Mf43 + C.sig[2] * Mf43 * u + C.sig[3] * Mf * Mf);
-        if (C.gimr) psi += gwh_gimr_bin(C, 1.0 / v);
+        if (C.gimr) psi += gwh_gimr_bin(C, rcp_newton(v));
     } else if (f > C.f2p) {
-        psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] / Mf + C.alpha[3] * (C.M34 * bp.f34) +
-              C.alpha[4] * atan((f - C.alpha[5]) / C.fdamp);
+        psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] * rcp_newton(Mf) + C.alpha[3] * (C.M34 * bp.f34) +
+              C.alpha[4] * gwh_atan_seeded((f - C.alpha[5]) * C.inv_fdamp);
     } else {
-        psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] / (Mf * Mf * Mf);
+        psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] * rcp_newton(Mf * Mf * Mf);
     }
     if (C.ppe_n > 0 && (C.ppe_imr || f < C.f1p)) psi += gwh_ppe_bin(C, bp.lf);
     phi = psi - (C.tc_total * (f - C.f_ref) + C.phic);

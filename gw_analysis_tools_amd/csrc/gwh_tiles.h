@@ -53,14 +53,6 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double *__
 /* per-bin mixed-precision evaluators                                         */
 /* ------------------------------------------------------------------------- */
 
-/* 1/z in FP64: hardware reciprocal and one Newton step (no IEEE division sequence) */
-__device__ __forceinline__ double rcp_newton(double z)
-{
-    double r = __builtin_amdgcn_rcp(z);
-    r = fma(r, fma(-z, r, 1.0), r);
-    return r;
-}
-
 /* z^{-1/2} in FP64 from an FP32 v_rsq_f32 seed and two Newton steps */
 __device__ __forceinline__ double rsqrt_mixed(double z)
 {
@@ -112,7 +104,7 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, c
         if (C.gimr) psi += gwh_gimr_bin(C, iv);
     } else if (f > C.f2p) {
         psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] * (r2 * r) + C.alpha[3] * (C.M34 * bp.f34) +
-              C.alpha[4] * atan((f - C.alpha[5]) * C.inv_fdamp);
+              C.alpha[4] * gwh_atan_seeded((f - C.alpha[5]) * C.inv_fdamp);
     } else {
         double r3 = r2 * r;
         psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] * (r3 * r3 * r3);

@@ -109,4 +109,24 @@ __device__ __forceinline__ double exp_fp32(double x)
     return ldexp((double)__builtin_amdgcn_exp2f(t), (int)n);
 }
 
+/* 1/z in FP64: hardware reciprocal and one Newton step (no IEEE division sequence) */
+__device__ __forceinline__ double rcp_newton(double z)
+{
+    double r = __builtin_amdgcn_rcp(z);
+    r = fma(r, fma(-z, r, 1.0), r);
+    return r;
+}
+
+/* arctangent in FP64 from an FP32 seed: t0 = atanf(x) (within ~1e-7), then the exact correction
+ * atan(x) = t0 + atan((x cos t0 - sin t0) / (cos t0 + x sin t0)) with the table sin/cos, where the inner argument
+ * is below ~1e-7, so its arctangent is the argument itself to within 1e-21; the denominator is
+ * sqrt(1 + x^2) >= 1.  Error a few ulp of pi / 2 for finite x. */
+__device__ __forceinline__ double gwh_atan_seeded(double x)
+{
+    const double t0 = (double)atanf((float)x);
+    double s, c;
+    gwh_sincos_table(t0, s, c);
+    return t0 + fma(x, c, -s) * rcp_newton(fma(x, s, c));
+}
+
 #endif

@@ -7,7 +7,10 @@ same constants and operation order, against 50-digit mpmath values:
   to 1e6 rad (the per-bin phases of the likelihood reach ~1e5-1e6 rad);
 * e^x through the table is within 1e-15 relative over [-700, 5];
 * a planted error (one table entry off by ~1e-13, a wrong factorial in the series, one 2^(j/64) entry off by
-  1e-12 relative) is caught."""
+  1e-12 relative) is caught;
+* the arctangent seeded in FP32 and corrected once through the table (gwh_atan_seeded) is within 8 ulp of pi/2
+  of the exact value for |x| up to 1e6, with any FP32 seed within 2^-22 relative of atan(x); a correction with
+  the wrong sign is caught."""
 import os
 import re
 
@@ -109,3 +112,29 @@ def test_exp_through_the_table():
     bad[5] *= 1 + 1e-12
     x = 5 * np.log(2) / 64
     assert abs(exp_table(x, bad) / float(mpmath.exp(x)) - 1) > 1e-13
+
+
+def atan_seeded(x, tab, sign=1.0):
+    """gwh_atan_seeded, operation by operation (the FP32 seed: numpy's float32 arctangent)"""
+    t0 = float(np.arctan(np.float32(x)))
+    sn, cs, _ = sincos_table(t0, tab)
+    num = _fma(x, cs, -sn)
+    den = _fma(x, sn, cs)
+    return t0 + sign * num * (1.0 / den)
+
+
+def test_atan_seeded():
+    tab, _ = _tables()
+    g = np.random.default_rng(9)
+    xs = np.concatenate([g.uniform(-2, 2, 300), g.uniform(-1e3, 1e3, 200), g.uniform(-1e6, 1e6, 100),
+                         [0.0, 1e-300, 1.0, -1.0, 1e6, -1e6, 3.0e-8]])
+    worst = max(abs(atan_seeded(float(x), tab) - float(mpmath.atan(mpmath.mpf(float(x))))) for x in xs)
+    assert worst <= 8 * 2.0 ** -52 * np.pi / 2, worst
+    # a seed perturbed within FP32 accuracy is corrected all the same
+    x = 0.7
+    t0 = float(np.arctan(np.float32(x))) * (1 + 2.0 ** -22)
+    sn, cs, _ = sincos_table(t0, tab)
+    fixed = t0 + _fma(x, cs, -sn) * (1.0 / _fma(x, sn, cs))
+    assert abs(fixed - float(mpmath.atan(x))) < 1e-15
+    # planted: the correction subtracted instead of added
+    assert max(abs(atan_seeded(float(x), tab, -1.0) - float(mpmath.atan(mpmath.mpf(float(x))))) for x in xs[:50]) > 1e-9
