@@ -1191,9 +1191,9 @@ __device__ __forceinline__ void gwh_pv2_fp64(const InstCoef &C, double f, const 
     double v = u * 1.4645918875615231;
     double logw = C.lgPiM + bp.lf; /* log(pi M f) */
     double x = v * v;
-    double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) / v;
-    double s = C.SP / (L + C.SL);
-    double cb = 1. / sqrt(1.0 + s * s);
+    double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) * rcp_newton(v);
+    double s = C.SP * rcp_newton(L + C.SL);
+    double cb = rsqrt_seeded(1.0 + s * s);
     double cbh = sqrt((1.0 + cb) / 2.0), sbh = sqrt((1.0 - cb) / 2.0);
     double c2 = cbh * cbh, s2 = sbh * sbh, c3 = c2 * cbh, s3 = s2 * sbh;
     double d2[5] = {s3 * sbh, 2 * cbh * s3, GWH_SQRT_6 * s2 * c2, 2 * c3 * sbh, c3 * cbh};

@@ -53,15 +53,6 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double *__
 /* per-bin mixed-precision evaluators                                         */
 /* ------------------------------------------------------------------------- */
 
-/* z^{-1/2} in FP64 from an FP32 v_rsq_f32 seed and two Newton steps */
-__device__ __forceinline__ double rsqrt_mixed(double z)
-{
-    double y = (double)__builtin_amdgcn_rsqf((float)z);
-    y = y * (1.5 - 0.5 * z * y * y);
-    y = y * (1.5 - 0.5 * z * y * y);
-    return y;
-}
-
 /* IMRPhenomD in mixed precision: the FP64 evaluator with the merger-ringdown amplitude's exponential in FP32
  * (TABLE_EXP false: ~6e-8 relative per bin) or through the FP64 table (true: a few ulp); the phase, including its
  * arctangent, stays FP64 (an FP32 atan there was the largest error of the path, ~7e-4 in logL for heavy
@@ -189,9 +180,9 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                         const double v = u * 1.4645918875615231;
                         const double logw = C.lgPiM + bp.lf; /* log(pi M f) */
                         const double x = v * v;
-                        const double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) / v;
-                        const double sv = C.SP / (L + C.SL);
-                        const double cbd = rsqrt_mixed(1.0 + sv * sv);
+                        const double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) * rcp_newton(v);
+                        const double sv = C.SP * rcp_newton(L + C.SL);
+                        const double cbd = rsqrt_seeded(1.0 + sv * sv);
                         const double cbhd = sqrt(0.5 * (1.0 + cbd)), sbhd = sqrt(0.5 * (1.0 - cbd));
                         const double c2d = cbhd * cbhd, s2d = sbhd * sbhd;
                         const double d2[5] = {s2d * s2d, 2. * cbhd * s2d * sbhd, GWH_SQRT_6 * s2d * c2d,

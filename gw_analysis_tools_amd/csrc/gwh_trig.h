@@ -117,6 +117,15 @@ __device__ __forceinline__ double rcp_newton(double z)
     return r;
 }
 
+/* z^{-1/2} in FP64 from an FP32 v_rsq_f32 seed (~1e-7) and two Newton steps (~1e-16) */
+__device__ __forceinline__ double rsqrt_seeded(double z)
+{
+    double y = (double)__builtin_amdgcn_rsqf((float)z);
+    y = y * (1.5 - 0.5 * z * y * y);
+    y = y * (1.5 - 0.5 * z * y * y);
+    return y;
+}
+
 /* arctangent in FP64 from an FP32 seed: t0 = atanf(x) (within ~1e-7), then the exact correction
  * atan(x) = t0 + atan((x cos t0 - sin t0) / (cos t0 + x sin t0)) with the table sin/cos, where the inner argument
  * is below ~1e-7, so its arctangent is the argument itself to within 1e-21; the denominator is
