@@ -1137,7 +1137,7 @@ __device__ __forceinline__ double gwh_ppe_bin(const InstCoef &C, double lf)
 
 template <bool FP32_EXP = false>
 __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, const BinPow &bp, double &A, double &phi,
-                                                 double &u_out, double &lx_out)
+                                                 double &u_out, double &lx_out, const double2 *tab = gwh_turn_table)
 {
     double Mf = C.M * f;
     double u = C.M13 * bp.f13;      /* (Mf)^{1/3} */
@@ -1170,7 +1170,7 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, co
         if (C.gimr) psi += gwh_gimr_bin(C, rcp_newton(v));
     } else if (f > C.f2p) {
         psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] * rcp_newton(Mf) + C.alpha[3] * (C.M34 * bp.f34) +
-              C.alpha[4] * gwh_atan_seeded((f - C.alpha[5]) * C.inv_fdamp);
+              C.alpha[4] * gwh_atan_seeded((f - C.alpha[5]) * C.inv_fdamp, tab);
     } else {
         psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] * rcp_newton(Mf * Mf * Mf);
     }
@@ -1183,10 +1183,11 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, co
 /* IMRPhenomPv2 twist-up at one bin (IMRPhenomP.cpp:290-361): returns the common amplitude
  * (already including the 1/2), the total phase and the complex h+ / hx twist-up factors. */
 __device__ __forceinline__ void gwh_pv2_fp64(const InstCoef &C, double f, const BinPow &bp, double &A, double &Phi,
-                                             double &hpr, double &hpi, double &hcr, double &hci)
+                                             double &hpr, double &hpi, double &hcr, double &hci,
+                                             const double2 *tab = gwh_turn_table)
 {
     double u, lx;
-    gwh_phenomd_fp64(C, f, bp, A, Phi, u, lx);
+    gwh_phenomd_fp64(C, f, bp, A, Phi, u, lx, tab);
     /* gwh_phenomd_fp64 already applied -tc (f - f_ref) - phic; Pv2 adds 2 eps and the t_c correction */
     double v = u * 1.4645918875615231;
     double logw = C.lgPiM + bp.lf; /* log(pi M f) */
@@ -1200,7 +1201,7 @@ __device__ __forceinline__ void gwh_pv2_fp64(const InstCoef &C, double f, const 
     double alpha = gwh_euler_eval(C.a_c, v, logw) + C.alpha_add;
     double eps = gwh_euler_eval(C.e_c, v, logw) + C.eps_add;
     double sa, ca;
-    gwh_sincos_table(alpha, sa, ca);
+    gwh_sincos_tab(alpha, sa, ca, tab);
     /* e^{i m alpha}, m=-2..2 */
     double er[5], ei[5];
     er[2] = 1;

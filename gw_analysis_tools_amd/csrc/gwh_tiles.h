@@ -59,7 +59,7 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double *__
  * binaries). */
 template <bool TABLE_EXP>
 __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, const BinPow &bp, double &a, double &phi,
-                                                  double &u_out, double &lx_out)
+                                                  double &u_out, double &lx_out, const double2 *tab = gwh_turn_table)
 {
     double Mf = C.M * f;
     double r = C.Mm13 * bp.fm13;     /* (Mf)^{-1/3} */
@@ -95,7 +95,7 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, c
         if (C.gimr) psi += gwh_gimr_bin(C, iv);
     } else if (f > C.f2p) {
         psi = C.alpha[0] + C.alpha[1] * Mf - C.alpha[2] * (r2 * r) + C.alpha[3] * (C.M34 * bp.f34) +
-              C.alpha[4] * gwh_atan_seeded((f - C.alpha[5]) * C.inv_fdamp);
+              C.alpha[4] * gwh_atan_seeded((f - C.alpha[5]) * C.inv_fdamp, tab);
     } else {
         double r3 = r2 * r;
         psi = C.beta[0] + C.beta[1] * Mf + C.beta[2] * (C.lgM + bp.lf) - C.beta[3] * (r3 * r3 * r3);
@@ -121,7 +121,7 @@ template <int MODEL, int PREC, int BPL, int TPB>
 __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDesc *__restrict__ events,
                                                  const InstCoef *__restrict__ coefs, double *__restrict__ partial,
                                                  const int *__restrict__ active, const int inst, const int tile,
-                                                 double (*lds)[GWH_PARTIAL_STRIDE])
+                                                 double (*lds)[GWH_PARTIAL_STRIDE], double2 *__restrict__ tab)
 {
     {
         const int ev = W.inst_event ? W.inst_event[inst] : W.event0;
@@ -138,6 +138,11 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
         constexpr int kLevel = gwh_tile_level(TPB * BPL);
         const double fmin_tile = gwh_g(E.tile_fmin)[E.fmin_off[kLevel] + tile];
         if (!(fmin_tile > C.fcut)) {
+            /* the sin/cos table in LDS: the detector phases (one gather per detector and bin), the merger-ringdown
+               arctangent's correction and the IMRPhenomPv2 Euler angle read it there instead of through the
+               vector memory path */
+            for (int q = threadIdx.x; q < 256; q += TPB) tab[q] = gwh_turn_table[q];
+            __syncthreads();
 #pragma unroll
             for (int j = 0; j < BPL; j++) {
                 /* Lanes past the end of the grid or above 0.2/M evaluate the last grid bin with a zero amplitude
@@ -153,14 +158,14 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                 if (MODEL == GWH_IMRPHENOMD) {
                     double a, phi, u, lx;
                     /* mixed: the merger-ringdown exponential in FP32, everything else as FP64 */
-                    gwh_phenomd_fp64<PREC == GWH_MIXED>(C, f, bp, a, phi, u, lx);
+                    gwh_phenomd_fp64<PREC == GWH_MIXED>(C, f, bp, a, phi, u, lx, tab);
                     a = live ? a : 0.0;
                     phi = live ? phi : 0.0;
 #pragma unroll
                     for (int d = 0; d < GWH_MAXDET; d++) {
                         if (d >= D) break;
                         double s, c;
-                        gwh_sincos_table(phi + C.det[d][3] * f, s, c);
+                        gwh_sincos_tab(phi + C.det[d][3] * f, s, c, tab);
                         const double ws = gwh_g(E.wS)[(size_t)d * N + k];
                         const double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
                         acc[3 * d] += a * a * ws;
@@ -170,10 +175,10 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                 } else {
                     double A, Phi, hpr, hpi, hcr, hci;
                     if (PREC == GWH_FP64) {
-                        gwh_pv2_fp64(C, f, bp, A, Phi, hpr, hpi, hcr, hci);
+                        gwh_pv2_fp64(C, f, bp, A, Phi, hpr, hpi, hcr, hci, tab);
                     } else {
                         double u, lx;
-                        gwh_phenomd_mixed<true>(C, f, bp, A, Phi, u, lx);
+                        gwh_phenomd_mixed<true>(C, f, bp, A, Phi, u, lx, tab);
                         /* precession angles and the sin/cos of the Euler angle alpha in FP64 (alpha's rounding
                          * would enter |h|^2 directly), like the detector phases below: table-driven sin/cos and
                          * exponential (the FP32 ones left ~2.5e-4 in logL on short, loud precessing events) */
@@ -190,7 +195,7 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                         const double alpha = gwh_euler_eval(C.a_c, v, logw) + C.alpha_add;
                         const double eps = gwh_euler_eval(C.e_c, v, logw) + C.eps_add;
                         double sa, ca;
-                        gwh_sincos_table(alpha, sa, ca);
+                        gwh_sincos_tab(alpha, sa, ca, tab);
                         double er[5], ei[5];
                         er[2] = 1.;
                         ei[2] = 0.;
@@ -225,7 +230,7 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                         const double P = C.det[d][0], Q = C.det[d][1];
                         const double Gr = P * hpr + Q * hcr, Gi = P * hpi + Q * hci;
                         double s, c;
-                        gwh_sincos_table(Phi + C.det[d][3] * f, s, c);
+                        gwh_sincos_tab(Phi + C.det[d][3] * f, s, c, tab);
                         const double ws = gwh_g(E.wS)[(size_t)d * N + k];
                         const double2 dw = gwh_ld2(E.dwS, (size_t)d * N + k);
                         acc[3 * d] += A * A * (Gr * Gr + Gi * Gi) * ws;
@@ -249,10 +254,11 @@ __global__ __launch_bounds__(TPB) void gwh_loglike_wl_kernel(WorkMap W, const Ev
                                                              const int *__restrict__ count)
 {
     __shared__ double lds[TPB / 64][GWH_PARTIAL_STRIDE];
+    __shared__ double2 tab[256];
     const int w = blockIdx.y * gridDim.x + blockIdx.x;
     if (w >= *count) return;
     const int2 wi = work[w];
-    gwh_loglike_tile<MODEL, PREC, BPL, TPB>(W, events, coefs, partial, active, wi.x, wi.y, lds);
+    gwh_loglike_tile<MODEL, PREC, BPL, TPB>(W, events, coefs, partial, active, wi.x, wi.y, lds, tab);
 }
 
 /* One block per (instance, tile) item, in tile-major order (item w = tile * n + instance).  Consecutive blocks go
@@ -273,10 +279,11 @@ __global__ __launch_bounds__(TPB) void gwh_loglike_grid_kernel(WorkMap W, const 
                                                                const int *__restrict__ count)
 {
     __shared__ double lds[TPB / 64][GWH_PARTIAL_STRIDE];
+    __shared__ double2 tab[256];
     const int nn = count ? min(*count, W.n) : W.n;
     const int w = blockIdx.y * gridDim.x + blockIdx.x; /* items fold over y beyond 2^20 blocks */
     if (nn <= 0 || w >= nn * W.tiles0) return;
-    gwh_loglike_tile<MODEL, PREC, BPL, TPB>(W, events, coefs, partial, active, w % nn, w / nn, lds);
+    gwh_loglike_tile<MODEL, PREC, BPL, TPB>(W, events, coefs, partial, active, w % nn, w / nn, lds, tab);
 }
 
 /* ------------------------------------------------------------------------- */

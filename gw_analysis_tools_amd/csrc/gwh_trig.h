@@ -130,11 +130,11 @@ __device__ __forceinline__ double rsqrt_seeded(double z)
  * atan(x) = t0 + atan((x cos t0 - sin t0) / (cos t0 + x sin t0)) with the table sin/cos, where the inner argument
  * is below ~1e-7, so its arctangent is the argument itself to within 1e-21; the denominator is
  * sqrt(1 + x^2) >= 1.  Error a few ulp of pi / 2 for finite x. */
-__device__ __forceinline__ double gwh_atan_seeded(double x)
+__device__ __forceinline__ double gwh_atan_seeded(double x, const double2 *tab = gwh_turn_table)
 {
     const double t0 = (double)atanf((float)x);
     double s, c;
-    gwh_sincos_table(t0, s, c);
+    gwh_sincos_tab(t0, s, c, tab);
     return t0 + fma(x, c, -s) * rcp_newton(fma(x, s, c));
 }
 
