@@ -1137,7 +1137,8 @@ __device__ __forceinline__ double gwh_ppe_bin(const InstCoef &C, double lf)
 
 template <bool FP32_EXP = false>
 __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, const BinPow &bp, double &A, double &phi,
-                                                 double &u_out, double &lx_out, const double2 *tab = gwh_turn_table)
+                                                 double &u_out, double &lx_out, const double2 *tab = gwh_turn_table,
+                                                 const double *etab = gwh_exp2_table)
 {
     double Mf = C.M * f;
     double u = C.M13 * bp.f13;      /* (Mf)^{1/3} */
@@ -1150,7 +1151,7 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, co
         ar = pn + nr;
     } else if (f > C.f3) {
         double dl = f - C.fRD;
-        ar = C.mr_k * (FP32_EXP ? exp_fp32(-C.mr_g2 * dl) : gwh_exp_table(-C.mr_g2 * dl)) * rcp_newton(dl * dl + C.mr_w2);
+        ar = C.mr_k * (FP32_EXP ? exp_fp32(-C.mr_g2 * dl) : gwh_exp_table(-C.mr_g2 * dl, etab)) * rcp_newton(dl * dl + C.mr_w2);
     } else {
         ar = C.delta[0] + Mf * (C.delta[1] + Mf * (C.delta[2] + Mf * (C.delta[3] + Mf * C.delta[4])));
     }
@@ -1184,10 +1185,10 @@ __device__ __forceinline__ void gwh_phenomd_fp64(const InstCoef &C, double f, co
  * (already including the 1/2), the total phase and the complex h+ / hx twist-up factors. */
 __device__ __forceinline__ void gwh_pv2_fp64(const InstCoef &C, double f, const BinPow &bp, double &A, double &Phi,
                                              double &hpr, double &hpi, double &hcr, double &hci,
-                                             const double2 *tab = gwh_turn_table)
+                                             const double2 *tab = gwh_turn_table, const double *etab = gwh_exp2_table)
 {
     double u, lx;
-    gwh_phenomd_fp64(C, f, bp, A, Phi, u, lx, tab);
+    gwh_phenomd_fp64(C, f, bp, A, Phi, u, lx, tab, etab);
     /* gwh_phenomd_fp64 already applied -tc (f - f_ref) - phic; Pv2 adds 2 eps and the t_c correction */
     double v = u * 1.4645918875615231;
     double logw = C.lgPiM + bp.lf; /* log(pi M f) */
@@ -1195,7 +1196,9 @@ __device__ __forceinline__ void gwh_pv2_fp64(const InstCoef &C, double f, const 
     double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) * rcp_newton(v);
     double s = C.SP * rcp_newton(L + C.SL);
     double cb = rsqrt_seeded(1.0 + s * s);
-    double cbh = sqrt((1.0 + cb) / 2.0), sbh = sqrt((1.0 - cb) / 2.0);
+    /* cos and sin of beta/2: 1 - cos b = s^2 cos^2 b / (1 + cos b), so sin(b/2) = |s| cos b / (2 cos(b/2)), free of
+       the cancellation of sqrt((1 - cos b) / 2) for small s and one square root fewer */
+    double cbh = sqrt((1.0 + cb) / 2.0), sbh = fabs(s) * cb * (0.5 * rcp_newton(cbh));
     double c2 = cbh * cbh, s2 = sbh * sbh, c3 = c2 * cbh, s3 = s2 * sbh;
     double d2[5] = {s3 * sbh, 2 * cbh * s3, GWH_SQRT_6 * s2 * c2, 2 * c3 * sbh, c3 * cbh};
     double alpha = gwh_euler_eval(C.a_c, v, logw) + C.alpha_add;

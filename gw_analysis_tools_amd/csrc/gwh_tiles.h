@@ -59,7 +59,8 @@ __device__ __forceinline__ void block_reduce_store(double (&acc)[NV], double *__
  * binaries). */
 template <bool TABLE_EXP>
 __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, const BinPow &bp, double &a, double &phi,
-                                                  double &u_out, double &lx_out, const double2 *tab = gwh_turn_table)
+                                                  double &u_out, double &lx_out, const double2 *tab = gwh_turn_table,
+                                                  const double *etab = gwh_exp2_table)
 {
     double Mf = C.M * f;
     double r = C.Mm13 * bp.fm13;     /* (Mf)^{-1/3} */
@@ -73,7 +74,7 @@ __device__ __forceinline__ void gwh_phenomd_mixed(const InstCoef &C, double f, c
         ar = pn + u3 * u3 * u * (C.rho[0] + u * (C.rho[1] + u * C.rho[2]));
     } else if (f > C.f3) {
         double dl = f - C.fRD;
-        ar = C.mr_k * (TABLE_EXP ? gwh_exp_table(-C.mr_g2 * dl) : exp_fp32(-C.mr_g2 * dl)) * rcp_newton(dl * dl + C.mr_w2);
+        ar = C.mr_k * (TABLE_EXP ? gwh_exp_table(-C.mr_g2 * dl, etab) : exp_fp32(-C.mr_g2 * dl)) * rcp_newton(dl * dl + C.mr_w2);
     } else {
         ar = C.delta[0] + Mf * (C.delta[1] + Mf * (C.delta[2] + Mf * (C.delta[3] + Mf * C.delta[4])));
     }
@@ -121,7 +122,8 @@ template <int MODEL, int PREC, int BPL, int TPB>
 __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDesc *__restrict__ events,
                                                  const InstCoef *__restrict__ coefs, double *__restrict__ partial,
                                                  const int *__restrict__ active, const int inst, const int tile,
-                                                 double (*lds)[GWH_PARTIAL_STRIDE], double2 *__restrict__ tab)
+                                                 double (*lds)[GWH_PARTIAL_STRIDE], double2 *__restrict__ tab,
+                                                 double *__restrict__ etab)
 {
     {
         const int ev = W.inst_event ? W.inst_event[inst] : W.event0;
@@ -138,10 +140,11 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
         constexpr int kLevel = gwh_tile_level(TPB * BPL);
         const double fmin_tile = gwh_g(E.tile_fmin)[E.fmin_off[kLevel] + tile];
         if (!(fmin_tile > C.fcut)) {
-            /* the sin/cos table in LDS: the detector phases (one gather per detector and bin), the merger-ringdown
-               arctangent's correction and the IMRPhenomPv2 Euler angle read it there instead of through the
-               vector memory path */
+            /* the sin/cos and exponential tables in LDS: the detector phases (one gather per detector and bin), the
+               merger-ringdown arctangent's correction and exponential and the IMRPhenomPv2 Euler angle read them
+               there instead of through the vector memory path */
             for (int q = threadIdx.x; q < 256; q += TPB) tab[q] = gwh_turn_table[q];
+            if (threadIdx.x < 64) etab[threadIdx.x] = gwh_exp2_table[threadIdx.x];
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < BPL; j++) {
@@ -158,7 +161,7 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                 if (MODEL == GWH_IMRPHENOMD) {
                     double a, phi, u, lx;
                     /* mixed: the merger-ringdown exponential in FP32, everything else as FP64 */
-                    gwh_phenomd_fp64<PREC == GWH_MIXED>(C, f, bp, a, phi, u, lx, tab);
+                    gwh_phenomd_fp64<PREC == GWH_MIXED>(C, f, bp, a, phi, u, lx, tab, etab);
                     a = live ? a : 0.0;
                     phi = live ? phi : 0.0;
 #pragma unroll
@@ -175,10 +178,10 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                 } else {
                     double A, Phi, hpr, hpi, hcr, hci;
                     if (PREC == GWH_FP64) {
-                        gwh_pv2_fp64(C, f, bp, A, Phi, hpr, hpi, hcr, hci, tab);
+                        gwh_pv2_fp64(C, f, bp, A, Phi, hpr, hpi, hcr, hci, tab, etab);
                     } else {
                         double u, lx;
-                        gwh_phenomd_mixed<true>(C, f, bp, A, Phi, u, lx, tab);
+                        gwh_phenomd_mixed<true>(C, f, bp, A, Phi, u, lx, tab, etab);
                         /* precession angles and the sin/cos of the Euler angle alpha in FP64 (alpha's rounding
                          * would enter |h|^2 directly), like the detector phases below: table-driven sin/cos and
                          * exponential (the FP32 ones left ~2.5e-4 in logL on short, loud precessing events) */
@@ -188,7 +191,8 @@ __device__ __forceinline__ void gwh_loglike_tile(const WorkMap &W, const EventDe
                         const double L = C.eta * (1.0 + C.L_c1 * x + C.L_c2 * x * x) * rcp_newton(v);
                         const double sv = C.SP * rcp_newton(L + C.SL);
                         const double cbd = rsqrt_seeded(1.0 + sv * sv);
-                        const double cbhd = sqrt(0.5 * (1.0 + cbd)), sbhd = sqrt(0.5 * (1.0 - cbd));
+                        /* sin(beta/2) = |s| cos b / (2 cos(beta/2)): no cancellation, one square root */
+                        const double cbhd = sqrt(0.5 * (1.0 + cbd)), sbhd = fabs(sv) * cbd * (0.5 * rcp_newton(cbhd));
                         const double c2d = cbhd * cbhd, s2d = sbhd * sbhd;
                         const double d2[5] = {s2d * s2d, 2. * cbhd * s2d * sbhd, GWH_SQRT_6 * s2d * c2d,
                                               2. * c2d * cbhd * sbhd, c2d * c2d};
@@ -255,10 +259,11 @@ __global__ __launch_bounds__(TPB) void gwh_loglike_wl_kernel(WorkMap W, const Ev
 {
     __shared__ double lds[TPB / 64][GWH_PARTIAL_STRIDE];
     __shared__ double2 tab[256];
+    __shared__ double etab[64];
     const int w = blockIdx.y * gridDim.x + blockIdx.x;
     if (w >= *count) return;
     const int2 wi = work[w];
-    gwh_loglike_tile<MODEL, PREC, BPL, TPB>(W, events, coefs, partial, active, wi.x, wi.y, lds, tab);
+    gwh_loglike_tile<MODEL, PREC, BPL, TPB>(W, events, coefs, partial, active, wi.x, wi.y, lds, tab, etab);
 }
 
 /* One block per (instance, tile) item, in tile-major order (item w = tile * n + instance).  Consecutive blocks go
@@ -280,10 +285,11 @@ __global__ __launch_bounds__(TPB) void gwh_loglike_grid_kernel(WorkMap W, const 
 {
     __shared__ double lds[TPB / 64][GWH_PARTIAL_STRIDE];
     __shared__ double2 tab[256];
+    __shared__ double etab[64];
     const int nn = count ? min(*count, W.n) : W.n;
     const int w = blockIdx.y * gridDim.x + blockIdx.x; /* items fold over y beyond 2^20 blocks */
     if (nn <= 0 || w >= nn * W.tiles0) return;
-    gwh_loglike_tile<MODEL, PREC, BPL, TPB>(W, events, coefs, partial, active, w % nn, w / nn, lds, tab);
+    gwh_loglike_tile<MODEL, PREC, BPL, TPB>(W, events, coefs, partial, active, w % nn, w / nn, lds, tab, etab);
 }
 
 /* ------------------------------------------------------------------------- */

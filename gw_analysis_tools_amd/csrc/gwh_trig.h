@@ -90,14 +90,14 @@ __device__ __forceinline__ void gwh_sincos_table(double phase, double &s, double
 /* e^x for |x| < 700 through the 2^(j/64) table: x = n ln2 / 64 + r with n = rint(64 x / ln2) and the reduction
  * by a two-part ln2 / 64 (r exact to ~1e-19, |r| <= ln2 / 128), e^r by its Taylor series to r^5 (truncation
  * below 4e-17), scaled by 2^(n mod 64 / 64) from the table and 2^(n div 64) exactly.  Error a few ulp. */
-__device__ __forceinline__ double gwh_exp_table(double x)
+__device__ __forceinline__ double gwh_exp_table(double x, const double *tab = gwh_exp2_table)
 {
     const double n = rint(x * 92.332482616893656877); /* 64 / ln 2 */
     double r = fma(-n, 0.01083042469326756, x);        /* ln2/64, high 32 bits (n * hi exact for |n| < 2^21) */
     r = fma(-n, 2.9815858269852933e-12, r);            /* ln2/64, the rest */
     const double p = fma(r, fma(r, fma(r, fma(r, fma(r, 1.0 / 120.0, 1.0 / 24.0), 1.0 / 6.0), 0.5), 1.0), 1.0);
     const int ni = (int)n;
-    return ldexp(gwh_exp2_table[ni & 63] * p, ni >> 6);
+    return ldexp(tab[ni & 63] * p, ni >> 6);
 }
 
 /* e^x for x <= 0 of moderate size: FP64 split x log2(e) = n + t, FP32 2^t (~6e-8 relative), exact scaling */
