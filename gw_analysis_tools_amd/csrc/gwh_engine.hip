@@ -785,11 +785,23 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const 
                                                                const int *__restrict__ active, int model, int what,
                                                                double *__restrict__ logl, const int *__restrict__ count)
 {
-    /* the instance index is uniform over the wave: scalar loads, and the loads that do not depend on each other
-       (tile count, event id, the instance's detector constants) are issued together before the partial sums */
+    /* The instance index is uniform over the wave (scalar loads).  Each lane's first tile record (every tile slot
+       lies inside the buffer) is requested first, then the loads that do not depend on each other (tile count,
+       event id, the instance's detector constants); slots at or past the tile count are added as zeros, so the
+       sums are those of the tiles in order, without a round trip for the count before the partial sums. */
     const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * (GWH_TPB / 64) + (threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     if (i >= W.n || (count && i >= *count)) return; /* whole waves leave together */
+    const size_t t0 = (size_t)i * W.tiles0;
+    double v0[GWH_PARTIAL_STRIDE];
+    const bool has0 = lane < W.tiles0; /* lanes without a tile slot read nothing */
+#pragma unroll
+    for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) v0[j] = 0.0;
+    if (has0) {
+        const double *p = partial + (t0 + lane) * GWH_PARTIAL_STRIDE;
+#pragma unroll
+        for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) v0[j] = p[j];
+    }
     const int nt = active[i];
     const int ev = W.inst_event ? W.inst_event[i] : W.event0;
     const InstCoef &C = coefs[i];
@@ -798,7 +810,6 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const 
     for (int d = 0; d < GWH_MAXDET; d++)
         for (int q = 0; q < 3; q++) cdet[d][q] = C.det[d][q];
     const EventDesc &E = events[ev];
-    const size_t t0 = (size_t)i * W.tiles0;
     if (nt < 0) { /* instance not evaluated (count or mask of the launch) */
         if (lane == 0) logl[i] = __longlong_as_double(0x7ff8000000000000ll);
         return;
@@ -806,9 +817,10 @@ __global__ __launch_bounds__(GWH_TPB) void gwh_finalize_kernel(WorkMap W, const 
     /* every slot of a tile's partial record is summed (slots past 3 D are unused), so the loads need not wait
        for the event's detector count */
     double s[GWH_PARTIAL_STRIDE];
+    const bool use0 = has0 && lane < nt;
 #pragma unroll
-    for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) s[j] = 0;
-    for (int t = lane; t < nt; t += 64) {
+    for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) s[j] = use0 ? v0[j] : 0.0;
+    for (int t = lane + 64; t < nt; t += 64) {
         const double *p = partial + (t0 + t) * GWH_PARTIAL_STRIDE;
 #pragma unroll
         for (int j = 0; j < GWH_PARTIAL_STRIDE; j++) s[j] += p[j];
