@@ -10,7 +10,13 @@ same constants and operation order, against 50-digit mpmath values:
   1e-12 relative) is caught;
 * the arctangent seeded in FP32 and corrected once through the table (gwh_atan_seeded) is within 8 ulp of pi/2
   of the exact value for |x| up to 1e6, with any FP32 seed within 2^-22 relative of atan(x); a correction with
-  the wrong sign is caught."""
+  the wrong sign is caught;
+* the reciprocal refined by one Newton step (rcp_newton) squares its seed's relative error (within 2 ulp of 1/z
+  for seeds within 2^-26, as the hardware FP64 reciprocal's), and the reciprocal square root from an FP32 seed
+  and two Newton steps (rsqrt_seeded) is within 4 ulp of z^(-1/2) over [1, 1e12] for any seed within 2^-22; a
+  step with the wrong sign, or one step short, is caught;
+* the IMRPhenomPv2 half angle sin(b/2) = |s| cos b / (2 cos(b/2)) with cos b = (1 + s^2)^(-1/2) is within 4 ulp of
+  sin(atan(s) / 2) for s from 1e-8 to 1e4, where sqrt((1 - cos b) / 2) loses digits as s -> 0."""
 import os
 import re
 
@@ -138,3 +144,52 @@ def test_atan_seeded():
     assert abs(fixed - float(mpmath.atan(x))) < 1e-15
     # planted: the correction subtracted instead of added
     assert max(abs(atan_seeded(float(x), tab, -1.0) - float(mpmath.atan(mpmath.mpf(float(x))))) for x in xs[:50]) > 1e-9
+
+
+def rcp_newton(z, seed_err):
+    """rcp_newton, operation by operation, from a seed with relative error seed_err"""
+    r = (1.0 / z) * (1.0 + seed_err)
+    return _fma(r, _fma(-z, r, 1.0), r)
+
+
+def rsqrt_seeded(z, seed_err, steps=2, sign=1.0):
+    """rsqrt_seeded, operation by operation: the FP32 seed (numpy's float32 square root) perturbed by seed_err"""
+    y = float(np.float32(1.0 / np.sqrt(np.float32(z)))) * (1.0 + seed_err)
+    for _ in range(steps):
+        y = y * (1.5 - sign * 0.5 * z * y * y)
+    return y
+
+
+def test_newton_reciprocals():
+    g = np.random.default_rng(11)
+    zs = np.concatenate([10.0 ** g.uniform(-3, 3, 300), [1.0, 0.5, 3.0, 1e-3, 1e3]])
+    ulp = 2.0 ** -52
+    for e in (2.0 ** -26, -(2.0 ** -26), 0.0):
+        worst = max(abs(rcp_newton(float(z), e) * float(z) - 1.0) for z in zs)
+        assert worst <= 2 * ulp + e * e, (e, worst)
+    # one Newton step squares the seed's error: a 2^-20 seed leaves ~2^-40
+    assert abs(rcp_newton(3.0, 2.0 ** -20) * 3.0 - 1.0) < 2.0 ** -39
+    zr = np.concatenate([1.0 + 10.0 ** g.uniform(-8, 12, 300), [1.0, 2.0, 1e12]])
+    for e in (2.0 ** -22, -(2.0 ** -22), 0.0):
+        worst = max(abs(rsqrt_seeded(float(z), e) / float(1 / mpmath.sqrt(mpmath.mpf(float(z)))) - 1.0) for z in zr)
+        assert worst <= 4 * ulp, (e, worst)
+    # planted: one Newton step short, or a step with the wrong sign
+    assert max(abs(rsqrt_seeded(float(z), 2.0 ** -22, steps=1) / float(1 / mpmath.sqrt(mpmath.mpf(float(z)))) - 1.0)
+               for z in zr[:20]) > 1e-13
+    assert abs(rsqrt_seeded(2.0, 2.0 ** -22, sign=-1.0) * np.sqrt(2.0) - 1.0) > 1e-3
+
+
+def test_pv2_half_angle():
+    g = np.random.default_rng(12)
+    ss = np.concatenate([10.0 ** g.uniform(-8, 4, 400), [1e-8, 1.0, 1e4]])
+    ulp = 2.0 ** -52
+    worst, worst_sqrt = 0.0, 0.0
+    for s in ss:
+        cb = 1.0 / np.sqrt(1.0 + s * s)
+        cbh = np.sqrt((1.0 + cb) / 2.0)
+        sbh = abs(s) * cb * (0.5 * (1.0 / cbh))
+        exact = mpmath.sin(mpmath.atan(mpmath.mpf(float(s))) / 2)
+        worst = max(worst, abs(float(sbh / exact) - 1.0))
+        worst_sqrt = max(worst_sqrt, abs(float(np.sqrt((1.0 - cb) / 2.0) / exact) - 1.0))
+    assert worst <= 4 * ulp, worst
+    assert worst_sqrt > 1e-6  # the cancellation the identity avoids
