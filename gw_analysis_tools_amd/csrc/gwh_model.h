@@ -649,8 +649,11 @@ __device__ inline void gwh_euler_coeffs(double q, double chil, double chip, doub
 
 __device__ __forceinline__ double gwh_euler_eval(const double *c, double v, double logw)
 {
-    double v2 = v * v;
-    return c[0] / (v2 * v) + c[1] / v2 + c[2] / v + c[3] * logw + c[4] * v;
+    /* the negative powers of v from one Newton-refined reciprocal (the two angles of a bin share it once inlined)
+       instead of three FP64 division sequences; the sum as explicit fused multiply-adds, so every kernel that
+       inlines it (the GR and the modified-gravity setup among them) rounds it the same way */
+    const double iv = rcp_newton(v), iv2 = iv * iv;
+    return fma(c[0], iv2 * iv, fma(c[1], iv2, fma(c[2], iv, fma(c[3], logw, c[4] * v))));
 }
 
 /* ------------------------------------------------------------------------- */
